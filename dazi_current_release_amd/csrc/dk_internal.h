@@ -1,0 +1,65 @@
+// Internal device-side descriptors shared by the kernels and the host API.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dk {
+
+enum { DK_STATUS_BADMODEL = -1 };
+
+// A batch of columns with refined, flattened layered models.
+//   uniform = 1: every column has Lu layers (at col*Lu) and the Ku shared
+//                periods; solve index s = k*ncol + col (column fastest, the
+//                reference's pvRc(nx*ny,kmax) layout).
+//   uniform = 0: column col has L[col] layers at loff[col] and K[col] periods
+//                at toff[col]; solve index s = toff[col] + k.
+struct BatchDev {
+  int ncol;
+  int uniform;
+  int Lu, Ku;
+  const int* L;
+  const int64_t* loff;
+  const int* K;
+  const int64_t* toff;
+  const float4* layers;  // (d, a, b, rho) flattened
+  const float* btp;      // flattening velocity factor per layer
+  const double* periods;
+  const double* cc;      // start velocity (= cm) per column
+  const float* betmx;    // max flattened shear velocity per column
+  const int* bad;        // prep status per column (0 ok)
+  double* croot;         // refined root per solve (0 where not found)
+  double* pv;            // float32-rounded phase velocity per solve
+  int* status;           // 0, or 1-based first failed period, or -1 bad model
+  int* work;             // persistent work counter
+  double tol;            // refinement bracket width (km/s)
+};
+
+// Kernel-pass outputs.
+//   node_mode = 1: node kernels in the reference layout sen(ncol, kmax, nz)
+//                  (needs uniform batches and nsub per node interval);
+//   node_mode = 0: refined-layer kernels at s + nsolves*m.
+struct KernDev {
+  int node_mode;
+  int nz;
+  const int* nsub;
+  int64_t nsolves;
+  const int* solve_col;  // ragged batches: column of each solve
+  double* dvs;
+  double* dvp;
+  double* drho;
+};
+
+hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
+                             const float* depz, float minthk0, int L, float4* layers, float* btp,
+                             double* cc, float* betmx, int* bad, hipStream_t st);
+hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
+                              const float* vp, const float* vs, const float* rho, float4* layers,
+                              float* btp, double* cc, float* betmx, int* bad, hipStream_t st);
+hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, int cols_per_wg,
+                          int persistent, int grid, int lds_stride, hipStream_t st);
+int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
+hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
+                          hipStream_t st);
+hipError_t launch_zero_int(int* p, int n, hipStream_t st);
+
+}  // namespace dk

@@ -1,0 +1,796 @@
+// HIP kernels of the depth-kernel stage for gfx950 (CDNA4).
+//
+//   k_prep_nodes   node model (vel, depz, minthk) -> refined, earth-flattened
+//                  layered model + start velocity; one thread per column.
+//                  (CalSurfG.f90:2490-2549, DepthKernel1D.f90:187-200,
+//                   surfdisp96.f:99-207 with sphere :480-547, gtsolh :361-382)
+//   k_prep_layers  the same for layered (ragged) models.
+//   k_forward<S>   Rayleigh fundamental-mode root search: S lanes of a wave own
+//                  one column and walk its periods in order, reproducing the
+//                  reference's scan grid, direction, floor and failure rules
+//                  (surfdisp96.f:223-348, getsol :384-476); each scan step
+//                  evaluates S trial velocities at once and a __ballot finds
+//                  the first sign change; the bracket is refined by
+//                  S-point multisection and a final secant step.
+//   k_kernels      per (column, period) implicit differentiation of the
+//                  secular function at the root, dc/dm = -(dF/dm)/(dF/dc),
+//                  via a forward tape and an adjoint sweep with dual numbers;
+//                  maps refined-layer partials to the node or layer outputs.
+//
+// All secular-function arithmetic is FP64 on the vector ALUs; nothing here is
+// a dense contraction, so MFMA is not used.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dk_physics.h"
+#include "dk_internal.h"
+
+namespace dk {
+
+// ------------------------------------------------------------------ float32-exact model prep
+// Evaluated with contraction off so that every float32 and float64 operation
+// rounds as the reference's separate multiply/add instructions do.
+__device__ float brocher_vp(float v) {
+#pragma clang fp contract(off)
+  float v2 = v * v, v3 = v2 * v, v4 = v3 * v;
+  return 0.9409f + 2.0947f * v - 0.8206f * v2 + 0.2683f * v3 - 0.0251f * v4;
+}
+__device__ float brocher_rho(float p) {
+#pragma clang fp contract(off)
+  float p2 = p * p, p3 = p2 * p, p4 = p3 * p, p5 = p4 * p;
+  return 1.6612f * p - 0.4721f * p2 + 0.0671f * p3 - 0.0043f * p4 + 0.000106f * p5;
+}
+
+__device__ float gtsolh_f32(float a, float b) {
+#pragma clang fp contract(off)
+  float c = 0.95f * b;
+  for (int i = 0; i < 5; i++) {
+    float gamma = b / a;
+    float kappa = c / b;
+    float k2 = kappa * kappa;
+    float gk = gamma * kappa;
+    float gk2 = gk * gk;
+    float fac1 = sqrtf(1.0f - gk2);
+    float fac2 = sqrtf(1.0f - k2);
+    float fr = (2.0f - k2) * (2.0f - k2) - 4.0f * fac1 * fac2;
+    float frp = -(4.0f * (2.0f - k2) * kappa) + 4.0f * fac2 * gamma * gamma * kappa / fac1 +
+                4.0f * fac1 * kappa / fac2;
+    frp = frp / b;
+    c = c - fr / frp;
+  }
+  return c;
+}
+
+// Earth flattening of one column in place (sphere(0,0) then sphere(2,1) for
+// Rayleigh waves), followed by the start-velocity rule of surfdisp96.
+// lay[m] = (thk, vp, vs, rho) on input, flattened (d, a, b, rho) on output.
+__device__ void flatten_column(float4* lay, float* btp_out, int L, double* cc_out,
+                               float* betmx_out, int* bad) {
+#pragma clang fp contract(off)
+  const double ar = 6370.0;
+  double dr = 0.0, r0 = ar;
+  float betmx = -1.e20f, betmn = 1.e20f;
+  int jmn = 0, jsol = 1;
+  for (int m = 0; m < L; m++) {
+    float4 l = lay[m];
+    float thk = (m == L - 1) ? 1.0f : l.x;
+    dr = dr + (double)thk;
+    double r1 = ar - dr;
+    double z0 = ar * log(ar / r0);
+    double z1 = ar * log(ar / r1);
+    float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
+    double tmp = (ar + ar) / (r0 + r1);
+    float a = (float)((double)l.y * tmp);
+    float b = (float)((double)l.z * tmp);
+    float bt = (float)tmp;
+    float rho = l.w * powf(bt, -2.275f);
+    r0 = r1;
+    lay[m] = make_float4(d, a, b, rho);
+    if (btp_out) btp_out[m] = bt;
+    if (b > 0.01f && b < betmn) {
+      betmn = b;
+      jmn = m;
+      jsol = 1;
+    } else if (b <= 0.01f && a < betmn) {
+      betmn = a;
+      jmn = m;
+      jsol = 0;
+    }
+    if (b > betmx) betmx = b;
+  }
+  // a fluid layer (b <= 0.01) is outside this stage's model class, and so is
+  // any non-finite or non-positive layer value
+  if (jsol == 0 || lay[0].z <= 0.0f) *bad = 1;
+  for (int m = 0; m < L; m++) {
+    float4 l = lay[m];
+    bool ok = isfinite(l.x) && isfinite(l.y) && isfinite(l.z) && isfinite(l.w) && l.y > 0.0f &&
+              l.z > 0.0f && l.w > 0.0f && (l.x > 0.0f || m == L - 1);
+    if (!ok) *bad = 1;
+  }
+  float cc1 = gtsolh_f32(lay[jmn].y, lay[jmn].z);
+  cc1 = 0.95f * cc1;
+  cc1 = 0.90f * cc1;
+  if (!(cc1 > 0.0f) || !isfinite(cc1)) *bad = 1;
+  *cc_out = (double)cc1;
+  *betmx_out = betmx;
+}
+
+// Node models: vel is (ncol, nz) column-fastest (Fortran vel(nx,ny,nz)).
+// vp/rho optional (both or neither); otherwise Brocher from vs.
+__global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
+                             const float* __restrict__ vpin, const float* __restrict__ rhoin,
+                             const float* __restrict__ depz, float minthk0, int L,
+                             float4* __restrict__ layers, float* __restrict__ btp,
+                             double* __restrict__ cc, float* __restrict__ betmx,
+                             int* __restrict__ bad_out) {
+#pragma clang fp contract(off)
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  float4* lay = layers + (size_t)col * L;
+  int k = 0;
+  float vs_i = vel[col], vp_i, rho_i;
+  if (vpin) {
+    vp_i = vpin[col];
+    rho_i = rhoin[col];
+  } else {
+    vp_i = brocher_vp(vs_i);
+    rho_i = brocher_rho(vp_i);
+  }
+  for (int i = 0; i < nz - 1; i++) {
+    size_t o = (size_t)(i + 1) * ncol + col;
+    float vs_n = vel[o], vp_n, rho_n;
+    if (vpin) {
+      vp_n = vpin[o];
+      rho_n = rhoin[o];
+    } else {
+      vp_n = brocher_vp(vs_n);
+      rho_n = brocher_rho(vp_n);
+    }
+    float thk = depz[i + 1] - depz[i];
+    float minthk = thk / minthk0;
+    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
+    float newthk = thk / (float)ns;
+    float den = (float)(2 * ns);
+    for (int j = 1; j <= ns && k < L - 1; j++, k++) {
+      float w = (float)(2 * j - 1);
+      lay[k] = make_float4(newthk, vp_i + w * (vp_n - vp_i) / den, vs_i + w * (vs_n - vs_i) / den,
+                           rho_i + w * (rho_n - rho_i) / den);
+    }
+    vs_i = vs_n;
+    vp_i = vp_n;
+    rho_i = rho_n;
+  }
+  lay[L - 1] = make_float4(0.0f, vp_i, vs_i, rho_i);
+  int bad = (k != L - 1);
+  flatten_column(lay, btp + (size_t)col * L, L, cc + col, betmx + col, &bad);
+  bad_out[col] = bad;
+}
+
+// Layered models (ragged): thk/vp/vs/rho concatenated at loff[col].
+__global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_t* __restrict__ loff,
+                              const float* __restrict__ thk, const float* __restrict__ vp,
+                              const float* __restrict__ vs, const float* __restrict__ rho,
+                              float4* __restrict__ layers, float* __restrict__ btp,
+                              double* __restrict__ cc, float* __restrict__ betmx,
+                              int* __restrict__ bad_out) {
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  int L = Lc[col];
+  int64_t o = loff[col];
+  float4* lay = layers + o;
+  for (int m = 0; m < L; m++) lay[m] = make_float4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]);
+  int bad = (L < 1);
+  if (L >= 1) flatten_column(lay, btp + o, L, cc + col, betmx + col, &bad);
+  bad_out[col] = bad;
+}
+
+// ------------------------------------------------------------------ forward root search
+enum { ST_START = 0, ST_UP = 1, ST_DN = 2, ST_REF = 3, ST_DONE = 4 };
+#define DK_MAX_REFINE 48
+#define DK_MAX_ROUNDS 100000
+
+__device__ __forceinline__ void col_geom(const BatchDev& B, int col, int* L, int64_t* loff, int* K,
+                                         int64_t* toff) {
+  if (B.uniform) {
+    *L = B.Lu;
+    *loff = (int64_t)col * B.Lu;
+    *K = B.Ku;
+    *toff = 0;
+  } else {
+    *L = B.L[col];
+    *loff = B.loff[col];
+    *K = B.K[col];
+    *toff = B.toff[col];
+  }
+}
+__device__ __forceinline__ int64_t solve_index(const BatchDev& B, int col, int64_t toff, int k) {
+  return B.uniform ? (int64_t)k * B.ncol + col : toff + k;
+}
+__device__ __forceinline__ double period_of(const BatchDev& B, int64_t toff, int k) {
+  return B.uniform ? B.periods[k] : B.periods[toff + k];
+}
+
+template <int S>
+__device__ __forceinline__ uint64_t group_bits(uint64_t mask, int gbase) {
+  if (S == 64) return mask;
+  return (mask >> gbase) & ((1ull << S) - 1ull);
+}
+__device__ __forceinline__ int ffs64(uint64_t m) { return m ? __ffsll((unsigned long long)m) - 1 : -1; }
+
+// Value-and-exponent helpers for the bracket ends.
+struct FV {
+  double m;
+  int e;
+};
+__device__ __forceinline__ double fv_ratio_root(double lo, double hi, FV flo, FV fhi) {
+  // root of the chord through (lo, flo) and (hi, fhi), flo and fhi of opposite sign
+  int eb = max(flo.e, fhi.e);
+  double a = ldexp(flo.m, flo.e - eb);
+  double b = ldexp(fhi.m, fhi.e - eb);
+  double den = a - b;
+  if (!(fabs(den) > 0.0)) return 0.5 * (lo + hi);
+  double t = a / den;
+  if (!(t >= 0.0 && t <= 1.0)) return 0.5 * (lo + hi);
+  return lo + (hi - lo) * t;
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int col_end,
+                                                 int persistent, int lds_stride) {
+  extern __shared__ float4 lds[];
+  const int lane = threadIdx.x & 63;
+  const int r = lane & (S - 1);
+  const int gbase = lane & ~(S - 1);
+  const int gpb = blockDim.x / S;  // groups per block
+  const int gid_blk = threadIdx.x / S;
+  float4* mdl = lds + (size_t)gid_blk * lds_stride;
+  const double dc = (double)0.005f;
+  const double onea = 1.5;
+  const double twopi = 2.0 * 3.141592653589793;
+
+  int col;
+  if (persistent) {
+    int c0 = 0;
+    if (r == 0) c0 = atomicAdd(B.work, 1);
+    col = __shfl(c0, gbase, 64) + col_begin;
+  } else {
+    col = col_begin + blockIdx.x * gpb + gid_blk;
+  }
+
+  // group state (identical in all S lanes of the group)
+  int L = 0, K = 0, k = 0, st = ST_DONE;
+  int64_t loff = 0, toff = 0;
+  double cm = 0, c1 = 0, clow = 0, omega = 1, betmx_d = 0;
+  FV d1 = {0, 0};
+  double d1x = 0;  // abscissa of the stored del1 value
+  int s1st = 0;    // sign bit of del1st
+  double lo = 0, hi = 0;
+  FV flo = {0, 0}, fhi = {0, 0};
+  int slo = 0, lo_ok = 0, hi_ok = 0;
+  int nref = 0;     // multisection rounds of the current bracket
+  int nround = 0;   // evaluation rounds of the current column (hang guard)
+
+  auto begin_column = [&]() {
+    if (col >= col_end) {
+      st = ST_DONE;
+      return;
+    }
+    col_geom(B, col, &L, &loff, &K, &toff);
+    for (int m = r; m < L; m += S) mdl[m] = B.layers[loff + m];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (B.bad[col] != 0 || K <= 0) {
+      // bad model: every period is zero-filled
+      for (int kk = r; kk < K; kk += S) {
+        int64_t s = solve_index(B, col, toff, kk);
+        B.croot[s] = 0.0;
+        B.pv[s] = 0.0;
+      }
+      if (r == 0) B.status[col] = B.bad[col] ? DK_STATUS_BADMODEL : 0;
+      st = ST_DONE;
+      return;
+    }
+    cm = B.cc[col];
+    betmx_d = (double)B.betmx[col];
+    nround = 0;
+    k = 0;
+    c1 = cm;
+    clow = cm;
+    omega = twopi / period_of(B, toff, 0);
+    st = ST_START;
+  };
+  begin_column();
+
+  auto fail_from = [&](int kf) {
+    for (int kk = kf + r; kk < K; kk += S) {
+      int64_t s = solve_index(B, col, toff, kk);
+      B.croot[s] = 0.0;
+      B.pv[s] = 0.0;
+    }
+    if (r == 0) B.status[col] = kf + 1;
+    st = ST_DONE;
+  };
+
+  while (true) {
+    // a wave runs while any of its groups has work; finished groups in a
+    // persistent launch fetch the next column
+    if (persistent && st == ST_DONE && col < col_end) {
+      int c0 = 0;
+      if (r == 0) c0 = atomicAdd(B.work, 1);
+      col = __shfl(c0, gbase, 64) + col_begin;
+      begin_column();
+    }
+    if (!__any(st != ST_DONE)) break;
+
+    // ---- trial point of this lane
+    double x;
+    bool elig = true;
+    if (st == ST_START)
+      x = c1 + (double)r * dc;
+    else if (st == ST_UP)
+      x = c1 + (double)(r + 1) * dc;
+    else if (st == ST_DN) {
+      x = c1 - (double)(r + 1) * dc;
+      elig = x > clow;
+    } else if (st == ST_REF)
+      x = lo + (double)(r + 1) * ((hi - lo) / (double)(S + 1));
+    else {
+      x = 1.0;
+      elig = false;
+    }
+    FV f = {1.0, 0};
+    if (st != ST_DONE && elig) {
+      auto lay = [&](int m) { return mdl[m]; };
+      f.m = secular(x, omega, L, lay, &f.e);
+    }
+    const int sg = signbit(f.m) ? 1 : 0;
+    const uint64_t neg = group_bits<S>(__ballot(sg), gbase);
+    const uint64_t okm = group_bits<S>(__ballot(elig), gbase);
+    // sign of the lane before me inside the group
+    auto sign_at = [&](int j) { return (int)((neg >> j) & 1ull); };
+
+    if (st == ST_DONE) continue;
+    // every state is bounded by the betmx test, the clow floor or the bracket
+    // width; this cap only guards against non-finite arithmetic
+    if (++nround > DK_MAX_ROUNDS) {
+      fail_from(k);
+      continue;
+    }
+
+    if (st == ST_START || st == ST_UP) {
+      int first = (st == ST_START) ? 1 : 0;
+      if (st == ST_START) {
+        int s0 = sign_at(0);
+        FV f0 = {__shfl(f.m, gbase, 64), __shfl(f.e, gbase, 64)};
+        d1 = f0;
+        d1x = c1;
+        int idir = +1;
+        if (k == 0)
+          s1st = s0;
+        else if (s0 != s1st)
+          idir = -1;
+        if (idir < 0) {
+          st = ST_DN;
+          continue;
+        }
+      }
+      // change[j]: sign differs from the previous point; fail[j]: step j leaves the range
+      uint64_t chg = 0, flm = 0;
+      for (int j = first; j < S; j++) {
+        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(j - 1);
+        if (sign_at(j) != prev) chg |= 1ull << j;
+      }
+      bool fl = (x < cm) || (x >= betmx_d + dc);
+      flm = group_bits<S>(__ballot(fl), gbase);
+      if (first) flm &= ~1ull;
+      int jc = ffs64(chg), jf = ffs64(flm);
+      if (jc >= 0 && (jf < 0 || jc <= jf)) {
+        double xa = (jc == 0) ? c1 : __shfl(x, gbase + jc - 1, 64);
+        FV fa = (jc == 0) ? d1 : FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
+        int sa = (jc == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(jc - 1);
+        int aok = (jc == 0) ? (d1x == c1) : 1;
+        double xb = __shfl(x, gbase + jc, 64);
+        FV fb = {__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
+        lo = xa;
+        hi = xb;
+        flo = fa;
+        fhi = fb;
+        slo = sa;
+
+        lo_ok = aok;
+        hi_ok = 1;
+        nref = 0;
+        st = ST_REF;
+      } else if (jf >= 0) {
+        fail_from(k);
+        continue;
+      } else {
+        c1 = __shfl(x, gbase + S - 1, 64);
+        d1 = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        d1x = c1;
+        st = ST_UP;
+        continue;
+      }
+    } else if (st == ST_DN) {
+      int nv = __popcll(okm);  // eligible lanes form a prefix
+      uint64_t chg = 0;
+      for (int j = 0; j < nv; j++) {
+        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(j - 1);
+        if (sign_at(j) != prev) chg |= 1ull << j;
+      }
+      int jc = ffs64(chg);
+      if (jc >= 0) {
+        // bracket (x_jc, x_jc-1): lower end is the new point
+        double xb = (jc == 0) ? c1 : __shfl(x, gbase + jc - 1, 64);
+        FV fb = (jc == 0) ? d1 : FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
+        int sb = (jc == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(jc - 1);
+        int bok = (jc == 0) ? (d1x == c1) : 1;
+        lo = __shfl(x, gbase + jc, 64);
+        flo = FV{__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
+        slo = sign_at(jc);
+        lo_ok = 1;
+        hi = xb;
+        fhi = fb;
+
+        hi_ok = bok;
+        nref = 0;
+        st = ST_REF;
+      } else if (nv < S) {
+        // floor reached: continue upward from clow with the last value kept
+        if (nv > 0) {
+          d1 = FV{__shfl(f.m, gbase + nv - 1, 64), __shfl(f.e, gbase + nv - 1, 64)};
+          d1x = __shfl(x, gbase + nv - 1, 64);
+        }
+        c1 = clow;
+        st = ST_UP;
+        continue;
+      } else {
+        c1 = __shfl(x, gbase + S - 1, 64);
+        d1 = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        d1x = c1;
+        if (c1 < cm) {
+          fail_from(k);
+          continue;
+        }
+        continue;
+      }
+    } else if (st == ST_REF) {
+      uint64_t chg = 0;
+      for (int j = 0; j < S; j++) {
+        int prev = (j == 0) ? slo : sign_at(j - 1);
+        if (sign_at(j) != prev) chg |= 1ull << j;
+      }
+      int jc = ffs64(chg);
+      if (jc >= 0) {
+        if (jc > 0) {
+          lo = __shfl(x, gbase + jc - 1, 64);
+          flo = FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
+          slo = sign_at(jc - 1);
+          lo_ok = 1;
+        }
+        hi = __shfl(x, gbase + jc, 64);
+        fhi = FV{__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
+
+        hi_ok = 1;
+      } else {
+        lo = __shfl(x, gbase + S - 1, 64);
+        flo = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        slo = sign_at(S - 1);
+        lo_ok = 1;
+      }
+    }
+
+    if (st == ST_REF) nref++;
+    if (st == ST_REF && ((hi - lo) <= B.tol || nref >= DK_MAX_REFINE)) {
+      double root;
+      bool opp = (signbit(flo.m) != signbit(fhi.m));
+      if (lo_ok && hi_ok && opp)
+        root = fv_ratio_root(lo, hi, flo, fhi);
+      else
+        root = 0.5 * (lo + hi);
+      if (root > betmx_d) {
+        fail_from(k);
+        continue;
+      }
+      if (r == 0) {
+        int64_t s = solve_index(B, col, toff, k);
+        B.croot[s] = root;
+        B.pv[s] = (double)(float)root;
+      }
+      k++;
+      if (k >= K) {
+        if (r == 0) B.status[col] = 0;
+        st = ST_DONE;
+      } else {
+        omega = twopi / period_of(B, toff, k);
+        c1 = root - onea * dc;
+        clow = cm;
+        st = ST_START;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ kernel pass
+// Tape layout: tape[(m*6 + i) * nthr + tid], i<5 vector entries, i==5 exponent.
+__device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, int col, int k,
+                                           int64_t s, double* __restrict__ tape, int64_t nthr,
+                                           int tid) {
+  int L;
+  int64_t loff, toff;
+  int K;
+  col_geom(B, col, &L, &loff, &K, &toff);
+  const double c = B.croot[s];
+  const int nout = O.node_mode ? O.nz : L;
+  auto out_index = [&](int m) -> int64_t {
+    if (O.node_mode) return (int64_t)col + (int64_t)B.ncol * ((int64_t)k + (int64_t)K * m);
+    return s + O.nsolves * (int64_t)m;
+  };
+  if (!(c > 0.0)) {
+    for (int m = 0; m < nout; m++) {
+      int64_t o = out_index(m);
+      if (O.dvs) O.dvs[o] = 0.0;
+      if (O.dvp) O.dvp[o] = 0.0;
+      if (O.drho) O.drho[o] = 0.0;
+    }
+    return;
+  }
+  const float4* lay = B.layers + loff;
+  const float* btp = B.btp + loff;
+  double omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
+  if (omega < 1.0e-4) omega = 1.0e-4;
+  const double wv = omega / c;
+  const double wv2 = wv * wv;
+
+  // ---- sweep 1: bottom-up values; tape holds the vector entering layer m
+  double e[5];
+  {
+    float4 h = lay[L - 1];
+    halfspace_e<double>(wv, wv2, (double)h.y, (double)h.z, (double)h.w, omega, e);
+  }
+  int Ef = pow2_normalize(e);
+  for (int m = L - 2; m >= 0; m--) {
+    double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+#pragma unroll
+    for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i];
+    tp[(int64_t)5 * nthr] = (double)Ef;
+    float4 l = lay[m];
+    double ca[5][5];
+    layer_ca<double>(wv, wv2, (double)l.y, (double)l.z, (double)l.w, (double)l.x, omega, ca);
+    double ee[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      double cr = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) cr = fma(e[j], ca[j][i], cr);
+      ee[i] = cr;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) e[i] = ee[i];
+    Ef += pow2_normalize(e);
+  }
+  const int EF = Ef;  // exponent of the top vector: reference scale
+
+  // ---- sweep 2: adjoint top-down with Dual<4> layer matrices
+  // g = dF/de (true scale g * 2^Eg); indices 0:a 1:b 2:rho 3:c
+  double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
+  int Eg = 0;
+  double dFdc = 0.0;
+  Dual<4> wvd = dconst<4>(wv);
+  wvd.g[3] = -wv / c;
+  Dual<4> wv2d = wvd * wvd;
+  // node accumulation state
+  int node = 0, sub = 0;
+  double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
+  for (int m = 0; m <= L - 1; m++) {
+    float4 l = lay[m];
+    double part[4];
+    if (m < L - 1) {
+      Dual<4> ca[5][5];
+      layer_ca<Dual<4>>(wvd, wv2d, dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
+                        dvar<4>((double)l.w, 2), (double)l.x, omega, ca);
+      const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+      double et[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) et[i] = tp[(int64_t)i * nthr];
+      int Ee = (int)tp[(int64_t)5 * nthr];
+      // contribution: sum_ji e_j dCA_ji g_i, scaled 2^(Ee+Eg-EF)
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          double t = 0.0;
+#pragma unroll
+          for (int i = 0; i < 5; i++) t = fma(ca[j][i].g[p], g[i], t);
+          acc = fma(et[j], t, acc);
+        }
+        part[p] = ldexp(acc, Ee + Eg - EF);
+      }
+      // propagate g <- CA g
+      double gn[5];
+#pragma unroll
+      for (int j = 0; j < 5; j++) {
+        double t = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; i++) t = fma(ca[j][i].v, g[i], t);
+        gn[j] = t;
+      }
+#pragma unroll
+      for (int j = 0; j < 5; j++) g[j] = gn[j];
+      Eg += pow2_normalize(g);
+    } else {
+      Dual<4> eh[5];
+      halfspace_e<Dual<4>>(wvd, wv2d, dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
+                           dvar<4>((double)l.w, 2), omega, eh);
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
+        part[p] = ldexp(acc, Eg - EF);
+      }
+    }
+    dFdc += part[3];
+    // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
+    // rho_flat = rho*btp^-2.275
+    double bt = (double)btp[m];
+    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * pow(bt, -2.275);
+    if (!O.node_mode) {
+      int64_t o = out_index(m);
+      if (O.dvp) O.dvp[o] = pa;
+      if (O.dvs) O.dvs[o] = pb;
+      if (O.drho) O.drho[o] = pr;
+      continue;
+    }
+    // node mode: sublayer j (1-based) of interval `node` with ns sublayers
+    if (m < L - 1) {
+      int ns = (node < O.nz - 1) ? O.nsub[node] : 1;
+      int j = sub + 1;
+      double whi = (double)(2 * j - 1) / (double)(2 * ns);
+      double wlo = 1.0 - whi;
+      acc_lo[0] += wlo * pb;
+      acc_lo[1] += wlo * pa;
+      acc_lo[2] += wlo * pr;
+      acc_hi[0] += whi * pb;
+      acc_hi[1] += whi * pa;
+      acc_hi[2] += whi * pr;
+      sub++;
+      if (sub == ns && node < O.nz - 1) {
+        int64_t o = out_index(node);
+        if (O.dvs) O.dvs[o] = acc_lo[0];
+        if (O.dvp) O.dvp[o] = acc_lo[1];
+        if (O.drho) O.drho[o] = acc_lo[2];
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          acc_lo[q] = acc_hi[q];
+          acc_hi[q] = 0.0;
+        }
+        node++;
+        sub = 0;
+      }
+    } else {
+      int64_t o = out_index(node);
+      if (O.dvs) O.dvs[o] = acc_lo[0] + pb;
+      if (O.dvp) O.dvp[o] = acc_lo[1] + pa;
+      if (O.drho) O.drho[o] = acc_lo[2] + pr;
+    }
+  }
+  // ---- rescale: dc/dm = -(dF/dm)/(dF/dc)
+  const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
+  for (int m = 0; m < nout; m++) {
+    int64_t o = out_index(m);
+    if (O.dvs) O.dvs[o] *= sc;
+    if (O.dvp) O.dvp[o] *= sc;
+    if (O.drho) O.drho[o] *= sc;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_kernels(BatchDev B, KernDev O, double* tape) {
+  const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t s = tid; s < O.nsolves; s += nthr) {
+    int col, k;
+    if (B.uniform) {
+      col = (int)(s % B.ncol);
+      k = (int)(s / B.ncol);
+    } else {
+      col = O.solve_col[s];
+      k = (int)(s - B.toff[col]);
+    }
+    kern_solve(B, O, col, k, s, tape, nthr, tid);
+  }
+}
+
+__global__ void k_zero_int(int* p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------------------ launchers
+template <int S>
+static hipError_t launch_forward_S(const BatchDev& B, int col_begin, int col_end, int cols_per_wg,
+                                   int persistent, int grid, int lds_stride, hipStream_t st) {
+  int block = S * cols_per_wg;
+  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(float4);
+  if (!persistent) grid = (col_end - col_begin + cols_per_wg - 1) / cols_per_wg;
+  if (grid <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_forward<S>, dim3(grid), dim3(block), lds, st, B, col_begin, col_end,
+                     persistent, lds_stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, int cols_per_wg,
+                          int persistent, int grid, int lds_stride, hipStream_t st) {
+  switch (S) {
+    case 8:
+      return launch_forward_S<8>(B, col_begin, col_end, cols_per_wg, persistent, grid, lds_stride, st);
+    case 16:
+      return launch_forward_S<16>(B, col_begin, col_end, cols_per_wg, persistent, grid, lds_stride, st);
+    case 32:
+      return launch_forward_S<32>(B, col_begin, col_end, cols_per_wg, persistent, grid, lds_stride, st);
+    case 64:
+      return launch_forward_S<64>(B, col_begin, col_end, cols_per_wg, persistent, grid, lds_stride, st);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
+  int block = S * cols_per_wg;
+  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(float4);
+  int nb = 0;
+  hipError_t e = hipErrorInvalidValue;
+  switch (S) {
+    case 8: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward<8>, block, lds); break;
+    case 16: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward<16>, block, lds); break;
+    case 32: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward<32>, block, lds); break;
+    case 64: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_forward<64>, block, lds); break;
+  }
+  if (e != hipSuccess || nb < 1) nb = 1;
+  return nb;
+}
+
+hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
+                             const float* depz, float minthk0, int L, float4* layers, float* btp,
+                             double* cc, float* betmx, int* bad, hipStream_t st) {
+  int block = 128;
+  int grid = (ncol + block - 1) / block;
+  hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(block), 0, st, ncol, nz, vel, vp, rho, depz,
+                     minthk0, L, layers, btp, cc, betmx, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
+                              const float* vp, const float* vs, const float* rho, float4* layers,
+                              float* btp, double* cc, float* betmx, int* bad, hipStream_t st) {
+  int block = 128;
+  int grid = (ncol + block - 1) / block;
+  hipLaunchKernelGGL(k_prep_layers, dim3(grid), dim3(block), 0, st, ncol, Lc, loff, thk, vp, vs,
+                     rho, layers, btp, cc, betmx, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
+                          hipStream_t st) {
+  int block = 256;
+  int grid = std::max(1, grid_threads / block);
+  hipLaunchKernelGGL(k_kernels, dim3(grid), dim3(block), 0, st, B, O, tape);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+}  // namespace dk

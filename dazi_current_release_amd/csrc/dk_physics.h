@@ -1,0 +1,361 @@
+// Rayleigh-wave physics of the depth-kernel stage, written once for the
+// device (and a host build used by CPU self-checks of the derivative code).
+//
+// Secular function: Dunkin compound-matrix recursion from the half-space up
+// (reference: code/src_inv_iso_joint/surfdisp96.f:767-865 dltar4, :868-985 var,
+// :1018-1062 dnka). Differences from the reference, by design:
+//   * normalisation of the 5-vector after each layer is an exact power of two
+//     (frexp/ldexp) instead of a division by max|e| (surfdisp96.f:989-1014),
+//     and the exponents are accumulated, so the returned value m*2^E is the
+//     un-normalised, exp(-exa)-scaled determinant: continuous in c and with the
+//     reference's sign;
+//   * every function is templated on the number type, so the same code
+//     evaluated on Dual<N> yields exact partial derivatives of each layer's
+//     compound matrix with respect to (alpha, beta, rho, c).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define DK_HD __host__ __device__ inline
+
+namespace dk {
+
+// ----------------------------------------------------------------- dual numbers
+template <int N>
+struct Dual {
+  double v;
+  double g[N];
+};
+
+template <int N>
+DK_HD Dual<N> dconst(double v) {
+  Dual<N> r;
+  r.v = v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = 0.0;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> dvar(double v, int idx) {
+  Dual<N> r = dconst<N>(v);
+  r.g[idx] = 1.0;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator+(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] + b.g[i];
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator-(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] - b.g[i];
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator-(const Dual<N>& a) {
+  Dual<N> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = -a.g[i];
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator*(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] * b.v + a.v * b.g[i];
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator/(const Dual<N>& a, const Dual<N>& b) {
+  Dual<N> r;
+  double ib = 1.0 / b.v;
+  r.v = a.v * ib;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = (a.g[i] - r.v * b.g[i]) * ib;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator+(const Dual<N>& a, double b) {
+  Dual<N> r = a;
+  r.v += b;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator+(double b, const Dual<N>& a) {
+  return a + b;
+}
+template <int N>
+DK_HD Dual<N> operator-(const Dual<N>& a, double b) {
+  Dual<N> r = a;
+  r.v -= b;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator-(double b, const Dual<N>& a) {
+  Dual<N> r = -a;
+  r.v += b;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator*(const Dual<N>& a, double b) {
+  Dual<N> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] * b;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> operator*(double b, const Dual<N>& a) {
+  return a * b;
+}
+template <int N>
+DK_HD Dual<N> operator/(const Dual<N>& a, double b) {
+  return a * (1.0 / b);
+}
+template <int N>
+DK_HD Dual<N> operator/(double a, const Dual<N>& b) {
+  Dual<N> r;
+  double ib = 1.0 / b.v;
+  r.v = a * ib;
+  double f = -r.v * ib;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = b.g[i] * f;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> dsqrt(const Dual<N>& a) {
+  Dual<N> r;
+  r.v = sqrt(a.v);
+  double h = 0.5 / r.v;
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] * h;
+  return r;
+}
+template <int N>
+DK_HD Dual<N> dexp(const Dual<N>& a) {
+  Dual<N> r;
+  r.v = exp(a.v);
+#pragma unroll
+  for (int i = 0; i < N; i++) r.g[i] = a.g[i] * r.v;
+  return r;
+}
+template <int N>
+DK_HD void dsincos(const Dual<N>& a, Dual<N>* s, Dual<N>* c) {
+  double sv, cv;
+  sincos(a.v, &sv, &cv);
+  s->v = sv;
+  c->v = cv;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    s->g[i] = a.g[i] * cv;
+    c->g[i] = -a.g[i] * sv;
+  }
+}
+template <int N>
+DK_HD double val(const Dual<N>& a) {
+  return a.v;
+}
+
+DK_HD double dsqrt(double a) { return sqrt(a); }
+DK_HD double dexp(double a) { return exp(a); }
+DK_HD void dsincos(double a, double* s, double* c) { sincos(a, s, c); }
+DK_HD double val(double a) { return a; }
+template <class T>
+DK_HD T tconst(double v);
+template <>
+DK_HD double tconst<double>(double v) {
+  return v;
+}
+template <>
+DK_HD Dual<4> tconst<Dual<4>>(double v) {
+  return dconst<4>(v);
+}
+
+// ----------------------------------------------------------------- layer matrix
+// Vertical wavenumber sqrt((k+kx)|k-kx|) written as sqrt(|k^2-kx^2|): the two
+// forms agree in exact arithmetic and the latter keeps the derivative simple.
+template <class T>
+DK_HD T vwave(const T& wvno, const T& kx) {
+  T s = (wvno + kx) * (wvno - kx);
+  if (val(s) < 0.0) s = -s;
+  return dsqrt(s);
+}
+
+// One layer: Dunkin's reduced compound matrix ca (surfdisp96.f:808-831 plus
+// var and dnka), scaled by exp(-exa). a,b,rho are the flattened layer values,
+// d its flattened thickness.
+template <class T>
+DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho, double d,
+                    double omega, T ca[5][5]) {
+  T xka = omega / a;
+  T xkb = omega / b;
+  T ra = vwave(wvno, xka);
+  T rb = vwave(wvno, xkb);
+  T t = b * (1.0 / omega);
+  T gammk = 2.0 * t * t;
+  T gam = gammk * wvno2;
+  T p = ra * d;
+  T q = rb * d;
+  // ---- var (surfdisp96.f:868-985)
+  T cosp, sinp, w, x, cosq, sinq, y, z;
+  T pex = tconst<T>(0.0), sex = tconst<T>(0.0);
+  if (val(wvno) < val(xka)) {
+    dsincos(p, &sinp, &cosp);
+    w = sinp / ra;
+    x = -(ra * sinp);
+  } else if (val(wvno) == val(xka)) {
+    cosp = tconst<T>(1.0);
+    w = tconst<T>(d);
+    x = tconst<T>(0.0);
+  } else {
+    pex = p;
+    T fac = tconst<T>(0.0);
+    if (val(p) < 16.0) fac = dexp(-2.0 * p);
+    cosp = (1.0 + fac) * 0.5;
+    sinp = (1.0 - fac) * 0.5;
+    w = sinp / ra;
+    x = ra * sinp;
+  }
+  if (val(wvno) < val(xkb)) {
+    dsincos(q, &sinq, &cosq);
+    y = sinq / rb;
+    z = -(rb * sinq);
+  } else if (val(wvno) == val(xkb)) {
+    cosq = tconst<T>(1.0);
+    y = tconst<T>(d);
+    z = tconst<T>(0.0);
+  } else {
+    sex = q;
+    T fac = tconst<T>(0.0);
+    if (val(q) < 16.0) fac = dexp(-2.0 * q);
+    cosq = (1.0 + fac) * 0.5;
+    sinq = (1.0 - fac) * 0.5;
+    y = sinq / rb;
+    z = rb * sinq;
+  }
+  T exa = pex + sex;
+  T a0 = tconst<T>(0.0);
+  if (val(exa) < 60.0) a0 = dexp(-exa);
+  T cpcq = cosp * cosq;
+  T cpy = cosp * y;
+  T cpz = cosp * z;
+  T cqw = cosq * w;
+  T cqx = cosq * x;
+  T xy = x * y;
+  T xz = x * z;
+  T wy = w * y;
+  T wz = w * z;
+  // ---- dnka (surfdisp96.f:1018-1062)
+  T gamm1 = gam - 1.0;
+  T twgm1 = gam + gamm1;
+  T gmgmk = gam * gammk;
+  T gmgm1 = gam * gamm1;
+  T gm1sq = gamm1 * gamm1;
+  T irho = 1.0 / rho;
+  T rho2 = rho * rho;
+  T irho2 = irho * irho;
+  T a0pq = a0 - cpcq;
+  ca[0][0] = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+  ca[0][1] = (wvno2 * cpy - cqx) * irho;
+  ca[0][2] = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
+  ca[0][3] = (cpz - wvno2 * cqw) * irho;
+  ca[0][4] = -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2;
+  ca[1][0] = (gmgmk * cpz - gm1sq * cqw) * rho;
+  ca[1][1] = cpcq;
+  ca[1][2] = gammk * cpz - gamm1 * cqw;
+  ca[1][3] = -wz;
+  ca[1][4] = ca[0][3];
+  ca[3][0] = (gm1sq * cpy - gmgmk * cqx) * rho;
+  ca[3][1] = -xy;
+  ca[3][2] = gamm1 * cpy - gammk * cqx;
+  ca[3][3] = ca[1][1];
+  ca[3][4] = ca[0][1];
+  ca[4][0] = -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
+  ca[4][1] = ca[3][0];
+  ca[4][2] = -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
+  ca[4][3] = ca[1][0];
+  ca[4][4] = ca[0][0];
+  T tt = -2.0 * wvno2;
+  ca[2][0] = tt * ca[4][2];
+  ca[2][1] = tt * ca[3][2];
+  ca[2][2] = a0 + 2.0 * (cpcq - ca[0][0]);
+  ca[2][3] = tt * ca[1][2];
+  ca[2][4] = tt * ca[0][2];
+}
+
+// Half-space 5-vector (surfdisp96.f:777-802).
+template <class T>
+DK_HD void halfspace_e(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
+                       double omega, T e[5]) {
+  T xka = omega / a;
+  T xkb = omega / b;
+  T ra = vwave(wvno, xka);
+  T rb = vwave(wvno, xkb);
+  T t = b * (1.0 / omega);
+  T gammk = 2.0 * t * t;
+  T gam = gammk * wvno2;
+  T gamm1 = gam - 1.0;
+  T rarb = ra * rb;
+  e[0] = rho * rho * (gamm1 * gamm1 - gam * gammk * rarb);
+  e[1] = -(rho * ra);
+  e[2] = rho * (gamm1 - gammk * rarb);
+  e[3] = rho * rb;
+  e[4] = wvno2 - rarb;
+}
+
+// ----------------------------------------------------------------- value-only recursion
+// Exact power-of-two normalisation: scale e by 2^-k with k = exponent of max|e|.
+DK_HD int pow2_normalize(double e[5]) {
+  double m = fabs(e[0]);
+#pragma unroll
+  for (int i = 1; i < 5; i++) m = fmax(m, fabs(e[i]));
+  if (!(m > 1e-300) || !isfinite(m)) return 0;
+  int k;
+  frexp(m, &k);
+#pragma unroll
+  for (int i = 0; i < 5; i++) e[i] = ldexp(e[i], -k);
+  return k;
+}
+
+// Layers are given as float4 (d, a, b, rho) of the flattened model; index
+// L-1 is the half-space. Returns the mantissa; *ex receives the exponent.
+template <class LayerFn>
+DK_HD double secular(double c, double omega, int L, LayerFn lay, int* ex) {
+  if (omega < 1.0e-4) omega = 1.0e-4;
+  double wvno = omega / c;
+  double wvno2 = wvno * wvno;
+  double e[5];
+  float4 h = lay(L - 1);
+  halfspace_e<double>(wvno, wvno2, (double)h.y, (double)h.z, (double)h.w, omega, e);
+  int E = pow2_normalize(e);
+  for (int m = L - 2; m >= 0; m--) {
+    float4 l = lay(m);
+    double ca[5][5];
+    layer_ca<double>(wvno, wvno2, (double)l.y, (double)l.z, (double)l.w, (double)l.x, omega, ca);
+    double ee[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      double cr = 0.0;
+#pragma unroll
+      for (int j = 0; j < 5; j++) cr = fma(e[j], ca[j][i], cr);
+      ee[i] = cr;
+    }
+#pragma unroll
+    for (int i = 0; i < 5; i++) e[i] = ee[i];
+    E += pow2_normalize(e);
+  }
+  *ex = E;
+  return e[0];
+}
+
+}  // namespace dk

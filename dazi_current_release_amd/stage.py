@@ -1,0 +1,194 @@
+"""Host mirror of the reference's depth-kernel stage interface.
+
+``depthkernel`` keeps the argument list, argument meaning, in-place output
+semantics and log lines of the reference subroutine
+(code/src_inv_iso_joint/CalSurfG.f90:1-113); the work runs in the HIP library
+through the C ABI.  ``iso_dev`` / ``layered_dev`` are the device-tensor entry
+points used by the PyTorch-ROCm wrapper and by the benchmark.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import sys
+
+import numpy as np
+
+from . import _lib
+
+_BACKEND_NAME = "hip"
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _fortran(a, dtype, shape, name):
+    a = np.asarray(a)
+    if a.dtype != dtype:
+        raise TypeError(f"{name} must be {np.dtype(dtype).name}, got {a.dtype}")
+    if a.size != int(np.prod(shape)):
+        raise ValueError(f"{name} has {a.size} elements, expected {shape}")
+    if not (a.flags.f_contiguous or a.ndim <= 1 or a.flags.c_contiguous and a.ndim == 1):
+        raise ValueError(f"{name} must be Fortran-contiguous")
+    return a
+
+
+def depthkernel(nx, ny, nz, vel, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc, iwave, igr, kmaxRc, tRc,
+                depz, minthk, *, vp=None, rho=None, log=sys.stdout):
+    """Reference: depthkernel (CalSurfG.f90:1-113).
+
+    vel (nx,ny,nz) float32, depz (nz,) float32, tRc (kmaxRc,) float64 ascending;
+    fills pvRc (nx*ny,kmaxRc) and sen_*Rc (nx*ny,kmaxRc,nz) float64, all
+    Fortran-ordered, in place.  Optional vp/rho (nx,ny,nz) replace the
+    Brocher relation (the reference always uses Brocher).  Raises DaziDkError
+    with the ABI code on failure (the reference stops with a code).
+    """
+    ncol = nx * ny
+    vel = _fortran(vel, np.float32, (nx, ny, nz), "vel")
+    depz = np.ascontiguousarray(depz, np.float32)
+    tRc = np.ascontiguousarray(tRc, np.float64)
+    pvRc = _fortran(pvRc, np.float64, (ncol, kmaxRc), "pvRc")
+    outs = [sen_vsRc, sen_vpRc, sen_rhoRc]
+    if any(o is not None for o in outs):
+        outs = [_fortran(o, np.float64, (ncol, kmaxRc, nz), n)
+                for o, n in zip(outs, ("sen_vsRc", "sen_vpRc", "sen_rhoRc"))]
+    if log is not None:
+        print(f"Depth kernel backend: {_BACKEND_NAME}", file=log)
+        print(f"Depth kernel grid/periods: nx={nx} ny={ny} nz={nz} kmax={kmaxRc}", file=log)
+    mode = 0
+    if vp is not None or rho is not None:
+        vp = _fortran(vp, np.float32, (nx, ny, nz), "vp")
+        rho = _fortran(rho, np.float32, (nx, ny, nz), "rho")
+        mode = 1
+    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr, 64, mode, float(minthk), 0)
+    rc = _lib.lib().dazi_dk_iso(C.byref(d), _ptr(vel), _ptr(vp), _ptr(rho), _ptr(depz),
+                                _ptr(tRc), _ptr(pvRc), *[_ptr(o) for o in outs])
+    _lib.check(rc)
+
+
+def run_depthkernel(vel, tRc, depz, minthk, vp=None, rho=None, kernels=True):
+    """Convenience wrapper: allocates the outputs of depthkernel and returns them
+    as (pv (ncol,K), sen_vs, sen_vp, sen_rho (ncol,K,nz)) Fortran-ordered arrays."""
+    vel = np.asfortranarray(vel, np.float32)
+    nx, ny, nz = vel.shape
+    K = len(tRc)
+    pv = np.zeros((nx * ny, K), np.float64, order="F")
+    sens = [np.zeros((nx * ny, K, nz), np.float64, order="F") for _ in range(3)] if kernels \
+        else [None] * 3
+    depthkernel(nx, ny, nz, vel, pv, *sens, 2, 0, K, tRc, depz, minthk,
+                vp=None if vp is None else np.asfortranarray(vp, np.float32),
+                rho=None if rho is None else np.asfortranarray(rho, np.float32), log=None)
+    return (pv, *sens) if kernels else pv
+
+
+# ----------------------------------------------------------------- device tensors
+def _tptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream_ptr(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def iso_dev(vel, depz, periods, minthk, vp=None, rho=None, kernels=True, tune=None, out=None,
+            stream=None):
+    """Node-model stage on device tensors.
+
+    vel/vp/rho: (nz, ncol) float32 (memory order of the reference's vel(nx,ny,nz));
+    depz (nz,) float32, periods (K,) float64, all on the GPU.
+    Returns dict(pv (K, ncol), sen_vs/sen_vp/sen_rho (nz, K, ncol) or None, status (ncol,)).
+    """
+    import torch
+    nz, ncol = vel.shape
+    K = periods.numel()
+    dev = vel.device
+    if out is None:
+        out = dict(pv=torch.empty((K, ncol), dtype=torch.float64, device=dev),
+                   status=torch.empty(ncol, dtype=torch.int32, device=dev))
+        if kernels:
+            for k in ("sen_vs", "sen_vp", "sen_rho"):
+                out[k] = torch.empty((nz, K, ncol), dtype=torch.float64, device=dev)
+    mode = 1 if vp is not None else 0
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, mode, float(minthk), 0)
+    t = tune if tune is not None else None
+    rc = _lib.lib().dazi_dk_iso_dev(
+        C.byref(d), None if t is None else C.byref(t), _tptr(vel), _tptr(vp), _tptr(rho),
+        _tptr(depz), _tptr(periods), _tptr(out["pv"]),
+        _tptr(out.get("sen_vs")) if kernels else None,
+        _tptr(out.get("sen_vp")) if kernels else None,
+        _tptr(out.get("sen_rho")) if kernels else None, _tptr(out["status"]),
+        _stream_ptr(stream))
+    _lib.check(rc)
+    return out
+
+
+class LayeredBatch:
+    """Ragged layered batch resident on the GPU (the batched solver's input)."""
+
+    def __init__(self, nl, nk, thk, vp, vs, rho, periods, device="cuda"):
+        import torch
+        nl = np.asarray(nl, np.int32)
+        nk = np.asarray(nk, np.int32)
+        self.ncol = int(nl.size)
+        self.max_layers = int(nl.max()) if nl.size else 1
+        self.nsolves = int(nk.sum())
+        loff = np.concatenate([[0], np.cumsum(nl, dtype=np.int64)[:-1]]).astype(np.int64)
+        toff = np.concatenate([[0], np.cumsum(nk, dtype=np.int64)[:-1]]).astype(np.int64)
+        solve_col = np.repeat(np.arange(self.ncol, dtype=np.int32), nk)
+        T = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=device)  # noqa
+        self.nl = T(nl, torch.int32)
+        self.nk = T(nk, torch.int32)
+        self.loff = T(loff, torch.int64)
+        self.toff = T(toff, torch.int64)
+        self.solve_col = T(solve_col, torch.int32)
+        self.thk = T(thk, torch.float32)
+        self.vp = T(vp, torch.float32)
+        self.vs = T(vs, torch.float32)
+        self.rho = T(rho, torch.float32)
+        self.periods = T(periods, torch.float64)
+        self.nl_host = nl
+        self.nk_host = nk
+        self.loff_host = loff
+        self.toff_host = toff
+        self.device = device
+
+    def alloc_outputs(self, kernels=False):
+        import torch
+        out = dict(c=torch.empty(self.nsolves, dtype=torch.float64, device=self.device),
+                   c_unrounded=torch.empty(self.nsolves, dtype=torch.float64,
+                                           device=self.device),
+                   status=torch.empty(self.ncol, dtype=torch.int32, device=self.device))
+        if kernels:
+            for k in ("dc_dvs", "dc_dvp", "dc_drho"):
+                out[k] = torch.zeros((self.max_layers, self.nsolves), dtype=torch.float64,
+                                     device=self.device)
+        return out
+
+
+def layered_dev(batch: LayeredBatch, kernels=False, tune=None, out=None, stream=None,
+                vs=None, vp=None, rho=None):
+    """Root search (+ refined-layer kernels) of a ragged layered batch.
+
+    Returns dict(c (nsolves,) float32-rounded, c_unrounded (nsolves,), status (ncol,),
+    dc_dvs/dc_dvp/dc_drho (max_layers, nsolves)).
+    vs/vp/rho override the batch's model tensors (same layout) when given.
+    """
+    if out is None:
+        out = batch.alloc_outputs(kernels)
+    b = _lib.Layered(batch.ncol, batch.max_layers, batch.nsolves, batch.nl.data_ptr(),
+                     batch.loff.data_ptr(), batch.nk.data_ptr(), batch.toff.data_ptr(),
+                     batch.solve_col.data_ptr(), batch.thk.data_ptr(),
+                     (vp if vp is not None else batch.vp).data_ptr(),
+                     (vs if vs is not None else batch.vs).data_ptr(),
+                     (rho if rho is not None else batch.rho).data_ptr(),
+                     batch.periods.data_ptr(), out["c"].data_ptr(), out["status"].data_ptr(),
+                     out["dc_dvs"].data_ptr() if kernels else None,
+                     out["dc_dvp"].data_ptr() if kernels else None,
+                     out["dc_drho"].data_ptr() if kernels else None,
+                     out["c_unrounded"].data_ptr() if "c_unrounded" in out else None)
+    rc = _lib.lib().dazi_dk_layered_dev(C.byref(b), None if tune is None else C.byref(tune),
+                                        _stream_ptr(stream))
+    _lib.check(rc)
+    return out

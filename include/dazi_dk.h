@@ -1,0 +1,138 @@
+/*
+ * dazi_dk.h — C ABI of the MI355X depth-kernel stage.
+ *
+ * The stage computes, for every grid column and period, the Rayleigh
+ * fundamental-mode phase velocity and its depth sensitivity kernels.  It is a
+ * drop-in for the reference's two Fortran subroutines
+ *
+ *   depthkernel(nx,ny,nz,vel,pvRc,sen_vsRc,sen_vpRc,sen_rhoRc,
+ *               iwave,igr,kmaxRc,tRc,depz,minthk)
+ *       code/src_inv_iso_joint/CalSurfG.f90:1-15   (ISO kernels, FD backend :86-107)
+ *   depthkernelTI(nx,ny,nz,vel,pvRc,iwave,igr,kmaxRc,tRc,depz,minthk,Lsen_Gsc)
+ *       code/src_inv_iso_joint/depthkernelTI.f90:2-34 (TI dc/dL splice :152-162)
+ *
+ * All arrays use the reference's Fortran column-major layout: the column index
+ * post = ii + (jj-1)*nx is the fastest dimension of every output.  Entry points
+ * taking host pointers are synchronous; entry points with the _dev suffix take
+ * device pointers and enqueue on the given HIP stream.  Calls from several host
+ * threads at once are not supported (the reference is not reentrant either).
+ *
+ * Return codes: 0 ok; otherwise one of DAZI_DK_E* (the Fortran shim turns a
+ * non-zero code into `stop <code>`), and dazi_dk_last_error() describes it.
+ * A period whose root is not found is not an error: as in the reference
+ * (surfdisp96.f:307-348) that period and all later ones of the column are 0.
+ */
+#ifndef DAZI_DK_H
+#define DAZI_DK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DAZI_DK_ABI_VERSION 1
+
+#define DAZI_DK_OK 0
+#define DAZI_DK_EINVAL 9301      /* invalid argument or unsupported option */
+#define DAZI_DK_EHIP 9302        /* HIP runtime error */
+#define DAZI_DK_ELIMIT 9303      /* size beyond a documented limit */
+#define DAZI_DK_ENODEV 9304      /* no usable GPU */
+
+/* Stage description: the reference's scalar arguments plus the para.in tile
+ * controls (pdsurf_tile_columns / ti_tile_columns). */
+typedef struct dazi_dk_desc {
+  int32_t nx, ny, nz;      /* columns = nx*ny; nz depth nodes */
+  int32_t kmax;            /* number of periods (kmaxRc) */
+  int32_t iwave;           /* 2 = Rayleigh: the only value the reference stage uses */
+  int32_t igr;             /* 0 = phase velocity: the only value the reference stage uses */
+  int32_t tile_columns;    /* columns per launch in plain mode (default 64) */
+  int32_t vp_rho_mode;     /* 0: vp, rho from Brocher(vs) as depthkernel does;
+                              1: explicit vp and rho node arrays */
+  float minthk0;           /* para.in "number of sublayers" (minthk) */
+  int32_t reserved;
+} dazi_dk_desc;
+
+/* Launch configuration of the root search. Zero fields take defaults. */
+typedef struct dazi_dk_tune {
+  int32_t lanes_per_scan;  /* trial velocities per scan step: 8, 16, 32 or 64 */
+  int32_t cols_per_wg;     /* columns per workgroup (lanes_per_scan*cols_per_wg <= 256) */
+  int32_t mode;            /* 0: batched persistent launch (default);
+                              1: plain, one launch per tile of tile_columns columns */
+  int32_t tile_columns;    /* plain-mode tile (default 64 = pdsurf_tile_columns) */
+  double refine_tol;       /* root bracket width before the final secant step, km/s
+                              (default 2e-6) */
+} dazi_dk_tune;
+
+/* Ragged batch of layered models, all pointers on the device.  Column j has
+ * nl[j] layers at loff[j] of thk/vp/vs/rho (half-space last, its thickness is
+ * ignored) and nk[j] ascending periods at toff[j] of periods.  Outputs are per
+ * solve s = toff[j]+k: c (float32-rounded phase velocity, 0 where not found)
+ * and, when non-NULL, refined-layer kernels dc/dVs, dc/dVp, dc/drho of the
+ * unflattened model at kern[s + nsolves*m] (m < nl[j]; rows beyond are
+ * untouched).  status[j] = 0, or the 1-based first failed period, or -1 for a
+ * model the stage does not handle (fluid layer). */
+typedef struct dazi_dk_layered {
+  int32_t ncol;
+  int32_t max_layers;      /* max over nl[] */
+  int64_t nsolves;         /* sum of nk[] */
+  const int32_t* nl;
+  const int64_t* loff;
+  const int32_t* nk;
+  const int64_t* toff;
+  const int32_t* solve_col; /* column of each solve (needed only for kernels) */
+  const float* thk;
+  const float* vp;
+  const float* vs;
+  const float* rho;
+  const double* periods;
+  double* c;
+  int32_t* status;
+  double* dc_dvs;
+  double* dc_dvp;
+  double* dc_drho;
+  double* c_unrounded;     /* optional (may be NULL): the float64 root before the
+                              float32 rounding of c, 0 where not found */
+} dazi_dk_layered;
+
+int32_t dazi_dk_abi_version(void);
+const char* dazi_dk_last_error(void);
+
+/* Default launch configuration used by every entry point without a tune
+ * argument (the Fortran/host path). */
+int32_t dazi_dk_set_tune(const dazi_dk_tune* tune);
+int32_t dazi_dk_get_tune(dazi_dk_tune* tune);
+
+/* Replaces depthkernel (CalSurfG.f90:1-15); host pointers, synchronous.
+ *   vel(nx,ny,nz) f32; vp/rho (nx,ny,nz) f32 only when desc->vp_rho_mode = 1;
+ *   depz(nz) f32; tRc(kmax) f64 ascending;
+ *   pvRc(nx*ny,kmax) f64; sen_*(nx*ny,kmax,nz) f64 node kernels (may all be
+ *   NULL for phase velocities only). */
+int32_t dazi_dk_iso(const dazi_dk_desc* desc, const float* vel, const float* vp,
+                    const float* rho, const float* depz, const double* tRc, double* pvRc,
+                    double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc);
+
+/* Same stage on device pointers, enqueued on `stream` (a hipStream_t, NULL =
+ * default stream).  status(nx*ny) i32 may be NULL.  tune may be NULL. */
+int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                        const float* vp, const float* rho, const float* depz, const double* tRc,
+                        double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
+                        int32_t* status, void* stream);
+
+/* Ragged layered batch (the batched persistent solver's native input). */
+int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
+                            void* stream);
+
+/* Event timings (ms) of the last call's phases on its stream:
+ * [0] model prep, [1] root search, [2] kernel pass, [3] number of forward
+ * launches.  Waits for those events. Returns the number of values written. */
+int32_t dazi_dk_last_timings(double* out, int32_t n);
+
+/* Enables (1) or disables (0) recording of the timings above. */
+int32_t dazi_dk_enable_timing(int32_t on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DAZI_DK_H */

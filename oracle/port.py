@@ -1,0 +1,293 @@
+"""ORACLE — test infrastructure only (tests/, smoke() and bench.py's cpu_baseline leg).
+
+ctypes access to two CPU implementations of the reference's per-column stage:
+
+* ``liborc_port.so`` — this repository's C restatement (``surfdisp96_port.c``);
+* ``_ref/libdazi_ref.so`` — the reference's own Fortran (surfdisp96.f, tregn96.f,
+  DepthKernel1D.f90, CalSurfG.f90, depthkernel_ti_worker.f90) compiled in place
+  with flang by ``oracle/Makefile``.  Present only where it was built.
+
+Array conventions follow the reference: Fortran column-major, float32 models,
+float64 periods and outputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PORT = os.path.join(_HERE, "liborc_port.so")
+_REF = os.path.join(_HERE, "_ref", "libdazi_ref.so")
+
+_port = None
+_ref = None
+
+f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+NL = 200
+NP = 160
+
+
+def port():
+    global _port
+    if _port is None:
+        if not os.path.exists(_PORT):
+            raise RuntimeError(f"oracle port library missing: {_PORT} (run make -C oracle)")
+        L = C.CDLL(_PORT)
+        L.orc_gtsolh.restype = C.c_float
+        L.orc_gtsolh.argtypes = [C.c_float, C.c_float]
+        L.orc_dltar4.restype = C.c_double
+        L.orc_dltar4.argtypes = [C.c_double, C.c_double, f32p, f32p, f32p, f32p, C.c_int, C.c_int]
+        L.orc_surfdisp96_x.restype = C.c_int
+        L.orc_surfdisp96_x.argtypes = [f32p, f32p, f32p, f32p, C.c_int, C.c_int, C.c_int, f64p,
+                                       f64p, C.c_double, C.c_int, C.c_void_p, C.c_void_p]
+        L.orc_brocher.argtypes = [C.c_int, f32p, f32p, f32p]
+        L.orc_refine.restype = C.c_int
+        L.orc_refine.argtypes = [C.c_float, C.c_int, f32p, f32p, f32p, f32p, f32p, f32p, f32p,
+                                 f32p, C.c_void_p]
+        L.orc_column_fd.restype = C.c_int
+        L.orc_column_fd.argtypes = [C.c_int, f32p, C.c_void_p, C.c_void_p, C.c_int, f64p, f32p,
+                                    C.c_float, C.c_float, C.c_float, C.c_float, C.c_double,
+                                    C.c_int, C.c_int, f64p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.orc_depthkernel.argtypes = [C.c_int, C.c_int, C.c_int, f32p, C.c_void_p, C.c_void_p,
+                                      f64p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, f64p,
+                                      f32p, C.c_float, C.c_int]
+        L.orc_forward_layered.argtypes = [C.c_int, i32p, i64p, i32p, i64p, f32p, f32p, f32p,
+                                          f32p, f64p, f64p, C.c_void_p, C.c_int]
+        L.orc_max_threads.restype = C.c_int
+        _port = L
+    return _port
+
+
+def ref_available() -> bool:
+    return os.path.exists(_REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError(f"reference library missing: {_REF} (run make -C oracle ref)")
+        _ref = C.CDLL(_REF)
+        _ref.dltar4_.restype = C.c_double
+    return _ref
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size < n:
+        b = np.zeros(n, np.float32)
+        b[: a.size] = a
+        return b
+    return a
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.size < n:
+        b = np.zeros(n, np.float64)
+        b[: a.size] = a
+        return b
+    return a
+
+
+# ----------------------------------------------------------------- models
+def brocher(vs, lib="port"):
+    vs = _f32(vs)
+    n = vs.size
+    vp = np.zeros(n, np.float32)
+    rho = np.zeros(n, np.float32)
+    if lib == "port":
+        port().orc_brocher(n, vs, vp, rho)
+    else:
+        ref().depthkernel_empirical_vp_rho_(C.byref(C.c_int(n)), _ptr(vs), _ptr(vp), _ptr(rho))
+    return vp, rho
+
+
+def refine(minthk, depz, vp, vs, rho, lib="port"):
+    """Node model -> refined layered model (thk, vp, vs, rho), half-space last."""
+    depz, vp, vs, rho = (_f32(x) for x in (depz, vp, vs, rho))
+    nz = depz.size
+    out = [np.zeros(NL, np.float32) for _ in range(5)]
+    if lib == "port":
+        L = port().orc_refine(C.c_float(minthk), nz, depz, vp, vs, rho, out[1], out[2], out[3],
+                              out[0], None)
+        if L < 0:
+            raise ValueError("refined model exceeds NL")
+    else:
+        Lc = C.c_int(0)
+        ref().refinegrid2layermdl_(C.byref(C.c_float(minthk)), C.byref(C.c_int(nz)), _ptr(depz),
+                                   _ptr(vp), _ptr(vs), _ptr(rho), C.byref(Lc), _ptr(out[4]),
+                                   _ptr(out[1]), _ptr(out[2]), _ptr(out[3]), _ptr(out[0]))
+        L = Lc.value
+    return out[0][:L].copy(), out[1][:L].copy(), out[2][:L].copy(), out[3][:L].copy()
+
+
+# ----------------------------------------------------------------- forward
+def surfdisp96(thk, vp, vs, rho, periods, iflsph=1, lib="port", tol_rel=1e-6, round_f32=True,
+               want_flat=False, want_nevals=False):
+    """Rayleigh fundamental-mode phase velocities of one layered model.
+
+    Returns cg (kmax,) float64 with zeros from the first failed period on.
+    lib='ref' calls the reference's own surfdisp96 (tol_rel/round_f32 fixed).
+    """
+    n = len(thk)
+    periods = _f64(periods)
+    kmax = periods.size
+    if lib == "port":
+        cg = np.zeros(kmax, np.float64)
+        flat = np.zeros(4 * n, np.float32) if want_flat else None
+        nev = C.c_int(0)
+        port().orc_surfdisp96_x(_f32(thk), _f32(vp), _f32(vs), _f32(rho), n, iflsph, kmax,
+                                periods, cg, tol_rel, 1 if round_f32 else 0,
+                                _ptr(flat) if want_flat else None, C.byref(nev))
+        out = [cg]
+        if want_flat:
+            out.append(flat.reshape(4, n))
+        if want_nevals:
+            out.append(nev.value)
+        return out[0] if len(out) == 1 else tuple(out)
+    cg = np.zeros(NP, np.float64)
+    ref().surfdisp96_(_ptr(_f32(thk, NL)), _ptr(_f32(vp, NL)), _ptr(_f32(vs, NL)),
+                      _ptr(_f32(rho, NL)), C.byref(C.c_int(n)), C.byref(C.c_int(iflsph)),
+                      C.byref(C.c_int(2)), C.byref(C.c_int(1)), C.byref(C.c_int(0)),
+                      C.byref(C.c_int(kmax)), _ptr(_f64(periods, NP)), _ptr(cg))
+    return cg[:kmax].copy()
+
+
+def gtsolh(a, b, lib="port"):
+    if lib == "port":
+        return np.float32(port().orc_gtsolh(a, b))
+    c = C.c_float(0)
+    ref().gtsolh_(C.byref(C.c_float(a)), C.byref(C.c_float(b)), C.byref(c))
+    return np.float32(c.value)
+
+
+def dltar4(c, period, d, a, b, rho, lib="port", llw=1):
+    """Secular function at phase velocity c and period, on an already-flattened model."""
+    omega = 2.0 * 3.141592653589793 / period
+    wvno = omega / c
+    n = len(d)
+    if lib == "port":
+        return port().orc_dltar4(wvno, omega, _f32(d), _f32(a), _f32(b), _f32(rho), n, llw)
+    z = np.zeros(NL, np.float32)
+    return ref().dltar4_(C.byref(C.c_double(wvno)), C.byref(C.c_double(omega)), _ptr(_f32(d, NL)),
+                         _ptr(_f32(a, NL)), _ptr(_f32(b, NL)), _ptr(_f32(rho, NL)), _ptr(z),
+                         _ptr(z), _ptr(z), C.byref(C.c_int(n)), C.byref(C.c_int(llw)),
+                         C.byref(C.c_double(2 * np.pi)))
+
+
+# ----------------------------------------------------------------- FD depth kernels
+def column_fd(vs, periods, depz, minthk, vp=None, rho=None, lib="port", dln=0.01,
+              tol_rel=1e-6, round_f32=True, kernels=True, exact_step=False):
+    """One column of the reference FD depth kernel.
+
+    Returns pv (kmax,), sen_vs/sen_vp/sen_rho (kmax, nz) (None when kernels=False).
+    tol_rel=1e-13, round_f32=False, exact_step=True give a tight FD oracle.
+    """
+    vs = _f32(vs)
+    nz = vs.size
+    periods = _f64(periods)
+    kmax = periods.size
+    depz = _f32(depz)
+    pv = np.zeros(kmax, np.float64)
+    if lib == "port":
+        sens = [np.zeros(kmax * nz, np.float64) for _ in range(3)] if kernels else [None] * 3
+        vpa = _f32(vp) if vp is not None else None
+        rhoa = _f32(rho) if rho is not None else None
+        port().orc_column_fd(nz, vs, _ptr(vpa) if vpa is not None else None,
+                             _ptr(rhoa) if rhoa is not None else None, kmax, periods, depz,
+                             C.c_float(minthk), dln, dln, dln, tol_rel, 1 if round_f32 else 0,
+                             1 if exact_step else 0, pv,
+                             *[_ptr(s) if s is not None else None for s in sens])
+        if not kernels:
+            return pv, None, None, None
+        return (pv,) + tuple(s.reshape(nz, kmax).T.copy() for s in sens)
+    if vp is not None:
+        raise ValueError("the reference column driver derives vp/rho from vs")
+    sens = [np.zeros(kmax * nz, np.float64) for _ in range(3)]
+    d = C.c_float(dln)
+    ref().depthkernel_column_fd_(C.byref(C.c_int(nz)), _ptr(vs), C.byref(C.c_int(kmax)),
+                                 _ptr(periods), C.byref(C.c_int(2)), C.byref(C.c_int(0)),
+                                 _ptr(depz), C.byref(C.c_float(minthk)), C.byref(d), C.byref(d),
+                                 C.byref(d), _ptr(pv), *[_ptr(s) for s in sens])
+    return (pv,) + tuple(s.reshape(nz, kmax).T.copy() for s in sens)
+
+
+def depthkernel(vel, tRc, depz, minthk, lib="port", nthreads=0, kernels=True, vp=None, rho=None):
+    """The stage call: vel (nx, ny, nz) -> pv (nx*ny, kmax), sen_* (nx*ny, kmax, nz).
+
+    Arrays use Fortran order semantics (column index fastest); numpy arrays are
+    returned with shape (ncol, kmax[, nz]) in Fortran memory order.
+    """
+    vel = np.asarray(vel, np.float32)
+    nx, ny, nz = vel.shape
+    velF = np.ascontiguousarray(vel.reshape(-1, order="F"))
+    tRc = _f64(tRc)
+    kmax = tRc.size
+    ncol = nx * ny
+    pv = np.zeros(ncol * kmax, np.float64)
+    sens = [np.zeros(ncol * kmax * nz, np.float64) for _ in range(3)] if kernels else None
+    if lib == "port":
+        vpF = np.ascontiguousarray(np.asarray(vp, np.float32).reshape(-1, order="F")) \
+            if vp is not None else None
+        rhoF = np.ascontiguousarray(np.asarray(rho, np.float32).reshape(-1, order="F")) \
+            if rho is not None else None
+        port().orc_depthkernel(nx, ny, nz, velF, _ptr(vpF) if vpF is not None else None,
+                               _ptr(rhoF) if rhoF is not None else None, pv,
+                               *([_ptr(s) for s in sens] if kernels else [None] * 3), kmax, tRc,
+                               _f32(depz), C.c_float(minthk), nthreads)
+    else:
+        if not kernels or vp is not None:
+            raise ValueError("reference depthkernel always computes kernels from Brocher vp/rho")
+        if nthreads > 0:
+            os.environ["OMP_NUM_THREADS"] = str(nthreads)
+        os.environ["DAZI_DEPTHKERNEL_BACKEND"] = "default_fortran_fd"
+        os.environ.pop("DAZI_DEPTHKERNEL_DUMP_DIR", None)
+        i = lambda v: C.byref(C.c_int(v))  # noqa: E731
+        ref().depthkernel_(i(nx), i(ny), i(nz), _ptr(velF), _ptr(pv), *[_ptr(s) for s in sens],
+                           i(2), i(0), i(kmax), _ptr(tRc), _ptr(_f32(depz)),
+                           C.byref(C.c_float(minthk)))
+    pv = pv.reshape(ncol, kmax, order="F")
+    if not kernels:
+        return pv
+    return (pv,) + tuple(s.reshape(ncol, kmax, nz, order="F") for s in sens)
+
+
+def forward_layered(nl, kk, thk, vp, vs, rho, periods, nthreads=0):
+    """Ragged batch forward: per column surfdisp96 over flat concatenated arrays."""
+    nl = np.ascontiguousarray(nl, np.int32)
+    kk = np.ascontiguousarray(kk, np.int32)
+    loff = np.concatenate([[0], np.cumsum(nl)[:-1]]).astype(np.int64)
+    toff = np.concatenate([[0], np.cumsum(kk)[:-1]]).astype(np.int64)
+    cg = np.zeros(int(kk.sum()), np.float64)
+    fail = np.zeros(nl.size, np.int32)
+    port().orc_forward_layered(nl.size, nl, loff, kk, toff, _f32(thk), _f32(vp), _f32(vs),
+                               _f32(rho), _f64(periods), cg, _ptr(fail), nthreads)
+    return cg, fail
+
+
+def ti_column(vs, periods, depz, minthk):
+    """Reference TI column (depthkernel_ti_column): pv (kmax,), lsen (kmax, nz-1) float32."""
+    if not ref_available():
+        raise RuntimeError("reference library not built")
+    vs = _f32(vs)
+    nz = vs.size
+    periods = _f64(periods)
+    kmax = periods.size
+    pv = np.zeros(kmax, np.float64)
+    lsen = np.zeros(kmax * (nz - 1), np.float32)
+    i = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    ref().depthkernel_ti_column_(i(nz), _ptr(vs), i(kmax), _ptr(periods), i(2), i(0),
+                                 _ptr(_f32(depz)), C.byref(C.c_float(minthk)), _ptr(pv),
+                                 _ptr(lsen))
+    return pv, lsen.reshape(nz - 1, kmax).T.copy()

@@ -1,0 +1,143 @@
+"""Writes the golden fixtures of tests/golden/ (run once in the dev container).
+
+Sources (nothing from them is copied except data values):
+  * the reference's example data files
+      examples/test4_Yunnan/voro/MOD                 (initial node model, 38x42x18)
+      examples/test4_Yunnan/voro/period_phaseVMOD.dat (its phase-velocity map, f10.4)
+    -> yunnan_golden.npz
+  * the reference's own Fortran compiled in place by oracle/Makefile
+    (oracle/_ref/libdazi_ref.so), run single-threaded on seeded synthetic
+    columns -> ref_*.npz
+The manifest records the seeds, sizes and the digests of the source files.
+
+Usage: python tests/golden/make_fixtures.py   (needs /root/reference and `make -C oracle ref`)
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from dazi_current_release_amd import configs  # noqa: E402
+from oracle import port as P  # noqa: E402
+
+REF = "/root/reference/examples/test4_Yunnan/voro"
+
+
+def sha(path):
+    with open(path, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def yunnan():
+    nx, ny, nz = 38, 42, 18
+    tok = open(os.path.join(REF, "MOD")).read().split()
+    depz = np.array(tok[:nz], np.float32)
+    vals = np.array(tok[nz:], np.float32)
+    vel = np.zeros((nx, ny, nz), np.float32)
+    idx = 0
+    for k in range(nz):            # Main_Jt.f90:518-526: read(11,*)(vsf(i,j,k),i=1,nx)
+        for j in range(ny):
+            vel[:, j, k] = vals[idx:idx + nx]
+            idx += nx
+    g = np.loadtxt(os.path.join(REF, "period_phaseVMOD.dat"))
+    periods = np.arange(5, 41, dtype=np.float64)   # para.in:19-20
+    ni = (nx - 2) * (ny - 2)
+    # WTPeriodPhaseV (Main_Jt.f90:2019-2038): rows ordered (period, jj, ii)
+    gold = g[:, 3].reshape(len(periods), ni).T      # (interior column, period)
+    np.savez_compressed(os.path.join(HERE, "yunnan_golden.npz"), vel=vel, depz=depz,
+                        periods=periods, minthk=np.float32(4.0),
+                        c_gold_e4=np.round(gold * 1e4).astype(np.int32),
+                        lon=g[:ni, 0].astype(np.float32), lat=g[:ni, 1].astype(np.float32))
+    return dict(MOD=sha(os.path.join(REF, "MOD")),
+                period_phaseVMOD=sha(os.path.join(REF, "period_phaseVMOD.dat")))
+
+
+def ref_node_columns(name, ncol, seed):
+    """Reference depthkernel_column_fd on the first ncol columns (Brocher vp/rho)."""
+    W = configs.node_workload(name, ncol=ncol)
+    vel = W["vel"][:, 0, :]
+    pv, svs, svp, srho = [], [], [], []
+    for j in range(ncol):
+        out = P.column_fd(vel[j], W["periods"], W["depz"], W["minthk"], lib="ref")
+        for lst, a in zip((pv, svs, svp, srho), out):
+            lst.append(a)
+    ti_pv, ti_l = [], []
+    for j in range(min(ncol, 16)):
+        a, b = P.ti_column(vel[j], W["periods"], W["depz"], W["minthk"])
+        ti_pv.append(a)
+        ti_l.append(b)
+    np.savez_compressed(os.path.join(HERE, f"ref_{name}.npz"), vel=vel, depz=W["depz"],
+                        periods=W["periods"], minthk=np.float32(W["minthk"]),
+                        pv=np.array(pv), sen_vs=np.array(svs), sen_vp=np.array(svp),
+                        sen_rho=np.array(srho), ti_pv=np.array(ti_pv), ti_lsen=np.array(ti_l))
+
+
+def ref_layered(ncol):
+    """Reference surfdisp96 on ragged layered columns (forward only)."""
+    W = configs.layered_workload("ragged_16384", ncol=ncol)
+    nl, nk = W["nl"], W["nk"]
+    loff = np.concatenate([[0], np.cumsum(nl)[:-1]])
+    toff = np.concatenate([[0], np.cumsum(nk)[:-1]])
+    c = np.zeros(int(nk.sum()))
+    for j in range(ncol):
+        o, t = loff[j], toff[j]
+        c[t:t + nk[j]] = P.surfdisp96(W["thk"][o:o + nl[j]], W["vp"][o:o + nl[j]],
+                                      W["vs"][o:o + nl[j]], W["rho"][o:o + nl[j]],
+                                      W["periods"][t:t + nk[j]], lib="ref")
+    np.savez_compressed(os.path.join(HERE, "ref_ragged.npz"), c=c, **W)
+
+
+def ref_functions(seed=11):
+    """Per-function vectors: gtsolh, refinement, flattened dltar4 values."""
+    rng = np.random.default_rng(seed)
+    b = rng.uniform(0.5, 5.0, 200).astype(np.float32)
+    a = (b * rng.uniform(1.5, 2.2, 200)).astype(np.float32)
+    g = np.array([P.gtsolh(ai, bi, "ref") for ai, bi in zip(a, b)], np.float32)
+    # dltar4 on flattened 4-layer models at assorted (c, T)
+    recs = []
+    for i in range(64):
+        L = 4
+        thk = rng.uniform(1, 20, L).astype(np.float32)
+        vs = rng.uniform(2.5, 4.8, L).astype(np.float32)
+        vp, rho = P.brocher(vs, "ref")
+        _, flat = P.surfdisp96(thk, vp, vs, rho, [10.0], lib="port", want_flat=True)
+        for c, T in ((rng.uniform(2.0, 4.8), rng.uniform(1, 60)) for _ in range(4)):
+            v = P.dltar4(c, T, *flat, lib="ref")
+            recs.append([i, c, T, v, *flat.reshape(-1)])
+    np.savez_compressed(os.path.join(HERE, "ref_functions.npz"), gt_a=a, gt_b=b, gt_c=g,
+                        dl=np.array(recs))
+
+
+def main():
+    if not P.ref_available():
+        raise SystemExit("build the reference first: make -C oracle ref")
+    man = dict(generator="tests/golden/make_fixtures.py",
+               reference_build="oracle/Makefile (flang -O2 -ffp-contract=off), serial")
+    man["sources"] = yunnan()
+    ref_node_columns("plain_1024", 32, 1)
+    ref_node_columns("plain_4096", 8, 2)
+    ref_layered(48)
+    ref_functions()
+    man["fixtures"] = {
+        "yunnan_golden.npz": "MOD 38x42x18 + period_phaseVMOD.dat (1440 interior columns x 36 periods)",
+        "ref_plain_1024.npz": "reference column FD (Brocher) on 32 columns of plain_1024 (seed 1); "
+                              "TI depthkernel_ti_column on the first 16",
+        "ref_plain_4096.npz": "same on 8 columns of plain_4096 (seed 2)",
+        "ref_ragged.npz": "reference surfdisp96 on 48 ragged columns of ragged_16384 (seed 3)",
+        "ref_functions.npz": "gtsolh (200 pairs, seed 11) and dltar4 on 64 flattened 4-layer models",
+    }
+    with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
+        json.dump(man, f, indent=1)
+    print("fixtures written")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,65 @@
+"""CPU tests of the C ABI library: it loads and exports every declared entry point.
+
+No compute call is made here (no GPU in this environment); argument checks that
+return before any HIP call are exercised.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+from dazi_current_release_amd import _lib
+
+HEADER = os.path.join(ROOT, "include", "dazi_dk.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(dazi_dk_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = _lib.lib()
+    names = declared_functions()
+    assert len(names) >= 9
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert L.dazi_dk_abi_version() == 1
+
+
+def test_struct_layouts_match_header():
+    assert C.sizeof(_lib.Desc) == 40
+    assert C.sizeof(_lib.Tune) == 24
+    assert C.sizeof(_lib.Layered) == 8 + 8 + 8 * 16
+
+
+def test_invalid_sizes_rejected_before_any_gpu_call():
+    L = _lib.lib()
+    d = _lib.Desc(0, 1, 13, 18, 2, 0, 64, 0, 0.5, 0)
+    buf = np.zeros(16, np.float64)
+    p = buf.ctypes.data_as(C.c_void_p)
+    rc = L.dazi_dk_iso(C.byref(d), p, None, None, p, p, p, None, None, None)
+    assert rc == _lib.DAZI_DK_EINVAL
+    assert b"bad sizes" in L.dazi_dk_last_error()
+    rc = L.dazi_dk_iso(C.byref(d), None, None, None, p, p, p, None, None, None)
+    assert rc == _lib.DAZI_DK_EINVAL
+
+
+def test_tune_validation():
+    L = _lib.lib()
+    saved = _lib.Tune()
+    assert L.dazi_dk_get_tune(C.byref(saved)) == 0
+    bad = _lib.make_tune(12, 4, 0, 64, 2e-6)
+    assert L.dazi_dk_set_tune(C.byref(bad)) == _lib.DAZI_DK_EINVAL
+    bad = _lib.make_tune(16, 3, 0, 64, 2e-6)      # 48 threads: not whole waves
+    assert L.dazi_dk_set_tune(C.byref(bad)) == _lib.DAZI_DK_EINVAL
+    good = _lib.make_tune(32, 8, 1, 64, 1e-6)
+    assert L.dazi_dk_set_tune(C.byref(good)) == 0
+    got = _lib.Tune()
+    L.dazi_dk_get_tune(C.byref(got))
+    assert (got.lanes_per_scan, got.cols_per_wg, got.mode) == (32, 8, 1)
+    assert L.dazi_dk_set_tune(C.byref(saved)) == 0

@@ -1,0 +1,158 @@
+"""GPU tests of the batched (ragged, persistent) solver through the C ABI.
+
+Bars: found/zero-filled pattern and status identical to the oracle; |dc| <= 1e-5
+km/s; refined-layer kernels <= 1e-4 (per-solve max-norm) against a tight
+central-difference oracle; exact invariants of the kernels at full size.
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden
+from oracle import port as P
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dk():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    import dazi_current_release_amd as m
+    m.lib()
+    return m
+
+
+def run(dk, W, kernels=False, tune=None, periods=None):
+    b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"],
+                        W["periods"] if periods is None else periods)
+    out = dk.layered_dev(b, kernels=kernels, tune=tune)
+    return b, {k: v.cpu().numpy() for k, v in out.items()}
+
+
+def offsets(n):
+    return np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+
+
+def test_ragged_vs_reference_fixture(dk):
+    f = load_golden("ref_ragged.npz")
+    _, o = run(dk, f)
+    assert np.array_equal(o["c"] == 0, f["c"] == 0)
+    assert np.abs(o["c"] - f["c"]).max() <= 1e-5
+
+
+@pytest.mark.parametrize("lanes,cpw,mode", [(16, 16, 0), (64, 1, 0), (8, 8, 0), (32, 2, 1)])
+def test_ragged_2000_vs_oracle(dk, lanes, cpw, mode):
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload("ragged_16384", ncol=2000)
+    ref, fail = P.forward_layered(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"],
+                                  W["periods"], nthreads=16)
+    _, o = run(dk, W, tune=dk.make_tune(lanes, cpw, mode, 64, 2e-6))
+    assert np.array_equal(o["c"] == 0, ref == 0)
+    assert np.abs(o["c"] - ref).max() <= 1e-5
+    assert np.array_equal(o["status"], fail)
+
+
+def stress_models(n, seed=5):
+    """Inverted / low-velocity-zone layered models, some of which make the
+    reference's root search fail or scan downward."""
+    rng = np.random.default_rng(seed)
+    nl = rng.integers(3, 12, size=n).astype(np.int32)
+    nk = rng.integers(5, 20, size=n).astype(np.int32)
+    thk, vs, per = [], [], []
+    for j in range(n):
+        L = int(nl[j])
+        t = rng.uniform(1.0, 25.0, L)
+        t[-1] = 0
+        v = rng.uniform(1.5, 4.8, L)
+        if rng.uniform() < 0.5:
+            v = np.sort(v)[::-1].copy()            # velocity decreasing with depth
+        thk.append(t)
+        vs.append(v)
+        per.append(np.sort(rng.uniform(1, 80, int(nk[j]))))
+    from dazi_current_release_amd import configs
+    vs = np.concatenate(vs).astype(np.float32)
+    vp = configs.brocher_vp(vs)
+    return dict(nl=nl, nk=nk, thk=np.concatenate(thk).astype(np.float32), vs=vs, vp=vp,
+                rho=configs.brocher_rho(vp), periods=np.concatenate(per))
+
+
+def test_stress_models_failures_and_reversals(dk):
+    W = stress_models(3000)
+    ref, fail = P.forward_layered(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"],
+                                  W["periods"], nthreads=16)
+    assert (fail > 0).sum() > 20, "the stress set must contain failing columns"
+    _, o = run(dk, W)
+    same = np.array_equal(o["status"], fail) and np.array_equal(o["c"] == 0, ref == 0)
+    if not same:
+        bad = np.nonzero(o["status"] != fail)[0]
+        raise AssertionError(f"{bad.size} columns differ in status, e.g. {bad[:10]}")
+    assert np.abs(o["c"] - ref).max() <= 1e-5
+
+
+def test_layer_kernels_vs_tight_fd(dk):
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload("ragged_16384", ncol=6)
+    _, o = run(dk, W, kernels=True)
+    loff, toff = offsets(W["nl"]), offsets(W["nk"])
+    ns = int(W["nk"].sum())
+    for j in range(6):
+        L, K = int(W["nl"][j]), int(W["nk"][j])
+        sl = slice(loff[j], loff[j] + L)
+        per = W["periods"][toff[j]:toff[j] + K]
+        mdl = [W[k][sl].copy() for k in ("thk", "vp", "vs", "rho")]
+        def central(which, m, rel):
+            v0 = mdl[which][m]
+            h = np.float32(rel) * v0
+            lo, hi = np.float32(v0 - h), np.float32(v0 + h)
+            mdl[which][m] = lo
+            c1 = P.surfdisp96(*mdl, per, tol_rel=1e-13, round_f32=False)
+            mdl[which][m] = hi
+            c2 = P.surfdisp96(*mdl, per, tol_rel=1e-13, round_f32=False)
+            mdl[which][m] = v0
+            return (c2 - c1) / (float(hi) - float(lo))
+
+        for which, key in ((2, "dc_dvs"), (1, "dc_dvp"), (3, "dc_drho")):
+            fd = np.zeros((K, L))
+            for m in range(L):
+                # Richardson: removes the O(h^2) truncation of the central difference
+                fd[:, m] = (4 * central(which, m, 0.0025) - central(which, m, 0.005)) / 3
+            g = o[key][:L, toff[j]:toff[j] + K].T
+            scale = np.abs(fd).max(axis=1, keepdims=True)
+            assert (np.abs(g - fd) / scale).max() <= 1e-4, (j, key)
+    assert o["dc_dvs"].shape == (int(W["nl"].max()), ns)
+
+
+def test_invariants_full_ragged(dk):
+    """Exact identities of the refined-layer kernels on 16,384 ragged columns:
+    sum rho dc/drho = 0 and sum (a dc/da + b dc/db) = c + T dc/dT."""
+    import torch
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload("ragged_16384")
+    # dc/dT comes from a central difference in T, which amplifies root error by
+    # 1/(2 eps): solve these roots to a 1e-11 km/s bracket
+    tight = dk.make_tune(16, 16, 0, 64, 1e-11)
+    b, o = run(dk, W, kernels=True, tune=tight)
+    eps = 1e-4
+    _, op = run(dk, W, periods=W["periods"] * (1 + eps), tune=tight)
+    _, om = run(dk, W, periods=W["periods"] * (1 - eps), tune=tight)
+    loff = offsets(W["nl"])
+    ns = b.nsolves
+    col = np.repeat(np.arange(b.ncol), W["nk"])
+    mask = np.zeros((b.max_layers, ns), bool)
+    idx = np.zeros((b.max_layers, ns), np.int64)
+    for m in range(b.max_layers):
+        mask[m] = m < W["nl"][col]
+        idx[m] = np.where(mask[m], loff[col] + m, 0)
+    rho, vp, vs = W["rho"][idx], W["vp"][idx], W["vs"][idx]
+    c = o["c_unrounded"]
+    ok = (c > 0) & (op["c_unrounded"] > 0) & (om["c_unrounded"] > 0)
+    wr = np.where(mask, rho * o["dc_drho"], 0.0)
+    # exact up to the float32 rounding of the flattened model (~6e-8 per layer);
+    # scaled by c, since a wave trapped in a thick slow layer has dc/drho ~ 0
+    assert (np.abs(wr.sum(0)) / c)[ok].max() < 1e-6
+    lhs = np.where(mask, vp * o["dc_dvp"] + vs * o["dc_dvs"], 0.0).sum(0)
+    dcdT = (op["c_unrounded"] - om["c_unrounded"]) / (2 * eps * W["periods"])
+    rhs = c + W["periods"] * dcdT
+    assert (np.abs(lhs - rhs) / c)[ok].max() < 1e-5
+    assert ok.mean() > 0.99
+    del torch
