@@ -3,13 +3,15 @@
 Workload (BASELINE metric's production configuration): plain_4096 — 64x64 = 4,096
 columns, 25 depth nodes 0-120 km refined to 49 layers, 30 log-spaced periods
 1-60 s; one step = model prep + Rayleigh fundamental-mode root search (batched
-persistent launch) + isotropic node kernels for every column x period, FP64.
-Inputs are resident in HBM before the timed region.
+persistent launch) + isotropic node kernels dc/dVs, dc/dVp, dc/drho + the TI
+dc/dA, dc/dL splice Lsen_Gsc for every column x period, FP64.  Inputs are
+resident in HBM before the timed region.
 
 Prints one JSON line with value = column x period solves per second, the
-roofline of the dominant kernel (root search, FP64 VALU bound) and a CPU
-baseline: the reference's own Fortran depthkernel (flang build in
-oracle/_ref/, OpenMP over columns) timed on a bounded column sample.
+roofline of the dominant kernel (FP64 VALU bound) and a CPU baseline: the
+reference's own Fortran (flang build in oracle/_ref/): depthkernel with OpenMP
+over columns plus depthkernel_ti_column in a process pool, on a bounded column
+sample.
 """
 from __future__ import annotations
 
@@ -29,12 +31,23 @@ PEAK_FP64 = 78.6e12          # MI355X FP64 vector (and dense matrix) peak, FLOP/
 FLOP_PER_LAYER_EVAL = 172    # FP64 ops per layer per secular-function evaluation
 N_EV = 12                    # evaluations per root of the reference's bracket + refinement
 N_K_ISO = 3                  # evaluation-equivalents of the isotropic kernel pass
+N_K_TI = 2                   # evaluation-equivalents of the TI kernel pass
+
+
+def _ti_chunk(args):
+    from oracle import port as P
+    vel, periods, depz, minthk = args
+    for v in vel:
+        P.ti_column(v, periods, depz, minthk)
+    return len(vel)
 
 
 def cpu_baseline_child(ncols: int, threads: int) -> None:
-    """Child process: the reference depthkernel (or the C port when the reference
-    library is not built) on the first ncols columns of the workload."""
+    """Child process: the reference stage on the first ncols columns of the
+    workload — depthkernel (FD kernels, OpenMP over columns) and the TI column
+    routine in a pool of `threads` processes (tregn96 keeps global state)."""
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    import multiprocessing as mp
     import numpy as np
     from oracle import port as P
     from dazi_current_release_amd import configs
@@ -45,10 +58,21 @@ def cpu_baseline_child(ncols: int, threads: int) -> None:
     # period counts as the GPU workload
     t0 = time.perf_counter()
     P.depthkernel(W["vel"], W["periods"], W["depz"], W["minthk"], lib=lib, nthreads=threads)
-    dt = time.perf_counter() - t0
+    t_iso = time.perf_counter() - t0
+    t_ti = 0.0
+    if kind == "reference":
+        vel = W["vel"][:, 0, :]
+        chunks = [(vel[i::threads], W["periods"], W["depz"], W["minthk"]) for i in range(threads)]
+        ctx = mp.get_context("fork")
+        with ctx.Pool(threads) as pool:
+            t1 = time.perf_counter()
+            pool.map(_ti_chunk, chunks)
+            t_ti = time.perf_counter() - t1
+    dt = t_iso + t_ti
     nsol = ncols * len(W["periods"])
     sys.stdout.flush()
-    print("CPUBASE " + json.dumps(dict(value=nsol / dt, seconds=dt, kind=kind, cols=ncols,
+    print("CPUBASE " + json.dumps(dict(value=nsol / dt, seconds=dt, iso_seconds=t_iso,
+                                       ti_seconds=t_ti, kind=kind, cols=ncols,
                                        threads=threads, solves=nsol)), flush=True)
 
 
@@ -67,8 +91,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-cols", type=int, default=320,
-                    help="columns of the CPU baseline sample (about 10 s at 16 threads)")
+    ap.add_argument("--cpu-cols", type=int, default=640,
+                    help="columns of the CPU baseline sample (about 10-20 s at 16 threads)")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -108,8 +132,8 @@ def main():
     stream = torch.cuda.current_stream()
 
     def step(out=None, t=tune):
-        return dk.iso_dev(vel, depz, per, W["minthk"], vp=vp, rho=rho, kernels=True, tune=t,
-                          out=out, stream=stream)
+        return dk.stage_dev(vel, depz, per, W["minthk"], vp=vp, rho=rho, iso=True, ti=True,
+                            tune=t, out=out, stream=stream)
 
     out = step()
     for _ in range(args.warmup):
@@ -132,9 +156,10 @@ def main():
         ph.append(dk.last_timings())
     _lib.lib().dazi_dk_enable_timing(0)
     ph = np.array(ph)
+    prep_ms = float(ph[:, 0].mean())
     fwd_ms = float(ph[:, 1].mean())
     kern_ms = float(ph[:, 2].mean())
-    prep_ms = float(ph[:, 0].mean())
+    ti_ms = float(ph[:, 3].mean())
 
     # accuracy guard of the measured output: zero-fill pattern and |dc| against
     # the oracle on a column subset
@@ -142,16 +167,24 @@ def main():
     pv = out["pv"].cpu().numpy()
 
     L = 49
-    flop_fwd = nsolves * N_EV * FLOP_PER_LAYER_EVAL * L
-    achieved = flop_fwd / (fwd_ms * 1e-3)
+    # algorithmic FP64 work per solve: 172 ops per layer per evaluation-equivalent
+    phases = {"k_forward (root search)": (fwd_ms, N_EV),
+              "k_kernels (isotropic adjoint)": (kern_ms, N_K_ISO),
+              "k_ti (TI kernels)": (ti_ms, N_K_TI)}
+    dom = max(phases, key=lambda k: phases[k][0])
+    dom_ms, dom_nev = phases[dom]
+    flop_dom = nsolves * dom_nev * FLOP_PER_LAYER_EVAL * L
+    achieved = flop_dom / (dom_ms * 1e-3)
+    flop_step = nsolves * (N_EV + N_K_ISO + N_K_TI) * FLOP_PER_LAYER_EVAL * L
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12,
                     unit="TFLOP/s", frac=achieved / PEAK_FP64, traffic=None,
-                    kernel="k_forward (root search)", pipe="FP64 VALU",
-                    flop_per_launch=flop_fwd, avg_launch_ms=fwd_ms)
+                    kernel=dom, pipe="FP64 VALU (no MFMA: not a contraction)",
+                    flop_per_launch=flop_dom, avg_launch_ms=dom_ms,
+                    step_frac=flop_step / (ms_per_step * 1e-3) / PEAK_FP64)
 
     plain = None
     if not args.no_plain:
-        tp = dk.make_tune(args.lanes, 4, 1, 64, 2e-6)
+        tp = dk.make_tune(args.lanes, max(1, 64 // args.lanes), 1, 64, 2e-6)
         step(out, tp)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
@@ -169,9 +202,11 @@ def main():
             cb = run_cpu_baseline(args.cpu_cols, threads)
             cpu = dict(value=cb["value"], unit="solves/s", cores=cb["threads"], kind=cb["kind"],
                        sample=(f"first {cb['cols']} columns of plain_4096 ({cb['solves']} "
-                               f"column-period solves, c + FD node kernels = 151 surfdisp96 "
-                               f"curves per column), reference depthkernel with OpenMP, "
-                               f"{cb['seconds']:.1f} s"))
+                               f"column-period solves): reference depthkernel, FD node "
+                               f"kernels = 151 surfdisp96 curves per column, OpenMP "
+                               f"{cb['iso_seconds']:.1f} s + reference depthkernel_ti_column "
+                               f"(surfdisp96 + tregn96) in a process pool "
+                               f"{cb['ti_seconds']:.1f} s"))
         except Exception as e:  # the GPU figure stands on its own
             cpu = dict(value=None, unit="solves/s", cores=threads, kind="reference",
                        sample=f"failed: {e}")
@@ -181,10 +216,10 @@ def main():
         warmup=args.warmup, ms_per_step=ms_per_step, higher_is_better=True, scaling="weak",
         vs_baseline=None, dtype="f64", data="synthetic",
         config=dict(workload="plain_4096", columns=ncol, nodes=nz, layers=L, periods=K,
-                    outputs="c + dc/dVs, dc/dVp, dc/drho node kernels",
+                    outputs="c + dc/dVs, dc/dVp, dc/drho node kernels + Lsen_Gsc (TI)",
                     mode="batched persistent", lanes_per_scan=args.lanes,
                     cols_per_wg=args.cpw, phase_ms=dict(prep=prep_ms, root_search=fwd_ms,
-                                                        kernels=kern_ms),
+                                                        iso_kernels=kern_ms, ti_kernels=ti_ms),
                     failed_columns=int((status != 0).sum()),
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,

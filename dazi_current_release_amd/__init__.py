@@ -6,7 +6,9 @@ behind a C ABI (include/dazi_dk.h).  The Python layer mirrors the reference's
 ``depthkernel`` interface and exposes device-tensor entry points.
 """
 from ._lib import DaziDkError, lib, make_tune, last_timings  # noqa: F401
-from .stage import depthkernel, run_depthkernel, iso_dev, layered_dev, LayeredBatch  # noqa: F401
+from .stage import (depthkernel, run_depthkernel, iso_dev, layered_dev, LayeredBatch,  # noqa: F401
+                    depthkernelTI, run_depthkernelTI, stage_dev)
 
 __all__ = ["DaziDkError", "lib", "make_tune", "last_timings", "depthkernel", "run_depthkernel",
-           "iso_dev", "layered_dev", "LayeredBatch"]
+           "iso_dev", "layered_dev", "LayeredBatch", "depthkernelTI", "run_depthkernelTI",
+           "stage_dev"]

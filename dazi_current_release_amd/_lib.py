@@ -20,7 +20,7 @@ DAZI_DK_ENODEV = 9304
 EXPORTS = [
     "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
-    "dazi_dk_enable_timing",
+    "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
 ]
 
 
@@ -77,6 +77,9 @@ def lib():
         L.dazi_dk_iso.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 9
         L.dazi_dk_iso_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 11
         L.dazi_dk_layered_dev.argtypes = [C.POINTER(Layered), C.c_void_p, C.c_void_p]
+        L.dazi_dk_ti.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 5
+        L.dazi_dk_ti_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 9
+        L.dazi_dk_stage_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 12
         L.dazi_dk_last_timings.argtypes = [C.POINTER(C.c_double), C.c_int32]
         L.dazi_dk_last_timings.restype = C.c_int32
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]
@@ -98,6 +101,7 @@ def make_tune(lanes_per_scan=0, cols_per_wg=0, mode=0, tile_columns=0, refine_to
 
 
 def last_timings():
-    buf = (C.c_double * 4)()
-    n = lib().dazi_dk_last_timings(buf, 4)
+    """[prep, root search, iso kernels, TI kernels] in ms and the launch count."""
+    buf = (C.c_double * 5)()
+    n = lib().dazi_dk_last_timings(buf, 5)
     return list(buf)[:n]

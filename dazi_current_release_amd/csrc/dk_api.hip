@@ -58,7 +58,10 @@ enum Slot {
   S_H_SVS,
   S_H_SVP,
   S_H_SRHO,
-  S_PV_DUMMY,
+  S_TI_LAYERS,
+  S_TI_SCRATCH,
+  S_TISTAT,
+  S_H_LSEN,
   S_COUNT
 };
 
@@ -67,7 +70,7 @@ struct Pool {
   void* p[S_COUNT] = {};
   size_t n[S_COUNT] = {};
   int ncu = 0;
-  hipEvent_t ev[6] = {};
+  hipEvent_t ev[8] = {};
   int nforward = 0;
   bool have_events = false;
 };
@@ -85,7 +88,7 @@ int get_pool(Pool** out) {
     hipDeviceProp_t prop;
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     P->ncu = prop.multiProcessorCount;
-    for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
   }
   *out = P;
   return 0;
@@ -181,10 +184,20 @@ int kernel_threads(Pool* P, int64_t nsolves) {
   return (int)n;
 }
 
-int iso_dev_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
-                 const float* vp, const float* rho, const float* depz_dev,
-                 const std::vector<float>& depz_host, const double* tRc, double* pv, double* svs,
-                 double* svp, double* srho, int32_t* status, hipStream_t st) {
+int ti_threads(Pool* P, int64_t nsolves) {
+  int64_t cap = (int64_t)P->ncu * 256;
+  int64_t n = nsolves < cap ? nsolves : cap;
+  n = (n + 127) / 128 * 128;
+  if (n < 128) n = 128;
+  return (int)n;
+}
+
+// The whole stage on device pointers: root search, optional isotropic node
+// kernels (svs/svp/srho) and optional TI splice (lsen).
+int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
+               const float* vp, const float* rho, const float* depz_dev,
+               const std::vector<float>& depz_host, const double* tRc, double* pv, double* svs,
+               double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st) {
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
   if (d->iwave != 2) return set_err(DAZI_DK_EINVAL, "iwave=%d: only Rayleigh (2) is supported", d->iwave);
@@ -264,6 +277,20 @@ int iso_dev_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* v
     HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
   }
   rec(P, 3, st);
+  if (lsen) {
+    void *til, *scr;
+    if (int rc = pool_get(P, S_TI_LAYERS, (size_t)ncol * L * sizeof(dk::TiLayerDev), &til))
+      return rc;
+    int nthr = ti_threads(P, nsolves);
+    if (int rc = pool_get(P, S_TI_SCRATCH, (size_t)dk::TI_SCRATCH * L * nthr * 8, &scr))
+      return rc;
+    HIPCHK(dk::launch_prep_ti(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                              d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
+                              (dk::TiLayerDev*)til, st));
+    HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
+                         (const int*)nsubd, lsen, (double*)scr, nthr, nullptr, st));
+  }
+  rec(P, 4, st);
   return 0;
 }
 
@@ -302,14 +329,14 @@ int32_t dazi_dk_last_timings(double* out, int32_t n) {
   Pool* P;
   if (int rc = get_pool(&P)) return -rc;
   if (!P->have_events || n < 1) return 0;
-  double v[4] = {0, 0, 0, (double)P->nforward};
-  for (int i = 0; i < 3; i++) {
+  double v[5] = {0, 0, 0, 0, (double)P->nforward};
+  for (int i = 0; i < 4; i++) {
     float ms = 0.f;
     if (hipEventSynchronize(P->ev[i + 1]) != hipSuccess) return 0;
     if (hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) != hipSuccess) return 0;
     v[i] = ms;
   }
-  int m = n < 4 ? n : 4;
+  int m = n < 5 ? n : 5;
   for (int i = 0; i < m; i++) out[i] = v[i];
   return m;
 }
@@ -325,8 +352,70 @@ int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, cons
   std::vector<float> dz(desc->nz > 0 ? desc->nz : 1);
   HIPCHK(hipMemcpyAsync(dz.data(), depz, sizeof(float) * desc->nz, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  return iso_dev_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, sen_vsRc, sen_vpRc,
-                      sen_rhoRc, status, st);
+  return stage_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc,
+                    nullptr, status, st);
+}
+
+static int dev_entry(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                     const float* vp, const float* rho, const float* depz, const double* tRc,
+                     double* pvRc, double* svs, double* svp, double* srho, float* lsen,
+                     int32_t* status, void* stream) {
+  if (!desc || !vel || !depz || !tRc || !pvRc)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  hipStream_t st = (hipStream_t)stream;
+  std::vector<float> dz(desc->nz > 0 ? desc->nz : 1);
+  HIPCHK(hipMemcpyAsync(dz.data(), depz, sizeof(float) * desc->nz, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  return stage_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, svs, svp, srho, lsen, status,
+                    st);
+}
+
+int32_t dazi_dk_ti_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                       const float* vp, const float* rho, const float* depz, const double* tRc,
+                       double* pvRc, float* Lsen_Gsc, int32_t* status, void* stream) {
+  if (!Lsen_Gsc) return set_err(DAZI_DK_EINVAL, "NULL Lsen_Gsc");
+  return dev_entry(desc, tune, vel, vp, rho, depz, tRc, pvRc, nullptr, nullptr, nullptr, Lsen_Gsc,
+                   status, stream);
+}
+
+int32_t dazi_dk_stage_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                          const float* vp, const float* rho, const float* depz, const double* tRc,
+                          double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
+                          float* Lsen_Gsc, int32_t* status, void* stream) {
+  return dev_entry(desc, tune, vel, vp, rho, depz, tRc, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc,
+                   Lsen_Gsc, status, stream);
+}
+
+int32_t dazi_dk_ti(const dazi_dk_desc* d, const float* vel, const float* depz, const double* tRc,
+                   double* pvRc, float* Lsen_Gsc) {
+  if (!d || !vel || !depz || !tRc || !pvRc || !Lsen_Gsc)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d", d->nx, d->ny, d->nz,
+                   d->kmax);
+  if (d->vp_rho_mode != 0)
+    return set_err(DAZI_DK_EINVAL, "dazi_dk_ti derives vp and rho from vs (vp_rho_mode 0)");
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  size_t ncol = (size_t)d->nx * d->ny, nz = d->nz, K = d->kmax;
+  void *dvel, *ddepz, *dt, *dpv, *dls;
+  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
+  if (int rc = pool_get(P, S_H_DEPZ, nz * 4, &ddepz)) return rc;
+  if (int rc = pool_get(P, S_H_T, K * 8, &dt)) return rc;
+  if (int rc = pool_get(P, S_H_PV, ncol * K * 8, &dpv)) return rc;
+  if (int rc = pool_get(P, S_H_LSEN, ncol * K * (nz - 1) * 4, &dls)) return rc;
+  HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
+  std::vector<float> dz(depz, depz + nz);
+  int rc = stage_impl(d, nullptr, (const float*)dvel, nullptr, nullptr, (const float*)ddepz, dz,
+                      (const double*)dt, (double*)dpv, nullptr, nullptr, nullptr, (float*)dls,
+                      nullptr, nullptr);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(nullptr));
+  HIPCHK(hipMemcpy(pvRc, dpv, ncol * K * 8, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(Lsen_Gsc, dls, ncol * K * (nz - 1) * 4, hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, const float* rho,
@@ -365,9 +454,9 @@ int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, co
   HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
   std::vector<float> dz(depz, depz + nz);
-  int rc = iso_dev_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
-                        (const float*)ddepz, dz, (const double*)dt, (double*)dpv, (double*)ds1,
-                        (double*)ds2, (double*)ds3, nullptr, st);
+  int rc = stage_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
+                      (const float*)ddepz, dz, (const double*)dt, (double*)dpv, (double*)ds1,
+                      (double*)ds2, (double*)ds3, nullptr, nullptr, st);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(st));
   HIPCHK(hipMemcpy(pvRc, dpv, ncol * K * 8, hipMemcpyDeviceToHost));

@@ -49,6 +49,25 @@ struct KernDev {
   double* drho;
 };
 
+// One refined layer of the TI model: unflattened values used by the splice
+// (d, rvp, rvs, rrho, TA, TL, TF) and flattened ones used by the eigen-solution
+// (df, A = C, L = N, F, rho) plus the sphericity factor vtp; float32 as in the
+// reference.
+struct TiLayerDev {
+  float d, rvp, rvs, rrho, TA, TL, TF;
+  float df, A, L, F, rho, vtp;
+};
+
+// Doubles of TI scratch per layer and thread.
+constexpr int TI_SCRATCH = 19;
+
+hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
+                          const float* depz, float minthk0, int L, TiLayerDev* out,
+                          hipStream_t st);
+hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
+                     const double* periods, const double* pv, const int* nsub, float* lsen,
+                     double* scratch, int nthr, int* tistat, hipStream_t st);
+
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
                              const float* depz, float minthk0, int L, float4* layers, float* btp,
                              double* cc, float* betmx, int* bad, hipStream_t st);

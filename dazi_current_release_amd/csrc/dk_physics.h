@@ -191,6 +191,74 @@ DK_HD T vwave(const T& wvno, const T& kx) {
   return dsqrt(s);
 }
 
+// The three functions of one wave type in a layer, of X = k^2 - kx^2 (signed):
+//   C = cosh(sqrt(X) d),  S = sinh(sqrt(X) d)/sqrt(X),  R = sqrt(X) sinh(sqrt(X) d),
+// entire functions of X (for X < 0 they are cos, sin/., -.sin of sqrt(-X) d).
+// For X > 0 the values are scaled by exp(-sqrt(X) d), which is returned as the
+// exponent ex (surfdisp96.f:923-962 var).  dC, dS, dR are the derivatives of the
+// entire functions with respect to X, scaled the same way: the exponent
+// scaling multiplies the whole layer matrix, so for the root's implicit
+// derivative it is a constant (its own derivative times F vanishes at F = 0).
+DK_HD double hyp3(double X, double d, double* C, double* S, double* R, double* dC, double* dS,
+                  double* dR) {
+  double ex = 0.0, scale = 1.0;
+  if (X > 0.0) {
+    double ra = sqrt(X), p = ra * d;
+    double fac = (p < 16.0) ? exp(-2.0 * p) : 0.0;
+    double sh = (1.0 - fac) * 0.5;
+    *C = (1.0 + fac) * 0.5;
+    *S = sh / ra;
+    *R = ra * sh;
+    ex = p;
+    scale = (p < 700.0) ? exp(-p) : 0.0;
+  } else if (X < 0.0) {
+    double ra = sqrt(-X), p = ra * d, sp, cp;
+    sincos(p, &sp, &cp);
+    *C = cp;
+    *S = sp / ra;
+    *R = -ra * sp;
+  } else {
+    *C = 1.0;
+    *S = d;
+    *R = 0.0;
+  }
+  *dC = 0.5 * d * (*S);
+  *dR = 0.5 * ((*S) + d * (*C));
+  double u = X * d * d;
+  if (fabs(u) < 0.1) {
+    double d3 = d * d * d;
+    *dS = scale * d3 * (1.0 / 6.0 + u * (1.0 / 60.0 + u * (1.0 / 1680.0 + u * (1.0 / 90720.0))));
+  } else {
+    *dS = (d * (*C) - (*S)) / (2.0 * X);
+  }
+  return ex;
+}
+
+template <class T>
+DK_HD T hyp3_t(const T& X, double d, T* S, T* R, double* ex);
+template <>
+DK_HD double hyp3_t<double>(const double& X, double d, double* S, double* R, double* ex) {
+  double C, dC, dS, dR;
+  *ex = hyp3(X, d, &C, S, R, &dC, &dS, &dR);
+  return C;
+}
+template <>
+DK_HD Dual<4> hyp3_t<Dual<4>>(const Dual<4>& X, double d, Dual<4>* S, Dual<4>* R, double* ex) {
+  double C, Sv, Rv, dC, dS, dR;
+  *ex = hyp3(X.v, d, &C, &Sv, &Rv, &dC, &dS, &dR);
+  Dual<4> c;
+  c.v = C;
+  S->v = Sv;
+  R->v = Rv;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    c.g[i] = X.g[i] * dC;
+    S->g[i] = X.g[i] * dS;
+    R->g[i] = X.g[i] * dR;
+  }
+  return c;
+}
+
 // One layer: Dunkin's reduced compound matrix ca (surfdisp96.f:808-831 plus
 // var and dnka), scaled by exp(-exa). a,b,rho are the flattened layer values,
 // d its flattened thickness.
@@ -199,53 +267,16 @@ DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const
                     double omega, T ca[5][5]) {
   T xka = omega / a;
   T xkb = omega / b;
-  T ra = vwave(wvno, xka);
-  T rb = vwave(wvno, xkb);
   T t = b * (1.0 / omega);
   T gammk = 2.0 * t * t;
   T gam = gammk * wvno2;
-  T p = ra * d;
-  T q = rb * d;
-  // ---- var (surfdisp96.f:868-985)
-  T cosp, sinp, w, x, cosq, sinq, y, z;
-  T pex = tconst<T>(0.0), sex = tconst<T>(0.0);
-  if (val(wvno) < val(xka)) {
-    dsincos(p, &sinp, &cosp);
-    w = sinp / ra;
-    x = -(ra * sinp);
-  } else if (val(wvno) == val(xka)) {
-    cosp = tconst<T>(1.0);
-    w = tconst<T>(d);
-    x = tconst<T>(0.0);
-  } else {
-    pex = p;
-    T fac = tconst<T>(0.0);
-    if (val(p) < 16.0) fac = dexp(-2.0 * p);
-    cosp = (1.0 + fac) * 0.5;
-    sinp = (1.0 - fac) * 0.5;
-    w = sinp / ra;
-    x = ra * sinp;
-  }
-  if (val(wvno) < val(xkb)) {
-    dsincos(q, &sinq, &cosq);
-    y = sinq / rb;
-    z = -(rb * sinq);
-  } else if (val(wvno) == val(xkb)) {
-    cosq = tconst<T>(1.0);
-    y = tconst<T>(d);
-    z = tconst<T>(0.0);
-  } else {
-    sex = q;
-    T fac = tconst<T>(0.0);
-    if (val(q) < 16.0) fac = dexp(-2.0 * q);
-    cosq = (1.0 + fac) * 0.5;
-    sinq = (1.0 - fac) * 0.5;
-    y = sinq / rb;
-    z = rb * sinq;
-  }
-  T exa = pex + sex;
-  T a0 = tconst<T>(0.0);
-  if (val(exa) < 60.0) a0 = dexp(-exa);
+  // ---- var (surfdisp96.f:868-985): cosp=C_P, w=S_P, x=R_P; cosq=C_S, y=S_S, z=R_S
+  double pex, sex;
+  T w, x, y, z;
+  T cosp = hyp3_t<T>((wvno + xka) * (wvno - xka), d, &w, &x, &pex);
+  T cosq = hyp3_t<T>((wvno + xkb) * (wvno - xkb), d, &y, &z, &sex);
+  double exa = pex + sex;
+  T a0 = tconst<T>((exa < 60.0) ? exp(-exa) : 0.0);
   T cpcq = cosp * cosq;
   T cpy = cosp * y;
   T cpz = cosp * z;

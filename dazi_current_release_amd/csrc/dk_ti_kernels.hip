@@ -1,0 +1,192 @@
+// HIP kernels of the TI (dc/dA, dc/dL -> Lsen_Gsc) part of the stage.
+//
+//   k_prep_ti   node model -> refined layers (refineLayerMdl), TI moduli
+//               TA = TC = rho Vp^2, TL = TN = rho Vs^2, TF = TA - 2 TL, their
+//               earth flattening and sphericity factors; one thread per column
+//               (depthkernelTI.f90:96-138, tregn96.f:774-841, :1301-1383).
+//   k_ti        one thread per (column, period): tregn96's eigenfunctions and
+//               kernels at the float32 phase velocity of the root search, and
+//               the float32 splice into Lsen_Gsc(post, period, interval)
+//               (depthkernelTI.f90:139-162).
+#include <hip/hip_runtime.h>
+
+#include "dk_internal.h"
+#include "dk_ti.h"
+
+namespace dk {
+
+__device__ float brocher_vp_ti(float v) {
+#pragma clang fp contract(off)
+  float v2 = v * v, v3 = v2 * v, v4 = v3 * v;
+  return 0.9409f + 2.0947f * v - 0.8206f * v2 + 0.2683f * v3 - 0.0251f * v4;
+}
+__device__ float brocher_rho_ti(float p) {
+#pragma clang fp contract(off)
+  float p2 = p * p, p3 = p2 * p, p4 = p3 * p, p5 = p4 * p;
+  return 1.6612f * p - 0.4721f * p2 + 0.0671f * p3 - 0.0043f * p4 + 0.000106f * p5;
+}
+
+__global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
+                          const float* __restrict__ vpin, const float* __restrict__ rhoin,
+                          const float* __restrict__ depz, float minthk0, int L,
+                          TiLayerDev* __restrict__ out) {
+#pragma clang fp contract(off)
+  int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  TiLayerDev* lay = out + (size_t)col * L;
+  // refined, unflattened layers (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196)
+  int k = 0;
+  float vs_i = vel[col], vp_i, rho_i;
+  if (vpin) {
+    vp_i = vpin[col];
+    rho_i = rhoin[col];
+  } else {
+    vp_i = brocher_vp_ti(vs_i);
+    rho_i = brocher_rho_ti(vp_i);
+  }
+  for (int i = 0; i < nz - 1; i++) {
+    size_t o = (size_t)(i + 1) * ncol + col;
+    float vs_n = vel[o], vp_n, rho_n;
+    if (vpin) {
+      vp_n = vpin[o];
+      rho_n = rhoin[o];
+    } else {
+      vp_n = brocher_vp_ti(vs_n);
+      rho_n = brocher_rho_ti(vp_n);
+    }
+    float thk = depz[i + 1] - depz[i];
+    float minthk = thk / minthk0;
+    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
+    float newthk = thk / (float)ns;
+    float den = (float)(2 * ns);
+    for (int j = 1; j <= ns && k < L - 1; j++, k++) {
+      float w = (float)(2 * j - 1);
+      lay[k].d = newthk;
+      lay[k].rvp = vp_i + w * (vp_n - vp_i) / den;
+      lay[k].rvs = vs_i + w * (vs_n - vs_i) / den;
+      lay[k].rrho = rho_i + w * (rho_n - rho_i) / den;
+    }
+    vs_i = vs_n;
+    vp_i = vp_n;
+    rho_i = rho_n;
+  }
+  lay[L - 1].d = 0.0f;
+  lay[L - 1].rvp = vp_i;
+  lay[L - 1].rvs = vs_i;
+  lay[L - 1].rrho = rho_i;
+  for (int m = 0; m < L; m++) {
+    TiLayerDev& t = lay[m];
+    float v2 = t.rvp * t.rvp, s2 = t.rvs * t.rvs;
+    t.TA = t.rrho * v2;
+    t.TL = t.rrho * s2;
+    float tl2 = 2.0f * t.TL;
+    t.TF = 1.0f * (t.TA - tl2);
+  }
+  // earth flattening (radius 6371) and sphericity factors (radius 6370)
+  const double ar = 6371.0;
+  const double e_r = (double)(-2.275f), e_m = (double)(-0.2750f);
+  double r0 = ar;
+  for (int i = 0; i < L; i++) {
+    TiLayerDev& t = lay[i];
+    float td = (i == L - 1) ? 1.0f : t.d;
+    double r1 = r0 - (double)td;
+    double z0 = ar * log(ar / r0);
+    double z1 = ar * log(ar / r1);
+    float tdf = (float)(z1 - z0);
+    double tmp = (ar + ar) / (r0 + r1);
+    t.rho = (float)((double)t.rrho * pow(tmp, e_r));
+    double fm = pow(tmp, e_m);
+    t.A = (float)((double)t.TA * fm);
+    t.L = (float)((double)t.TL * fm);
+    t.F = t.TF;
+    t.df = (i == L - 1) ? 0.0f : tdf;
+    r0 = r1;
+  }
+  const double ar2 = 6370.0;
+  r0 = ar2;
+  for (int i = 0; i < L; i++) {
+    TiLayerDev& t = lay[i];
+    double zd = (i == L - 1) ? 1.0 : (double)t.df;
+    double r1 = r0 * exp(-zd / ar2);
+    double tmp = (ar2 + ar2) / (r0 + r1);
+    t.vtp = (float)tmp;
+    r0 = r1;
+  }
+}
+
+__global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
+                                            const TiLayerDev* __restrict__ layers,
+                                            const double* __restrict__ periods,
+                                            const double* __restrict__ pv,
+                                            const int* __restrict__ nsub, float* __restrict__ lsen,
+                                            double* __restrict__ scratch, int* __restrict__ tistat) {
+  const long nthr = (long)gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const long nsolves = (long)ncol * K;
+  const float twopi = 2.0f * 3.141592654f;
+  for (long s = tid; s < nsolves; s += nthr) {
+    const int col = (int)(s % ncol), k = (int)(s / ncol);
+    const TiLayerDev* lay = layers + (size_t)col * L;
+    auto out = [&](int j) -> float& { return lsen[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * j)]; };
+    const double c = (double)(float)pv[s];
+    if (!(c > 0.0)) {
+      // no phase velocity for this period (root search failed): no kernel
+      for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+      continue;
+    }
+    const double omega = (double)twopi / (double)(float)periods[k];
+    int interval = 0, sub = 0;
+    float acc = 0.0f;
+    auto layf = [&](int m) {
+      const TiLayerDev& t = lay[m];
+      ti::Layer r;
+      r.d = (double)t.df;
+      r.A = (double)t.A;
+      r.C = (double)t.A;
+      r.F = (double)t.F;
+      r.L = (double)t.L;
+      r.N = (double)t.L;
+      r.rho = (double)t.rho;
+      return r;
+    };
+    auto vtpf = [&](int m) { return lay[m].vtp; };
+    auto emit = [&](int m, float dah, float dbv, float dn) {
+      if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
+      const TiLayerDev& t = lay[m];
+      acc = acc + ti::splice_term(t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn);
+      if (++sub == nsub[interval]) {
+        out(interval) = acc;
+        acc = 0.0f;
+        sub = 0;
+        interval++;
+      }
+    };
+    int rc = ti::solve(L, layf, vtpf, omega, c, scratch + tid, nthr, emit);
+    if (rc != 0) {
+      for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+      if (tistat) atomicAdd(tistat, 1);
+    }
+  }
+}
+
+hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
+                          const float* depz, float minthk0, int L, TiLayerDev* out,
+                          hipStream_t st) {
+  int block = 128;
+  hipLaunchKernelGGL(k_prep_ti, dim3((ncol + block - 1) / block), dim3(block), 0, st, ncol, nz,
+                     vel, vp, rho, depz, minthk0, L, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
+                     const double* periods, const double* pv, const int* nsub, float* lsen,
+                     double* scratch, int nthr, int* tistat, hipStream_t st) {
+  int block = 128;
+  int grid = nthr / block;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_ti, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods, pv,
+                     nsub, lsen, scratch, tistat);
+  return hipGetLastError();
+}
+
+}  // namespace dk

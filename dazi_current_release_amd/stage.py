@@ -192,3 +192,66 @@ def layered_dev(batch: LayeredBatch, kernels=False, tune=None, out=None, stream=
                                         _stream_ptr(stream))
     _lib.check(rc)
     return out
+
+
+def depthkernelTI(nx, ny, nz, vel, pvRc, iwave, igr, kmaxRc, tRc, depz, minthk, Lsen_Gsc, *,
+                  log=sys.stdout):
+    """Reference: depthkernelTI (depthkernelTI.f90:2-193).
+
+    vel (nx,ny,nz) float32; fills pvRc (nx*ny,kmaxRc) float64 and Lsen_Gsc
+    (nx*ny,kmaxRc,nz-1) float32 (zeroed first, as the reference does), in place.
+    """
+    ncol = nx * ny
+    vel = _fortran(vel, np.float32, (nx, ny, nz), "vel")
+    depz = np.ascontiguousarray(depz, np.float32)
+    tRc = np.ascontiguousarray(tRc, np.float64)
+    pvRc = _fortran(pvRc, np.float64, (ncol, kmaxRc), "pvRc")
+    Lsen_Gsc = _fortran(Lsen_Gsc, np.float32, (ncol, kmaxRc, nz - 1), "Lsen_Gsc")
+    if log is not None:
+        print(f"Depth kernel TI backend: {_BACKEND_NAME}", file=log)
+        print(f"Depth kernel TI grid/periods: nx={nx} ny={ny} nz={nz} kmax={kmaxRc}", file=log)
+    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr, 64, 0, float(minthk), 0)
+    rc = _lib.lib().dazi_dk_ti(C.byref(d), _ptr(vel), _ptr(depz), _ptr(tRc), _ptr(pvRc),
+                               _ptr(Lsen_Gsc))
+    _lib.check(rc)
+
+
+def run_depthkernelTI(vel, tRc, depz, minthk):
+    """Allocating wrapper: returns (pv (ncol,K), Lsen (ncol,K,nz-1)) Fortran-ordered."""
+    vel = np.asfortranarray(vel, np.float32)
+    nx, ny, nz = vel.shape
+    K = len(tRc)
+    pv = np.zeros((nx * ny, K), np.float64, order="F")
+    ls = np.zeros((nx * ny, K, nz - 1), np.float32, order="F")
+    depthkernelTI(nx, ny, nz, vel, pv, 2, 0, K, tRc, depz, minthk, ls, log=None)
+    return pv, ls
+
+
+def stage_dev(vel, depz, periods, minthk, vp=None, rho=None, iso=True, ti=True, tune=None,
+              out=None, stream=None):
+    """Both subroutines' outputs on device tensors from one root search.
+
+    vel/vp/rho (nz, ncol) float32; returns dict(pv (K,ncol), sen_vs/sen_vp/sen_rho
+    (nz,K,ncol) when iso, lsen (nz-1,K,ncol) float32 when ti, status (ncol,)).
+    """
+    import torch
+    nz, ncol = vel.shape
+    K = periods.numel()
+    dev = vel.device
+    if out is None:
+        out = dict(pv=torch.empty((K, ncol), dtype=torch.float64, device=dev),
+                   status=torch.empty(ncol, dtype=torch.int32, device=dev))
+        if iso:
+            for k in ("sen_vs", "sen_vp", "sen_rho"):
+                out[k] = torch.empty((nz, K, ncol), dtype=torch.float64, device=dev)
+        if ti:
+            out["lsen"] = torch.empty((nz - 1, K, ncol), dtype=torch.float32, device=dev)
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 1 if vp is not None else 0, float(minthk), 0)
+    rc = _lib.lib().dazi_dk_stage_dev(
+        C.byref(d), None if tune is None else C.byref(tune), _tptr(vel), _tptr(vp), _tptr(rho),
+        _tptr(depz), _tptr(periods), _tptr(out["pv"]),
+        _tptr(out.get("sen_vs")) if iso else None, _tptr(out.get("sen_vp")) if iso else None,
+        _tptr(out.get("sen_rho")) if iso else None, _tptr(out.get("lsen")) if ti else None,
+        _tptr(out["status"]), _stream_ptr(stream))
+    _lib.check(rc)
+    return out

@@ -119,13 +119,36 @@ int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, cons
                         double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
                         int32_t* status, void* stream);
 
+/* Replaces depthkernelTI (depthkernelTI.f90:2-34); host pointers, synchronous.
+ * vp and rho come from Brocher(vs) as in the reference (desc->vp_rho_mode 0).
+ *   pvRc(nx*ny,kmax) f64 phase velocities (the reference recomputes them too);
+ *   Lsen_Gsc(nx*ny,kmax,nz-1) f32 = sum over the sublayers of interval j of
+ *   A dc/dA + L dc/dL (tregn96 kernels at the float32 phase velocity). */
+int32_t dazi_dk_ti(const dazi_dk_desc* desc, const float* vel, const float* depz,
+                   const double* tRc, double* pvRc, float* Lsen_Gsc);
+
+/* TI stage on device pointers (vp/rho explicit when desc->vp_rho_mode = 1). */
+int32_t dazi_dk_ti_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                       const float* vp, const float* rho, const float* depz, const double* tRc,
+                       double* pvRc, float* Lsen_Gsc, int32_t* status, void* stream);
+
+/* Both subroutines' outputs from one root search (the G builder calls
+ * depthkernelTI and depthkernel back to back on the same model,
+ * CalSurfGAniso_Joint.f90:478,486).  sen_* may be NULL (all three) and
+ * Lsen_Gsc may be NULL. */
+int32_t dazi_dk_stage_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
+                          const float* vp, const float* rho, const float* depz, const double* tRc,
+                          double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
+                          float* Lsen_Gsc, int32_t* status, void* stream);
+
 /* Ragged layered batch (the batched persistent solver's native input). */
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
                             void* stream);
 
 /* Event timings (ms) of the last call's phases on its stream:
- * [0] model prep, [1] root search, [2] kernel pass, [3] number of forward
- * launches.  Waits for those events. Returns the number of values written. */
+ * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels,
+ * [4] number of root-search launches.  Waits for those events.  Returns the
+ * number of values written. */
 int32_t dazi_dk_last_timings(double* out, int32_t n);
 
 /* Enables (1) or disables (0) recording of the timings above. */
