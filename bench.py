@@ -86,6 +86,30 @@ def run_cpu_baseline(ncols: int, threads: int):
     raise RuntimeError(f"cpu baseline failed: rc={r.returncode} {r.stderr[-2000:]}")
 
 
+def bench_vjp(dk, configs, np, torch, reps=5):
+    """Differentiable path on diff_4096 (4,096 columns x 40 layers x 30 periods,
+    float64 models): forward root search vs the backward VJP kernel."""
+    W = configs.layered_workload("diff_4096")
+    b = dk.LayeredBatch(W["nl"], W["nk"], *(W[k].astype(np.float64)
+                                            for k in ("thk", "vp", "vs", "rho")), W["periods"])
+    out = dk.layered_dev(b)
+    gc = torch.ones(b.nsolves, dtype=torch.float64, device="cuda")
+    dk.layered_vjp(b, out["c_unrounded"], gc)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dk.layered_dev(b, out=out)
+    torch.cuda.synchronize()
+    t_fwd = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dk.layered_vjp(b, out["c_unrounded"], gc)
+    torch.cuda.synchronize()
+    t_vjp = (time.perf_counter() - t0) / reps
+    return dict(workload="diff_4096", columns=b.ncol, layers=40, periods=30,
+                forward_ms=1e3 * t_fwd, vjp_ms=1e3 * t_vjp, vjp_over_forward=t_vjp / t_fwd)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,6 +120,7 @@ def main():
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-vjp", action="store_true")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--cpw", type=int, default=16)
@@ -196,6 +221,10 @@ def main():
         plain = dict(solves_per_s=nsolves / plain_s, ms_per_step=1e3 * plain_s,
                      launches_per_step=int(np.ceil(ncol / 64)))
 
+    vjp = None
+    if not args.no_vjp:
+        vjp = bench_vjp(dk, configs, np, torch)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -223,7 +252,8 @@ def main():
                     failed_columns=int((status != 0).sum()),
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
-                    batched_over_plain=(value / plain["solves_per_s"]) if plain else None),
+                    batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
+                    vjp=vjp),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 

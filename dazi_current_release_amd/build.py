@@ -43,6 +43,31 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FDRIVER = os.path.join(FORTRAN, "dk_fdriver")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compiles the Fortran binding (module dazi_dk) and its example host
+    dk_fdriver against libdazi_dk.so (rpath $ORIGIN/..)."""
+    tmp = os.path.join(FORTRAN, "_fbuild")
+    os.makedirs(tmp, exist_ok=True)
+    cmds = [
+        [FLANG, "-O2", "-module-dir", tmp, "-c", os.path.join(FORTRAN, "dazi_dk.f90"),
+         "-o", os.path.join(tmp, "dazi_dk.o")],
+        [FLANG, "-O2", "-module-dir", tmp, "-I", tmp, os.path.join(FORTRAN, "dk_fdriver.f90"),
+         os.path.join(tmp, "dazi_dk.o"), "-L", HERE, "-ldazi_dk", "-Wl,-rpath,$ORIGIN/..",
+         "-o", FDRIVER + ".tmp"],
+    ]
+    for cmd in cmds:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+    os.replace(FDRIVER + ".tmp", FDRIVER)
+    return FDRIVER
+
+
 def ensure_built() -> str:
     if not os.path.exists(LIB):
         build()
@@ -51,3 +76,4 @@ def ensure_built() -> str:
 
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

@@ -220,8 +220,8 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   int ncol = d->nx * d->ny;
   int64_t nsolves = (int64_t)ncol * d->kmax;
   void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work, *nsubd;
-  if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 16, &layers)) return rc;
-  if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 4, &btp)) return rc;
+  if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 32, &layers)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 8, &btp)) return rc;
   if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)ncol * 4, &bad)) return rc;
@@ -240,7 +240,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
                                d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                               (float4*)layers, (float*)btp, (double*)cc, (float*)betmx,
+                               (double4*)layers, (double*)btp, (double*)cc, (float*)betmx,
                                (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
@@ -248,8 +248,8 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   B.uniform = 1;
   B.Lu = L;
   B.Ku = d->kmax;
-  B.layers = (const float4*)layers;
-  B.btp = (const float*)btp;
+  B.layers = (const double4*)layers;
+  B.btp = (const double*)btp;
   B.periods = tRc;
   B.cc = (const double*)cc;
   B.betmx = (const float*)betmx;
@@ -469,8 +469,10 @@ int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, co
 }
 
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, void* stream) {
-  if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !b->thk || !b->vp || !b->vs || !b->rho ||
-      !b->periods || !b->c || !b->status)
+  bool m32 = b && b->thk && b->vp && b->vs && b->rho;
+  bool m64 = b && b->thk64 && b->vp64 && b->vs64 && b->rho64;
+  if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !(m32 || m64) || !b->periods || !b->c ||
+      !b->status)
     return set_err(DAZI_DK_EINVAL, "NULL required argument in dazi_dk_layered");
   if (b->ncol < 0 || b->max_layers < 1 || b->nsolves < 0)
     return set_err(DAZI_DK_EINVAL, "bad sizes ncol=%d max_layers=%d", b->ncol, b->max_layers);
@@ -491,8 +493,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   HIPCHK(hipStreamSynchronize(st));
   nlay_total = lo_last + nl_last;
   void *layers, *btp, *cc, *betmx, *bad, *croot, *work;
-  if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 16, &layers)) return rc;
-  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 4, &btp)) return rc;
+  if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;
   if (int rc = pool_get(P, S_CC, (size_t)b->ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)b->ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)b->ncol * 4, &bad)) return rc;
@@ -506,8 +508,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   P->have_events = g_timing;
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
-                                (float4*)layers, (float*)btp, (double*)cc, (float*)betmx,
-                                (int*)bad, st));
+                                b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
+                                (double*)btp, (double*)cc, (float*)betmx, (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;
@@ -516,8 +518,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   B.loff = b->loff;
   B.K = b->nk;
   B.toff = b->toff;
-  B.layers = (const float4*)layers;
-  B.btp = (const float*)btp;
+  B.layers = (const double4*)layers;
+  B.btp = (const double*)btp;
   B.periods = b->periods;
   B.cc = (const double*)cc;
   B.betmx = (const float*)betmx;
@@ -545,6 +547,60 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
   }
   rec(P, 3, st);
+  return 0;
+}
+
+int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, double* grad_vs,
+                                double* grad_vp, double* grad_rho, void* stream) {
+  bool m32 = b && b->thk && b->vp && b->vs && b->rho;
+  bool m64 = b && b->thk64 && b->vp64 && b->vs64 && b->rho64;
+  if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !(m32 || m64) || !b->periods ||
+      !b->c_unrounded || !grad_c || !grad_vs || !grad_vp || !grad_rho)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument (the VJP needs c_unrounded)");
+  if (b->ncol < 0 || b->max_layers < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes ncol=%d max_layers=%d", b->ncol, b->max_layers);
+  if (b->ncol == 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  int32_t nl_last = 0;
+  int64_t lo_last = 0;
+  HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  int64_t nlay_total = lo_last + nl_last;
+  void *layers, *btp, *cc, *betmx, *bad, *tape;
+  if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;
+  if (int rc = pool_get(P, S_CC, (size_t)b->ncol * 8, &cc)) return rc;
+  if (int rc = pool_get(P, S_BETMX, (size_t)b->ncol * 4, &betmx)) return rc;
+  if (int rc = pool_get(P, S_BAD, (size_t)b->ncol * 4, &bad)) return rc;
+  int nblocks = P->ncu * 8;
+  if (nblocks > b->ncol) nblocks = b->ncol;
+  int L = b->max_layers;
+  if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L > 1 ? L - 1 : 1) * nblocks * 64 * 8, &tape))
+    return rc;
+  P->have_events = g_timing;
+  rec(P, 0, st);
+  HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
+                                b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
+                                (double*)btp, (double*)cc, (float*)betmx, (int*)bad, st));
+  rec(P, 1, st);
+  dk::BatchDev B = {};
+  B.ncol = b->ncol;
+  B.uniform = 0;
+  B.L = b->nl;
+  B.loff = b->loff;
+  B.K = b->nk;
+  B.toff = b->toff;
+  B.layers = (const double4*)layers;
+  B.btp = (const double*)btp;
+  B.periods = b->periods;
+  B.croot = b->c_unrounded;
+  HIPCHK(dk::launch_vjp(B, grad_c, grad_vs, grad_vp, grad_rho, (double*)tape, nblocks, st));
+  rec(P, 2, st);
+  rec(P, 3, st);
+  rec(P, 4, st);
   return 0;
 }
 

@@ -21,8 +21,8 @@ struct BatchDev {
   const int64_t* loff;
   const int* K;
   const int64_t* toff;
-  const float4* layers;  // (d, a, b, rho) flattened
-  const float* btp;      // flattening velocity factor per layer
+  const double4* layers;  // (d, a, b, rho) flattened
+  const double* btp;      // flattening velocity factor per layer
   const double* periods;
   const double* cc;      // start velocity (= cm) per column
   const float* betmx;    // max flattened shear velocity per column
@@ -69,16 +69,20 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      double* scratch, int nthr, int* tistat, hipStream_t st);
 
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, float4* layers, float* btp,
+                             const float* depz, float minthk0, int L, double4* layers, double* btp,
                              double* cc, float* betmx, int* bad, hipStream_t st);
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
-                              const float* vp, const float* vs, const float* rho, float4* layers,
-                              float* btp, double* cc, float* betmx, int* bad, hipStream_t st);
+                              const float* vp, const float* vs, const float* rho,
+                              const double* thk64, const double* vp64, const double* vs64,
+                              const double* rho64, double4* layers, double* btp, double* cc,
+                              float* betmx, int* bad, hipStream_t st);
 hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, int cols_per_wg,
                           int persistent, int grid, int lds_stride, hipStream_t st);
 int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
 hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
                           hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
+hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
+                      double* grho, double* tape, int nblocks, hipStream_t st);
 
 }  // namespace dk

@@ -71,30 +71,15 @@ __device__ float gtsolh_f32(float a, float b) {
 
 // Earth flattening of one column in place (sphere(0,0) then sphere(2,1) for
 // Rayleigh waves), followed by the start-velocity rule of surfdisp96.
-// lay[m] = (thk, vp, vs, rho) on input, flattened (d, a, b, rho) on output.
-__device__ void flatten_column(float4* lay, float* btp_out, int L, double* cc_out,
-                               float* betmx_out, int* bad) {
+// lay[m] = (thk, vp, vs, rho) on input (float32 values), flattened (d, a, b, rho)
+// on output, rounded to float32 as the reference stores them.
+__device__ void start_velocity(const double4* lay, int L, double* cc_out, float* betmx_out,
+                               int* bad) {
 #pragma clang fp contract(off)
-  const double ar = 6370.0;
-  double dr = 0.0, r0 = ar;
   float betmx = -1.e20f, betmn = 1.e20f;
   int jmn = 0, jsol = 1;
   for (int m = 0; m < L; m++) {
-    float4 l = lay[m];
-    float thk = (m == L - 1) ? 1.0f : l.x;
-    dr = dr + (double)thk;
-    double r1 = ar - dr;
-    double z0 = ar * log(ar / r0);
-    double z1 = ar * log(ar / r1);
-    float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
-    double tmp = (ar + ar) / (r0 + r1);
-    float a = (float)((double)l.y * tmp);
-    float b = (float)((double)l.z * tmp);
-    float bt = (float)tmp;
-    float rho = l.w * powf(bt, -2.275f);
-    r0 = r1;
-    lay[m] = make_float4(d, a, b, rho);
-    if (btp_out) btp_out[m] = bt;
+    float a = (float)lay[m].y, b = (float)lay[m].z;
     if (b > 0.01f && b < betmn) {
       betmn = b;
       jmn = m;
@@ -108,14 +93,14 @@ __device__ void flatten_column(float4* lay, float* btp_out, int L, double* cc_ou
   }
   // a fluid layer (b <= 0.01) is outside this stage's model class, and so is
   // any non-finite or non-positive layer value
-  if (jsol == 0 || lay[0].z <= 0.0f) *bad = 1;
+  if (jsol == 0 || lay[0].z <= 0.0) *bad = 1;
   for (int m = 0; m < L; m++) {
-    float4 l = lay[m];
-    bool ok = isfinite(l.x) && isfinite(l.y) && isfinite(l.z) && isfinite(l.w) && l.y > 0.0f &&
-              l.z > 0.0f && l.w > 0.0f && (l.x > 0.0f || m == L - 1);
+    double4 l = lay[m];
+    bool ok = isfinite(l.x) && isfinite(l.y) && isfinite(l.z) && isfinite(l.w) && l.y > 0.0 &&
+              l.z > 0.0 && l.w > 0.0 && (l.x > 0.0 || m == L - 1);
     if (!ok) *bad = 1;
   }
-  float cc1 = gtsolh_f32(lay[jmn].y, lay[jmn].z);
+  float cc1 = gtsolh_f32((float)lay[jmn].y, (float)lay[jmn].z);
   cc1 = 0.95f * cc1;
   cc1 = 0.90f * cc1;
   if (!(cc1 > 0.0f) || !isfinite(cc1)) *bad = 1;
@@ -123,18 +108,63 @@ __device__ void flatten_column(float4* lay, float* btp_out, int L, double* cc_ou
   *betmx_out = betmx;
 }
 
+__device__ void flatten_column(double4* lay, double* btp_out, int L, double* cc_out,
+                               float* betmx_out, int* bad) {
+#pragma clang fp contract(off)
+  const double ar = 6370.0;
+  double dr = 0.0, r0 = ar;
+  for (int m = 0; m < L; m++) {
+    double4 l = lay[m];
+    float thk = (m == L - 1) ? 1.0f : (float)l.x;
+    dr = dr + (double)thk;
+    double r1 = ar - dr;
+    double z0 = ar * log(ar / r0);
+    double z1 = ar * log(ar / r1);
+    float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
+    double tmp = (ar + ar) / (r0 + r1);
+    float a = (float)((double)(float)l.y * tmp);
+    float b = (float)((double)(float)l.z * tmp);
+    float bt = (float)tmp;
+    float rho = (float)l.w * powf(bt, -2.275f);
+    r0 = r1;
+    lay[m] = make_double4(d, a, b, rho);
+    if (btp_out) btp_out[m] = bt;
+  }
+  start_velocity(lay, L, cc_out, betmx_out, bad);
+}
+
+// The same flattening carried out in float64 for float64 models (the
+// differentiable path); the start velocity follows the float32 rule.
+__device__ void flatten_column_f64(double4* lay, double* btp_out, int L, double* cc_out,
+                                   float* betmx_out, int* bad) {
+  const double ar = 6370.0;
+  double dr = 0.0, r0 = ar;
+  for (int m = 0; m < L; m++) {
+    double4 l = lay[m];
+    double thk = (m == L - 1) ? 1.0 : l.x;
+    dr = dr + thk;
+    double r1 = ar - dr;
+    double d = (m == L - 1) ? 0.0 : ar * log(ar / r1) - ar * log(ar / r0);
+    double tmp = (ar + ar) / (r0 + r1);
+    r0 = r1;
+    lay[m] = make_double4(d, l.y * tmp, l.z * tmp, l.w * pow(tmp, -2.275));
+    if (btp_out) btp_out[m] = tmp;
+  }
+  start_velocity(lay, L, cc_out, betmx_out, bad);
+}
+
 // Node models: vel is (ncol, nz) column-fastest (Fortran vel(nx,ny,nz)).
 // vp/rho optional (both or neither); otherwise Brocher from vs.
 __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
                              const float* __restrict__ vpin, const float* __restrict__ rhoin,
                              const float* __restrict__ depz, float minthk0, int L,
-                             float4* __restrict__ layers, float* __restrict__ btp,
+                             double4* __restrict__ layers, double* __restrict__ btp,
                              double* __restrict__ cc, float* __restrict__ betmx,
                              int* __restrict__ bad_out) {
 #pragma clang fp contract(off)
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
-  float4* lay = layers + (size_t)col * L;
+  double4* lay = layers + (size_t)col * L;
   int k = 0;
   float vs_i = vel[col], vp_i, rho_i;
   if (vpin) {
@@ -161,34 +191,45 @@ __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
     float den = (float)(2 * ns);
     for (int j = 1; j <= ns && k < L - 1; j++, k++) {
       float w = (float)(2 * j - 1);
-      lay[k] = make_float4(newthk, vp_i + w * (vp_n - vp_i) / den, vs_i + w * (vs_n - vs_i) / den,
-                           rho_i + w * (rho_n - rho_i) / den);
+      float rvp = vp_i + w * (vp_n - vp_i) / den;
+      float rvs = vs_i + w * (vs_n - vs_i) / den;
+      float rrho = rho_i + w * (rho_n - rho_i) / den;
+      lay[k] = make_double4(newthk, rvp, rvs, rrho);
     }
     vs_i = vs_n;
     vp_i = vp_n;
     rho_i = rho_n;
   }
-  lay[L - 1] = make_float4(0.0f, vp_i, vs_i, rho_i);
+  lay[L - 1] = make_double4(0.0, vp_i, vs_i, rho_i);
   int bad = (k != L - 1);
   flatten_column(lay, btp + (size_t)col * L, L, cc + col, betmx + col, &bad);
   bad_out[col] = bad;
 }
 
-// Layered models (ragged): thk/vp/vs/rho concatenated at loff[col].
+// Layered models (ragged): thk/vp/vs/rho concatenated at loff[col], float32
+// (reference semantics) or, when m64 is set, float64 arrays flattened in float64.
 __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_t* __restrict__ loff,
                               const float* __restrict__ thk, const float* __restrict__ vp,
                               const float* __restrict__ vs, const float* __restrict__ rho,
-                              float4* __restrict__ layers, float* __restrict__ btp,
+                              const double* __restrict__ thk64, const double* __restrict__ vp64,
+                              const double* __restrict__ vs64, const double* __restrict__ rho64,
+                              double4* __restrict__ layers, double* __restrict__ btp,
                               double* __restrict__ cc, float* __restrict__ betmx,
                               int* __restrict__ bad_out) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   int L = Lc[col];
   int64_t o = loff[col];
-  float4* lay = layers + o;
-  for (int m = 0; m < L; m++) lay[m] = make_float4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]);
+  double4* lay = layers + o;
   int bad = (L < 1);
-  if (L >= 1) flatten_column(lay, btp + o, L, cc + col, betmx + col, &bad);
+  if (thk64) {
+    for (int m = 0; m < L; m++)
+      lay[m] = make_double4(thk64[o + m], vp64[o + m], vs64[o + m], rho64[o + m]);
+    if (L >= 1) flatten_column_f64(lay, btp + o, L, cc + col, betmx + col, &bad);
+  } else {
+    for (int m = 0; m < L; m++) lay[m] = make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]);
+    if (L >= 1) flatten_column(lay, btp + o, L, cc + col, betmx + col, &bad);
+  }
   bad_out[col] = bad;
 }
 
@@ -245,13 +286,13 @@ __device__ __forceinline__ double fv_ratio_root(double lo, double hi, FV flo, FV
 template <int S>
 __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int col_end,
                                                  int persistent, int lds_stride) {
-  extern __shared__ float4 lds[];
+  extern __shared__ double4 lds[];
   const int lane = threadIdx.x & 63;
   const int r = lane & (S - 1);
   const int gbase = lane & ~(S - 1);
   const int gpb = blockDim.x / S;  // groups per block
   const int gid_blk = threadIdx.x / S;
-  float4* mdl = lds + (size_t)gid_blk * lds_stride;
+  double4* mdl = lds + (size_t)gid_blk * lds_stride;
   const double dc = (double)0.005f;
   const double onea = 1.5;
   const double twopi = 2.0 * 3.141592653589793;
@@ -544,8 +585,8 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     }
     return;
   }
-  const float4* lay = B.layers + loff;
-  const float* btp = B.btp + loff;
+  const double4* lay = B.layers + loff;
+  const double* btp = B.btp + loff;
   double omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
   if (omega < 1.0e-4) omega = 1.0e-4;
   const double wv = omega / c;
@@ -554,7 +595,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   // ---- sweep 1: bottom-up values; tape holds the vector entering layer m
   double e[5];
   {
-    float4 h = lay[L - 1];
+    double4 h = lay[L - 1];
     halfspace_e<double>(wv, wv2, (double)h.y, (double)h.z, (double)h.w, omega, e);
   }
   int Ef = pow2_normalize(e);
@@ -563,7 +604,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
 #pragma unroll
     for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i];
     tp[(int64_t)5 * nthr] = (double)Ef;
-    float4 l = lay[m];
+    double4 l = lay[m];
     double ca[5][5];
     layer_ca<double>(wv, wv2, (double)l.y, (double)l.z, (double)l.w, (double)l.x, omega, ca);
     double ee[5];
@@ -592,7 +633,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   int node = 0, sub = 0;
   double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
   for (int m = 0; m <= L - 1; m++) {
-    float4 l = lay[m];
+    double4 l = lay[m];
     double part[4];
     if (m < L - 1) {
       Dual<4> ca[5][5];
@@ -711,6 +752,186 @@ __global__ void __launch_bounds__(256) k_kernels(BatchDev B, KernDev O, double* 
   }
 }
 
+// ------------------------------------------------------------------ VJP kernel
+// grad_m = sum_k grad_c[s(col,k)] dc_k/dm for the unflattened layer vs, vp, rho
+// of a ragged batch (outputs laid out like the model, at loff[col] + m).
+// One wavefront per column, lane k owning period k (columns with more than 64
+// periods take several chunks).  Sweep 1 carries the tangent dF/dc with the
+// tape; sweep 2 is the adjoint with (alpha, beta, rho) duals, already scaled by
+// -grad_c/(dF/dc); per-layer sums reduce across lanes with shuffles, so the
+// result does not depend on scheduling.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+__global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict__ grad_c,
+                                            double* __restrict__ gvs, double* __restrict__ gvp,
+                                            double* __restrict__ grho, double* __restrict__ tape) {
+  const int lane = threadIdx.x;
+  const long nslot = (long)gridDim.x * 64;
+  const long slot = (long)blockIdx.x * 64 + lane;
+  for (int col = blockIdx.x; col < B.ncol; col += gridDim.x) {
+    int L, K;
+    int64_t loff, toff;
+    col_geom(B, col, &L, &loff, &K, &toff);
+    const double4* lay = B.layers + loff;
+    const double* btp = B.btp + loff;
+    // a column without periods still writes its (zero) gradients
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += 64) {
+      const int k = k0 + lane;
+      double c = 0.0, gc = 0.0;
+      if (k < K) {
+        int64_t s = solve_index(B, col, toff, k);
+        c = B.croot[s];
+        gc = grad_c[s];
+      }
+      const bool act = (c > 0.0) && (gc != 0.0) && isfinite(gc);
+      double omega = 1.0, wv = 1.0, wv2 = 1.0;
+      if (act) {
+        omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
+        if (omega < 1.0e-4) omega = 1.0e-4;
+        wv = omega / c;
+        wv2 = wv * wv;
+      }
+      // ---- sweep 1: values (taped) and the tangent in c
+      double lane_scale = 0.0;
+      int EF = 0;
+      if (act) {
+        Dual<1> wvd = dconst<1>(wv);
+        wvd.g[0] = -wv / c;
+        Dual<1> wv2d = wvd * wvd;
+        Dual<1> e[5];
+        double4 h = lay[L - 1];
+        halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
+                             dconst<1>((double)h.w), omega, e);
+        int Ef = 0;
+        for (int m = L - 1; m >= 0; m--) {
+          if (m < L - 1) {
+            double4 l = lay[m];
+            Dual<1> ca[5][5];
+            layer_ca<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
+                              dconst<1>((double)l.w), (double)l.x, omega, ca);
+            Dual<1> ee[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+              double v = 0.0, g = 0.0;
+#pragma unroll
+              for (int j = 0; j < 5; j++) {
+                v = fma(e[j].v, ca[j][i].v, v);
+                g = fma(e[j].g[0], ca[j][i].v, fma(e[j].v, ca[j][i].g[0], g));
+              }
+              ee[i].v = v;
+              ee[i].g[0] = g;
+            }
+#pragma unroll
+            for (int i = 0; i < 5; i++) e[i] = ee[i];
+          }
+          // normalise values and tangent together (exact power of two)
+          double mx = 0.0;
+#pragma unroll
+          for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(e[i].v));
+          int kk = 0;
+          if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
+#pragma unroll
+          for (int i = 0; i < 5; i++) {
+            e[i].v = ldexp(e[i].v, -kk);
+            e[i].g[0] = ldexp(e[i].g[0], -kk);
+          }
+          Ef += kk;
+          if (m > 0) {
+            double* tp = tape + (long)((m - 1) * 6) * nslot + slot;
+#pragma unroll
+            for (int i = 0; i < 5; i++) tp[(long)i * nslot] = e[i].v;
+            tp[(long)5 * nslot] = (double)Ef;
+          }
+        }
+        EF = Ef;
+        const double dFdc = e[0].g[0];
+        lane_scale = (dFdc != 0.0) ? -gc / dFdc : 0.0;
+      }
+      // ---- sweep 2: adjoint, per-layer contributions reduced over the lanes
+      double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
+      int Eg = 0;
+      for (int m = 0; m < L; m++) {
+        double part[3] = {0.0, 0.0, 0.0};
+        if (act) {
+          double4 l = lay[m];
+          if (m < L - 1) {
+            Dual<3> ca[5][5];
+            layer_ca<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
+                              dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), (double)l.x,
+                              omega, ca);
+            const double* tp = tape + (long)(m * 6) * nslot + slot;
+            double et[5];
+#pragma unroll
+            for (int i = 0; i < 5; i++) et[i] = tp[(long)i * nslot];
+            const int Ee = (int)tp[(long)5 * nslot];
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              double acc = 0.0;
+#pragma unroll
+              for (int j = 0; j < 5; j++) {
+                double t = 0.0;
+#pragma unroll
+                for (int i = 0; i < 5; i++) t = fma(ca[j][i].g[p], g[i], t);
+                acc = fma(et[j], t, acc);
+              }
+              part[p] = ldexp(acc, Ee + Eg - EF);
+            }
+            double gn[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) {
+              double t = 0.0;
+#pragma unroll
+              for (int i = 0; i < 5; i++) t = fma(ca[j][i].v, g[i], t);
+              gn[j] = t;
+            }
+#pragma unroll
+            for (int j = 0; j < 5; j++) g[j] = gn[j];
+            Eg += pow2_normalize(g);
+          } else {
+            Dual<3> eh[5];
+            halfspace_e<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
+                                 dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), omega, eh);
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              double acc = 0.0;
+#pragma unroll
+              for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
+              part[p] = ldexp(acc, Eg - EF);
+            }
+          }
+          const double bt = (double)btp[m];
+          part[0] *= lane_scale * bt;
+          part[1] *= lane_scale * bt;
+          part[2] *= lane_scale * pow(bt, -2.275);
+        }
+        const double sa = wave_sum(part[0]), sb = wave_sum(part[1]), sr = wave_sum(part[2]);
+        if (lane == 0) {
+          const int64_t o = loff + m;
+          if (k0 == 0) {
+            gvp[o] = sa;
+            gvs[o] = sb;
+            grho[o] = sr;
+          } else {
+            gvp[o] += sa;
+            gvs[o] += sb;
+            grho[o] += sr;
+          }
+        }
+      }
+    }
+  }
+}
+
+hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
+                      double* grho, double* tape, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_vjp, dim3(nblocks), dim3(64), 0, st, B, grad_c, gvs, gvp, grho, tape);
+  return hipGetLastError();
+}
+
 __global__ void k_zero_int(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -721,7 +942,7 @@ template <int S>
 static hipError_t launch_forward_S(const BatchDev& B, int col_begin, int col_end, int cols_per_wg,
                                    int persistent, int grid, int lds_stride, hipStream_t st) {
   int block = S * cols_per_wg;
-  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(float4);
+  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(double4);
   if (!persistent) grid = (col_end - col_begin + cols_per_wg - 1) / cols_per_wg;
   if (grid <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_forward<S>, dim3(grid), dim3(block), lds, st, B, col_begin, col_end,
@@ -747,7 +968,7 @@ hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, 
 
 int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
   int block = S * cols_per_wg;
-  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(float4);
+  size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(double4);
   int nb = 0;
   hipError_t e = hipErrorInvalidValue;
   switch (S) {
@@ -761,7 +982,7 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
 }
 
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, float4* layers, float* btp,
+                             const float* depz, float minthk0, int L, double4* layers, double* btp,
                              double* cc, float* betmx, int* bad, hipStream_t st) {
   int block = 128;
   int grid = (ncol + block - 1) / block;
@@ -771,12 +992,14 @@ hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp
 }
 
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
-                              const float* vp, const float* vs, const float* rho, float4* layers,
-                              float* btp, double* cc, float* betmx, int* bad, hipStream_t st) {
+                              const float* vp, const float* vs, const float* rho,
+                              const double* thk64, const double* vp64, const double* vs64,
+                              const double* rho64, double4* layers, double* btp, double* cc,
+                              float* betmx, int* bad, hipStream_t st) {
   int block = 128;
   int grid = (ncol + block - 1) / block;
   hipLaunchKernelGGL(k_prep_layers, dim3(grid), dim3(block), 0, st, ncol, Lc, loff, thk, vp, vs,
-                     rho, layers, btp, cc, betmx, bad);
+                     rho, thk64, vp64, vs64, rho64, layers, btp, cc, betmx, bad);
   return hipGetLastError();
 }
 

@@ -171,14 +171,18 @@ DK_HD double dexp(double a) { return exp(a); }
 DK_HD void dsincos(double a, double* s, double* c) { sincos(a, s, c); }
 DK_HD double val(double a) { return a; }
 template <class T>
-DK_HD T tconst(double v);
+struct Num;
 template <>
-DK_HD double tconst<double>(double v) {
-  return v;
-}
-template <>
-DK_HD Dual<4> tconst<Dual<4>>(double v) {
-  return dconst<4>(v);
+struct Num<double> {
+  static DK_HD double make(double v) { return v; }
+};
+template <int N>
+struct Num<Dual<N>> {
+  static DK_HD Dual<N> make(double v) { return dconst<N>(v); }
+};
+template <class T>
+DK_HD T tconst(double v) {
+  return Num<T>::make(v);
 }
 
 // ----------------------------------------------------------------- layer matrix
@@ -234,24 +238,21 @@ DK_HD double hyp3(double X, double d, double* C, double* S, double* R, double* d
   return ex;
 }
 
-template <class T>
-DK_HD T hyp3_t(const T& X, double d, T* S, T* R, double* ex);
-template <>
-DK_HD double hyp3_t<double>(const double& X, double d, double* S, double* R, double* ex) {
+DK_HD double hyp3_t(const double& X, double d, double* S, double* R, double* ex) {
   double C, dC, dS, dR;
   *ex = hyp3(X, d, &C, S, R, &dC, &dS, &dR);
   return C;
 }
-template <>
-DK_HD Dual<4> hyp3_t<Dual<4>>(const Dual<4>& X, double d, Dual<4>* S, Dual<4>* R, double* ex) {
+template <int N>
+DK_HD Dual<N> hyp3_t(const Dual<N>& X, double d, Dual<N>* S, Dual<N>* R, double* ex) {
   double C, Sv, Rv, dC, dS, dR;
   *ex = hyp3(X.v, d, &C, &Sv, &Rv, &dC, &dS, &dR);
-  Dual<4> c;
+  Dual<N> c;
   c.v = C;
   S->v = Sv;
   R->v = Rv;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
+  for (int i = 0; i < N; i++) {
     c.g[i] = X.g[i] * dC;
     S->g[i] = X.g[i] * dS;
     R->g[i] = X.g[i] * dR;
@@ -273,8 +274,8 @@ DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const
   // ---- var (surfdisp96.f:868-985): cosp=C_P, w=S_P, x=R_P; cosq=C_S, y=S_S, z=R_S
   double pex, sex;
   T w, x, y, z;
-  T cosp = hyp3_t<T>((wvno + xka) * (wvno - xka), d, &w, &x, &pex);
-  T cosq = hyp3_t<T>((wvno + xkb) * (wvno - xkb), d, &y, &z, &sex);
+  T cosp = hyp3_t((wvno + xka) * (wvno - xka), d, &w, &x, &pex);
+  T cosq = hyp3_t((wvno + xkb) * (wvno - xkb), d, &y, &z, &sex);
   double exa = pex + sex;
   T a0 = tconst<T>((exa < 60.0) ? exp(-exa) : 0.0);
   T cpcq = cosp * cosq;
@@ -358,7 +359,7 @@ DK_HD int pow2_normalize(double e[5]) {
   return k;
 }
 
-// Layers are given as float4 (d, a, b, rho) of the flattened model; index
+// Layers are given as double4 (d, a, b, rho) of the flattened model; index
 // L-1 is the half-space. Returns the mantissa; *ex receives the exponent.
 template <class LayerFn>
 DK_HD double secular(double c, double omega, int L, LayerFn lay, int* ex) {
@@ -366,11 +367,11 @@ DK_HD double secular(double c, double omega, int L, LayerFn lay, int* ex) {
   double wvno = omega / c;
   double wvno2 = wvno * wvno;
   double e[5];
-  float4 h = lay(L - 1);
+  double4 h = lay(L - 1);
   halfspace_e<double>(wvno, wvno2, (double)h.y, (double)h.z, (double)h.w, omega, e);
   int E = pow2_normalize(e);
   for (int m = L - 2; m >= 0; m--) {
-    float4 l = lay(m);
+    double4 l = lay(m);
     double ca[5][5];
     layer_ca<double>(wvno, wvno2, (double)l.y, (double)l.z, (double)l.w, (double)l.x, omega, ca);
     double ee[5];

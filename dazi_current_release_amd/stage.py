@@ -128,6 +128,8 @@ class LayeredBatch:
     """Ragged layered batch resident on the GPU (the batched solver's input)."""
 
     def __init__(self, nl, nk, thk, vp, vs, rho, periods, device="cuda"):
+        """Model arrays may be float32 (the reference's precision) or float64
+        (flattened in float64: the differentiable path)."""
         import torch
         nl = np.asarray(nl, np.int32)
         nk = np.asarray(nk, np.int32)
@@ -143,10 +145,12 @@ class LayeredBatch:
         self.loff = T(loff, torch.int64)
         self.toff = T(toff, torch.int64)
         self.solve_col = T(solve_col, torch.int32)
-        self.thk = T(thk, torch.float32)
-        self.vp = T(vp, torch.float32)
-        self.vs = T(vs, torch.float32)
-        self.rho = T(rho, torch.float32)
+        self.f64 = np.asarray(vs).dtype == np.float64
+        mt = torch.float64 if self.f64 else torch.float32
+        self.thk = T(thk, mt)
+        self.vp = T(vp, mt)
+        self.vs = T(vs, mt)
+        self.rho = T(rho, mt)
         self.periods = T(periods, torch.float64)
         self.nl_host = nl
         self.nk_host = nk
@@ -177,21 +181,60 @@ def layered_dev(batch: LayeredBatch, kernels=False, tune=None, out=None, stream=
     """
     if out is None:
         out = batch.alloc_outputs(kernels)
-    b = _lib.Layered(batch.ncol, batch.max_layers, batch.nsolves, batch.nl.data_ptr(),
-                     batch.loff.data_ptr(), batch.nk.data_ptr(), batch.toff.data_ptr(),
-                     batch.solve_col.data_ptr(), batch.thk.data_ptr(),
-                     (vp if vp is not None else batch.vp).data_ptr(),
-                     (vs if vs is not None else batch.vs).data_ptr(),
-                     (rho if rho is not None else batch.rho).data_ptr(),
-                     batch.periods.data_ptr(), out["c"].data_ptr(), out["status"].data_ptr(),
-                     out["dc_dvs"].data_ptr() if kernels else None,
-                     out["dc_dvp"].data_ptr() if kernels else None,
-                     out["dc_drho"].data_ptr() if kernels else None,
-                     out["c_unrounded"].data_ptr() if "c_unrounded" in out else None)
+    b = _layered_struct(batch, vs, vp, rho)
+    b.c = out["c"].data_ptr()
+    b.status = out["status"].data_ptr()
+    if kernels:
+        b.dc_dvs = out["dc_dvs"].data_ptr()
+        b.dc_dvp = out["dc_dvp"].data_ptr()
+        b.dc_drho = out["dc_drho"].data_ptr()
+    if "c_unrounded" in out:
+        b.c_unrounded = out["c_unrounded"].data_ptr()
     rc = _lib.lib().dazi_dk_layered_dev(C.byref(b), None if tune is None else C.byref(tune),
                                         _stream_ptr(stream))
     _lib.check(rc)
     return out
+
+
+def _layered_struct(batch, vs=None, vp=None, rho=None):
+    vs = batch.vs if vs is None else vs
+    vp = batch.vp if vp is None else vp
+    rho = batch.rho if rho is None else rho
+    for t in (vs, vp, rho):
+        if t.dtype != batch.thk.dtype or not t.is_contiguous() or t.numel() != batch.thk.numel():
+            raise ValueError("model tensors must match the batch's dtype, size and layout")
+    b = _lib.Layered()
+    b.ncol, b.max_layers, b.nsolves = batch.ncol, batch.max_layers, batch.nsolves
+    b.nl, b.loff = batch.nl.data_ptr(), batch.loff.data_ptr()
+    b.nk, b.toff = batch.nk.data_ptr(), batch.toff.data_ptr()
+    b.solve_col = batch.solve_col.data_ptr()
+    b.periods = batch.periods.data_ptr()
+    if batch.f64:
+        b.thk64, b.vp64, b.vs64, b.rho64 = (batch.thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
+                                            rho.data_ptr())
+    else:
+        b.thk, b.vp, b.vs, b.rho = (batch.thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
+                                    rho.data_ptr())
+    return b
+
+
+def layered_vjp(batch: LayeredBatch, c_unrounded, grad_c, vs=None, vp=None, rho=None,
+                stream=None):
+    """grad_x[layer] = sum over the column's periods of grad_c * dc/dx, x in (vs, vp, rho),
+    by implicit differentiation at the roots c_unrounded of the same model.
+    Returns three float64 tensors laid out like the model."""
+    import torch
+    dev = batch.thk.device
+    g = [torch.empty(batch.thk.numel(), dtype=torch.float64, device=dev) for _ in range(3)]
+    b = _layered_struct(batch, vs, vp, rho)
+    b.c_unrounded = c_unrounded.data_ptr()
+    gc = grad_c.to(torch.float64).contiguous()
+    rc = _lib.lib().dazi_dk_layered_vjp_dev(C.byref(b), C.c_void_p(gc.data_ptr()),
+                                            C.c_void_p(g[0].data_ptr()),
+                                            C.c_void_p(g[1].data_ptr()),
+                                            C.c_void_p(g[2].data_ptr()), _stream_ptr(stream))
+    _lib.check(rc)
+    return g[0], g[1], g[2]
 
 
 def depthkernelTI(nx, ny, nz, vel, pvRc, iwave, igr, kmaxRc, tRc, depz, minthk, Lsen_Gsc, *,

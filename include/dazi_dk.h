@@ -93,6 +93,11 @@ typedef struct dazi_dk_layered {
   double* dc_drho;
   double* c_unrounded;     /* optional (may be NULL): the float64 root before the
                               float32 rounding of c, 0 where not found */
+  const double* thk64;     /* optional float64 model, used instead of thk/vp/vs/rho
+                              when all four are set (flattened in float64; for the */
+  const double* vp64;      /* differentiable path: the reference's own models are */
+  const double* vs64;      /* float32).  Kernels and VJPs are then with respect to */
+  const double* rho64;     /* these float64 values. */
 } dazi_dk_layered;
 
 int32_t dazi_dk_abi_version(void);
@@ -144,6 +149,16 @@ int32_t dazi_dk_stage_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, co
 /* Ragged layered batch (the batched persistent solver's native input). */
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
                             void* stream);
+
+/* Vector-Jacobian product of the layered batch's phase velocities: for every
+ * layer m of every column, grad_x[loff+m] = sum over the column's periods of
+ * grad_c[s] * dc_s/dx_m, x in {vs, vp, rho} of the unflattened model, by
+ * implicit differentiation of the secular function at the root
+ * (dc/dx = -(dF/dx)/(dF/dc)).  batch->c_unrounded must hold the roots of a
+ * dazi_dk_layered_dev call on the same batch; periods without a root
+ * contribute nothing.  Deterministic (no atomics). */
+int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* batch, const double* grad_c,
+                                double* grad_vs, double* grad_vp, double* grad_rho, void* stream);
 
 /* Event timings (ms) of the last call's phases on its stream:
  * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels,
