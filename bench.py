@@ -121,6 +121,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device-to-device copies of MIB MiB (calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
     ap.add_argument("--cpu-baseline-child", type=int, default=0)
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--cpw", type=int, default=16)
@@ -138,6 +141,16 @@ def main():
 
     import numpy as np
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps(dict(copy_probe_bytes=n, copies=5)), flush=True)
+        return
 
     import dazi_current_release_amd as dk
     from dazi_current_release_amd import configs, _lib

@@ -234,7 +234,7 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
 }
 
 // ------------------------------------------------------------------ forward root search
-enum { ST_START = 0, ST_UP = 1, ST_DN = 2, ST_REF = 3, ST_DONE = 4 };
+enum { ST_START = 0, ST_UP = 1, ST_DN = 2, ST_REF = 3, ST_VER = 4, ST_DONE = 5 };
 #define DK_MAX_REFINE 48
 #define DK_MAX_ROUNDS 100000
 
@@ -283,6 +283,25 @@ __device__ __forceinline__ double fv_ratio_root(double lo, double hi, FV flo, FV
   return lo + (hi - lo) * t;
 }
 
+// Root at y = 0 of the cubic x(y) through the points (X[t], Y[t]) with ok[t]
+// set (inverse interpolation of the secular function around a bracket).
+__device__ __forceinline__ double inv_interp4(const double* X, const double* Y, const bool* ok) {
+  double c = 0.0;
+#pragma unroll
+  for (int p = 0; p < 4; p++) {
+    double w = 1.0;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (q != p && ok[q]) w *= Y[q] / (Y[q] - Y[p]);
+    c += ok[p] ? w * X[p] : 0.0;
+  }
+  return c;
+}
+
+// Lanes of a group that check the window around the interpolated root in a
+// verification round; the other lanes scan the next period.
+constexpr int DK_NW = 4;
+
 template <int S>
 __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int col_end,
                                                  int persistent, int lds_stride) {
@@ -296,6 +315,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
   const double dc = (double)0.005f;
   const double onea = 1.5;
   const double twopi = 2.0 * 3.141592653589793;
+  const double tol = B.tol;
 
   int col;
   if (persistent) {
@@ -318,6 +338,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
   int slo = 0, lo_ok = 0, hi_ok = 0;
   int nref = 0;     // multisection rounds of the current bracket
   int nround = 0;   // evaluation rounds of the current column (hang guard)
+  double cstar = 0, c1n = 0, omega_n = 1;  // verification round: estimate, next scan start
 
   auto begin_column = [&]() {
     if (col >= col_end) {
@@ -361,6 +382,27 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
     st = ST_DONE;
   };
 
+  // stores the root of period k and moves to period k+1; false when the
+  // column is finished (all periods done, or the root lies above betmx)
+  auto emit = [&](double root) -> bool {
+    if (root > betmx_d) {
+      fail_from(k);
+      return false;
+    }
+    if (r == 0) {
+      int64_t s = solve_index(B, col, toff, k);
+      B.croot[s] = root;
+      B.pv[s] = (double)(float)root;
+    }
+    k++;
+    if (k >= K) {
+      if (r == 0) B.status[col] = 0;
+      st = ST_DONE;
+      return false;
+    }
+    return true;
+  };
+
   while (true) {
     // a wave runs while any of its groups has work; finished groups in a
     // persistent launch fetch the next column
@@ -373,7 +415,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
     if (!__any(st != ST_DONE)) break;
 
     // ---- trial point of this lane
-    double x;
+    double x, om = omega;
     bool elig = true;
     if (st == ST_START)
       x = c1 + (double)r * dc;
@@ -384,20 +426,36 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       elig = x > clow;
     } else if (st == ST_REF)
       x = lo + (double)(r + 1) * ((hi - lo) / (double)(S + 1));
-    else {
+    else if (st == ST_VER) {
+      if (r < DK_NW) {
+        // window cstar -/+ {0.95, 0.05} tol: every sub-interval is below tol
+        const double off = (r == 0) ? -0.95 : (r == 1) ? -0.05 : (r == 2) ? 0.05 : 0.95;
+        x = cstar + off * tol;
+        elig = (x > lo) && (x < hi);
+      } else if (k + 1 < K) {
+        x = c1n + (double)(r - DK_NW) * dc;
+        om = omega_n;
+      } else {
+        x = 1.0;
+        elig = false;
+      }
+    } else {
       x = 1.0;
       elig = false;
     }
     FV f = {1.0, 0};
     if (st != ST_DONE && elig) {
       auto lay = [&](int m) { return mdl[m]; };
-      f.m = secular(x, omega, L, lay, &f.e);
+      f.m = secular(x, om, L, lay, &f.e);
     }
     const int sg = signbit(f.m) ? 1 : 0;
     const uint64_t neg = group_bits<S>(__ballot(sg), gbase);
     const uint64_t okm = group_bits<S>(__ballot(elig), gbase);
-    // sign of the lane before me inside the group
+    const bool fl = (x < cm) || (x >= betmx_d + dc);
+    const uint64_t flall = group_bits<S>(__ballot(fl), gbase);
     auto sign_at = [&](int j) { return (int)((neg >> j) & 1ull); };
+    auto fv_at = [&](int j) { return FV{__shfl(f.m, gbase + j, 64), __shfl(f.e, gbase + j, 64)}; };
+    auto x_at = [&](int j) { return __shfl(x, gbase + j, 64); };
 
     if (st == ST_DONE) continue;
     // every state is bounded by the betmx test, the clow floor or the bracket
@@ -406,13 +464,17 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       fail_from(k);
       continue;
     }
+    // the lanes' trial points belong to the state this round started in
+    const int st0 = st;
 
-    if (st == ST_START || st == ST_UP) {
-      int first = (st == ST_START) ? 1 : 0;
-      if (st == ST_START) {
-        int s0 = sign_at(0);
-        FV f0 = {__shfl(f.m, gbase, 64), __shfl(f.e, gbase, 64)};
-        d1 = f0;
+    // Upward scan over the n lanes from `base` (the reference's getsol steps,
+    // surfdisp96.f:384-476); first = 1 when lane `base` is c1 itself (the
+    // period's first step, which also sets the scan direction).  A bracket
+    // leads to a verification round around the interpolated root.
+    auto scan = [&](int base, int n, int first) {
+      if (first) {
+        int s0 = sign_at(base);
+        d1 = fv_at(base);
         d1x = c1;
         int idir = +1;
         if (k == 0)
@@ -421,47 +483,122 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
           idir = -1;
         if (idir < 0) {
           st = ST_DN;
-          continue;
+          return;
         }
       }
-      // change[j]: sign differs from the previous point; fail[j]: step j leaves the range
-      uint64_t chg = 0, flm = 0;
-      for (int j = first; j < S; j++) {
-        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(j - 1);
-        if (sign_at(j) != prev) chg |= 1ull << j;
+      uint64_t chg = 0;
+      for (int j = first; j < n; j++) {
+        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(base + j - 1);
+        if (sign_at(base + j) != prev) chg |= 1ull << j;
       }
-      bool fl = (x < cm) || (x >= betmx_d + dc);
-      flm = group_bits<S>(__ballot(fl), gbase);
+      uint64_t flm = (flall >> base) & ((n >= 64) ? ~0ull : ((1ull << n) - 1ull));
       if (first) flm &= ~1ull;
       int jc = ffs64(chg), jf = ffs64(flm);
       if (jc >= 0 && (jf < 0 || jc <= jf)) {
-        double xa = (jc == 0) ? c1 : __shfl(x, gbase + jc - 1, 64);
-        FV fa = (jc == 0) ? d1 : FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
-        int sa = (jc == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(jc - 1);
-        int aok = (jc == 0) ? (d1x == c1) : 1;
-        double xb = __shfl(x, gbase + jc, 64);
-        FV fb = {__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
-        lo = xa;
-        hi = xb;
-        flo = fa;
-        fhi = fb;
-        slo = sa;
-
-        lo_ok = aok;
+        lo = (jc == 0) ? c1 : x_at(base + jc - 1);
+        flo = (jc == 0) ? d1 : fv_at(base + jc - 1);
+        slo = (jc == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(base + jc - 1);
+        lo_ok = (jc == 0) ? (d1x == c1) : 1;
+        hi = x_at(base + jc);
+        fhi = fv_at(base + jc);
         hi_ok = 1;
         nref = 0;
-        st = ST_REF;
+        if (!lo_ok) {
+          st = ST_REF;
+          return;
+        }
+        // inverse interpolation through the grid points jc-2 .. jc+1 (j = -1
+        // is the stored del1 point of an upward continuation)
+        double PX[4], PY[4];
+        int PE[4];
+        bool ok[4];
+        int emax = -2147483647;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+          const int j = jc - 2 + t;
+          const int jj = min(max(j, 0), n - 1);
+          PX[t] = x_at(base + jj);
+          FV fj = fv_at(base + jj);
+          ok[t] = (j >= 0 && j < n);
+          if (j == -1 && !first && d1x == c1) {
+            PX[t] = c1;
+            fj = d1;
+            ok[t] = true;
+          }
+          PY[t] = fj.m;
+          PE[t] = fj.e;
+          if (ok[t]) emax = max(emax, fj.e);
+        }
+#pragma unroll
+        for (int t = 0; t < 4; t++) PY[t] = ok[t] ? ldexp(PY[t], PE[t] - emax) : 0.0;
+        double cs = inv_interp4(PX, PY, ok);
+        if (!(cs > lo && cs < hi)) cs = fv_ratio_root(lo, hi, flo, fhi);
+        cstar = cs;
+        c1n = cs - onea * dc;
+        omega_n = (k + 1 < K) ? twopi / period_of(B, toff, k + 1) : omega;
+        st = ST_VER;
       } else if (jf >= 0) {
         fail_from(k);
-        continue;
       } else {
-        c1 = __shfl(x, gbase + S - 1, 64);
-        d1 = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        c1 = x_at(base + n - 1);
+        d1 = fv_at(base + n - 1);
         d1x = c1;
         st = ST_UP;
-        continue;
       }
-    } else if (st == ST_DN) {
+    };
+
+    // scan lanes of this round: all S lanes, or the lanes after the window of
+    // a verification round that accepted its root
+    int scan_base = -1, scan_first = 0;
+    if (st == ST_START || st == ST_UP) {
+      scan_base = 0;
+      scan_first = (st == ST_START);
+    } else if (st == ST_VER) {
+      // the window points inside (lo, hi), ascending: first sign change
+      double vlo = lo, vhi = hi;
+      FV vflo = flo, vfhi = fhi;
+      int vslo = slo;
+      bool found = false;
+      for (int j = 0; j < DK_NW; j++) {
+        double xj = x_at(j);
+        FV fj = fv_at(j);
+        if (!found && ((okm >> j) & 1ull)) {
+          if (sign_at(j) != vslo) {
+            vhi = xj;
+            vfhi = fj;
+            found = true;
+          } else {
+            vlo = xj;
+            vflo = fj;
+          }
+        }
+      }
+      lo = vlo;
+      hi = vhi;
+      flo = vflo;
+      fhi = vfhi;
+      if (hi - lo <= tol) {
+        double root = (signbit(flo.m) != signbit(fhi.m)) ? fv_ratio_root(lo, hi, flo, fhi)
+                                                          : 0.5 * (lo + hi);
+        if (emit(root)) {
+          // the next period's first scan step ran in the same round
+          omega = omega_n;
+          c1 = c1n;
+          clow = cm;
+          scan_base = DK_NW;
+          scan_first = 1;
+        }
+      } else {
+        nref = 0;
+        st = ST_REF;
+      }
+    }
+    if (scan_base >= 0) {
+      scan(scan_base, S - scan_base, scan_first);
+      continue;
+    }
+    if (st0 == ST_VER || st == ST_DONE) continue;
+    if (st == ST_DN) {
       int nv = __popcll(okm);  // eligible lanes form a prefix
       uint64_t chg = 0;
       for (int j = 0; j < nv; j++) {
@@ -471,37 +608,32 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       int jc = ffs64(chg);
       if (jc >= 0) {
         // bracket (x_jc, x_jc-1): lower end is the new point
-        double xb = (jc == 0) ? c1 : __shfl(x, gbase + jc - 1, 64);
-        FV fb = (jc == 0) ? d1 : FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
-        int sb = (jc == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(jc - 1);
+        double xb = (jc == 0) ? c1 : x_at(jc - 1);
+        FV fb = (jc == 0) ? d1 : fv_at(jc - 1);
         int bok = (jc == 0) ? (d1x == c1) : 1;
-        lo = __shfl(x, gbase + jc, 64);
-        flo = FV{__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
+        lo = x_at(jc);
+        flo = fv_at(jc);
         slo = sign_at(jc);
         lo_ok = 1;
         hi = xb;
         fhi = fb;
-
         hi_ok = bok;
         nref = 0;
         st = ST_REF;
       } else if (nv < S) {
         // floor reached: continue upward from clow with the last value kept
         if (nv > 0) {
-          d1 = FV{__shfl(f.m, gbase + nv - 1, 64), __shfl(f.e, gbase + nv - 1, 64)};
-          d1x = __shfl(x, gbase + nv - 1, 64);
+          d1 = fv_at(nv - 1);
+          d1x = x_at(nv - 1);
         }
         c1 = clow;
         st = ST_UP;
         continue;
       } else {
-        c1 = __shfl(x, gbase + S - 1, 64);
-        d1 = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        c1 = x_at(S - 1);
+        d1 = fv_at(S - 1);
         d1x = c1;
-        if (c1 < cm) {
-          fail_from(k);
-          continue;
-        }
+        if (c1 < cm) fail_from(k);
         continue;
       }
     } else if (st == ST_REF) {
@@ -513,50 +645,35 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       int jc = ffs64(chg);
       if (jc >= 0) {
         if (jc > 0) {
-          lo = __shfl(x, gbase + jc - 1, 64);
-          flo = FV{__shfl(f.m, gbase + jc - 1, 64), __shfl(f.e, gbase + jc - 1, 64)};
+          lo = x_at(jc - 1);
+          flo = fv_at(jc - 1);
           slo = sign_at(jc - 1);
           lo_ok = 1;
         }
-        hi = __shfl(x, gbase + jc, 64);
-        fhi = FV{__shfl(f.m, gbase + jc, 64), __shfl(f.e, gbase + jc, 64)};
-
+        hi = x_at(jc);
+        fhi = fv_at(jc);
         hi_ok = 1;
       } else {
-        lo = __shfl(x, gbase + S - 1, 64);
-        flo = FV{__shfl(f.m, gbase + S - 1, 64), __shfl(f.e, gbase + S - 1, 64)};
+        lo = x_at(S - 1);
+        flo = fv_at(S - 1);
         slo = sign_at(S - 1);
         lo_ok = 1;
       }
     }
 
     if (st == ST_REF) nref++;
-    if (st == ST_REF && ((hi - lo) <= B.tol || nref >= DK_MAX_REFINE)) {
+    if (st == ST_REF && ((hi - lo) <= tol || nref >= DK_MAX_REFINE)) {
       double root;
       bool opp = (signbit(flo.m) != signbit(fhi.m));
       if (lo_ok && hi_ok && opp)
         root = fv_ratio_root(lo, hi, flo, fhi);
       else
         root = 0.5 * (lo + hi);
-      if (root > betmx_d) {
-        fail_from(k);
-        continue;
-      }
-      if (r == 0) {
-        int64_t s = solve_index(B, col, toff, k);
-        B.croot[s] = root;
-        B.pv[s] = (double)(float)root;
-      }
-      k++;
-      if (k >= K) {
-        if (r == 0) B.status[col] = 0;
-        st = ST_DONE;
-      } else {
-        omega = twopi / period_of(B, toff, k);
-        c1 = root - onea * dc;
-        clow = cm;
-        st = ST_START;
-      }
+      if (!emit(root)) continue;
+      omega = twopi / period_of(B, toff, k);
+      c1 = root - onea * dc;
+      clow = cm;
+      st = ST_START;
     }
   }
 }

@@ -10,7 +10,9 @@
 ! Outputs are written in place in the reference's layouts; the tile controls
 ! DAZI_PDSURF_TILE_COLUMNS / DAZI_DEPTHKERNEL_TI_TILE_COLUMNS are read as the
 ! reference reads them (CalSurfG.f90:173-174, depthkernelTI.f90:250-253).
-! A non-zero ABI code prints the library's message and stops with that code.
+! A non-zero ABI code prints the library's message (stdout and unit 66) and
+! stops with that code (9301-9304).  DAZI_DEPTHKERNEL_DUMP_DIR and
+! DAZI_DEPTHKERNEL_TI_DUMP_DIR write the reference's dump files.
 module dazi_dk
   use, intrinsic :: iso_c_binding
   implicit none
@@ -119,11 +121,18 @@ contains
     if (log_open) write(66, '(A)') text
   end subroutine log_line
 
+  ! Writes the error to stdout and unit 66 (as the reference's backends do) and
+  ! stops with the ABI code.
   subroutine check(rc, where)
     integer(c_int32_t), intent(in) :: rc
     character(len=*), intent(in) :: where
+    character(len=:), allocatable :: msg
+    character(len=16) :: code
     if (rc == DAZI_DK_OK) return
-    write(0, '(A,A,I0,A,A)') where, ': dazi_dk error ', rc, ': ', dazi_dk_error_text()
+    write(code, '(I0)') rc
+    msg = 'ERROR: ' // where // ' hip backend: dazi_dk error ' // trim(code) // ': ' // &
+          dazi_dk_error_text()
+    call log_line(msg)
     select case (rc)
     case (DAZI_DK_EINVAL)
       stop 9301
@@ -138,7 +147,44 @@ contains
     end select
   end subroutine check
 
-  ! Reference: depthkernel (CalSurfG.f90:1-113), all three node kernels.
+  ! Dump directory from `var` (set and non-empty), with the reference's log line.
+  logical function dump_dir_of(var, label, dir)
+    character(len=*), intent(in) :: var, label
+    character(len=:), allocatable, intent(out) :: dir
+    character(len=1024) :: v
+    integer :: st
+    logical :: log_open
+    call get_environment_variable(var, v, status=st)
+    dump_dir_of = (st == 0 .and. len_trim(v) > 0)
+    dir = trim(adjustl(v))
+    if (st /= 0) return
+    inquire(unit=66, opened=log_open)
+    if (log_open) write(66, '(A,A)') label, dir
+    if (dump_dir_of) call execute_command_line('mkdir -p ' // dir)
+  end function dump_dir_of
+
+  subroutine dump_f64(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    real*8, intent(in) :: a(n)
+    integer :: u
+    open(newunit=u, file=path, form='unformatted', access='stream', status='replace')
+    write(u) a
+    close(u)
+  end subroutine dump_f64
+
+  subroutine dump_f32(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    real*4, intent(in) :: a(n)
+    integer :: u
+    open(newunit=u, file=path, form='unformatted', access='stream', status='replace')
+    write(u) a
+    close(u)
+  end subroutine dump_f32
+
+  ! Reference: depthkernel (CalSurfG.f90:1-113), all three node kernels; dumps
+  ! as depthkernel_dump_arrays (CalSurfG.f90:115-140) under DAZI_DEPTHKERNEL_DUMP_DIR.
   subroutine depthkernel_hip(nx, ny, nz, vel, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc, &
                              iwave, igr, kmaxRc, tRc, depz, minthk)
     integer nx, ny, nz
@@ -152,6 +198,7 @@ contains
     real*8 pvRc(nx*ny, kmaxRc)
     type(dazi_dk_desc) :: d
     character(len=160) :: line
+    character(len=:), allocatable :: dir
 
     d = dazi_dk_desc(nx, ny, nz, kmaxRc, iwave, igr, &
                      env_int('DAZI_PDSURF_TILE_COLUMNS', 64), 0, minthk, 0)
@@ -161,6 +208,12 @@ contains
     call log_line(trim(line))
     call check(c_dazi_dk_iso(d, vel, c_null_ptr, c_null_ptr, depz, tRc, pvRc, &
                              sen_vsRc, sen_vpRc, sen_rhoRc), 'depthkernel')
+    if (dump_dir_of('DAZI_DEPTHKERNEL_DUMP_DIR', 'Depth kernel dump dir: ', dir)) then
+      call dump_f64(dir // '/pv_depthkernel.bin', pvRc, nx*ny*kmaxRc)
+      call dump_f64(dir // '/sen_vs_depthkernel.bin', sen_vsRc, nx*ny*kmaxRc*nz)
+      call dump_f64(dir // '/sen_vp_depthkernel.bin', sen_vpRc, nx*ny*kmaxRc*nz)
+      call dump_f64(dir // '/sen_rho_depthkernel.bin', sen_rhoRc, nx*ny*kmaxRc*nz)
+    end if
   end subroutine depthkernel_hip
 
   ! Reference: depthkernelTI (depthkernelTI.f90:2-34): pvRc and Lsen_Gsc.
@@ -177,13 +230,21 @@ contains
     real*4 Lsen_Gsc(nx*ny, kmaxRc, nz-1)
     type(dazi_dk_desc) :: d
     character(len=160) :: line
+    character(len=:), allocatable :: dir
 
     d = dazi_dk_desc(nx, ny, nz, kmaxRc, iwave, igr, &
                      env_int('DAZI_DEPTHKERNEL_TI_TILE_COLUMNS', 64), 0, minthk, 0)
     call log_line('Depth kernel TI backend: hip')
+    write(line, '(A,I0,A,I0,A,I0,A,I0)') 'Depth kernel TI grid/periods: nx=', nx, ' ny=', ny, &
+        ' nz=', nz, ' kmax=', kmaxRc
+    call log_line(trim(line))
     write(line, '(A,I0)') 'Depth kernel TI batch tile columns: ', d%tile_columns
     call log_line(trim(line))
     call check(c_dazi_dk_ti(d, vel, depz, tRc, pvRc, Lsen_Gsc), 'depthkernelTI')
+    if (dump_dir_of('DAZI_DEPTHKERNEL_TI_DUMP_DIR', 'Depth kernel TI dump dir: ', dir)) then
+      call dump_f64(dir // '/pv_ti.bin', pvRc, nx*ny*kmaxRc)
+      call dump_f32(dir // '/lsen_gsc_ti.bin', Lsen_Gsc, nx*ny*kmaxRc*(nz-1))
+    end if
   end subroutine depthkernelTI_hip
 
 end module dazi_dk

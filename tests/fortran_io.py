@@ -34,8 +34,11 @@ def read_output(path, ncol, K, nz):
     return out
 
 
-def run(tmpdir, vel, depz, periods, minthk, timeout=300):
+def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None):
     fin, fout = os.path.join(tmpdir, "in.bin"), os.path.join(tmpdir, "out.bin")
     write_input(fin, vel, depz, periods, minthk)
-    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=timeout)
+    e = dict(os.environ)
+    e.update(env or {})
+    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=timeout,
+                       env=e)
     return r, fout

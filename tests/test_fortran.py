@@ -24,4 +24,4 @@ def test_no_device_stops_with_abi_code(tmp_path):
     r, _ = FIO.run(str(tmp_path), vel, f["depz"], f["periods"], float(f["minthk"]))
     assert r.returncode == 9304 % 256, (r.returncode, r.stderr)
     assert "Depth kernel backend: hip" in r.stdout
-    assert "dazi_dk error 9304" in r.stderr
+    assert "ERROR: depthkernel hip backend: dazi_dk error 9304" in r.stdout

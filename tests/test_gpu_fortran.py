@@ -17,7 +17,10 @@ def test_fortran_host_matches_python_and_reference(tmp_path):
     f = load_golden("ref_plain_1024.npz")
     nz = f["vel"].shape[1]
     vel = np.asfortranarray(f["vel"].reshape(8, 4, nz, order="F"))
-    r, fout = FIO.run(str(tmp_path), vel, f["depz"], f["periods"], float(f["minthk"]))
+    dump, dump_ti = tmp_path / "dump", tmp_path / "dump_ti"
+    r, fout = FIO.run(str(tmp_path), vel, f["depz"], f["periods"], float(f["minthk"]),
+                      env=dict(DAZI_DEPTHKERNEL_DUMP_DIR=str(dump),
+                               DAZI_DEPTHKERNEL_TI_DUMP_DIR=str(dump_ti)))
     assert r.returncode == 0, r.stderr
     assert "Depth kernel backend: hip" in r.stdout
     o = FIO.read_output(fout, 32, len(f["periods"]), nz)
@@ -26,6 +29,14 @@ def test_fortran_host_matches_python_and_reference(tmp_path):
         assert np.array_equal(o[name], want), name
     assert np.array_equal(o["pv"] == 0, f["pv"] == 0)
     assert np.abs(o["pv"] - f["pv"]).max() <= 1e-5
+    # reference dump files (CalSurfG.f90:115-140, depthkernelTI.f90:195-213)
+    for name, fn in (("pv", "pv_depthkernel.bin"), ("sen_vs", "sen_vs_depthkernel.bin"),
+                     ("sen_vp", "sen_vp_depthkernel.bin"), ("sen_rho", "sen_rho_depthkernel.bin")):
+        assert np.array_equal(np.fromfile(dump / fn, np.float64), o[name].ravel(order="F"))
+    assert np.array_equal(np.fromfile(dump_ti / "pv_ti.bin", np.float64),
+                          o["ti_pv"].ravel(order="F"))
+    assert np.array_equal(np.fromfile(dump_ti / "lsen_gsc_ti.bin", np.float32),
+                          o["ti_lsen"].ravel(order="F"))
     tpv, tls = dk.run_depthkernelTI(vel, f["periods"], f["depz"], float(f["minthk"]))
     assert np.array_equal(o["ti_pv"], tpv) and np.array_equal(o["ti_lsen"], tls)
     n = f["ti_lsen"].shape[0]
