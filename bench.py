@@ -214,8 +214,18 @@ def main():
     flop_dom = nsolves * dom_nev * FLOP_PER_LAYER_EVAL * L
     achieved = flop_dom / (dom_ms * 1e-3)
     flop_step = nsolves * (N_EV + N_K_ISO + N_K_TI) * FLOP_PER_LAYER_EVAL * L
+    # HBM bytes per launch of the dominant kernel: FETCH_SIZE / WRITE_SIZE
+    # counter passes of this workload, scaled by a copy of known size
+    traffic, traffic_src = None, None
+    tj = os.path.join(HERE, "profiles", "k_forward_traffic.json")
+    if dom.startswith("k_forward") and os.path.exists(tj):
+        with open(tj) as fh:
+            t = json.load(fh)
+        if t.get("workload") == "plain_4096":
+            traffic, traffic_src = t["bytes_per_launch"], t["source"]
     roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12,
-                    unit="TFLOP/s", frac=achieved / PEAK_FP64, traffic=None,
+                    unit="TFLOP/s", frac=achieved / PEAK_FP64, traffic=traffic,
+                    traffic_source=traffic_src,
                     kernel=dom, pipe="FP64 VALU (no MFMA: not a contraction)",
                     flop_per_launch=flop_dom, avg_launch_ms=dom_ms,
                     step_frac=flop_step / (ms_per_step * 1e-3) / PEAK_FP64)
