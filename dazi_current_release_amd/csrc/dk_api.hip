@@ -151,8 +151,9 @@ void rec(Pool* P, int i, hipStream_t st) {
 int run_forward(Pool* P, const dk::BatchDev& B, const dazi_dk_tune& T, int lmax,
                 hipStream_t st) {
   int S = T.lanes_per_scan, cpw = T.cols_per_wg;
-  int lds_stride = lmax;
-  size_t lds = (size_t)cpw * lds_stride * 16;
+  // two double4 records per layer and column in LDS
+  int lds_stride = 2 * lmax;
+  size_t lds = (size_t)cpw * lds_stride * 32;
   if (lds > 64 * 1024)
     return set_err(DAZI_DK_ELIMIT, "model too deep for LDS staging: %d layers x %d columns", lmax,
                    cpw);

@@ -233,6 +233,129 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
   bad_out[col] = bad;
 }
 
+// ------------------------------------------------------------------ value-only secular function
+// The root search only needs values of F.  Each layer is staged in LDS as two
+// double4, (d, 1/a^2, 1/b^2, b^2) and (rho, 1/rho, rho^2, 1/rho^2) of the
+// flattened layer, so an evaluation divides by no layer constant; per wave type
+// one exp gives both exp(-p) and exp(-2p), and exp(-(p+q)) is their product.
+// Same function as dk::secular (surfdisp96.f:767-865 dltar4 with var/dnka),
+// rounded differently.
+__device__ __forceinline__ void layer_rec(const double4& l, double4* g, double4* r) {
+  *g = make_double4(l.x, 1.0 / (l.y * l.y), 1.0 / (l.z * l.z), l.z * l.z);
+  *r = make_double4(l.w, 1.0 / l.w, l.w * l.w, 1.0 / (l.w * l.w));
+}
+
+// One wave type of a layer for X = k^2 - kx^2: C = cosh(p), S = sinh(p)/sqrt(X),
+// R = sqrt(X) sinh(p), p = sqrt(X) d, all scaled by exp(-p) (X > 0), or
+// cos, sin/sqrt(-X), -sqrt(-X) sin (X < 0).  Returns the scale, *ex = p or 0.
+__device__ __forceinline__ double hyp_fwd(double X, double d, double* C, double* S, double* R,
+                                          double* ex) {
+  const double ax = fabs(X);
+  if (!(ax > 0.0)) {
+    *C = 1.0;
+    *S = d;
+    *R = 0.0;
+    *ex = 0.0;
+    return 1.0;
+  }
+  double y = __builtin_amdgcn_rsq(ax);
+  y = y * fma(-0.5 * ax * y, y, 1.5);  // one Newton step: 1/sqrt|X|
+  const double ra = ax * y;            // sqrt|X|
+  const double p = ra * d;
+  if (X > 0.0) {
+    const double s = (p < 700.0) ? exp(-p) : 0.0;
+    const double fac = (p < 16.0) ? s * s : 0.0;
+    const double sh = 0.5 - 0.5 * fac;
+    *C = 0.5 + 0.5 * fac;
+    *S = sh * y;
+    *R = ra * sh;
+    *ex = p;
+    return s;
+  }
+  double sp, cp;
+  sincos(p, &sp, &cp);
+  *C = cp;
+  *S = sp * y;
+  *R = -ra * sp;
+  *ex = 0.0;
+  return 1.0;
+}
+
+__device__ __forceinline__ void halfspace_fwd(double wvno2, double om2, double iom2,
+                                              const double4& g, const double4& r, double e[5]) {
+  const double ra = sqrt(fabs(fma(-om2, g.y, wvno2)));
+  const double rb = sqrt(fabs(fma(-om2, g.z, wvno2)));
+  const double gammk = 2.0 * g.w * iom2;
+  const double gam = gammk * wvno2;
+  const double gamm1 = gam - 1.0;
+  const double rarb = ra * rb;
+  const double rho = r.x;
+  e[0] = r.z * (gamm1 * gamm1 - gam * gammk * rarb);
+  e[1] = -(rho * ra);
+  e[2] = rho * (gamm1 - gammk * rarb);
+  e[3] = rho * rb;
+  e[4] = wvno2 - rarb;
+}
+
+// e <- e * CA(layer) with the Dunkin matrix of dnka, fused (no 5x5 array).
+__device__ __forceinline__ void layer_step_fwd(double wvno2, double om2, double iom2,
+                                               const double4& g, const double4& r, double e[5]) {
+  const double d = g.x;
+  const double gammk = 2.0 * g.w * iom2;
+  const double gam = gammk * wvno2;
+  double cosp, w, x, cosq, y, z, pex, sex;
+  const double sP = hyp_fwd(fma(-om2, g.y, wvno2), d, &cosp, &w, &x, &pex);
+  const double sS = hyp_fwd(fma(-om2, g.z, wvno2), d, &cosq, &y, &z, &sex);
+  const double a0 = (pex + sex < 60.0) ? sP * sS : 0.0;
+  const double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w,
+               cqx = cosq * x, xy = x * y, xz = x * z, wy = w * y, wz = w * z;
+  const double gamm1 = gam - 1.0, twgm1 = gam + gamm1, gmgmk = gam * gammk, gmgm1 = gam * gamm1,
+               gm1sq = gamm1 * gamm1;
+  const double rho = r.x, irho = r.y, rho2 = r.z, irho2 = r.w;
+  const double a0pq = a0 - cpcq;
+  const double c00 = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+  const double c01 = (wvno2 * cpy - cqx) * irho;
+  const double c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
+  const double c03 = (cpz - wvno2 * cqw) * irho;
+  const double c04 = -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2;
+  const double c10 = (gmgmk * cpz - gm1sq * cqw) * rho;
+  const double c12 = gammk * cpz - gamm1 * cqw;
+  const double c13 = -wz;
+  const double c30 = (gm1sq * cpy - gmgmk * cqx) * rho;
+  const double c31 = -xy;
+  const double c32 = gamm1 * cpy - gammk * cqx;
+  const double c40 = -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
+  const double c42 = -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
+  const double tt = -2.0 * wvno2;
+  const double c20 = tt * c42, c21 = tt * c32, c22 = a0 + 2.0 * (cpcq - c00), c23 = tt * c12,
+               c24 = tt * c02;
+  // rows of CA: 1: (c10, cpcq, c12, c13, c03); 3: (c30, c31, c32, cpcq, c01);
+  //             4: (c40, c30, c42, c10, c00)
+  const double e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+  e[0] = fma(e0, c00, fma(e1, c10, fma(e2, c20, fma(e3, c30, e4 * c40))));
+  e[1] = fma(e0, c01, fma(e1, cpcq, fma(e2, c21, fma(e3, c31, e4 * c30))));
+  e[2] = fma(e0, c02, fma(e1, c12, fma(e2, c22, fma(e3, c32, e4 * c42))));
+  e[3] = fma(e0, c03, fma(e1, c13, fma(e2, c23, fma(e3, cpcq, e4 * c10))));
+  e[4] = fma(e0, c04, fma(e1, c03, fma(e2, c24, fma(e3, c01, e4 * c00))));
+}
+
+// F(c, omega) as mantissa and exponent over the LDS-staged records of L layers
+// (half-space last).
+__device__ __forceinline__ double secular_fwd(double c, double omega, int L, const double4* mdl,
+                                              int* ex) {
+  if (omega < 1.0e-4) omega = 1.0e-4;
+  const double wvno = omega / c, wvno2 = wvno * wvno, om2 = omega * omega, iom2 = 1.0 / om2;
+  double e[5];
+  halfspace_fwd(wvno2, om2, iom2, mdl[2 * (L - 1)], mdl[2 * (L - 1) + 1], e);
+  int E = pow2_normalize(e);
+  for (int m = L - 2; m >= 0; m--) {
+    layer_step_fwd(wvno2, om2, iom2, mdl[2 * m], mdl[2 * m + 1], e);
+    E += pow2_normalize(e);
+  }
+  *ex = E;
+  return e[0];
+}
+
 // ------------------------------------------------------------------ forward root search
 enum { ST_START = 0, ST_UP = 1, ST_DN = 2, ST_REF = 3, ST_VER = 4, ST_DONE = 5 };
 #define DK_MAX_REFINE 48
@@ -346,7 +469,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       return;
     }
     col_geom(B, col, &L, &loff, &K, &toff);
-    for (int m = r; m < L; m += S) mdl[m] = B.layers[loff + m];
+    for (int m = r; m < L; m += S) layer_rec(B.layers[loff + m], &mdl[2 * m], &mdl[2 * m + 1]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -444,10 +567,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       elig = false;
     }
     FV f = {1.0, 0};
-    if (st != ST_DONE && elig) {
-      auto lay = [&](int m) { return mdl[m]; };
-      f.m = secular(x, om, L, lay, &f.e);
-    }
+    if (st != ST_DONE && elig) f.m = secular_fwd(x, om, L, mdl, &f.e);
     const int sg = signbit(f.m) ? 1 : 0;
     const uint64_t neg = group_bits<S>(__ballot(sg), gbase);
     const uint64_t okm = group_bits<S>(__ballot(elig), gbase);
