@@ -75,134 +75,161 @@ struct Layer {  // flattened TI layer (float32 values of the reference, as doubl
   double d, A, C, F, L, N, rho;
 };
 
-// Eigen-solution of one layer (gettiegn).
+// Eigen-solution of one layer (gettiegn) at real (omega, k).  The squared
+// vertical wavenumbers L1 (P) and L2 (SV) are the roots of a real quadratic;
+// for the models depthkernelTI builds (F = A - 2L before flattening, so the
+// flattened layers are within a fraction of a percent of isotropy) its
+// discriminant is positive, both roots are real, and every quantity of the up
+// and down sweeps is real: nu = sqrt(L) is real (evanescent) or i*sqrt(-L)
+// (oscillatory).  ok = 0 flags a negative discriminant (a complex pair, which
+// only strongly anisotropic layers produce).
 struct Eig {
-  cx rp, rsv;
-  cx x11, x21, x31, x41, x12, x22, x32, x42;
-  cx NP, NSV;
+  double L1, L2;      // nu_P^2, nu_SV^2
+  double nP, nS;      // |nu_P|, |nu_SV|
+  int iP, iS;         // nu imaginary
+  double x11, x21, x31, x41, x12, x22, x32, x42;
+  double NP, NSV;
+  int ok;
 };
 
 DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
   const double om2 = omega * omega, wvno2 = wvno * wvno;
-  cx a = mk(wvno * m.F / m.C);
-  cx b = mk(1.0 / m.C);
-  cx c = mk(-m.rho * omega * omega + wvno * wvno * (m.A - m.F * m.F / m.C));
-  cx d = mk(-wvno);
-  cx e = mk(1.0 / m.L);
-  cx f = mk(-m.rho * omega * omega);
-  cx ddef = mk(wvno * wvno - m.rho * omega * omega / m.L);
-  cx aabc = mk(wvno * wvno * m.A / m.C - m.rho * omega * omega / m.C);
-  cx bb = 2.0 * (a * d) + e * c + f * b;
-  cx cc = ddef * aabc;
-  cx s = csqrt_(bb * bb - 4.0 * cc);
-  if (s.i < 0.0) s = -s;
-  cx L1, L2;
-  if (bb.r < 0.0 && s.r < 0.0) {
-    L2 = (bb - s) / 2.0;
-    L1 = (cabs_(L2) > 0.0) ? cc / L2 : (bb + s) / 2.0;
-  } else {
-    L1 = (bb + s) / 2.0;
-    L2 = (cabs_(L1) > 0.0) ? cc / L1 : (bb - s) / 2.0;
-  }
-  cx xka2 = mk(wvno2) - L1, xkb2 = mk(wvno2) - L2;
-  if (cabs_(xkb2) < cabs_(xka2)) {
-    cx t = L1;
+  const double iC = 1.0 / m.C, iL = 1.0 / m.L;
+  const double a = wvno * m.F * iC;
+  const double b = iC;
+  const double c = -m.rho * om2 + wvno2 * (m.A - m.F * m.F * iC);
+  const double d = -wvno;
+  const double e = iL;
+  const double f = -m.rho * om2;
+  const double ddef = wvno2 - m.rho * om2 * iL;
+  const double aabc = wvno2 * m.A * iC - m.rho * om2 * iC;
+  const double bb = 2.0 * (a * d) + e * c + f * b;
+  const double cc = ddef * aabc;
+  const double disc = bb * bb - 4.0 * cc;
+  o->ok = disc >= 0.0;
+  const double s = sqrt(fmax(disc, 0.0));
+  double L1 = (bb + s) * 0.5, L2;
+  L2 = (L1 != 0.0) ? cc / L1 : (bb - s) * 0.5;
+  if (fabs(wvno2 - L2) < fabs(wvno2 - L1)) {
+    double t = L1;
     L1 = L2;
     L2 = t;
   }
-  o->rp = csqrt_(L1);
-  o->rsv = csqrt_(L2);
-  if (o->rp.r < 0.0) o->rp = -o->rp;
-  if (o->rsv.r < 0.0) o->rsv = -o->rsv;
-  cx bcaa = b * c + a * a, ddef2 = d * d + e * f;
-  o->x12 = b * d - a * e;
-  o->x22 = b * L2 - e * bcaa;
-  o->x32 = L2 - (a * d + c * e);
-  o->x42 = -(a * L2) + d * bcaa;
-  o->x11 = -(e * L1) + b * ddef2;
-  o->x21 = b * d - a * e;
-  o->x31 = d * L1 - a * ddef2;
-  o->x41 = -(L1 - a * d - b * f);
+  o->L1 = L1;
+  o->L2 = L2;
+  o->iP = L1 < 0.0;
+  o->iS = L2 < 0.0;
+  o->nP = sqrt(fabs(L1));
+  o->nS = sqrt(fabs(L2));
+  const double bcaa = b * c + a * a, ddef2 = d * d + e * f;
+  double x12 = b * d - a * e;
+  double x22 = b * L2 - e * bcaa;
+  double x32 = L2 - (a * d + c * e);
+  double x42 = -(a * L2) + d * bcaa;
+  double x11 = -(e * L1) + b * ddef2;
+  double x21 = b * d - a * e;
+  double x31 = d * L1 - a * ddef2;
+  double x41 = -(L1 - a * d - b * f);
   if (wvno != 0.0) {
-    cx z = mk(wvno) / o->x11;
-    o->x11 = o->x11 * z;
-    o->x21 = o->x21 * z;
-    o->x31 = o->x31 * z;
-    o->x41 = o->x41 * z;
-    z = mk(wvno) / o->x22;
-    o->x12 = o->x12 * z;
-    o->x22 = o->x22 * z;
-    o->x32 = o->x32 * z;
-    o->x42 = o->x42 * z;
+    double z = wvno / x11;
+    x11 *= z;
+    x21 *= z;
+    x31 *= z;
+    x41 *= z;
+    z = wvno / x22;
+    x12 *= z;
+    x22 *= z;
+    x32 *= z;
+    x42 *= z;
   }
-  o->NP = o->x11 * o->x41 - o->x21 * o->x31;
-  o->NSV = o->x12 * o->x42 - o->x22 * o->x32;
-  (void)om2;
+  o->x11 = x11;
+  o->x21 = x21;
+  o->x31 = x31;
+  o->x41 = x41;
+  o->x12 = x12;
+  o->x22 = x22;
+  o->x32 = x32;
+  o->x42 = x42;
+  o->NP = x11 * x41 - x21 * x31;
+  o->NSV = x12 * x42 - x22 * x32;
 }
+
+// Complex vertical wavenumbers of the eigen-solution.
+DK_HD cx nu_P(const Eig& g) { return g.iP ? cx{0.0, g.nP} : cx{g.nP, 0.0}; }
+DK_HD cx nu_S(const Eig& g) { return g.iS ? cx{0.0, g.nS} : cx{g.nS, 0.0}; }
 
 // Eigenvector matrix E (columns P+, S+, P-, S-) and its inverse G (evalg).
 DK_HD void eig_matrices(const Eig& g, cx E[4][4], cx G[4][4]) {
-  const cx rp = g.rp, rs = g.rsv;
-  cx dP = 2.0 * (rp * g.NP), dS = 2.0 * (rs * g.NSV);
-  G[0][0] = g.x41 * rp / dP;
-  G[1][0] = g.x42 / dS;
-  G[2][0] = g.x41 * rp / dP;
-  G[3][0] = -(g.x42 / dS);
-  G[0][1] = -(g.x31 / dP);
-  G[1][1] = -(g.x32 * rs / dS);
-  G[2][1] = g.x31 / dP;
-  G[3][1] = -(g.x32 * rs / dS);
-  G[0][2] = -(g.x21 * rp / dP);
-  G[1][2] = -(g.x22 / dS);
-  G[2][2] = -(g.x21 * rp / dP);
-  G[3][2] = g.x22 / dS;
-  G[0][3] = g.x11 / dP;
-  G[1][3] = g.x12 * rs / dS;
-  G[2][3] = -(g.x11 / dP);
-  G[3][3] = g.x12 * rs / dS;
-  E[0][0] = g.x11;
+  const cx rp = nu_P(g), rs = nu_S(g);
+  // 1/(2 nu N): nu real -> real, nu = i|nu| -> -i/(2|nu|N)
+  const cx iP = g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
+  const cx iS = g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
+  const double hP = 0.5 / g.NP, hS = 0.5 / g.NSV;  // nu/(2 nu N)
+  G[0][0] = mk(g.x41 * hP);
+  G[1][0] = g.x42 * iS;
+  G[2][0] = mk(g.x41 * hP);
+  G[3][0] = -(g.x42 * iS);
+  G[0][1] = -(g.x31 * iP);
+  G[1][1] = mk(-(g.x32 * hS));
+  G[2][1] = g.x31 * iP;
+  G[3][1] = mk(-(g.x32 * hS));
+  G[0][2] = mk(-(g.x21 * hP));
+  G[1][2] = -(g.x22 * iS);
+  G[2][2] = mk(-(g.x21 * hP));
+  G[3][2] = g.x22 * iS;
+  G[0][3] = g.x11 * iP;
+  G[1][3] = mk(g.x12 * hS);
+  G[2][3] = -(g.x11 * iP);
+  G[3][3] = mk(g.x12 * hS);
+  E[0][0] = mk(g.x11);
   E[1][0] = g.x21 * rp;
-  E[2][0] = g.x31;
+  E[2][0] = mk(g.x31);
   E[3][0] = g.x41 * rp;
   E[0][1] = g.x12 * rs;
-  E[1][1] = g.x22;
+  E[1][1] = mk(g.x22);
   E[2][1] = g.x32 * rs;
-  E[3][1] = g.x42;
-  E[0][2] = g.x11;
+  E[3][1] = mk(g.x42);
+  E[0][2] = mk(g.x11);
   E[1][2] = -(g.x21 * rp);
-  E[2][2] = g.x31;
+  E[2][2] = mk(g.x31);
   E[3][2] = -(g.x41 * rp);
   E[0][3] = -(g.x12 * rs);
-  E[1][3] = g.x22;
+  E[1][3] = mk(g.x22);
   E[2][3] = -(g.x32 * rs);
-  E[3][3] = g.x42;
+  E[3][3] = mk(g.x42);
 }
 
-// Exponentially scaled cosh/sinh of the P and SV phases (varsv).
+// Exponentially scaled cosh/sinh of the P and SV phases (varsv), real:
+// nu real: cosh, nu sinh, sinh/nu of p = nu d scaled by e^{-p}; nu = i|nu|:
+// cos, -|nu| sin, sin/|nu| of |nu| d.
 struct Hyp {
-  cx cosp, rsinp, sinpr, cossv, rsinsv, sinsvr;
+  double cosp, rsinp, sinpr, cossv, rsinsv, sinsvr;
   double pex, svex;
 };
 
+DK_HD void hyp_wave(int imag, double n, double dm, double* co, double* rsi, double* sir,
+                    double* ex) {
+  const double p = n * dm;
+  if (!imag) {
+    const double fac = (p < 15.0) ? exp(-2.0 * p) : 0.0;
+    const double sh = 0.5 - 0.5 * fac;
+    *co = 0.5 + 0.5 * fac;
+    *rsi = n * sh;
+    *sir = (p < 1.0e-5 && n < 1.0e-5) ? dm : sh / n;
+    *ex = p;
+  } else {
+    double s, c;
+    sincos(p, &s, &c);
+    *co = c;
+    *rsi = -n * s;
+    *sir = (n < 1.0e-5) ? dm : s / n;
+    *ex = 0.0;
+  }
+}
+
 DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
-  cx p = g.rp * dm, q = g.rsv * dm;
-  double s, c;
-  h->pex = p.r;
-  sincos(p.i, &s, &c);
-  cx epp = cx{c / 2.0, s / 2.0}, epm = cx{c / 2.0, -s / 2.0};
-  double pfac = (p.r < 15.0) ? exp(-2.0 * p.r) : 0.0;
-  h->cosp = epp + pfac * epm;
-  cx sinp = epp - pfac * epm;
-  h->rsinp = g.rp * sinp;
-  h->sinpr = (fabs(p.r) < 1.0e-5 && cabs_(g.rp) < 1.0e-5) ? mk(dm) : sinp / g.rp;
-  h->svex = q.r;
-  sincos(q.i, &s, &c);
-  cx eqp = cx{c / 2.0, s / 2.0}, eqm = cx{c / 2.0, -s / 2.0};
-  double sfac = (q.r < 15.0) ? exp(-2.0 * q.r) : 0.0;
-  h->cossv = eqp + sfac * eqm;
-  cx sinq = eqp - sfac * eqm;
-  h->rsinsv = g.rsv * sinq;
-  h->sinsvr = (fabs(q.r) < 1.0e-5 && cabs_(g.rsv) < 1.0e-5) ? mk(dm) : sinq / g.rsv;
+  hyp_wave(g.iP, g.nP, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex);
+  hyp_wave(g.iS, g.nS, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex);
 }
 
 // Per wave type, H = [u v~] M [a~ b~]^T / N with
@@ -210,47 +237,54 @@ DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
 //      M = [[cosp, sinpr],[rsinp, cosp]]
 //   S: u=(x12,0,x32,0) v~=(0,x22,0,x42) a~=(x42,0,-x22,0) b~=(0,-x32,0,x12),
 //      M = [[cossv, rsinsv],[sinsvr, cossv]]
+// u and v~ have disjoint supports (rows 0,2 and 1,3), as have a~ and b~
+// (columns 0,2 and 1,3), so H(i,j) = w_i * M(i%2, j%2) * z_j / N with
+// w = (x11, x21, x31, x41) and z = (x41, -x31, -x21, x11) for P.
 struct Wave {
-  cx u[4], v[4], a[4], b[4];
-  cx m00, m01, m10, m11;  // already divided by N
+  double w[4], z[4];
+  double m00, m01, m10;  // divided by N (m11 = m00)
 };
 
 DK_HD void waves(const Eig& g, const Hyp& h, double sP, double sS, Wave* P, Wave* S) {
-  cx z = mk(0.0);
-  cx uP[4] = {g.x11, z, g.x31, z}, vP[4] = {z, g.x21, z, g.x41};
-  cx aP[4] = {g.x41, z, -g.x21, z}, bP[4] = {z, -g.x31, z, g.x11};
-  cx uS[4] = {g.x12, z, g.x32, z}, vS[4] = {z, g.x22, z, g.x42};
-  cx aS[4] = {g.x42, z, -g.x22, z}, bS[4] = {z, -g.x32, z, g.x12};
-  for (int i = 0; i < 4; i++) {
-    P->u[i] = uP[i];
-    P->v[i] = vP[i];
-    P->a[i] = aP[i];
-    P->b[i] = bP[i];
-    S->u[i] = uS[i];
-    S->v[i] = vS[i];
-    S->a[i] = aS[i];
-    S->b[i] = bS[i];
-  }
-  P->m00 = sP * h.cosp / g.NP;
-  P->m01 = sP * h.sinpr / g.NP;
-  P->m10 = sP * h.rsinp / g.NP;
-  P->m11 = P->m00;
-  S->m00 = sS * h.cossv / g.NSV;
-  S->m01 = sS * h.rsinsv / g.NSV;
-  S->m10 = sS * h.sinsvr / g.NSV;
-  S->m11 = S->m00;
+  P->w[0] = g.x11;
+  P->w[1] = g.x21;
+  P->w[2] = g.x31;
+  P->w[3] = g.x41;
+  P->z[0] = g.x41;
+  P->z[1] = -g.x31;
+  P->z[2] = -g.x21;
+  P->z[3] = g.x11;
+  S->w[0] = g.x12;
+  S->w[1] = g.x22;
+  S->w[2] = g.x32;
+  S->w[3] = g.x42;
+  S->z[0] = g.x42;
+  S->z[1] = -g.x32;
+  S->z[2] = -g.x22;
+  S->z[3] = g.x12;
+  const double iNP = sP / g.NP, iNS = sS / g.NSV;
+  P->m00 = h.cosp * iNP;
+  P->m01 = h.sinpr * iNP;
+  P->m10 = h.rsinp * iNP;
+  S->m00 = h.cossv * iNS;
+  S->m01 = h.rsinsv * iNS;
+  S->m10 = h.sinsvr * iNS;
 }
 
-DK_HD cx wave_entry(const Wave& w, int i, int j) {
-  return w.u[i] * (w.m00 * w.a[j] + w.m01 * w.b[j]) + w.v[i] * (w.m10 * w.a[j] + w.m11 * w.b[j]);
+DK_HD double wave_entry(const Wave& w, int i, int j) {
+  const double m = ((i & 1) == 0) ? (((j & 1) == 0) ? w.m00 : w.m01)
+                                  : (((j & 1) == 0) ? w.m10 : w.m00);
+  return w.w[i] * m * w.z[j];
 }
 
 // Real layer propagator (hska), P part scaled by sP and S part by sS.
 DK_HD void propagator(const Eig& g, const Hyp& h, double sP, double sS, double H[4][4]) {
   Wave P, S;
   waves(g, h, sP, sS, &P, &S);
+#pragma unroll
   for (int i = 0; i < 4; i++)
-    for (int j = 0; j < 4; j++) H[i][j] = (wave_entry(P, i, j) + wave_entry(S, i, j)).r;
+#pragma unroll
+    for (int j = 0; j < 4; j++) H[i][j] = wave_entry(P, i, j) + wave_entry(S, i, j);
 }
 
 // Minor index pairs in the order (12),(13),(14),(23),(24),(34); the reduced
@@ -260,45 +294,63 @@ __host__ __device__ constexpr int PI1(int k) { return k == 0 ? 1 : k == 1 ? 2 : 
 // reduced index -> pair index
 __host__ __device__ constexpr int RED(int r) { return r < 3 ? r : r + 1; }
 
-// Reduced 5x5 compound of the layer propagator, scaled by e^{-(pex+svex)} (dnka_tregn).
-DK_HD void compound(const Eig& g, const Hyp& h, cx CA[5][5]) {
-  double ex = h.pex + h.svex;
-  double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
+// e <- e * CA with CA the reduced 5x5 compound of the layer propagator scaled
+// by e^{-(pex+svex)} (dnka_tregn): pure-wave compounds
+// dfac (u^v~)(a~^b~)^T / N^2 plus the mixed terms of H_P and H_S; row (14)
+// absorbs (23) = -(14).  CA is formed column by column and never stored.
+DK_HD void compound_step(const Eig& g, const Hyp& h, double e[5]) {
+  const double ex = h.pex + h.svex;
+  const double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
   Wave P, S;
   waves(g, h, 1.0, 1.0, &P, &S);
-  cx HP[4][4], HS[4][4];
+  double HP[4][4], HS[4][4];
+#pragma unroll
   for (int i = 0; i < 4; i++)
+#pragma unroll
     for (int j = 0; j < 4; j++) {
       HP[i][j] = wave_entry(P, i, j);
       HS[i][j] = wave_entry(S, i, j);
     }
-  // pure-wave compounds (u^v~)(a~^b~)/N^2, unscaled, times dfac
-  cx iNP2 = mk(1.0) / (g.NP * g.NP), iNS2 = mk(1.0) / (g.NSV * g.NSV);
-  cx T6[6][6];
+  // (u^v~) and (a~^b~) of each wave: u, v~ = w on even / odd rows, a~, b~ = z
+  // on even / odd columns
+  const double iNP2 = dfac / (g.NP * g.NP), iNS2 = dfac / (g.NSV * g.NSV);
+  double uvP[6], uvS[6], abP[6], abS[6];
+#pragma unroll
   for (int r = 0; r < 6; r++) {
-    int i = PI0(r), j = PI1(r);
-    cx uvP = P.u[i] * P.v[j] - P.u[j] * P.v[i];
-    cx uvS = S.u[i] * S.v[j] - S.u[j] * S.v[i];
-    for (int s = 0; s < 6; s++) {
-      int k = PI0(s), l = PI1(s);
-      cx abP = P.a[k] * P.b[l] - P.a[l] * P.b[k];
-      cx abS = S.a[k] * S.b[l] - S.a[l] * S.b[k];
-      cx pure = dfac * (uvP * abP * iNP2 + uvS * abS * iNS2);
-      cx mixed = HP[i][k] * HS[j][l] - HP[i][l] * HS[j][k] + HS[i][k] * HP[j][l] -
-                 HS[i][l] * HP[j][k];
-      T6[r][s] = pure + mixed;
-    }
+    const int i = PI0(r), j = PI1(r);
+    auto wedge = [&](const double* w, int i, int j) {
+      // u = w on even indices, v~ = w on odd indices
+      const double ui = (i & 1) ? 0.0 : w[i], uj = (j & 1) ? 0.0 : w[j];
+      const double vi = (i & 1) ? w[i] : 0.0, vj = (j & 1) ? w[j] : 0.0;
+      return ui * vj - uj * vi;
+    };
+    uvP[r] = wedge(P.w, i, j) * iNP2;
+    uvS[r] = wedge(S.w, i, j) * iNS2;
+    abP[r] = wedge(P.z, i, j);
+    abS[r] = wedge(S.z, i, j);
   }
-  for (int r = 0; r < 5; r++)
-    for (int s = 0; s < 5; s++) {
-      cx v = T6[RED(r)][RED(s)];
-      if (r == 2) v = v - T6[3][RED(s)];  // row (14) absorbs (23) = -(14)
-      CA[r][s] = v;
+  double out[5];
+#pragma unroll
+  for (int sc = 0; sc < 5; sc++) {
+    const int s = RED(sc), k = PI0(s), l = PI1(s);
+    double acc = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < 6; rr++) {
+      const int i = PI0(rr), j = PI1(rr);
+      const double v = uvP[rr] * abP[s] + uvS[rr] * abS[s] + HP[i][k] * HS[j][l] -
+                       HP[i][l] * HS[j][k] + HS[i][k] * HP[j][l] - HS[i][l] * HP[j][k];
+      // reduced rows: 0,1,2 -> pairs 0,1,2; 3,4 -> pairs 4,5; row 2 minus pair 3
+      const double coef = (rr == 3) ? -e[2] : (rr < 3 ? e[rr] : e[rr - 1]);
+      acc = fma(coef, v, acc);
     }
+    out[sc] = acc;
+  }
+#pragma unroll
+  for (int q = 0; q < 5; q++) e[q] = out[q];
 }
 
 // Half-space radiation constraint: minors of rows P+ and S+ of G (evalg gbr).
-DK_HD void halfspace_constraint(const Eig& g, cx cd[5]) {
+DK_HD void halfspace_constraint(const Eig& g, double cd[5]) {
   cx E[4][4], G[4][4];
   eig_matrices(g, E, G);
   cx CG[6];
@@ -306,16 +358,17 @@ DK_HD void halfspace_constraint(const Eig& g, cx cd[5]) {
     int i = PI0(r), j = PI1(r);
     CG[r] = G[0][i] * G[1][j] - G[0][j] * G[1][i];
   }
-  cd[0] = mk(CG[0].r);
-  cd[1] = mk(CG[1].r);
-  cd[2] = mk(CG[2].r);
-  cd[3] = mk(CG[4].r);
-  cd[4] = mk(CG[5].r);
+  cd[0] = CG[0].r;
+  cd[1] = CG[1].r;
+  cd[2] = CG[2].r;
+  cd[3] = CG[4].r;
+  cd[4] = CG[5].r;
 }
 
 // ffunc, gfunc, h1func, h2func (tregn96.f:1385-1473)
+DK_HD double cabs2_(cx a) { return a.r * a.r + a.i * a.i; }
 DK_HD cx ffunc(cx nub, double dm) {
-  if (cabs_(nub) < 1.0e-08) return mk(dm);
+  if (cabs2_(nub) < 1.0e-16) return mk(dm);
   cx arg = nub * dm;
   cx ex = (arg.r < 40.0) ? cexp_(-2.0 * arg) : mk(0.0);
   return (mk(1.0) - ex) / (2.0 * nub);
@@ -325,13 +378,13 @@ DK_HD cx gfunc(cx nub, double dm) {
   return (arg.r < 75.0) ? cexp_(-arg) * dm : mk(0.0);
 }
 DK_HD cx h1func(cx nua, cx nub, double dm) {
-  if (cabs_(nub + nua) < 1.0e-08) return mk(dm);
+  if (cabs2_(nub + nua) < 1.0e-16) return mk(dm);
   cx arg = (nua + nub) * dm;
   cx ex = (arg.r < 40.0) ? cexp_(-arg) : mk(0.0);
   return (mk(1.0) - ex) / (nub + nua);
 }
 DK_HD cx h2func(cx nua, cx nub, double dm) {
-  if (cabs_(nub - nua) < 1.0e-08) return mk(dm);
+  if (cabs2_(nub - nua) < 1.0e-16) return mk(dm);
   cx ap = nua * dm, aq = nub * dm;
   cx exp_ = (ap.r < 40.0) ? cexp_(-ap) : mk(0.0);
   cx exq = (aq.r < 40.0) ? cexp_(-aq) : mk(0.0);
@@ -345,7 +398,7 @@ DK_HD void layer_integrals(const Eig& g, double dm, const double yt[4], const do
                            int half, double out[6]) {
   cx E[4][4], G[4][4];
   eig_matrices(g, E, G);
-  cx ra = g.rp, rb = g.rsv;
+  cx ra = nu_P(g), rb = nu_S(g);
   cx k[4];  // kmpu, kmsu (from yb), km1pd, km1sd (from yt)
   for (int r = 0; r < 4; r++) {
     const double* y = (r < 2) ? yb : yt;
@@ -353,44 +406,54 @@ DK_HD void layer_integrals(const Eig& g, double dm, const double yt[4], const do
   }
   const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
   if (half) {
+    const cx k22 = k[2] * k[2] / (2.0 * ra), k23 = k[2] * k[3] / (ra + rb),
+             k33 = k[3] * k[3] / (2.0 * rb);
     for (int t = 0; t < 6; t++) {
       int i = I[t], j = J[t];
-      cx v = E[i][2] * E[j][2] * k[2] * k[2] / (2.0 * ra) +
-             (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k[2] * k[3] / (ra + rb) +
-             E[i][3] * E[j][3] * k[3] * k[3] / (2.0 * rb);
+      cx v = E[i][2] * E[j][2] * k22 + (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k23 +
+             E[i][3] * E[j][3] * k33;
       out[t] = v.r;
     }
     return;
   }
-  cx FA = ffunc(ra, dm), GA = gfunc(ra, dm), FB = ffunc(rb, dm), GB = gfunc(rb, dm);
-  cx H1 = h1func(ra, rb, dm), H2 = h2func(ra, rb, dm);
+  const cx FA = ffunc(ra, dm), GA = gfunc(ra, dm), FB = ffunc(rb, dm), GB = gfunc(rb, dm);
+  const cx H1 = h1func(ra, rb, dm), H2 = h2func(ra, rb, dm);
+  const cx k00 = k[0] * k[0] * FA, k22 = k[2] * k[2] * FA, k11 = k[1] * k[1] * FB,
+           k33 = k[3] * k[3] * FB, k01 = H1 * (k[0] * k[1]), k23 = H1 * (k[2] * k[3]),
+           k03 = H2 * (k[0] * k[3]), k21 = H2 * (k[2] * k[1]), k02 = GA * (k[0] * k[2]),
+           k13 = GB * (k[1] * k[3]);
   for (int t = 0; t < 6; t++) {
     int i = I[t], j = J[t];
-    cx v = E[i][0] * E[j][0] * k[0] * k[0] * FA + E[i][2] * E[j][2] * k[2] * k[2] * FA +
-           E[i][1] * E[j][1] * k[1] * k[1] * FB + E[i][3] * E[j][3] * k[3] * k[3] * FB +
-           H1 * ((E[i][0] * E[j][1] + E[i][1] * E[j][0]) * k[0] * k[1] +
-                 (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k[2] * k[3]) +
-           H2 * ((E[i][0] * E[j][3] + E[i][3] * E[j][0]) * k[0] * k[3] +
-                 (E[i][1] * E[j][2] + E[i][2] * E[j][1]) * k[2] * k[1]) +
-           GA * (E[i][0] * E[j][2] + E[i][2] * E[j][0]) * k[0] * k[2] +
-           GB * (E[i][1] * E[j][3] + E[i][3] * E[j][1]) * k[1] * k[3];
+    cx v = E[i][0] * E[j][0] * k00 + E[i][2] * E[j][2] * k22 + E[i][1] * E[j][1] * k11 +
+           E[i][3] * E[j][3] * k33 + (E[i][0] * E[j][1] + E[i][1] * E[j][0]) * k01 +
+           (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k23 +
+           (E[i][0] * E[j][3] + E[i][3] * E[j][0]) * k03 +
+           (E[i][1] * E[j][2] + E[i][2] * E[j][1]) * k21 +
+           (E[i][0] * E[j][2] + E[i][2] * E[j][0]) * k02 +
+           (E[i][1] * E[j][3] + E[i][3] * E[j][1]) * k13;
     out[t] = v.r;
   }
 }
 
-DK_HD double cnorm5(cx e[5]) {
+// Exact power-of-two normalisations; they return the natural log of the
+// factor divided out, as cnormc / rnormc do for their max-norm factor.
+DK_HD double norm5(double e[5]) {
   double t = 0.0;
-  for (int i = 0; i < 5; i++) t = fmax(t, cabs_(e[i]));
-  if (t < 1.0e-40) t = 1.0;
-  for (int i = 0; i < 5; i++) e[i] = e[i] / t;
-  return log(t);
+  for (int i = 0; i < 5; i++) t = fmax(t, fabs(e[i]));
+  if (!(t >= 1.0e-40) || !isfinite(t)) return 0.0;
+  int k;
+  frexp(t, &k);
+  for (int i = 0; i < 5; i++) e[i] = ldexp(e[i], -k);
+  return (double)k * 0.69314718055994530942;
 }
-DK_HD double rnorm4(double e[4]) {
+DK_HD double norm4(double e[4]) {
   double t = 0.0;
   for (int i = 0; i < 4; i++) t = fmax(t, fabs(e[i]));
-  if (t < 1.0e-40) t = 1.0;
-  for (int i = 0; i < 4; i++) e[i] = e[i] / t;
-  return log(t);
+  if (!(t >= 1.0e-40) || !isfinite(t)) return 0.0;
+  int k;
+  frexp(t, &k);
+  for (int i = 0; i < 4; i++) e[i] = ldexp(e[i], -k);
+  return (double)k * 0.69314718055994530942;
 }
 
 // ----------------------------------------------------------------- model prep
@@ -437,59 +500,50 @@ DK_HD void flatten(int L, const float* thk, const float* TA, const float* TL, co
 }
 
 // ----------------------------------------------------------------- one (column, period)
-// Scratch: per layer 10 doubles of up-sweep compound vector + 1 exponent, 4 of
-// interface eigenfunction and 4 of unnormalised kernel factors; element
-// (m, q) lives at scratch[(m * 19 + q) * stride].
+// Scratch per layer and thread (element (m, q) at scratch[(m * 14 + q) * stride]):
+// the up-sweep compound vector (5) and its accumulated log scale (1), the
+// interface eigenfunction (4) and the unnormalised kernel factors (4).
+enum { Q_CD = 0, Q_EX = 5, Q_Y = 6, Q_F = 10, Q_N = 14 };
 // emit(m, dcdah, dcdbv, dcdn) receives, top layer first, tregn96's kernels after
 // the sphericity back-mapping, rounded to float as chksiz does.  lay(m) returns
-// the flattened Layer, vtpf(m) the float sphericity factor.
+// the flattened Layer, vtpf(m) the float sphericity factor.  Returns 0, or -1
+// when the surface minor vanishes or a layer's eigenvalues form a complex pair.
 template <class LayerFn, class VtpFn, class EmitFn>
 DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* scratch,
                 long stride, EmitFn emit) {
-  auto S = [&](int m, int q) -> double& { return scratch[(long)(m * 19 + q) * stride]; };
+  auto S = [&](int m, int q) -> double& { return scratch[(long)(m * Q_N + q) * stride]; };
   const double wvno = omega / c, wvno2 = wvno * wvno, om2 = omega * omega;
   Eig g;
   Hyp h;
+  int ok = 1;
   // ---- up: compound constraint vectors
-  cx cd[5];
+  double cd[5];
   eigen(lay(L - 1), omega, wvno, &g);
+  ok &= g.ok;
   halfspace_constraint(g, cd);
-  for (int q = 0; q < 5; q++) {
-    S(L - 1, 2 * q) = cd[q].r;
-    S(L - 1, 2 * q + 1) = cd[q].i;
-  }
-  S(L - 1, 10) = 0.0;
+  for (int q = 0; q < 5; q++) S(L - 1, Q_CD + q) = cd[q];
+  S(L - 1, Q_EX) = 0.0;
   double exsum = 0.0;
   for (int m = L - 2; m >= 0; m--) {
     const Layer ly = lay(m);
     eigen(ly, omega, wvno, &g);
+    ok &= g.ok;
     hyper(g, ly.d, &h);
-    cx CA[5][5];
-    compound(g, h, CA);
-    cx ee[5];
-    for (int i = 0; i < 5; i++) {
-      cx cr = mk(0.0);
-      for (int j = 0; j < 5; j++) cr = cr + cd[j] * CA[j][i];
-      ee[i] = cr;
-    }
-    double exn = cnorm5(ee);
+    compound_step(g, h, cd);
+    const double exn = norm5(cd);
     exsum = exsum + h.pex + h.svex + exn;
-    for (int q = 0; q < 5; q++) {
-      cd[q] = ee[q];
-      S(m, 2 * q) = ee[q].r;
-      S(m, 2 * q + 1) = ee[q].i;
-    }
-    S(m, 10) = exsum;
+    for (int q = 0; q < 5; q++) S(m, Q_CD + q) = cd[q];
+    S(m, Q_EX) = exsum;
   }
+  if (!ok) return -1;
   // ---- surface eigenfunction
-  const double exe1 = S(0, 10);
-  cx c0_2 = mk(S(0, 2), S(0, 3)), c0_3 = mk(S(0, 4), S(0, 5));
-  const double f1213 = -S(0, 2);
+  const double exe1 = S(0, Q_EX);
+  const double f1213 = -S(0, Q_CD + 1);
   if (!(f1213 != 0.0) || !isfinite(f1213)) return -1;
-  S(0, 11) = (c0_3 / c0_2).r;
-  S(0, 12) = 1.0;
-  S(0, 13) = 0.0;
-  S(0, 14) = 0.0;
+  S(0, Q_Y + 0) = S(0, Q_CD + 2) / S(0, Q_CD + 1);
+  S(0, Q_Y + 1) = 1.0;
+  S(0, Q_Y + 2) = 0.0;
+  S(0, Q_Y + 3) = 0.0;
   // ---- down sweep, interface eigenfunctions and layer integrals
   double vv[4] = {1.0, 0.0, 0.0, 0.0};
   double exd = 0.0;
@@ -512,19 +566,19 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       double a0[4];
       for (int i = 0; i < 4; i++)
         a0[i] = AA[i][0] * vv[0] + AA[i][1] * vv[1] + AA[i][2] * vv[2] + AA[i][3] * vv[3];
-      double ex2 = rnorm4(a0);
+      double ex2 = norm4(a0);
       exd = exd + cpex + ex2;
       for (int i = 0; i < 4; i++) vv[i] = a0[i];
       // contraction at interface m+1 (svfunc)
       const int n = m + 1;
-      double cd1 = S(n, 0), cd2 = S(n, 2), cd3 = S(n, 4), cd4 = -S(n, 4), cd5 = S(n, 6),
-             cd6 = S(n, 8);
+      double cd1 = S(n, Q_CD + 0), cd2 = S(n, Q_CD + 1), cd3 = S(n, Q_CD + 2), cd4 = -S(n, Q_CD + 2),
+             cd5 = S(n, Q_CD + 3), cd6 = S(n, Q_CD + 4);
       double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
       double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
       double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
       double uu3 = tz1 * cd5 - tz2 * cd3 + tz4 * cd1;
       double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
-      double ext = exd + S(n, 10) - exe1;
+      double ext = exd + S(n, Q_EX) - exe1;
       double y[4] = {0, 0, 0, 0};
       if (ext > -80.0 && ext < 80.0) {
         double fact = exp(ext);
@@ -533,12 +587,12 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
         y[2] = uu3 * fact / f1213;
         y[3] = uu4 * fact / f1213;
       }
-      for (int i = 0; i < 4; i++) S(n, 11 + i) = y[i];
+      for (int i = 0; i < 4; i++) S(n, Q_Y + i) = y[i];
     }
     double yt[4], yb[4];
     for (int i = 0; i < 4; i++) {
-      yt[i] = S(m, 11 + i);
-      yb[i] = (m < L - 1) ? S(m + 1, 11 + i) : 0.0;
+      yt[i] = S(m, Q_Y + i);
+      yb[i] = (m < L - 1) ? S(m + 1, Q_Y + i) : 0.0;
     }
     double I[6];
     layer_integrals(g, ly.d, yt, yb, m == L - 1, I);
@@ -558,10 +612,10 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     sumi1 = sumi1 + ly.L * UZUZ + ly.A * URUR;
     sumi2 = sumi2 + ly.L * UZDUR - ly.F * URDUZ;
     sumi3 = sumi3 + ly.L * DURDUR + ly.C * DUZDUZ;
-    S(m, 15) = rho * ah * (URUR - 2.0 * eta * URDUZ / wvno);                          // facah
-    S(m, 16) = rho * av * DUZDUZ / wvno2;                                             // facav
-    S(m, 17) = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
-    S(m, 18) = -ly.F * URDUZ / (wvno * eta);                                          // facn
+    S(m, Q_F + 0) = rho * ah * (URUR - 2.0 * eta * URDUZ / wvno);                          // facah
+    S(m, Q_F + 1) = rho * av * DUZDUZ / wvno2;                                             // facav
+    S(m, Q_F + 2) = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
+    S(m, Q_F + 3) = -ly.F * URDUZ / (wvno * eta);                                          // facn
   }
   const double ugr = (wvno * sumi1 + sumi2) / (omega * sumi0);
   const double nrm = ugr * sumi0;
@@ -574,8 +628,8 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   for (int m = 0; m < L; m++) {
     const Layer ly = lay(m);
     double ah = sqrt(ly.A / ly.rho), av = sqrt(ly.C / ly.rho), bv = sqrt(ly.L / ly.rho);
-    double xs = (S(m, 17) / nrm) * bv * qbi;
-    double xp = (S(m, 16) / nrm) * av * qai + (S(m, 15) / nrm) * ah * qai;
+    double xs = (S(m, Q_F + 2) / nrm) * bv * qbi;
+    double xp = (S(m, Q_F + 1) / nrm) * av * qai + (S(m, Q_F + 0) / nrm) * ah * qai;
     dc = dc + lw * xs + lw * xp;
   }
   const double cq = omega / wvno + dc;
@@ -586,7 +640,8 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   auto f32 = [](double x) -> float { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; };
   for (int m = 0; m < L; m++) {
     const double vt = (double)vtpf(m);
-    emit(m, f32(S(m, 15) / nrm * vt / tm3), f32(S(m, 17) / nrm * vt / tm3), f32(S(m, 18) / nrm));
+    emit(m, f32(S(m, Q_F + 0) / nrm * vt / tm3), f32(S(m, Q_F + 2) / nrm * vt / tm3),
+         f32(S(m, Q_F + 3) / nrm));
   }
   return 0;
 }

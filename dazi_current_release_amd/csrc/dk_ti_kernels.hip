@@ -15,6 +15,8 @@
 
 namespace dk {
 
+static_assert(TI_SCRATCH == ti::Q_N, "TI scratch doubles per layer");
+
 __device__ float brocher_vp_ti(float v) {
 #pragma clang fp contract(off)
   float v2 = v * v, v3 = v2 * v, v4 = v3 * v;
