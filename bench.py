@@ -110,6 +110,40 @@ def bench_vjp(dk, configs, np, torch, reps=5):
                 forward_ms=1e3 * t_fwd, vjp_ms=1e3 * t_vjp, vjp_over_forward=t_vjp / t_fwd)
 
 
+def bench_c3(dk, configs, np, torch, reps=5):
+    """ragged_16384 (16,384 layered columns, 8-64 layers, 10-40 periods, c and
+    refined-layer kernels): plain one launch per 64-column tile vs the batched
+    persistent launch with the autotuned shape, eager and replayed as a HIP
+    graph.  Times are wall-clock per call with inputs resident on the device."""
+    from dazi_current_release_amd import batched
+    W = configs.layered_workload("ragged_16384")
+    b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"], W["periods"])
+    out = b.alloc_outputs(True)
+
+    def timed(fn, n):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / n
+
+    plain_tune = dk.make_tune(16, 4, 1, 64, 2e-6)
+    t_plain = timed(lambda: dk.layered_dev(b, kernels=True, tune=plain_tune, out=out), 2)
+    tune, table = batched.autotune(lambda t: dk.layered_dev(b, kernels=True, tune=t, out=out))
+    t_batch = timed(lambda: dk.layered_dev(b, kernels=True, tune=tune, out=out), reps)
+    g = batched.GraphRunner(lambda: dk.layered_dev(b, kernels=True, tune=tune, out=out))
+    t_graph = timed(g.replay, reps)
+    ns = b.nsolves
+    return dict(workload="ragged_16384", columns=b.ncol, solves=ns,
+                plain_solves_per_s=ns / t_plain, plain_launches=int(np.ceil(b.ncol / 64)),
+                batched_solves_per_s=ns / t_batch, graph_solves_per_s=ns / t_graph,
+                batched_over_plain=t_plain / t_batch, graph_over_plain=t_plain / t_graph,
+                tuned=dict(lanes_per_scan=tune.lanes_per_scan, cols_per_wg=tune.cols_per_wg),
+                autotune_ms={f"{l}x{c}": round(ms, 4) for l, c, ms in table})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +155,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -247,6 +282,9 @@ def main():
     vjp = None
     if not args.no_vjp:
         vjp = bench_vjp(dk, configs, np, torch)
+    c3 = None
+    if not args.no_c3:
+        c3 = bench_c3(dk, configs, np, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -276,7 +314,7 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp),
+                    vjp=vjp, ragged=c3),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 

@@ -21,7 +21,7 @@ EXPORTS = [
     "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
-    "dazi_dk_layered_vjp_dev",
+    "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry",
 ]
 
 
@@ -50,7 +50,7 @@ class Layered(C.Structure):
                 ("periods", C.c_void_p), ("c", C.c_void_p), ("status", C.c_void_p),
                 ("dc_dvs", C.c_void_p), ("dc_dvp", C.c_void_p), ("dc_drho", C.c_void_p),
                 ("c_unrounded", C.c_void_p), ("thk64", C.c_void_p), ("vp64", C.c_void_p),
-                ("vs64", C.c_void_p), ("rho64", C.c_void_p)]
+                ("vs64", C.c_void_p), ("rho64", C.c_void_p), ("nlayers_total", C.c_int64)]
 
 
 _lib = None
@@ -86,6 +86,7 @@ def lib():
         L.dazi_dk_last_timings.argtypes = [C.POINTER(C.c_double), C.c_int32]
         L.dazi_dk_last_timings.restype = C.c_int32
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]
+        L.dazi_dk_reset_geometry.argtypes = []
         for name in EXPORTS:
             if not hasattr(L, name):
                 raise RuntimeError(f"{LIB_PATH} does not export {name}")

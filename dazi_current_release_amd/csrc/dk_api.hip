@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/dazi_dk.h"
@@ -73,6 +74,14 @@ struct Pool {
   hipEvent_t ev[8] = {};
   int nforward = 0;
   bool have_events = false;
+  // node geometry of the last call that took depz as a device pointer: the
+  // sublayer counts per node interval, resident in the S_NSUB slot
+  const float* geo_depz = nullptr;
+  int geo_nz = 0;
+  float geo_minthk = 0.0f;
+  int geo_L = 0;
+  bool geo_valid = false;
+  std::vector<void*> retired;  // buffers replaced by growth (see pool_get)
 };
 
 Pool g_pool[16];
@@ -94,14 +103,19 @@ int get_pool(Pool** out) {
   return 0;
 }
 
+// A slot only grows (by at least half its size).  The buffer it replaces may
+// still be read by work in flight or by a captured HIP graph, so it is retired
+// rather than freed: retired buffers live until the process ends, at most about
+// twice the final size of each slot.
 int pool_get(Pool* P, Slot s, size_t bytes, void** out) {
   if (bytes == 0) bytes = 8;
   if (P->n[s] < bytes) {
-    if (P->p[s]) HIPCHK(hipFree(P->p[s]));
-    P->p[s] = nullptr;
-    P->n[s] = 0;
-    HIPCHK(hipMalloc(&P->p[s], bytes));
-    P->n[s] = bytes;
+    size_t nb = std::max(bytes, P->n[s] + P->n[s] / 2);
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, nb));
+    if (P->p[s]) P->retired.push_back(P->p[s]);
+    P->p[s] = q;
+    P->n[s] = nb;
   }
   *out = P->p[s];
   return 0;
@@ -193,11 +207,49 @@ int ti_threads(Pool* P, int64_t nsolves) {
   return (int)n;
 }
 
+// Refined geometry (L and the per-interval sublayer counts on the device).  With
+// depz_host NULL, depz_dev is read back once per (pointer, nz, minthk) and the
+// result is reused by later calls, which then enqueue kernels only (no host
+// synchronisation: such calls can be captured in a HIP graph).
+int geometry(Pool* P, const dazi_dk_desc* d, const float* depz_dev, const float* depz_host,
+             hipStream_t st, int* L, void** nsubd) {
+  if (!depz_host && P->geo_valid && P->geo_depz == depz_dev && P->geo_nz == d->nz &&
+      P->geo_minthk == d->minthk0) {
+    *L = P->geo_L;
+    return pool_get(P, S_NSUB, (size_t)(d->nz - 1) * 4, nsubd);
+  }
+  std::vector<float> dz;
+  const bool from_dev = depz_host == nullptr;
+  if (from_dev) {
+    dz.resize(d->nz);
+    HIPCHK(hipMemcpyAsync(dz.data(), depz_dev, sizeof(float) * d->nz, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    depz_host = dz.data();
+  }
+  std::vector<int> nsub;
+  int Lc = node_geometry(d->nz, depz_host, d->minthk0, &nsub);
+  if (Lc < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
+  if (Lc > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", Lc);
+  P->geo_valid = false;
+  if (int rc = pool_get(P, S_NSUB, nsub.size() * 4, nsubd)) return rc;
+  HIPCHK(hipMemcpyAsync(*nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
+  if (from_dev) {
+    P->geo_depz = depz_dev;
+    P->geo_nz = d->nz;
+    P->geo_minthk = d->minthk0;
+    P->geo_L = Lc;
+    P->geo_valid = true;
+  }
+  *L = Lc;
+  return 0;
+}
+
 // The whole stage on device pointers: root search, optional isotropic node
 // kernels (svs/svp/srho) and optional TI splice (lsen).
 int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
                const float* vp, const float* rho, const float* depz_dev,
-               const std::vector<float>& depz_host, const double* tRc, double* pv, double* svs,
+               const float* depz_host, const double* tRc, double* pv, double* svs,
                double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st) {
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
@@ -214,13 +266,12 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     return set_err(DAZI_DK_EINVAL, "sen_vs, sen_vp, sen_rho must be all set or all NULL");
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  std::vector<int> nsub;
-  int L = node_geometry(d->nz, depz_host.data(), d->minthk0, &nsub);
-  if (L < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
-  if (L > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", L);
+  int L;
+  void* nsubd;
+  if (int rc = geometry(P, d, depz_dev, depz_host, st, &L, &nsubd)) return rc;
   int ncol = d->nx * d->ny;
   int64_t nsolves = (int64_t)ncol * d->kmax;
-  void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work, *nsubd;
+  void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work;
   if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 32, &layers)) return rc;
   if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 8, &btp)) return rc;
   if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
@@ -233,9 +284,6 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     return rc;
   }
   if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
-  if (int rc = pool_get(P, S_NSUB, nsub.size() * 4, &nsubd)) return rc;
-  HIPCHK(hipMemcpyAsync(nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
 
   P->have_events = g_timing;
   rec(P, 0, st);
@@ -342,6 +390,13 @@ int32_t dazi_dk_last_timings(double* out, int32_t n) {
   return m;
 }
 
+int32_t dazi_dk_reset_geometry(void) {
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  P->geo_valid = false;
+  return 0;
+}
+
 int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
                         const float* vp, const float* rho, const float* depz, const double* tRc,
                         double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
@@ -349,11 +404,7 @@ int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, cons
   if (!desc || !vel || !depz || !tRc || !pvRc)
     return set_err(DAZI_DK_EINVAL, "NULL required argument");
   hipStream_t st = (hipStream_t)stream;
-  // the refined geometry is needed on the host to size the batch
-  std::vector<float> dz(desc->nz > 0 ? desc->nz : 1);
-  HIPCHK(hipMemcpyAsync(dz.data(), depz, sizeof(float) * desc->nz, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  return stage_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc,
+  return stage_impl(desc, tune, vel, vp, rho, depz, nullptr, tRc, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc,
                     nullptr, status, st);
 }
 
@@ -364,10 +415,7 @@ static int dev_entry(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const f
   if (!desc || !vel || !depz || !tRc || !pvRc)
     return set_err(DAZI_DK_EINVAL, "NULL required argument");
   hipStream_t st = (hipStream_t)stream;
-  std::vector<float> dz(desc->nz > 0 ? desc->nz : 1);
-  HIPCHK(hipMemcpyAsync(dz.data(), depz, sizeof(float) * desc->nz, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  return stage_impl(desc, tune, vel, vp, rho, depz, dz, tRc, pvRc, svs, svp, srho, lsen, status,
+  return stage_impl(desc, tune, vel, vp, rho, depz, nullptr, tRc, pvRc, svs, svp, srho, lsen, status,
                     st);
 }
 
@@ -408,8 +456,7 @@ int32_t dazi_dk_ti(const dazi_dk_desc* d, const float* vel, const float* depz, c
   HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
-  std::vector<float> dz(depz, depz + nz);
-  int rc = stage_impl(d, nullptr, (const float*)dvel, nullptr, nullptr, (const float*)ddepz, dz,
+  int rc = stage_impl(d, nullptr, (const float*)dvel, nullptr, nullptr, (const float*)ddepz, depz,
                       (const double*)dt, (double*)dpv, nullptr, nullptr, nullptr, (float*)dls,
                       nullptr, nullptr);
   if (rc) return rc;
@@ -454,9 +501,8 @@ int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, co
   }
   HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
-  std::vector<float> dz(depz, depz + nz);
   int rc = stage_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
-                      (const float*)ddepz, dz, (const double*)dt, (double*)dpv, (double*)ds1,
+                      (const float*)ddepz, depz, (const double*)dt, (double*)dpv, (double*)ds1,
                       (double*)ds2, (double*)ds3, nullptr, nullptr, st);
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(st));
@@ -485,14 +531,16 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   hipStream_t st = (hipStream_t)stream;
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  int64_t nlay_total = 0;
-  // total layers: loff[ncol-1] + nl[ncol-1] read back from the device
-  int32_t nl_last = 0;
-  int64_t lo_last = 0;
-  HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  nlay_total = lo_last + nl_last;
+  int64_t nlay_total = b->nlayers_total;
+  if (nlay_total <= 0) {
+    // total layers: loff[ncol-1] + nl[ncol-1] read back from the device
+    int32_t nl_last = 0;
+    int64_t lo_last = 0;
+    HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    nlay_total = lo_last + nl_last;
+  }
   void *layers, *btp, *cc, *betmx, *bad, *croot, *work;
   if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
   if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;
@@ -548,6 +596,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
   }
   rec(P, 3, st);
+  rec(P, 4, st);  // no TI phase
   return 0;
 }
 
@@ -564,12 +613,15 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   hipStream_t st = (hipStream_t)stream;
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  int32_t nl_last = 0;
-  int64_t lo_last = 0;
-  HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
-  int64_t nlay_total = lo_last + nl_last;
+  int64_t nlay_total = b->nlayers_total;
+  if (nlay_total <= 0) {
+    int32_t nl_last = 0;
+    int64_t lo_last = 0;
+    HIPCHK(hipMemcpyAsync(&nl_last, b->nl + (b->ncol - 1), 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&lo_last, b->loff + (b->ncol - 1), 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    nlay_total = lo_last + nl_last;
+  }
   void *layers, *btp, *cc, *betmx, *bad, *tape;
   if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
   if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;

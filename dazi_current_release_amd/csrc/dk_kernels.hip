@@ -348,8 +348,17 @@ __device__ __forceinline__ double secular_fwd(double c, double omega, int L, con
   double e[5];
   halfspace_fwd(wvno2, om2, iom2, mdl[2 * (L - 1)], mdl[2 * (L - 1) + 1], e);
   int E = pow2_normalize(e);
-  for (int m = L - 2; m >= 0; m--) {
+  // two layers per step: the second layer's matrix is independent of the
+  // first layer's update, and one normalisation per pair keeps the vector far
+  // inside the double range (a layer scales it by at most ~1e4)
+  int m = L - 2;
+  for (; m >= 1; m -= 2) {
     layer_step_fwd(wvno2, om2, iom2, mdl[2 * m], mdl[2 * m + 1], e);
+    layer_step_fwd(wvno2, om2, iom2, mdl[2 * m - 2], mdl[2 * m - 1], e);
+    E += pow2_normalize(e);
+  }
+  if (m == 0) {
+    layer_step_fwd(wvno2, om2, iom2, mdl[0], mdl[1], e);
     E += pow2_normalize(e);
   }
   *ex = E;

@@ -209,6 +209,7 @@ def _layered_struct(batch, vs=None, vp=None, rho=None):
     b.nk, b.toff = batch.nk.data_ptr(), batch.toff.data_ptr()
     b.solve_col = batch.solve_col.data_ptr()
     b.periods = batch.periods.data_ptr()
+    b.nlayers_total = int(batch.nl_host.sum())
     if batch.f64:
         b.thk64, b.vp64, b.vs64, b.rho64 = (batch.thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
                                             rho.data_ptr())

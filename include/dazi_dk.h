@@ -17,6 +17,10 @@
  * device pointers and enqueue on the given HIP stream.  Calls from several host
  * threads at once are not supported (the reference is not reentrant either).
  *
+ * Device work buffers belong to a per-device pool that only grows; buffers a
+ * growth replaces are kept until the process ends, so a HIP graph captured
+ * around a device entry point stays valid after later, larger calls.
+ *
  * Return codes: 0 ok; otherwise one of DAZI_DK_E* (the Fortran shim turns a
  * non-zero code into `stop <code>`), and dazi_dk_last_error() describes it.
  * A period whose root is not found is not an error: as in the reference
@@ -98,6 +102,8 @@ typedef struct dazi_dk_layered {
   const double* vp64;      /* differentiable path: the reference's own models are */
   const double* vs64;      /* float32).  Kernels and VJPs are then with respect to */
   const double* rho64;     /* these float64 values. */
+  int64_t nlayers_total;   /* sum of nl[] when > 0; 0 makes the call read it back
+                              from the device (a host synchronisation) */
 } dazi_dk_layered;
 
 int32_t dazi_dk_abi_version(void);
@@ -118,7 +124,16 @@ int32_t dazi_dk_iso(const dazi_dk_desc* desc, const float* vel, const float* vp,
                     double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc);
 
 /* Same stage on device pointers, enqueued on `stream` (a hipStream_t, NULL =
- * default stream).  status(nx*ny) i32 may be NULL.  tune may be NULL. */
+ * default stream).  status(nx*ny) i32 may be NULL.  tune may be NULL.
+ * The refined node geometry is derived from depz on the first call with a
+ * given (depz pointer, nz, minthk) and kept on the device; later calls with
+ * the same triple only enqueue kernels (no host synchronisation), so after one
+ * such call an identical call can be captured in a HIP graph.  After
+ * overwriting depz in place call dazi_dk_reset_geometry(). */
+/* Forgets the cached node geometry of the current device (no reference
+ * counterpart: the reference recomputes it on every call). */
+int32_t dazi_dk_reset_geometry(void);
+
 int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
                         const float* vp, const float* rho, const float* depz, const double* tRc,
                         double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,

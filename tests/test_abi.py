@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 def test_struct_layouts_match_header():
     assert C.sizeof(_lib.Desc) == 40
     assert C.sizeof(_lib.Tune) == 24
-    assert C.sizeof(_lib.Layered) == 8 + 8 + 8 * 20
+    assert C.sizeof(_lib.Layered) == 8 + 8 + 8 * 21
 
 
 def test_invalid_sizes_rejected_before_any_gpu_call():
