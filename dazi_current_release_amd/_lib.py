@@ -82,7 +82,7 @@ def lib():
         L.dazi_dk_ti.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 5
         L.dazi_dk_ti_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 9
         L.dazi_dk_stage_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 12
-        L.dazi_dk_layered_vjp_dev.argtypes = [C.POINTER(Layered)] + [C.c_void_p] * 5
+        L.dazi_dk_layered_vjp_dev.argtypes = [C.POINTER(Layered)] + [C.c_void_p] * 6
         L.dazi_dk_last_timings.argtypes = [C.POINTER(C.c_double), C.c_int32]
         L.dazi_dk_last_timings.restype = C.c_int32
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]

@@ -1,11 +1,13 @@
 """torch.autograd.Function over the stage's ragged layered solver.
 
 forward: Rayleigh fundamental-mode phase velocities c (float64, before the
-reference's float32 rounding) of a LayeredBatch for given layer vs, vp, rho.
-backward: the hand-written VJP kernel (implicit differentiation of the secular
-function at each root, dc/dm = -(dF/dm)/(dF/dc)) returns the gradients with
-respect to vs, vp and rho of every layer.  Periods whose root was not found
-(c = 0) contribute no gradient, as no kernel exists there.
+reference's float32 rounding) of a LayeredBatch for given layer vs, vp, rho and
+thickness.  backward: the hand-written VJP kernel (implicit differentiation of
+the secular function at each root, dc/dm = -(dF/dm)/(dF/dc)) returns the
+gradients with respect to vs, vp, rho and thickness of every layer (the
+thickness gradient carries the earth flattening of the deeper layers and is 0
+for the half-space).  Periods whose root was not found (c = 0) contribute no
+gradient, as no kernel exists there.
 """
 from __future__ import annotations
 
@@ -16,24 +18,24 @@ from . import stage
 
 class Dispersion(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, vs, vp, rho, batch, tune=None):
-        out = stage.layered_dev(batch, vs=vs.detach().contiguous(), vp=vp.detach().contiguous(),
-                                rho=rho.detach().contiguous(), tune=tune)
+    def forward(ctx, vs, vp, rho, thk, batch, tune=None):
+        m = [t.detach().contiguous() for t in (vs, vp, rho, thk)]
+        out = stage.layered_dev(batch, vs=m[0], vp=m[1], rho=m[2], thk=m[3], tune=tune)
         c = out["c_unrounded"]
         ctx.batch = batch
-        ctx.save_for_backward(vs, vp, rho, c)
+        ctx.save_for_backward(*m, c)
         ctx.mark_non_differentiable(out["status"])
         return c, out["status"]
 
     @staticmethod
     def backward(ctx, grad_c, grad_status):
-        vs, vp, rho, c = ctx.saved_tensors
-        gvs, gvp, grho = stage.layered_vjp(ctx.batch, c, grad_c, vs=vs.detach().contiguous(),
-                                           vp=vp.detach().contiguous(),
-                                           rho=rho.detach().contiguous())
-        return gvs.to(vs.dtype), gvp.to(vp.dtype), grho.to(rho.dtype), None, None
+        vs, vp, rho, thk, c = ctx.saved_tensors
+        g = stage.layered_vjp(ctx.batch, c, grad_c, vs=vs, vp=vp, rho=rho, thk=thk)
+        return (g[0].to(vs.dtype), g[1].to(vp.dtype), g[2].to(rho.dtype), g[3].to(thk.dtype),
+                None, None)
 
 
-def dispersion(vs, vp, rho, batch, tune=None):
-    """c (nsolves,) float64 and status (ncol,) for layer tensors laid out like batch.vs."""
-    return Dispersion.apply(vs, vp, rho, batch, tune)
+def dispersion(vs, vp, rho, batch, thk=None, tune=None):
+    """c (nsolves,) float64 and status (ncol,) for layer tensors laid out like batch.vs;
+    thk defaults to the batch's thicknesses."""
+    return Dispersion.apply(vs, vp, rho, batch.thk if thk is None else thk, batch, tune)

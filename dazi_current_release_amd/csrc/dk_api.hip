@@ -601,7 +601,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
 }
 
 int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, double* grad_vs,
-                                double* grad_vp, double* grad_rho, void* stream) {
+                                double* grad_vp, double* grad_rho, double* grad_thk,
+                                void* stream) {
   bool m32 = b && b->thk && b->vp && b->vs && b->rho;
   bool m64 = b && b->thk64 && b->vp64 && b->vs64 && b->rho64;
   if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !(m32 || m64) || !b->periods ||
@@ -650,7 +651,8 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   B.btp = (const double*)btp;
   B.periods = b->periods;
   B.croot = b->c_unrounded;
-  HIPCHK(dk::launch_vjp(B, grad_c, grad_vs, grad_vp, grad_rho, (double*)tape, nblocks, st));
+  HIPCHK(dk::launch_vjp(B, grad_c, grad_vs, grad_vp, grad_rho, grad_thk, m64 ? nullptr : b->thk,
+                        m64 ? b->thk64 : nullptr, (double*)tape, nblocks, st));
   rec(P, 2, st);
   rec(P, 3, st);
   rec(P, 4, st);

@@ -83,6 +83,7 @@ hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int
                           hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
-                      double* grho, double* tape, int nblocks, hipStream_t st);
+                      double* grho, double* gthk, const float* thk32, const double* thk64,
+                      double* tape, int nblocks, hipStream_t st);
 
 }  // namespace dk

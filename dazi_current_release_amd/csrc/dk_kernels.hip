@@ -1014,7 +1014,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict__ grad_c,
                                             double* __restrict__ gvs, double* __restrict__ gvp,
-                                            double* __restrict__ grho, double* __restrict__ tape) {
+                                            double* __restrict__ grho, double* __restrict__ gthk,
+                                            const float* __restrict__ thk32,
+                                            const double* __restrict__ thk64,
+                                            double* __restrict__ tape) {
   const int lane = threadIdx.x;
   const long nslot = (long)gridDim.x * 64;
   const long slot = (long)blockIdx.x * 64 + lane;
@@ -1097,25 +1100,36 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
         const double dFdc = e[0].g[0];
         lane_scale = (dFdc != 0.0) ? -gc / dFdc : 0.0;
       }
-      // ---- sweep 2: adjoint, per-layer contributions reduced over the lanes
+      // ---- sweep 2: adjoint, per-layer contributions reduced over the lanes.
+      // Partials in the flattened (a, b, rho, d) of each layer; the thickness
+      // chain goes through the layer radii r0 (top) and r1 (bottom) of the
+      // flattening (d = ar ln(r0/r1), tmp = 2 ar/(r0 + r1) scales a, b and
+      // rho^-2.275): R0, R1 = dF/dr0, dF/dr1, and a thickness change moves r1
+      // of its layer and both radii of every deeper layer.
       double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
       int Eg = 0;
+      double Z = 0.0, prevR1 = 0.0;
+      const double ar = 6370.0;
       for (int m = 0; m < L; m++) {
-        double part[3] = {0.0, 0.0, 0.0};
+        const double thk_m = (m == L - 1) ? 1.0 : (thk64 ? thk64[loff + m] : (double)thk32[loff + m]);
+        const double r0 = ar - Z, r1 = r0 - thk_m;
+        Z += thk_m;
+        double part[4] = {0.0, 0.0, 0.0, 0.0};
+        double R0 = 0.0, R1 = 0.0;
         if (act) {
           double4 l = lay[m];
           if (m < L - 1) {
-            Dual<3> ca[5][5];
-            layer_ca<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
-                              dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), (double)l.x,
-                              omega, ca);
+            Dual<4> ca[5][5];
+            layer_ca<Dual<4>, Dual<4>>(dconst<4>(wv), dconst<4>(wv2), dvar<4>((double)l.y, 0),
+                                       dvar<4>((double)l.z, 1), dvar<4>((double)l.w, 2),
+                                       dvar<4>((double)l.x, 3), omega, ca);
             const double* tp = tape + (long)(m * 6) * nslot + slot;
             double et[5];
 #pragma unroll
             for (int i = 0; i < 5; i++) et[i] = tp[(long)i * nslot];
             const int Ee = (int)tp[(long)5 * nslot];
 #pragma unroll
-            for (int p = 0; p < 3; p++) {
+            for (int p = 0; p < 4; p++) {
               double acc = 0.0;
 #pragma unroll
               for (int j = 0; j < 5; j++) {
@@ -1124,7 +1138,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
                 for (int i = 0; i < 5; i++) t = fma(ca[j][i].g[p], g[i], t);
                 acc = fma(et[j], t, acc);
               }
-              part[p] = ldexp(acc, Ee + Eg - EF);
+              part[p] = ldexp(acc, Ee + Eg - EF) * lane_scale;
             }
             double gn[5];
 #pragma unroll
@@ -1146,15 +1160,26 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
               double acc = 0.0;
 #pragma unroll
               for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
-              part[p] = ldexp(acc, Eg - EF);
+              part[p] = ldexp(acc, Eg - EF) * lane_scale;
             }
           }
+          // part = grad_c * dc/d(flattened a, b, rho, d)
           const double bt = (double)btp[m];
-          part[0] *= lane_scale * bt;
-          part[1] *= lane_scale * bt;
-          part[2] *= lane_scale * pow(bt, -2.275);
+          const double dtmp = part[0] * ((double)l.y / bt) + part[1] * ((double)l.z / bt) -
+                              2.275 * part[2] * ((double)l.w / bt);
+          const double ttm = -dtmp * bt / (r0 + r1);
+          R0 = ttm + ((m < L - 1) ? part[3] * (ar / r0) : 0.0);
+          R1 = ttm - ((m < L - 1) ? part[3] * (ar / r1) : 0.0);
+          part[0] *= bt;
+          part[1] *= bt;
+          part[2] *= pow(bt, -2.275);
         }
         const double sa = wave_sum(part[0]), sb = wave_sum(part[1]), sr = wave_sum(part[2]);
+        double s0 = 0.0, s1 = 0.0;
+        if (gthk) {
+          s0 = wave_sum(R0);
+          s1 = wave_sum(R1);
+        }
         if (lane == 0) {
           const int64_t o = loff + m;
           if (k0 == 0) {
@@ -1166,15 +1191,43 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
             gvs[o] += sb;
             grho[o] += sr;
           }
+          if (gthk) {
+            // W[m-1] = R1[m-1] + R0[m]; gthk holds W until the suffix sum below
+            if (m > 0) {
+              const double w = prevR1 + s0;
+              if (k0 == 0)
+                gthk[o - 1] = w;
+              else
+                gthk[o - 1] += w;
+            }
+            if (m == L - 1) {
+              if (k0 == 0)
+                gthk[o] = s1;
+              else
+                gthk[o] += s1;
+            }
+            prevR1 = s1;
+          }
         }
       }
+    }
+    // dc/dthk[i] = -sum_{j >= i} W[j]; the half-space thickness does not enter
+    if (gthk && lane == 0) {
+      double run = 0.0;
+      for (int m = L - 1; m >= 0; m--) {
+        run += gthk[loff + m];
+        gthk[loff + m] = -run;
+      }
+      gthk[loff + L - 1] = 0.0;
     }
   }
 }
 
 hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
-                      double* grho, double* tape, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL(k_vjp, dim3(nblocks), dim3(64), 0, st, B, grad_c, gvs, gvp, grho, tape);
+                      double* grho, double* gthk, const float* thk32, const double* thk64,
+                      double* tape, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_vjp, dim3(nblocks), dim3(64), 0, st, B, grad_c, gvs, gvp, grho, gthk, thk32,
+                     thk64, tape);
   return hipGetLastError();
 }
 

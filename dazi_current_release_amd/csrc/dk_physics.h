@@ -243,6 +243,24 @@ DK_HD double hyp3_t(const double& X, double d, double* S, double* R, double* ex)
   *ex = hyp3(X, d, &C, S, R, &dC, &dS, &dR);
   return C;
 }
+// X and d both carry derivatives: with the exponent scaling held constant,
+// dC/dd = R, dS/dd = C, dR/dd = X C.
+template <int N>
+DK_HD Dual<N> hyp3_t(const Dual<N>& X, const Dual<N>& d, Dual<N>* S, Dual<N>* R, double* ex) {
+  double C, Sv, Rv, dC, dS, dR;
+  *ex = hyp3(X.v, d.v, &C, &Sv, &Rv, &dC, &dS, &dR);
+  Dual<N> c;
+  c.v = C;
+  S->v = Sv;
+  R->v = Rv;
+#pragma unroll
+  for (int i = 0; i < N; i++) {
+    c.g[i] = X.g[i] * dC + d.g[i] * Rv;
+    S->g[i] = X.g[i] * dS + d.g[i] * C;
+    R->g[i] = X.g[i] * dR + d.g[i] * X.v * C;
+  }
+  return c;
+}
 template <int N>
 DK_HD Dual<N> hyp3_t(const Dual<N>& X, double d, Dual<N>* S, Dual<N>* R, double* ex) {
   double C, Sv, Rv, dC, dS, dR;
@@ -262,10 +280,10 @@ DK_HD Dual<N> hyp3_t(const Dual<N>& X, double d, Dual<N>* S, Dual<N>* R, double*
 
 // One layer: Dunkin's reduced compound matrix ca (surfdisp96.f:808-831 plus
 // var and dnka), scaled by exp(-exa). a,b,rho are the flattened layer values,
-// d its flattened thickness.
-template <class T>
-DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho, double d,
-                    double omega, T ca[5][5]) {
+// d its flattened thickness (a double, or a Dual<N> to differentiate in d).
+template <class T, class D = double>
+DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
+                    const D& d, double omega, T ca[5][5]) {
   T xka = omega / a;
   T xkb = omega / b;
   T t = b * (1.0 / omega);

@@ -172,16 +172,16 @@ class LayeredBatch:
 
 
 def layered_dev(batch: LayeredBatch, kernels=False, tune=None, out=None, stream=None,
-                vs=None, vp=None, rho=None):
+                vs=None, vp=None, rho=None, thk=None):
     """Root search (+ refined-layer kernels) of a ragged layered batch.
 
     Returns dict(c (nsolves,) float32-rounded, c_unrounded (nsolves,), status (ncol,),
     dc_dvs/dc_dvp/dc_drho (max_layers, nsolves)).
-    vs/vp/rho override the batch's model tensors (same layout) when given.
+    vs/vp/rho/thk override the batch's model tensors (same layout) when given.
     """
     if out is None:
         out = batch.alloc_outputs(kernels)
-    b = _layered_struct(batch, vs, vp, rho)
+    b = _layered_struct(batch, vs, vp, rho, thk)
     b.c = out["c"].data_ptr()
     b.status = out["status"].data_ptr()
     if kernels:
@@ -196,11 +196,12 @@ def layered_dev(batch: LayeredBatch, kernels=False, tune=None, out=None, stream=
     return out
 
 
-def _layered_struct(batch, vs=None, vp=None, rho=None):
+def _layered_struct(batch, vs=None, vp=None, rho=None, thk=None):
     vs = batch.vs if vs is None else vs
     vp = batch.vp if vp is None else vp
     rho = batch.rho if rho is None else rho
-    for t in (vs, vp, rho):
+    thk = batch.thk if thk is None else thk
+    for t in (vs, vp, rho, thk):
         if t.dtype != batch.thk.dtype or not t.is_contiguous() or t.numel() != batch.thk.numel():
             raise ValueError("model tensors must match the batch's dtype, size and layout")
     b = _lib.Layered()
@@ -211,31 +212,32 @@ def _layered_struct(batch, vs=None, vp=None, rho=None):
     b.periods = batch.periods.data_ptr()
     b.nlayers_total = int(batch.nl_host.sum())
     if batch.f64:
-        b.thk64, b.vp64, b.vs64, b.rho64 = (batch.thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
+        b.thk64, b.vp64, b.vs64, b.rho64 = (thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
                                             rho.data_ptr())
     else:
-        b.thk, b.vp, b.vs, b.rho = (batch.thk.data_ptr(), vp.data_ptr(), vs.data_ptr(),
-                                    rho.data_ptr())
+        b.thk, b.vp, b.vs, b.rho = (thk.data_ptr(), vp.data_ptr(), vs.data_ptr(), rho.data_ptr())
     return b
 
 
 def layered_vjp(batch: LayeredBatch, c_unrounded, grad_c, vs=None, vp=None, rho=None,
-                stream=None):
-    """grad_x[layer] = sum over the column's periods of grad_c * dc/dx, x in (vs, vp, rho),
-    by implicit differentiation at the roots c_unrounded of the same model.
-    Returns three float64 tensors laid out like the model."""
+                thk=None, stream=None):
+    """grad_x[layer] = sum over the column's periods of grad_c * dc/dx, x in (vs, vp, rho,
+    thk), by implicit differentiation at the roots c_unrounded of the same model.
+    Returns four float64 tensors laid out like the model (thickness gradient 0 for
+    the half-space)."""
     import torch
     dev = batch.thk.device
-    g = [torch.empty(batch.thk.numel(), dtype=torch.float64, device=dev) for _ in range(3)]
-    b = _layered_struct(batch, vs, vp, rho)
+    g = [torch.empty(batch.thk.numel(), dtype=torch.float64, device=dev) for _ in range(4)]
+    b = _layered_struct(batch, vs, vp, rho, thk)
     b.c_unrounded = c_unrounded.data_ptr()
     gc = grad_c.to(torch.float64).contiguous()
     rc = _lib.lib().dazi_dk_layered_vjp_dev(C.byref(b), C.c_void_p(gc.data_ptr()),
                                             C.c_void_p(g[0].data_ptr()),
                                             C.c_void_p(g[1].data_ptr()),
-                                            C.c_void_p(g[2].data_ptr()), _stream_ptr(stream))
+                                            C.c_void_p(g[2].data_ptr()),
+                                            C.c_void_p(g[3].data_ptr()), _stream_ptr(stream))
     _lib.check(rc)
-    return g[0], g[1], g[2]
+    return g[0], g[1], g[2], g[3]
 
 
 def depthkernelTI(nx, ny, nz, vel, pvRc, iwave, igr, kmaxRc, tRc, depz, minthk, Lsen_Gsc, *,

@@ -167,13 +167,17 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tu
 
 /* Vector-Jacobian product of the layered batch's phase velocities: for every
  * layer m of every column, grad_x[loff+m] = sum over the column's periods of
- * grad_c[s] * dc_s/dx_m, x in {vs, vp, rho} of the unflattened model, by
+ * grad_c[s] * dc_s/dx_m, x in {vs, vp, rho, thk} of the unflattened model, by
  * implicit differentiation of the secular function at the root
- * (dc/dx = -(dF/dx)/(dF/dc)).  batch->c_unrounded must hold the roots of a
- * dazi_dk_layered_dev call on the same batch; periods without a root
- * contribute nothing.  Deterministic (no atomics). */
+ * (dc/dx = -(dF/dx)/(dF/dc)); the thickness gradient includes the earth
+ * flattening of every deeper layer (surfdisp96.f:480-547) and is 0 for the
+ * half-space.  grad_thk may be NULL.  batch->c_unrounded must hold the roots of
+ * a dazi_dk_layered_dev call on the same batch; periods without a root
+ * contribute nothing.  Deterministic (no atomics).  No reference counterpart
+ * (the reference differentiates by finite differences, DepthKernel1D.f90:146-183). */
 int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* batch, const double* grad_c,
-                                double* grad_vs, double* grad_vp, double* grad_rho, void* stream);
+                                double* grad_vs, double* grad_vp, double* grad_rho,
+                                double* grad_thk, void* stream);
 
 /* Event timings (ms) of the last call's phases on its stream:
  * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels,

@@ -55,7 +55,7 @@ def test_vjp_equals_kernel_contraction(dk):
     out = dk.layered_dev(b, kernels=True)
     rng = np.random.default_rng(11)
     gc = rng.standard_normal(b.nsolves)
-    gvs, gvp, grho = dk.layered_vjp(b, out["c_unrounded"], torch.as_tensor(gc, device="cuda"))
+    gvs, gvp, grho, _ = dk.layered_vjp(b, out["c_unrounded"], torch.as_tensor(gc, device="cuda"))
     ref = contraction(b, out, gc)
     for got, want in zip((gvs, gvp, grho), ref):
         got = got.cpu().numpy()
@@ -75,7 +75,7 @@ def test_vjp_ragged_long_and_empty_columns(dk):
     b = dk.LayeredBatch(W["nl"], nk, W["thk"], W["vp"], W["vs"], W["rho"], per)
     out = dk.layered_dev(b, kernels=True)
     gc = rng.standard_normal(b.nsolves)
-    g = dk.layered_vjp(b, out["c_unrounded"], torch.as_tensor(gc, device="cuda"))
+    g = dk.layered_vjp(b, out["c_unrounded"], torch.as_tensor(gc, device="cuda"))[:3]
     ref = contraction(b, out, gc)
     for got, want in zip(g, ref):
         got = got.cpu().numpy()
@@ -84,12 +84,12 @@ def test_vjp_ragged_long_and_empty_columns(dk):
             assert np.all(got[b.loff_host[j]:b.loff_host[j] + b.nl_host[j]] == 0)
 
 
-def loss_of(dk, b, vs, vp, rho, w, tune):
-    out = dk.layered_dev(b, vs=vs, vp=vp, rho=rho, tune=tune)
+def loss_of(dk, b, vs, vp, rho, w, tune, thk=None):
+    out = dk.layered_dev(b, vs=vs, vp=vp, rho=rho, thk=thk, tune=tune)
     return float((out["c_unrounded"].cpu().numpy() * w).sum()), out
 
 
-@pytest.mark.parametrize("which", ["vs", "vp", "rho"])
+@pytest.mark.parametrize("which", ["vs", "vp", "rho", "thk"])
 def test_vjp_vs_central_fd_64_columns(dk, which):
     """Directional central differences with a relative model step of 1e-4 on
     64 columns, and per-entry differences on single layers."""
@@ -98,12 +98,12 @@ def test_vjp_vs_central_fd_64_columns(dk, which):
     W = configs.layered_workload("diff_4096", ncol=64)
     b = f64_batch(dk, W)
     tune = tight(dk)
-    rng = np.random.default_rng({"vs": 1, "vp": 2, "rho": 3}[which])
+    rng = np.random.default_rng({"vs": 1, "vp": 2, "rho": 3, "thk": 4}[which])
     w = rng.standard_normal(b.nsolves)
-    base = {"vs": b.vs, "vp": b.vp, "rho": b.rho}
+    base = {"vs": b.vs, "vp": b.vp, "rho": b.rho, "thk": b.thk}
     _, out0 = loss_of(dk, b, b.vs, b.vp, b.rho, w, tune)
     assert int((out0["c_unrounded"] == 0).sum()) == 0
-    g = dict(zip(("vs", "vp", "rho"),
+    g = dict(zip(("vs", "vp", "rho", "thk"),
                  dk.layered_vjp(b, out0["c_unrounded"], torch.as_tensor(w, device="cuda"))))
     grad = g[which].cpu().numpy()
     m0 = base[which]
@@ -113,19 +113,25 @@ def test_vjp_vs_central_fd_64_columns(dk, which):
         m = m0 + torch.as_tensor(delta, device="cuda")
         args = dict(base)
         args[which] = m.contiguous()
-        val, o = loss_of(dk, b, args["vs"], args["vp"], args["rho"], w, tune)
+        val, o = loss_of(dk, b, args["vs"], args["vp"], args["rho"], w, tune, thk=args["thk"])
         assert torch.equal(o["status"], out0["status"])
         return val
 
     mh = m0.cpu().numpy()
     r = rng.uniform(-1.0, 1.0, mh.size)
+    if which == "thk":
+        # the half-space thickness does not enter: zero gradient, no perturbation
+        last = b.loff_host + b.nl_host - 1
+        assert np.all(grad[last] == 0.0)
+        r[last] = 0.0
     fd = (L_at(h * mh * r) - L_at(-h * mh * r)) / (2 * h)
     an = float((grad * mh * r).sum())
     assert abs(fd - an) <= 1e-4 * abs(an), (fd, an)
 
-    # single entries: top sediment, mid crust, mantle lid, half-space
+    # single entries: top sediment, mid crust, mantle lid, half-space (deepest
+    # finite layer for thickness)
     gmax = np.abs(grad).max()
-    for j, m in ((0, 0), (5, 12), (33, 30), (60, 39)):
+    for j, m in ((0, 0), (5, 12), (33, 30), (60, 39 if which != "thk" else 38)):
         i = int(b.loff_host[j]) + m
         d = np.zeros(mh.size)
         d[i] = h * mh[i]
@@ -148,7 +154,8 @@ def test_autograd_checkerboard_misfit(dk):
     vs = b.vs.clone().requires_grad_(True)
     vp = b.vp.clone().requires_grad_(True)
     rho = b.rho.clone().requires_grad_(True)
-    c, status = dispersion(vs, vp, rho, b, tune)
+    thk = b.thk.clone().requires_grad_(True)
+    c, status = dispersion(vs, vp, rho, b, thk=thk, tune=tune)
     assert int((c == 0).sum()) == 0
     loss = 0.5 * ((c - c_obs) ** 2).sum()
     loss.backward()
@@ -156,15 +163,16 @@ def test_autograd_checkerboard_misfit(dk):
 
     rng = np.random.default_rng(7)
     h = 1e-4
-    dirs = [torch.as_tensor(rng.uniform(-1, 1, b.vs.numel()), device="cuda") for _ in range(3)]
+    dirs = [torch.as_tensor(rng.uniform(-1, 1, b.vs.numel()), device="cuda") for _ in range(4)]
 
     def L_at(s):
         with torch.no_grad():
             cc, _ = dispersion(b.vs * (1 + s * dirs[0]), b.vp * (1 + s * dirs[1]),
-                               b.rho * (1 + s * dirs[2]), b, tune)
+                               b.rho * (1 + s * dirs[2]), b, thk=b.thk * (1 + s * dirs[3]),
+                               tune=tune)
             return float(0.5 * ((cc - c_obs) ** 2).sum())
 
     fd = (L_at(h) - L_at(-h)) / (2 * h)
     an = float((vs.grad * b.vs * dirs[0]).sum() + (vp.grad * b.vp * dirs[1]).sum()
-               + (rho.grad * b.rho * dirs[2]).sum())
+               + (rho.grad * b.rho * dirs[2]).sum() + (thk.grad * b.thk * dirs[3]).sum())
     assert abs(fd - an) <= 1e-4 * abs(an), (fd, an)
