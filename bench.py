@@ -144,6 +144,52 @@ def bench_c3(dk, configs, np, torch, reps=5):
                 autotune_ms={f"{l}x{c}": round(ms, 4) for l, c, ms in table})
 
 
+def bench_c5(dk, configs, np, torch, chunk=40960):
+    """voronoi_100: 100 Voronoi realizations (9 layers x 100 Vs cells + 50 ANI cells
+    per layer, projected onto 64x64 columns x 25 nodes) = 409,600 node columns x 30
+    periods.  Per chunk of `chunk` columns one graph replay computes c, the
+    isotropic node kernels and Lsen_Gsc, grad_c = c - c_obs and the
+    effective-dVs VJP; the cell gradients are accumulated after each chunk.
+    c_obs: the stage's c on an independent realization (the same map for every
+    realization)."""
+    from dazi_current_release_amd.realizations import RealizationStep, cell_gradient
+    W = configs.voronoi_workload(100)
+    N, nz = W["vel"].shape[0], W["vel"].shape[-1]
+    vel = torch.as_tensor(np.ascontiguousarray(W["vel"][:, 0, :].T), device="cuda")
+    depz = torch.as_tensor(W["depz"], device="cuda")
+    per = torch.as_tensor(W["periods"], device="cuda")
+    T = configs.voronoi_workload(1, seed=6)
+    vt = torch.as_tensor(np.ascontiguousarray(T["vel"][:, 0, :].T), device="cuda")
+    c_obs = dk.stage_dev(vt, depz, per, W["minthk"], iso=False, ti=False)["pv"].repeat(1, 100)
+    lon = torch.as_tensor(W["layer_of_node"], device="cuda")
+    cell = torch.as_tensor(W["cell"], dtype=torch.int64, device="cuda")
+    acc = torch.zeros((100, 9, 100), dtype=torch.float64, device="cuda")
+    torch.cuda.reset_peak_memory_stats()
+    step = RealizationStep(nz, depz, per, W["minthk"], chunk)
+    misfit = torch.zeros((), dtype=torch.float64, device="cuda")
+
+    def consume(c0, n, s):
+        misfit.add_(0.5 * (s.gc[:, :n] ** 2).sum())
+        cell_gradient(s.grad["grad_vs"], lon, cell, 100, c0, n, 4096, acc)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step.run(vel, c_obs, consume)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    pool = dk.lib()
+    import ctypes
+    nb = ctypes.c_int64(0)
+    pool.dazi_dk_pool_bytes(ctypes.byref(nb))
+    solves = N * per.numel()
+    return dict(workload="voronoi_100", realizations=100, columns=N, solves=solves,
+                seconds=dt, solves_per_s=solves / dt, chunk_columns=chunk,
+                chunks=int(np.ceil(N / chunk)), outputs="c, node kernels, Lsen_Gsc, VJP",
+                torch_peak_bytes=int(torch.cuda.max_memory_allocated()),
+                library_pool_bytes=int(nb.value), misfit=float(misfit),
+                cell_grad_norm=float(acc.norm()))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +202,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -285,6 +332,9 @@ def main():
     c3 = None
     if not args.no_c3:
         c3 = bench_c3(dk, configs, np, torch)
+    c5 = None
+    if not args.no_c5:
+        c5 = bench_c5(dk, configs, np, torch)
 
     cpu = None
     if not args.no_cpu:
@@ -314,7 +364,7 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp, ragged=c3),
+                    vjp=vjp, ragged=c3, realizations=c5),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 

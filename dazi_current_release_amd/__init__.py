@@ -7,8 +7,8 @@ behind a C ABI (include/dazi_dk.h).  The Python layer mirrors the reference's
 """
 from ._lib import DaziDkError, lib, make_tune, last_timings  # noqa: F401
 from .stage import (depthkernel, run_depthkernel, iso_dev, layered_dev, LayeredBatch,  # noqa: F401
-                    depthkernelTI, run_depthkernelTI, stage_dev, layered_vjp)
+                    depthkernelTI, run_depthkernelTI, stage_dev, layered_vjp, effective_dev)
 
 __all__ = ["DaziDkError", "lib", "make_tune", "last_timings", "depthkernel", "run_depthkernel",
            "iso_dev", "layered_dev", "LayeredBatch", "depthkernelTI", "run_depthkernelTI",
-           "stage_dev", "layered_vjp"]
+           "stage_dev", "layered_vjp", "effective_dev"]

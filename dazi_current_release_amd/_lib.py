@@ -21,7 +21,8 @@ EXPORTS = [
     "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
-    "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry",
+    "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
+    "dazi_dk_pool_bytes",
 ]
 
 
@@ -87,6 +88,8 @@ def lib():
         L.dazi_dk_last_timings.restype = C.c_int32
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]
         L.dazi_dk_reset_geometry.argtypes = []
+        L.dazi_dk_effective_dev.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 8
+        L.dazi_dk_pool_bytes.argtypes = [C.POINTER(C.c_int64)]
         for name in EXPORTS:
             if not hasattr(L, name):
                 raise RuntimeError(f"{LIB_PATH} does not export {name}")

@@ -15,6 +15,12 @@ layered workloads (ragged layered models, the batched solver's input):
                Vp and rho from Brocher.
   diff_4096    4,096 columns x 40 layers x 30 periods (1-60 s), for the VJP workload;
                c_obs from a +-3 % Vs checkerboard of 4x4-column cells.
+Voronoi realizations (an inversion step's population of models):
+  voronoi_100  100 realizations; each has 9 depth layers, per layer 100 Vs cells with
+               uniform-random nuclei over the 64x64 grid (nearest-nucleus assignment)
+               and 50 ANI cells; projected onto 64x64 columns x 25 nodes 0:5:120 km
+               (node z takes its layer's cell value): 409,600 node columns x 30
+               periods; vp, rho by Brocher (the depthkernel semantics).
 """
 from __future__ import annotations
 
@@ -100,6 +106,47 @@ def node_workload(name: str, seed: int | None = None, ncol: int | None = None):
         vs, vp, rho = sel(vs), sel(vp), sel(rho)
         nx, ny = ncol, 1
     return dict(vel=vs, vp=vp, rho=rho, depz=depz, periods=periods, minthk=minthk, nx=nx, ny=ny)
+
+
+VORONOI_LAYER_TOPS = np.array([0.0, 5.0, 10.0, 20.0, 30.0, 45.0, 60.0, 80.0, 100.0])
+
+
+def voronoi_workload(nreal: int = 100, seed: int | None = None, ncells: int = 100,
+                     nani: int = 50):
+    """dict(vel (nreal*4096, 1, 25) f32 node Vs (column j of realization r at
+    r*4096 + j, j = ix + 64*iy), depz, periods, minthk, layer_of_node (25,),
+    cell (nreal, 9, 4096) Vs-cell index per layer and column, ani_cell (nreal,
+    9, 4096), cell_vs (nreal, 9, ncells) f32)."""
+    rng = np.random.default_rng(5 if seed is None else seed)
+    n = 64
+    depz = np.arange(0, 121, 5, dtype=F)
+    z = depz.astype(np.float64)
+    layer_of_node = np.searchsorted(VORONOI_LAYER_TOPS, z, side="right") - 1
+    nl = VORONOI_LAYER_TOPS.size
+    gx, gy = np.meshgrid(np.arange(n) + 0.5, np.arange(n) + 0.5, indexing="xy")
+    pts = np.stack([gx.ravel(), gy.ravel()], axis=1)  # column j = ix + 64 iy
+    mid = np.append(0.5 * (VORONOI_LAYER_TOPS[1:] + VORONOI_LAYER_TOPS[:-1]), 110.0)
+    base = np.where(mid < 35.0, 3.2 + 0.6 * mid / 35.0, 4.3 + 0.3 * (mid - 35.0) / 85.0)
+
+    p2 = (pts ** 2).sum(1)[:, None]
+
+    def assign(nuclei):
+        d2 = p2 - 2.0 * (pts @ nuclei.T) + (nuclei ** 2).sum(1)[None, :]
+        return np.argmin(d2, axis=1).astype(np.int32)
+
+    cell = np.empty((nreal, nl, n * n), np.int32)
+    ani = np.empty((nreal, nl, n * n), np.int32)
+    cell_vs = np.empty((nreal, nl, ncells), F)
+    vel = np.empty((nreal, n * n, z.size), F)
+    for r in range(nreal):
+        for l in range(nl):
+            cell[r, l] = assign(rng.uniform(0.0, n, size=(ncells, 2)))
+            ani[r, l] = assign(rng.uniform(0.0, n, size=(nani, 2)))
+            cell_vs[r, l] = (base[l] * (1.0 + rng.uniform(-0.05, 0.05, ncells))).astype(F)
+        vel[r] = cell_vs[r][layer_of_node[None, :], cell[r][layer_of_node].T]
+    return dict(vel=vel.reshape(nreal * n * n, 1, z.size), depz=depz,
+                periods=np.geomspace(1.0, 60.0, 30), minthk=1.0, layer_of_node=layer_of_node,
+                cell=cell, ani_cell=ani, cell_vs=cell_vs, nx=nreal * n * n, ny=1)
 
 
 def _layered_profile(rng, L, moho, sed_thk, sed_vs, lvz, bottom=150.0):

@@ -390,6 +390,30 @@ int32_t dazi_dk_last_timings(double* out, int32_t n) {
   return m;
 }
 
+int32_t dazi_dk_effective_dev(const dazi_dk_desc* d, const float* vel, const double* sen_vs,
+                              const double* sen_vp, const double* sen_rho, const double* grad_c,
+                              double* eff, double* grad_vs, void* stream) {
+  if (!d || !vel || !sen_vs || !sen_vp || !sen_rho)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (grad_vs && !grad_c) return set_err(DAZI_DK_EINVAL, "grad_vs needs grad_c");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d", d->nx, d->ny, d->nz,
+                   d->kmax);
+  if (!eff && !grad_vs) return 0;
+  HIPCHK(dk::launch_effective(d->nx * d->ny, d->nz, d->kmax, vel, sen_vs, sen_vp, sen_rho, grad_c,
+                              eff, grad_vs, (hipStream_t)stream));
+  return 0;
+}
+
+int32_t dazi_dk_pool_bytes(int64_t* out) {
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  int64_t n = 0;
+  for (int s = 0; s < S_COUNT; s++) n += (int64_t)P->n[s];
+  *out = n;
+  return 0;
+}
+
 int32_t dazi_dk_reset_geometry(void) {
   Pool* P;
   if (int rc = get_pool(&P)) return rc;

@@ -82,6 +82,9 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
 hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
                           hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
+hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const double* svs,
+                            const double* svp, const double* srho, const double* gc, double* eff,
+                            double* grad, hipStream_t st);
 hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
                       double* grho, double* gthk, const float* thk32, const double* thk64,
                       double* tape, int nblocks, hipStream_t st);

@@ -301,3 +301,28 @@ def stage_dev(vel, depz, periods, minthk, vp=None, rho=None, iso=True, ti=True, 
         _tptr(out["status"]), _stream_ptr(stream))
     _lib.check(rc)
     return out
+
+
+def effective_dev(vel, sen_vs, sen_vp, sen_rho, grad_c=None, eff=True, grad=True, out=None,
+                  stream=None):
+    """Effective dc/dVs of the G builder's Vs row under the Brocher relation
+    (code/src_inv_iso_joint/CalSurfGAniso_Joint.f90:750-765) and its contraction
+    with grad_c over the periods (the VJP with respect to node Vs).
+
+    vel (nz, ncol) f32, sen_* (nz, K, ncol) f64, grad_c (K, ncol) f64; returns
+    dict(eff (nz-1, K, ncol), grad_vs (nz, ncol)) as requested."""
+    import torch
+    nz, K, ncol = sen_vs.shape
+    dev = sen_vs.device
+    if out is None:
+        out = {}
+        if eff:
+            out["eff"] = torch.empty((nz - 1, K, ncol), dtype=torch.float64, device=dev)
+        if grad and grad_c is not None:
+            out["grad_vs"] = torch.empty((nz, ncol), dtype=torch.float64, device=dev)
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 0, 1.0, 0)
+    rc = _lib.lib().dazi_dk_effective_dev(
+        C.byref(d), _tptr(vel), _tptr(sen_vs), _tptr(sen_vp), _tptr(sen_rho), _tptr(grad_c),
+        _tptr(out.get("eff")), _tptr(out.get("grad_vs")), _stream_ptr(stream))
+    _lib.check(rc)
+    return out

@@ -179,6 +179,24 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* batch, const double* grad
                                 double* grad_vs, double* grad_vp, double* grad_rho,
                                 double* grad_thk, void* stream);
 
+/* Effective dc/dVs of the G builder's Vs row (CalSurfGAniso_Joint.f90:750-765):
+ * with vp and rho tied to vs by Brocher,
+ *   eff(z,k,col) = sen_vs + sen_vp*coe_a + sen_rho*coe_rho, z < nz-1 (the
+ *   reference's rows 1:nz-1), coefficients in float32 from vel(col, z);
+ * and, when grad_vs is set, its contraction with grad_c(col, k):
+ *   grad_vs(col, z) = sum_k grad_c * eff for every node z (the VJP of c with
+ *   respect to node Vs under the Brocher relation).
+ * Device pointers in the stage's layouts: vel (nx*ny, nz) f32; sen_*
+ * (nx*ny, kmax, nz) f64; grad_c (nx*ny, kmax); eff (nx*ny, kmax, nz-1) f64 or
+ * NULL; grad_vs (nx*ny, nz) f64 or NULL. */
+int32_t dazi_dk_effective_dev(const dazi_dk_desc* desc, const float* vel, const double* sen_vs,
+                              const double* sen_vp, const double* sen_rho, const double* grad_c,
+                              double* eff, double* grad_vs, void* stream);
+
+/* Bytes of device memory the library's pool holds on the current device
+ * (current slots; buffers retired by growth are not counted). */
+int32_t dazi_dk_pool_bytes(int64_t* out);
+
 /* Event timings (ms) of the last call's phases on its stream:
  * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels,
  * [4] number of root-search launches.  Waits for those events.  Returns the
