@@ -172,7 +172,12 @@ def bench_c5(dk, configs, np, torch, chunk=40960):
         misfit.add_(0.5 * (s.gc[:, :n] ** 2).sum())
         cell_gradient(s.grad["grad_vs"], lon, cell, 100, c0, n, 4096, acc)
 
+    # the first pass uploads the graph and fills the allocator's cache; the
+    # second is timed
+    step.run(vel, c_obs, consume)
     torch.cuda.synchronize()
+    acc.zero_()
+    misfit.zero_()
     t0 = time.perf_counter()
     step.run(vel, c_obs, consume)
     torch.cuda.synchronize()
