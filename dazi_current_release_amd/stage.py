@@ -33,6 +33,34 @@ def _fortran(a, dtype, shape, name):
     return a
 
 
+def _env_int(name, default):
+    import os
+    try:
+        v = int(os.environ.get(name, "").strip() or default)
+    except ValueError:
+        v = default
+    return max(v, 1)
+
+
+def _dump(var, label, log, arrays):
+    """Writes arrays {file name: Fortran-ordered array} as raw stream files into
+    the directory named by environment variable `var`, as the reference's
+    depthkernel_dump_arrays (CalSurfG.f90:115-140) and depthkernelTI_dump_arrays
+    (depthkernelTI.f90:195-213) do."""
+    import os
+    d = os.environ.get(var)
+    if d is None:
+        return
+    d = d.strip()
+    if log is not None:
+        print(f"{label}{d}", file=log)
+    if not d:
+        return
+    os.makedirs(d, exist_ok=True)
+    for name, a in arrays.items():
+        np.asarray(a).ravel(order="F").tofile(os.path.join(d, name))
+
+
 def depthkernel(nx, ny, nz, vel, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc, iwave, igr, kmaxRc, tRc,
                 depz, minthk, *, vp=None, rho=None, log=sys.stdout):
     """Reference: depthkernel (CalSurfG.f90:1-113).
@@ -60,10 +88,15 @@ def depthkernel(nx, ny, nz, vel, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc, iwave, igr
         vp = _fortran(vp, np.float32, (nx, ny, nz), "vp")
         rho = _fortran(rho, np.float32, (nx, ny, nz), "rho")
         mode = 1
-    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr, 64, mode, float(minthk), 0)
+    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr, _env_int("DAZI_PDSURF_TILE_COLUMNS", 64), mode,
+                  float(minthk), 0)
     rc = _lib.lib().dazi_dk_iso(C.byref(d), _ptr(vel), _ptr(vp), _ptr(rho), _ptr(depz),
                                 _ptr(tRc), _ptr(pvRc), *[_ptr(o) for o in outs])
     _lib.check(rc)
+    if outs[0] is not None:
+        _dump("DAZI_DEPTHKERNEL_DUMP_DIR", "Depth kernel dump dir: ", log,
+              {"pv_depthkernel.bin": pvRc, "sen_vs_depthkernel.bin": outs[0],
+               "sen_vp_depthkernel.bin": outs[1], "sen_rho_depthkernel.bin": outs[2]})
 
 
 def run_depthkernel(vel, tRc, depz, minthk, vp=None, rho=None, kernels=True):
@@ -256,10 +289,13 @@ def depthkernelTI(nx, ny, nz, vel, pvRc, iwave, igr, kmaxRc, tRc, depz, minthk, 
     if log is not None:
         print(f"Depth kernel TI backend: {_BACKEND_NAME}", file=log)
         print(f"Depth kernel TI grid/periods: nx={nx} ny={ny} nz={nz} kmax={kmaxRc}", file=log)
-    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr, 64, 0, float(minthk), 0)
+    d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr,
+                  _env_int("DAZI_DEPTHKERNEL_TI_TILE_COLUMNS", 64), 0, float(minthk), 0)
     rc = _lib.lib().dazi_dk_ti(C.byref(d), _ptr(vel), _ptr(depz), _ptr(tRc), _ptr(pvRc),
                                _ptr(Lsen_Gsc))
     _lib.check(rc)
+    _dump("DAZI_DEPTHKERNEL_TI_DUMP_DIR", "Depth kernel TI dump dir: ", log,
+          {"pv_ti.bin": pvRc, "lsen_gsc_ti.bin": Lsen_Gsc})
 
 
 def run_depthkernelTI(vel, tRc, depz, minthk):

@@ -44,3 +44,29 @@ def test_fortran_host_matches_python_and_reference(tmp_path):
     d = o["ti_lsen"][:n] - f["ti_lsen"]
     assert np.abs(d).max() <= 1e-4
     assert (np.abs(d) / np.abs(f["ti_lsen"]).max(axis=-1, keepdims=True)).max() <= 1e-4
+
+
+def test_fortran_dumps_ab_against_reference(tmp_path):
+    """The reference's A/B workflow: its own FD dumps (the fixture, written in the
+    dump layout) against the hip backend's dumps, compared with the A/B metrics and
+    held to the reference's ISO bands (run_iso_depthkernel_ab.py:324-338)."""
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from dazi_current_release_amd import compare
+    f = load_golden("ref_plain_1024.npz")
+    nz, K = f["vel"].shape[1], len(f["periods"])
+    vel = np.asfortranarray(f["vel"].reshape(8, 4, nz, order="F"))
+    dump = tmp_path / "hip"
+    r, _ = FIO.run(str(tmp_path), vel, f["depz"], f["periods"], float(f["minthk"]),
+                   env=dict(DAZI_DEPTHKERNEL_DUMP_DIR=str(dump)))
+    assert r.returncode == 0, r.stderr
+    base = tmp_path / "ref"
+    base.mkdir()
+    for name, fn in (("pv", "pv_depthkernel.bin"), ("sen_vs", "sen_vs_depthkernel.bin"),
+                     ("sen_vp", "sen_vp_depthkernel.bin"), ("sen_rho", "sen_rho_depthkernel.bin")):
+        np.asfortranarray(f[name]).ravel(order="F").tofile(base / fn)
+    rows = {x["array"]: x for x in compare.compare_dirs(base, dump, 8, 4, nz, K, vel)}
+    assert rows["pv"]["max_abs"] <= 1e-5 and rows["pv"]["cross_abs_0.0001"] == 0
+    for name, (l2, mx) in (("sen_vs", (1e-3, 2e-4)), ("sen_vp", (2e-3, 2e-4)),
+                           ("sen_rho", (2e-3, 2e-4)), ("effective_dcdvs", (1e-3, 3e-4))):
+        assert rows[name]["rel_l2"] <= l2 and rows[name]["max_abs"] <= mx, (name, rows[name])
