@@ -200,7 +200,8 @@ int kernel_threads(Pool* P, int64_t nsolves) {
 }
 
 int ti_threads(Pool* P, int64_t nsolves) {
-  int64_t cap = (int64_t)P->ncu * 256;
+  // two waves per SIMD (k_ti fits two in the register file)
+  int64_t cap = (int64_t)P->ncu * 512;
   int64_t n = nsolves < cap ? nsolves : cap;
   n = (n + 127) / 128 * 128;
   if (n < 128) n = 128;

@@ -59,7 +59,7 @@ struct TiLayerDev {
 };
 
 // Doubles of TI scratch per layer and thread.
-constexpr int TI_SCRATCH = 14;
+constexpr int TI_SCRATCH = 9;
 
 hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
                           const float* depz, float minthk0, int L, TiLayerDev* out,

@@ -394,43 +394,50 @@ DK_HD cx h2func(cx nua, cx nub, double dm) {
 // Layer integrals INT11, INT13, INT22, INT24, INT33, INT44 of the field whose
 // up-going amplitudes come from the bottom interface value yb and down-going
 // ones from the top value yt (intijr); half = 1 for the half-space.
+// With the eigenvector matrix E written as E[i] = (a_i, b_i, s_i a_i, -s_i b_i),
+// a = (x11, x21 nuP, x31, x41 nuP), b = (x12 nuS, x22, x32 nuS, x42), s_i = +1
+// for even rows and -1 for odd rows, and every integral pairing rows of equal
+// parity, the sum over the ten wave-pair products of intijr collapses to
+//   Re(a_i a_j A_s + b_i b_j B_s + (a_i b_j + b_i a_j) C_s).
 DK_HD void layer_integrals(const Eig& g, double dm, const double yt[4], const double yb[4],
                            int half, double out[6]) {
-  cx E[4][4], G[4][4];
-  eig_matrices(g, E, G);
-  cx ra = nu_P(g), rb = nu_S(g);
-  cx k[4];  // kmpu, kmsu (from yb), km1pd, km1sd (from yt)
-  for (int r = 0; r < 4; r++) {
-    const double* y = (r < 2) ? yb : yt;
-    k[r] = G[r][0] * y[0] + G[r][1] * y[1] + G[r][2] * y[2] + G[r][3] * y[3];
-  }
+  const cx ra = nu_P(g), rb = nu_S(g);
+  // rows of G = E^-1: 1/(2 nu N) (real or imaginary) and nu/(2 nu N) = 1/(2N)
+  const cx iP = g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
+  const cx iS = g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
+  const double hP = 0.5 / g.NP, hS = 0.5 / g.NSV;
+  // amplitudes: up-going P, SV from yb; down-going P, SV from yt
+  const cx k0 = mk(hP * (g.x41 * yb[0] - g.x21 * yb[2])) + iP * (g.x11 * yb[3] - g.x31 * yb[1]);
+  const cx k1 = iS * (g.x42 * yb[0] - g.x22 * yb[2]) + mk(hS * (g.x12 * yb[3] - g.x32 * yb[1]));
+  const cx k2 = mk(hP * (g.x41 * yt[0] - g.x21 * yt[2])) + iP * (g.x31 * yt[1] - g.x11 * yt[3]);
+  const cx k3 = iS * (g.x22 * yt[2] - g.x42 * yt[0]) + mk(hS * (g.x12 * yt[3] - g.x32 * yt[1]));
+  const cx a[4] = {mk(g.x11), g.x21 * ra, mk(g.x31), g.x41 * ra};
+  const cx b[4] = {g.x12 * rb, mk(g.x22), g.x32 * rb, mk(g.x42)};
   const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
+  cx Ae, Be, Ce, Ao, Bo, Co;
   if (half) {
-    const cx k22 = k[2] * k[2] / (2.0 * ra), k23 = k[2] * k[3] / (ra + rb),
-             k33 = k[3] * k[3] / (2.0 * rb);
-    for (int t = 0; t < 6; t++) {
-      int i = I[t], j = J[t];
-      cx v = E[i][2] * E[j][2] * k22 + (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k23 +
-             E[i][3] * E[j][3] * k33;
-      out[t] = v.r;
-    }
-    return;
+    Ae = Ao = k2 * k2 / (2.0 * ra);
+    Be = Bo = k3 * k3 / (2.0 * rb);
+    Ce = Co = -(k2 * k3 / (ra + rb));
+  } else {
+    const cx FA = ffunc(ra, dm), GA = gfunc(ra, dm), FB = ffunc(rb, dm), GB = gfunc(rb, dm);
+    const cx H1 = h1func(ra, rb, dm), H2 = h2func(ra, rb, dm);
+    const cx p00 = (k0 * k0 + k2 * k2) * FA, p02 = 2.0 * (GA * (k0 * k2));
+    const cx p11 = (k1 * k1 + k3 * k3) * FB, p13 = 2.0 * (GB * (k1 * k3));
+    const cx p01 = H1 * (k0 * k1 - k2 * k3), p21 = H2 * (k2 * k1 - k0 * k3);
+    Ae = p00 + p02;
+    Ao = p00 - p02;
+    Be = p11 - p13;
+    Bo = p11 + p13;
+    Ce = p01 + p21;
+    Co = p01 - p21;
   }
-  const cx FA = ffunc(ra, dm), GA = gfunc(ra, dm), FB = ffunc(rb, dm), GB = gfunc(rb, dm);
-  const cx H1 = h1func(ra, rb, dm), H2 = h2func(ra, rb, dm);
-  const cx k00 = k[0] * k[0] * FA, k22 = k[2] * k[2] * FA, k11 = k[1] * k[1] * FB,
-           k33 = k[3] * k[3] * FB, k01 = H1 * (k[0] * k[1]), k23 = H1 * (k[2] * k[3]),
-           k03 = H2 * (k[0] * k[3]), k21 = H2 * (k[2] * k[1]), k02 = GA * (k[0] * k[2]),
-           k13 = GB * (k[1] * k[3]);
+#pragma unroll
   for (int t = 0; t < 6; t++) {
-    int i = I[t], j = J[t];
-    cx v = E[i][0] * E[j][0] * k00 + E[i][2] * E[j][2] * k22 + E[i][1] * E[j][1] * k11 +
-           E[i][3] * E[j][3] * k33 + (E[i][0] * E[j][1] + E[i][1] * E[j][0]) * k01 +
-           (E[i][2] * E[j][3] + E[i][3] * E[j][2]) * k23 +
-           (E[i][0] * E[j][3] + E[i][3] * E[j][0]) * k03 +
-           (E[i][1] * E[j][2] + E[i][2] * E[j][1]) * k21 +
-           (E[i][0] * E[j][2] + E[i][2] * E[j][0]) * k02 +
-           (E[i][1] * E[j][3] + E[i][3] * E[j][1]) * k13;
+    const int i = I[t], j = J[t];
+    const bool ev = (i & 1) == 0;
+    const cx v = a[i] * a[j] * (ev ? Ae : Ao) + b[i] * b[j] * (ev ? Be : Bo) +
+                 (a[i] * b[j] + b[i] * a[j]) * (ev ? Ce : Co);
     out[t] = v.r;
   }
 }
@@ -500,10 +507,11 @@ DK_HD void flatten(int L, const float* thk, const float* TA, const float* TL, co
 }
 
 // ----------------------------------------------------------------- one (column, period)
-// Scratch per layer and thread (element (m, q) at scratch[(m * 14 + q) * stride]):
-// the up-sweep compound vector (5) and its accumulated log scale (1), the
-// interface eigenfunction (4) and the unnormalised kernel factors (4).
-enum { Q_CD = 0, Q_EX = 5, Q_Y = 6, Q_F = 10, Q_N = 14 };
+// Scratch per layer and thread (element (m, q) at scratch[(m * 9 + q) * stride]):
+// the up-sweep compound vector (5) and its accumulated log scale (1), and the
+// unnormalised kernel factors facah, facbv, facn (3).  The interface
+// eigenfunction lives in registers (each layer needs its top and bottom value).
+enum { Q_CD = 0, Q_EX = 5, Q_F = 6, Q_N = 9 };
 // emit(m, dcdah, dcdbv, dcdn) receives, top layer first, tregn96's kernels after
 // the sphericity back-mapping, rounded to float as chksiz does.  lay(m) returns
 // the flattened Layer, vtpf(m) the float sphericity factor.  Returns 0, or -1
@@ -540,10 +548,14 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   const double exe1 = S(0, Q_EX);
   const double f1213 = -S(0, Q_CD + 1);
   if (!(f1213 != 0.0) || !isfinite(f1213)) return -1;
-  S(0, Q_Y + 0) = S(0, Q_CD + 2) / S(0, Q_CD + 1);
-  S(0, Q_Y + 1) = 1.0;
-  S(0, Q_Y + 2) = 0.0;
-  S(0, Q_Y + 3) = 0.0;
+  double yt[4] = {S(0, Q_CD + 2) / S(0, Q_CD + 1), 1.0, 0.0, 0.0};
+  // Q dispersion of c (gammap: Qp = 150, Qs = 50, fref = 1 Hz), accumulated
+  // over the layers and normalised at the end; used only by the sphericity
+  // factor tm below
+  const double pi = 3.141592653589493;
+  const double qai = (double)(1.0f / 150.0f), qbi = (double)(1.0f / 50.0f);
+  const double lw = log(omega / (2.0 * pi * 1.0)) / pi;
+  double dcq = 0.0;
   // ---- down sweep, interface eigenfunctions and layer integrals
   double vv[4] = {1.0, 0.0, 0.0, 0.0};
   double exd = 0.0;
@@ -551,6 +563,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   for (int m = 0; m < L; m++) {
     const Layer ly = lay(m);
     eigen(ly, omega, wvno, &g);
+    double yb[4] = {0.0, 0.0, 0.0, 0.0};  // interface m+1 (zero below the half-space top)
     if (m < L - 1) {
       hyper(g, ly.d, &h);
       double sP = 1.0, sS = 1.0, cpex;
@@ -579,20 +592,13 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       double uu3 = tz1 * cd5 - tz2 * cd3 + tz4 * cd1;
       double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
       double ext = exd + S(n, Q_EX) - exe1;
-      double y[4] = {0, 0, 0, 0};
       if (ext > -80.0 && ext < 80.0) {
         double fact = exp(ext);
-        y[0] = uu1 * fact / f1213;
-        y[1] = uu2 * fact / f1213;
-        y[2] = uu3 * fact / f1213;
-        y[3] = uu4 * fact / f1213;
+        yb[0] = uu1 * fact / f1213;
+        yb[1] = uu2 * fact / f1213;
+        yb[2] = uu3 * fact / f1213;
+        yb[3] = uu4 * fact / f1213;
       }
-      for (int i = 0; i < 4; i++) S(n, Q_Y + i) = y[i];
-    }
-    double yt[4], yb[4];
-    for (int i = 0; i < 4; i++) {
-      yt[i] = S(m, Q_Y + i);
-      yb[i] = (m < L - 1) ? S(m + 1, Q_Y + i) : 0.0;
     }
     double I[6];
     layer_integrals(g, ly.d, yt, yb, m == L - 1, I);
@@ -612,26 +618,20 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     sumi1 = sumi1 + ly.L * UZUZ + ly.A * URUR;
     sumi2 = sumi2 + ly.L * UZDUR - ly.F * URDUZ;
     sumi3 = sumi3 + ly.L * DURDUR + ly.C * DUZDUZ;
-    S(m, Q_F + 0) = rho * ah * (URUR - 2.0 * eta * URDUZ / wvno);                          // facah
-    S(m, Q_F + 1) = rho * av * DUZDUZ / wvno2;                                             // facav
-    S(m, Q_F + 2) = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
-    S(m, Q_F + 3) = -ly.F * URDUZ / (wvno * eta);                                          // facn
+    const double facah = rho * ah * (URUR - 2.0 * eta * URDUZ / wvno);
+    const double facav = rho * av * DUZDUZ / wvno2;
+    const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
+    const double facn = -ly.F * URDUZ / (wvno * eta);
+    S(m, Q_F + 0) = facah;
+    S(m, Q_F + 1) = facbv;
+    S(m, Q_F + 2) = facn;
+    dcq = dcq + lw * (facbv * bv * qbi) + lw * (facav * av * qai + facah * ah * qai);
+#pragma unroll
+    for (int i = 0; i < 4; i++) yt[i] = yb[i];
   }
   const double ugr = (wvno * sumi1 + sumi2) / (omega * sumi0);
   const double nrm = ugr * sumi0;
-  // ---- Q dispersion of c (gammap: Qp = 150, Qs = 50, fref = 1 Hz), used only by
-  //      the sphericity factor tm below
-  const double pi = 3.141592653589493;
-  const double qai = (double)(1.0f / 150.0f), qbi = (double)(1.0f / 50.0f);
-  const double lw = log(omega / (2.0 * pi * 1.0)) / pi;
-  double dc = 0.0;
-  for (int m = 0; m < L; m++) {
-    const Layer ly = lay(m);
-    double ah = sqrt(ly.A / ly.rho), av = sqrt(ly.C / ly.rho), bv = sqrt(ly.L / ly.rho);
-    double xs = (S(m, Q_F + 2) / nrm) * bv * qbi;
-    double xp = (S(m, Q_F + 1) / nrm) * av * qai + (S(m, Q_F + 0) / nrm) * ah * qai;
-    dc = dc + lw * xs + lw * xp;
-  }
+  const double dc = dcq / nrm;
   const double cq = omega / wvno + dc;
   const double ar = 6370.0;
   const double tm = sqrt(1.0 + (cq / (2.0 * ar * omega)) * (cq / (2.0 * ar * omega)));
@@ -640,8 +640,8 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   auto f32 = [](double x) -> float { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; };
   for (int m = 0; m < L; m++) {
     const double vt = (double)vtpf(m);
-    emit(m, f32(S(m, Q_F + 0) / nrm * vt / tm3), f32(S(m, Q_F + 2) / nrm * vt / tm3),
-         f32(S(m, Q_F + 3) / nrm));
+    emit(m, f32(S(m, Q_F + 0) / nrm * vt / tm3), f32(S(m, Q_F + 1) / nrm * vt / tm3),
+         f32(S(m, Q_F + 2) / nrm));
   }
   return 0;
 }
