@@ -838,96 +838,101 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   const double wv = omega / c;
   const double wv2 = wv * wv;
 
-  // ---- sweep 1: bottom-up values; tape holds the vector entering layer m
-  double e[5];
+  // ---- sweep 1: bottom-up values and their tangent in c; the tape holds the
+  // (normalised) vector entering layer m and its exponent
+  Dual<1> wvd = dconst<1>(wv);
+  wvd.g[0] = -wv / c;
+  const Dual<1> wv2d = wvd * wvd;
+  Dual<1> e[5];
   {
     double4 h = lay[L - 1];
-    halfspace_e<double>(wv, wv2, (double)h.y, (double)h.z, (double)h.w, omega, e);
+    halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
+                         dconst<1>((double)h.w), omega, e);
   }
-  int Ef = pow2_normalize(e);
-  for (int m = L - 2; m >= 0; m--) {
-    double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+  int Ef = 0;
+  for (int m = L - 1; m >= 0; m--) {
+    if (m < L - 1) {
+      double* tp = tape + (int64_t)(m * 6) * nthr + tid;
 #pragma unroll
-    for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i];
-    tp[(int64_t)5 * nthr] = (double)Ef;
-    double4 l = lay[m];
-    double ca[5][5];
-    layer_ca<double>(wv, wv2, (double)l.y, (double)l.z, (double)l.w, (double)l.x, omega, ca);
-    double ee[5];
+      for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i].v;
+      tp[(int64_t)5 * nthr] = (double)Ef;
+      double4 l = lay[m];
+      double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      layer_ca_visit<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
+                              dconst<1>((double)l.w), (double)l.x, omega,
+                              [&](int j, int i, const Dual<1>& v) {
+                                ev[i] = fma(e[j].v, v.v, ev[i]);
+                                eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
+                              });
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        e[i].v = ev[i];
+        e[i].g[0] = eg[i];
+      }
+    }
+    // normalise values and tangent together (exact power of two)
+    double mx = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(e[i].v));
+    int kk = 0;
+    if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      double cr = 0.0;
-#pragma unroll
-      for (int j = 0; j < 5; j++) cr = fma(e[j], ca[j][i], cr);
-      ee[i] = cr;
+      e[i].v = ldexp(e[i].v, -kk);
+      e[i].g[0] = ldexp(e[i].g[0], -kk);
     }
-#pragma unroll
-    for (int i = 0; i < 5; i++) e[i] = ee[i];
-    Ef += pow2_normalize(e);
+    Ef += kk;
   }
   const int EF = Ef;  // exponent of the top vector: reference scale
+  // dc/dm = -(dF/dm)/(dF/dc), both at the reference scale 2^EF
+  const double dFdc = e[0].g[0];
+  const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
 
-  // ---- sweep 2: adjoint top-down with Dual<4> layer matrices
-  // g = dF/de (true scale g * 2^Eg); indices 0:a 1:b 2:rho 3:c
+  // ---- sweep 2: adjoint top-down with Dual<3> layer matrices
+  // g = dF/de (true scale g * 2^Eg); indices 0:a 1:b 2:rho
   double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
   int Eg = 0;
-  double dFdc = 0.0;
-  Dual<4> wvd = dconst<4>(wv);
-  wvd.g[3] = -wv / c;
-  Dual<4> wv2d = wvd * wvd;
   // node accumulation state
   int node = 0, sub = 0;
   double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
   for (int m = 0; m <= L - 1; m++) {
     double4 l = lay[m];
-    double part[4];
+    double part[3];
     if (m < L - 1) {
-      Dual<4> ca[5][5];
-      layer_ca<Dual<4>>(wvd, wv2d, dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
-                        dvar<4>((double)l.w, 2), (double)l.x, omega, ca);
       const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
       double et[5];
 #pragma unroll
       for (int i = 0; i < 5; i++) et[i] = tp[(int64_t)i * nthr];
       int Ee = (int)tp[(int64_t)5 * nthr];
-      // contribution: sum_ji e_j dCA_ji g_i, scaled 2^(Ee+Eg-EF)
+      // contribution sum_ji e_j dCA_ji g_i and the propagation g <- CA g,
+      // reduced entry by entry as the layer matrix is formed
+      double acc[3] = {0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      layer_ca_visit<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
+                              dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), (double)l.x,
+                              omega, [&](int j, int i, const Dual<3>& v) {
+                                const double w = et[j] * g[i];
+                                acc[0] = fma(w, v.g[0], acc[0]);
+                                acc[1] = fma(w, v.g[1], acc[1]);
+                                acc[2] = fma(w, v.g[2], acc[2]);
+                                gn[j] = fma(v.v, g[i], gn[j]);
+                              });
 #pragma unroll
-      for (int p = 0; p < 4; p++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-          double t = 0.0;
-#pragma unroll
-          for (int i = 0; i < 5; i++) t = fma(ca[j][i].g[p], g[i], t);
-          acc = fma(et[j], t, acc);
-        }
-        part[p] = ldexp(acc, Ee + Eg - EF);
-      }
-      // propagate g <- CA g
-      double gn[5];
-#pragma unroll
-      for (int j = 0; j < 5; j++) {
-        double t = 0.0;
-#pragma unroll
-        for (int i = 0; i < 5; i++) t = fma(ca[j][i].v, g[i], t);
-        gn[j] = t;
-      }
+      for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * sc;
 #pragma unroll
       for (int j = 0; j < 5; j++) g[j] = gn[j];
       Eg += pow2_normalize(g);
     } else {
-      Dual<4> eh[5];
-      halfspace_e<Dual<4>>(wvd, wv2d, dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
-                           dvar<4>((double)l.w, 2), omega, eh);
+      Dual<3> eh[5];
+      halfspace_e<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
+                           dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), omega, eh);
 #pragma unroll
-      for (int p = 0; p < 4; p++) {
+      for (int p = 0; p < 3; p++) {
         double acc = 0.0;
 #pragma unroll
         for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
-        part[p] = ldexp(acc, Eg - EF);
+        part[p] = ldexp(acc, Eg - EF) * sc;
       }
     }
-    dFdc += part[3];
     // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
     // rho_flat = rho*btp^-2.275
     double bt = (double)btp[m];
@@ -971,14 +976,6 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       if (O.dvp) O.dvp[o] = acc_lo[1] + pa;
       if (O.drho) O.drho[o] = acc_lo[2] + pr;
     }
-  }
-  // ---- rescale: dc/dm = -(dF/dm)/(dF/dc)
-  const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
-  for (int m = 0; m < nout; m++) {
-    int64_t o = out_index(m);
-    if (O.dvs) O.dvs[o] *= sc;
-    if (O.dvp) O.dvp[o] *= sc;
-    if (O.drho) O.drho[o] *= sc;
   }
 }
 
@@ -1119,35 +1116,23 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
         if (act) {
           double4 l = lay[m];
           if (m < L - 1) {
-            Dual<4> ca[5][5];
-            layer_ca<Dual<4>, Dual<4>>(dconst<4>(wv), dconst<4>(wv2), dvar<4>((double)l.y, 0),
-                                       dvar<4>((double)l.z, 1), dvar<4>((double)l.w, 2),
-                                       dvar<4>((double)l.x, 3), omega, ca);
             const double* tp = tape + (long)(m * 6) * nslot + slot;
             double et[5];
 #pragma unroll
             for (int i = 0; i < 5; i++) et[i] = tp[(long)i * nslot];
             const int Ee = (int)tp[(long)5 * nslot];
+            double acc[4] = {0.0, 0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            layer_ca_visit<Dual<4>, Dual<4>>(
+                dconst<4>(wv), dconst<4>(wv2), dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
+                dvar<4>((double)l.w, 2), dvar<4>((double)l.x, 3), omega,
+                [&](int j, int i, const Dual<4>& v) {
+                  const double w = et[j] * g[i];
 #pragma unroll
-            for (int p = 0; p < 4; p++) {
-              double acc = 0.0;
+                  for (int p = 0; p < 4; p++) acc[p] = fma(w, v.g[p], acc[p]);
+                  gn[j] = fma(v.v, g[i], gn[j]);
+                });
 #pragma unroll
-              for (int j = 0; j < 5; j++) {
-                double t = 0.0;
-#pragma unroll
-                for (int i = 0; i < 5; i++) t = fma(ca[j][i].g[p], g[i], t);
-                acc = fma(et[j], t, acc);
-              }
-              part[p] = ldexp(acc, Ee + Eg - EF) * lane_scale;
-            }
-            double gn[5];
-#pragma unroll
-            for (int j = 0; j < 5; j++) {
-              double t = 0.0;
-#pragma unroll
-              for (int i = 0; i < 5; i++) t = fma(ca[j][i].v, g[i], t);
-              gn[j] = t;
-            }
+            for (int p = 0; p < 4; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * lane_scale;
 #pragma unroll
             for (int j = 0; j < 5; j++) g[j] = gn[j];
             Eg += pow2_normalize(g);

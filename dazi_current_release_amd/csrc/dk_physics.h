@@ -281,9 +281,11 @@ DK_HD Dual<N> hyp3_t(const Dual<N>& X, double d, Dual<N>* S, Dual<N>* R, double*
 // One layer: Dunkin's reduced compound matrix ca (surfdisp96.f:808-831 plus
 // var and dnka), scaled by exp(-exa). a,b,rho are the flattened layer values,
 // d its flattened thickness (a double, or a Dual<N> to differentiate in d).
-template <class T, class D = double>
-DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
-                    const D& d, double omega, T ca[5][5]) {
+// visit(j, i, value) receives every entry ca[j][i] once as it is formed (so a
+// consumer can reduce the matrix without storing it); layer_ca stores them.
+template <class T, class D = double, class V>
+DK_HD void layer_ca_visit(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
+                          const D& d, double omega, V&& visit) {
   T xka = omega / a;
   T xkb = omega / b;
   T t = b * (1.0 / omega);
@@ -315,32 +317,67 @@ DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const
   T rho2 = rho * rho;
   T irho2 = irho * irho;
   T a0pq = a0 - cpcq;
-  ca[0][0] = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
-  ca[0][1] = (wvno2 * cpy - cqx) * irho;
-  ca[0][2] = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
-  ca[0][3] = (cpz - wvno2 * cqw) * irho;
-  ca[0][4] = -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2;
-  ca[1][0] = (gmgmk * cpz - gm1sq * cqw) * rho;
-  ca[1][1] = cpcq;
-  ca[1][2] = gammk * cpz - gamm1 * cqw;
-  ca[1][3] = -wz;
-  ca[1][4] = ca[0][3];
-  ca[3][0] = (gm1sq * cpy - gmgmk * cqx) * rho;
-  ca[3][1] = -xy;
-  ca[3][2] = gamm1 * cpy - gammk * cqx;
-  ca[3][3] = ca[1][1];
-  ca[3][4] = ca[0][1];
-  ca[4][0] = -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
-  ca[4][1] = ca[3][0];
-  ca[4][2] = -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
-  ca[4][3] = ca[1][0];
-  ca[4][4] = ca[0][0];
   T tt = -2.0 * wvno2;
-  ca[2][0] = tt * ca[4][2];
-  ca[2][1] = tt * ca[3][2];
-  ca[2][2] = a0 + 2.0 * (cpcq - ca[0][0]);
-  ca[2][3] = tt * ca[1][2];
-  ca[2][4] = tt * ca[0][2];
+  {
+    const T c00 = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+    visit(0, 0, c00);
+    visit(4, 4, c00);
+    visit(2, 2, a0 + 2.0 * (cpcq - c00));
+  }
+  {
+    const T c01 = (wvno2 * cpy - cqx) * irho;
+    visit(0, 1, c01);
+    visit(3, 4, c01);
+  }
+  {
+    const T c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
+    visit(0, 2, c02);
+    visit(2, 4, tt * c02);
+  }
+  {
+    const T c03 = (cpz - wvno2 * cqw) * irho;
+    visit(0, 3, c03);
+    visit(1, 4, c03);
+  }
+  visit(0, 4, -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2);
+  {
+    const T c10 = (gmgmk * cpz - gm1sq * cqw) * rho;
+    visit(1, 0, c10);
+    visit(4, 3, c10);
+  }
+  visit(1, 1, cpcq);
+  visit(3, 3, cpcq);
+  {
+    const T c12 = gammk * cpz - gamm1 * cqw;
+    visit(1, 2, c12);
+    visit(2, 3, tt * c12);
+  }
+  visit(1, 3, -wz);
+  {
+    const T c30 = (gm1sq * cpy - gmgmk * cqx) * rho;
+    visit(3, 0, c30);
+    visit(4, 1, c30);
+  }
+  visit(3, 1, -xy);
+  {
+    const T c32 = gamm1 * cpy - gammk * cqx;
+    visit(3, 2, c32);
+    visit(2, 1, tt * c32);
+  }
+  visit(4, 0, -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2);
+  {
+    const T c42 =
+        -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
+    visit(4, 2, c42);
+    visit(2, 0, tt * c42);
+  }
+}
+
+template <class T, class D = double>
+DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
+                    const D& d, double omega, T ca[5][5]) {
+  layer_ca_visit<T, D>(wvno, wvno2, a, b, rho, d, omega,
+                       [&](int j, int i, const T& v) { ca[j][i] = v; });
 }
 
 // Half-space 5-vector (surfdisp96.f:777-802).
