@@ -108,29 +108,69 @@ __device__ void start_velocity(const double4* lay, int L, double* cc_out, float*
   *betmx_out = betmx;
 }
 
-__device__ void flatten_column(double4* lay, double* btp_out, int L, double* cc_out,
-                               float* betmx_out, int* bad) {
+// Earth flattening of a column (sphere(0,0) then sphere(2,1) for Rayleigh
+// waves, surfdisp96.f:480-547) followed by the start-velocity rule of
+// surfdisp96 (:139-207), in streaming form: layers (thk, vp, vs, rho as float32
+// values) are pushed top to bottom and flattened as soon as they are formed (no
+// read-back of the column from memory) into (d, a, b, rho) rounded to float32
+// as the reference stores them, while the start-velocity state is tracked.
+struct FlatState {
+  double dr = 0.0, r0 = 6370.0;
+  float betmx = -1.e20f, betmn = 1.e20f;
+  float amn = 0.0f, bmn = 0.0f;  // flattened a, b of the start-velocity layer
+  int jsol = 1;
+  int bad = 0;
+};
+
+__device__ double4 flat_push(FlatState& s, double4 l, int m, int L, double* btp_m) {
 #pragma clang fp contract(off)
   const double ar = 6370.0;
-  double dr = 0.0, r0 = ar;
-  for (int m = 0; m < L; m++) {
-    double4 l = lay[m];
-    float thk = (m == L - 1) ? 1.0f : (float)l.x;
-    dr = dr + (double)thk;
-    double r1 = ar - dr;
-    double z0 = ar * log(ar / r0);
-    double z1 = ar * log(ar / r1);
-    float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
-    double tmp = (ar + ar) / (r0 + r1);
-    float a = (float)((double)(float)l.y * tmp);
-    float b = (float)((double)(float)l.z * tmp);
-    float bt = (float)tmp;
-    float rho = (float)l.w * powf(bt, -2.275f);
-    r0 = r1;
-    lay[m] = make_double4(d, a, b, rho);
-    if (btp_out) btp_out[m] = bt;
+  float thk = (m == L - 1) ? 1.0f : (float)l.x;
+  s.dr = s.dr + (double)thk;
+  double r1 = ar - s.dr;
+  double z0 = ar * log(ar / s.r0);
+  double z1 = ar * log(ar / r1);
+  float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
+  double tmp = (ar + ar) / (s.r0 + r1);
+  float a = (float)((double)(float)l.y * tmp);
+  float b = (float)((double)(float)l.z * tmp);
+  float bt = (float)tmp;
+  float rho = (float)l.w * powf(bt, -2.275f);
+  s.r0 = r1;
+  if (btp_m) *btp_m = bt;
+  const double4 o = make_double4(d, a, b, rho);
+  if (m == 0) {
+    s.amn = a;
+    s.bmn = b;
+    if (o.z <= 0.0) s.bad = 1;
   }
-  start_velocity(lay, L, cc_out, betmx_out, bad);
+  if (b > 0.01f && b < s.betmn) {
+    s.betmn = b;
+    s.amn = a;
+    s.bmn = b;
+    s.jsol = 1;
+  } else if (b <= 0.01f && a < s.betmn) {
+    s.betmn = a;
+    s.amn = a;
+    s.bmn = b;
+    s.jsol = 0;
+  }
+  if (b > s.betmx) s.betmx = b;
+  bool ok = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(o.w) && o.y > 0.0 &&
+            o.z > 0.0 && o.w > 0.0 && (o.x > 0.0 || m == L - 1);
+  if (!ok) s.bad = 1;
+  return o;
+}
+
+__device__ void flat_finish(const FlatState& s, double* cc_out, float* betmx_out, int* bad) {
+#pragma clang fp contract(off)
+  if (s.jsol == 0 || s.bad) *bad = 1;
+  float cc1 = gtsolh_f32(s.amn, s.bmn);
+  cc1 = 0.95f * cc1;
+  cc1 = 0.90f * cc1;
+  if (!(cc1 > 0.0f) || !isfinite(cc1)) *bad = 1;
+  *cc_out = (double)cc1;
+  *betmx_out = s.betmx;
 }
 
 // The same flattening carried out in float64 for float64 models (the
@@ -165,6 +205,8 @@ __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   double4* lay = layers + (size_t)col * L;
+  double* bt = btp + (size_t)col * L;
+  FlatState fs;
   int k = 0;
   float vs_i = vel[col], vp_i, rho_i;
   if (vpin) {
@@ -194,15 +236,15 @@ __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
       float rvp = vp_i + w * (vp_n - vp_i) / den;
       float rvs = vs_i + w * (vs_n - vs_i) / den;
       float rrho = rho_i + w * (rho_n - rho_i) / den;
-      lay[k] = make_double4(newthk, rvp, rvs, rrho);
+      lay[k] = flat_push(fs, make_double4(newthk, rvp, rvs, rrho), k, L, bt + k);
     }
     vs_i = vs_n;
     vp_i = vp_n;
     rho_i = rho_n;
   }
-  lay[L - 1] = make_double4(0.0, vp_i, vs_i, rho_i);
   int bad = (k != L - 1);
-  flatten_column(lay, btp + (size_t)col * L, L, cc + col, betmx + col, &bad);
+  lay[L - 1] = flat_push(fs, make_double4(0.0, vp_i, vs_i, rho_i), L - 1, L, bt + L - 1);
+  flat_finish(fs, cc + col, betmx + col, &bad);
   bad_out[col] = bad;
 }
 
@@ -226,9 +268,12 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
     for (int m = 0; m < L; m++)
       lay[m] = make_double4(thk64[o + m], vp64[o + m], vs64[o + m], rho64[o + m]);
     if (L >= 1) flatten_column_f64(lay, btp + o, L, cc + col, betmx + col, &bad);
-  } else {
-    for (int m = 0; m < L; m++) lay[m] = make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]);
-    if (L >= 1) flatten_column(lay, btp + o, L, cc + col, betmx + col, &bad);
+  } else if (L >= 1) {
+    FlatState fs;
+    for (int m = 0; m < L; m++)
+      lay[m] = flat_push(fs, make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]), m, L,
+                         btp + o + m);
+    flat_finish(fs, cc + col, betmx + col, &bad);
   }
   bad_out[col] = bad;
 }

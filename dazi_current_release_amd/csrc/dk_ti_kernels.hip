@@ -28,6 +28,51 @@ __device__ float brocher_rho_ti(float p) {
   return 1.6612f * p - 0.4721f * p2 + 0.0671f * p3 - 0.0043f * p4 + 0.000106f * p5;
 }
 
+// One refined layer of the TI model: moduli (depthkernelTI.f90:124-138), TI
+// flattening with radius 6371 (sphere_tdisp96, tregn96.f:774-841) and the
+// sphericity factor with radius 6370 (bldsph, tregn96.f:1301-1383), pushed top
+// to bottom so that each layer is finished as soon as it is formed.
+struct TiFlat {
+  double r0 = 6371.0;   // flattening recurrence
+  double rv = 6370.0;   // sphericity recurrence
+};
+
+__device__ TiLayerDev ti_push(TiFlat& s, float d, float rvp, float rvs, float rrho, bool half) {
+#pragma clang fp contract(off)
+  TiLayerDev t;
+  t.d = d;
+  t.rvp = rvp;
+  t.rvs = rvs;
+  t.rrho = rrho;
+  float v2 = t.rvp * t.rvp, s2 = t.rvs * t.rvs;
+  t.TA = t.rrho * v2;
+  t.TL = t.rrho * s2;
+  float tl2 = 2.0f * t.TL;
+  t.TF = 1.0f * (t.TA - tl2);
+  const double ar = 6371.0;
+  const double e_r = (double)(-2.275f), e_m = (double)(-0.2750f);
+  float td = half ? 1.0f : t.d;
+  double r1 = s.r0 - (double)td;
+  double z0 = ar * log(ar / s.r0);
+  double z1 = ar * log(ar / r1);
+  float tdf = (float)(z1 - z0);
+  double tmp = (ar + ar) / (s.r0 + r1);
+  t.rho = (float)((double)t.rrho * pow(tmp, e_r));
+  double fm = pow(tmp, e_m);
+  t.A = (float)((double)t.TA * fm);
+  t.L = (float)((double)t.TL * fm);
+  t.F = t.TF;
+  t.df = half ? 0.0f : tdf;
+  s.r0 = r1;
+  const double ar2 = 6370.0;
+  double zd = half ? 1.0 : (double)t.df;
+  double rv1 = s.rv * exp(-zd / ar2);
+  double tv = (ar2 + ar2) / (s.rv + rv1);
+  t.vtp = (float)tv;
+  s.rv = rv1;
+  return t;
+}
+
 __global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
                           const float* __restrict__ vpin, const float* __restrict__ rhoin,
                           const float* __restrict__ depz, float minthk0, int L,
@@ -36,6 +81,7 @@ __global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   TiLayerDev* lay = out + (size_t)col * L;
+  TiFlat fs;
   // refined, unflattened layers (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196)
   int k = 0;
   float vs_i = vel[col], vp_i, rho_i;
@@ -63,57 +109,14 @@ __global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
     float den = (float)(2 * ns);
     for (int j = 1; j <= ns && k < L - 1; j++, k++) {
       float w = (float)(2 * j - 1);
-      lay[k].d = newthk;
-      lay[k].rvp = vp_i + w * (vp_n - vp_i) / den;
-      lay[k].rvs = vs_i + w * (vs_n - vs_i) / den;
-      lay[k].rrho = rho_i + w * (rho_n - rho_i) / den;
+      lay[k] = ti_push(fs, newthk, vp_i + w * (vp_n - vp_i) / den, vs_i + w * (vs_n - vs_i) / den,
+                       rho_i + w * (rho_n - rho_i) / den, false);
     }
     vs_i = vs_n;
     vp_i = vp_n;
     rho_i = rho_n;
   }
-  lay[L - 1].d = 0.0f;
-  lay[L - 1].rvp = vp_i;
-  lay[L - 1].rvs = vs_i;
-  lay[L - 1].rrho = rho_i;
-  for (int m = 0; m < L; m++) {
-    TiLayerDev& t = lay[m];
-    float v2 = t.rvp * t.rvp, s2 = t.rvs * t.rvs;
-    t.TA = t.rrho * v2;
-    t.TL = t.rrho * s2;
-    float tl2 = 2.0f * t.TL;
-    t.TF = 1.0f * (t.TA - tl2);
-  }
-  // earth flattening (radius 6371) and sphericity factors (radius 6370)
-  const double ar = 6371.0;
-  const double e_r = (double)(-2.275f), e_m = (double)(-0.2750f);
-  double r0 = ar;
-  for (int i = 0; i < L; i++) {
-    TiLayerDev& t = lay[i];
-    float td = (i == L - 1) ? 1.0f : t.d;
-    double r1 = r0 - (double)td;
-    double z0 = ar * log(ar / r0);
-    double z1 = ar * log(ar / r1);
-    float tdf = (float)(z1 - z0);
-    double tmp = (ar + ar) / (r0 + r1);
-    t.rho = (float)((double)t.rrho * pow(tmp, e_r));
-    double fm = pow(tmp, e_m);
-    t.A = (float)((double)t.TA * fm);
-    t.L = (float)((double)t.TL * fm);
-    t.F = t.TF;
-    t.df = (i == L - 1) ? 0.0f : tdf;
-    r0 = r1;
-  }
-  const double ar2 = 6370.0;
-  r0 = ar2;
-  for (int i = 0; i < L; i++) {
-    TiLayerDev& t = lay[i];
-    double zd = (i == L - 1) ? 1.0 : (double)t.df;
-    double r1 = r0 * exp(-zd / ar2);
-    double tmp = (ar2 + ar2) / (r0 + r1);
-    t.vtp = (float)tmp;
-    r0 = r1;
-  }
+  lay[L - 1] = ti_push(fs, 0.0f, vp_i, vs_i, rho_i, true);
 }
 
 __global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
