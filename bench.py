@@ -195,6 +195,25 @@ def bench_c5(dk, configs, np, torch, chunk=40960):
                 cell_grad_norm=float(acc.norm()))
 
 
+def bench_host_path(dk, configs, np, reps=3):
+    """The Fortran drop-in's host entry points on plain_4096: depthkernel then
+    depthkernelTI through the C ABI with host arrays (inputs copied in, all
+    outputs copied back: the PCIe-inclusive rate)."""
+    W = configs.node_workload("plain_4096")
+    args = (W["vel"], W["periods"], W["depz"], W["minthk"])
+    dk.run_depthkernel(*args, vp=W["vp"], rho=W["rho"])
+    dk.run_depthkernelTI(*args)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dk.run_depthkernel(*args, vp=W["vp"], rho=W["rho"])
+        dk.run_depthkernelTI(*args)
+    dt = (time.perf_counter() - t0) / reps
+    ns = W["vel"].shape[0] * W["vel"].shape[1] * len(W["periods"])
+    return dict(workload="plain_4096", seconds_per_step=dt, solves_per_s=ns / dt,
+                note="two host calls (iso with explicit vp/rho, TI with Brocher), each "
+                     "with its own root search, H2D inputs and D2H outputs")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -208,6 +227,7 @@ def main():
     ap.add_argument("--no-vjp", action="store_true")
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -340,6 +360,9 @@ def main():
     c5 = None
     if not args.no_c5:
         c5 = bench_c5(dk, configs, np, torch)
+    host = None
+    if not args.no_host:
+        host = bench_host_path(dk, configs, np)
 
     cpu = None
     if not args.no_cpu:
@@ -369,7 +392,7 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp, ragged=c3, realizations=c5),
+                    vjp=vjp, ragged=c3, realizations=c5, host_path=host),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 
