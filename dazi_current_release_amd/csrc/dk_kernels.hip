@@ -516,6 +516,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
   int nref = 0;     // multisection rounds of the current bracket
   int nround = 0;   // evaluation rounds of the current column (hang guard)
   double cstar = 0, c1n = 0, omega_n = 1;  // verification round: estimate, next scan start
+  int nver = 0;                            // verification rounds of the current period
 
   auto begin_column = [&]() {
     if (col >= col_end) {
@@ -710,6 +711,7 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
         cstar = cs;
         c1n = cs - onea * dc;
         omega_n = (k + 1 < K) ? twopi / period_of(B, toff, k + 1) : omega;
+        nver = 0;
         st = ST_VER;
       } else if (jf >= 0) {
         fail_from(k);
@@ -751,7 +753,27 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
       hi = vhi;
       flo = vflo;
       fhi = vfhi;
-      if (hi - lo <= tol) {
+      nver++;
+      // a missed window: one Newton step from the window point next to the
+      // root with the window's own slope, verified once more
+      double cnew = -1.0;
+      if (hi - lo > tol && nver == 1) {
+        const int j0 = found ? 0 : DK_NW - 2;  // root below / above the window
+        if (((okm >> j0) & 3ull) == 3ull) {
+          const double xa = x_at(j0), xb = x_at(j0 + 1);
+          const FV fa = fv_at(j0), fb = fv_at(j0 + 1);
+          const int em = max(fa.e, fb.e);
+          const double va = ldexp(fa.m, fa.e - em), vb = ldexp(fb.m, fb.e - em);
+          const double slope = (vb - va) / (xb - xa);
+          const double xn = found ? xa : xb, vn = found ? va : vb;
+          const double cn = xn - vn / slope;
+          if (cn > lo && cn < hi) cnew = cn;
+        }
+      }
+      if (cnew > 0.0) {
+        cstar = cnew;
+        c1n = cnew - onea * dc;
+      } else if (hi - lo <= tol) {
         double root = (signbit(flo.m) != signbit(fhi.m)) ? fv_ratio_root(lo, hi, flo, fhi)
                                                           : 0.5 * (lo + hi);
         if (emit(root)) {
