@@ -955,7 +955,8 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   const double dFdc = e[0].g[0];
   const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
 
-  // ---- sweep 2: adjoint top-down with Dual<3> layer matrices
+  // ---- sweep 2: adjoint top-down with Dual<2> layer matrices in (a, b); the
+  // rho partial comes from the entries' rho powers
   // g = dF/de (true scale g * 2^Eg); indices 0:a 1:b 2:rho
   double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
   int Eg = 0;
@@ -974,31 +975,35 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       // contribution sum_ji e_j dCA_ji g_i and the propagation g <- CA g,
       // reduced entry by entry as the layer matrix is formed
       double acc[3] = {0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      layer_ca_visit<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
-                              dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), (double)l.x,
-                              omega, [&](int j, int i, const Dual<3>& v) {
+      layer_ca_visit<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
+                              dvar<2>((double)l.z, 1), dconst<2>((double)l.w), (double)l.x,
+                              omega, [&](int j, int i, const Dual<2>& v) {
                                 const double w = et[j] * g[i];
                                 acc[0] = fma(w, v.g[0], acc[0]);
                                 acc[1] = fma(w, v.g[1], acc[1]);
-                                acc[2] = fma(w, v.g[2], acc[2]);
+                                acc[2] = fma(w * (double)rho_pow(j, i), v.v, acc[2]);
                                 gn[j] = fma(v.v, g[i], gn[j]);
                               });
+      acc[2] /= (double)l.w;
 #pragma unroll
       for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * sc;
 #pragma unroll
       for (int j = 0; j < 5; j++) g[j] = gn[j];
       Eg += pow2_normalize(g);
     } else {
-      Dual<3> eh[5];
-      halfspace_e<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
-                           dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), omega, eh);
+      Dual<2> eh[5];
+      halfspace_e<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
+                           dvar<2>((double)l.z, 1), dconst<2>((double)l.w), omega, eh);
+      double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-      for (int p = 0; p < 3; p++) {
-        double acc = 0.0;
-#pragma unroll
-        for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
-        part[p] = ldexp(acc, Eg - EF) * sc;
+      for (int i = 0; i < 5; i++) {
+        acc[0] = fma(eh[i].g[0], g[i], acc[0]);
+        acc[1] = fma(eh[i].g[1], g[i], acc[1]);
+        acc[2] = fma((double)half_rho_pow(i) * eh[i].v, g[i], acc[2]);
       }
+      acc[2] /= (double)l.w;
+#pragma unroll
+      for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Eg - EF) * sc;
     }
     // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
     // rho_flat = rho*btp^-2.275
@@ -1188,32 +1193,39 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
 #pragma unroll
             for (int i = 0; i < 5; i++) et[i] = tp[(long)i * nslot];
             const int Ee = (int)tp[(long)5 * nslot];
+            // Dual<3> in (a, b, d); the rho partial from the entries' rho powers
             double acc[4] = {0.0, 0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            layer_ca_visit<Dual<4>, Dual<4>>(
-                dconst<4>(wv), dconst<4>(wv2), dvar<4>((double)l.y, 0), dvar<4>((double)l.z, 1),
-                dvar<4>((double)l.w, 2), dvar<4>((double)l.x, 3), omega,
-                [&](int j, int i, const Dual<4>& v) {
+            layer_ca_visit<Dual<3>, Dual<3>>(
+                dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0), dvar<3>((double)l.z, 1),
+                dconst<3>((double)l.w), dvar<3>((double)l.x, 2), omega,
+                [&](int j, int i, const Dual<3>& v) {
                   const double w = et[j] * g[i];
-#pragma unroll
-                  for (int p = 0; p < 4; p++) acc[p] = fma(w, v.g[p], acc[p]);
+                  acc[0] = fma(w, v.g[0], acc[0]);
+                  acc[1] = fma(w, v.g[1], acc[1]);
+                  acc[2] = fma(w * (double)rho_pow(j, i), v.v, acc[2]);
+                  acc[3] = fma(w, v.g[2], acc[3]);
                   gn[j] = fma(v.v, g[i], gn[j]);
                 });
+            acc[2] /= (double)l.w;
 #pragma unroll
             for (int p = 0; p < 4; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * lane_scale;
 #pragma unroll
             for (int j = 0; j < 5; j++) g[j] = gn[j];
             Eg += pow2_normalize(g);
           } else {
-            Dual<3> eh[5];
-            halfspace_e<Dual<3>>(dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0),
-                                 dvar<3>((double)l.z, 1), dvar<3>((double)l.w, 2), omega, eh);
+            Dual<2> eh[5];
+            halfspace_e<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
+                                 dvar<2>((double)l.z, 1), dconst<2>((double)l.w), omega, eh);
+            double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int p = 0; p < 3; p++) {
-              double acc = 0.0;
-#pragma unroll
-              for (int i = 0; i < 5; i++) acc = fma(eh[i].g[p], g[i], acc);
-              part[p] = ldexp(acc, Eg - EF) * lane_scale;
+            for (int i = 0; i < 5; i++) {
+              acc[0] = fma(eh[i].g[0], g[i], acc[0]);
+              acc[1] = fma(eh[i].g[1], g[i], acc[1]);
+              acc[2] = fma((double)half_rho_pow(i) * eh[i].v, g[i], acc[2]);
             }
+            acc[2] /= (double)l.w;
+#pragma unroll
+            for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Eg - EF) * lane_scale;
           }
           // part = grad_c * dc/d(flattened a, b, rho, d)
           const double bt = (double)btp[m];

@@ -373,6 +373,20 @@ DK_HD void layer_ca_visit(const T& wvno, const T& wvno2, const T& a, const T& b,
   }
 }
 
+// Every entry of the layer matrix is rho^p times a rho-free factor, p =
+// RHO_POW[j][i] (dnka's rho, 1/rho, rho^2, 1/rho^2 factors), so its rho
+// derivative is p * ca[j][i] / rho; likewise the half-space vector with powers
+// HALF_RHO_POW.
+__host__ __device__ constexpr int rho_pow(int j, int i) {
+  constexpr int P[5][5] = {{0, -1, -1, -1, -2},
+                           {1, 0, 0, 0, -1},
+                           {1, 0, 0, 0, -1},
+                           {1, 0, 0, 0, -1},
+                           {2, 1, 1, 1, 0}};
+  return P[j][i];
+}
+__host__ __device__ constexpr int half_rho_pow(int i) { return i == 0 ? 2 : (i == 4 ? 0 : 1); }
+
 template <class T, class D = double>
 DK_HD void layer_ca(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
                     const D& d, double omega, T ca[5][5]) {
