@@ -386,8 +386,9 @@ def main():
         config=dict(workload="plain_4096", columns=ncol, nodes=nz, layers=L, periods=K,
                     outputs="c + dc/dVs, dc/dVp, dc/drho node kernels + Lsen_Gsc (TI)",
                     mode="batched persistent", lanes_per_scan=args.lanes,
-                    cols_per_wg=args.cpw, phase_ms=dict(prep=prep_ms, root_search=fwd_ms,
-                                                        iso_kernels=kern_ms, ti_kernels=ti_ms),
+                    cols_per_wg=args.cpw,
+                    phase_ms=dict(prep=prep_ms, root_search=fwd_ms, iso_kernels=kern_ms,
+                                  ti_kernels=ti_ms),
                     failed_columns=int((status != 0).sum()),
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,

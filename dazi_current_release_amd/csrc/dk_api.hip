@@ -72,6 +72,11 @@ struct Pool {
   size_t n[S_COUNT] = {};
   int ncu = 0;
   hipEvent_t ev[8] = {};
+  // side stream for work that runs beside the caller's stream (the TI model
+  // prep of a stage call, beside the root search), joined back with events;
+  // default priority
+  hipStream_t side = nullptr;
+  hipEvent_t fk[4] = {};
   int nforward = 0;
   bool have_events = false;
   // node geometry of the last call that took depz as a device pointer: the
@@ -98,6 +103,8 @@ int get_pool(Pool** out) {
     HIPCHK(hipGetDeviceProperties(&prop, dev));
     P->ncu = prop.multiProcessorCount;
     for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&P->ev[i]));
+    HIPCHK(hipStreamCreateWithFlags(&P->side, hipStreamNonBlocking));
+    for (int i = 0; i < 4; i++) HIPCHK(hipEventCreateWithFlags(&P->fk[i], hipEventDisableTiming));
   }
   *out = P;
   return 0;
@@ -286,8 +293,36 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
 
+  // every buffer is sized before the first launch, so no error return can
+  // leave side-stream work unjoined
+  void* tape = nullptr;
+  int kern_nthr = 0;
+  if (kern) {
+    kern_nthr = kernel_threads(P, nsolves);
+    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * kern_nthr * 8, &tape))
+      return rc;
+  }
+  // the TI model prep needs only the inputs: it runs on the side stream from
+  // the start of the call
+  void *til = nullptr, *tiscr = nullptr;
+  int ti_nthr = 0;
+  if (lsen) {
+    if (int rc = pool_get(P, S_TI_LAYERS, (size_t)ncol * L * sizeof(dk::TiLayerDev), &til))
+      return rc;
+    ti_nthr = ti_threads(P, nsolves);
+    if (int rc = pool_get(P, S_TI_SCRATCH, (size_t)dk::TI_SCRATCH * L * ti_nthr * 8, &tiscr))
+      return rc;
+  }
   P->have_events = g_timing;
   rec(P, 0, st);
+  if (lsen) {
+    HIPCHK(hipEventRecord(P->fk[0], st));
+    HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
+    HIPCHK(dk::launch_prep_ti(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                              d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
+                              (dk::TiLayerDev*)til, P->side));
+    HIPCHK(hipEventRecord(P->fk[2], P->side));
+  }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
                                d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
                                (double4*)layers, (double*)btp, (double*)cc, (float*)betmx,
@@ -320,25 +355,14 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     O.dvs = svs;
     O.dvp = svp;
     O.drho = srho;
-    int nthr = kernel_threads(P, nsolves);
-    void* tape;
-    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * nthr * 8, &tape))
-      return rc;
-    HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
+    HIPCHK(dk::launch_kernels(B, O, (double*)tape, kern_nthr, st));
   }
   rec(P, 3, st);
   if (lsen) {
-    void *til, *scr;
-    if (int rc = pool_get(P, S_TI_LAYERS, (size_t)ncol * L * sizeof(dk::TiLayerDev), &til))
-      return rc;
-    int nthr = ti_threads(P, nsolves);
-    if (int rc = pool_get(P, S_TI_SCRATCH, (size_t)dk::TI_SCRATCH * L * nthr * 8, &scr))
-      return rc;
-    HIPCHK(dk::launch_prep_ti(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                              d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                              (dk::TiLayerDev*)til, st));
+    // join the TI model prep, then the TI kernels (the roots are ready)
+    HIPCHK(hipStreamWaitEvent(st, P->fk[2], 0));
     HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
-                         (const int*)nsubd, lsen, (double*)scr, nthr, nullptr, st));
+                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, nullptr, st));
   }
   rec(P, 4, st);
   return 0;

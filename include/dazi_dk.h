@@ -198,8 +198,9 @@ int32_t dazi_dk_effective_dev(const dazi_dk_desc* desc, const float* vel, const 
 int32_t dazi_dk_pool_bytes(int64_t* out);
 
 /* Event timings (ms) of the last call's phases on its stream:
- * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels,
- * [4] number of root-search launches.  Waits for those events.  Returns the
+ * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels
+ * (their model prep runs on a second stream beside the root search), [4]
+ * number of root-search launches.  Waits for those events.  Returns the
  * number of values written. */
 int32_t dazi_dk_last_timings(double* out, int32_t n);
 
