@@ -61,6 +61,7 @@ enum Slot {
   S_H_SRHO,
   S_TI_LAYERS,
   S_TI_SCRATCH,
+  S_TI_QUEUE,
   S_TISTAT,
   S_H_LSEN,
   S_COUNT
@@ -304,14 +305,17 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   // the TI model prep needs only the inputs: it runs on the side stream from
   // the start of the call
-  void *til = nullptr, *tiscr = nullptr;
+  void *til = nullptr, *tiscr = nullptr, *tiq = nullptr;
   int ti_nthr = 0;
   if (lsen) {
+    if (nsolves >= INT32_MAX)
+      return set_err(DAZI_DK_ELIMIT, "TI splice: %lld solves (limit 2^31 - 1)", (long long)nsolves);
     if (int rc = pool_get(P, S_TI_LAYERS, (size_t)ncol * L * sizeof(dk::TiLayerDev), &til))
       return rc;
     ti_nthr = ti_threads(P, nsolves);
     if (int rc = pool_get(P, S_TI_SCRATCH, (size_t)dk::TI_SCRATCH * L * ti_nthr * 8, &tiscr))
       return rc;
+    if (int rc = pool_get(P, S_TI_QUEUE, (size_t)(1 + nsolves) * 4, &tiq)) return rc;
   }
   P->have_events = g_timing;
   rec(P, 0, st);
@@ -320,7 +324,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
     HIPCHK(dk::launch_prep_ti(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
                               d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                              (dk::TiLayerDev*)til, P->side));
+                              (dk::TiLayerDev*)til, (int*)tiq, P->side));
     HIPCHK(hipEventRecord(P->fk[2], P->side));
   }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
@@ -362,7 +366,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     // join the TI model prep, then the TI kernels (the roots are ready)
     HIPCHK(hipStreamWaitEvent(st, P->fk[2], 0));
     HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
-                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, nullptr, st));
+                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, (int*)tiq, st));
   }
   rec(P, 4, st);
   return 0;

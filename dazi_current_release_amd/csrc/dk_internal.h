@@ -58,15 +58,19 @@ struct TiLayerDev {
   float df, A, L, F, rho, vtp;
 };
 
-// Doubles of TI scratch per layer and thread.
+// Doubles of TI scratch per layer and thread: real solves, complex-pair solves.
 constexpr int TI_SCRATCH = 9;
+constexpr int TI_SCRATCH_CX = 14;
 
+// k_prep_ti also clears queue[0]; launch_ti runs k_ti, which appends the solves
+// with a complex-conjugate eigenvalue pair to queue[1..], then k_ti_cx over
+// them (queue holds 1 + ncol * K ints).
 hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                          const float* depz, float minthk0, int L, TiLayerDev* out,
+                          const float* depz, float minthk0, int L, TiLayerDev* out, int* queue,
                           hipStream_t st);
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* tistat, hipStream_t st);
+                     double* scratch, int nthr, int* queue, hipStream_t st);
 
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
                              const float* depz, float minthk0, int L, double4* layers, double* btp,

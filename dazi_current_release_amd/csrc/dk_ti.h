@@ -55,7 +55,21 @@ DK_HD cx operator/(cx a, cx b) {
   return cx{(a.r * t + a.i) / d, (a.i * t - a.r) / d};
 }
 DK_HD cx operator/(cx a, double s) { return cx{a.r / s, a.i / s}; }
+DK_HD cx operator/(double s, cx b) { return cx{s, 0.0} / b; }
 DK_HD double cabs_(cx a) { return hypot(a.r, a.i); }
+// Scalar-generic helpers: the layer algebra below is written once for real
+// (double) and complex (cx) eigenvector components.
+DK_HD cx mk(cx a) { return a; }
+DK_HD double re(double a) { return a; }
+DK_HD double re(cx a) { return a.r; }
+DK_HD double mad(double a, double b, double c) { return fma(a, b, c); }
+DK_HD cx mad(cx a, cx b, cx c) { return a * b + c; }
+template <class T>
+DK_HD T lift(double v);
+template <>
+DK_HD double lift<double>(double v) { return v; }
+template <>
+DK_HD cx lift<cx>(double v) { return cx{v, 0.0}; }
 DK_HD cx csqrt_(cx z) {
   // principal square root; a zero imaginary part takes its sign (branch cut)
   if (z.r == 0.0 && z.i == 0.0) return cx{0.0, z.i};
@@ -75,15 +89,77 @@ struct Layer {  // flattened TI layer (float32 values of the reference, as doubl
   double d, A, C, F, L, N, rho;
 };
 
+// Coefficients of gettiegn's characteristic equation L^2 - bb L + cc = 0 in
+// L = nu^2 (tregn96.f:3212-3244), with the eigenvector algebra's a..f.
+struct Quad {
+  double a, b, c, d, e, f, bb, cc;
+};
+
+DK_HD Quad quad(const Layer& m, double omega, double wvno) {
+  const double om2 = omega * omega, wvno2 = wvno * wvno;
+  const double iC = 1.0 / m.C, iL = 1.0 / m.L;
+  Quad q;
+  q.a = wvno * m.F * iC;
+  q.b = iC;
+  q.c = -m.rho * om2 + wvno2 * (m.A - m.F * m.F * iC);
+  q.d = -wvno;
+  q.e = iL;
+  q.f = -m.rho * om2;
+  const double ddef = wvno2 - m.rho * om2 * iL;
+  const double aabc = wvno2 * m.A * iC - m.rho * om2 * iC;
+  q.bb = 2.0 * (q.a * q.d) + q.e * q.c + q.f * q.b;
+  q.cc = ddef * aabc;
+  return q;
+}
+
+// Eigenvector components from the two roots (gettiegn, tregn96.f:3301-3340),
+// each column normalised so that its leading component is k.  T is double or cx.
+template <class T>
+DK_HD void eigvec(const Quad& q, double wvno, T L1, T L2, T x[8]) {
+  const double bcaa = q.b * q.c + q.a * q.a, ddef2 = q.d * q.d + q.e * q.f;
+  T x12 = lift<T>(q.b * q.d - q.a * q.e);
+  T x22 = q.b * L2 - lift<T>(q.e * bcaa);
+  T x32 = L2 - lift<T>(q.a * q.d + q.c * q.e);
+  T x42 = -(q.a * L2) + lift<T>(q.d * bcaa);
+  T x11 = -(q.e * L1) + lift<T>(q.b * ddef2);
+  T x21 = lift<T>(q.b * q.d - q.a * q.e);
+  T x31 = q.d * L1 - lift<T>(q.a * ddef2);
+  T x41 = -(L1 - lift<T>(q.a * q.d) - lift<T>(q.b * q.f));
+  if (wvno != 0.0) {
+    T z = wvno / x11;
+    x11 = x11 * z;
+    x21 = x21 * z;
+    x31 = x31 * z;
+    x41 = x41 * z;
+    z = wvno / x22;
+    x12 = x12 * z;
+    x22 = x22 * z;
+    x32 = x32 * z;
+    x42 = x42 * z;
+  }
+  x[0] = x11;
+  x[1] = x21;
+  x[2] = x31;
+  x[3] = x41;
+  x[4] = x12;
+  x[5] = x22;
+  x[6] = x32;
+  x[7] = x42;
+}
+
 // Eigen-solution of one layer (gettiegn) at real (omega, k).  The squared
-// vertical wavenumbers L1 (P) and L2 (SV) are the roots of a real quadratic;
-// for the models depthkernelTI builds (F = A - 2L before flattening, so the
-// flattened layers are within a fraction of a percent of isotropy) its
-// discriminant is positive, both roots are real, and every quantity of the up
-// and down sweeps is real: nu = sqrt(L) is real (evanescent) or i*sqrt(-L)
-// (oscillatory).  ok = 0 flags a negative discriminant (a complex pair, which
-// only strongly anisotropic layers produce).
+// vertical wavenumbers L1 (P) and L2 (SV) are the roots of a real quadratic.
+// When its discriminant is non-negative, both roots are real and every
+// quantity of the up and down sweeps is real: nu = sqrt(L) is real
+// (evanescent) or i*sqrt(-L) (oscillatory).  ok = 0 flags a negative
+// discriminant: L1, L2 are then a complex-conjugate pair (near-isotropic
+// layers where the P and SV wavenumbers nearly coincide), which EigC handles.
+struct Hyp;
+struct HypC;
+
 struct Eig {
+  using T = double;   // eigenvector component type
+  using H = Hyp;      // its layer phase functions
   double L1, L2;      // nu_P^2, nu_SV^2
   double nP, nS;      // |nu_P|, |nu_SV|
   int iP, iS;         // nu imaginary
@@ -93,18 +169,9 @@ struct Eig {
 };
 
 DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
-  const double om2 = omega * omega, wvno2 = wvno * wvno;
-  const double iC = 1.0 / m.C, iL = 1.0 / m.L;
-  const double a = wvno * m.F * iC;
-  const double b = iC;
-  const double c = -m.rho * om2 + wvno2 * (m.A - m.F * m.F * iC);
-  const double d = -wvno;
-  const double e = iL;
-  const double f = -m.rho * om2;
-  const double ddef = wvno2 - m.rho * om2 * iL;
-  const double aabc = wvno2 * m.A * iC - m.rho * om2 * iC;
-  const double bb = 2.0 * (a * d) + e * c + f * b;
-  const double cc = ddef * aabc;
+  const double wvno2 = wvno * wvno;
+  const Quad q = quad(m, omega, wvno);
+  const double bb = q.bb, cc = q.cc;
   const double disc = bb * bb - 4.0 * cc;
   o->ok = disc >= 0.0;
   const double s = sqrt(fmax(disc, 0.0));
@@ -121,50 +188,105 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
   o->iS = L2 < 0.0;
   o->nP = sqrt(fabs(L1));
   o->nS = sqrt(fabs(L2));
-  const double bcaa = b * c + a * a, ddef2 = d * d + e * f;
-  double x12 = b * d - a * e;
-  double x22 = b * L2 - e * bcaa;
-  double x32 = L2 - (a * d + c * e);
-  double x42 = -(a * L2) + d * bcaa;
-  double x11 = -(e * L1) + b * ddef2;
-  double x21 = b * d - a * e;
-  double x31 = d * L1 - a * ddef2;
-  double x41 = -(L1 - a * d - b * f);
-  if (wvno != 0.0) {
-    double z = wvno / x11;
-    x11 *= z;
-    x21 *= z;
-    x31 *= z;
-    x41 *= z;
-    z = wvno / x22;
-    x12 *= z;
-    x22 *= z;
-    x32 *= z;
-    x42 *= z;
-  }
-  o->x11 = x11;
-  o->x21 = x21;
-  o->x31 = x31;
-  o->x41 = x41;
-  o->x12 = x12;
-  o->x22 = x22;
-  o->x32 = x32;
-  o->x42 = x42;
-  o->NP = x11 * x41 - x21 * x31;
-  o->NSV = x12 * x42 - x22 * x32;
+  double x[8];
+  eigvec(q, wvno, L1, L2, x);
+  o->x11 = x[0];
+  o->x21 = x[1];
+  o->x31 = x[2];
+  o->x41 = x[3];
+  o->x12 = x[4];
+  o->x22 = x[5];
+  o->x32 = x[6];
+  o->x42 = x[7];
+  o->NP = x[0] * x[3] - x[1] * x[2];
+  o->NSV = x[4] * x[7] - x[5] * x[6];
 }
 
 // Complex vertical wavenumbers of the eigen-solution.
 DK_HD cx nu_P(const Eig& g) { return g.iP ? cx{0.0, g.nP} : cx{g.nP, 0.0}; }
 DK_HD cx nu_S(const Eig& g) { return g.iS ? cx{0.0, g.nS} : cx{g.nS, 0.0}; }
+// 1/(2 nu N): nu real -> real, nu = i|nu| -> -i/(2|nu|N)
+DK_HD cx inv2nuN_P(const Eig& g) {
+  return g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
+}
+DK_HD cx inv2nuN_S(const Eig& g) {
+  return g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
+}
+DK_HD double half_inv(double N) { return 0.5 / N; }
+
+// General complex eigen-solution: the form tregn96 evaluates in complex*16
+// throughout.  Layers with real roots take exactly Eig's values (same
+// branches); a complex-conjugate pair takes L1 = (bb + i sqrt(-disc))/2,
+// L2 = cc/L1, the P/SV labels by the smaller |k^2 - L|, and the principal
+// square roots (Re nu > 0) (tregn96.f:3246-3300).
+struct EigC {
+  using T = cx;
+  using H = HypC;
+  cx rp, rs;  // nu_P, nu_SV
+  cx x11, x21, x31, x41, x12, x22, x32, x42;
+  cx NP, NSV;
+  int ok;     // always 1
+};
+
+DK_HD void eigen(const Layer& m, double omega, double wvno, EigC* o) {
+  Eig g;
+  eigen(m, omega, wvno, &g);
+  o->ok = 1;
+  if (g.ok) {
+    o->rp = nu_P(g);
+    o->rs = nu_S(g);
+    o->x11 = mk(g.x11);
+    o->x21 = mk(g.x21);
+    o->x31 = mk(g.x31);
+    o->x41 = mk(g.x41);
+    o->x12 = mk(g.x12);
+    o->x22 = mk(g.x22);
+    o->x32 = mk(g.x32);
+    o->x42 = mk(g.x42);
+    o->NP = mk(g.NP);
+    o->NSV = mk(g.NSV);
+    return;
+  }
+  const double wvno2 = wvno * wvno;
+  const Quad q = quad(m, omega, wvno);
+  const double disc = q.bb * q.bb - 4.0 * q.cc;
+  cx L1 = cx{0.5 * q.bb, 0.5 * sqrt(fmax(-disc, 0.0))};
+  cx L2 = q.cc / L1;
+  if (cabs_(mk(wvno2) - L2) < cabs_(mk(wvno2) - L1)) {
+    cx t = L1;
+    L1 = L2;
+    L2 = t;
+  }
+  o->rp = csqrt_(L1);
+  o->rs = csqrt_(L2);
+  if (o->rp.r < 0.0) o->rp = -o->rp;
+  if (o->rs.r < 0.0) o->rs = -o->rs;
+  cx x[8];
+  eigvec(q, wvno, L1, L2, x);
+  o->x11 = x[0];
+  o->x21 = x[1];
+  o->x31 = x[2];
+  o->x41 = x[3];
+  o->x12 = x[4];
+  o->x22 = x[5];
+  o->x32 = x[6];
+  o->x42 = x[7];
+  o->NP = x[0] * x[3] - x[1] * x[2];
+  o->NSV = x[4] * x[7] - x[5] * x[6];
+}
+
+DK_HD cx nu_P(const EigC& g) { return g.rp; }
+DK_HD cx nu_S(const EigC& g) { return g.rs; }
+DK_HD cx inv2nuN_P(const EigC& g) { return 0.5 / (g.rp * g.NP); }
+DK_HD cx inv2nuN_S(const EigC& g) { return 0.5 / (g.rs * g.NSV); }
+DK_HD cx half_inv(cx N) { return 0.5 / N; }
 
 // Eigenvector matrix E (columns P+, S+, P-, S-) and its inverse G (evalg).
-DK_HD void eig_matrices(const Eig& g, cx E[4][4], cx G[4][4]) {
+template <class G_>
+DK_HD void eig_matrices(const G_& g, cx E[4][4], cx G[4][4]) {
   const cx rp = nu_P(g), rs = nu_S(g);
-  // 1/(2 nu N): nu real -> real, nu = i|nu| -> -i/(2|nu|N)
-  const cx iP = g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
-  const cx iS = g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
-  const double hP = 0.5 / g.NP, hS = 0.5 / g.NSV;  // nu/(2 nu N)
+  const cx iP = inv2nuN_P(g), iS = inv2nuN_S(g);
+  const auto hP = half_inv(g.NP), hS = half_inv(g.NSV);  // nu/(2 nu N)
   G[0][0] = mk(g.x41 * hP);
   G[1][0] = g.x42 * iS;
   G[2][0] = mk(g.x41 * hP);
@@ -232,6 +354,31 @@ DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
   hyp_wave(g.iS, g.nS, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex);
 }
 
+// The same for complex nu (varsv's general form, tregn96.f:3363-3440): with
+// p = nu d, cosh p, nu sinh p and sinh p / nu scaled by e^{-Re p}.
+struct HypC {
+  cx cosp, rsinp, sinpr, cossv, rsinsv, sinsvr;
+  double pex, svex;
+};
+
+DK_HD void hyp_wave(cx n, double dm, cx* co, cx* rsi, cx* sir, double* ex) {
+  const cx p = n * dm;
+  double s, c;
+  sincos(p.i, &s, &c);
+  const cx ep = cx{0.5 * c, 0.5 * s}, em = cx{0.5 * c, -0.5 * s};
+  const double fac = (p.r < 15.0) ? exp(-2.0 * p.r) : 0.0;
+  const cx sh = ep - fac * em;
+  *co = ep + fac * em;
+  *rsi = n * sh;
+  *sir = (fabs(p.r) < 1.0e-5 && cabs_(n) < 1.0e-5) ? mk(dm) : sh / n;
+  *ex = p.r;
+}
+
+DK_HD void hyper(const EigC& g, double dm, HypC* h) {
+  hyp_wave(g.rp, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex);
+  hyp_wave(g.rs, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex);
+}
+
 // Per wave type, H = [u v~] M [a~ b~]^T / N with
 //   P: u=(x11,0,x31,0) v~=(0,x21,0,x41) a~=(x41,0,-x21,0) b~=(0,-x31,0,x11),
 //      M = [[cosp, sinpr],[rsinp, cosp]]
@@ -240,12 +387,14 @@ DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
 // u and v~ have disjoint supports (rows 0,2 and 1,3), as have a~ and b~
 // (columns 0,2 and 1,3), so H(i,j) = w_i * M(i%2, j%2) * z_j / N with
 // w = (x11, x21, x31, x41) and z = (x41, -x31, -x21, x11) for P.
+template <class T>
 struct Wave {
-  double w[4], z[4];
-  double m00, m01, m10;  // divided by N (m11 = m00)
+  T w[4], z[4];
+  T m00, m01, m10;  // divided by N (m11 = m00)
 };
 
-DK_HD void waves(const Eig& g, const Hyp& h, double sP, double sS, Wave* P, Wave* S) {
+template <class G_, class H_, class T = typename G_::T>
+DK_HD void waves(const G_& g, const H_& h, double sP, double sS, Wave<T>* P, Wave<T>* S) {
   P->w[0] = g.x11;
   P->w[1] = g.x21;
   P->w[2] = g.x31;
@@ -262,7 +411,7 @@ DK_HD void waves(const Eig& g, const Hyp& h, double sP, double sS, Wave* P, Wave
   S->z[1] = -g.x32;
   S->z[2] = -g.x22;
   S->z[3] = g.x12;
-  const double iNP = sP / g.NP, iNS = sS / g.NSV;
+  const T iNP = sP / g.NP, iNS = sS / g.NSV;
   P->m00 = h.cosp * iNP;
   P->m01 = h.sinpr * iNP;
   P->m10 = h.rsinp * iNP;
@@ -271,20 +420,23 @@ DK_HD void waves(const Eig& g, const Hyp& h, double sP, double sS, Wave* P, Wave
   S->m10 = h.sinsvr * iNS;
 }
 
-DK_HD double wave_entry(const Wave& w, int i, int j) {
-  const double m = ((i & 1) == 0) ? (((j & 1) == 0) ? w.m00 : w.m01)
-                                  : (((j & 1) == 0) ? w.m10 : w.m00);
+template <class T>
+DK_HD T wave_entry(const Wave<T>& w, int i, int j) {
+  const T m = ((i & 1) == 0) ? (((j & 1) == 0) ? w.m00 : w.m01)
+                             : (((j & 1) == 0) ? w.m10 : w.m00);
   return w.w[i] * m * w.z[j];
 }
 
-// Real layer propagator (hska), P part scaled by sP and S part by sS.
-DK_HD void propagator(const Eig& g, const Hyp& h, double sP, double sS, double H[4][4]) {
-  Wave P, S;
+// Real layer propagator (hska: the real part of the complex one), P part
+// scaled by sP and S part by sS.
+template <class G_, class H_>
+DK_HD void propagator(const G_& g, const H_& h, double sP, double sS, double H[4][4]) {
+  Wave<typename G_::T> P, S;
   waves(g, h, sP, sS, &P, &S);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) H[i][j] = wave_entry(P, i, j) + wave_entry(S, i, j);
+    for (int j = 0; j < 4; j++) H[i][j] = re(wave_entry(P, i, j) + wave_entry(S, i, j));
 }
 
 // Minor index pairs in the order (12),(13),(14),(23),(24),(34); the reduced
@@ -298,12 +450,13 @@ __host__ __device__ constexpr int RED(int r) { return r < 3 ? r : r + 1; }
 // by e^{-(pex+svex)} (dnka_tregn): pure-wave compounds
 // dfac (u^v~)(a~^b~)^T / N^2 plus the mixed terms of H_P and H_S; row (14)
 // absorbs (23) = -(14).  CA is formed column by column and never stored.
-DK_HD void compound_step(const Eig& g, const Hyp& h, double e[5]) {
+template <class G_, class H_, class T = typename G_::T>
+DK_HD void compound_step(const G_& g, const H_& h, T e[5]) {
   const double ex = h.pex + h.svex;
   const double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
-  Wave P, S;
+  Wave<T> P, S;
   waves(g, h, 1.0, 1.0, &P, &S);
-  double HP[4][4], HS[4][4];
+  T HP[4][4], HS[4][4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -313,15 +466,15 @@ DK_HD void compound_step(const Eig& g, const Hyp& h, double e[5]) {
     }
   // (u^v~) and (a~^b~) of each wave: u, v~ = w on even / odd rows, a~, b~ = z
   // on even / odd columns
-  const double iNP2 = dfac / (g.NP * g.NP), iNS2 = dfac / (g.NSV * g.NSV);
-  double uvP[6], uvS[6], abP[6], abS[6];
+  const T iNP2 = dfac / (g.NP * g.NP), iNS2 = dfac / (g.NSV * g.NSV);
+  T uvP[6], uvS[6], abP[6], abS[6];
 #pragma unroll
   for (int r = 0; r < 6; r++) {
     const int i = PI0(r), j = PI1(r);
-    auto wedge = [&](const double* w, int i, int j) {
+    auto wedge = [&](const T* w, int i, int j) {
       // u = w on even indices, v~ = w on odd indices
-      const double ui = (i & 1) ? 0.0 : w[i], uj = (j & 1) ? 0.0 : w[j];
-      const double vi = (i & 1) ? w[i] : 0.0, vj = (j & 1) ? w[j] : 0.0;
+      const T ui = (i & 1) ? T{} : w[i], uj = (j & 1) ? T{} : w[j];
+      const T vi = (i & 1) ? w[i] : T{}, vj = (j & 1) ? w[j] : T{};
       return ui * vj - uj * vi;
     };
     uvP[r] = wedge(P.w, i, j) * iNP2;
@@ -329,19 +482,19 @@ DK_HD void compound_step(const Eig& g, const Hyp& h, double e[5]) {
     abP[r] = wedge(P.z, i, j);
     abS[r] = wedge(S.z, i, j);
   }
-  double out[5];
+  T out[5];
 #pragma unroll
   for (int sc = 0; sc < 5; sc++) {
     const int s = RED(sc), k = PI0(s), l = PI1(s);
-    double acc = 0.0;
+    T acc{};
 #pragma unroll
     for (int rr = 0; rr < 6; rr++) {
       const int i = PI0(rr), j = PI1(rr);
-      const double v = uvP[rr] * abP[s] + uvS[rr] * abS[s] + HP[i][k] * HS[j][l] -
-                       HP[i][l] * HS[j][k] + HS[i][k] * HP[j][l] - HS[i][l] * HP[j][k];
+      const T v = uvP[rr] * abP[s] + uvS[rr] * abS[s] + HP[i][k] * HS[j][l] -
+                  HP[i][l] * HS[j][k] + HS[i][k] * HP[j][l] - HS[i][l] * HP[j][k];
       // reduced rows: 0,1,2 -> pairs 0,1,2; 3,4 -> pairs 4,5; row 2 minus pair 3
-      const double coef = (rr == 3) ? -e[2] : (rr < 3 ? e[rr] : e[rr - 1]);
-      acc = fma(coef, v, acc);
+      const T coef = (rr == 3) ? -e[2] : (rr < 3 ? e[rr] : e[rr - 1]);
+      acc = mad(coef, v, acc);
     }
     out[sc] = acc;
   }
@@ -349,8 +502,10 @@ DK_HD void compound_step(const Eig& g, const Hyp& h, double e[5]) {
   for (int q = 0; q < 5; q++) e[q] = out[q];
 }
 
-// Half-space radiation constraint: minors of rows P+ and S+ of G (evalg gbr).
-DK_HD void halfspace_constraint(const Eig& g, double cd[5]) {
+// Half-space radiation constraint: minors of rows P+ and S+ of G (evalg gbr),
+// real parts as up takes them (tregn96.f:1926-1930).
+template <class G_, class T = typename G_::T>
+DK_HD void halfspace_constraint(const G_& g, T cd[5]) {
   cx E[4][4], G[4][4];
   eig_matrices(g, E, G);
   cx CG[6];
@@ -358,11 +513,11 @@ DK_HD void halfspace_constraint(const Eig& g, double cd[5]) {
     int i = PI0(r), j = PI1(r);
     CG[r] = G[0][i] * G[1][j] - G[0][j] * G[1][i];
   }
-  cd[0] = CG[0].r;
-  cd[1] = CG[1].r;
-  cd[2] = CG[2].r;
-  cd[3] = CG[4].r;
-  cd[4] = CG[5].r;
+  cd[0] = lift<T>(CG[0].r);
+  cd[1] = lift<T>(CG[1].r);
+  cd[2] = lift<T>(CG[2].r);
+  cd[3] = lift<T>(CG[4].r);
+  cd[4] = lift<T>(CG[5].r);
 }
 
 // ffunc, gfunc, h1func, h2func (tregn96.f:1385-1473)
@@ -399,13 +554,13 @@ DK_HD cx h2func(cx nua, cx nub, double dm) {
 // for even rows and -1 for odd rows, and every integral pairing rows of equal
 // parity, the sum over the ten wave-pair products of intijr collapses to
 //   Re(a_i a_j A_s + b_i b_j B_s + (a_i b_j + b_i a_j) C_s).
-DK_HD void layer_integrals(const Eig& g, double dm, const double yt[4], const double yb[4],
+template <class G_>
+DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const double yb[4],
                            int half, double out[6]) {
   const cx ra = nu_P(g), rb = nu_S(g);
-  // rows of G = E^-1: 1/(2 nu N) (real or imaginary) and nu/(2 nu N) = 1/(2N)
-  const cx iP = g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
-  const cx iS = g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
-  const double hP = 0.5 / g.NP, hS = 0.5 / g.NSV;
+  // rows of G = E^-1: 1/(2 nu N) and nu/(2 nu N) = 1/(2N)
+  const cx iP = inv2nuN_P(g), iS = inv2nuN_S(g);
+  const auto hP = half_inv(g.NP), hS = half_inv(g.NSV);
   // amplitudes: up-going P, SV from yb; down-going P, SV from yt
   const cx k0 = mk(hP * (g.x41 * yb[0] - g.x21 * yb[2])) + iP * (g.x11 * yb[3] - g.x31 * yb[1]);
   const cx k1 = iS * (g.x42 * yb[0] - g.x22 * yb[2]) + mk(hS * (g.x12 * yb[3] - g.x32 * yb[1]));
@@ -451,6 +606,15 @@ DK_HD double norm5(double e[5]) {
   int k;
   frexp(t, &k);
   for (int i = 0; i < 5; i++) e[i] = ldexp(e[i], -k);
+  return (double)k * 0.69314718055994530942;
+}
+DK_HD double norm5(cx e[5]) {
+  double t = 0.0;
+  for (int i = 0; i < 5; i++) t = fmax(t, cabs_(e[i]));
+  if (!(t >= 1.0e-40) || !isfinite(t)) return 0.0;
+  int k;
+  frexp(t, &k);
+  for (int i = 0; i < 5; i++) e[i] = cx{ldexp(e[i].r, -k), ldexp(e[i].i, -k)};
   return (double)k * 0.69314718055994530942;
 }
 DK_HD double norm4(double e[4]) {
@@ -507,30 +671,53 @@ DK_HD void flatten(int L, const float* thk, const float* TA, const float* TL, co
 }
 
 // ----------------------------------------------------------------- one (column, period)
-// Scratch per layer and thread (element (m, q) at scratch[(m * 9 + q) * stride]):
-// the up-sweep compound vector (5) and its accumulated log scale (1), and the
-// unnormalised kernel factors facah, facbv, facn (3).  The interface
-// eigenfunction lives in registers (each layer needs its top and bottom value).
-enum { Q_CD = 0, Q_EX = 5, Q_F = 6, Q_N = 9 };
+// Scratch per layer and thread (element (m, q) at scratch[(m * N + q) * stride]):
+// the up-sweep compound vector (5 values of T, real and imaginary parts
+// adjacent for cx) and its accumulated log scale (1), and the unnormalised
+// kernel factors facah, facbv, facn (3).  The interface eigenfunction lives in
+// registers (each layer needs its top and bottom value).
+template <class T>
+struct Slots {
+  static constexpr int W = sizeof(T) / sizeof(double);
+  static constexpr int CD = 0, EX = 5 * W, F = 5 * W + 1, N = 5 * W + 4;
+};
+enum { Q_N = Slots<double>::N, Q_NC = Slots<cx>::N };
+// solve<Eig> runs in real arithmetic and returns -2, before any emit, when a
+// layer's eigenvalues form a complex-conjugate pair; solve<EigC> takes every
+// layer in complex arithmetic (the same values where the roots are real).
 // emit(m, dcdah, dcdbv, dcdn) receives, top layer first, tregn96's kernels after
 // the sphericity back-mapping, rounded to float as chksiz does.  lay(m) returns
-// the flattened Layer, vtpf(m) the float sphericity factor.  Returns 0, or -1
-// when the surface minor vanishes or a layer's eigenvalues form a complex pair.
-template <class LayerFn, class VtpFn, class EmitFn>
+// the flattened Layer, vtpf(m) the float sphericity factor.  Returns 0, -1 when
+// the surface minor vanishes, or -2 as above.
+template <class G_, class LayerFn, class VtpFn, class EmitFn>
 DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* scratch,
                 long stride, EmitFn emit) {
-  auto S = [&](int m, int q) -> double& { return scratch[(long)(m * Q_N + q) * stride]; };
+  using T = typename G_::T;
+  using Q = Slots<T>;
+  auto S = [&](int m, int q) -> double& { return scratch[(long)(m * Q::N + q) * stride]; };
+  auto put_cd = [&](int m, const T* e) {
+    for (int q = 0; q < 5; q++) {
+      if constexpr (Q::W == 1) {
+        S(m, Q::CD + q) = re(e[q]);
+      } else {
+        S(m, Q::CD + 2 * q) = re(e[q]);
+        S(m, Q::CD + 2 * q + 1) = e[q].i;
+      }
+    }
+  };
+  // real part of compound entry q at interface m (svfunc reads real parts)
+  auto cd_re = [&](int m, int q) -> double { return S(m, Q::CD + Q::W * q); };
   const double wvno = omega / c, wvno2 = wvno * wvno, om2 = omega * omega;
-  Eig g;
-  Hyp h;
+  G_ g;
+  typename G_::H h;
   int ok = 1;
   // ---- up: compound constraint vectors
-  double cd[5];
+  T cd[5];
   eigen(lay(L - 1), omega, wvno, &g);
   ok &= g.ok;
   halfspace_constraint(g, cd);
-  for (int q = 0; q < 5; q++) S(L - 1, Q_CD + q) = cd[q];
-  S(L - 1, Q_EX) = 0.0;
+  put_cd(L - 1, cd);
+  S(L - 1, Q::EX) = 0.0;
   double exsum = 0.0;
   for (int m = L - 2; m >= 0; m--) {
     const Layer ly = lay(m);
@@ -540,15 +727,21 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     compound_step(g, h, cd);
     const double exn = norm5(cd);
     exsum = exsum + h.pex + h.svex + exn;
-    for (int q = 0; q < 5; q++) S(m, Q_CD + q) = cd[q];
-    S(m, Q_EX) = exsum;
+    put_cd(m, cd);
+    S(m, Q::EX) = exsum;
   }
-  if (!ok) return -1;
-  // ---- surface eigenfunction
-  const double exe1 = S(0, Q_EX);
-  const double f1213 = -S(0, Q_CD + 1);
+  if (!ok) return -2;
+  // ---- surface eigenfunction: ur = Re(cd(1,3)/cd(1,2)), f1213 = -Re cd(1,2)
+  const double exe1 = S(0, Q::EX);
+  const double f1213 = -cd_re(0, 1);
   if (!(f1213 != 0.0) || !isfinite(f1213)) return -1;
-  double yt[4] = {S(0, Q_CD + 2) / S(0, Q_CD + 1), 1.0, 0.0, 0.0};
+  double ur0;
+  if constexpr (Q::W == 1) {
+    ur0 = S(0, Q::CD + 2) / S(0, Q::CD + 1);
+  } else {
+    ur0 = (cx{S(0, Q::CD + 4), S(0, Q::CD + 5)} / cx{S(0, Q::CD + 2), S(0, Q::CD + 3)}).r;
+  }
+  double yt[4] = {ur0, 1.0, 0.0, 0.0};
   // Q dispersion of c (gammap: Qp = 150, Qs = 50, fref = 1 Hz), accumulated
   // over the layers and normalised at the end; used only by the sphericity
   // factor tm below
@@ -584,14 +777,14 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       for (int i = 0; i < 4; i++) vv[i] = a0[i];
       // contraction at interface m+1 (svfunc)
       const int n = m + 1;
-      double cd1 = S(n, Q_CD + 0), cd2 = S(n, Q_CD + 1), cd3 = S(n, Q_CD + 2), cd4 = -S(n, Q_CD + 2),
-             cd5 = S(n, Q_CD + 3), cd6 = S(n, Q_CD + 4);
+      double cd1 = cd_re(n, 0), cd2 = cd_re(n, 1), cd3 = cd_re(n, 2), cd4 = -cd_re(n, 2),
+             cd5 = cd_re(n, 3), cd6 = cd_re(n, 4);
       double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
       double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
       double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
       double uu3 = tz1 * cd5 - tz2 * cd3 + tz4 * cd1;
       double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
-      double ext = exd + S(n, Q_EX) - exe1;
+      double ext = exd + S(n, Q::EX) - exe1;
       if (ext > -80.0 && ext < 80.0) {
         double fact = exp(ext);
         yb[0] = uu1 * fact / f1213;
@@ -622,9 +815,9 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     const double facav = rho * av * DUZDUZ / wvno2;
     const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
     const double facn = -ly.F * URDUZ / (wvno * eta);
-    S(m, Q_F + 0) = facah;
-    S(m, Q_F + 1) = facbv;
-    S(m, Q_F + 2) = facn;
+    S(m, Q::F + 0) = facah;
+    S(m, Q::F + 1) = facbv;
+    S(m, Q::F + 2) = facn;
     dcq = dcq + lw * (facbv * bv * qbi) + lw * (facav * av * qai + facah * ah * qai);
 #pragma unroll
     for (int i = 0; i < 4; i++) yt[i] = yb[i];
@@ -640,22 +833,23 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   auto f32 = [](double x) -> float { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; };
   for (int m = 0; m < L; m++) {
     const double vt = (double)vtpf(m);
-    emit(m, f32(S(m, Q_F + 0) / nrm * vt / tm3), f32(S(m, Q_F + 1) / nrm * vt / tm3),
-         f32(S(m, Q_F + 2) / nrm));
+    emit(m, f32(S(m, Q::F + 0) / nrm * vt / tm3), f32(S(m, Q::F + 1) / nrm * vt / tm3),
+         f32(S(m, Q::F + 2) / nrm));
   }
   return 0;
 }
 
 // depthkernelTI splice in float32 (depthkernelTI.f90:152-162): interval j sums
-// its nsub[j] sublayers of dc/dA*TA + dc/dL*TL.
-DK_HD float splice_term(float rrho, float rvp, float rvs, float TA, float TL, float TF,
-                        float dah, float dbv, float dn) {
+// its nsub[j] sublayers, each as acc = (acc + dc/dA*TA) + dc/dL*TL (the
+// reference's left-to-right order).
+DK_HD float splice_add(float acc, float rrho, float rvp, float rvs, float TA, float TL, float TF,
+                       float dah, float dbv, float dn) {
 #pragma clang fp contract(off)
   float den = TA - 2.0f * TL;
   float den2 = den * den;
   float dA = 0.5f / (rrho * rvp) * dah - TF / den2 * dn;
   float dL = 0.5f / (rrho * rvs) * dbv + 2.0f * TF / den2 * dn;
-  return dA * TA + dL * TL;
+  return (acc + dA * TA) + dL * TL;
 }
 
 }  // namespace ti

@@ -7,7 +7,12 @@
 //   k_ti        one thread per (column, period): tregn96's eigenfunctions and
 //               kernels at the float32 phase velocity of the root search, and
 //               the float32 splice into Lsen_Gsc(post, period, interval)
-//               (depthkernelTI.f90:139-162).
+//               (depthkernelTI.f90:139-162), in real arithmetic.  A solve with
+//               a layer whose P/SV eigenvalues form a complex-conjugate pair is
+//               appended to a queue instead.
+//   k_ti_cx     the queued solves in complex arithmetic (tregn96's complex*16
+//               form), in a kernel of their own so that k_ti keeps its
+//               register budget.
 #include <hip/hip_runtime.h>
 
 #include "dk_internal.h"
@@ -16,6 +21,7 @@
 namespace dk {
 
 static_assert(TI_SCRATCH == ti::Q_N, "TI scratch doubles per layer");
+static_assert(TI_SCRATCH_CX == ti::Q_NC, "complex TI scratch doubles per layer");
 
 __device__ float brocher_vp_ti(float v) {
 #pragma clang fp contract(off)
@@ -76,9 +82,10 @@ __device__ TiLayerDev ti_push(TiFlat& s, float d, float rvp, float rvs, float rr
 __global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
                           const float* __restrict__ vpin, const float* __restrict__ rhoin,
                           const float* __restrict__ depz, float minthk0, int L,
-                          TiLayerDev* __restrict__ out) {
+                          TiLayerDev* __restrict__ out, int* __restrict__ queue) {
 #pragma clang fp contract(off)
   int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col == 0) queue[0] = 0;  // the complex-pair queue of the k_ti launch that follows
   if (col >= ncol) return;
   TiLayerDev* lay = out + (size_t)col * L;
   TiFlat fs;
@@ -119,78 +126,120 @@ __global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
   lay[L - 1] = ti_push(fs, 0.0f, vp_i, vs_i, rho_i, true);
 }
 
+// One (column, period) solve s of the TI splice: writes Lsen_Gsc of every
+// interval, or zeros where no kernel exists (no phase velocity, vanishing
+// surface minor).  Returns solve<G_>'s code; on -2 nothing is written.
+template <class G_>
+__device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* __restrict__ layers,
+                      const double* __restrict__ periods, const double* __restrict__ pv,
+                      const int* __restrict__ nsub, float* __restrict__ lsen, double* scratch,
+                      long stride) {
+  const float twopi = 2.0f * 3.141592654f;
+  const int col = (int)(s % ncol), k = (int)(s / ncol);
+  const TiLayerDev* lay = layers + (size_t)col * L;
+  auto out = [&](int j) -> float& { return lsen[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * j)]; };
+  const double c = (double)(float)pv[s];
+  if (!(c > 0.0)) {
+    // no phase velocity for this period (root search failed): no kernel
+    for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+    return 0;
+  }
+  const double omega = (double)twopi / (double)(float)periods[k];
+  int interval = 0, sub = 0;
+  float acc = 0.0f;
+  auto layf = [&](int m) {
+    const TiLayerDev& t = lay[m];
+    ti::Layer r;
+    r.d = (double)t.df;
+    r.A = (double)t.A;
+    r.C = (double)t.A;
+    r.F = (double)t.F;
+    r.L = (double)t.L;
+    r.N = (double)t.L;
+    r.rho = (double)t.rho;
+    return r;
+  };
+  auto vtpf = [&](int m) { return lay[m].vtp; };
+  auto emit = [&](int m, float dah, float dbv, float dn) {
+    if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
+    const TiLayerDev& t = lay[m];
+    acc = ti::splice_add(acc, t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn);
+    if (++sub == nsub[interval]) {
+      out(interval) = acc;
+      acc = 0.0f;
+      sub = 0;
+      interval++;
+    }
+  };
+  const int rc = ti::solve<G_>(L, layf, vtpf, omega, c, scratch, stride, emit);
+  if (rc == -1)
+    for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+  return rc;
+}
+
 __global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
                                             const TiLayerDev* __restrict__ layers,
                                             const double* __restrict__ periods,
                                             const double* __restrict__ pv,
                                             const int* __restrict__ nsub, float* __restrict__ lsen,
-                                            double* __restrict__ scratch, int* __restrict__ tistat) {
+                                            double* __restrict__ scratch, int* __restrict__ queue) {
   const long nthr = (long)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const long nsolves = (long)ncol * K;
-  const float twopi = 2.0f * 3.141592654f;
   for (long s = tid; s < nsolves; s += nthr) {
-    const int col = (int)(s % ncol), k = (int)(s / ncol);
-    const TiLayerDev* lay = layers + (size_t)col * L;
-    auto out = [&](int j) -> float& { return lsen[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * j)]; };
-    const double c = (double)(float)pv[s];
-    if (!(c > 0.0)) {
-      // no phase velocity for this period (root search failed): no kernel
-      for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
-      continue;
-    }
-    const double omega = (double)twopi / (double)(float)periods[k];
-    int interval = 0, sub = 0;
-    float acc = 0.0f;
-    auto layf = [&](int m) {
-      const TiLayerDev& t = lay[m];
-      ti::Layer r;
-      r.d = (double)t.df;
-      r.A = (double)t.A;
-      r.C = (double)t.A;
-      r.F = (double)t.F;
-      r.L = (double)t.L;
-      r.N = (double)t.L;
-      r.rho = (double)t.rho;
-      return r;
-    };
-    auto vtpf = [&](int m) { return lay[m].vtp; };
-    auto emit = [&](int m, float dah, float dbv, float dn) {
-      if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
-      const TiLayerDev& t = lay[m];
-      acc = acc + ti::splice_term(t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn);
-      if (++sub == nsub[interval]) {
-        out(interval) = acc;
-        acc = 0.0f;
-        sub = 0;
-        interval++;
-      }
-    };
-    int rc = ti::solve(L, layf, vtpf, omega, c, scratch + tid, nthr, emit);
-    if (rc != 0) {
-      for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
-      if (tistat) atomicAdd(tistat, 1);
-    }
+    const int rc = ti_one<ti::Eig>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen,
+                                   scratch + tid, nthr);
+    if (rc == -2) queue[1 + atomicAdd(queue, 1)] = (int)s;
   }
 }
 
+__global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
+                                              const TiLayerDev* __restrict__ layers,
+                                              const double* __restrict__ periods,
+                                              const double* __restrict__ pv,
+                                              const int* __restrict__ nsub,
+                                              float* __restrict__ lsen,
+                                              double* __restrict__ scratch,
+                                              const int* __restrict__ queue) {
+  const long nthr = (long)gridDim.x * blockDim.x;
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = queue[0];
+  for (long i = tid; i < n; i += nthr)
+    ti_one<ti::EigC>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen,
+                     scratch + tid, nthr);
+}
+
 hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                          const float* depz, float minthk0, int L, TiLayerDev* out,
+                          const float* depz, float minthk0, int L, TiLayerDev* out, int* queue,
                           hipStream_t st) {
   int block = 128;
   hipLaunchKernelGGL(k_prep_ti, dim3((ncol + block - 1) / block), dim3(block), 0, st, ncol, nz,
-                     vel, vp, rho, depz, minthk0, L, out);
+                     vel, vp, rho, depz, minthk0, L, out, queue);
   return hipGetLastError();
+}
+
+int ti_cx_threads(int nthr) {
+  // the complex solves reuse k_ti's scratch: TI_SCRATCH_CX * n <= TI_SCRATCH * nthr
+  int n = (int)((long)nthr * TI_SCRATCH / TI_SCRATCH_CX) / 64 * 64;
+  return n < 64 ? 64 : n;
 }
 
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* tistat, hipStream_t st) {
+                     double* scratch, int nthr, int* queue, hipStream_t st) {
   int block = 128;
   int grid = nthr / block;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_ti, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods, pv,
-                     nsub, lsen, scratch, tistat);
+                     nsub, lsen, scratch, queue);
+  if (hipError_t e = hipGetLastError()) return e;
+  // the queued complex-pair solves; the grid is sized for the worst case and
+  // exits at once when the queue is empty
+  int ncx = ti_cx_threads(nthr);
+  int cap = ncol * K;
+  if (ncx > (cap + 63) / 64 * 64) ncx = (cap + 63) / 64 * 64;
+  hipLaunchKernelGGL(k_ti_cx, dim3(ncx / 64), dim3(64), 0, st, ncol, K, L, nz, layers, periods,
+                     pv, nsub, lsen, scratch, (const int*)queue);
   return hipGetLastError();
 }
 

@@ -291,3 +291,57 @@ def ti_column(vs, periods, depz, minthk):
                                  _ptr(_f32(depz)), C.byref(C.c_float(minthk)), _ptr(pv),
                                  _ptr(lsen))
     return pv, lsen.reshape(nz - 1, kmax).T.copy()
+
+
+def ti_at(vs, periods, cp, depz, minthk):
+    """Reference TI kernels at GIVEN float32 phase velocities cp (kmax,): the model
+    of depthkernel_ti_column restated (Brocher and refineLayerMdl through this
+    port, moduli TA = rho vp^2, TL = rho vs^2, TF = TA - 2 TL in float32,
+    depthkernel_ti_worker.f90:118-142), the reference's own tregn96
+    (tregn96.f:52) and the float32 splice restated
+    (depthkernel_ti_worker.f90:155-164).  lsen (kmax, nz-1) float32; NaN rows
+    where cp = 0 (tregn96 at c = 0), as the reference."""
+    if not ref_available():
+        raise RuntimeError("reference library not built")
+    vs = _f32(vs)
+    nz = vs.size
+    vp, rho = brocher(vs)
+    depz = _f32(depz)
+    out = [np.zeros(NL, np.float32) for _ in range(4)]
+    nsub = np.zeros(NL, np.int32)
+    L = port().orc_refine(C.c_float(minthk), nz, depz, vp, vs, rho, out[0], out[1], out[2],
+                          out[3], _ptr(nsub))
+    if L < 0:
+        raise ValueError("refined model exceeds NL")
+    rvp, rvs, rrho, rthk = out
+    f = np.float32
+    TA = rrho * (rvp * rvp)
+    TL = rrho * (rvs * rvs)
+    TF = f(1.0) * (TA - f(2.0) * TL)
+    qp = np.full(NL, 150.0, np.float32)
+    qs = np.full(NL, 50.0, np.float32)
+    zero = np.zeros(NL, np.float32)
+    one = np.ones(NL, np.float32)
+    kmax = len(periods)
+    t_in = _f32(np.asarray(periods, np.float64).astype(np.float32), NP)
+    c_in = _f32(np.asarray(cp, np.float32), NP)
+    dah, dbv, dn = (np.zeros(NP * NL, np.float32) for _ in range(3))
+    i = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    ref().tregn96_(i(L), _ptr(rthk), _ptr(TA), _ptr(TA), _ptr(TF), _ptr(TL), _ptr(TL),
+                   _ptr(rrho), _ptr(qp), _ptr(qs), _ptr(zero), _ptr(zero), _ptr(one), _ptr(one),
+                   i(kmax), _ptr(t_in), _ptr(c_in), _ptr(dah), _ptr(dbv), _ptr(dn))
+    dah, dbv, dn = (a.reshape(NL, NP).T[:kmax, :L] for a in (dah, dbv, dn))
+    lsen = np.zeros((kmax, nz - 1), np.float32)
+    with np.errstate(invalid="ignore", over="ignore"):
+        den2 = (TA[:L] - f(2.0) * TL[:L]) * (TA[:L] - f(2.0) * TL[:L])
+        dA = f(0.5) / (rrho[:L] * rvp[:L]) * dah - TF[:L] / den2 * dn
+        dL = f(0.5) / (rrho[:L] * rvs[:L]) * dbv + f(2.0) * TF[:L] / den2 * dn
+        tA = dA * TA[:L]
+        tL = dL * TL[:L]
+        k = 0
+        for j in range(nz - 1):
+            for _ in range(nsub[j]):
+                # left to right, as the reference: (lsen + dA*TA) + dL*TL
+                lsen[:, j] = (lsen[:, j] + tA[:, k]) + tL[:, k]
+                k += 1
+    return lsen

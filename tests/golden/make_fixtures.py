@@ -31,6 +31,11 @@ from oracle import port as P  # noqa: E402
 REF = "/root/reference/examples/test4_Yunnan/voro"
 
 
+FAILURES_NOTE = ("first 4 random inverted/LVZ node models (seed 0) on which the reference's root "
+                 "search fails: depthkernel pv and depthkernel_ti_column pv, Lsen_Gsc (NaN at "
+                 "failed periods)")
+
+
 def sha(path):
     with open(path, "rb") as f:
         return hashlib.sha256(f.read()).hexdigest()
@@ -116,9 +121,44 @@ def ref_functions(seed=11):
                         dl=np.array(recs))
 
 
+def ref_ti_failures(n=4, seed=0):
+    """Random inverted / low-velocity node models (13 nodes 0:5:60 km, minthk 1,
+    20 periods 1-60 s) on which the reference's root search fails part way: the
+    first n such models, with the reference's depthkernel phase velocities and
+    depthkernel_ti_column outputs (whose Lsen_Gsc is NaN at the failed periods,
+    where tregn96 runs with c = 0)."""
+    depz = np.arange(0, 61, 5, dtype=np.float32)
+    per = np.geomspace(1, 60, 20)
+    rng = np.random.default_rng(seed)
+    vel, tpv, tls = [], [], []
+    while len(vel) < n:
+        vs = rng.uniform(1.0, 4.8, depz.size).astype(np.float32)
+        if rng.uniform() < 0.5:
+            vs = np.sort(vs)[::-1].copy()
+        pv, ls = P.ti_column(vs, per, depz, 1.0)
+        if (pv == 0).any():
+            vel.append(vs)
+            tpv.append(pv)
+            tls.append(ls)
+    vel = np.array(vel)
+    pv_iso = P.depthkernel(vel[:, None, :], per, depz, 1.0, lib="ref")[0]
+    np.savez_compressed(os.path.join(HERE, "ref_ti_failures.npz"), vel=vel, depz=depz,
+                        periods=per, minthk=np.float32(1.0), pv=pv_iso, ti_pv=np.array(tpv),
+                        ti_lsen=np.array(tls))
+
+
 def main():
     if not P.ref_available():
         raise SystemExit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["ti_failures"]:
+        ref_ti_failures()
+        with open(os.path.join(HERE, "MANIFEST.json")) as f:
+            man = json.load(f)
+        man["fixtures"]["ref_ti_failures.npz"] = FAILURES_NOTE
+        with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
+            json.dump(man, f, indent=1)
+        print("ref_ti_failures.npz written")
+        return
     man = dict(generator="tests/golden/make_fixtures.py",
                reference_build="oracle/Makefile (flang -O2 -ffp-contract=off), serial")
     man["sources"] = yunnan()
@@ -133,7 +173,9 @@ def main():
         "ref_plain_4096.npz": "same on 8 columns of plain_4096 (seed 2)",
         "ref_ragged.npz": "reference surfdisp96 on 48 ragged columns of ragged_16384 (seed 3)",
         "ref_functions.npz": "gtsolh (200 pairs, seed 11) and dltar4 on 64 flattened 4-layer models",
+        "ref_ti_failures.npz": FAILURES_NOTE,
     }
+    ref_ti_failures()
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
         json.dump(man, f, indent=1)
     print("fixtures written")

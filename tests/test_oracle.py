@@ -154,3 +154,36 @@ def test_fd_kernel_density_scaling_invariant():
                                  tol_rel=1e-13, round_f32=False)
     tot = (s3 * rho[None, :]).sum(axis=1)
     assert np.abs(tot).max() < 2e-4 * np.abs(s3 * rho[None, :]).max()
+
+
+def test_failure_zero_fill_matches_reference_fixture():
+    """Models on which the reference's root search fails part way: the port
+    zero-fills from the same period with the same values before it."""
+    f = load_golden("ref_ti_failures.npz")
+    pv = P.depthkernel(f["vel"][:, None, :], f["periods"], f["depz"], float(f["minthk"]),
+                       lib="port", kernels=False)
+    assert np.array_equal(pv, f["pv"])
+    assert np.array_equal(f["pv"] == 0, f["ti_pv"] == 0)
+    # the reference's TI splice is NaN exactly where its root search failed
+    assert np.array_equal(np.isnan(f["ti_lsen"]).any(axis=2), f["ti_pv"] == 0)
+
+
+@pytest.mark.skipif(not P.ref_available(), reason="reference library not built")
+def test_ti_at_reproduces_reference_columns(tmp_path, monkeypatch):
+    """ti_at (the reference's tregn96 at given phase velocities, model and splice
+    restated) equals the reference's own depthkernel_ti_column bit for bit when
+    given that column's phase velocities; NaN where c = 0."""
+    monkeypatch.chdir(tmp_path)  # the reference writes unit-66 messages to cwd
+    f = load_golden("ref_ti_failures.npz")
+    for j in range(f["vel"].shape[0]):
+        ls = P.ti_at(f["vel"][j], f["periods"], f["ti_pv"][j].astype(np.float32), f["depz"],
+                     float(f["minthk"]))
+        ok = f["ti_pv"][j] > 0
+        assert np.array_equal(ls[ok], f["ti_lsen"][j][ok])
+        assert np.isnan(ls[~ok]).any(axis=1).all()
+    g = load_golden("ref_plain_1024.npz")
+    n = g["ti_pv"].shape[0]
+    for j in range(n):
+        ls = P.ti_at(g["vel"][j], g["periods"], g["ti_pv"][j].astype(np.float32), g["depz"],
+                     float(g["minthk"]))
+        assert np.array_equal(ls, g["ti_lsen"][j])
