@@ -195,6 +195,39 @@ def bench_c5(dk, configs, np, torch, chunk=40960):
                 cell_grad_norm=float(acc.norm()))
 
 
+def bench_grows(dk, configs, np, torch, vel, out, nvx, nvz, reps=5):
+    """G-row assembly on the plain_4096 stage outputs (64x64 columns: a 62x62
+    interior grid, 24 depth rows, 30 periods): 1,000 synthetic straight rays per
+    period, all 30,000 rays in one call (count, scan, fill).  Algorithmic HBM
+    bytes per call: 12 B per emitted triplet + 16 B per (ray, cell) input (cell
+    index and three weights); the stage arrays it gathers from (~140 MB) are
+    re-read from cache across rays.  Timed with HIP events on the launch stream."""
+    from dazi_current_release_amd import grows
+    K = out["pv"].shape[0]
+    R = configs.ray_workload(nvx, nvz, K, 1000, seed=7)
+    rays = grows.rays_to_device(R)
+    args = (rays, nvx, nvz, vel, out["sen_vs"], out["sen_vp"], out["sen_rho"], out["lsen"])
+    g = grows.g_rows_dev(*args)
+    nar = int(g["nar"].item())
+    buf = dict(rw=g["rw"], iw=g["iw"], col=g["col"])
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    grows.g_rows_dev(*args, out=buf)
+    e0.record(st)
+    for _ in range(reps):
+        grows.g_rows_dev(*args, out=buf)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    ncell = int(R["cell_off"][-1])
+    nray = len(R["ray_k"])
+    alg = 12 * nar + 16 * ncell
+    return dict(workload="plain_4096 + 30,000 rays", rays=nray, ray_cells=ncell, triplets=nar,
+                ms=ms, rays_per_s=nray / (ms * 1e-3), triplets_per_s=nar / (ms * 1e-3),
+                roofline=dict(bound="hbm", achieved=alg / (ms * 1e-3) / 1e9, peak=8000.0,
+                              unit="GB/s", frac=alg / (ms * 1e-3) / 8e12, bytes=alg))
+
+
 def bench_host_path(dk, configs, np, reps=3):
     """The Fortran drop-in's host entry points on plain_4096: depthkernel then
     depthkernelTI through the C ABI with host arrays (inputs copied in, all
@@ -228,6 +261,7 @@ def main():
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-host", action="store_true")
+    ap.add_argument("--no-grows", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -363,6 +397,9 @@ def main():
     host = None
     if not args.no_host:
         host = bench_host_path(dk, configs, np)
+    grow = None
+    if not args.no_grows:
+        grow = bench_grows(dk, configs, np, torch, vel, out, W["nx"] - 2, W["ny"] - 2)
 
     cpu = None
     if not args.no_cpu:
@@ -393,7 +430,7 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp, ragged=c3, realizations=c5, host_path=host),
+                    vjp=vjp, ragged=c3, realizations=c5, host_path=host, g_rows=grow),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 

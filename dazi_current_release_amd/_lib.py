@@ -22,7 +22,7 @@ EXPORTS = [
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
     "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
-    "dazi_dk_pool_bytes",
+    "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows",
 ]
 
 
@@ -52,6 +52,11 @@ class Layered(C.Structure):
                 ("dc_dvs", C.c_void_p), ("dc_dvp", C.c_void_p), ("dc_drho", C.c_void_p),
                 ("c_unrounded", C.c_void_p), ("thk64", C.c_void_p), ("vp64", C.c_void_p),
                 ("vs64", C.c_void_p), ("rho64", C.c_void_p), ("nlayers_total", C.c_int64)]
+
+
+class GrowDesc(C.Structure):
+    _fields_ = [("nvx", C.c_int32), ("nvz", C.c_int32), ("nz", C.c_int32), ("kmax", C.c_int32),
+                ("nray", C.c_int64), ("row_base", C.c_int32), ("ftol", C.c_float)]
 
 
 _lib = None
@@ -90,6 +95,10 @@ def lib():
         L.dazi_dk_reset_geometry.argtypes = []
         L.dazi_dk_effective_dev.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 8
         L.dazi_dk_pool_bytes.argtypes = [C.POINTER(C.c_int64)]
+        L.dazi_dk_grows_dev.argtypes = ([C.POINTER(GrowDesc)] + [C.c_void_p] * 11 +
+                                        [C.c_int64] + [C.c_void_p] * 6)
+        L.dazi_dk_grows.argtypes = ([C.POINTER(GrowDesc)] + [C.c_void_p] * 11 + [C.c_int64] +
+                                    [C.c_void_p] * 4)
         for name in EXPORTS:
             if not hasattr(L, name):
                 raise RuntimeError(f"{LIB_PATH} does not export {name}")

@@ -11,7 +11,15 @@ threshold the count of base values below it and of entries that cross it.
 With the node Vs it also compares the G builder's effective dc/dVs
 (CalSurfGAniso_Joint.f90:754-765).
 
+With --nparpi it compares two G-triplet dumps instead (DAZI_G_TRIPLET_DUMP_DIR:
+rw_triplets.bin f32, iw_triplets.bin and col_triplets.bin i32,
+CalSurfG.f90:1572-1594), keyed by (iw, col), with the sets and metrics of the
+reference's triplet comparator (code/depthkernel_lab/compare_g_triplet_dumps.py:
+53-123): all entries, the common keys' differences (per Vs / Gc / Gs family),
+and the keys present on one side only with their counts near ftol.
+
     python -m dazi_current_release_amd.compare --base A --test B --nx 64 --ny 64 --nz 25 --kmax 30 [--vel-bin vel.bin] [--out metrics.csv]
+    python -m dazi_current_release_amd.compare --base A --test B --nparpi 92256 [--out metrics.csv]
 """
 from __future__ import annotations
 
@@ -93,31 +101,105 @@ def compare_dirs(base, test, nx, ny, nz, kmax, vel=None):
     return rows
 
 
+NEAR_FTOL = (1.0e-4, 1.1e-4, 1.5e-4, 2.0e-4, 5.0e-4, 1.0e-3)
+
+
+def read_triplets(d):
+    rw = np.fromfile(os.path.join(d, "rw_triplets.bin"), np.float32)
+    iw = np.fromfile(os.path.join(d, "iw_triplets.bin"), np.int32)
+    col = np.fromfile(os.path.join(d, "col_triplets.bin"), np.int32)
+    if not rw.size == iw.size == col.size:
+        raise ValueError(f"{d}: triplet arrays of sizes {rw.size}, {iw.size}, {col.size}")
+    return iw, col, rw
+
+
+def _abs_stats(v):
+    a = np.abs(np.asarray(v, np.float64))
+    if a.size == 0:
+        return dict(n=0, min_abs=0.0, max_abs=0.0, mean_abs=0.0, rms=0.0)
+    return dict(n=int(a.size), min_abs=float(a.min()), max_abs=float(a.max()),
+                mean_abs=float(a.mean()), rms=float(np.sqrt((a * a).mean())))
+
+
+def _families(col, nparpi):
+    col = np.asarray(col)
+    return dict(Vs=int(np.count_nonzero(col <= nparpi)),
+                Gc=int(np.count_nonzero((col > nparpi) & (col <= 2 * nparpi))),
+                Gs=int(np.count_nonzero(col > 2 * nparpi)))
+
+
+def compare_triplets(base, test, nparpi):
+    """Rows of the G-triplet A/B metrics for the triplets (iw, col, rw) of two
+    runs (each a tuple, or a dump directory)."""
+    if isinstance(base, str):
+        base = read_triplets(base)
+    if isinstance(test, str):
+        test = read_triplets(test)
+    (bi, bc, bv), (ti, tc, tv) = base, test
+    key = lambda i, c: i.astype(np.int64) * (3 * nparpi + 1) + c.astype(np.int64)  # noqa: E731
+    kb, kt = key(bi, bc), key(ti, tc)
+    common, ib, it = np.intersect1d(kb, kt, return_indices=True)
+    bonly = np.setdiff1d(np.arange(kb.size), ib)
+    tonly = np.setdiff1d(np.arange(kt.size), it)
+    d = tv[it].astype(np.float64) - bv[ib].astype(np.float64)
+    cb = bv[ib].astype(np.float64)
+    rel = lambda dd, bb: float(np.linalg.norm(dd) / max(np.linalg.norm(bb), 1.0e-30))  # noqa
+    rows = [dict(set="base_all", **_abs_stats(bv), **_families(bc, nparpi)),
+            dict(set="test_all", **_abs_stats(tv), **_families(tc, nparpi)),
+            dict(set="common_diff", **_abs_stats(d), rel_l2=rel(d, cb))]
+    cc = bc[ib]
+    for fam, m in (("Vs", cc <= nparpi), ("Gc", (cc > nparpi) & (cc <= 2 * nparpi)),
+                   ("Gs", cc > 2 * nparpi)):
+        rows.append(dict(set="common_diff_" + fam, **_abs_stats(d[m]), rel_l2=rel(d[m], cb[m])))
+    for name, idx, vals, cols in (("base_only", bonly, bv, bc), ("test_only", tonly, tv, tc)):
+        r = dict(set=name, **_abs_stats(vals[idx]), **_families(cols[idx], nparpi))
+        a = np.abs(vals[idx].astype(np.float64))
+        for lev in NEAR_FTOL:
+            r[f"abs_le_{lev:g}"] = int(np.count_nonzero(a <= lev))
+        rows.append(r)
+    return rows
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--base", required=True)
     ap.add_argument("--test", required=True)
-    ap.add_argument("--nx", type=int, required=True)
-    ap.add_argument("--ny", type=int, required=True)
-    ap.add_argument("--nz", type=int, required=True)
-    ap.add_argument("--kmax", type=int, required=True)
+    ap.add_argument("--nx", type=int)
+    ap.add_argument("--ny", type=int)
+    ap.add_argument("--nz", type=int)
+    ap.add_argument("--kmax", type=int)
     ap.add_argument("--vel-bin", help="float32 node Vs (nx, ny, nz) Fortran order")
+    ap.add_argument("--nparpi", type=int, help="G triplets: model columns per family")
     ap.add_argument("--out", help="CSV of all metrics")
     a = ap.parse_args(argv)
+    if a.nparpi:
+        rows = compare_triplets(a.base, a.test, a.nparpi)
+        _write_csv(a.out, rows)
+        for r in rows:
+            print(f"{r['set']}: n={r['n']} max_abs={r['max_abs']:.6g} "
+                  + (f"rel_l2={r['rel_l2']:.6g}" if "rel_l2" in r else ""))
+        return rows
+    if None in (a.nx, a.ny, a.nz, a.kmax):
+        ap.error("--nx, --ny, --nz and --kmax are required for depth-kernel dumps")
     vel = None
     if a.vel_bin:
         vel = np.fromfile(a.vel_bin, np.float32).reshape((a.nx, a.ny, a.nz), order="F")
     rows = compare_dirs(a.base, a.test, a.nx, a.ny, a.nz, a.kmax, vel)
-    if a.out:
-        keys = sorted({k for r in rows for k in r})
-        with open(a.out, "w", newline="") as fh:
-            w = csv.DictWriter(fh, fieldnames=keys)
-            w.writeheader()
-            w.writerows(rows)
+    _write_csv(a.out, rows)
     for r in rows:
         print(f"{r['array']}: max_abs={r['max_abs']:.6g} rms_abs={r['rms_abs']:.6g} "
               f"rel_l2={r['rel_l2']:.6g} cross_abs_1e-4={r['cross_abs_0.0001']}")
     return rows
+
+
+def _write_csv(path, rows):
+    if not path:
+        return
+    keys = sorted({k for r in rows for k in r})
+    with open(path, "w", newline="") as fh:
+        w = csv.DictWriter(fh, fieldnames=keys)
+        w.writeheader()
+        w.writerows(rows)
 
 
 if __name__ == "__main__":

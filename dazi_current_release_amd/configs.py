@@ -232,3 +232,68 @@ def checkerboard_vs(vs, nl, ncol_side=64, cell=4, amp=0.03):
         o = loff[j]
         out[o:o + nl[j] - 1] *= F(1.0 + sgn * amp)
     return out
+
+
+def ray_workload(nvx: int, nvz: int, K: int, rays_per_period: int, seed: int = 7,
+                 cell_km: float = 25.0):
+    """Synthetic ray paths for the G-row assembly: straight rays between uniform
+    random points of the interior grid (cells (jj, kk), jj = 1..nvz, kk = 1..nvx,
+    cell kk spanning x in [kk - 0.5, kk + 0.5)), rays_per_period per period,
+    period-major as the reference's knumi loop.  fdm = path length in the cell
+    (km, sampled every 1/40 cell), fdmc = fdm cos 2phi, fdms = fdm sin 2phi with
+    phi the ray azimuth.  Returns dict(ray_k (nray,) i32, cell_off (nray+1,) i64,
+    cell (n,) i32 = (jj-1)*nvx + kk-1 ascending within a ray, fdm, fdmc, fdms (n,)
+    f32, nvx, nvz)."""
+    rng = np.random.default_rng(seed)
+    nray = K * rays_per_period
+    ray_k = np.repeat(np.arange(K, dtype=np.int32), rays_per_period)
+    lo = np.array([0.5, 0.5])
+    hi = np.array([nvx + 0.5, nvz + 0.5])
+    p0 = rng.uniform(lo, hi - 1e-6, size=(nray, 2))
+    p1 = rng.uniform(lo, hi - 1e-6, size=(nray, 2))
+    d = p1 - p0
+    length = np.hypot(d[:, 0], d[:, 1])
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    step = 1.0 / 40.0
+    ns = np.maximum(np.ceil(length / step).astype(np.int64), 1)
+    cells = nvx * nvz
+    offs, cl, w = [0], [], []
+    chunk = 4096
+    for a in range(0, nray, chunk):
+        b = min(a + chunk, nray)
+        n = ns[a:b]
+        rid = np.repeat(np.arange(a, b), n)
+        t = (np.arange(n.sum()) - np.repeat(np.cumsum(n) - n, n) + 0.5) / np.repeat(n, n)
+        x = p0[rid, 0] + t * d[rid, 0]
+        z = p0[rid, 1] + t * d[rid, 1]
+        kk = np.clip(np.floor(x + 0.5).astype(np.int64), 1, nvx)
+        jj = np.clip(np.floor(z + 0.5).astype(np.int64), 1, nvz)
+        key = rid * cells + (jj - 1) * nvx + (kk - 1)
+        u, inv = np.unique(key, return_inverse=True)
+        seg = (length[rid] / np.repeat(n, n)) * cell_km
+        wsum = np.bincount(inv, weights=seg)
+        r_of = u // cells
+        cnt = np.bincount(r_of - a, minlength=b - a)
+        offs.extend((offs[-1] + np.cumsum(cnt)).tolist())
+        cl.append((u % cells).astype(np.int32))
+        w.append(wsum)
+    cell = np.concatenate(cl)
+    fdm = np.concatenate(w)
+    rr = np.repeat(np.arange(nray), np.diff(np.array(offs)))
+    return dict(ray_k=ray_k, cell_off=np.array(offs, np.int64), cell=cell,
+                fdm=fdm.astype(F), fdmc=(fdm * np.cos(2 * phi[rr])).astype(F),
+                fdms=(fdm * np.sin(2 * phi[rr])).astype(F), nvx=nvx, nvz=nvz)
+
+
+def ray_dense(R, r):
+    """fdm, fdmc, fdms of ray r of a ray_workload as the reference's dense maps
+    fdm(0:nvz+1, 0:nvx+1): three (nvz+2, nvx+2) float32 arrays indexed [jj, kk]."""
+    nvx, nvz = R["nvx"], R["nvz"]
+    s = slice(R["cell_off"][r], R["cell_off"][r + 1])
+    c = R["cell"][s]
+    out = []
+    for key in ("fdm", "fdmc", "fdms"):
+        m = np.zeros((nvz + 2, nvx + 2), F)
+        m[c // nvx + 1, c % nvx + 1] = R[key][s]
+        out.append(m)
+    return out

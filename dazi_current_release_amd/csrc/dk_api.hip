@@ -64,6 +64,19 @@ enum Slot {
   S_TI_QUEUE,
   S_TISTAT,
   S_H_LSEN,
+  S_G_RAYK,
+  S_G_CELLOFF,
+  S_G_CELL,
+  S_G_W,
+  S_G_CNT,
+  S_G_ROWOFF,
+  S_G_NAR,
+  S_G_RW,
+  S_G_IW,
+  S_G_COL,
+  S_G_EFF,
+  S_G_LSENT,
+  S_G_TSUM,
   S_COUNT
 };
 
@@ -372,6 +385,71 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   return 0;
 }
 
+int check_grow_desc(const dazi_dk_grow_desc* d) {
+  if (!d) return set_err(DAZI_DK_EINVAL, "NULL descriptor");
+  if (d->nvx < 1 || d->nvz < 1 || d->nz < 2 || d->kmax < 1 || d->nray < 0)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nvx=%d nvz=%d nz=%d kmax=%d nray=%lld", d->nvx,
+                   d->nvz, d->nz, d->kmax, (long long)d->nray);
+  if (!(d->ftol >= 0.0f)) return set_err(DAZI_DK_EINVAL, "ftol must be >= 0");
+  const int64_t nmodel = 3 * (int64_t)d->nvx * d->nvz * (d->nz - 1);
+  if (nmodel > INT32_MAX)
+    return set_err(DAZI_DK_ELIMIT, "3*nparpi = %lld model columns exceed int32",
+                   (long long)nmodel);
+  if (d->nray > INT32_MAX || (int64_t)d->row_base + d->nray - 1 > INT32_MAX)
+    return set_err(DAZI_DK_ELIMIT, "data rows beyond int32");
+  return 0;
+}
+
+int grows_impl(const dazi_dk_grow_desc* d, const int32_t* ray_k, const int64_t* cell_off,
+               const int32_t* cell, const float* fdm, const float* fdmc, const float* fdms,
+               const float* vel, const double* svs, const double* svp, const double* srho,
+               const float* lsen, int64_t cap, float* rw, int32_t* iw, int32_t* col,
+               int64_t* row_nnz, int64_t* nar, hipStream_t st) {
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  void *cnt = row_nnz, *off;
+  if (!cnt)
+    if (int rc = pool_get(P, S_G_CNT, (size_t)(d->nray > 0 ? d->nray : 1) * 8, &cnt)) return rc;
+  if (int rc = pool_get(P, S_G_ROWOFF, (size_t)(d->nray > 0 ? d->nray : 1) * 8, &off)) return rc;
+  void *eff, *lst, *tsum;
+  const size_t neff = (size_t)(d->nvx + 2) * (d->nvz + 2) * d->kmax * (d->nz - 1);
+  if (int rc = pool_get(P, S_G_EFF, neff * 8, &eff)) return rc;
+  if (int rc = pool_get(P, S_G_LSENT, neff * 4, &lst)) return rc;
+  const long nt = dk::grow_scan_tiles(d->nray);
+  if (int rc = pool_get(P, S_G_TSUM, (size_t)(nt > 0 ? nt : 1) * 8, &tsum)) return rc;
+  dk::GrowDev g;
+  g.nvx = d->nvx;
+  g.nvz = d->nvz;
+  g.nz = d->nz;
+  g.K = d->kmax;
+  g.nray = d->nray;
+  g.row_base = d->row_base;
+  g.ftol = d->ftol;
+  g.ray_k = ray_k;
+  g.cell_off = cell_off;
+  g.cell = cell;
+  g.fdm = fdm;
+  g.fdmc = fdmc;
+  g.fdms = fdms;
+  g.vel = vel;
+  g.svs = svs;
+  g.svp = svp;
+  g.srho = srho;
+  g.lsen = lsen;
+  g.effT = (double*)eff;
+  g.lsenT = (float*)lst;
+  g.tsum = (int64_t*)tsum;
+  g.cap = cap;
+  g.rw = rw;
+  g.iw = iw;
+  g.col = col;
+  g.row_nnz = (int64_t*)cnt;
+  g.row_off = (int64_t*)off;
+  g.nar = nar;
+  HIPCHK(dk::launch_grows(g, st));
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -431,6 +509,109 @@ int32_t dazi_dk_effective_dev(const dazi_dk_desc* d, const float* vel, const dou
   if (!eff && !grad_vs) return 0;
   HIPCHK(dk::launch_effective(d->nx * d->ny, d->nz, d->kmax, vel, sen_vs, sen_vp, sen_rho, grad_c,
                               eff, grad_vs, (hipStream_t)stream));
+  return 0;
+}
+
+int32_t dazi_dk_grows_dev(const dazi_dk_grow_desc* d, const int32_t* ray_k,
+                          const int64_t* cell_off, const int32_t* cell, const float* fdm,
+                          const float* fdmc, const float* fdms, const float* vel,
+                          const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                          const float* lsen, int64_t cap, float* rw, int32_t* iw, int32_t* col,
+                          int64_t* row_nnz, int64_t* nar, void* stream) {
+  if (int rc = check_grow_desc(d)) return rc;
+  if (!nar) return set_err(DAZI_DK_EINVAL, "NULL nar");
+  if (d->nray > 0 && (!ray_k || !cell_off || !cell || !fdm || !fdmc || !fdms || !vel ||
+                      !sen_vs || !sen_vp || !sen_rho || !lsen))
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (cap < 0 || (cap > 0 && (!rw || !iw || !col)))
+    return set_err(DAZI_DK_EINVAL, "cap > 0 needs rw, iw and col");
+  return grows_impl(d, ray_k, cell_off, cell, fdm, fdmc, fdms, vel, sen_vs, sen_vp, sen_rho, lsen,
+                    cap, rw, iw, col, row_nnz, nar, (hipStream_t)stream);
+}
+
+int32_t dazi_dk_grows(const dazi_dk_grow_desc* d, const int32_t* ray_k, const int64_t* cell_off,
+                      const int32_t* cell, const float* fdm, const float* fdmc, const float* fdms,
+                      const float* vel, const double* sen_vs, const double* sen_vp,
+                      const double* sen_rho, const float* lsen, int64_t cap, float* rw,
+                      int32_t* iw, int32_t* col, int64_t* nar) {
+  if (int rc = check_grow_desc(d)) return rc;
+  if (!nar || !cell_off) return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (d->nray > 0 && (!ray_k || !cell || !fdm || !fdmc || !fdms || !vel || !sen_vs || !sen_vp ||
+                      !sen_rho || !lsen))
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  const int64_t nray = d->nray;
+  const int cells = d->nvx * d->nvz;
+  if (cell_off[0] != 0) return set_err(DAZI_DK_EINVAL, "cell_off[0] must be 0");
+  for (int64_t r = 0; r < nray; r++) {
+    if (cell_off[r + 1] < cell_off[r])
+      return set_err(DAZI_DK_EINVAL, "cell_off decreases at ray %lld", (long long)r);
+    if (ray_k[r] < 0 || ray_k[r] >= d->kmax)
+      return set_err(DAZI_DK_EINVAL, "ray %lld: period index %d outside 0..%d", (long long)r,
+                     ray_k[r], d->kmax - 1);
+    for (int64_t i = cell_off[r]; i < cell_off[r + 1]; i++) {
+      if (cell[i] < 0 || cell[i] >= cells)
+        return set_err(DAZI_DK_EINVAL, "ray %lld: cell %d outside 0..%d", (long long)r, cell[i],
+                       cells - 1);
+      if (i > cell_off[r] && cell[i] <= cell[i - 1])
+        return set_err(DAZI_DK_EINVAL, "ray %lld: cells not strictly increasing", (long long)r);
+    }
+  }
+  if (cap < 0 || (cap > 0 && (!rw || !iw || !col)))
+    return set_err(DAZI_DK_EINVAL, "cap > 0 needs rw, iw and col");
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  const size_t ncol = (size_t)(d->nvx + 2) * (d->nvz + 2), nz = d->nz, K = d->kmax;
+  const size_t nc = (size_t)cell_off[nray], nr = (size_t)(nray > 0 ? nray : 1);
+  void *dk_, *doff, *dcell, *dw, *dvel, *d1, *d2, *d3, *dls, *dnar, *drw = nullptr,
+       *diw = nullptr, *dcol = nullptr;
+  if (int rc = pool_get(P, S_G_RAYK, nr * 4, &dk_)) return rc;
+  if (int rc = pool_get(P, S_G_CELLOFF, (nr + 1) * 8, &doff)) return rc;
+  if (int rc = pool_get(P, S_G_CELL, (nc > 0 ? nc : 1) * 4, &dcell)) return rc;
+  if (int rc = pool_get(P, S_G_W, (nc > 0 ? nc : 1) * 12, &dw)) return rc;
+  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
+  if (int rc = pool_get(P, S_H_SVS, ncol * K * nz * 8, &d1)) return rc;
+  if (int rc = pool_get(P, S_H_SVP, ncol * K * nz * 8, &d2)) return rc;
+  if (int rc = pool_get(P, S_H_SRHO, ncol * K * nz * 8, &d3)) return rc;
+  if (int rc = pool_get(P, S_H_LSEN, ncol * K * (nz - 1) * 4, &dls)) return rc;
+  if (int rc = pool_get(P, S_G_NAR, 8, &dnar)) return rc;
+  if (cap > 0) {
+    if (int rc = pool_get(P, S_G_RW, (size_t)cap * 4, &drw)) return rc;
+    if (int rc = pool_get(P, S_G_IW, (size_t)cap * 4, &diw)) return rc;
+    if (int rc = pool_get(P, S_G_COL, (size_t)cap * 4, &dcol)) return rc;
+  }
+  float* w = (float*)dw;
+  if (nray > 0) {
+    HIPCHK(hipMemcpy(dk_, ray_k, nray * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(doff, cell_off, (nray + 1) * 8, hipMemcpyHostToDevice));
+    if (nc > 0) {
+      HIPCHK(hipMemcpy(dcell, cell, nc * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w, fdm, nc * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w + nc, fdmc, nc * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(w + 2 * nc, fdms, nc * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d1, sen_vs, ncol * K * nz * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d2, sen_vp, ncol * K * nz * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(d3, sen_rho, ncol * K * nz * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dls, lsen, ncol * K * (nz - 1) * 4, hipMemcpyHostToDevice));
+  }
+  hipStream_t st = nullptr;
+  if (int rc = grows_impl(d, (const int32_t*)dk_, (const int64_t*)doff, (const int32_t*)dcell, w,
+                          w + nc, w + 2 * nc, (const float*)dvel, (const double*)d1,
+                          (const double*)d2, (const double*)d3, (const float*)dls, cap,
+                          (float*)drw, (int32_t*)diw, (int32_t*)dcol, nullptr, (int64_t*)dnar, st))
+    return rc;
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipMemcpy(nar, dnar, 8, hipMemcpyDeviceToHost));
+  const int64_t n = *nar < cap ? *nar : cap;
+  if (n > 0) {
+    HIPCHK(hipMemcpy(rw, drw, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(iw, diw, (size_t)n * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(col, dcol, (size_t)n * 4, hipMemcpyDeviceToHost));
+  }
+  if (*nar > cap)
+    return set_err(DAZI_DK_ELIMIT, "%lld entries exceed cap %lld (nar holds the count)",
+                   (long long)*nar, (long long)cap);
   return 0;
 }
 

@@ -86,6 +86,38 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
 hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
                           hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
+// G-row assembly (CalSurfGAniso_Joint.f90:750-797), device pointers; see
+// dazi_dk_grows_dev.  row_nnz, row_off: per ray; nar: one int64.
+struct GrowDev {
+  int nvx, nvz, nz, K;
+  int64_t nray;
+  int row_base;
+  float ftol;
+  const int* ray_k;
+  const int64_t* cell_off;
+  const int* cell;
+  const float* fdm;
+  const float* fdmc;
+  const float* fdms;
+  const float* vel;
+  const double* svs;
+  const double* svp;
+  const double* srho;
+  const float* lsen;
+  double* effT;  // work: effective dc/dVs, (K, ncol, nz-1) depth fastest
+  float* lsenT;  // work: Lsen_Gsc, same layout
+  int64_t* tsum;  // work: grow_scan_tiles(nray) tile sums
+  int64_t cap;
+  float* rw;
+  int* iw;
+  int* col;
+  int64_t* row_nnz;
+  int64_t* row_off;
+  int64_t* nar;
+};
+hipError_t launch_grows(const GrowDev& g, hipStream_t st);
+long grow_scan_tiles(long nray);
+
 hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const double* svs,
                             const double* svp, const double* srho, const double* gc, double* eff,
                             double* grad, hipStream_t st);

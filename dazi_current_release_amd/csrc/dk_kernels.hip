@@ -30,6 +30,7 @@
 #include <string>
 #include <vector>
 
+#include "dk_brocher.h"
 #include "dk_physics.h"
 #include "dk_internal.h"
 
@@ -1297,19 +1298,8 @@ hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, doub
 
 // ------------------------------------------------------------------ effective dc/dVs
 // The G builder's Vs row (CalSurfGAniso_Joint.f90:750-765): with vp and rho
-// tied to vs by Brocher, dc/dVs_eff = sen_vs + sen_vp coe_a + sen_rho coe_rho,
-// coe_a = dVp/dVs and coe_rho = dRho/dVp * coe_a evaluated in float32 at the
-// node vs (the reference's real arithmetic, left to right).
-__device__ void brocher_coeffs(float v, float* coe_a, float* coe_rho) {
-#pragma clang fp contract(off)
-  const float v2 = v * v, v3 = v2 * v, v4 = v3 * v;
-  const float a = 2.0947f - 0.8206f * 2.0f * v + 0.2683f * 3.0f * v2 - 0.0251f * 4.0f * v3;
-  const float p = 0.9409f + 2.0947f * v - 0.8206f * v2 + 0.2683f * v3 - 0.0251f * v4;
-  const float p2 = p * p, p3 = p2 * p, p4 = p3 * p;
-  *coe_a = a;
-  *coe_rho = a * (1.6612f - 0.4721f * 2.0f * p + 0.0671f * 3.0f * p2 - 0.0043f * 4.0f * p3 +
-                  0.000106f * 5.0f * p4);
-}
+// tied to vs by Brocher, dc/dVs_eff = sen_vs + sen_vp coe_a + sen_rho coe_rho
+// (brocher_coeffs, effective_vs).
 
 // One thread per (node, column): eff[(z*K + k)*ncol + col] for z < nz-1 (the
 // reference's 1:nz-1 rows) and grad[z*ncol + col] = sum_k gc[k*ncol + col] *
@@ -1323,11 +1313,10 @@ __global__ void k_effective(int ncol, int nz, int K, const float* __restrict__ v
   const int z = (int)(t / ncol), col = (int)(t % ncol);
   float ca, cr;
   brocher_coeffs(vel[t], &ca, &cr);
-  const double a = (double)ca, r = (double)cr;
   double acc = 0.0;
   for (int k = 0; k < K; k++) {
     const long o = ((long)z * K + k) * ncol + col;
-    const double e = svp[o] * a + srho[o] * r + svs[o];
+    const double e = effective_vs(svs[o], svp[o], srho[o], ca, cr);
     if (eff && z < nz - 1) eff[o] = e;
     if (grad) acc = fma(gc[(long)k * ncol + col], e, acc);
   }

@@ -13,11 +13,19 @@
 ! A non-zero ABI code prints the library's message (stdout and unit 66) and
 ! stops with that code (9301-9304).  DAZI_DEPTHKERNEL_DUMP_DIR and
 ! DAZI_DEPTHKERNEL_TI_DUMP_DIR write the reference's dump files.
+!
+! g_rows_hip_add_ray / g_rows_hip_flush replace the G builder's row fill and
+! compression (CalSurfGAniso_Joint.f90:746-797): inside the record loop, in
+! place of the row fill, add_ray takes the ray's fdm, fdmc, fdms maps and
+! period index; flush assembles every buffered ray on the GPU and appends the
+! (rw, iw, col) triplets at nar+1 in the reference's order.
+! DAZI_G_TRIPLET_DUMP_DIR is left to the caller's dump_g_triplets.
 module dazi_dk
   use, intrinsic :: iso_c_binding
   implicit none
   private
   public :: depthkernel_hip, depthkernelTI_hip, dazi_dk_desc, dazi_dk_tune
+  public :: g_rows_hip_add_ray, g_rows_hip_flush, dazi_dk_grow_desc, c_dazi_dk_grows
   public :: c_dazi_dk_iso, c_dazi_dk_ti, c_dazi_dk_set_tune, dazi_dk_error_text
 
   integer(c_int32_t), parameter, public :: DAZI_DK_OK = 0
@@ -37,7 +45,37 @@ module dazi_dk
     real(c_double) :: refine_tol
   end type dazi_dk_tune
 
+  type, bind(c) :: dazi_dk_grow_desc
+    integer(c_int32_t) :: nvx, nvz, nz, kmax
+    integer(c_int64_t) :: nray
+    integer(c_int32_t) :: row_base
+    real(c_float) :: ftol
+  end type dazi_dk_grow_desc
+
+  ! rays buffered by g_rows_hip_add_ray (cells in the reference's jj, kk order)
+  integer :: gb_nray = 0, gb_ncell = 0, gb_nvx = 0, gb_nvz = 0
+  integer(c_int32_t), allocatable :: gb_k(:), gb_cell(:)
+  integer(c_int64_t), allocatable :: gb_off(:)
+  real(c_float), allocatable :: gb_w(:), gb_wc(:), gb_ws(:)
+  real, parameter :: GROW_FTOL = 1e-4   ! CalSurfGAniso_Joint.f90:330
+
   interface
+    function c_dazi_dk_grows(desc, ray_k, cell_off, cell, fdm, fdmc, fdms, vel, sen_vs, sen_vp, &
+                             sen_rho, lsen, cap, rw, iw, col, nar) &
+        bind(c, name='dazi_dk_grows') result(rc)
+      import :: c_int32_t, c_int64_t, c_float, c_double, dazi_dk_grow_desc
+      type(dazi_dk_grow_desc), intent(in) :: desc
+      integer(c_int32_t), intent(in) :: ray_k(*), cell(*)
+      integer(c_int64_t), intent(in) :: cell_off(*)
+      real(c_float), intent(in) :: fdm(*), fdmc(*), fdms(*), vel(*), lsen(*)
+      real(c_double), intent(in) :: sen_vs(*), sen_vp(*), sen_rho(*)
+      integer(c_int64_t), value :: cap
+      real(c_float), intent(inout) :: rw(*)
+      integer(c_int32_t), intent(inout) :: iw(*), col(*)
+      integer(c_int64_t), intent(out) :: nar
+      integer(c_int32_t) :: rc
+    end function c_dazi_dk_grows
+
     function c_dazi_dk_iso(desc, vel, vp, rho, depz, tRc, pvRc, sen_vs, sen_vp, sen_rho) &
         bind(c, name='dazi_dk_iso') result(rc)
       import :: c_int32_t, c_float, c_double, c_ptr, dazi_dk_desc
@@ -246,5 +284,119 @@ contains
       call dump_f32(dir // '/lsen_gsc_ti.bin', Lsen_Gsc, nx*ny*kmaxRc*(nz-1))
     end if
   end subroutine depthkernelTI_hip
+
+  ! Buffers one ray of period index knumi (1-based, as the reference's knumi):
+  ! the cells (jj, kk) with |fdm(jj,kk)| >= ftol, in the reference's loop order
+  ! (jj outer, kk inner), with their fdm, fdmc, fdms.
+  subroutine g_rows_hip_add_ray(knumi, nvz, nvx, fdm, fdmc, fdms)
+    integer, intent(in) :: knumi, nvz, nvx
+    real, intent(in) :: fdm(0:nvz+1, 0:nvx+1), fdmc(0:nvz+1, 0:nvx+1), fdms(0:nvz+1, 0:nvx+1)
+    integer :: jj, kk, n
+    if (gb_nray == 0) then
+      gb_nvx = nvx
+      gb_nvz = nvz
+      if (.not. allocated(gb_k)) then
+        allocate(gb_k(1024), gb_off(1025), gb_cell(65536), gb_w(65536), gb_wc(65536), &
+                 gb_ws(65536))
+      end if
+      gb_off(1) = 0
+      gb_ncell = 0
+    else if (nvx /= gb_nvx .or. nvz /= gb_nvz) then
+      write(*, '(A)') 'g_rows_hip_add_ray: the grid changed within one batch'
+      stop DAZI_DK_EINVAL
+    end if
+    if (gb_nray + 1 > size(gb_k)) then
+      call grow_i32(gb_k, 2 * size(gb_k))
+      call grow_i64(gb_off, 2 * size(gb_off))
+    end if
+    n = count(abs(fdm(1:nvz, 1:nvx)) >= GROW_FTOL)
+    if (gb_ncell + n > size(gb_cell)) then
+      call grow_i32(gb_cell, max(2 * size(gb_cell), gb_ncell + n))
+      call grow_f32(gb_w, size(gb_cell))
+      call grow_f32(gb_wc, size(gb_cell))
+      call grow_f32(gb_ws, size(gb_cell))
+    end if
+    do jj = 1, nvz
+      do kk = 1, nvx
+        if (abs(fdm(jj, kk)) >= GROW_FTOL) then
+          gb_ncell = gb_ncell + 1
+          gb_cell(gb_ncell) = (jj - 1) * nvx + (kk - 1)
+          gb_w(gb_ncell) = fdm(jj, kk)
+          gb_wc(gb_ncell) = fdmc(jj, kk)
+          gb_ws(gb_ncell) = fdms(jj, kk)
+        end if
+      end do
+    end do
+    gb_nray = gb_nray + 1
+    gb_k(gb_nray) = knumi - 1
+    gb_off(gb_nray + 1) = gb_ncell
+  end subroutine g_rows_hip_add_ray
+
+  ! Assembles the buffered rays (data rows first_row, first_row+1, ...) from the
+  ! stage outputs in the reference's layouts and appends their triplets to
+  ! rw/iw/col at nar+1 (at most maxnar entries in all); nar is updated and the
+  ! buffer emptied.
+  subroutine g_rows_hip_flush(nx, ny, nz, kmaxRc, vels, sen_vsRc, sen_vpRc, sen_rhoRc, Lsen_Gsc, &
+                              first_row, maxnar, rw, iw, col, nar)
+    integer, intent(in) :: nx, ny, nz, kmaxRc, first_row
+    real, intent(in) :: vels(nx, ny, nz)
+    real*8, intent(in) :: sen_vsRc(nx*ny, kmaxRc, nz), sen_vpRc(nx*ny, kmaxRc, nz)
+    real*8, intent(in) :: sen_rhoRc(nx*ny, kmaxRc, nz)
+    real*4, intent(in) :: Lsen_Gsc(nx*ny, kmaxRc, nz-1)
+    integer(kind=8), intent(in) :: maxnar
+    real, intent(inout) :: rw(*)
+    integer, intent(inout) :: iw(*), col(*)
+    integer(kind=8), intent(inout) :: nar
+    type(dazi_dk_grow_desc) :: d
+    integer(c_int64_t) :: n
+    integer(c_int32_t) :: dummy_i(1)
+    real(c_float) :: dummy_f(1)
+    if (gb_nray == 0) return
+    if (gb_nvx /= nx - 2 .or. gb_nvz /= ny - 2) then
+      write(*, '(A)') 'g_rows_hip_flush: the rays do not match the nx x ny grid'
+      stop DAZI_DK_EINVAL
+    end if
+    call log_line('G row assembly backend: hip')
+    d = dazi_dk_grow_desc(nx - 2, ny - 2, nz, kmaxRc, int(gb_nray, c_int64_t), first_row, &
+                          GROW_FTOL)
+    if (maxnar > nar) then
+      call check(c_dazi_dk_grows(d, gb_k, gb_off, gb_cell, gb_w, gb_wc, gb_ws, vels, sen_vsRc, &
+                                 sen_vpRc, sen_rhoRc, Lsen_Gsc, maxnar - nar, rw(nar + 1), &
+                                 iw(nar + 1), col(nar + 1), n), 'g_rows')
+    else
+      call check(c_dazi_dk_grows(d, gb_k, gb_off, gb_cell, gb_w, gb_wc, gb_ws, vels, sen_vsRc, &
+                                 sen_vpRc, sen_rhoRc, Lsen_Gsc, 0_c_int64_t, dummy_f, dummy_i, &
+                                 dummy_i, n), 'g_rows')
+    end if
+    nar = nar + n
+    gb_nray = 0
+  end subroutine g_rows_hip_flush
+
+  subroutine grow_i32(a, n)
+    integer(c_int32_t), allocatable, intent(inout) :: a(:)
+    integer, intent(in) :: n
+    integer(c_int32_t), allocatable :: b(:)
+    allocate(b(n))
+    b(1:size(a)) = a
+    call move_alloc(b, a)
+  end subroutine grow_i32
+
+  subroutine grow_i64(a, n)
+    integer(c_int64_t), allocatable, intent(inout) :: a(:)
+    integer, intent(in) :: n
+    integer(c_int64_t), allocatable :: b(:)
+    allocate(b(n))
+    b(1:size(a)) = a
+    call move_alloc(b, a)
+  end subroutine grow_i64
+
+  subroutine grow_f32(a, n)
+    real(c_float), allocatable, intent(inout) :: a(:)
+    integer, intent(in) :: n
+    real(c_float), allocatable :: b(:)
+    allocate(b(n))
+    b(1:size(a)) = a
+    call move_alloc(b, a)
+  end subroutine grow_f32
 
 end module dazi_dk

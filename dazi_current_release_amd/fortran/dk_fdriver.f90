@@ -5,6 +5,13 @@
 !           depz(nz) f32, tRc(kmax) f64
 !   output: pvRc(nx*ny,kmax) f64, sen_vs, sen_vp, sen_rho (nx*ny,kmax,nz) f64,
 !           pvTI(nx*ny,kmax) f64, Lsen_Gsc(nx*ny,kmax,nz-1) f32
+! With two more arguments it then assembles G rows as the reference's G
+! builder would, ray by ray through g_rows_hip_add_ray and one
+! g_rows_hip_flush:
+!   rays   : nray (int32), knumi(nray) int32 (1-based), then per ray the maps
+!            fdm, fdmc, fdms (0:ny-1, 0:nx-1) f32 (the reference's
+!            fdm(0:nvz+1, 0:nvx+1))
+!   output : nar (int64), rw(nar) f32, iw(nar) int32, col(nar) int32
 program dk_fdriver
   use dazi_dk
   implicit none
@@ -13,7 +20,11 @@ program dk_fdriver
   real, allocatable :: vel(:,:,:), depz(:)
   real*8, allocatable :: tRc(:), pv(:,:), svs(:,:,:), svp(:,:,:), srho(:,:,:), pvti(:,:)
   real*4, allocatable :: lsen(:,:,:)
-  character(len=1024) :: fin, fout
+  character(len=1024) :: fin, fout, frays, fg
+  integer :: nray, r
+  integer, allocatable :: knumi(:), iw(:), col(:)
+  real, allocatable :: fdm(:,:), fdmc(:,:), fdms(:,:), rw(:)
+  integer(kind=8) :: nar, maxnar
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -28,5 +39,24 @@ program dk_fdriver
   call depthkernelTI_hip(nx, ny, nz, vel, pvti, 2, 0, kmax, tRc, depz, minthk, lsen)
   open(newunit=u, file=trim(fout), form='unformatted', access='stream', status='replace')
   write(u) pv, svs, svp, srho, pvti, lsen
+  close(u)
+  if (command_argument_count() < 4) stop
+  call get_command_argument(3, frays)
+  call get_command_argument(4, fg)
+  open(newunit=u, file=trim(frays), form='unformatted', access='stream', status='old')
+  read(u) nray
+  allocate(knumi(nray), fdm(0:ny-1, 0:nx-1), fdmc(0:ny-1, 0:nx-1), fdms(0:ny-1, 0:nx-1))
+  read(u) knumi
+  do r = 1, nray
+    read(u) fdm, fdmc, fdms
+    call g_rows_hip_add_ray(knumi(r), ny - 2, nx - 2, fdm, fdmc, fdms)
+  end do
+  close(u)
+  maxnar = 3_8 * (nz - 1) * (nx - 2) * (ny - 2) * nray
+  allocate(rw(maxnar), iw(maxnar), col(maxnar))
+  nar = 0
+  call g_rows_hip_flush(nx, ny, nz, kmax, vel, svs, svp, srho, lsen, 1, maxnar, rw, iw, col, nar)
+  open(newunit=u, file=trim(fg), form='unformatted', access='stream', status='replace')
+  write(u) nar, rw(1:nar), iw(1:nar), col(1:nar)
   close(u)
 end program dk_fdriver

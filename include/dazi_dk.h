@@ -193,6 +193,56 @@ int32_t dazi_dk_effective_dev(const dazi_dk_desc* desc, const float* vel, const 
                               const double* sen_vp, const double* sen_rho, const double* grad_c,
                               double* eff, double* grad_vs, void* stream);
 
+/* G-row assembly: the sparse rows (rw, iw, col) of the joint inversion's G
+ * matrix that the reference builds ray by ray from the path weights and the
+ * stage outputs (CalSurfGAniso_Joint.f90:750-797, inside the source/period/
+ * record loops :511-799; replaces that loop body's row fill and compression).
+ * For ray r (data row iw = row_base + r, 1-based) of period index ray_k[r]
+ * (0-based, the reference's knumi - 1) and each of its cells with
+ * |fdm| >= ftol, the row holds over z = 0..nz-2
+ *   Vs block: real((sen_vp*coe_a + sen_rho*coe_rho + sen_vs)(post,k,z) * fdm)
+ *   Gc block: Lsen_Gsc(post,k,z) * fdmc,   Gs block: Lsen_Gsc(post,k,z) * fdms
+ * at model column col = block*nparpi + z*nvx*nvz + cell + 1 (1-based),
+ * nparpi = nvx*nvz*(nz-1), post = (cell/nvx + 1)*(nvx+2) + cell%nvx + 1; the
+ * entries with |value| > ftol are emitted in increasing (iw, col), exactly the
+ * reference's order.  A ray's cells are given as cell = (jj-1)*nvx + (kk-1)
+ * (the interior cell of fdm(jj,kk)), strictly increasing, in
+ * cell[cell_off[r] .. cell_off[r+1]) with their fdm, fdmc, fdms.  The stage
+ * arrays keep the stage's layouts over the full nx*ny = (nvx+2)*(nvz+2) grid:
+ * vel (nx*ny, nz) f32, sen_* (nx*ny, kmax, nz) f64, lsen (nx*ny, kmax, nz-1)
+ * f32.  ftol is the reference's 1e-4. */
+typedef struct dazi_dk_grow_desc {
+  int32_t nvx, nvz;        /* interior grid: nx-2, ny-2 */
+  int32_t nz;              /* depth nodes of the stage arrays */
+  int32_t kmax;            /* periods of the stage arrays */
+  int64_t nray;            /* rays (data rows) in this call */
+  int32_t row_base;        /* iw of ray 0 (the reference's count1 of that ray) */
+  float ftol;              /* 1e-4 in the reference */
+} dazi_dk_grow_desc;
+
+/* Device pointers; enqueues on stream.  Writes the row count of each ray to
+ * row_nnz (device, nray; NULL = internal) and the total to *nar (device
+ * int64), and the first min(total, cap) entries to rw, iw, col (device).  A
+ * caller that needs all entries sizes cap from *nar (an upper bound is
+ * 3*(nz-1)*cell_off[nray]).  Rays with an invalid period or cell index get no
+ * entries for it (dazi_dk_grows rejects them). */
+int32_t dazi_dk_grows_dev(const dazi_dk_grow_desc* desc, const int32_t* ray_k,
+                          const int64_t* cell_off, const int32_t* cell, const float* fdm,
+                          const float* fdmc, const float* fdms, const float* vel,
+                          const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                          const float* lsen, int64_t cap, float* rw, int32_t* iw, int32_t* col,
+                          int64_t* row_nnz, int64_t* nar, void* stream);
+
+/* Host pointers; synchronous.  Validates the ray description (period and cell
+ * ranges, increasing cells), uploads, assembles, and returns the entries with
+ * their count in *nar; DAZI_DK_ELIMIT (with *nar set) when they exceed cap. */
+int32_t dazi_dk_grows(const dazi_dk_grow_desc* desc, const int32_t* ray_k,
+                      const int64_t* cell_off, const int32_t* cell, const float* fdm,
+                      const float* fdmc, const float* fdms, const float* vel,
+                      const double* sen_vs, const double* sen_vp, const double* sen_rho,
+                      const float* lsen, int64_t cap, float* rw, int32_t* iw, int32_t* col,
+                      int64_t* nar);
+
 /* Bytes of device memory the library's pool holds on the current device
  * (current slots; buffers retired by growth are not counted). */
 int32_t dazi_dk_pool_bytes(int64_t* out);

@@ -345,3 +345,65 @@ def ti_at(vs, periods, cp, depz, minthk):
                 lsen[:, j] = (lsen[:, j] + tA[:, k]) + tL[:, k]
                 k += 1
     return lsen
+
+
+# ----------------------------------------------------------------- G rows
+def brocher_row_coeffs(v):
+    """coe_a, coe_rho of the G builder's Vs row (CalSurfGAniso_Joint.f90:754-760),
+    float32 arrays, left to right as the reference's real arithmetic."""
+    f = np.float32
+    v = np.asarray(v, np.float32)
+    v2 = v * v
+    v3 = v2 * v
+    v4 = v3 * v
+    a = f(2.0947) - f(0.8206) * f(2.0) * v + f(0.2683) * f(3.0) * v2 - f(0.0251) * f(4.0) * v3
+    p = f(0.9409) + f(2.0947) * v - f(0.8206) * v2 + f(0.2683) * v3 - f(0.0251) * v4
+    p2 = p * p
+    p3 = p2 * p
+    p4 = p3 * p
+    r = a * (f(1.6612) - f(0.4721) * f(2.0) * p + f(0.0671) * f(3.0) * p2 -
+             f(0.0043) * f(4.0) * p3 + f(0.000106) * f(5.0) * p4)
+    return a, r
+
+
+def g_rows(nvx, nvz, nz, ray_k, fdm, fdmc, fdms, vel, sen_vs, sen_vp, sen_rho, lsen,
+           row_base=1, ftol=1.0e-4):
+    """The G builder's row fill and compression (CalSurfGAniso_Joint.f90:746-797)
+    restated, ray by ray: a dense float32 row of 3*nparpi entries, filled for each
+    interior cell (jj, kk) with |fdm(jj,kk)| >= ftol, then (rw, iw, col) of the
+    entries with |row| > ftol in increasing col.
+
+    fdm, fdmc, fdms: (nray, nvz+2, nvx+2) float32, [r, jj, kk] = fdm(jj, kk) of ray
+    r; ray_k: 0-based period index (knumi - 1); vel (nz, ncol) f32, sen_* (nz, K,
+    ncol) f64, lsen (nz-1, K, ncol) f32 with ncol = (nvx+2)*(nvz+2), column
+    post = jj*(nvx+2) + kk.  Returns rw f32, iw i32, col i32 (1-based)."""
+    f = np.float32
+    ftol = f(ftol)
+    nd = nz - 1
+    cells = nvx * nvz
+    nparpi = cells * nd
+    nx = nvx + 2
+    ca, cr = brocher_row_coeffs(vel[:nd])          # (nd, ncol)
+    out_rw, out_iw, out_col = [], [], []
+    for r in range(len(ray_k)):
+        k = int(ray_k[r])
+        row = np.zeros(3 * nparpi, np.float32)
+        for jj in range(1, nvz + 1):
+            for kk in range(1, nvx + 1):
+                w = fdm[r, jj, kk]
+                if not abs(w) >= ftol:
+                    continue
+                post = jj * nx + kk
+                z = np.arange(nd)
+                idx = z * cells + (jj - 1) * nvx + (kk - 1)
+                eff = (sen_vp[:nd, k, post] * ca[:, post].astype(np.float64) +
+                       sen_rho[:nd, k, post] * cr[:, post].astype(np.float64)) + sen_vs[:nd, k, post]
+                row[idx] = (eff * np.float64(w)).astype(np.float32)
+                row[nparpi + idx] = lsen[:, k, post] * fdmc[r, jj, kk]
+                row[2 * nparpi + idx] = lsen[:, k, post] * fdms[r, jj, kk]
+        keep = np.nonzero(np.abs(row) > ftol)[0]
+        out_rw.append(row[keep])
+        out_iw.append(np.full(keep.size, row_base + r, np.int32))
+        out_col.append((keep + 1).astype(np.int32))
+    cat = lambda x, t: np.concatenate(x).astype(t) if x else np.zeros(0, t)  # noqa: E731
+    return cat(out_rw, np.float32), cat(out_iw, np.int32), cat(out_col, np.int32)

@@ -34,11 +34,41 @@ def read_output(path, ncol, K, nz):
     return out
 
 
-def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None):
+def write_rays(path, knumi, maps):
+    """knumi (nray,) 1-based; maps: per ray (fdm, fdmc, fdms) arrays [jj, kk] of
+    shape (ny, nx), written as the reference's fdm(0:nvz+1, 0:nvx+1)."""
+    with open(path, "wb") as fh:
+        np.array([len(knumi)], np.int32).tofile(fh)
+        np.asarray(knumi, np.int32).tofile(fh)
+        for m in maps:
+            for a in m:
+                np.asarray(a, np.float32).ravel(order="F").tofile(fh)
+
+
+def read_triplets(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw, np.int64, count=1)[0])
+    rw = np.frombuffer(raw, np.float32, count=n, offset=8)
+    iw = np.frombuffer(raw, np.int32, count=n, offset=8 + 4 * n)
+    col = np.frombuffer(raw, np.int32, count=n, offset=8 + 8 * n)
+    assert len(raw) == 8 + 12 * n, "unexpected triplet file size"
+    return rw, iw, col
+
+
+def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None, rays=None):
+    """Runs the driver; rays = (knumi, maps) adds the G-row step, whose
+    triplets are read back as a third return value."""
     fin, fout = os.path.join(tmpdir, "in.bin"), os.path.join(tmpdir, "out.bin")
     write_input(fin, vel, depz, periods, minthk)
+    args = [DRIVER, fin, fout]
+    fg = None
+    if rays is not None:
+        frays, fg = os.path.join(tmpdir, "rays.bin"), os.path.join(tmpdir, "g.bin")
+        write_rays(frays, *rays)
+        args += [frays, fg]
     e = dict(os.environ)
     e.update(env or {})
-    r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=timeout,
-                       env=e)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=timeout, env=e)
+    if rays is not None:
+        return r, fout, fg
     return r, fout

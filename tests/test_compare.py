@@ -65,3 +65,29 @@ def test_brocher_coefficients_are_derivatives():
     h = 1e-6
     assert np.allclose(a, (vp(vs + h) - vp(vs - h)) / (2 * h), rtol=1e-5)
     assert np.allclose(r, (rho(vp(vs + h)) - rho(vp(vs - h))) / (2 * h), rtol=1e-5)
+
+
+def test_g_triplet_comparison_sets_and_families(tmp_path):
+    """compare_triplets keys by (iw, col): equal runs give zero differences; a
+    changed value, a dropped and an extra entry land in the right sets and
+    Vs / Gc / Gs families (compare_g_triplet_dumps.py:53-123)."""
+    from dazi_current_release_amd.compare import compare_triplets, main
+    from dazi_current_release_amd.grows import dump_g_triplets
+    nparpi = 10
+    iw = np.array([1, 1, 1, 2, 2], np.int32)
+    col = np.array([3, 12, 25, 5, 11], np.int32)
+    rw = np.array([0.5, -1.0, 2.0, 0.25, 3.0], np.float32)
+    rows = {r["set"]: r for r in compare_triplets((iw, col, rw), (iw, col, rw), nparpi)}
+    assert rows["common_diff"]["max_abs"] == 0.0 and rows["base_only"]["n"] == 0
+    assert (rows["base_all"]["Vs"], rows["base_all"]["Gc"], rows["base_all"]["Gs"]) == (2, 2, 1)
+    t_iw = np.array([1, 1, 2, 2, 2], np.int32)
+    t_col = np.array([3, 12, 5, 11, 29], np.int32)
+    t_rw = np.array([0.5, -1.5, 0.25, 3.0, 1.2e-4], np.float32)
+    dump_g_triplets(rw, iw, col, str(tmp_path / "a"))
+    dump_g_triplets(t_rw, t_iw, t_col, str(tmp_path / "b"))
+    rows = {r["set"]: r for r in main(["--base", str(tmp_path / "a"), "--test",
+                                       str(tmp_path / "b"), "--nparpi", str(nparpi)])}
+    assert rows["common_diff"]["n"] == 4 and rows["common_diff"]["max_abs"] == 0.5
+    assert rows["common_diff_Gc"]["max_abs"] == 0.5 and rows["common_diff_Vs"]["max_abs"] == 0
+    assert rows["base_only"]["n"] == 1 and rows["base_only"]["Gs"] == 1
+    assert rows["test_only"]["n"] == 1 and rows["test_only"]["abs_le_0.00015"] == 1

@@ -70,3 +70,38 @@ def test_fortran_dumps_ab_against_reference(tmp_path):
     for name, (l2, mx) in (("sen_vs", (1e-3, 2e-4)), ("sen_vp", (2e-3, 2e-4)),
                            ("sen_rho", (2e-3, 2e-4)), ("effective_dcdvs", (1e-3, 3e-4))):
         assert rows[name]["rel_l2"] <= l2 and rows[name]["max_abs"] <= mx, (name, rows[name])
+
+
+def test_fortran_g_rows_match_the_device_path_and_the_oracle(tmp_path):
+    """A Fortran G builder's loop: g_rows_hip_add_ray per ray (dense fdm maps,
+    the reference's argument form) and one g_rows_hip_flush, against the
+    Python host entry on the same stage outputs and against the oracle."""
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from oracle import port as P
+    from dazi_current_release_amd import configs, grows
+    f = load_golden("ref_plain_1024.npz")
+    nz, K = f["vel"].shape[1], len(f["periods"])
+    nx, ny = 8, 4
+    vel = np.asfortranarray(f["vel"].reshape(nx, ny, nz, order="F"))
+    R = configs.ray_workload(nx - 2, ny - 2, K, 3, seed=12, cell_km=5.0)
+    nray = len(R["ray_k"])
+    maps = [configs.ray_dense(R, r) for r in range(nray)]
+    r, fout, fg = FIO.run(str(tmp_path), vel, f["depz"], f["periods"], float(f["minthk"]),
+                          rays=(R["ray_k"] + 1, maps))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "G row assembly backend: hip" in r.stdout
+    rw, iw, col = FIO.read_triplets(fg)
+    o = FIO.read_output(fout, nx * ny, K, nz)
+    # stage arrays in the library's (node, period, column) order
+    st = lambda a: np.ascontiguousarray(np.moveaxis(a, [0, 1, 2], [2, 1, 0]))  # noqa: E731
+    velz = np.ascontiguousarray(vel.reshape(nx * ny, nz, order="F").T)
+    args = (velz, st(o["sen_vs"]), st(o["sen_vp"]), st(o["sen_rho"]), st(o["ti_lsen"]))
+    want = grows.g_rows(R, nx - 2, ny - 2, *args)
+    assert rw.size == want[0].size > 0
+    for got, w in zip((rw, iw, col), want):
+        assert np.array_equal(got, w)
+    fdm, fdmc, fdms = (np.array([m[i] for m in maps]) for i in range(3))
+    ref = P.g_rows(nx - 2, ny - 2, nz, R["ray_k"], fdm, fdmc, fdms, *args)
+    for got, w in zip((rw, iw, col), ref):
+        assert np.array_equal(got, w)
