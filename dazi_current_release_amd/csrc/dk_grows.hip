@@ -58,12 +58,52 @@ struct GrowArgs {
 constexpr int GROW_WAVES = 4;   // rays per workgroup, one wave each
 constexpr int NXCD = 8;         // MI355X: workgroups are dealt to the 8 XCDs round robin
 
-// effT[(k*ncol + col)*nd + z] = eff(z, k, col), lsenT likewise from Lsen_Gsc;
-// one thread per (column, period).
-__global__ void k_grow_planes(int ncol, int nz, int K, const float* __restrict__ vel,
-                              const double* __restrict__ svs, const double* __restrict__ svp,
-                              const double* __restrict__ srho, const float* __restrict__ lsen,
-                              double* __restrict__ effT, float* __restrict__ lsenT) {
+// effT[(k*ncol + col)*nd + z] = eff(z, k, col), lsenT likewise from Lsen_Gsc.
+// k_grow_planes: one workgroup per (64-column tile, period); the tile is read
+// along columns (coalesced), transposed in LDS (row stride nd + 1: the
+// column-strided LDS writes fall in distinct banks) and written as one
+// contiguous run.  k_grow_planes_wide: one thread per (column, period), for
+// depth counts whose tile does not fit in LDS.
+constexpr int PT = 64;  // columns per planes tile
+
+__global__ void __launch_bounds__(256) k_grow_planes(int ncol, int nz, int K,
+                                                     const float* __restrict__ vel,
+                                                     const double* __restrict__ svs,
+                                                     const double* __restrict__ svp,
+                                                     const double* __restrict__ srho,
+                                                     const float* __restrict__ lsen,
+                                                     double* __restrict__ effT,
+                                                     float* __restrict__ lsenT) {
+  extern __shared__ double pe[];  // PT x (nd + 1) doubles, then PT x (nd + 1) floats
+  const int nd = nz - 1, rs = nd + 1;
+  float* pl = (float*)(pe + PT * rs);
+  const int col0 = blockIdx.x * PT, k = blockIdx.y;
+  const int nc = min(PT, ncol - col0);
+  for (int idx = threadIdx.x; idx < nd * PT; idx += blockDim.x) {
+    const int z = idx / PT, c = idx - z * PT;
+    if (c < nc) {
+      const int col = col0 + c;
+      float ca, cr;
+      brocher_coeffs(vel[(long)z * ncol + col], &ca, &cr);
+      const long o = ((long)z * K + k) * ncol + col;
+      pe[c * rs + z] = effective_vs(svs[o], svp[o], srho[o], ca, cr);
+      pl[c * rs + z] = lsen[o];
+    }
+  }
+  __syncthreads();
+  const long base = ((long)k * ncol + col0) * nd;
+  for (int idx = threadIdx.x; idx < nc * nd; idx += blockDim.x) {
+    const int c = idx / nd, z = idx - c * nd;
+    effT[base + idx] = pe[c * rs + z];
+    lsenT[base + idx] = pl[c * rs + z];
+  }
+}
+
+__global__ void k_grow_planes_wide(int ncol, int nz, int K, const float* __restrict__ vel,
+                                   const double* __restrict__ svs, const double* __restrict__ svp,
+                                   const double* __restrict__ srho,
+                                   const float* __restrict__ lsen, double* __restrict__ effT,
+                                   float* __restrict__ lsenT) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long)ncol * K) return;
   const int col = (int)(t % ncol), k = (int)(t / ncol);
@@ -335,9 +375,16 @@ hipError_t launch_grows(const GrowDev& g, hipStream_t st) {
                        (const long*)g.tsum, (long*)g.nar);
     return hipGetLastError();
   }
-  const long npl = (long)a.ncol * g.K;
-  hipLaunchKernelGGL(k_grow_planes, dim3((unsigned)((npl + 255) / 256)), dim3(256), 0, st, a.ncol,
-                     g.nz, g.K, g.vel, g.svs, g.svp, g.srho, g.lsen, g.effT, g.lsenT);
+  const size_t plds = (size_t)PT * g.nz * 12;  // PT x (nd + 1) x (8 + 4) bytes
+  if (plds <= 48 * 1024) {
+    hipLaunchKernelGGL(k_grow_planes, dim3((unsigned)((a.ncol + PT - 1) / PT), (unsigned)g.K),
+                       dim3(256), plds, st, a.ncol, g.nz, g.K, g.vel, g.svs, g.svp, g.srho,
+                       g.lsen, g.effT, g.lsenT);
+  } else {
+    const long npl = (long)a.ncol * g.K;
+    hipLaunchKernelGGL(k_grow_planes_wide, dim3((unsigned)((npl + 255) / 256)), dim3(256), 0, st,
+                       a.ncol, g.nz, g.K, g.vel, g.svs, g.svp, g.srho, g.lsen, g.effT, g.lsenT);
+  }
   const unsigned nb = grow_blocks(g.nray);
   hipLaunchKernelGGL(k_grows<false>, dim3(nb), dim3(64 * GROW_WAVES), 0, st, a, (int)nb);
   const int nt = (int)grow_scan_tiles(g.nray);

@@ -117,11 +117,11 @@ def test_cap_empty_and_subset_batches(setup):
         assert torch.equal(s[k][:ns], full[k][:n][sel])
 
 
-@pytest.mark.parametrize("nz", [7, 9])
+@pytest.mark.parametrize("nz", [7, 9, 70])
 def test_scalar_path_long_rays_and_other_depth_counts(setup, nz):
     """Rays of more than 128 cells (a 150-cell-wide grid) and depth counts that
-    are not a multiple of 4 (nz = 7) take the scalar-load path; nz = 9 mixes
-    vector rays with long ones."""
+    are not a multiple of 4 (nz = 7, 70) take the scalar-load path; nz = 9 mixes
+    vector rays with long ones; nz = 70 also takes the untiled planes kernel."""
     import torch
     from dazi_current_release_amd import configs, grows
     nvx, nvz, K = 150, 6, 3
@@ -130,7 +130,7 @@ def test_scalar_path_long_rays_and_other_depth_counts(setup, nz):
     vel = rng.uniform(2.5, 4.5, (nz, ncol)).astype(np.float32)
     sv = [rng.standard_normal((nz, K, ncol)) * 0.05 for _ in range(3)]
     ls = (rng.standard_normal((nz - 1, K, ncol)) * 0.05).astype(np.float32)
-    R = configs.ray_workload(nvx, nvz, K, 5, seed=nz, cell_km=0.5)
+    R = configs.ray_workload(nvx, nvz, K, 5, seed=7, cell_km=0.5)  # a 138-cell ray
     ncell = np.diff(R["cell_off"])
     print("cells per ray: max", ncell.max(), "min", ncell.min())
     assert ncell.max() > 128
