@@ -22,7 +22,8 @@ EXPORTS = [
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
     "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
-    "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows",
+    "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows", "dazi_dk_azimuth_map_dev",
+    "dazi_dk_azimuth_map",
 ]
 
 
@@ -52,6 +53,10 @@ class Layered(C.Structure):
                 ("dc_dvs", C.c_void_p), ("dc_dvp", C.c_void_p), ("dc_drho", C.c_void_p),
                 ("c_unrounded", C.c_void_p), ("thk64", C.c_void_p), ("vp64", C.c_void_p),
                 ("vs64", C.c_void_p), ("rho64", C.c_void_p), ("nlayers_total", C.c_int64)]
+
+
+class AzimDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("kmax", C.c_int32)]
 
 
 class GrowDesc(C.Structure):
@@ -99,6 +104,8 @@ def lib():
                                         [C.c_int64] + [C.c_void_p] * 6)
         L.dazi_dk_grows.argtypes = ([C.POINTER(GrowDesc)] + [C.c_void_p] * 11 + [C.c_int64] +
                                     [C.c_void_p] * 4)
+        L.dazi_dk_azimuth_map_dev.argtypes = [C.POINTER(AzimDesc)] + [C.c_void_p] * 10
+        L.dazi_dk_azimuth_map.argtypes = [C.POINTER(AzimDesc)] + [C.c_void_p] * 9
         for name in EXPORTS:
             if not hasattr(L, name):
                 raise RuntimeError(f"{LIB_PATH} does not export {name}")

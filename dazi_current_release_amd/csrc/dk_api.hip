@@ -77,6 +77,9 @@ enum Slot {
   S_G_EFF,
   S_G_LSENT,
   S_G_TSUM,
+  S_A_G,
+  S_A_T,
+  S_A_OUT,
   S_COUNT
 };
 
@@ -612,6 +615,55 @@ int32_t dazi_dk_grows(const dazi_dk_grow_desc* d, const int32_t* ray_k, const in
   if (*nar > cap)
     return set_err(DAZI_DK_ELIMIT, "%lld entries exceed cap %lld (nar holds the count)",
                    (long long)*nar, (long long)cap);
+  return 0;
+}
+
+int32_t dazi_dk_azimuth_map_dev(const dazi_dk_azim_desc* d, const float* lsen, const float* gcf,
+                                const float* gsf, const double* tRcV, float* cos_rc,
+                                float* sin_rc, float* amp, float* amp_rela, float* angle,
+                                void* stream) {
+  if (!d) return set_err(DAZI_DK_EINVAL, "NULL descriptor");
+  if (d->nx < 3 || d->ny < 3 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d (nx, ny >= 3)", d->nx,
+                   d->ny, d->nz, d->kmax);
+  if (!lsen || !gcf || !gsf || !cos_rc || !sin_rc || (amp_rela && !tRcV))
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  HIPCHK(dk::launch_azimuth_map(d->nx - 2, d->ny - 2, d->nz, d->kmax, lsen, gcf, gsf, tRcV,
+                                cos_rc, sin_rc, amp, amp_rela, angle, (hipStream_t)stream));
+  return 0;
+}
+
+int32_t dazi_dk_azimuth_map(const dazi_dk_azim_desc* d, const float* lsen, const float* gcf,
+                            const float* gsf, const double* tRcV, float* cos_rc, float* sin_rc,
+                            float* amp, float* amp_rela, float* angle) {
+  if (!d) return set_err(DAZI_DK_EINVAL, "NULL descriptor");
+  if (d->nx < 3 || d->ny < 3 || d->nz < 2 || d->kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d (nx, ny >= 3)", d->nx,
+                   d->ny, d->nz, d->kmax);
+  if (!lsen || !gcf || !gsf || !tRcV || !cos_rc || !sin_rc || !amp || !amp_rela || !angle)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  const size_t ncol = (size_t)d->nx * d->ny, nd = d->nz - 1, K = d->kmax;
+  const size_t cells = (size_t)(d->nx - 2) * (d->ny - 2);
+  void *dl, *dg, *dt, *dout;
+  if (int rc = pool_get(P, S_H_LSEN, ncol * K * nd * 4, &dl)) return rc;
+  if (int rc = pool_get(P, S_A_G, cells * nd * 8, &dg)) return rc;
+  if (int rc = pool_get(P, S_A_T, cells * K * 8, &dt)) return rc;
+  if (int rc = pool_get(P, S_A_OUT, cells * K * 4 * 5, &dout)) return rc;
+  float* g = (float*)dg;
+  float* o = (float*)dout;
+  const size_t m = cells * K;
+  HIPCHK(hipMemcpy(dl, lsen, ncol * K * nd * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g, gcf, cells * nd * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g + cells * nd, gsf, cells * nd * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dt, tRcV, m * 8, hipMemcpyHostToDevice));
+  HIPCHK(dk::launch_azimuth_map(d->nx - 2, d->ny - 2, d->nz, d->kmax, (const float*)dl, g,
+                                g + cells * nd, (const double*)dt, o, o + m, o + 2 * m, o + 3 * m,
+                                o + 4 * m, nullptr));
+  HIPCHK(hipStreamSynchronize(nullptr));
+  float* outs[5] = {cos_rc, sin_rc, amp, amp_rela, angle};
+  for (int q = 0; q < 5; q++) HIPCHK(hipMemcpy(outs[q], o + q * m, m * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 

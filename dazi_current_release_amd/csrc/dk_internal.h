@@ -118,6 +118,12 @@ struct GrowDev {
 hipError_t launch_grows(const GrowDev& g, hipStream_t st);
 long grow_scan_tiles(long nray);
 
+// Period azimuthal-anisotropy map (FwdAzimuthalAniMap.f90:44-89); see
+// dazi_dk_azimuth_map_dev.
+hipError_t launch_azimuth_map(int nvx, int nvz, int nz, int K, const float* lsen, const float* gcf,
+                              const float* gsf, const double* tRcV, float* cos_rc, float* sin_rc,
+                              float* amp, float* amp_rela, float* angle, hipStream_t st);
+
 hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const double* svs,
                             const double* svp, const double* srho, const double* gc, double* eff,
                             double* grad, hipStream_t st);

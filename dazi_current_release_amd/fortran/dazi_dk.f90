@@ -20,12 +20,16 @@
 ! period index; flush assembles every buffered ray on the GPU and appends the
 ! (rw, iw, col) triplets at nar+1 in the reference's order.
 ! DAZI_G_TRIPLET_DUMP_DIR is left to the caller's dump_g_triplets.
+!
+! FwdAzimuthalAniMap_hip takes the argument list of FwdAzimuthalAniMap
+! (FwdAzimuthalAniMap.f90:6-8) and writes the same unit-42 lines.
 module dazi_dk
   use, intrinsic :: iso_c_binding
   implicit none
   private
   public :: depthkernel_hip, depthkernelTI_hip, dazi_dk_desc, dazi_dk_tune
   public :: g_rows_hip_add_ray, g_rows_hip_flush, dazi_dk_grow_desc, c_dazi_dk_grows
+  public :: FwdAzimuthalAniMap_hip, dazi_dk_azim_desc, c_dazi_dk_azimuth_map
   public :: c_dazi_dk_iso, c_dazi_dk_ti, c_dazi_dk_set_tune, dazi_dk_error_text
 
   integer(c_int32_t), parameter, public :: DAZI_DK_OK = 0
@@ -52,6 +56,10 @@ module dazi_dk
     real(c_float) :: ftol
   end type dazi_dk_grow_desc
 
+  type, bind(c) :: dazi_dk_azim_desc
+    integer(c_int32_t) :: nx, ny, nz, kmax
+  end type dazi_dk_azim_desc
+
   ! rays buffered by g_rows_hip_add_ray (cells in the reference's jj, kk order)
   integer :: gb_nray = 0, gb_ncell = 0, gb_nvx = 0, gb_nvz = 0
   integer(c_int32_t), allocatable :: gb_k(:), gb_cell(:)
@@ -75,6 +83,16 @@ module dazi_dk
       integer(c_int64_t), intent(out) :: nar
       integer(c_int32_t) :: rc
     end function c_dazi_dk_grows
+
+    function c_dazi_dk_azimuth_map(desc, lsen, gcf, gsf, tRcV, cos_rc, sin_rc, amp, amp_rela, &
+                                   angle) bind(c, name='dazi_dk_azimuth_map') result(rc)
+      import :: c_int32_t, c_float, c_double, dazi_dk_azim_desc
+      type(dazi_dk_azim_desc), intent(in) :: desc
+      real(c_float), intent(in) :: lsen(*), gcf(*), gsf(*)
+      real(c_double), intent(in) :: tRcV(*)
+      real(c_float), intent(out) :: cos_rc(*), sin_rc(*), amp(*), amp_rela(*), angle(*)
+      integer(c_int32_t) :: rc
+    end function c_dazi_dk_azimuth_map
 
     function c_dazi_dk_iso(desc, vel, vp, rho, depz, tRc, pvRc, sen_vs, sen_vp, sen_rho) &
         bind(c, name='dazi_dk_iso') result(rc)
@@ -371,6 +389,40 @@ contains
     nar = nar + n
     gb_nray = 0
   end subroutine g_rows_hip_flush
+
+  ! Reference: FwdAzimuthalAniMap (FwdAzimuthalAniMap.f90:6-89): the map on the
+  ! GPU, then the same lines on unit 42 (opened by the caller), which is closed.
+  subroutine FwdAzimuthalAniMap_hip(nx, ny, nz, maxvp, goxd, gozd, dvxd, dvzd, kmaxRc, tRc, &
+                                    gcf, gsf, Lsen_Gsc, tRcV)
+    integer nx, ny, nz
+    integer maxvp
+    real goxd, gozd, dvxd, dvzd
+    integer kmaxRc
+    real*8 tRc(kmaxRc)
+    real gcf(nx-2, ny-2, nz-1), gsf(nx-2, ny-2, nz-1)
+    real*8 tRcV((nx-2)*(ny-2), kmaxRc)
+    real*4 Lsen_Gsc(nx*ny, kmaxRc, nz-1)
+    real, allocatable :: cosRc(:,:), sinRc(:,:), amp(:,:), rela(:,:), ang(:,:)
+    type(dazi_dk_azim_desc) :: d
+    integer :: ii, jj, tt, c
+    real :: isoC
+    allocate(cosRc((nx-2)*(ny-2), kmaxRc), sinRc((nx-2)*(ny-2), kmaxRc), &
+             amp((nx-2)*(ny-2), kmaxRc), rela((nx-2)*(ny-2), kmaxRc), ang((nx-2)*(ny-2), kmaxRc))
+    d = dazi_dk_azim_desc(nx, ny, nz, kmaxRc)
+    call check(c_dazi_dk_azimuth_map(d, Lsen_Gsc, gcf, gsf, tRcV, cosRc, sinRc, amp, rela, &
+                                     ang), 'FwdAzimuthalAniMap')
+    do tt = 1, kmaxRc
+      do jj = 1, ny-2
+        do ii = 1, nx-2
+          c = (jj-1)*(nx-2) + ii
+          isoC = tRcV(c, tt)
+          write(42, '(10f10.5)') gozd+(jj-1)*dvzd, goxd-(ii-1)*dvxd, tRc(tt), isoC, &
+              ang(c, tt), rela(c, tt), amp(c, tt), cosRc(c, tt), sinRc(c, tt)
+        end do
+      end do
+    end do
+    close(42)
+  end subroutine FwdAzimuthalAniMap_hip
 
   subroutine grow_i32(a, n)
     integer(c_int32_t), allocatable, intent(inout) :: a(:)

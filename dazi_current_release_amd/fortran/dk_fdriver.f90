@@ -12,6 +12,11 @@
 !            fdm, fdmc, fdms (0:ny-1, 0:nx-1) f32 (the reference's
 !            fdm(0:nvz+1, 0:nvx+1))
 !   output : nar (int64), rw(nar) f32, iw(nar) int32, col(nar) int32
+! (a rays argument "-" skips this step).  With two more arguments it calls
+! FwdAzimuthalAniMap_hip on the TI output:
+!   azm in : goxd, gozd, dvxd, dvzd (f32), gcf, gsf (nx-2,ny-2,nz-1) f32,
+!            tRcV((nx-2)*(ny-2),kmax) f64
+!   azm out: the unit-42 text lines
 program dk_fdriver
   use dazi_dk
   implicit none
@@ -25,6 +30,10 @@ program dk_fdriver
   integer, allocatable :: knumi(:), iw(:), col(:)
   real, allocatable :: fdm(:,:), fdmc(:,:), fdms(:,:), rw(:)
   integer(kind=8) :: nar, maxnar
+  character(len=1024) :: fazin, fazout
+  real :: goxd, gozd, dvxd, dvzd
+  real, allocatable :: gcf(:,:,:), gsf(:,:,:)
+  real*8, allocatable :: tRcV(:,:)
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fout)
@@ -43,6 +52,21 @@ program dk_fdriver
   if (command_argument_count() < 4) stop
   call get_command_argument(3, frays)
   call get_command_argument(4, fg)
+  if (trim(frays) /= '-') call g_rows_step()
+  if (command_argument_count() < 6) stop
+  call get_command_argument(5, fazin)
+  call get_command_argument(6, fazout)
+  open(newunit=u, file=trim(fazin), form='unformatted', access='stream', status='old')
+  allocate(gcf(nx-2, ny-2, nz-1), gsf(nx-2, ny-2, nz-1), tRcV((nx-2)*(ny-2), kmax))
+  read(u) goxd, gozd, dvxd, dvzd, gcf, gsf, tRcV
+  close(u)
+  open(42, file=trim(fazout), status='replace', action='write')
+  call FwdAzimuthalAniMap_hip(nx, ny, nz, 0, goxd, gozd, dvxd, dvzd, kmax, tRc, gcf, gsf, lsen, &
+                              tRcV)
+
+contains
+
+  subroutine g_rows_step()
   open(newunit=u, file=trim(frays), form='unformatted', access='stream', status='old')
   read(u) nray
   allocate(knumi(nray), fdm(0:ny-1, 0:nx-1), fdmc(0:ny-1, 0:nx-1), fdms(0:ny-1, 0:nx-1))
@@ -59,4 +83,5 @@ program dk_fdriver
   open(newunit=u, file=trim(fg), form='unformatted', access='stream', status='replace')
   write(u) nar, rw(1:nar), iw(1:nar), col(1:nar)
   close(u)
+  end subroutine g_rows_step
 end program dk_fdriver

@@ -130,14 +130,14 @@ int32_t dazi_dk_iso(const dazi_dk_desc* desc, const float* vel, const float* vp,
  * the same triple only enqueue kernels (no host synchronisation), so after one
  * such call an identical call can be captured in a HIP graph.  After
  * overwriting depz in place call dazi_dk_reset_geometry(). */
-/* Forgets the cached node geometry of the current device (no reference
- * counterpart: the reference recomputes it on every call). */
-int32_t dazi_dk_reset_geometry(void);
-
 int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
                         const float* vp, const float* rho, const float* depz, const double* tRc,
                         double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
                         int32_t* status, void* stream);
+
+/* Forgets the cached node geometry of the current device (no reference
+ * counterpart: the reference recomputes it on every call). */
+int32_t dazi_dk_reset_geometry(void);
 
 /* Replaces depthkernelTI (depthkernelTI.f90:2-34); host pointers, synchronous.
  * vp and rho come from Brocher(vs) as in the reference (desc->vp_rho_mode 0).
@@ -242,6 +242,36 @@ int32_t dazi_dk_grows(const dazi_dk_grow_desc* desc, const int32_t* ray_k,
                       const double* sen_vs, const double* sen_vp, const double* sen_rho,
                       const float* lsen, int64_t cap, float* rw, int32_t* iw, int32_t* col,
                       int64_t* nar);
+
+/* Period azimuthal-anisotropy map: the forward 2-psi anisotropy of every
+ * interior cell and period from Lsen_Gsc and the Gc, Gs layer models
+ * (replaces the computation of FwdAzimuthalAniMap,
+ * code/src_inv_iso_joint/FwdAzimuthalAniMap.f90:44-89; its unit-42 text
+ * output stays with the caller):
+ *   cos_rc = sum_k Lsen_Gsc(post, t, k) * gcf(ii, jj, k),  sin_rc likewise with gsf,
+ *   amp = 0.5*sqrt(cos^2 + sin^2),  amp_rela = amp / real(tRcV(cell, t)),
+ *   angle = 0.5*(atan2(sin, cos)/pi*180, +360 when negative)  [degrees],
+ * post = jj*nx + ii (1-based interior ii, jj; 0-based column), in float32 as
+ * the reference.  Layouts: lsen (nx*ny, kmax, nz-1) f32, gcf and gsf
+ * (nx-2, ny-2, nz-1) f32, tRcV ((nx-2)*(ny-2), kmax) f64, outputs
+ * ((nx-2)*(ny-2), kmax) f32, all Fortran order.  angle agrees with the
+ * reference to 2 units in the last place (its libm atan2f is not correctly
+ * rounded); the other outputs are exact. */
+typedef struct dazi_dk_azim_desc {
+  int32_t nx, ny, nz, kmax;
+} dazi_dk_azim_desc;
+
+/* Device pointers; enqueues on stream.  amp, amp_rela, angle may be NULL
+ * (tRcV then too, when amp_rela is NULL). */
+int32_t dazi_dk_azimuth_map_dev(const dazi_dk_azim_desc* desc, const float* lsen,
+                                const float* gcf, const float* gsf, const double* tRcV,
+                                float* cos_rc, float* sin_rc, float* amp, float* amp_rela,
+                                float* angle, void* stream);
+
+/* Host pointers, all outputs required; synchronous. */
+int32_t dazi_dk_azimuth_map(const dazi_dk_azim_desc* desc, const float* lsen, const float* gcf,
+                            const float* gsf, const double* tRcV, float* cos_rc, float* sin_rc,
+                            float* amp, float* amp_rela, float* angle);
 
 /* Bytes of device memory the library's pool holds on the current device
  * (current slots; buffers retired by growth are not counted). */

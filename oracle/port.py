@@ -407,3 +407,81 @@ def g_rows(nvx, nvz, nz, ray_k, fdm, fdmc, fdms, vel, sen_vs, sen_vp, sen_rho, l
         out_col.append((keep + 1).astype(np.int32))
     cat = lambda x, t: np.concatenate(x).astype(t) if x else np.zeros(0, t)  # noqa: E731
     return cat(out_rw, np.float32), cat(out_iw, np.int32), cat(out_col, np.int32)
+
+
+# ----------------------------------------------------------------- azimuthal map
+# the reference's pi: a default-real literal stored in a real*8
+# (FwdAzimuthalAniMap.f90:35)
+AZ_PI = float(np.float32(3.1415926535898))
+
+
+_LIBM = None
+
+
+def _libm_atan2f(y, x):
+    """Elementwise atan2f of the C library (the function the flang build calls;
+    numpy's float32 arctan2 differs from it in the last bit for some inputs)."""
+    global _LIBM
+    if _LIBM is None:
+        import ctypes.util
+        _LIBM = C.CDLL(ctypes.util.find_library("m"))
+        _LIBM.atan2f.restype = C.c_float
+        _LIBM.atan2f.argtypes = [C.c_float, C.c_float]
+    y = np.asarray(y, np.float32)
+    x = np.asarray(x, np.float32)
+    out = np.empty(y.shape, np.float32)
+    fy, fx, fo = y.ravel(), x.ravel(), out.reshape(-1)
+    for q in range(fy.size):
+        fo[q] = _LIBM.atan2f(float(fy[q]), float(fx[q]))
+    return out
+
+
+def azimuth_map(nvx, nvz, lsen, gcf, gsf, tRcV):
+    """FwdAzimuthalAniMap (FwdAzimuthalAniMap.f90:44-89) restated in float32:
+    per interior cell and period, cosRc = sum_k Lsen*gcf and sinRc = sum_k Lsen*gsf
+    (in k order), AzimAmp = 0.5 sqrt(cos^2 + sin^2), AzimAmpRela = AzimAmp /
+    real(tRcV), AzimAng = 0.5 * (atan2(sin, cos)/pi*180, +360 when negative).
+
+    lsen (nz-1, K, ncol) f32 with ncol = (nvx+2)*(nvz+2); gcf, gsf (nz-1, nvz, nvx)
+    f32 (the reference's gcf(nx-2, ny-2, nz-1)); tRcV (K, nvx*nvz) f64.  Returns
+    dict of (K, nvx*nvz) float32 arrays: cos, sin, amp, amp_rela, angle."""
+    f = np.float32
+    nd, K, ncol = lsen.shape
+    jj, ii = np.meshgrid(np.arange(1, nvz + 1), np.arange(1, nvx + 1), indexing="ij")
+    post = (jj * (nvx + 2) + ii).ravel()
+    c = np.zeros((K, nvx * nvz), f)
+    s = np.zeros((K, nvx * nvz), f)
+    for kk in range(nd):
+        l = lsen[kk][:, post]
+        c = c + l * gcf[kk].ravel()
+        s = s + l * gsf[kk].ravel()
+    amp = f(0.5) * np.sqrt(c * c + s * s)
+    rela = amp / np.asarray(tRcV, np.float64).astype(f)
+    ang = (_libm_atan2f(s, c).astype(np.float64) / AZ_PI * 180.0).astype(f)
+    ang = np.where(ang < f(0.0), ang + f(360.0), ang)
+    ang = f(0.5) * ang
+    return dict(cos=c, sin=s, amp=amp, amp_rela=rela, angle=ang)
+
+
+def ref_azimuth_map(nx, ny, nz, goxd, gozd, dvxd, dvzd, tRc, gcf, gsf, lsen_f, tRcV_f):
+    """Runs the reference's FwdAzimuthalAniMap (flang build) in the current
+    directory and returns the lines it writes to unit 42 as an (n, 9) array:
+    lat, lon, period, isoC, AzimAng, AzimAmpRela, AzimAmp, cos, sin.
+    gcf, gsf (nx-2, ny-2, nz-1), lsen_f (nx*ny, K, nz-1), tRcV_f ((nx-2)*(ny-2), K)
+    in Fortran order."""
+    if not ref_available():
+        raise RuntimeError("reference library not built")
+    i = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    r = lambda v: C.byref(C.c_float(v))  # noqa: E731
+    F32 = lambda a: np.asfortranarray(a, np.float32)  # noqa: E731
+    if os.path.exists("fort.42"):
+        os.remove("fort.42")
+    ref().fwdazimuthalanimap_(i(nx), i(ny), i(nz), i(0), r(goxd), r(gozd), r(dvxd), r(dvzd),
+                              i(len(tRc)), _ptr(_f64(tRc)), _ptr(F32(gcf)), _ptr(F32(gsf)),
+                              _ptr(F32(lsen_f)), _ptr(np.asfortranarray(tRcV_f, np.float64)))
+    rows = []
+    with open("fort.42") as fh:
+        for line in fh:
+            line = line.rstrip("\n")
+            rows.append([float(line[10 * q:10 * q + 10]) for q in range(9)])
+    return np.array(rows)

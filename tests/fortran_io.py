@@ -55,9 +55,10 @@ def read_triplets(path):
     return rw, iw, col
 
 
-def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None, rays=None):
+def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None, rays=None, extra=None):
     """Runs the driver; rays = (knumi, maps) adds the G-row step, whose
-    triplets are read back as a third return value."""
+    triplets are read back as a third return value; extra: further driver
+    arguments as given."""
     fin, fout = os.path.join(tmpdir, "in.bin"), os.path.join(tmpdir, "out.bin")
     write_input(fin, vel, depz, periods, minthk)
     args = [DRIVER, fin, fout]
@@ -66,6 +67,7 @@ def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None, rays=None):
         frays, fg = os.path.join(tmpdir, "rays.bin"), os.path.join(tmpdir, "g.bin")
         write_rays(frays, *rays)
         args += [frays, fg]
+    args += list(extra or [])
     e = dict(os.environ)
     e.update(env or {})
     r = subprocess.run(args, capture_output=True, text=True, timeout=timeout, env=e)
