@@ -228,6 +228,33 @@ def bench_grows(dk, configs, np, torch, vel, out, nvx, nvz, reps=5):
                               unit="GB/s", frac=alg / (ms * 1e-3) / 8e12, bytes=alg))
 
 
+def bench_azimuth(dk, np, torch, out, nx, ny, reps=20):
+    """Period azimuthal-anisotropy map on the plain_4096 stage outputs (62 x 62
+    interior cells, 24 layers, 30 periods) with random Gc, Gs models.
+    Algorithmic HBM bytes per call: per (cell, period) 24 x 4 B of Lsen_Gsc, 8 B
+    of tRcV and 20 B out, plus 2 x 24 x 4 B of Gc, Gs per cell."""
+    from dazi_current_release_amd import azimuth
+    nd, K, ncol = out["lsen"].shape
+    cells = (nx - 2) * (ny - 2)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    gcf = 0.03 * torch.randn((nd, ny - 2, nx - 2), device="cuda", generator=g)
+    gsf = 0.03 * torch.randn((nd, ny - 2, nx - 2), device="cuda", generator=g)
+    tRcV = out["pv"][:, :cells].contiguous()
+    m = azimuth.azimuth_map_dev(out["lsen"], gcf, gsf, tRcV, nx, ny)
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        azimuth.azimuth_map_dev(out["lsen"], gcf, gsf, tRcV, nx, ny, out=m)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    alg = cells * K * (nd * 4 + 8 + 20) + cells * nd * 8
+    return dict(workload="plain_4096 azimuthal map", cells=cells, periods=K, layers=nd, ms=ms,
+                roofline=dict(bound="hbm", achieved=alg / (ms * 1e-3) / 1e9, peak=8000.0,
+                              unit="GB/s", frac=alg / (ms * 1e-3) / 8e12, bytes=alg))
+
+
 def bench_host_path(dk, configs, np, reps=3):
     """The Fortran drop-in's host entry points on plain_4096: depthkernel then
     depthkernelTI through the C ABI with host arrays (inputs copied in, all
@@ -400,6 +427,7 @@ def main():
     grow = None
     if not args.no_grows:
         grow = bench_grows(dk, configs, np, torch, vel, out, W["nx"] - 2, W["ny"] - 2)
+        grow["azimuth_map"] = bench_azimuth(dk, np, torch, out, W["nx"], W["ny"])
 
     cpu = None
     if not args.no_cpu:
