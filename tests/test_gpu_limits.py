@@ -92,3 +92,29 @@ def test_failed_periods_iso_and_ti(dk, tmp_path, monkeypatch):
             ref = P.ti_at(f["vel"][j], f["periods"], tpv[j].astype(np.float32), f["depz"],
                           float(f["minthk"]))
             lsen_bands(tls[j][ok[j]], ref[ok[j]])
+
+
+def test_ti_on_stress_models_with_complex_pair_layers(dk, tmp_path, monkeypatch):
+    """Random inverted, low-velocity-zone and strongly layered node models (Vs
+    1.0-4.8 km/s, minthk 1-3): many of their layers give the TI eigenvalues a
+    complex-conjugate pair, so many solves go through the complex fallback.
+    Lsen against the reference's tregn96 at this build's own phase velocities,
+    within the TI A/B bands."""
+    if not P.ref_available():
+        pytest.skip("reference library not built")
+    monkeypatch.chdir(tmp_path)  # the reference writes unit-66 messages to cwd
+    rng = np.random.default_rng(42)
+    depz = np.arange(0, 61, 5, dtype=np.float32)
+    per = np.geomspace(1, 60, 20)
+    for minthk in (1.0, 2.0, 3.0):
+        vel = rng.uniform(1.0, 4.8, (24, depz.size)).astype(np.float32)
+        vel[:8] = np.sort(vel[:8], axis=1)[:, ::-1]       # inverted
+        vel[8:16] = np.sort(vel[8:16], axis=1)            # increasing with steps
+        tpv, tls = dk.run_depthkernelTI(vel[:, None, :], per, depz, minthk)
+        for j in range(vel.shape[0]):
+            ok = tpv[j] > 0
+            if not ok.any():
+                continue
+            ref = P.ti_at(vel[j], per, tpv[j].astype(np.float32), depz, minthk)
+            lsen_bands(tls[j][ok], ref[ok])
+            assert np.all(tls[j][~ok] == 0.0)
