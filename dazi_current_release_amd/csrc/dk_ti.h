@@ -755,6 +755,11 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   double sumi0 = 0, sumi1 = 0, sumi2 = 0, sumi3 = 0;
   for (int m = 0; m < L; m++) {
     const Layer ly = lay(m);
+    // issue this iteration's reads of the up sweep (interface m+1) first, so
+    // that their latency overlaps the propagator work
+    const int n = m + 1 < L ? m + 1 : L - 1;
+    const double cd1 = cd_re(n, 0), cd2 = cd_re(n, 1), cd3 = cd_re(n, 2), cd5 = cd_re(n, 3),
+                 cd6 = cd_re(n, 4), exn_up = S(n, Q::EX);
     eigen(ly, omega, wvno, &g);
     double yb[4] = {0.0, 0.0, 0.0, 0.0};  // interface m+1 (zero below the half-space top)
     if (m < L - 1) {
@@ -776,15 +781,13 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       exd = exd + cpex + ex2;
       for (int i = 0; i < 4; i++) vv[i] = a0[i];
       // contraction at interface m+1 (svfunc)
-      const int n = m + 1;
-      double cd1 = cd_re(n, 0), cd2 = cd_re(n, 1), cd3 = cd_re(n, 2), cd4 = -cd_re(n, 2),
-             cd5 = cd_re(n, 3), cd6 = cd_re(n, 4);
+      const double cd4 = -cd3;
       double tz1 = -vv[3], tz2 = -vv[2], tz3 = vv[1], tz4 = vv[0];
       double uu1 = tz2 * cd6 - tz3 * cd5 + tz4 * cd4;
       double uu2 = -tz1 * cd6 + tz3 * cd3 - tz4 * cd2;
       double uu3 = tz1 * cd5 - tz2 * cd3 + tz4 * cd1;
       double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
-      double ext = exd + S(n, Q::EX) - exe1;
+      double ext = exd + exn_up - exe1;
       if (ext > -80.0 && ext < 80.0) {
         double fact = exp(ext);
         yb[0] = uu1 * fact / f1213;
