@@ -23,7 +23,8 @@ EXPORTS = [
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
     "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
     "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows", "dazi_dk_azimuth_map_dev",
-    "dazi_dk_azimuth_map",
+    "dazi_dk_azimuth_map", "dazi_dk_travel_plan_create", "dazi_dk_travel_run_dev",
+    "dazi_dk_travel_plan_destroy", "dazi_dk_travel",
 ]
 
 
@@ -62,6 +63,13 @@ class AzimDesc(C.Structure):
 class GrowDesc(C.Structure):
     _fields_ = [("nvx", C.c_int32), ("nvz", C.c_int32), ("nz", C.c_int32), ("kmax", C.c_int32),
                 ("nray", C.c_int64), ("row_base", C.c_int32), ("ftol", C.c_float)]
+
+
+class TravelDesc(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nvx", "nvz", "gdx", "gdz", "sgdl", "sgs", "asgr",
+                                         "fom")] + \
+               [(n, C.c_float) for n in ("goxd", "gozd", "dvxd", "dvzd", "snb", "earth")] + \
+               [("kmax", C.c_int32), ("slots", C.c_int32)]
 
 
 _lib = None
@@ -106,6 +114,11 @@ def lib():
                                     [C.c_void_p] * 4)
         L.dazi_dk_azimuth_map_dev.argtypes = [C.POINTER(AzimDesc)] + [C.c_void_p] * 10
         L.dazi_dk_azimuth_map.argtypes = [C.POINTER(AzimDesc)] + [C.c_void_p] * 9
+        L.dazi_dk_travel_plan_create.argtypes = ([C.POINTER(TravelDesc), C.c_int64] +
+                                                 [C.c_void_p] * 6 + [C.POINTER(C.c_void_p)])
+        L.dazi_dk_travel_run_dev.argtypes = [C.c_void_p] * 6
+        L.dazi_dk_travel_plan_destroy.argtypes = [C.c_void_p]
+        L.dazi_dk_travel.argtypes = [C.POINTER(TravelDesc), C.c_int64] + [C.c_void_p] * 10
         for name in EXPORTS:
             if not hasattr(L, name):
                 raise RuntimeError(f"{LIB_PATH} does not export {name}")

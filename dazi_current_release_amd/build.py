@@ -11,7 +11,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdazi_dk.so")
-SOURCES = ["dk_kernels.hip", "dk_ti_kernels.hip", "dk_grows.hip", "dk_azimuth.hip", "dk_api.hip"]
+SOURCES = ["dk_kernels.hip", "dk_ti_kernels.hip", "dk_grows.hip", "dk_azimuth.hip", "dk_travel.hip",
+           "dk_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DAZI_DK_ARCH", "gfx950")
 

@@ -297,3 +297,52 @@ def ray_dense(R, r):
         m[c // nvx + 1, c % nvx + 1] = R[key][s]
         out.append(m)
     return out
+
+
+def travel_geometry(nx: int, ny: int, K: int, nsta: int, goxd: float = 29.0, gozd: float = 98.0,
+                    dvxd: float = 0.25, dvzd: float = 0.25, seed: int = 11,
+                    periods_per_source: int | None = None):
+    """Station layout for the traveltime stage on an nx x ny inversion grid
+    (border included; the propagation grid spans its interior nodes from
+    latitude goxd southward and longitude gozd eastward).  nsta stations at
+    uniform random interior positions; every period uses every station as a
+    source (or periods_per_source of them, rotating) with all other stations as
+    its receivers, period-major as the reference's knumi/srcnum loops.
+    Coordinates are colatitude and longitude in radians, float32, as the G
+    builder's scxf/sczf.  Returns dict(job_k, scx, scz, rec_off, rcx, rcz)."""
+    rng = np.random.default_rng(seed)
+    pi = np.float32(3.1415926535898)
+    lat = rng.uniform(goxd - dvxd * (nx - 3) * 0.98, goxd - dvxd * (nx - 3) * 0.02, nsta)
+    lon = rng.uniform(gozd + dvzd * (ny - 3) * 0.02, gozd + dvzd * (ny - 3) * 0.98, nsta)
+    sx = ((np.float32(90.0) - lat.astype(F)) * pi / np.float32(180.0)).astype(F)
+    sz = (lon.astype(F) * pi / np.float32(180.0)).astype(F)
+    job_k, scx, scz, off, rcx, rcz = [], [], [], [0], [], []
+    for k in range(K):
+        srcs = range(nsta) if periods_per_source is None else \
+            [(k * periods_per_source + i) % nsta for i in range(periods_per_source)]
+        for s in srcs:
+            job_k.append(k)
+            scx.append(sx[s])
+            scz.append(sz[s])
+            others = np.array([r for r in range(nsta) if r != s], np.int64)
+            rcx.append(sx[others])
+            rcz.append(sz[others])
+            off.append(off[-1] + others.size)
+    return dict(job_k=np.array(job_k, np.int32), scx=np.array(scx, F), scz=np.array(scz, F),
+                rec_off=np.array(off, np.int64),
+                rcx=np.concatenate(rcx).astype(F) if rcx else np.zeros(0, F),
+                rcz=np.concatenate(rcz).astype(F) if rcz else np.zeros(0, F))
+
+
+def smooth_pv(nx: int, ny: int, K: int, seed: int = 5, lo: float = 2.8, hi: float = 4.2):
+    """Smooth random phase-velocity maps: (K, nx*ny) float64, the column index
+    ii + jj*nx fastest (pvRc in Fortran order, transposed)."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((K, nx * ny))
+    for k in range(K):
+        base = rng.standard_normal((ny, nx))
+        sx, sy = min(3.0, (nx - 1) / 5.0), min(3.0, (ny - 1) / 5.0)
+        m = _smooth(_smooth(base, sx).T, sy).T
+        m = (m - m.min()) / max(m.max() - m.min(), 1e-12)
+        out[k] = (lo + (hi - lo) * m).reshape(-1) + 0.02 * k
+    return out

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <memory>
 #include <vector>
 
 #include "../../include/dazi_dk.h"
@@ -664,6 +665,252 @@ int32_t dazi_dk_azimuth_map(const dazi_dk_azim_desc* d, const float* lsen, const
   HIPCHK(hipStreamSynchronize(nullptr));
   float* outs[5] = {cos_rc, sin_rc, amp, amp_rela, angle};
   for (int q = 0; q < 5; q++) HIPCHK(hipMemcpy(outs[q], o + q * m, m * 4, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"
+
+// ---- first-arrival traveltimes ----
+
+struct dazi_dk_travel_plan {
+  int dev = -1;
+  dk::TravelDev p = {};
+  std::vector<void*> bufs;
+  ~dazi_dk_travel_plan() {
+    for (void* b : bufs) (void)hipFree(b);
+  }
+};
+
+namespace {
+
+template <class T>
+int dev_alloc(dazi_dk_travel_plan* P, size_t n, T** out) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n * sizeof(T), 8)));
+  P->bufs.push_back(q);
+  *out = (T*)q;
+  return 0;
+}
+
+template <class T>
+int dev_upload(dazi_dk_travel_plan* P, const T* h, size_t n, const T** out) {
+  T* d;
+  if (int rc = dev_alloc(P, n, &d)) return rc;
+  if (n) HIPCHK(hipMemcpy(d, h, n * sizeof(T), hipMemcpyHostToDevice));
+  *out = d;
+  return 0;
+}
+
+// Grid constants of CalSurfGAniso_Joint.f90:396-409 and the refined spacing
+// of :570-571, in the reference's single-precision operation order.
+void travel_grid(const dazi_dk_travel_desc* d, dk::TravelDev& p) {
+#pragma clang fp contract(off)
+  const float pi = 3.1415926535898f;
+  const float dvx = d->dvxd * pi / 180.0f, dvz = d->dvzd * pi / 180.0f;
+  p.gox = (90.0f - d->goxd) * pi / 180.0f;
+  p.goz = d->gozd * pi / 180.0f;
+  p.dnx = dvx / (float)d->gdx;
+  p.dnz = dvz / (float)d->gdz;
+  p.drnx = dvx / (float)(d->gdx * d->sgdl);
+  p.drnz = dvz / (float)(d->gdz * d->sgdl);
+  p.nnx = (d->nvx - 1) * d->gdx + 1;
+  p.nnz = (d->nvz - 1) * d->gdz + 1;
+  p.maxbt_c = (int)std::lround(d->snb * (float)p.nnx * (float)p.nnz);
+}
+
+// Cell index of a coordinate, INT((x - o)/d) + 1 as the reference computes it.
+int cell_index(float x, float o, float dn) {
+#pragma clang fp contract(off)
+  return (int)((x - o) / dn) + 1;
+}
+
+// The sin tables the marching reads (libm sinf, as the reference):
+// risti = earth*sin(colatitude) per column of the coarse grid and of each
+// refined grid (by its first coarse column vnl), and srtimes' dpl.
+void travel_tables(const dazi_dk_travel_desc* d, dk::TravelDev& p, std::vector<float>& rc,
+                   std::vector<float>& rr) {
+#pragma clang fp contract(off)
+  rc.resize(p.nnx);
+  for (int ix = 1; ix <= p.nnx; ix++) rc[ix - 1] = d->earth * sinf(p.gox + (float)(ix - 1) * p.dnx);
+  rr.assign((size_t)p.nnx * p.nrx_max, 0.0f);
+  for (int vnl = 1; vnl <= p.nnx; vnl++) {
+    const float gorx = p.gox + p.dnx * (float)(vnl - 1);
+    for (int ix = 1; ix <= p.nrx_max; ix++)
+      rr[(size_t)(vnl - 1) * p.nrx_max + ix - 1] = d->earth * sinf(gorx + (float)(ix - 1) * p.drnx);
+  }
+  float dpl = p.dnx * d->earth;
+  float rd1 = p.dnz * d->earth * sinf(p.gox);
+  if (rd1 < dpl) dpl = rd1;
+  rd1 = p.dnz * d->earth * sinf(p.gox + (float)(p.nnx - 1) * p.dnx);
+  if (rd1 < dpl) dpl = rd1;
+  p.dpl = dpl;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dazi_dk_travel_plan_create(const dazi_dk_travel_desc* d, int64_t njob,
+                                   const int32_t* job_k, const float* scx, const float* scz,
+                                   const int64_t* rec_off, const float* rcx, const float* rcz,
+                                   dazi_dk_travel_plan** out) {
+  if (!d || !out) return set_err(DAZI_DK_EINVAL, "NULL descriptor or plan");
+  *out = nullptr;
+  if (d->nvx < 2 || d->nvz < 2 || d->gdx < 1 || d->gdz < 1 || d->sgdl < 1 || d->sgs < 1 ||
+      (d->asgr != 0 && d->asgr != 1) || (d->fom != 0 && d->fom != 1) || !(d->snb > 0.0f) ||
+      !(d->earth > 0.0f) || d->kmax < 1 || d->slots < 0 || !(d->dvxd > 0.0f) ||
+      !(d->dvzd > 0.0f))
+    return set_err(DAZI_DK_EINVAL,
+                   "bad travel descriptor (nvx=%d nvz=%d gdx=%d gdz=%d sgdl=%d sgs=%d asgr=%d "
+                   "fom=%d snb=%g earth=%g kmax=%d slots=%d)",
+                   d->nvx, d->nvz, d->gdx, d->gdz, d->sgdl, d->sgs, d->asgr, d->fom, d->snb,
+                   d->earth, d->kmax, d->slots);
+  if (njob < 0 || (njob > 0 && (!job_k || !scx || !scz || !rec_off)))
+    return set_err(DAZI_DK_EINVAL, "bad job arrays (njob=%lld)", (long long)njob);
+  auto P = std::make_unique<dazi_dk_travel_plan>();
+  dk::TravelDev& p = P->p;
+  p.nvx = d->nvx;
+  p.nvz = d->nvz;
+  p.gdx = d->gdx;
+  p.gdz = d->gdz;
+  p.sgdl = d->sgdl;
+  p.sgs = d->sgs;
+  p.asgr = d->asgr;
+  p.fom = d->fom;
+  p.K = d->kmax;
+  p.ncol = (long)(d->nvx + 2) * (d->nvz + 2);
+  p.earth = d->earth;
+  p.snb = d->snb;
+  travel_grid(d, p);
+  if (p.nnx > 32767 || p.nnz > 32767)
+    return set_err(DAZI_DK_EINVAL, "propagation grid %d x %d exceeds 32767 nodes per side", p.nnz,
+                   p.nnx);
+  p.nrx_max = std::min(2 * d->sgs, p.nnx - 1) * d->sgdl + 1;
+  const int nrz_max = std::min(2 * d->sgs, p.nnz - 1) * d->sgdl + 1;
+  if (d->asgr && (p.nrx_max > 32767 || nrz_max > 32767))
+    return set_err(DAZI_DK_EINVAL, "refined source grid exceeds 32767 nodes per side");
+  p.nr_max = d->asgr ? p.nrx_max * nrz_max : 1;
+  const int64_t nrec = njob > 0 ? rec_off[njob] : 0;
+  if (njob > 0 && rec_off[0] != 0) return set_err(DAZI_DK_EINVAL, "rec_off[0] must be 0");
+  for (int64_t q = 0; q < njob; q++) {
+    if (rec_off[q + 1] < rec_off[q])
+      return set_err(DAZI_DK_EINVAL, "rec_off decreases at job %lld", (long long)q);
+    if (job_k[q] < 0 || job_k[q] >= d->kmax)
+      return set_err(DAZI_DK_EINVAL, "job %lld: period index %d outside 0..%d", (long long)q,
+                     job_k[q], d->kmax - 1);
+    const int isx = std::isfinite(scx[q]) ? cell_index(scx[q], p.gox, p.dnx) : 0;
+    const int isz = std::isfinite(scz[q]) ? cell_index(scz[q], p.goz, p.dnz) : 0;
+    if (!std::isfinite(scx[q]) || !std::isfinite(scz[q]) || isx < 1 || isx > p.nnx || isz < 1 ||
+        isz > p.nnz)
+      return set_err(DAZI_DK_EINVAL, "job %lld: source (%g, %g) lies outside the model",
+                     (long long)q, scx[q], scz[q]);
+  }
+  if (nrec > 0 && (!rcx || !rcz)) return set_err(DAZI_DK_EINVAL, "NULL receiver arrays");
+  std::vector<float> srx((size_t)nrec);
+  for (int64_t r = 0; r < nrec; r++) {
+    const bool fin = std::isfinite(rcx[r]) && std::isfinite(rcz[r]);
+    const int irx = fin ? cell_index(rcx[r], p.gox, p.dnx) : 0;
+    const int irz = fin ? cell_index(rcz[r], p.goz, p.dnz) : 0;
+    if (!fin || irx < 1 || irx > p.nnx || irz < 1 || irz > p.nnz)
+      return set_err(DAZI_DK_EINVAL, "receiver %lld (%g, %g) lies outside the model",
+                     (long long)r, rcx[r], rcz[r]);
+    srx[r] = sinf(rcx[r]);
+  }
+  // tree capacity: the coarse grid's, or the larger refined grid's
+  const int mx = std::max(p.nrx_max, p.nnx), mz = std::max(nrz_max, p.nnz);
+  const long maxbt_max =
+      std::max<long>(p.maxbt_c, d->asgr ? std::lround(d->snb * (float)mx * (float)mz) : 0);
+  p.spill_cap = std::max<long>(maxbt_max - 1024, 0);
+  HIPCHK(hipGetDevice(&P->dev));
+  Pool* pool;
+  if (int rc = get_pool(&pool)) return rc;
+  const long nn = (long)p.nnx * p.nnz;
+  const double per_slot = 8.0 * nn + 12.0 * p.nr_max + 8.0 * p.spill_cap;
+  long slots = d->slots > 0 ? d->slots : (long)pool->ncu * 16;
+  slots = std::min<long>(slots, std::max<long>((long)(8.0e9 / per_slot), 1));
+  slots = std::max<long>(std::min<long>(slots, njob), 1);
+  p.slots = (int)slots;
+  p.njob = njob;
+  std::vector<float> rc_tab, rr_tab;
+  p.nrx_max = std::max(p.nrx_max, 1);
+  travel_tables(d, p, rc_tab, rr_tab);
+  dazi_dk_travel_plan* Q = P.get();
+  std::vector<int64_t> zero_off(1, 0);
+  if (int rc = dev_upload(Q, job_k, (size_t)njob, &p.job_k)) return rc;
+  if (int rc = dev_upload(Q, scx, (size_t)njob, &p.scx)) return rc;
+  if (int rc = dev_upload(Q, scz, (size_t)njob, &p.scz)) return rc;
+  if (int rc = dev_upload(Q, njob ? rec_off : zero_off.data(), (size_t)njob + 1, &p.rec_off))
+    return rc;
+  if (int rc = dev_upload(Q, rcx, (size_t)nrec, &p.rcx)) return rc;
+  if (int rc = dev_upload(Q, rcz, (size_t)nrec, &p.rcz)) return rc;
+  if (int rc = dev_upload(Q, srx.data(), srx.size(), &p.sin_rcx)) return rc;
+  if (int rc = dev_upload(Q, rc_tab.data(), rc_tab.size(), &p.risti_c)) return rc;
+  if (int rc = dev_upload(Q, rr_tab.data(), rr_tab.size(), &p.risti_r)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)nn * p.K, &p.veln)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)nn * slots, &p.node_c)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)p.nr_max * slots, &p.node_r)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)p.nr_max * slots, &p.vel_r)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)std::max<long>(p.spill_cap, 1) * slots, &p.spill)) return rc;
+  *out = P.release();
+  return 0;
+}
+
+int32_t dazi_dk_travel_run_dev(dazi_dk_travel_plan* P, const double* pvRc, float* times,
+                               float* ttn, int32_t* status, void* stream) {
+  if (!P) return set_err(DAZI_DK_EINVAL, "NULL plan");
+  if (!pvRc || (P->p.njob > 0 && !status)) return set_err(DAZI_DK_EINVAL, "NULL pvRc or status");
+  if (!times && P->p.njob > 0) return set_err(DAZI_DK_EINVAL, "NULL times");
+  int dev = -1;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != P->dev)
+    return set_err(DAZI_DK_EINVAL, "plan belongs to device %d, current device is %d", P->dev, dev);
+  dk::TravelDev p = P->p;
+  p.pv = pvRc;
+  p.times = times;
+  p.ttn = ttn;
+  p.status = status;
+  HIPCHK(dk::launch_travel(p, (hipStream_t)stream));
+  return 0;
+}
+
+int32_t dazi_dk_travel_plan_destroy(dazi_dk_travel_plan* P) {
+  if (!P) return 0;
+  int dev = -1;
+  const int own = P->dev;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev != own) HIPCHK(hipSetDevice(own));
+  HIPCHK(hipDeviceSynchronize());
+  delete P;
+  if (dev != own) HIPCHK(hipSetDevice(dev));
+  return 0;
+}
+
+int32_t dazi_dk_travel(const dazi_dk_travel_desc* d, int64_t njob, const int32_t* job_k,
+                       const float* scx, const float* scz, const int64_t* rec_off,
+                       const float* rcx, const float* rcz, const double* pvRc, float* times,
+                       float* ttn, int32_t* status) {
+  if (!pvRc || (njob > 0 && (!times || !status)))
+    return set_err(DAZI_DK_EINVAL, "NULL pvRc, times or status");
+  dazi_dk_travel_plan* P = nullptr;
+  if (int rc = dazi_dk_travel_plan_create(d, njob, job_k, scx, scz, rec_off, rcx, rcz, &P))
+    return rc;
+  std::unique_ptr<dazi_dk_travel_plan> own(P);
+  const dk::TravelDev& p = P->p;
+  const size_t nn = (size_t)p.nnx * p.nnz, nrec = njob > 0 ? (size_t)rec_off[njob] : 0;
+  double* dpv;
+  float *dt, *dttn = nullptr;
+  int32_t* dst;
+  if (int rc = dev_alloc(P, (size_t)p.ncol * p.K, &dpv)) return rc;
+  if (int rc = dev_alloc(P, nrec, &dt)) return rc;
+  if (ttn)
+    if (int rc = dev_alloc(P, nn * (size_t)njob, &dttn)) return rc;
+  if (int rc = dev_alloc(P, (size_t)njob, &dst)) return rc;
+  HIPCHK(hipMemcpy(dpv, pvRc, (size_t)p.ncol * p.K * 8, hipMemcpyHostToDevice));
+  if (int rc = dazi_dk_travel_run_dev(P, dpv, dt, dttn, dst, nullptr)) return rc;
+  HIPCHK(hipStreamSynchronize(nullptr));
+  if (nrec) HIPCHK(hipMemcpy(times, dt, nrec * 4, hipMemcpyDeviceToHost));
+  if (ttn && njob) HIPCHK(hipMemcpy(ttn, dttn, nn * njob * 4, hipMemcpyDeviceToHost));
+  if (njob) HIPCHK(hipMemcpy(status, dst, (size_t)njob * 4, hipMemcpyDeviceToHost));
   return 0;
 }
 

@@ -124,6 +124,44 @@ hipError_t launch_azimuth_map(int nvx, int nvz, int nz, int K, const float* lsen
                               const float* gsf, const double* tRcV, float* cos_rc, float* sin_rc,
                               float* amp, float* amp_rela, float* angle, hipStream_t st);
 
+// First-arrival traveltimes of (period, source) jobs on the propagation grid
+// (CalSurfGAniso_Joint.f90:511-725); see dazi_dk_travel_run_dev.
+struct TravelDev {
+  // grid (CalSurfGAniso_Joint.f90:375-409)
+  int nvx, nvz, gdx, gdz, sgdl, sgs, asgr, fom;
+  int nnx, nnz, K;
+  long ncol;
+  float gox, goz, dnx, dnz, drnx, drnz, earth, snb;
+  float dpl;           // srtimes' near-source distance threshold
+  int maxbt_c;         // tree capacity nint(snb*nnx*nnz)
+  int nrx_max, nr_max;  // refined-grid bounds
+  const double* pv;    // (ncol, K) phase velocities, column fastest
+  float* veln;         // work: (K, nnz*nnx) diced velocities
+  const float* risti_c;  // nnx: earth*sinf(gox + (ix-1)*dnx)
+  const float* risti_r;  // (nnx, nrx_max): the same on the refined grid starting at column vnl
+  // jobs and receivers
+  long njob;
+  const int* job_k;
+  const float* scx;
+  const float* scz;
+  const int64_t* rec_off;
+  const float* rcx;
+  const float* rcz;
+  const float* sin_rcx;
+  // outputs
+  float* times;
+  float* ttn;  // optional (njob, nnz*nnx)
+  int* status;
+  // per-slot work
+  int2* node_c;
+  int2* node_r;
+  float* vel_r;
+  int2* spill;
+  long spill_cap;
+  int slots;
+};
+hipError_t launch_travel(const TravelDev& p, hipStream_t st);
+
 hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const double* svs,
                             const double* svp, const double* srho, const double* gc, double* eff,
                             double* grad, hipStream_t st);

@@ -1,0 +1,755 @@
+// First-arrival traveltimes on the propagation grid: the consumer of the
+// phase-velocity maps pvRc in the joint G builder
+// (code/src_inv_iso_joint/CalSurfGAniso_Joint.f90:511-725).  A job is one
+// (period, source) pair:
+//   * the period's (nvz+2) x (nvx+2) control values are diced onto the
+//     nnz x nnx propagation grid by cubic B-splines (gridder,
+//     CalSurfG.f90:1596-1689): k_gridder, one thread per (period, node);
+//   * a refined grid around the source (bsplrefine, CalSurfG.f90:1698-1770)
+//     carries the first fast-marching pass until the narrow band touches an
+//     inner edge of it (travel with urg = 1, CalSurfG.f90:399-598); its nodes
+//     that coincide with coarse nodes seed the band of the coarse pass
+//     (CalSurfGAniso_Joint.f90:606-699), which runs to completion (urg = 2);
+//   * receiver times come from the coarse field (srtimes,
+//     CalSurfG.f90:1772-1906).
+//
+// Fast marching is a sequential chain (pop the earliest band node, update its
+// four neighbours), and its result depends on the order of the pops, so each
+// job runs it in the reference's order on one lane.  The parallelism is across
+// jobs: one 64-lane workgroup per job slot, the lanes sharing the data-parallel
+// steps (refined velocities, status clearing, band seeding, receivers) and
+// lane 0 running the marching.  The narrow band (a binary heap with the
+// reference's sift rules) lives in LDS for its first HEAP_LDS entries and
+// spills to a per-slot global array beyond that.  Each node record holds
+// (traveltime bits, status) in 8 bytes, so one load fetches both; per pop,
+// the records the four upwind updates read are loaded together before any of
+// the heap stores.
+//
+// Arithmetic is the reference's single precision in its operation order with
+// contraction off; division and sqrt are evaluated in double and rounded once
+// (the correctly rounded float result).  sin enters only through tables the
+// host fills with libm sinf (risti per grid column, sin of each receiver's
+// colatitude), as the reference evaluates it.
+#include <hip/hip_runtime.h>
+
+#include "dk_internal.h"
+
+namespace dk {
+namespace {
+
+constexpr int HEAP_LDS = 1024;
+constexpr int FAR = -1;
+
+__device__ __forceinline__ float fdiv(float a, float b) { return (float)((double)a / (double)b); }
+__device__ __forceinline__ float fsqrt(float a) { return (float)sqrt((double)a); }
+__device__ __forceinline__ float p2(float x) {
+#pragma clang fp contract(off)
+  return x * x;
+}
+__device__ __forceinline__ float p3(float x) {
+#pragma clang fp contract(off)
+  return x * x * x;
+}
+
+// Cubic B-spline basis at parameter (i-1)/n (CalSurfG.f90:1644-1651).
+__device__ __forceinline__ void basis(int i, int n, float w[4]) {
+#pragma clang fp contract(off)
+  float u = (float)n;
+  u = fdiv((float)(i - 1), u);
+  w[0] = fdiv(p3(1.0f - u), 6.0f);
+  w[1] = fdiv(4.0f - 6.0f * p2(u) + 3.0f * p3(u), 6.0f);
+  w[2] = fdiv(1.0f + 3.0f * u + 3.0f * p2(u) - 3.0f * p3(u), 6.0f);
+  w[3] = fdiv(p3(u), 6.0f);
+}
+
+__device__ __forceinline__ float velv(const TravelDev& p, const double* pvk, int i, int j) {
+  return (float)pvk[(long)i * (p.nvx + 2) + j];
+}
+
+// Cell and in-cell index of a diced node: node s (1-based) of cells of n
+// nodes, the last of the ncell cells holding n + 1.
+__device__ __forceinline__ void cell_of(int s, int n, int ncell, int* cell, int* in) {
+  int c = (s - 1) / n + 1;
+  if (c > ncell) c = ncell;
+  *cell = c;
+  *in = s - n * (c - 1);
+}
+
+__global__ void k_gridder(TravelDev p) {
+#pragma clang fp contract(off)
+  const long nn = (long)p.nnx * p.nnz;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nn * p.K) return;
+  const int k = (int)(t / nn);
+  const int node = (int)(t - (long)k * nn);
+  const int stz = node % p.nnz + 1, stx = node / p.nnz + 1;
+  int i, l, j, m;
+  cell_of(stz, p.gdz, p.nvz - 1, &i, &l);
+  cell_of(stx, p.gdx, p.nvx - 1, &j, &m);
+  float ui[4], vi[4];
+  basis(m, p.gdx, ui);
+  basis(l, p.gdz, vi);
+  const double* pvk = p.pv + (long)k * p.ncol;
+  float sumi = 0.0f;
+#pragma unroll
+  for (int i1 = 1; i1 <= 4; i1++) {
+    float sumj = 0.0f;
+#pragma unroll
+    for (int j1 = 1; j1 <= 4; j1++) sumj = sumj + ui[j1 - 1] * velv(p, pvk, i - 2 + i1, j - 2 + j1);
+    sumi = sumi + vi[i1 - 1] * sumj;
+  }
+  p.veln[t] = sumi;
+}
+
+// ---- one grid (coarse or refined) ----
+struct Grid {
+  int nx, nz;
+  float gox, goz, dnx, dnz, earth;
+  const float* vel;    // (nz, nx) column-major
+  int2* rec;           // node records: (traveltime bits, status)
+  const float* risti;  // earth * sin(colatitude) per column
+  int urg, vnl, vnr, vnt, vnb, maxbt, fom;
+  __device__ __forceinline__ int lin(int iz, int ix) const { return (iz - 1) + (ix - 1) * nz; }
+};
+
+__device__ __forceinline__ int pack(int iz, int ix) { return (ix << 16) | iz; }
+
+struct Heap {
+  float* k;  // LDS keys, entry p at p (1..HEAP_LDS)
+  int* nd;   // LDS packed nodes
+  int2* sp;  // spill: entry p > HEAP_LDS at sp[p - HEAP_LDS - 1]
+  int ntr;
+  __device__ __forceinline__ float key(int p) const {
+    return p <= HEAP_LDS ? k[p] : __int_as_float(sp[p - HEAP_LDS - 1].x);
+  }
+  __device__ __forceinline__ int node(int p) const {
+    return p <= HEAP_LDS ? nd[p] : sp[p - HEAP_LDS - 1].y;
+  }
+  __device__ __forceinline__ void put(int p, float kk, int n) {
+    if (p <= HEAP_LDS) {
+      k[p] = kk;
+      nd[p] = n;
+    } else {
+      sp[p - HEAP_LDS - 1] = make_int2(__float_as_int(kk), n);
+    }
+  }
+};
+
+__device__ __forceinline__ void set_status(const Grid& g, int packed, int s) {
+  g.rec[g.lin(packed & 0xffff, packed >> 16)].y = s;
+}
+
+// Up to four pending neighbour nodes of the current pop whose tree positions
+// must follow the moves of earlier heap operations in the same pop.
+struct Pending {
+  int node[4];
+  int pos[4];
+  __device__ __forceinline__ void moved(int n, int to) {
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (node[q] == n) pos[q] = to;
+  }
+};
+
+// addtree / updtree (CalSurfG.f90:879-925, 1005-1032): the entry of node n
+// with key tv, starting at position tpc, filters up past larger parents.
+__device__ __forceinline__ void sift_up(Heap& h, const Grid& g, int n, float tv, int tpc,
+                                        Pending& pd) {
+  int tpp = tpc / 2;
+  while (tpp > 0) {
+    const float kp = h.key(tpp);
+    if (tv < kp) {
+      const int np = h.node(tpp);
+      h.put(tpc, kp, np);
+      set_status(g, np, tpc);
+      pd.moved(np, tpc);
+      tpc = tpp;
+      tpp = tpc / 2;
+    } else {
+      break;
+    }
+  }
+  h.put(tpc, tv, n);
+  set_status(g, n, tpc);
+  pd.moved(n, tpc);
+}
+
+// downtree (CalSurfG.f90:927-1003): the last entry replaces the root and
+// filters down past smaller children.
+__device__ __forceinline__ void downtree(Heap& h, const Grid& g) {
+  if (h.ntr == 1) {
+    h.ntr = 0;
+    return;
+  }
+  const float kl = h.key(h.ntr);
+  const int nl = h.node(h.ntr);
+  h.ntr--;
+  const int ntr = h.ntr;
+  int tpp = 1, tpc = 2;
+  while (tpc < ntr) {
+    float kc = h.key(tpc);
+    const float k2 = h.key(tpc + 1);
+    if (kc > k2) {
+      tpc++;
+      kc = k2;
+    }
+    if (kc < kl) {
+      const int nc = h.node(tpc);
+      h.put(tpp, kc, nc);
+      set_status(g, nc, tpp);
+      tpp = tpc;
+      tpc = 2 * tpp;
+    } else {
+      tpc = ntr + 1;
+    }
+  }
+  if (tpc == ntr) {
+    const float kc = h.key(tpc);
+    if (kc < kl) {
+      const int nc = h.node(tpc);
+      h.put(tpp, kc, nc);
+      set_status(g, nc, tpp);
+      tpp = tpc;
+    }
+  }
+  h.put(tpp, kl, nl);
+  set_status(g, nl, tpp);
+}
+
+// The records one upwind update at node (iz, ix) reads: the neighbours j =
+// ix -/+ 1 and j2 = ix -/+ 2 along its row, k = iz -/+ 1 and k2 = iz -/+ 2
+// along its column (absent nodes read as far).
+struct Stencil {
+  int2 self;
+  int2 j[2], j2[2], k[2], k2[2];
+  bool vj[2], vk[2], vj2[2], vk2[2];
+  float vel, risti;
+};
+
+__device__ __forceinline__ int2 ld_rec(const Grid& g, bool ok, int iz, int ix) {
+  return ok ? g.rec[g.lin(iz, ix)] : make_int2(0, FAR);
+}
+
+__device__ __forceinline__ void load_stencil(const Grid& g, int iz, int ix, Stencil& s) {
+  s.self = g.rec[g.lin(iz, ix)];
+  s.vel = g.vel[g.lin(iz, ix)];
+  s.risti = g.risti[ix - 1];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int d = q ? 1 : -1;
+    s.vj[q] = ix + d >= 1 && ix + d <= g.nx;
+    s.vj2[q] = ix + 2 * d >= 1 && ix + 2 * d <= g.nx;
+    s.vk[q] = iz + d >= 1 && iz + d <= g.nz;
+    s.vk2[q] = iz + 2 * d >= 1 && iz + 2 * d <= g.nz;
+    s.j[q] = ld_rec(g, s.vj[q], iz, ix + d);
+    s.k[q] = ld_rec(g, s.vk[q], iz + d, ix);
+    if (g.fom) {
+      s.j2[q] = ld_rec(g, s.vj2[q], iz, ix + 2 * d);
+      s.k2[q] = ld_rec(g, s.vk2[q], iz + 2 * d, ix);
+    }
+  }
+}
+
+__device__ __forceinline__ float quad_root(float a, float b, float c) {
+#pragma clang fp contract(off)
+  float rd1 = p2(b) - 4.0f * a * c;
+  if (rd1 < 0.0f) rd1 = 0.0f;
+  return fdiv(-b + fsqrt(rd1), 2.0f * a);
+}
+
+__device__ __forceinline__ float tof(int2 r) { return __int_as_float(r.x); }
+__device__ __forceinline__ bool alive(int2 r) { return r.y == 0; }
+
+// fouds1 (CalSurfG.f90:608-696): first-order upwind update.
+__device__ float fouds1(const Grid& g, const Stencil& s) {
+#pragma clang fp contract(off)
+  const float slown = fdiv(1.0f, s.vel);
+  const float ri = g.earth, risti = s.risti;
+  bool have = false;
+  float travm = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (!s.vj[q]) continue;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      if (!s.vk[r]) continue;
+      bool sol = false;
+      float a = 0.0f, b = 0.0f, c = 0.0f, tref = 0.0f;
+      if (alive(s.j[q])) {
+        sol = true;
+        if (alive(s.k[r])) {
+          const float u = ri * g.dnx, v = risti * g.dnz;
+          const float em = tof(s.k[r]) - tof(s.j[q]);
+          a = p2(u) + p2(v);
+          b = -2.0f * p2(u) * em;
+          c = p2(u) * (p2(em) - p2(v) * p2(slown));
+          tref = tof(s.j[q]);
+        } else {
+          a = 1.0f;
+          b = 0.0f;
+          c = -p2(slown) * p2(ri) * p2(g.dnx);
+          tref = tof(s.j[q]);
+        }
+      } else if (alive(s.k[r])) {
+        sol = true;
+        a = 1.0f;
+        b = 0.0f;
+        c = -p2(slown * risti * g.dnz);
+        tref = tof(s.k[r]);
+      }
+      if (sol) {
+        const float trav = tref + quad_root(a, b, c);
+        travm = have ? fminf(trav, travm) : trav;
+        have = true;
+      }
+    }
+  }
+  return travm;
+}
+
+// fouds2 (CalSurfG.f90:698-870): mixed-order upwind update, second order
+// along a direction whose two upwind nodes are alive and monotone.
+__device__ float fouds2(const Grid& g, const Stencil& s) {
+#pragma clang fp contract(off)
+  const float slown = fdiv(1.0f, s.vel);
+  const float ri = g.earth, risti = s.risti;
+  bool have = false;
+  float travm = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (!s.vj[q]) continue;
+    const bool ajz = alive(s.j[q]);
+    const float tj = tof(s.j[q]), tj2 = tof(s.j2[q]);
+    const bool swj = ajz && s.vj2[q] && alive(s.j2[q]) && tj > tj2;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+      if (!s.vk[r]) continue;
+      const bool akx = alive(s.k[r]);
+      const float tk = tof(s.k[r]), tk2 = tof(s.k2[r]);
+      const bool swk = akx && s.vk2[r] && alive(s.k2[r]) && tk > tk2;
+      bool sol = false;
+      float a = 0.0f, b = 0.0f, c = 0.0f, tref = 0.0f, tdiv = 1.0f, u, v, em;
+      if (swj) {
+        sol = true;
+        if (swk) {
+          u = 2.0f * ri * g.dnx;
+          v = 2.0f * risti * g.dnz;
+          em = 4.0f * tj - tj2 - 4.0f * tk;
+          em = em + tk2;
+          a = p2(v) + p2(u);
+          b = 2.0f * em * p2(u);
+          c = p2(u) * (p2(em) - p2(slown) * p2(v));
+          tref = 4.0f * tj - tj2;
+          tdiv = 3.0f;
+        } else if (akx) {
+          u = risti * g.dnz;
+          v = 2.0f * ri * g.dnx;
+          em = 3.0f * tk - 4.0f * tj + tj2;
+          a = p2(v) + 9.0f * p2(u);
+          b = 6.0f * em * p2(u);
+          c = p2(u) * (p2(em) - p2(slown) * p2(v));
+          tref = tk;
+          tdiv = 1.0f;
+        } else {
+          u = 2.0f * ri * g.dnx;
+          a = 1.0f;
+          b = 0.0f;
+          c = -p2(u) * p2(slown);
+          tref = 4.0f * tj - tj2;
+          tdiv = 3.0f;
+        }
+      } else if (ajz) {
+        sol = true;
+        if (swk) {
+          u = ri * g.dnx;
+          v = 2.0f * risti * g.dnz;
+          em = 3.0f * tj - 4.0f * tk + tk2;
+          a = p2(v) + 9.0f * p2(u);
+          b = 6.0f * em * p2(u);
+          c = p2(u) * (p2(em) - p2(v) * p2(slown));
+          tref = tj;
+        } else if (akx) {
+          u = ri * g.dnx;
+          v = risti * g.dnz;
+          em = tk - tj;
+          a = p2(u) + p2(v);
+          b = -2.0f * p2(u) * em;
+          c = p2(u) * (p2(em) - p2(v) * p2(slown));
+          tref = tj;
+        } else {
+          a = 1.0f;
+          b = 0.0f;
+          c = -p2(slown) * p2(ri) * p2(g.dnx);
+          tref = tj;
+        }
+      } else if (swk) {
+        sol = true;
+        u = 2.0f * risti * g.dnz;
+        a = 1.0f;
+        b = 0.0f;
+        c = -p2(u) * p2(slown);
+        tref = 4.0f * tk - tk2;
+        tdiv = 3.0f;
+      } else if (akx) {
+        sol = true;
+        a = 1.0f;
+        b = 0.0f;
+        c = -p2(slown) * p2(risti) * p2(g.dnz);
+        tref = tk;
+      }
+      if (sol) {
+        const float trav = fdiv(tref + quad_root(a, b, c), tdiv);
+        travm = have ? fminf(trav, travm) : trav;
+        have = true;
+      }
+    }
+  }
+  return travm;
+}
+
+// bilinear (CalSurfG.f90:2466-2488); nv[i][j] = node (isz-1+j, isx-1+i).
+__device__ __forceinline__ float bilinear(const float nv[2][2], float dsx, float dsz, float dnx,
+                                          float dnz) {
+#pragma clang fp contract(off)
+  float biv = 0.0f;
+#pragma unroll
+  for (int i = 1; i <= 2; i++)
+#pragma unroll
+    for (int j = 1; j <= 2; j++) {
+      const float produ = (1.0f - fabsf(fdiv((float)(i - 1) * dnx - dsx, dnx))) *
+                          (1.0f - fabsf(fdiv((float)(j - 1) * dnz - dsz, dnz)));
+      biv = biv + nv[i - 1][j - 1] * produ;
+    }
+  return biv;
+}
+
+// Source start (travel, urg /= 2, CalSurfG.f90:423-486): the four nodes of
+// the source cell get straight-ray times and enter the band.  Returns 1 for a
+// source outside the grid (the reference stops), 2 when the band outgrows the
+// tree capacity.
+__device__ int start_at_source(Heap& h, const Grid& g, float scx, float scz) {
+#pragma clang fp contract(off)
+  int isx = (int)fdiv(scx - g.gox, g.dnx) + 1;
+  int isz = (int)fdiv(scz - g.goz, g.dnz) + 1;
+  if (isx < 1 || isx > g.nx || isz < 1 || isz > g.nz) return 1;  // the reference stops
+  if (isx == g.nx) isx--;
+  if (isz == g.nz) isz--;
+  float vss[2][2];
+#pragma unroll
+  for (int i = 1; i <= 2; i++)
+#pragma unroll
+    for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = g.vel[g.lin(isz - 1 + j, isx - 1 + i)];
+  const float dsx = (scx - g.gox) - (float)(isx - 1) * g.dnx;
+  const float dsz = (scz - g.goz) - (float)(isz - 1) * g.dnz;
+  const float vsrc = bilinear(vss, dsx, dsz, g.dnx, g.dnz);
+  Pending pd = {{-1, -1, -1, -1}, {0, 0, 0, 0}};
+#pragma unroll
+  for (int i = 1; i <= 2; i++)
+#pragma unroll
+    for (int j = 1; j <= 2; j++) {
+      const float ds = fsqrt(p2(dsx - (float)(i - 1) * g.dnx) + p2(dsz - (float)(j - 1) * g.dnz));
+      const float t = fdiv(2.0f * ds, vss[i - 1][j - 1] + vsrc);
+      const int iz = isz - 1 + j, ix = isx - 1 + i;
+      g.rec[g.lin(iz, ix)].x = __float_as_int(t);
+      if (h.ntr + 1 > g.maxbt) return 2;
+      h.ntr++;
+      sift_up(h, g, pack(iz, ix), t, h.ntr, pd);
+    }
+  return 0;
+}
+
+// The marching loop of travel (CalSurfG.f90:496-597).  Returns 0, or 2 when
+// the band outgrows the tree capacity (the reference writes past it).
+__device__ int march(Heap& h, const Grid& g) {
+  while (h.ntr > 0) {
+    const int n0 = h.node(1);
+    const int ix = n0 >> 16, iz = n0 & 0xffff;
+    if (g.urg == 1) {
+      const bool edge = (ix == 1 && g.vnl != 1) || (ix == g.nx && g.vnr != g.nx) ||
+                        (iz == 1 && g.vnt != 1) || (iz == g.nz && g.vnb != g.nz);
+      if (edge) {
+        set_status(g, n0, 0);
+        return 0;
+      }
+    }
+    set_status(g, n0, 0);
+    downtree(h, g);
+    // neighbours in the reference's order: x-1, x+1, z-1, z+1
+    const int nix[4] = {ix - 1, ix + 1, ix, ix};
+    const int niz[4] = {iz, iz, iz - 1, iz + 1};
+    bool ok[4];
+    Stencil st[4];
+    Pending pd;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      ok[q] = nix[q] >= 1 && nix[q] <= g.nx && niz[q] >= 1 && niz[q] <= g.nz;
+      if (ok[q]) load_stencil(g, niz[q], nix[q], st[q]);
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      pd.node[q] = ok[q] ? pack(niz[q], nix[q]) : -1;
+      pd.pos[q] = ok[q] ? st[q].self.y : FAR;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (!ok[q]) continue;
+      const int s = st[q].self.y;
+      if (s == 0) continue;
+      const float t = g.fom ? fouds2(g, st[q]) : fouds1(g, st[q]);
+      g.rec[g.lin(niz[q], nix[q])].x = __float_as_int(t);
+      if (s == FAR) {
+        if (h.ntr + 1 > g.maxbt) return 2;
+        h.ntr++;
+        sift_up(h, g, pd.node[q], t, h.ntr, pd);
+      } else {
+        sift_up(h, g, pd.node[q], t, pd.pos[q], pd);
+      }
+    }
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
+
+// srtimes (CalSurfG.f90:1772-1906) for one receiver on the coarse grid.
+__device__ int receiver_time(const TravelDev& p, const Grid& c, float scx, float scz, float rcx,
+                             float rcz, float srx, float* out) {
+#pragma clang fp contract(off)
+  int irx = (int)fdiv(rcx - c.gox, c.dnx) + 1;
+  int irz = (int)fdiv(rcz - c.goz, c.dnz) + 1;
+  if (irx == c.nx) irx--;
+  if (irz == c.nz) irz--;
+  const int isx = (int)fdiv(scx - c.gox, c.dnx) + 1;
+  const int isz = (int)fdiv(scz - c.goz, c.dnz) + 1;
+  float sred = p2((scx - rcx) * c.earth);
+  sred = sred + p2((scz - rcz) * c.earth * srx);
+  sred = fsqrt(sred);
+  const bool sw = sred < p.dpl || (isx == irx && isz == irz);
+  float trr;
+  if (sw) {
+    if (isx >= c.nx || isz >= c.nz) return 4;
+    float vss[2][2];
+#pragma unroll
+    for (int k = 1; k <= 2; k++)
+#pragma unroll
+      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.lin(isz - 1 + l, isx - 1 + k)];
+    float drx = (scx - c.gox) - (float)(isx - 1) * c.dnx;
+    float drz = (scz - c.goz) - (float)(isz - 1) * c.dnz;
+    const float vels = bilinear(vss, drx, drz, c.dnx, c.dnz);
+#pragma unroll
+    for (int k = 1; k <= 2; k++)
+#pragma unroll
+      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.lin(irz - 1 + l, irx - 1 + k)];
+    drx = (rcx - c.gox) - (float)(irx - 1) * c.dnx;
+    drz = (rcz - c.goz) - (float)(irz - 1) * c.dnz;
+    const float velr = bilinear(vss, drx, drz, c.dnx, c.dnz);
+    trr = fdiv(2.0f * sred, vels + velr);
+  } else {
+    const float drx = (rcx - c.gox) - (float)(irx - 1) * c.dnx;
+    const float drz = (rcz - c.goz) - (float)(irz - 1) * c.dnz;
+    trr = 0.0f;
+#pragma unroll
+    for (int k = 1; k <= 2; k++)
+#pragma unroll
+      for (int l = 1; l <= 2; l++) {
+        const float produ = (1.0f - fabsf(fdiv((float)(l - 1) * c.dnz - drz, c.dnz))) *
+                            (1.0f - fabsf(fdiv((float)(k - 1) * c.dnx - drx, c.dnx)));
+        trr = trr + tof(c.rec[c.lin(irz - 1 + l, irx - 1 + k)]) * produ;
+      }
+  }
+  *out = trr;
+  return 0;
+}
+
+__global__ __launch_bounds__(64) void k_travel(TravelDev p) {
+#pragma clang fp contract(off)
+  __shared__ float hk[HEAP_LDS + 1];
+  __shared__ int hn[HEAP_LDS + 1];
+  __shared__ int s_rc;
+  const int lane = threadIdx.x;
+  const long slot = blockIdx.x;
+  const long nn = (long)p.nnx * p.nnz;
+  int2* recc = p.node_c + slot * nn;
+  int2* recr = p.node_r + slot * (long)p.nr_max;
+  float* velr = p.vel_r + slot * (long)p.nr_max;
+  Heap h = {hk, hn, p.spill + slot * p.spill_cap, 0};
+
+  Grid c;
+  c.nx = p.nnx;
+  c.nz = p.nnz;
+  c.gox = p.gox;
+  c.goz = p.goz;
+  c.dnx = p.dnx;
+  c.dnz = p.dnz;
+  c.earth = p.earth;
+  c.rec = recc;
+  c.risti = p.risti_c;
+  c.urg = 2;
+  c.vnl = c.vnr = c.vnt = c.vnb = 0;
+  c.maxbt = p.maxbt_c;
+  c.fom = p.fom;
+
+  for (long q = slot; q < p.njob; q += gridDim.x) {
+    const int kk = p.job_k[q];
+    const float scx = p.scx[q], scz = p.scz[q];
+    const double* pvk = p.pv + (long)kk * p.ncol;
+    c.vel = p.veln + (long)kk * nn;
+    int rc = 0;
+    if (p.asgr) {
+      int isx = (int)fdiv(scx - p.gox, p.dnx) + 1, isz = (int)fdiv(scz - p.goz, p.dnz) + 1;
+      if (isx == p.nnx) isx--;
+      if (isz == p.nnz) isz--;
+      const int vnl = max(isx - p.sgs, 1), vnr = min(isx + p.sgs, p.nnx);
+      const int vnt = max(isz - p.sgs, 1), vnb = min(isz + p.sgs, p.nnz);
+      Grid r;
+      r.nx = (vnr - vnl) * p.sgdl + 1;
+      r.nz = (vnb - vnt) * p.sgdl + 1;
+      r.dnx = p.drnx;
+      r.dnz = p.drnz;
+      r.gox = p.gox + p.dnx * (float)(vnl - 1);
+      r.goz = p.goz + p.dnz * (float)(vnt - 1);
+      r.earth = p.earth;
+      r.vel = velr;
+      r.rec = recr;
+      r.risti = p.risti_r + (long)(vnl - 1) * p.nrx_max;
+      r.urg = 1;
+      r.vnl = vnl;
+      r.vnr = vnr;
+      r.vnt = vnt;
+      r.vnb = vnb;
+      r.fom = p.fom;
+      r.maxbt = (r.nx > p.nnx || r.nz > p.nnz)
+                    ? nint_f(p.snb * (float)max(r.nx, p.nnx) * (float)max(r.nz, p.nnz))
+                    : p.maxbt_c;
+      // refined velocities (bsplrefine) and far status, all lanes
+      const int nrzr = p.gdz * p.sgdl, nrxr = p.gdx * p.sgdl;
+      const int origz = (vnt - 1) * p.sgdl + 1, origx = (vnl - 1) * p.sgdl + 1;
+      const int nr = r.nx * r.nz;
+      for (int e = lane; e < nr; e += 64) {
+        const int idm1 = e % r.nz + 1, idm2 = e / r.nz + 1;
+        int i, kq, j, l;
+        cell_of(origz + idm1 - 1, nrzr, p.nvz - 1, &i, &kq);
+        cell_of(origx + idm2 - 1, nrxr, p.nvx - 1, &j, &l);
+        float ui[4], vi[4], sum[4];
+        basis(l, nrxr, ui);
+        basis(kq, nrzr, vi);
+#pragma unroll
+        for (int i1 = 1; i1 <= 4; i1++) {
+          sum[i1 - 1] = 0.0f;
+#pragma unroll
+          for (int j1 = 1; j1 <= 4; j1++)
+            sum[i1 - 1] = sum[i1 - 1] + ui[j1 - 1] * velv(p, pvk, i - 2 + i1, j - 2 + j1);
+          sum[i1 - 1] = vi[i1 - 1] * sum[i1 - 1];
+        }
+        velr[e] = sum[0] + sum[1] + sum[2] + sum[3];
+        recr[e] = make_int2(0, FAR);
+      }
+      __syncthreads();
+      if (lane == 0) {
+        h.ntr = 0;
+        rc = start_at_source(h, r, scx, scz);
+        if (rc == 0) rc = march(h, r);
+        s_rc = rc;
+      }
+      for (long e = lane; e < nn; e += 64) recc[e].y = FAR;
+      __syncthreads();
+      rc = s_rc;
+      // coincident nodes seed the coarse pass (CalSurfGAniso_Joint.f90:606-630)
+      const int bx = vnr - vnl + 1, bz = vnb - vnt + 1;
+      if (rc == 0)
+        for (int e = lane; e < bx * bz; e += 64) {
+          const int a = e % bz, b = e / bz;
+          const int2 v = recr[r.lin(a * p.sgdl + 1, b * p.sgdl + 1)];
+          int2& w = recc[c.lin(vnt + a, vnl + b)];
+          if (v.y >= 0) w = v;
+          else w.y = v.y;
+        }
+      __syncthreads();
+      // alive nodes next to far nodes rejoin the band (:663-683)
+      if (rc == 0)
+        for (int e = lane; e < bx * bz; e += 64) {
+          const int l = vnt + e % bz, k = vnl + e / bz;
+          if (recc[c.lin(l, k)].y != 0) continue;
+          const bool far = (l - 1 >= 1 && recc[c.lin(l - 1, k)].y == FAR) ||
+                           (l + 1 <= c.nz && recc[c.lin(l + 1, k)].y == FAR) ||
+                           (k - 1 >= 1 && recc[c.lin(l, k - 1)].y == FAR) ||
+                           (k + 1 <= c.nx && recc[c.lin(l, k + 1)].y == FAR);
+          if (far) recc[c.lin(l, k)].y = 1;
+        }
+      __syncthreads();
+      if (lane == 0 && rc == 0) {
+        // travel with urg = 2: the band in the reference's scan order (x outer)
+        h.ntr = 0;
+        c.maxbt = r.maxbt;
+        Pending pd = {{-1, -1, -1, -1}, {0, 0, 0, 0}};
+        for (int b = 0; b < bx && rc == 0; b++)
+          for (int a = 0; a < bz; a++) {
+            const int iz = vnt + a, ix = vnl + b;
+            const int2 v = recc[c.lin(iz, ix)];
+            if (v.y > 0) {
+              if (h.ntr + 1 > c.maxbt) {
+                rc = 2;
+                break;
+              }
+              h.ntr++;
+              sift_up(h, c, pack(iz, ix), tof(v), h.ntr, pd);
+            }
+          }
+        if (rc == 0) rc = march(h, c);
+        s_rc = rc;
+      }
+    } else {
+      for (long e = lane; e < nn; e += 64) recc[e] = make_int2(0, FAR);
+      __syncthreads();
+      if (lane == 0) {
+        h.ntr = 0;
+        c.urg = 0;
+        c.maxbt = p.maxbt_c;
+        rc = start_at_source(h, c, scx, scz);
+        if (rc == 0) rc = march(h, c);
+        s_rc = rc;
+      }
+    }
+    __syncthreads();
+    rc = s_rc;
+    if (rc == 0 && p.asgr) {
+      // The refined pass stops at the refined grid's far x (z) edge unless
+      // vnr (vnb), a coarse index, equals the refined node count
+      // (CalSurfG.f90:503-523).  For a source close to the model's south or
+      // east edge that happens before the band reaches a coarse node; the
+      // coarse band then starts empty and the reference leaves ttn unset.
+      bool unreached = false;
+      for (long e = lane; e < nn; e += 64) unreached |= recc[e].y != 0;
+      if (unreached) atomicMax(&s_rc, 3);
+      __syncthreads();
+      rc = s_rc;
+    }
+    const long r0 = p.rec_off[q], r1 = p.rec_off[q + 1];
+    int bad = 0;
+    for (long e = r0 + lane; e < r1; e += 64) {
+      float t = __int_as_float(0x7fc00000);
+      if (rc == 0 && receiver_time(p, c, scx, scz, p.rcx[e], p.rcz[e], p.sin_rcx[e], &t)) bad = 4;
+      p.times[e] = t;
+    }
+    if (p.ttn)
+      for (long e = lane; e < nn; e += 64) p.ttn[q * nn + e] = tof(recc[e]);
+    // a receiver read past the grid (source on the last node) marks the job
+    if (bad) atomicMax(&s_rc, 4);
+    __syncthreads();
+    if (lane == 0) p.status[q] = s_rc;
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+hipError_t launch_travel(const TravelDev& p, hipStream_t st) {
+  const long n = (long)p.nnx * p.nnz * p.K;
+  if (n > 0)
+    hipLaunchKernelGGL(k_gridder, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p);
+  if (p.njob > 0 && p.slots > 0)
+    hipLaunchKernelGGL(k_travel, dim3((unsigned)p.slots), dim3(64), 0, st, p);
+  return hipGetLastError();
+}
+
+}  // namespace dk

@@ -273,6 +273,73 @@ int32_t dazi_dk_azimuth_map(const dazi_dk_azim_desc* desc, const float* lsen, co
                             const float* gsf, const double* tRcV, float* cos_rc, float* sin_rc,
                             float* amp, float* amp_rela, float* angle);
 
+/* First-arrival traveltimes: the consumer of the phase-velocity maps pvRc.
+ * Replaces the traveltime part of the joint G builder's source loop
+ * (code/src_inv_iso_joint/CalSurfGAniso_Joint.f90:511-699 and the srtimes
+ * calls of :720-725): for every (period, source) job, gridder
+ * (CalSurfG.f90:1596-1689) dices the period's map onto the propagation grid,
+ * bsplrefine (:1698-1770) and travel (:399-598, fouds1/fouds2 :608-870, the
+ * binary tree :879-1032) march the first arrivals on the refined source grid
+ * and then on the coarse grid, and srtimes (:1772-1906) gives the time at each
+ * of the job's receivers.  Results are bit-identical to the reference's
+ * single-precision routines.
+ *
+ * Grid constants are those CalSurfGAniso_Joint.f90:375-409 sets (gdx = gdz =
+ * 5, sgdl = sgs = 8, asgr = 1, fom = 1, snb = 0.5, earth = 6371; nvx = nx-2,
+ * nvz = ny-2; goxd, gozd, dvxd, dvzd as given to the G builder).  The
+ * propagation grid has nnx = (nvx-1)*gdx+1 by nnz = (nvz-1)*gdz+1 nodes. */
+typedef struct dazi_dk_travel_desc {
+  int32_t nvx, nvz;   /* inversion grid without its border */
+  int32_t gdx, gdz;   /* propagation-grid dicing */
+  int32_t sgdl, sgs;  /* source-grid dicing level, its half-width in coarse nodes */
+  int32_t asgr, fom;  /* source-grid refinement (0/1), mixed-order scheme (0/1) */
+  float goxd, gozd;   /* latitude and longitude of the grid origin, degrees */
+  float dvxd, dvzd;   /* inversion-grid spacing, degrees */
+  float snb;          /* narrow-band capacity as a fraction of the grid's nodes */
+  float earth;        /* Earth radius, km */
+  int32_t kmax;       /* periods in pvRc */
+  int32_t slots;      /* concurrent jobs on the device (0: automatic) */
+} dazi_dk_travel_desc;
+
+typedef struct dazi_dk_travel_plan dazi_dk_travel_plan;
+
+/* Host arrays in: njob jobs, job q at period job_k[q] (0-based column of
+ * pvRc) with its source at colatitude scx[q] and longitude scz[q] (radians,
+ * as scxf/sczf of the G builder) and receivers rec_off[q] ..
+ * rec_off[q+1]-1 of rcx/rcz (radians).  Validates (the reference stops on a
+ * source or receiver outside the model), computes the sin tables the
+ * marching reads, uploads the geometry and reserves the per-slot work memory
+ * on the current device.  The plan belongs to that device. */
+int32_t dazi_dk_travel_plan_create(const dazi_dk_travel_desc* desc, int64_t njob,
+                                   const int32_t* job_k, const float* scx, const float* scz,
+                                   const int64_t* rec_off, const float* rcx, const float* rcz,
+                                   dazi_dk_travel_plan** plan);
+
+/* Device pointers; enqueues on stream.  pvRc (nx*ny, kmax) f64 as the stage
+ * writes it; times (rec_off[njob]) f32 in job then receiver order (the
+ * reference's dsurf order); ttn (njob, nnz*nnx) f32 coarse fields in the
+ * reference's ttn(nnz, nnx) layout, or NULL; status (njob) i32: 0, or 1 when
+ * the source falls outside its refined grid (the reference stops), 2 when a
+ * job's narrow band outgrows snb*nodes (the reference would write past its
+ * tree), 3 when the refined pass stopped before its band reached a coarse node
+ * (a source within about a coarse node of the south or east edge: the
+ * reference's exit test at CalSurfG.f90:503-523 compares the coarse vnr/vnb
+ * with the refined node count, and its coarse field stays unset), 4 when a
+ * receiver in the source cell meets a source on the last grid node (the
+ * reference reads past the grid); such jobs' times are NaN.
+ * Calls on one plan must not overlap. */
+int32_t dazi_dk_travel_run_dev(dazi_dk_travel_plan* plan, const double* pvRc, float* times,
+                               float* ttn, int32_t* status, void* stream);
+
+int32_t dazi_dk_travel_plan_destroy(dazi_dk_travel_plan* plan);
+
+/* Host pointers; synchronous: plan, upload pvRc, run, copy back, destroy.
+ * ttn may be NULL. */
+int32_t dazi_dk_travel(const dazi_dk_travel_desc* desc, int64_t njob, const int32_t* job_k,
+                       const float* scx, const float* scz, const int64_t* rec_off,
+                       const float* rcx, const float* rcz, const double* pvRc, float* times,
+                       float* ttn, int32_t* status);
+
 /* Bytes of device memory the library's pool holds on the current device
  * (current slots; buffers retired by growth are not counted). */
 int32_t dazi_dk_pool_bytes(int64_t* out);

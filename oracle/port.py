@@ -1,11 +1,14 @@
 """ORACLE — test infrastructure only (tests/, smoke() and bench.py's cpu_baseline leg).
 
-ctypes access to two CPU implementations of the reference's per-column stage:
+ctypes access to two CPU implementations of the reference's per-column stage
+and of the traveltime stage that consumes its phase velocities:
 
-* ``liborc_port.so`` — this repository's C restatement (``surfdisp96_port.c``);
+* ``liborc_port.so`` — this repository's C restatements (``surfdisp96_port.c``,
+  ``fmm_port.c``);
 * ``_ref/libdazi_ref.so`` — the reference's own Fortran (surfdisp96.f, tregn96.f,
-  DepthKernel1D.f90, CalSurfG.f90, depthkernel_ti_worker.f90) compiled in place
-  with flang by ``oracle/Makefile``.  Present only where it was built.
+  DepthKernel1D.f90, CalSurfG.f90, depthkernel_ti_worker.f90,
+  FwdAzimuthalAniMap.f90) compiled in place with flang by ``oracle/Makefile``,
+  with this directory's ``fmm_ref_driver.f90``.  Present only where it was built.
 
 Array conventions follow the reference: Fortran column-major, float32 models,
 float64 periods and outputs.
@@ -485,3 +488,83 @@ def ref_azimuth_map(nx, ny, nz, goxd, gozd, dvxd, dvzd, tRc, gcf, gsf, lsen_f, t
             line = line.rstrip("\n")
             rows.append([float(line[10 * q:10 * q + 10]) for q in range(9)])
     return np.array(rows)
+
+
+# ---- first-arrival traveltimes (fmm_port.c, fmm_ref_driver.f90) ----
+
+class FmmDesc(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("nvx", "nvz", "gdx", "gdz", "sgdl", "sgs", "asgr", "fom")] + \
+               [(n, C.c_float) for n in ("goxd", "gozd", "dvxd", "dvzd", "snb", "earth")]
+
+
+def fmm_desc(nvx, nvz, goxd, gozd, dvxd, dvzd, gdx=5, gdz=5, sgdl=8, sgs=8, asgr=1, fom=1,
+             snb=0.5, earth=6371.0):
+    """Grid constants of CalSurfGAniso_Joint.f90:375-409 (its defaults)."""
+    return FmmDesc(nvx, nvz, gdx, gdz, sgdl, sgs, asgr, fom, goxd, gozd, dvxd, dvzd, snb, earth)
+
+
+def fmm_grid(d):
+    """(nnx, nnz) of the propagation grid."""
+    return (d.nvx - 1) * d.gdx + 1, (d.nvz - 1) * d.gdz + 1
+
+
+def fmm_job(d, pv, scx, scz, rcx, rcz, lib="port"):
+    """One (period, source) job: pv the period's (nvz+2)*(nvx+2) phase
+    velocities (column fastest along x), source (scx, scz) and receivers in
+    radians.  lib "port" (fmm_port.c) or "ref" (the reference's routines).
+    Returns dict(rc, ttn (nnx*nnz, Fortran order), times, ttnr, nstsr (refined
+    pass, nrx*nrz), nrx, nrz)."""
+    nnx, nnz = fmm_grid(d)
+    pv = _f64(pv)
+    rcx, rcz = _f32(rcx), _f32(rcz)
+    nrc = rcx.size
+    nmax = ((2 * d.sgs) * d.sgdl + 1) ** 2 + 1
+    ttn = np.zeros(nnx * nnz, np.float32)
+    ttnr = np.zeros(nmax, np.float32)
+    nstsr = np.zeros(nmax, np.int32)
+    times = np.zeros(max(nrc, 1), np.float32)
+    nrx, nrz, stat = C.c_int(0), C.c_int(0), C.c_int(0)
+    if lib == "ref":
+        ref().ref_fmm_job(d.nvx, d.nvz, C.c_float(d.goxd), C.c_float(d.gozd), C.c_float(d.dvxd),
+                          C.c_float(d.dvzd), d.gdx, d.gdz, d.sgdl, d.sgs, d.asgr, d.fom,
+                          C.c_float(d.snb), C.c_float(d.earth), _ptr(pv), C.c_float(scx),
+                          C.c_float(scz), nrc, _ptr(rcx), _ptr(rcz), _ptr(ttn), C.byref(nrx),
+                          C.byref(nrz), _ptr(ttnr), _ptr(nstsr), _ptr(times), C.byref(stat))
+        rc = stat.value
+    else:
+        L = port()
+        L.orc_fmm_job.restype = C.c_int
+        rc = L.orc_fmm_job(C.byref(d), _ptr(pv), C.c_float(scx), C.c_float(scz), nrc, _ptr(rcx),
+                           _ptr(rcz), _ptr(ttn), C.byref(nrx), C.byref(nrz), _ptr(ttnr),
+                           _ptr(nstsr), _ptr(times))
+    n = nrx.value * nrz.value
+    return dict(rc=rc, ttn=ttn, times=times[:nrc], ttnr=ttnr[:n], nstsr=nstsr[:n],
+                nrx=nrx.value, nrz=nrz.value)
+
+
+def fmm_gridder(d, pv):
+    """gridder (CalSurfG.f90:1596-1689) of one period: (nnx*nnz,) float32."""
+    nnx, nnz = fmm_grid(d)
+    out = np.zeros(nnx * nnz, np.float32)
+    port().orc_fmm_gridder(C.byref(d), _ptr(_f64(pv)), _ptr(out))
+    return out
+
+
+def fmm_jobs(d, pv, job_k, scx, scz, rec_off, rcx, rcz, want_ttn=False, nthreads=0):
+    """Many jobs (OpenMP over jobs): pv (K, ncol) float64 (pvRc in Fortran
+    order), job q at period job_k[q] with receivers rec_off[q]..rec_off[q+1]-1.
+    Returns times, ttn ((njob, nnx*nnz) or None), status."""
+    L = port()
+    L.orc_fmm_jobs.restype = C.c_int
+    nnx, nnz = fmm_grid(d)
+    job_k = np.ascontiguousarray(job_k, np.int32)
+    rec_off = np.ascontiguousarray(rec_off, np.int64)
+    njob = job_k.size
+    times = np.zeros(max(int(rec_off[-1]), 1), np.float32)
+    ttn = np.zeros((njob, nnx * nnz), np.float32) if want_ttn else None
+    status = np.zeros(max(njob, 1), np.int32)
+    nth = nthreads or L.orc_max_threads()
+    L.orc_fmm_jobs(C.byref(d), _ptr(_f64(pv)), njob, _ptr(job_k), _ptr(_f32(scx)),
+                   _ptr(_f32(scz)), _ptr(rec_off), _ptr(_f32(rcx)), _ptr(_f32(rcz)), _ptr(times),
+                   _ptr(ttn) if want_ttn else None, _ptr(status), nth)
+    return times[:int(rec_off[-1])], ttn, status[:njob]

@@ -36,6 +36,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(_lib.Tune) == 24
     assert C.sizeof(_lib.Layered) == 8 + 8 + 8 * 21
     assert C.sizeof(_lib.GrowDesc) == 32
+    assert C.sizeof(_lib.TravelDesc) == 64
 
 
 def test_invalid_sizes_rejected_before_any_gpu_call():

@@ -1,0 +1,200 @@
+"""CPU tests of the traveltime stage's oracle and host logic.
+
+The oracle (oracle/fmm_port.c) restates the reference's gridder, bsplrefine,
+travel (fast marching with fouds1/fouds2 and the binary tree) and srtimes
+(CalSurfG.f90) with the per-source sequence of CalSurfGAniso_Joint.f90:511-725.
+It is pinned
+* bit-for-bit against the reference's own routines compiled from CalSurfG.f90
+  (oracle/_ref, when built here) on cases that reach every branch: both update
+  orders, with and without source-grid refinement, sources at the grid corners
+  and on grid nodes, receivers in the source cell, a refined grid larger than
+  the coarse one;
+* against the committed reference outputs (tests/golden/ref_travel.npz);
+* by what the mathematics fixes: cubic B-splines reproduce constant and linear
+  control values, and in a homogeneous medium the first arrival is the
+  great-circle distance over the velocity (to the scheme's discretisation
+  error, which a dropped term or swapped spacing would exceed).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import port as P
+from dazi_current_release_amd import _lib, configs
+from dazi_current_release_amd import travel as T
+
+F = np.float32
+PI = F(3.1415926535898)
+
+
+def grid_coords(d):
+    gox = (F(90.0) - F(d.goxd)) * PI / F(180.0)
+    goz = F(d.gozd) * PI / F(180.0)
+    dnx = F(d.dvxd) * PI / F(180.0) / F(d.gdx)
+    dnz = F(d.dvzd) * PI / F(180.0) / F(d.gdz)
+    return gox, goz, dnx, dnz
+
+
+def compare_ref(d, pv, sx, sz, rx, rz):
+    a = P.fmm_job(d, pv, sx, sz, rx, rz, lib="ref")
+    b = P.fmm_job(d, pv, sx, sz, rx, rz, lib="port")
+    assert a["rc"] == b["rc"] == 0
+    assert np.array_equal(a["ttn"], b["ttn"])
+    assert np.array_equal(a["times"], b["times"])
+    assert (a["nrx"], a["nrz"]) == (b["nrx"], b["nrz"])
+    if a["nrx"]:
+        assert np.array_equal(a["nstsr"], b["nstsr"])
+        m = a["nstsr"] >= 0
+        assert np.array_equal(a["ttnr"][m], b["ttnr"][m])
+    return a
+
+
+@pytest.mark.skipif(not P.ref_available(), reason="reference build (make -C oracle ref) absent")
+@pytest.mark.parametrize("nx,ny,asgr,fom", [(22, 26, 1, 1), (22, 26, 1, 0), (22, 26, 0, 1),
+                                            (22, 26, 0, 0), (11, 9, 1, 1), (40, 33, 1, 1)])
+def test_port_matches_reference_routines(nx, ny, asgr, fom):
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25, asgr=asgr, fom=fom)
+    pv = configs.smooth_pv(nx, ny, 1, seed=nx + ny)[0]
+    gox, goz, dnx, dnz = grid_coords(d)
+    nnx, nnz = P.fmm_grid(d)
+    rng = np.random.default_rng(nx * ny)
+    # sources: interior, near each corner (refined grids clipped at the model
+    # edges), exactly on a grid node
+    srcs = [(gox + F(0.37) * dnx * (nnx - 1), goz + F(0.61) * dnz * (nnz - 1)),
+            (gox + F(0.3) * dnx, goz + F(0.6) * dnz),
+            (gox + dnx * F(nnx - 3.6), goz + dnz * F(nnz - 3.3)),
+            (gox + dnx * F(7), goz + dnz * F(5))]
+    for sx, sz in srcs:
+        sx, sz = F(sx), F(sz)
+        # receivers: random, plus some inside the source cell (straight-ray branch)
+        rx = (gox + rng.uniform(0, 1, 12) * dnx * (nnx - 1)).astype(F)
+        rz = (goz + rng.uniform(0, 1, 12) * dnz * (nnz - 1)).astype(F)
+        rx[:3] = np.clip(sx + F(0.2) * dnx * np.array([0.1, 0.5, -0.3], F), gox, None)
+        rz[:3] = sz + F(0.2) * dnz * np.array([0.4, -0.2, 0.1], F)
+        compare_ref(d, pv, sx, sz, rx, rz)
+
+
+@pytest.mark.skipif(not P.ref_available(), reason="reference build (make -C oracle ref) absent")
+def test_source_by_the_far_edges_leaves_the_reference_field_unset():
+    """The refined pass stops at the refined grid's far x edge unless vnr (a
+    coarse index) equals the refined node count (CalSurfG.f90:503-523).  A
+    source within a few refined nodes of the model's south or east edge stops
+    it before the band reaches any coarse node: the coarse pass starts with an
+    empty band and the reference's ttn keeps whatever memory held.  The oracle
+    reports such a job as 3 instead of returning that memory."""
+    nx, ny = 22, 26
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25)
+    pv = configs.smooth_pv(nx, ny, 1, seed=1)[0]
+    gox, goz, dnx, dnz = grid_coords(d)
+    nnx, nnz = P.fmm_grid(d)
+    for sx, sz in [(gox + dnx * F(nnx - 1.4), goz + dnz * F(nnz - 1.7)),
+                   (gox + dnx * F(nnx - 1), goz + dnz * F(2.5))]:
+        a = P.fmm_job(d, pv, F(sx), F(sz), [], [], lib="ref")
+        b = P.fmm_job(d, pv, F(sx), F(sz), [], [], lib="port")
+        assert a["rc"] == 0 and b["rc"] == 3
+        # the refined pass of both stopped after a handful of pops with the
+        # same fields
+        assert np.array_equal(a["nstsr"], b["nstsr"]) and (a["nstsr"] == 0).sum() < 40
+
+
+def test_port_matches_committed_reference_outputs(golden):
+    g = golden("ref_travel.npz")
+    for name in ("mixed", "first", "plain", "small"):
+        nx, ny, K, asgr, fom = (int(v) for v in g[f"{name}_case"])
+        d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25, asgr=asgr, fom=fom)
+        times, ttn, st = P.fmm_jobs(d, g[f"{name}_pv"], g[f"{name}_job_k"], g[f"{name}_scx"],
+                                    g[f"{name}_scz"], g[f"{name}_rec_off"], g[f"{name}_rcx"],
+                                    g[f"{name}_rcz"], want_ttn=True)
+        assert np.array_equal(st, g[f"{name}_rc"])
+        assert np.array_equal(times, g[f"{name}_times"])
+        assert np.array_equal(ttn[g[f"{name}_ttn_jobs"]], g[f"{name}_ttn"])
+
+
+def test_gridder_reproduces_constant_and_linear_maps():
+    nx, ny = 14, 12
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25)
+    nnx, nnz = P.fmm_grid(d)
+    v = P.fmm_gridder(d, np.full(nx * ny, 3.25))
+    assert np.abs(v - 3.25).max() <= 4 * np.finfo(F).eps * 3.25
+    # control value a + b*i + c*j at vertex (row i, column j): the dicing is
+    # the same linear function of the node position
+    ii, jj = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+    pv = (3.0 + 0.01 * ii + 0.02 * jj).reshape(-1)
+    v = P.fmm_gridder(d, pv).reshape(nnx, nnz)      # [ix-1, iz-1]
+    ix, iz = np.meshgrid(np.arange(nnx), np.arange(nnz), indexing="ij")
+    want = 3.0 + 0.01 * (1 + iz / d.gdz) + 0.02 * (1 + ix / d.gdx)
+    assert np.abs(v - want).max() <= 1e-5
+
+
+def test_homogeneous_medium_gives_great_circle_times():
+    nx, ny = 30, 30
+    vel = 3.5
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25)
+    gox, goz, dnx, dnz = grid_coords(d)
+    nnx, nnz = P.fmm_grid(d)
+    sx, sz = F(gox + dnx * F(61.3)), F(goz + dnz * F(70.6))
+    rng = np.random.default_rng(0)
+    rx = (gox + rng.uniform(0.05, 0.95, 40) * dnx * (nnx - 1)).astype(F)
+    rz = (goz + rng.uniform(0.05, 0.95, 40) * dnz * (nnz - 1)).astype(F)
+    r = P.fmm_job(d, np.full(nx * ny, vel), sx, sz, rx, rz)
+    assert r["rc"] == 0
+    # great-circle distance between (colatitude, longitude) points
+    c1, c2 = float(sx), rx.astype(np.float64)
+    cosd = np.cos(c1) * np.cos(c2) + np.sin(c1) * np.sin(c2) * np.cos(rz - float(sz))
+    dist = 6371.0 * np.arccos(np.clip(cosd, -1, 1))
+    far = dist > 50.0
+    rel = np.abs(r["times"][far] - dist[far] / vel) / (dist[far] / vel)
+    # first-order-in-places fast marching on a ~5.5 km grid: about 1 % at worst
+    # along the diagonals; the colatitude factor sin(x) of the longitude spacing
+    # is 0.87 here, so swapping the spacings or dropping it errs by over 10 %
+    assert far.sum() > 30 and rel.max() < 1.5e-2 and np.median(rel) < 5e-3, rel.max()
+
+
+def test_band_overflow_is_reported_not_written_past():
+    nx, ny = 16, 16
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25, snb=0.002)
+    gox, goz, dnx, dnz = grid_coords(d)
+    r = P.fmm_job(d, np.full(nx * ny, 3.0), F(gox + dnx * 30.5), F(goz + dnz * 30.5),
+                  np.array([gox + dnx], F), np.array([goz + dnz], F))
+    assert r["rc"] == 2
+
+
+def _plan_rc(d, job_k, scx, scz, off, rcx, rcz):
+    p = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+    h = C.c_void_p()
+    rc = _lib.lib().dazi_dk_travel_plan_create(C.byref(d), len(job_k), p(job_k), p(scx), p(scz),
+                                               p(off), p(rcx), p(rcz), C.byref(h))
+    return rc, _lib.lib().dazi_dk_last_error()
+
+
+def test_plan_rejects_bad_geometry_before_any_gpu_call():
+    d = T.desc(22, 26, 29.0, 98.0, 0.25, 0.25, kmax=3)
+    G = configs.travel_geometry(22, 26, 3, 4, seed=1)
+    args = [G[k].copy() for k in ("job_k", "scx", "scz", "rec_off", "rcx", "rcz")]
+    bad = [a.copy() for a in args]
+    bad[1][2] = F(0.1)                      # source north of the model
+    assert _plan_rc(d, *bad) == (_lib.DAZI_DK_EINVAL, _plan_rc(d, *bad)[1])
+    assert b"source" in _plan_rc(d, *bad)[1]
+    bad = [a.copy() for a in args]
+    bad[5][3] = F(4.0)                      # receiver east of the model
+    rc, msg = _plan_rc(d, *bad)
+    assert rc == _lib.DAZI_DK_EINVAL and b"receiver" in msg
+    bad = [a.copy() for a in args]
+    bad[0][0] = 3                           # period index beyond kmax
+    rc, msg = _plan_rc(d, *bad)
+    assert rc == _lib.DAZI_DK_EINVAL and b"period index" in msg
+    bad = [a.copy() for a in args]
+    bad[1][0] = F(np.nan)
+    rc, msg = _plan_rc(d, *bad)
+    assert rc == _lib.DAZI_DK_EINVAL
+    d2 = T.desc(22, 26, 29.0, 98.0, 0.25, 0.25, kmax=3, fom=2)
+    rc, msg = _plan_rc(d2, *args)
+    assert rc == _lib.DAZI_DK_EINVAL and b"descriptor" in msg
+
+
+def test_travel_descriptor_layout():
+    assert C.sizeof(_lib.TravelDesc) == 64
+    d = T.desc(64, 64, 29.0, 98.0, 0.25, 0.25, kmax=30)
+    assert (d.nvx, d.nvz, d.gdx, d.sgdl, d.sgs, d.asgr, d.fom) == (62, 62, 5, 8, 8, 1, 1)
+    assert T.grid_shape(d) == (306, 306)
