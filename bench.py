@@ -255,6 +255,68 @@ def bench_azimuth(dk, np, torch, out, nx, ny, reps=20):
                               unit="GB/s", frac=alg / (ms * 1e-3) / 8e12, bytes=alg))
 
 
+def bench_travel(dk, configs, np, torch, out, nx, ny, threads, nsta=100, reps=3,
+                 cpu_jobs=128):
+    """First-arrival traveltimes on the plain_4096 stage's phase velocities
+    (64 x 64 columns: a 306 x 306 propagation grid, 30 periods): nsta stations,
+    every station a source at every period with the others as receivers, all
+    jobs in one call (gridder, refined and coarse fast marching, receiver
+    times).  Timed with HIP events on the launch stream.  The CPU figure is the
+    oracle's restatement (bit-identical to the reference's routines) with
+    OpenMP over jobs on a sample of cpu_jobs jobs, beside the reference's own
+    routines run serially on 4 jobs."""
+    from dazi_current_release_amd import travel
+    K = out["pv"].shape[0]
+    G = configs.travel_geometry(nx, ny, K, nsta, seed=11)
+    d = travel.desc(nx, ny, 29.0, 98.0, 0.25, 0.25, kmax=K)
+    plan = travel.TravelPlan(d, G["job_k"], G["scx"], G["scz"], G["rec_off"], G["rcx"], G["rcz"])
+    pv = out["pv"]
+    res = plan.run(pv)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        plan.run(pv, out=res)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    status = res["status"].cpu().numpy()
+    njob, nrec = plan.njob, plan.nrec
+    nnz, nnx = travel.grid_shape(d)
+    rec = dict(workload=f"plain_4096 pvRc, {nsta} stations x {K} periods", jobs=njob,
+               receivers=nrec, grid=[nnz, nnx], ms=ms,
+               jobs_per_s=njob / (ms * 1e-3), receivers_per_s=nrec / (ms * 1e-3),
+               node_updates_per_s=njob * nnx * nnz / (ms * 1e-3),
+               failed_jobs=int((status != 0).sum()))
+    plan.close()
+    try:
+        from oracle import port as P
+        od = P.fmm_desc(d.nvx, d.nvz, 29.0, 98.0, 0.25, 0.25)
+        pvh = pv.cpu().numpy()
+        n = min(cpu_jobs, njob)
+        sel = np.arange(n)
+        off = G["rec_off"][: n + 1]
+        t0 = time.perf_counter()
+        P.fmm_jobs(od, pvh, G["job_k"][sel], G["scx"][sel], G["scz"][sel], off,
+                   G["rcx"][: off[-1]], G["rcz"][: off[-1]], nthreads=threads)
+        t_port = time.perf_counter() - t0
+        cpu = dict(value=n / t_port, unit="jobs/s", cores=threads, kind="port",
+                   sample=f"first {n} jobs, oracle restatement, OpenMP over jobs")
+        if P.ref_available():
+            t0 = time.perf_counter()
+            for q in range(4):
+                a, b = G["rec_off"][q], G["rec_off"][q + 1]
+                P.fmm_job(od, pvh[G["job_k"][q]], G["scx"][q], G["scz"][q], G["rcx"][a:b],
+                          G["rcz"][a:b], lib="ref")
+            cpu["reference_serial_jobs_per_s"] = 4 / (time.perf_counter() - t0)
+        rec["cpu_baseline"] = cpu
+        rec["over_cpu"] = rec["jobs_per_s"] / cpu["value"]
+    except Exception as e:  # the GPU figure stands on its own
+        rec["cpu_baseline"] = dict(value=None, sample=f"failed: {e}")
+    return rec
+
+
 def bench_host_path(dk, configs, np, reps=3):
     """The Fortran drop-in's host entry points on plain_4096: depthkernel then
     depthkernelTI through the C ABI with host arrays (inputs copied in, all
@@ -289,6 +351,8 @@ def main():
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-host", action="store_true")
     ap.add_argument("--no-grows", action="store_true")
+    ap.add_argument("--no-travel", action="store_true")
+    ap.add_argument("--stations", type=int, default=100)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
@@ -428,6 +492,10 @@ def main():
     if not args.no_grows:
         grow = bench_grows(dk, configs, np, torch, vel, out, W["nx"] - 2, W["ny"] - 2)
         grow["azimuth_map"] = bench_azimuth(dk, np, torch, out, W["nx"], W["ny"])
+    trav = None
+    if not args.no_travel:
+        trav = bench_travel(dk, configs, np, torch, out, W["nx"], W["ny"], threads,
+                            nsta=args.stations)
 
     cpu = None
     if not args.no_cpu:
@@ -458,7 +526,8 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp, ragged=c3, realizations=c5, host_path=host, g_rows=grow),
+                    vjp=vjp, ragged=c3, realizations=c5, host_path=host, g_rows=grow,
+                    travel=trav),
         roofline=roofline, cpu_baseline=cpu)
     print(json.dumps(rec), flush=True)
 

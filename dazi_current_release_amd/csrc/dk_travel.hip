@@ -16,14 +16,14 @@
 // Fast marching is a sequential chain (pop the earliest band node, update its
 // four neighbours), and its result depends on the order of the pops, so each
 // job runs it in the reference's order on one lane.  The parallelism is across
-// jobs: one 64-lane workgroup per job slot, the lanes sharing the data-parallel
-// steps (refined velocities, status clearing, band seeding, receivers) and
-// lane 0 running the marching.  The narrow band (a binary heap with the
-// reference's sift rules) lives in LDS for its first HEAP_LDS entries and
-// spills to a per-slot global array beyond that.  Each node record holds
-// (traveltime bits, status) in 8 bytes, so one load fetches both; per pop,
-// the records the four upwind updates read are loaded together before any of
-// the heap stores.
+// jobs: one 64-lane workgroup per job slot.  The lanes share the
+// data-parallel steps (refined velocities, status clearing, band seeding,
+// receivers); in the marching the tree operations run uniformly on the whole
+// wave and the up to 16 quadrant solutions of a pop's four neighbours run one
+// per lane.  The narrow band (a binary heap with the reference's sift rules)
+// lives in LDS for its first HEAP_LDS entries and spills to a per-slot global
+// array beyond that.  Each node record holds (traveltime bits, status) in 8
+// bytes, so one load fetches both.
 //
 // Arithmetic is the reference's single precision in its operation order with
 // contraction off; division and sqrt are evaluated in double and rounded once
@@ -216,38 +216,8 @@ __device__ __forceinline__ void downtree(Heap& h, const Grid& g) {
   set_status(g, nl, tpp);
 }
 
-// The records one upwind update at node (iz, ix) reads: the neighbours j =
-// ix -/+ 1 and j2 = ix -/+ 2 along its row, k = iz -/+ 1 and k2 = iz -/+ 2
-// along its column (absent nodes read as far).
-struct Stencil {
-  int2 self;
-  int2 j[2], j2[2], k[2], k2[2];
-  bool vj[2], vk[2], vj2[2], vk2[2];
-  float vel, risti;
-};
-
 __device__ __forceinline__ int2 ld_rec(const Grid& g, bool ok, int iz, int ix) {
   return ok ? g.rec[g.lin(iz, ix)] : make_int2(0, FAR);
-}
-
-__device__ __forceinline__ void load_stencil(const Grid& g, int iz, int ix, Stencil& s) {
-  s.self = g.rec[g.lin(iz, ix)];
-  s.vel = g.vel[g.lin(iz, ix)];
-  s.risti = g.risti[ix - 1];
-#pragma unroll
-  for (int q = 0; q < 2; q++) {
-    const int d = q ? 1 : -1;
-    s.vj[q] = ix + d >= 1 && ix + d <= g.nx;
-    s.vj2[q] = ix + 2 * d >= 1 && ix + 2 * d <= g.nx;
-    s.vk[q] = iz + d >= 1 && iz + d <= g.nz;
-    s.vk2[q] = iz + 2 * d >= 1 && iz + 2 * d <= g.nz;
-    s.j[q] = ld_rec(g, s.vj[q], iz, ix + d);
-    s.k[q] = ld_rec(g, s.vk[q], iz + d, ix);
-    if (g.fom) {
-      s.j2[q] = ld_rec(g, s.vj2[q], iz, ix + 2 * d);
-      s.k2[q] = ld_rec(g, s.vk2[q], iz + 2 * d, ix);
-    }
-  }
 }
 
 __device__ __forceinline__ float quad_root(float a, float b, float c) {
@@ -260,151 +230,116 @@ __device__ __forceinline__ float quad_root(float a, float b, float c) {
 __device__ __forceinline__ float tof(int2 r) { return __int_as_float(r.x); }
 __device__ __forceinline__ bool alive(int2 r) { return r.y == 0; }
 
-// fouds1 (CalSurfG.f90:608-696): first-order upwind update.
-__device__ float fouds1(const Grid& g, const Stencil& s) {
+constexpr float NOSOL = __builtin_inff();
+
+// One quadrant of fouds1 (CalSurfG.f90:608-696): the node's neighbour j
+// along its row and k along its column; +inf when neither is alive.
+__device__ __forceinline__ float fouds1_quadrant(const Grid& g, float slown, float risti, int2 rj,
+                                                 int2 rk) {
 #pragma clang fp contract(off)
-  const float slown = fdiv(1.0f, s.vel);
-  const float ri = g.earth, risti = s.risti;
-  bool have = false;
-  float travm = 0.0f;
-#pragma unroll
-  for (int q = 0; q < 2; q++) {
-    if (!s.vj[q]) continue;
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-      if (!s.vk[r]) continue;
-      bool sol = false;
-      float a = 0.0f, b = 0.0f, c = 0.0f, tref = 0.0f;
-      if (alive(s.j[q])) {
-        sol = true;
-        if (alive(s.k[r])) {
-          const float u = ri * g.dnx, v = risti * g.dnz;
-          const float em = tof(s.k[r]) - tof(s.j[q]);
-          a = p2(u) + p2(v);
-          b = -2.0f * p2(u) * em;
-          c = p2(u) * (p2(em) - p2(v) * p2(slown));
-          tref = tof(s.j[q]);
-        } else {
-          a = 1.0f;
-          b = 0.0f;
-          c = -p2(slown) * p2(ri) * p2(g.dnx);
-          tref = tof(s.j[q]);
-        }
-      } else if (alive(s.k[r])) {
-        sol = true;
-        a = 1.0f;
-        b = 0.0f;
-        c = -p2(slown * risti * g.dnz);
-        tref = tof(s.k[r]);
-      }
-      if (sol) {
-        const float trav = tref + quad_root(a, b, c);
-        travm = have ? fminf(trav, travm) : trav;
-        have = true;
-      }
+  const float ri = g.earth;
+  float a, b, c, tref;
+  if (alive(rj)) {
+    if (alive(rk)) {
+      const float u = ri * g.dnx, v = risti * g.dnz;
+      const float em = tof(rk) - tof(rj);
+      a = p2(u) + p2(v);
+      b = -2.0f * p2(u) * em;
+      c = p2(u) * (p2(em) - p2(v) * p2(slown));
+    } else {
+      a = 1.0f;
+      b = 0.0f;
+      c = -p2(slown) * p2(ri) * p2(g.dnx);
     }
+    tref = tof(rj);
+  } else if (alive(rk)) {
+    a = 1.0f;
+    b = 0.0f;
+    c = -p2(slown * risti * g.dnz);
+    tref = tof(rk);
+  } else {
+    return NOSOL;
   }
-  return travm;
+  return tref + quad_root(a, b, c);
 }
 
-// fouds2 (CalSurfG.f90:698-870): mixed-order upwind update, second order
-// along a direction whose two upwind nodes are alive and monotone.
-__device__ float fouds2(const Grid& g, const Stencil& s) {
+// One quadrant of fouds2 (CalSurfG.f90:698-870): second order along a
+// direction whose two upwind nodes (j, j2 or k, k2) are alive and monotone.
+__device__ __forceinline__ float fouds2_quadrant(const Grid& g, float slown, float risti, int2 rj,
+                                                 int2 rj2, bool vj2, int2 rk, int2 rk2, bool vk2) {
 #pragma clang fp contract(off)
-  const float slown = fdiv(1.0f, s.vel);
-  const float ri = g.earth, risti = s.risti;
-  bool have = false;
-  float travm = 0.0f;
-#pragma unroll
-  for (int q = 0; q < 2; q++) {
-    if (!s.vj[q]) continue;
-    const bool ajz = alive(s.j[q]);
-    const float tj = tof(s.j[q]), tj2 = tof(s.j2[q]);
-    const bool swj = ajz && s.vj2[q] && alive(s.j2[q]) && tj > tj2;
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-      if (!s.vk[r]) continue;
-      const bool akx = alive(s.k[r]);
-      const float tk = tof(s.k[r]), tk2 = tof(s.k2[r]);
-      const bool swk = akx && s.vk2[r] && alive(s.k2[r]) && tk > tk2;
-      bool sol = false;
-      float a = 0.0f, b = 0.0f, c = 0.0f, tref = 0.0f, tdiv = 1.0f, u, v, em;
-      if (swj) {
-        sol = true;
-        if (swk) {
-          u = 2.0f * ri * g.dnx;
-          v = 2.0f * risti * g.dnz;
-          em = 4.0f * tj - tj2 - 4.0f * tk;
-          em = em + tk2;
-          a = p2(v) + p2(u);
-          b = 2.0f * em * p2(u);
-          c = p2(u) * (p2(em) - p2(slown) * p2(v));
-          tref = 4.0f * tj - tj2;
-          tdiv = 3.0f;
-        } else if (akx) {
-          u = risti * g.dnz;
-          v = 2.0f * ri * g.dnx;
-          em = 3.0f * tk - 4.0f * tj + tj2;
-          a = p2(v) + 9.0f * p2(u);
-          b = 6.0f * em * p2(u);
-          c = p2(u) * (p2(em) - p2(slown) * p2(v));
-          tref = tk;
-          tdiv = 1.0f;
-        } else {
-          u = 2.0f * ri * g.dnx;
-          a = 1.0f;
-          b = 0.0f;
-          c = -p2(u) * p2(slown);
-          tref = 4.0f * tj - tj2;
-          tdiv = 3.0f;
-        }
-      } else if (ajz) {
-        sol = true;
-        if (swk) {
-          u = ri * g.dnx;
-          v = 2.0f * risti * g.dnz;
-          em = 3.0f * tj - 4.0f * tk + tk2;
-          a = p2(v) + 9.0f * p2(u);
-          b = 6.0f * em * p2(u);
-          c = p2(u) * (p2(em) - p2(v) * p2(slown));
-          tref = tj;
-        } else if (akx) {
-          u = ri * g.dnx;
-          v = risti * g.dnz;
-          em = tk - tj;
-          a = p2(u) + p2(v);
-          b = -2.0f * p2(u) * em;
-          c = p2(u) * (p2(em) - p2(v) * p2(slown));
-          tref = tj;
-        } else {
-          a = 1.0f;
-          b = 0.0f;
-          c = -p2(slown) * p2(ri) * p2(g.dnx);
-          tref = tj;
-        }
-      } else if (swk) {
-        sol = true;
-        u = 2.0f * risti * g.dnz;
-        a = 1.0f;
-        b = 0.0f;
-        c = -p2(u) * p2(slown);
-        tref = 4.0f * tk - tk2;
-        tdiv = 3.0f;
-      } else if (akx) {
-        sol = true;
-        a = 1.0f;
-        b = 0.0f;
-        c = -p2(slown) * p2(risti) * p2(g.dnz);
-        tref = tk;
-      }
-      if (sol) {
-        const float trav = fdiv(tref + quad_root(a, b, c), tdiv);
-        travm = have ? fminf(trav, travm) : trav;
-        have = true;
-      }
+  const float ri = g.earth;
+  const bool ajz = alive(rj), akx = alive(rk);
+  const float tj = tof(rj), tj2 = tof(rj2), tk = tof(rk), tk2 = tof(rk2);
+  const bool swj = ajz && vj2 && alive(rj2) && tj > tj2;
+  const bool swk = akx && vk2 && alive(rk2) && tk > tk2;
+  float a, b, c, tref, tdiv = 1.0f, u, v, em;
+  if (swj) {
+    if (swk) {
+      u = 2.0f * ri * g.dnx;
+      v = 2.0f * risti * g.dnz;
+      em = 4.0f * tj - tj2 - 4.0f * tk;
+      em = em + tk2;
+      a = p2(v) + p2(u);
+      b = 2.0f * em * p2(u);
+      c = p2(u) * (p2(em) - p2(slown) * p2(v));
+      tref = 4.0f * tj - tj2;
+      tdiv = 3.0f;
+    } else if (akx) {
+      u = risti * g.dnz;
+      v = 2.0f * ri * g.dnx;
+      em = 3.0f * tk - 4.0f * tj + tj2;
+      a = p2(v) + 9.0f * p2(u);
+      b = 6.0f * em * p2(u);
+      c = p2(u) * (p2(em) - p2(slown) * p2(v));
+      tref = tk;
+    } else {
+      u = 2.0f * ri * g.dnx;
+      a = 1.0f;
+      b = 0.0f;
+      c = -p2(u) * p2(slown);
+      tref = 4.0f * tj - tj2;
+      tdiv = 3.0f;
     }
+  } else if (ajz) {
+    if (swk) {
+      u = ri * g.dnx;
+      v = 2.0f * risti * g.dnz;
+      em = 3.0f * tj - 4.0f * tk + tk2;
+      a = p2(v) + 9.0f * p2(u);
+      b = 6.0f * em * p2(u);
+      c = p2(u) * (p2(em) - p2(v) * p2(slown));
+      tref = tj;
+    } else if (akx) {
+      u = ri * g.dnx;
+      v = risti * g.dnz;
+      em = tk - tj;
+      a = p2(u) + p2(v);
+      b = -2.0f * p2(u) * em;
+      c = p2(u) * (p2(em) - p2(v) * p2(slown));
+      tref = tj;
+    } else {
+      a = 1.0f;
+      b = 0.0f;
+      c = -p2(slown) * p2(ri) * p2(g.dnx);
+      tref = tj;
+    }
+  } else if (swk) {
+    u = 2.0f * risti * g.dnz;
+    a = 1.0f;
+    b = 0.0f;
+    c = -p2(u) * p2(slown);
+    tref = 4.0f * tk - tk2;
+    tdiv = 3.0f;
+  } else if (akx) {
+    a = 1.0f;
+    b = 0.0f;
+    c = -p2(slown) * p2(risti) * p2(g.dnz);
+    tref = tk;
+  } else {
+    return NOSOL;
   }
-  return travm;
+  return fdiv(tref + quad_root(a, b, c), tdiv);
 }
 
 // bilinear (CalSurfG.f90:2466-2488); nv[i][j] = node (isz-1+j, isx-1+i).
@@ -426,8 +361,8 @@ __device__ __forceinline__ float bilinear(const float nv[2][2], float dsx, float
 // Source start (travel, urg /= 2, CalSurfG.f90:423-486): the four nodes of
 // the source cell get straight-ray times and enter the band.  Returns 1 for a
 // source outside the grid (the reference stops), 2 when the band outgrows the
-// tree capacity.
-__device__ int start_at_source(Heap& h, const Grid& g, float scx, float scz) {
+// tree capacity.  Uniform across the wave.
+__device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx, float scz) {
 #pragma clang fp contract(off)
   int isx = (int)fdiv(scx - g.gox, g.dnx) + 1;
   int isz = (int)fdiv(scz - g.goz, g.dnz) + 1;
@@ -458,9 +393,19 @@ __device__ int start_at_source(Heap& h, const Grid& g, float scx, float scz) {
   return 0;
 }
 
-// The marching loop of travel (CalSurfG.f90:496-597).  Returns 0, or 2 when
-// the band outgrows the tree capacity (the reference writes past it).
-__device__ int march(Heap& h, const Grid& g) {
+// The marching loop of travel (CalSurfG.f90:496-597), run by the whole wave:
+// the tree operations are uniform (every lane holds the same values and makes
+// the same stores, so each lane reads back its own writes), and the upwind
+// updates of the up to four neighbours of a popped node are spread over lanes
+// 0-15, one quadrant (neighbour n, row side q, column side r) per lane,
+// reduced by a minimum over each neighbour's four lanes (the reference's
+// running MIN over its quadrants; the times are finite, so the order of the
+// minimum does not matter).  Returns 0, or 2 when the band outgrows the tree
+// capacity (the reference writes past it).
+__device__ __forceinline__ int march(Heap& h, const Grid& g) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x;
+  const int qn = (lane >> 2) & 3, qj = (lane >> 1) & 1, qk = lane & 1;
   while (h.ntr > 0) {
     const int n0 = h.node(1);
     const int ix = n0 >> 16, iz = n0 & 0xffff;
@@ -475,34 +420,59 @@ __device__ int march(Heap& h, const Grid& g) {
     set_status(g, n0, 0);
     downtree(h, g);
     // neighbours in the reference's order: x-1, x+1, z-1, z+1
-    const int nix[4] = {ix - 1, ix + 1, ix, ix};
-    const int niz[4] = {iz, iz, iz - 1, iz + 1};
-    bool ok[4];
-    Stencil st[4];
+    float trav = NOSOL;
+    int self = 0;
+    if (lane < 16) {
+      const int ixn = ix + (qn == 0 ? -1 : qn == 1 ? 1 : 0);
+      const int izn = iz + (qn == 2 ? -1 : qn == 3 ? 1 : 0);
+      const bool okn = ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
+      if (okn) {
+        self = g.rec[g.lin(izn, ixn)].y;
+        const int dj = qj ? 1 : -1, dk = qk ? 1 : -1;
+        const bool vj = ixn + dj >= 1 && ixn + dj <= g.nx;
+        const bool vk = izn + dk >= 1 && izn + dk <= g.nz;
+        if (self != 0 && vj && vk) {
+          const float slown = fdiv(1.0f, g.vel[g.lin(izn, ixn)]);
+          const float risti = g.risti[ixn - 1];
+          const int2 rj = g.rec[g.lin(izn, ixn + dj)];
+          const int2 rk = g.rec[g.lin(izn + dk, ixn)];
+          if (g.fom) {
+            const bool vj2 = ixn + 2 * dj >= 1 && ixn + 2 * dj <= g.nx;
+            const bool vk2 = izn + 2 * dk >= 1 && izn + 2 * dk <= g.nz;
+            const int2 rj2 = ld_rec(g, vj2, izn, ixn + 2 * dj);
+            const int2 rk2 = ld_rec(g, vk2, izn + 2 * dk, ixn);
+            trav = fouds2_quadrant(g, slown, risti, rj, rj2, vj2, rk, rk2, vk2);
+          } else {
+            trav = fouds1_quadrant(g, slown, risti, rj, rk);
+          }
+        }
+      }
+    }
+    trav = fminf(trav, __shfl_xor(trav, 1));
+    trav = fminf(trav, __shfl_xor(trav, 2));
     Pending pd;
+    float tn[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      ok[q] = nix[q] >= 1 && nix[q] <= g.nx && niz[q] >= 1 && niz[q] <= g.nz;
-      if (ok[q]) load_stencil(g, niz[q], nix[q], st[q]);
+      const int ixn = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
+      const int izn = iz + (q == 2 ? -1 : q == 3 ? 1 : 0);
+      const bool okn = ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
+      pd.node[q] = okn ? pack(izn, ixn) : -1;
+      pd.pos[q] = __builtin_amdgcn_readlane(self, 4 * q);  // 0 (skip) off the grid
+      tn[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(trav), 4 * q));
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      pd.node[q] = ok[q] ? pack(niz[q], nix[q]) : -1;
-      pd.pos[q] = ok[q] ? st[q].self.y : FAR;
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      if (!ok[q]) continue;
-      const int s = st[q].self.y;
-      if (s == 0) continue;
-      const float t = g.fom ? fouds2(g, st[q]) : fouds1(g, st[q]);
-      g.rec[g.lin(niz[q], nix[q])].x = __float_as_int(t);
+      const int s = pd.pos[q];
+      if (pd.node[q] < 0 || s == 0) continue;
+      const int nq = pd.node[q];
+      g.rec[g.lin(nq & 0xffff, nq >> 16)].x = __float_as_int(tn[q]);
       if (s == FAR) {
         if (h.ntr + 1 > g.maxbt) return 2;
         h.ntr++;
-        sift_up(h, g, pd.node[q], t, h.ntr, pd);
+        sift_up(h, g, nq, tn[q], h.ntr, pd);
       } else {
-        sift_up(h, g, pd.node[q], t, pd.pos[q], pd);
+        sift_up(h, g, nq, tn[q], pd.pos[q], pd);
       }
     }
   }
@@ -512,7 +482,7 @@ __device__ int march(Heap& h, const Grid& g) {
 __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
 
 // srtimes (CalSurfG.f90:1772-1906) for one receiver on the coarse grid.
-__device__ int receiver_time(const TravelDev& p, const Grid& c, float scx, float scz, float rcx,
+__device__ __forceinline__ int receiver_time(const TravelDev& p, const Grid& c, float scx, float scz, float rcx,
                              float rcz, float srx, float* out) {
 #pragma clang fp contract(off)
   int irx = (int)fdiv(rcx - c.gox, c.dnx) + 1;
@@ -645,12 +615,10 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
         recr[e] = make_int2(0, FAR);
       }
       __syncthreads();
-      if (lane == 0) {
-        h.ntr = 0;
-        rc = start_at_source(h, r, scx, scz);
-        if (rc == 0) rc = march(h, r);
-        s_rc = rc;
-      }
+      h.ntr = 0;
+      rc = start_at_source(h, r, scx, scz);
+      if (rc == 0) rc = march(h, r);
+      if (lane == 0) s_rc = rc;
       for (long e = lane; e < nn; e += 64) recc[e].y = FAR;
       __syncthreads();
       rc = s_rc;
@@ -677,8 +645,9 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
           if (far) recc[c.lin(l, k)].y = 1;
         }
       __syncthreads();
-      if (lane == 0 && rc == 0) {
-        // travel with urg = 2: the band in the reference's scan order (x outer)
+      if (rc == 0) {
+        // travel with urg = 2: the band in the reference's scan order (x
+        // outer), the whole wave uniformly
         h.ntr = 0;
         c.maxbt = r.maxbt;
         Pending pd = {{-1, -1, -1, -1}, {0, 0, 0, 0}};
@@ -696,19 +665,17 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
             }
           }
         if (rc == 0) rc = march(h, c);
-        s_rc = rc;
+        if (lane == 0) s_rc = rc;
       }
     } else {
       for (long e = lane; e < nn; e += 64) recc[e] = make_int2(0, FAR);
       __syncthreads();
-      if (lane == 0) {
-        h.ntr = 0;
-        c.urg = 0;
-        c.maxbt = p.maxbt_c;
-        rc = start_at_source(h, c, scx, scz);
-        if (rc == 0) rc = march(h, c);
-        s_rc = rc;
-      }
+      h.ntr = 0;
+      c.urg = 0;
+      c.maxbt = p.maxbt_c;
+      rc = start_at_source(h, c, scx, scz);
+      if (rc == 0) rc = march(h, c);
+      if (lane == 0) s_rc = rc;
     }
     __syncthreads();
     rc = s_rc;
