@@ -114,89 +114,76 @@ struct Grid {
 
 __device__ __forceinline__ int pack(int iz, int ix) { return (ix << 16) | iz; }
 
+// The narrow band: a binary tree of (key bits, packed node) entries with the
+// reference's sift rules.  Positions 1..HEAP_LDS are in LDS; deeper ones
+// spill to global memory.  The reference also records each node's tree
+// position in nsts; here the status record only says far (-1), alive (0) or
+// close (1), and updtree finds a close node's position by a search of the
+// tree across the wave's lanes, so no tree move writes global memory.
+// Every operation comes in two forms: SP = false while the whole tree is in
+// LDS (plain LDS accesses), SP = true once it reaches the spill (an access
+// that may go to either memory is a flat access, which waits on both the LDS
+// and the global memory counters).
 struct Heap {
-  float* k;  // LDS keys, entry p at p (1..HEAP_LDS)
-  int* nd;   // LDS packed nodes
+  int2* e;   // LDS: entry p at e[p]
   int2* sp;  // spill: entry p > HEAP_LDS at sp[p - HEAP_LDS - 1]
   int ntr;
-  __device__ __forceinline__ float key(int p) const {
-    return p <= HEAP_LDS ? k[p] : __int_as_float(sp[p - HEAP_LDS - 1].x);
+  template <bool SP>
+  __device__ __forceinline__ int2 get(int p) const {
+    const int2 v = (!SP || p <= HEAP_LDS) ? e[p] : sp[p - HEAP_LDS - 1];
+    return make_int2(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y));
   }
-  __device__ __forceinline__ int node(int p) const {
-    return p <= HEAP_LDS ? nd[p] : sp[p - HEAP_LDS - 1].y;
+  template <bool SP>
+  __device__ __forceinline__ int node_at(int p) const {
+    return (!SP || p <= HEAP_LDS) ? e[p].y : sp[p - HEAP_LDS - 1].y;
   }
-  __device__ __forceinline__ void put(int p, float kk, int n) {
-    if (p <= HEAP_LDS) {
-      k[p] = kk;
-      nd[p] = n;
-    } else {
-      sp[p - HEAP_LDS - 1] = make_int2(__float_as_int(kk), n);
-    }
+  template <bool SP>
+  __device__ __forceinline__ void put(int p, int2 v) {
+    if (!SP || p <= HEAP_LDS) e[p] = v;
+    else sp[p - HEAP_LDS - 1] = v;
   }
 };
 
-__device__ __forceinline__ void set_status(const Grid& g, int packed, int s) {
-  g.rec[g.lin(packed & 0xffff, packed >> 16)].y = s;
-}
-
-// Up to four pending neighbour nodes of the current pop whose tree positions
-// must follow the moves of earlier heap operations in the same pop.
-struct Pending {
-  int node[4];
-  int pos[4];
-  __device__ __forceinline__ void moved(int n, int to) {
-#pragma unroll
-    for (int q = 0; q < 4; q++)
-      if (node[q] == n) pos[q] = to;
-  }
-};
+__device__ __forceinline__ float keyf(int2 v) { return __int_as_float(v.x); }
 
 // addtree / updtree (CalSurfG.f90:879-925, 1005-1032): the entry of node n
 // with key tv, starting at position tpc, filters up past larger parents.
-__device__ __forceinline__ void sift_up(Heap& h, const Grid& g, int n, float tv, int tpc,
-                                        Pending& pd) {
+template <bool SP>
+__device__ __forceinline__ void sift_up_t(Heap& h, int n, float tv, int tpc) {
   int tpp = tpc / 2;
   while (tpp > 0) {
-    const float kp = h.key(tpp);
-    if (tv < kp) {
-      const int np = h.node(tpp);
-      h.put(tpc, kp, np);
-      set_status(g, np, tpc);
-      pd.moved(np, tpc);
-      tpc = tpp;
-      tpp = tpc / 2;
-    } else {
-      break;
-    }
+    const int2 par = h.get<SP>(tpp);
+    if (!(tv < keyf(par))) break;
+    h.put<SP>(tpc, par);
+    tpc = tpp;
+    tpp = tpc / 2;
   }
-  h.put(tpc, tv, n);
-  set_status(g, n, tpc);
-  pd.moved(n, tpc);
+  h.put<SP>(tpc, make_int2(__float_as_int(tv), n));
+}
+
+__device__ __forceinline__ void sift_up(Heap& h, int n, float tv, int tpc) {
+  if (h.ntr <= HEAP_LDS) sift_up_t<false>(h, n, tv, tpc);
+  else sift_up_t<true>(h, n, tv, tpc);
 }
 
 // downtree (CalSurfG.f90:927-1003): the last entry replaces the root and
 // filters down past smaller children.
-__device__ __forceinline__ void downtree(Heap& h, const Grid& g) {
-  if (h.ntr == 1) {
-    h.ntr = 0;
-    return;
-  }
-  const float kl = h.key(h.ntr);
-  const int nl = h.node(h.ntr);
+template <bool SP>
+__device__ __forceinline__ void downtree_t(Heap& h) {
+  const int2 last = h.get<SP>(h.ntr);
+  const float kl = keyf(last);
   h.ntr--;
   const int ntr = h.ntr;
   int tpp = 1, tpc = 2;
   while (tpc < ntr) {
-    float kc = h.key(tpc);
-    const float k2 = h.key(tpc + 1);
-    if (kc > k2) {
+    int2 c = h.get<SP>(tpc);
+    const int2 c2 = h.get<SP>(tpc + 1);
+    if (keyf(c) > keyf(c2)) {
       tpc++;
-      kc = k2;
+      c = c2;
     }
-    if (kc < kl) {
-      const int nc = h.node(tpc);
-      h.put(tpp, kc, nc);
-      set_status(g, nc, tpp);
+    if (keyf(c) < kl) {
+      h.put<SP>(tpp, c);
       tpp = tpc;
       tpc = 2 * tpp;
     } else {
@@ -204,16 +191,50 @@ __device__ __forceinline__ void downtree(Heap& h, const Grid& g) {
     }
   }
   if (tpc == ntr) {
-    const float kc = h.key(tpc);
-    if (kc < kl) {
-      const int nc = h.node(tpc);
-      h.put(tpp, kc, nc);
-      set_status(g, nc, tpp);
+    const int2 c = h.get<SP>(tpc);
+    if (keyf(c) < kl) {
+      h.put<SP>(tpp, c);
       tpp = tpc;
     }
   }
-  h.put(tpp, kl, nl);
-  set_status(g, nl, tpp);
+  h.put<SP>(tpp, last);
+}
+
+__device__ __forceinline__ void downtree(Heap& h) {
+  if (h.ntr == 1) {
+    h.ntr = 0;
+    return;
+  }
+  if (h.ntr <= HEAP_LDS) downtree_t<false>(h);
+  else downtree_t<true>(h);
+}
+
+// Tree position of close node n: every lane compares four entries per round.
+template <bool SP>
+__device__ __forceinline__ int find_pos_t(const Heap& h, int n) {
+  const int lane = threadIdx.x;
+  for (int base = 1; base <= h.ntr; base += 256) {
+    bool hit[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int p = base + u * 64 + lane;
+      hit[u] = p <= h.ntr && h.node_at<SP>(p) == n;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const unsigned long long m = __ballot(hit[u]);
+      if (m) return base + u * 64 + __ffsll((long long)m) - 1;
+    }
+  }
+  return 0;
+}
+
+__device__ __forceinline__ int find_pos(const Heap& h, int n) {
+  return h.ntr <= HEAP_LDS ? find_pos_t<false>(h, n) : find_pos_t<true>(h, n);
+}
+
+__device__ __forceinline__ void set_status(const Grid& g, int packed, int s) {
+  g.rec[g.lin(packed & 0xffff, packed >> 16)].y = s;
 }
 
 __device__ __forceinline__ int2 ld_rec(const Grid& g, bool ok, int iz, int ix) {
@@ -377,7 +398,6 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
   const float dsx = (scx - g.gox) - (float)(isx - 1) * g.dnx;
   const float dsz = (scz - g.goz) - (float)(isz - 1) * g.dnz;
   const float vsrc = bilinear(vss, dsx, dsz, g.dnx, g.dnz);
-  Pending pd = {{-1, -1, -1, -1}, {0, 0, 0, 0}};
 #pragma unroll
   for (int i = 1; i <= 2; i++)
 #pragma unroll
@@ -385,10 +405,10 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
       const float ds = fsqrt(p2(dsx - (float)(i - 1) * g.dnx) + p2(dsz - (float)(j - 1) * g.dnz));
       const float t = fdiv(2.0f * ds, vss[i - 1][j - 1] + vsrc);
       const int iz = isz - 1 + j, ix = isx - 1 + i;
-      g.rec[g.lin(iz, ix)].x = __float_as_int(t);
+      g.rec[g.lin(iz, ix)] = make_int2(__float_as_int(t), 1);
       if (h.ntr + 1 > g.maxbt) return 2;
       h.ntr++;
-      sift_up(h, g, pack(iz, ix), t, h.ntr, pd);
+      sift_up(h, pack(iz, ix), t, h.ntr);
     }
   return 0;
 }
@@ -407,7 +427,7 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
   const int lane = threadIdx.x;
   const int qn = (lane >> 2) & 3, qj = (lane >> 1) & 1, qk = lane & 1;
   while (h.ntr > 0) {
-    const int n0 = h.node(1);
+    const int n0 = h.get<false>(1).y;
     const int ix = n0 >> 16, iz = n0 & 0xffff;
     if (g.urg == 1) {
       const bool edge = (ix == 1 && g.vnl != 1) || (ix == g.nx && g.vnr != g.nx) ||
@@ -418,61 +438,50 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
       }
     }
     set_status(g, n0, 0);
-    downtree(h, g);
-    // neighbours in the reference's order: x-1, x+1, z-1, z+1
+    // Neighbours in the reference's order x-1, x+1, z-1, z+1; lane 4n+2q+r
+    // holds quadrant (row side q, column side r) of neighbour n.  Its records
+    // are loaded before the tree work so that their latency overlaps it
+    // (downtree touches LDS only).
+    const int ixn = ix + (qn == 0 ? -1 : qn == 1 ? 1 : 0);
+    const int izn = iz + (qn == 2 ? -1 : qn == 3 ? 1 : 0);
+    const bool okn = lane < 16 && ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
+    const int dj = qj ? 1 : -1, dk = qk ? 1 : -1;
+    const bool vj = okn && ixn + dj >= 1 && ixn + dj <= g.nx;
+    const bool vk = okn && izn + dk >= 1 && izn + dk <= g.nz;
+    const bool vj2 = okn && g.fom && ixn + 2 * dj >= 1 && ixn + 2 * dj <= g.nx;
+    const bool vk2 = okn && g.fom && izn + 2 * dk >= 1 && izn + 2 * dk <= g.nz;
+    const int2 rs = okn ? g.rec[g.lin(izn, ixn)] : make_int2(0, 0);
+    const float vel = okn ? g.vel[g.lin(izn, ixn)] : 1.0f;
+    const float risti = okn ? g.risti[ixn - 1] : 0.0f;
+    const int2 rj = ld_rec(g, vj, izn, ixn + dj);
+    const int2 rk = ld_rec(g, vk, izn + dk, ixn);
+    const int2 rj2 = ld_rec(g, vj2, izn, ixn + 2 * dj);
+    const int2 rk2 = ld_rec(g, vk2, izn + 2 * dk, ixn);
+    downtree(h);
+    const int self = rs.y;  // 0 (nothing to update) off the grid
     float trav = NOSOL;
-    int self = 0;
-    if (lane < 16) {
-      const int ixn = ix + (qn == 0 ? -1 : qn == 1 ? 1 : 0);
-      const int izn = iz + (qn == 2 ? -1 : qn == 3 ? 1 : 0);
-      const bool okn = ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
-      if (okn) {
-        self = g.rec[g.lin(izn, ixn)].y;
-        const int dj = qj ? 1 : -1, dk = qk ? 1 : -1;
-        const bool vj = ixn + dj >= 1 && ixn + dj <= g.nx;
-        const bool vk = izn + dk >= 1 && izn + dk <= g.nz;
-        if (self != 0 && vj && vk) {
-          const float slown = fdiv(1.0f, g.vel[g.lin(izn, ixn)]);
-          const float risti = g.risti[ixn - 1];
-          const int2 rj = g.rec[g.lin(izn, ixn + dj)];
-          const int2 rk = g.rec[g.lin(izn + dk, ixn)];
-          if (g.fom) {
-            const bool vj2 = ixn + 2 * dj >= 1 && ixn + 2 * dj <= g.nx;
-            const bool vk2 = izn + 2 * dk >= 1 && izn + 2 * dk <= g.nz;
-            const int2 rj2 = ld_rec(g, vj2, izn, ixn + 2 * dj);
-            const int2 rk2 = ld_rec(g, vk2, izn + 2 * dk, ixn);
-            trav = fouds2_quadrant(g, slown, risti, rj, rj2, vj2, rk, rk2, vk2);
-          } else {
-            trav = fouds1_quadrant(g, slown, risti, rj, rk);
-          }
-        }
-      }
+    if (self != 0 && vj && vk) {
+      const float slown = fdiv(1.0f, vel);
+      trav = g.fom ? fouds2_quadrant(g, slown, risti, rj, rj2, vj2, rk, rk2, vk2)
+                   : fouds1_quadrant(g, slown, risti, rj, rk);
     }
     trav = fminf(trav, __shfl_xor(trav, 1));
     trav = fminf(trav, __shfl_xor(trav, 2));
-    Pending pd;
-    float tn[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int ixn = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
+      const int s = __builtin_amdgcn_readlane(self, 4 * q);  // 0 (skip) off the grid
+      if (s == 0) continue;
+      const float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(trav), 4 * q));
       const int izn = iz + (q == 2 ? -1 : q == 3 ? 1 : 0);
-      const bool okn = ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
-      pd.node[q] = okn ? pack(izn, ixn) : -1;
-      pd.pos[q] = __builtin_amdgcn_readlane(self, 4 * q);  // 0 (skip) off the grid
-      tn[q] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(trav), 4 * q));
-    }
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int s = pd.pos[q];
-      if (pd.node[q] < 0 || s == 0) continue;
-      const int nq = pd.node[q];
-      g.rec[g.lin(nq & 0xffff, nq >> 16)].x = __float_as_int(tn[q]);
+      const int ixn = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
+      const int nq = pack(izn, ixn);
+      g.rec[g.lin(izn, ixn)] = make_int2(__float_as_int(t), 1);
       if (s == FAR) {
         if (h.ntr + 1 > g.maxbt) return 2;
         h.ntr++;
-        sift_up(h, g, nq, tn[q], h.ntr, pd);
+        sift_up(h, nq, t, h.ntr);
       } else {
-        sift_up(h, g, nq, tn[q], pd.pos[q], pd);
+        sift_up(h, nq, t, find_pos(h, nq));
       }
     }
   }
@@ -533,8 +542,7 @@ __device__ __forceinline__ int receiver_time(const TravelDev& p, const Grid& c, 
 
 __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
 #pragma clang fp contract(off)
-  __shared__ float hk[HEAP_LDS + 1];
-  __shared__ int hn[HEAP_LDS + 1];
+  __shared__ int2 hent[HEAP_LDS + 1];
   __shared__ int s_rc;
   const int lane = threadIdx.x;
   const long slot = blockIdx.x;
@@ -542,7 +550,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
   int2* recc = p.node_c + slot * nn;
   int2* recr = p.node_r + slot * (long)p.nr_max;
   float* velr = p.vel_r + slot * (long)p.nr_max;
-  Heap h = {hk, hn, p.spill + slot * p.spill_cap, 0};
+  Heap h = {hent, p.spill + slot * p.spill_cap, 0};
 
   Grid c;
   c.nx = p.nnx;
@@ -650,7 +658,6 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
         // outer), the whole wave uniformly
         h.ntr = 0;
         c.maxbt = r.maxbt;
-        Pending pd = {{-1, -1, -1, -1}, {0, 0, 0, 0}};
         for (int b = 0; b < bx && rc == 0; b++)
           for (int a = 0; a < bz; a++) {
             const int iz = vnt + a, ix = vnl + b;
@@ -661,7 +668,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
                 break;
               }
               h.ntr++;
-              sift_up(h, c, pack(iz, ix), tof(v), h.ntr, pd);
+              sift_up(h, pack(iz, ix), tof(v), h.ntr);
             }
           }
         if (rc == 0) rc = march(h, c);
