@@ -465,8 +465,12 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
       trav = g.fom ? fouds2_quadrant(g, slown, risti, rj, rj2, vj2, rk, rk2, vk2)
                    : fouds1_quadrant(g, slown, risti, rj, rk);
     }
-    trav = fminf(trav, __shfl_xor(trav, 1));
-    trav = fminf(trav, __shfl_xor(trav, 2));
+    // minimum over each neighbour's quad of lanes (DPP quad permutations
+    // [1,0,3,2] and [2,3,0,1])
+    trav = fminf(trav, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(trav), 0xB1, 0xF,
+                                                               0xF, false)));
+    trav = fminf(trav, __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(trav), 0x4E, 0xF,
+                                                               0xF, false)));
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int s = __builtin_amdgcn_readlane(self, 4 * q);  // 0 (skip) off the grid
