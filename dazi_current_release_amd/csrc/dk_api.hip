@@ -790,6 +790,7 @@ int32_t dazi_dk_travel_plan_create(const dazi_dk_travel_desc* d, int64_t njob,
   if (d->asgr && (p.nrx_max > 32767 || nrz_max > 32767))
     return set_err(DAZI_DK_EINVAL, "refined source grid exceeds 32767 nodes per side");
   p.nr_max = d->asgr ? p.nrx_max * nrz_max : 1;
+  p.nrp_max = d->asgr ? (long)(p.nrx_max + 6) * (nrz_max + 6) : 1;
   const int64_t nrec = njob > 0 ? rec_off[njob] : 0;
   if (njob > 0 && rec_off[0] != 0) return set_err(DAZI_DK_EINVAL, "rec_off[0] must be 0");
   for (int64_t q = 0; q < njob; q++) {
@@ -824,8 +825,8 @@ int32_t dazi_dk_travel_plan_create(const dazi_dk_travel_desc* d, int64_t njob,
   HIPCHK(hipGetDevice(&P->dev));
   Pool* pool;
   if (int rc = get_pool(&pool)) return rc;
-  const long nn = (long)p.nnx * p.nnz;
-  const double per_slot = 8.0 * nn + 12.0 * p.nr_max + 8.0 * p.spill_cap;
+  const long nn = (long)p.nnx * p.nnz, ncp = (long)(p.nnx + 6) * (p.nnz + 6);
+  const double per_slot = 8.0 * ncp + 8.0 * p.nrp_max + 4.0 * p.nr_max + 8.0 * p.spill_cap;
   long slots = d->slots > 0 ? d->slots : (long)pool->ncu * 16;
   slots = std::min<long>(slots, std::max<long>((long)(8.0e9 / per_slot), 1));
   slots = std::max<long>(std::min<long>(slots, njob), 1);
@@ -847,8 +848,8 @@ int32_t dazi_dk_travel_plan_create(const dazi_dk_travel_desc* d, int64_t njob,
   if (int rc = dev_upload(Q, rc_tab.data(), rc_tab.size(), &p.risti_c)) return rc;
   if (int rc = dev_upload(Q, rr_tab.data(), rr_tab.size(), &p.risti_r)) return rc;
   if (int rc = dev_alloc(Q, (size_t)nn * p.K, &p.veln)) return rc;
-  if (int rc = dev_alloc(Q, (size_t)nn * slots, &p.node_c)) return rc;
-  if (int rc = dev_alloc(Q, (size_t)p.nr_max * slots, &p.node_r)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)ncp * slots, &p.node_c)) return rc;
+  if (int rc = dev_alloc(Q, (size_t)p.nrp_max * slots, &p.node_r)) return rc;
   if (int rc = dev_alloc(Q, (size_t)p.nr_max * slots, &p.vel_r)) return rc;
   if (int rc = dev_alloc(Q, (size_t)std::max<long>(p.spill_cap, 1) * slots, &p.spill)) return rc;
   *out = P.release();

@@ -134,7 +134,8 @@ struct TravelDev {
   float gox, goz, dnx, dnz, drnx, drnz, earth, snb;
   float dpl;           // srtimes' near-source distance threshold
   int maxbt_c;         // tree capacity nint(snb*nnx*nnz)
-  int nrx_max, nr_max;  // refined-grid bounds
+  int nrx_max, nr_max;  // refined-grid bounds (nodes)
+  long nrp_max;         // refined-grid node records with their halo
   const double* pv;    // (ncol, K) phase velocities, column fastest
   float* veln;         // work: (K, nnz*nnx) diced velocities
   const float* risti_c;  // nnx: earth*sinf(gox + (ix-1)*dnx)
@@ -152,9 +153,9 @@ struct TravelDev {
   float* times;
   float* ttn;  // optional (njob, nnz*nnx)
   int* status;
-  // per-slot work
-  int2* node_c;
-  int2* node_r;
+  // per-slot work: node records with a 3-node halo on every side
+  int2* node_c;  // (nnx+6)*(nnz+6) per slot
+  int2* node_r;  // nrp_max per slot
   float* vel_r;
   int2* spill;
   long spill_cap;

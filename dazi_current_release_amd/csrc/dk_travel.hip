@@ -102,14 +102,22 @@ __global__ void k_gridder(TravelDev p) {
 }
 
 // ---- one grid (coarse or refined) ----
+// Node records carry a halo of HALO far nodes on every side, so that the
+// records an upwind update reads (up to three nodes from the popped node) are
+// always inside the array and load without bounds branches.
+constexpr int HALO = 3;
+
 struct Grid {
-  int nx, nz;
+  int nx, nz, ld;      // nodes; ld = nz + 2*HALO, the records' leading dimension
   float gox, goz, dnx, dnz, earth;
-  const float* vel;    // (nz, nx) column-major
-  int2* rec;           // node records: (traveltime bits, status)
+  const float* vel;    // (nz, nx) column-major, no halo
+  int2* rec;           // node records (traveltime bits, status), with the halo
   const float* risti;  // earth * sin(colatitude) per column
   int urg, vnl, vnr, vnt, vnb, maxbt, fom;
-  __device__ __forceinline__ int lin(int iz, int ix) const { return (iz - 1) + (ix - 1) * nz; }
+  __device__ __forceinline__ int lin(int iz, int ix) const {
+    return (iz - 1 + HALO) + (ix - 1 + HALO) * ld;
+  }
+  __device__ __forceinline__ int vlin(int iz, int ix) const { return (iz - 1) + (ix - 1) * nz; }
 };
 
 __device__ __forceinline__ int pack(int iz, int ix) { return (ix << 16) | iz; }
@@ -235,10 +243,6 @@ __device__ __forceinline__ int find_pos(const Heap& h, int n) {
 
 __device__ __forceinline__ void set_status(const Grid& g, int packed, int s) {
   g.rec[g.lin(packed & 0xffff, packed >> 16)].y = s;
-}
-
-__device__ __forceinline__ int2 ld_rec(const Grid& g, bool ok, int iz, int ix) {
-  return ok ? g.rec[g.lin(iz, ix)] : make_int2(0, FAR);
 }
 
 __device__ __forceinline__ float quad_root(float a, float b, float c) {
@@ -394,7 +398,7 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
 #pragma unroll
   for (int i = 1; i <= 2; i++)
 #pragma unroll
-    for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = g.vel[g.lin(isz - 1 + j, isx - 1 + i)];
+    for (int j = 1; j <= 2; j++) vss[i - 1][j - 1] = g.vel[g.vlin(isz - 1 + j, isx - 1 + i)];
   const float dsx = (scx - g.gox) - (float)(isx - 1) * g.dnx;
   const float dsz = (scz - g.goz) - (float)(isz - 1) * g.dnz;
   const float vsrc = bilinear(vss, dsx, dsz, g.dnx, g.dnz);
@@ -425,7 +429,14 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
 __device__ __forceinline__ int march(Heap& h, const Grid& g) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x;
+  // lane 4n+2q+r: quadrant (row side q, column side r) of neighbour n, the
+  // neighbours in the reference's order x-1, x+1, z-1, z+1; record offsets
+  // from the popped node
   const int qn = (lane >> 2) & 3, qj = (lane >> 1) & 1, qk = lane & 1;
+  const int dxn = qn == 0 ? -1 : qn == 1 ? 1 : 0, dzn = qn == 2 ? -1 : qn == 3 ? 1 : 0;
+  const int dj = qj ? 1 : -1, dk = qk ? 1 : -1;
+  const int on = dzn + dxn * g.ld;
+  const int oj = on + dj * g.ld, oj2 = on + 2 * dj * g.ld, ok = on + dk, ok2 = on + 2 * dk;
   while (h.ntr > 0) {
     const int n0 = h.get<false>(1).y;
     const int ix = n0 >> 16, iz = n0 & 0xffff;
@@ -437,32 +448,33 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
         return 0;
       }
     }
-    set_status(g, n0, 0);
-    // Neighbours in the reference's order x-1, x+1, z-1, z+1; lane 4n+2q+r
-    // holds quadrant (row side q, column side r) of neighbour n.  Its records
-    // are loaded before the tree work so that their latency overlaps it
-    // (downtree touches LDS only).
-    const int ixn = ix + (qn == 0 ? -1 : qn == 1 ? 1 : 0);
-    const int izn = iz + (qn == 2 ? -1 : qn == 3 ? 1 : 0);
-    const bool okn = lane < 16 && ixn >= 1 && ixn <= g.nx && izn >= 1 && izn <= g.nz;
-    const int dj = qj ? 1 : -1, dk = qk ? 1 : -1;
-    const bool vj = okn && ixn + dj >= 1 && ixn + dj <= g.nx;
-    const bool vk = okn && izn + dk >= 1 && izn + dk <= g.nz;
-    const bool vj2 = okn && g.fom && ixn + 2 * dj >= 1 && ixn + 2 * dj <= g.nx;
-    const bool vk2 = okn && g.fom && izn + 2 * dk >= 1 && izn + 2 * dk <= g.nz;
-    const int2 rs = okn ? g.rec[g.lin(izn, ixn)] : make_int2(0, 0);
-    const float vel = okn ? g.vel[g.lin(izn, ixn)] : 1.0f;
-    const float risti = okn ? g.risti[ixn - 1] : 0.0f;
-    const int2 rj = ld_rec(g, vj, izn, ixn + dj);
-    const int2 rk = ld_rec(g, vk, izn + dk, ixn);
-    const int2 rj2 = ld_rec(g, vj2, izn, ixn + 2 * dj);
-    const int2 rk2 = ld_rec(g, vk2, izn + 2 * dk, ixn);
+    const int r0 = g.lin(iz, ix);
+    g.rec[r0].y = 0;
+    // The updates' records, loaded before the tree work so that their latency
+    // overlaps it (downtree touches LDS only); the halo keeps every address in
+    // the array, and out-of-grid rows and columns read as far (for j2, k2 the
+    // same as the reference's range test).
+    const int ixn = ix + dxn, izn = iz + dzn;
+    const bool okn = lane < 16 && (unsigned)(ixn - 1) < (unsigned)g.nx &&
+                     (unsigned)(izn - 1) < (unsigned)g.nz;
+    const bool vj = (unsigned)(ixn + dj - 1) < (unsigned)g.nx;
+    const bool vk = (unsigned)(izn + dk - 1) < (unsigned)g.nz;
+    const int2 rs = g.rec[r0 + on];
+    const int2 rj = g.rec[r0 + oj];
+    const int2 rk = g.rec[r0 + ok];
+    int2 rj2 = make_int2(0, FAR), rk2 = make_int2(0, FAR);
+    if (g.fom) {
+      rj2 = g.rec[r0 + oj2];
+      rk2 = g.rec[r0 + ok2];
+    }
+    const float vel = g.vel[okn ? g.vlin(izn, ixn) : 0];
+    const float risti = g.risti[okn ? ixn - 1 : 0];
     downtree(h);
-    const int self = rs.y;  // 0 (nothing to update) off the grid
+    const int self = okn ? rs.y : 0;  // 0: nothing to update
     float trav = NOSOL;
     if (self != 0 && vj && vk) {
       const float slown = fdiv(1.0f, vel);
-      trav = g.fom ? fouds2_quadrant(g, slown, risti, rj, rj2, vj2, rk, rk2, vk2)
+      trav = g.fom ? fouds2_quadrant(g, slown, risti, rj, rj2, true, rk, rk2, true)
                    : fouds1_quadrant(g, slown, risti, rj, rk);
     }
     // minimum over each neighbour's quad of lanes (DPP quad permutations
@@ -476,10 +488,10 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
       const int s = __builtin_amdgcn_readlane(self, 4 * q);  // 0 (skip) off the grid
       if (s == 0) continue;
       const float t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(trav), 4 * q));
-      const int izn = iz + (q == 2 ? -1 : q == 3 ? 1 : 0);
-      const int ixn = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
-      const int nq = pack(izn, ixn);
-      g.rec[g.lin(izn, ixn)] = make_int2(__float_as_int(t), 1);
+      const int izq = iz + (q == 2 ? -1 : q == 3 ? 1 : 0);
+      const int ixq = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
+      const int nq = pack(izq, ixq);
+      g.rec[g.lin(izq, ixq)] = make_int2(__float_as_int(t), 1);
       if (s == FAR) {
         if (h.ntr + 1 > g.maxbt) return 2;
         h.ntr++;
@@ -515,14 +527,14 @@ __device__ __forceinline__ int receiver_time(const TravelDev& p, const Grid& c, 
 #pragma unroll
     for (int k = 1; k <= 2; k++)
 #pragma unroll
-      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.lin(isz - 1 + l, isx - 1 + k)];
+      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.vlin(isz - 1 + l, isx - 1 + k)];
     float drx = (scx - c.gox) - (float)(isx - 1) * c.dnx;
     float drz = (scz - c.goz) - (float)(isz - 1) * c.dnz;
     const float vels = bilinear(vss, drx, drz, c.dnx, c.dnz);
 #pragma unroll
     for (int k = 1; k <= 2; k++)
 #pragma unroll
-      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.lin(irz - 1 + l, irx - 1 + k)];
+      for (int l = 1; l <= 2; l++) vss[k - 1][l - 1] = c.vel[c.vlin(irz - 1 + l, irx - 1 + k)];
     drx = (rcx - c.gox) - (float)(irx - 1) * c.dnx;
     drz = (rcz - c.goz) - (float)(irz - 1) * c.dnz;
     const float velr = bilinear(vss, drx, drz, c.dnx, c.dnz);
@@ -551,14 +563,16 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
   const int lane = threadIdx.x;
   const long slot = blockIdx.x;
   const long nn = (long)p.nnx * p.nnz;
-  int2* recc = p.node_c + slot * nn;
-  int2* recr = p.node_r + slot * (long)p.nr_max;
+  const long ncp = (long)(p.nnx + 2 * HALO) * (p.nnz + 2 * HALO);
+  int2* recc = p.node_c + slot * ncp;
+  int2* recr = p.node_r + slot * p.nrp_max;
   float* velr = p.vel_r + slot * (long)p.nr_max;
   Heap h = {hent, p.spill + slot * p.spill_cap, 0};
 
   Grid c;
   c.nx = p.nnx;
   c.nz = p.nnz;
+  c.ld = p.nnz + 2 * HALO;
   c.gox = p.gox;
   c.goz = p.goz;
   c.dnx = p.dnx;
@@ -586,6 +600,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
       Grid r;
       r.nx = (vnr - vnl) * p.sgdl + 1;
       r.nz = (vnb - vnt) * p.sgdl + 1;
+      r.ld = r.nz + 2 * HALO;
       r.dnx = p.drnx;
       r.dnz = p.drnz;
       r.gox = p.gox + p.dnx * (float)(vnl - 1);
@@ -624,14 +639,14 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
           sum[i1 - 1] = vi[i1 - 1] * sum[i1 - 1];
         }
         velr[e] = sum[0] + sum[1] + sum[2] + sum[3];
-        recr[e] = make_int2(0, FAR);
       }
+      for (int e = lane; e < (r.nx + 2 * HALO) * r.ld; e += 64) recr[e] = make_int2(0, FAR);
       __syncthreads();
       h.ntr = 0;
       rc = start_at_source(h, r, scx, scz);
       if (rc == 0) rc = march(h, r);
       if (lane == 0) s_rc = rc;
-      for (long e = lane; e < nn; e += 64) recc[e].y = FAR;
+      for (long e = lane; e < ncp; e += 64) recc[e] = make_int2(0, FAR);
       __syncthreads();
       rc = s_rc;
       // coincident nodes seed the coarse pass (CalSurfGAniso_Joint.f90:606-630)
@@ -679,7 +694,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
         if (lane == 0) s_rc = rc;
       }
     } else {
-      for (long e = lane; e < nn; e += 64) recc[e] = make_int2(0, FAR);
+      for (long e = lane; e < ncp; e += 64) recc[e] = make_int2(0, FAR);
       __syncthreads();
       h.ntr = 0;
       c.urg = 0;
@@ -697,7 +712,8 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
       // east edge that happens before the band reaches a coarse node; the
       // coarse band then starts empty and the reference leaves ttn unset.
       bool unreached = false;
-      for (long e = lane; e < nn; e += 64) unreached |= recc[e].y != 0;
+      for (long e = lane; e < nn; e += 64)
+        unreached |= recc[c.lin((int)(e % p.nnz) + 1, (int)(e / p.nnz) + 1)].y != 0;
       if (unreached) atomicMax(&s_rc, 3);
       __syncthreads();
       rc = s_rc;
@@ -710,7 +726,8 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
       p.times[e] = t;
     }
     if (p.ttn)
-      for (long e = lane; e < nn; e += 64) p.ttn[q * nn + e] = tof(recc[e]);
+      for (long e = lane; e < nn; e += 64)
+        p.ttn[q * nn + e] = tof(recc[c.lin((int)(e % p.nnz) + 1, (int)(e / p.nnz) + 1)]);
     // a receiver read past the grid (source on the last node) marks the job
     if (bad) atomicMax(&s_rc, 4);
     __syncthreads();
