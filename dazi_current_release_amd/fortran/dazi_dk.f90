@@ -30,6 +30,7 @@ module dazi_dk
   public :: depthkernel_hip, depthkernelTI_hip, dazi_dk_desc, dazi_dk_tune
   public :: g_rows_hip_add_ray, g_rows_hip_flush, dazi_dk_grow_desc, c_dazi_dk_grows
   public :: FwdAzimuthalAniMap_hip, dazi_dk_azim_desc, c_dazi_dk_azimuth_map
+  public :: surface_traveltimes_hip, dazi_dk_travel_desc, c_dazi_dk_travel
   public :: c_dazi_dk_iso, c_dazi_dk_ti, c_dazi_dk_set_tune, dazi_dk_error_text
 
   integer(c_int32_t), parameter, public :: DAZI_DK_OK = 0
@@ -55,6 +56,12 @@ module dazi_dk
     integer(c_int32_t) :: row_base
     real(c_float) :: ftol
   end type dazi_dk_grow_desc
+
+  type, bind(c) :: dazi_dk_travel_desc
+    integer(c_int32_t) :: nvx, nvz, gdx, gdz, sgdl, sgs, asgr, fom
+    real(c_float) :: goxd, gozd, dvxd, dvzd, snb, earth
+    integer(c_int32_t) :: kmax, slots
+  end type dazi_dk_travel_desc
 
   type, bind(c) :: dazi_dk_azim_desc
     integer(c_int32_t) :: nx, ny, nz, kmax
@@ -121,6 +128,21 @@ module dazi_dk
       type(dazi_dk_tune), intent(in) :: tune
       integer(c_int32_t) :: rc
     end function c_dazi_dk_set_tune
+
+    function c_dazi_dk_travel(desc, njob, job_k, scx, scz, rec_off, rcx, rcz, pvRc, times, &
+                              ttn, status) bind(c, name='dazi_dk_travel') result(rc)
+      import :: c_int32_t, c_int64_t, c_float, c_double, c_ptr, dazi_dk_travel_desc
+      type(dazi_dk_travel_desc), intent(in) :: desc
+      integer(c_int64_t), value :: njob
+      integer(c_int32_t), intent(in) :: job_k(*)
+      real(c_float), intent(in) :: scx(*), scz(*), rcx(*), rcz(*)
+      integer(c_int64_t), intent(in) :: rec_off(*)
+      real(c_double), intent(in) :: pvRc(*)
+      real(c_float), intent(out) :: times(*)
+      type(c_ptr), value :: ttn
+      integer(c_int32_t), intent(out) :: status(*)
+      integer(c_int32_t) :: rc
+    end function c_dazi_dk_travel
 
     function c_dazi_dk_last_error() bind(c, name='dazi_dk_last_error') result(p)
       import :: c_ptr
@@ -392,6 +414,71 @@ contains
 
   ! Reference: FwdAzimuthalAniMap (FwdAzimuthalAniMap.f90:6-89): the map on the
   ! GPU, then the same lines on unit 42 (opened by the caller), which is closed.
+  ! Replaces the traveltime part of the joint G builder's period/source/record
+  ! loops (CalSurfGAniso_Joint.f90:511-725: gridder, the refined source grid,
+  ! travel, srtimes) with one call for every (period, source) job, in the loops'
+  ! order: knumi = 1..kmax, srcnum = 1..nsrcsurf1(knumi), istep =
+  ! 1..nrc1(srcnum, knumi).  dsurf(count1) receives the receiver times in that
+  ! order (ndata of them), bit-identical to the reference's srtimes.  The grid
+  ! constants are the builder's (gdx = gdz = 5, sgdl = sgs = 8, asgr = 1,
+  ! fom = 1, snb = 0.5, earth = 6371).  A source or receiver outside the model
+  ! stops as the reference does; a job whose reference result is undefined
+  ! (status 2, 3 or 4 of dazi_dk_travel) stops with its index.
+  subroutine surface_traveltimes_hip(nx, ny, goxd, gozd, dvxd, dvzd, kmaxRc, pvRc, nsrcsurf, &
+                                     nrcf, kmax, nsrcsurf1, periods, nrc1, scxf, sczf, rcxf, &
+                                     rczf, dsurf, ndata)
+    integer nx, ny, kmaxRc, nsrcsurf, nrcf, kmax, ndata
+    real goxd, gozd, dvxd, dvzd
+    real*8 pvRc(nx*ny, kmaxRc)
+    integer nsrcsurf1(kmax), periods(nsrcsurf, kmax), nrc1(nsrcsurf, kmax)
+    real scxf(nsrcsurf, kmax), sczf(nsrcsurf, kmax)
+    real rcxf(nrcf, nsrcsurf, kmax), rczf(nrcf, nsrcsurf, kmax)
+    real dsurf(*)
+    type(dazi_dk_travel_desc) :: d
+    integer(c_int32_t), allocatable :: job_k(:), st(:)
+    real(c_float), allocatable :: scx(:), scz(:), rcx(:), rcz(:), times(:)
+    integer(c_int64_t), allocatable :: off(:)
+    integer :: knumi, srcnum, istep, q, njob, nrec
+    njob = 0
+    nrec = 0
+    do knumi = 1, kmax
+      do srcnum = 1, nsrcsurf1(knumi)
+        njob = njob + 1
+        nrec = nrec + nrc1(srcnum, knumi)
+      end do
+    end do
+    allocate(job_k(max(njob, 1)), st(max(njob, 1)), scx(max(njob, 1)), scz(max(njob, 1)), &
+             off(njob + 1), rcx(max(nrec, 1)), rcz(max(nrec, 1)), times(max(nrec, 1)))
+    q = 0
+    off(1) = 0
+    do knumi = 1, kmax
+      do srcnum = 1, nsrcsurf1(knumi)
+        q = q + 1
+        job_k(q) = periods(srcnum, knumi) - 1
+        scx(q) = scxf(srcnum, knumi)
+        scz(q) = sczf(srcnum, knumi)
+        do istep = 1, nrc1(srcnum, knumi)
+          rcx(off(q) + istep) = rcxf(istep, srcnum, knumi)
+          rcz(off(q) + istep) = rczf(istep, srcnum, knumi)
+        end do
+        off(q + 1) = off(q) + nrc1(srcnum, knumi)
+      end do
+    end do
+    d = dazi_dk_travel_desc(nx - 2, ny - 2, 5, 5, 8, 8, 1, 1, goxd, gozd, dvxd, dvzd, 0.5, &
+                            6371.0, kmaxRc, 0)
+    call check(c_dazi_dk_travel(d, int(njob, c_int64_t), job_k, scx, scz, off, rcx, rcz, pvRc, &
+                                times, c_null_ptr, st), 'surface traveltimes')
+    do q = 1, njob
+      if (st(q) /= 0) then
+        write(6, *) 'surface traveltimes: job ', q, ' has status ', st(q), &
+                    ' (the reference result is undefined there)'
+        stop 9303
+      end if
+    end do
+    dsurf(1:nrec) = times(1:nrec)
+    ndata = nrec
+  end subroutine surface_traveltimes_hip
+
   subroutine FwdAzimuthalAniMap_hip(nx, ny, nz, maxvp, goxd, gozd, dvxd, dvzd, kmaxRc, tRc, &
                                     gcf, gsf, Lsen_Gsc, tRcV)
     integer nx, ny, nz

@@ -17,6 +17,14 @@
 !   azm in : goxd, gozd, dvxd, dvzd (f32), gcf, gsf (nx-2,ny-2,nz-1) f32,
 !            tRcV((nx-2)*(ny-2),kmax) f64
 !   azm out: the unit-42 text lines
+! (an azm argument "-" skips it).  With two more arguments it computes the
+! receiver traveltimes on the depthkernel_hip phase velocities through
+! surface_traveltimes_hip, with the G builder's geometry arrays:
+!   trv in : goxd, gozd, dvxd, dvzd (f32), nsrcsurf, nrcf, kmaxd (int32),
+!            nsrcsurf1(kmaxd), periods(nsrcsurf,kmaxd), nrc1(nsrcsurf,kmaxd)
+!            int32, scxf, sczf (nsrcsurf,kmaxd) f32, rcxf, rczf
+!            (nrcf,nsrcsurf,kmaxd) f32
+!   trv out: ndata (int32), dsurf(ndata) f32
 program dk_fdriver
   use dazi_dk
   implicit none
@@ -56,15 +64,41 @@ program dk_fdriver
   if (command_argument_count() < 6) stop
   call get_command_argument(5, fazin)
   call get_command_argument(6, fazout)
-  open(newunit=u, file=trim(fazin), form='unformatted', access='stream', status='old')
-  allocate(gcf(nx-2, ny-2, nz-1), gsf(nx-2, ny-2, nz-1), tRcV((nx-2)*(ny-2), kmax))
-  read(u) goxd, gozd, dvxd, dvzd, gcf, gsf, tRcV
-  close(u)
-  open(42, file=trim(fazout), status='replace', action='write')
-  call FwdAzimuthalAniMap_hip(nx, ny, nz, 0, goxd, gozd, dvxd, dvzd, kmax, tRc, gcf, gsf, lsen, &
-                              tRcV)
+  if (trim(fazin) /= '-') then
+    open(newunit=u, file=trim(fazin), form='unformatted', access='stream', status='old')
+    allocate(gcf(nx-2, ny-2, nz-1), gsf(nx-2, ny-2, nz-1), tRcV((nx-2)*(ny-2), kmax))
+    read(u) goxd, gozd, dvxd, dvzd, gcf, gsf, tRcV
+    close(u)
+    open(42, file=trim(fazout), status='replace', action='write')
+    call FwdAzimuthalAniMap_hip(nx, ny, nz, 0, goxd, gozd, dvxd, dvzd, kmax, tRc, gcf, gsf, &
+                                lsen, tRcV)
+  end if
+  if (command_argument_count() < 8) stop
+  call travel_step()
 
 contains
+
+  subroutine travel_step()
+    character(len=1024) :: ftin, ftout
+    integer :: nsrcsurf, nrcf, kmaxd, ndata
+    integer, allocatable :: nsrcsurf1(:), periods(:,:), nrc1(:,:)
+    real, allocatable :: scxf(:,:), sczf(:,:), rcxf(:,:,:), rczf(:,:,:), dsurf(:)
+    call get_command_argument(7, ftin)
+    call get_command_argument(8, ftout)
+    open(newunit=u, file=trim(ftin), form='unformatted', access='stream', status='old')
+    read(u) goxd, gozd, dvxd, dvzd, nsrcsurf, nrcf, kmaxd
+    allocate(nsrcsurf1(kmaxd), periods(nsrcsurf, kmaxd), nrc1(nsrcsurf, kmaxd))
+    allocate(scxf(nsrcsurf, kmaxd), sczf(nsrcsurf, kmaxd), rcxf(nrcf, nsrcsurf, kmaxd), &
+             rczf(nrcf, nsrcsurf, kmaxd), dsurf(nrcf * nsrcsurf * kmaxd))
+    read(u) nsrcsurf1, periods, nrc1, scxf, sczf, rcxf, rczf
+    close(u)
+    call surface_traveltimes_hip(nx, ny, goxd, gozd, dvxd, dvzd, kmax, pv, nsrcsurf, nrcf, &
+                                 kmaxd, nsrcsurf1, periods, nrc1, scxf, sczf, rcxf, rczf, &
+                                 dsurf, ndata)
+    open(newunit=u, file=trim(ftout), form='unformatted', access='stream', status='replace')
+    write(u) ndata, dsurf(1:ndata)
+    close(u)
+  end subroutine travel_step
 
   subroutine g_rows_step()
   open(newunit=u, file=trim(frays), form='unformatted', access='stream', status='old')

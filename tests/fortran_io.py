@@ -74,3 +74,33 @@ def run(tmpdir, vel, depz, periods, minthk, timeout=300, env=None, rays=None, ex
     if rays is not None:
         return r, fout, fg
     return r, fout
+
+
+def write_travel(path, goxd, gozd, dvxd, dvzd, G, K):
+    """The G builder's geometry arrays for surface_traveltimes_hip from job
+    arrays G (configs.travel_geometry, the same number of sources and receivers
+    at every period): nsrcsurf1(K), periods, nrc1 (nsrcsurf, K), scxf, sczf
+    (nsrcsurf, K), rcxf, rczf (nrcf, nsrcsurf, K), Fortran order."""
+    njob = len(G["job_k"])
+    ns = njob // K
+    nr = int(np.diff(G["rec_off"]).max())
+    assert ns * K == njob and np.all(np.diff(G["rec_off"]) == nr)
+    assert np.array_equal(G["job_k"], np.repeat(np.arange(K), ns))
+    periods = (G["job_k"].reshape(K, ns).T + 1).astype(np.int32)
+    with open(path, "wb") as fh:
+        np.array([goxd, gozd, dvxd, dvzd], np.float32).tofile(fh)
+        np.array([ns, nr, K], np.int32).tofile(fh)
+        np.full(K, ns, np.int32).tofile(fh)
+        np.asfortranarray(periods).ravel(order="F").tofile(fh)
+        np.full(ns * K, nr, np.int32).tofile(fh)
+        for k in ("scx", "scz"):
+            np.asarray(G[k], np.float32).reshape(K, ns).T.ravel(order="F").tofile(fh)
+        for k in ("rcx", "rcz"):
+            a = np.asarray(G[k], np.float32).reshape(K, ns, nr)   # [k, s, r]
+            np.transpose(a, (2, 1, 0)).ravel(order="F").tofile(fh)
+
+
+def read_travel(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw, np.int32, count=1)[0])
+    return np.frombuffer(raw, np.float32, count=n, offset=4)

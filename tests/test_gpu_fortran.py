@@ -105,3 +105,31 @@ def test_fortran_g_rows_match_the_device_path_and_the_oracle(tmp_path):
     ref = P.g_rows(nx - 2, ny - 2, nz, R["ray_k"], fdm, fdmc, fdms, *args)
     for got, w in zip((rw, iw, col), ref):
         assert np.array_equal(got, w)
+
+
+def test_fortran_surface_traveltimes_match_the_oracle(tmp_path):
+    """The G builder's traveltime loop from a Fortran host: depthkernel_hip's
+    phase velocities into surface_traveltimes_hip with the builder's geometry
+    arrays, against the oracle's restatement of the reference's routines on
+    the same phase velocities (bit-exact)."""
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    from oracle import port as P
+    from dazi_current_release_amd import configs
+    W = configs.node_workload("plain_1024")
+    nx, ny = W["nx"], W["ny"]
+    nz, K = W["vel"].shape[-1], len(W["periods"])
+    vel = np.asfortranarray(W["vel"])
+    G = configs.travel_geometry(nx, ny, K, 4, seed=8)
+    ftin, ftout = str(tmp_path / "trv.bin"), str(tmp_path / "trv_out.bin")
+    FIO.write_travel(ftin, 29.0, 98.0, 0.25, 0.25, G, K)
+    r, fout = FIO.run(str(tmp_path), vel, W["depz"], W["periods"], float(W["minthk"]),
+                      extra=["-", "-", "-", "-", ftin, ftout])
+    assert r.returncode == 0, r.stdout + r.stderr
+    dsurf = FIO.read_travel(ftout)
+    o = FIO.read_output(fout, nx * ny, K, nz)
+    d = P.fmm_desc(nx - 2, ny - 2, 29.0, 98.0, 0.25, 0.25)
+    want, _, st = P.fmm_jobs(d, np.ascontiguousarray(o["pv"].T), G["job_k"], G["scx"], G["scz"],
+                             G["rec_off"], G["rcx"], G["rcz"])
+    assert np.all(st == 0)
+    assert dsurf.size == want.size and np.array_equal(dsurf, want)
