@@ -121,6 +121,30 @@ def test_edge_sources_overflow_and_plan_reuse():
     assert_same(got, o2)
 
 
+def test_band_beyond_the_lds_tree_spills_and_matches_oracle():
+    """A 486 x 486 propagation grid (100 x 100 columns) with sources near the
+    centre: the narrow band grows past the 1,024 tree entries kept in LDS
+    (about 1,270 at its largest), so the tree's spill path runs."""
+    nx = ny = 100
+    K = 2
+    pi = F(3.1415926535898)
+    gox = (F(90.0) - F(29.0)) * pi / F(180.0)
+    goz = F(98.0) * pi / F(180.0)
+    dn = F(0.25) * pi / F(180.0) / F(5)
+    rng = np.random.default_rng(17)
+    sx = np.array([gox + dn * F(228.4), gox + dn * F(240.7), gox + dn * F(251.2)], F)
+    sz = np.array([goz + dn * F(252.7), goz + dn * F(231.3), goz + dn * F(244.9)], F)
+    rx = (gox + dn * rng.uniform(5, 480, 30)).astype(F)
+    rz = (goz + dn * rng.uniform(5, 480, 30)).astype(F)
+    G = dict(job_k=np.array([0, 1, 1, 0, 1, 0], np.int32), scx=np.r_[sx, sx].astype(F),
+             scz=np.r_[sz, sz[::-1]].astype(F), rec_off=np.arange(0, 31, 5, dtype=np.int64),
+             rcx=rx, rcz=rz)
+    pv = configs.smooth_pv(nx, ny, K, seed=4)
+    d = T.desc(nx, ny, 29.0, 98.0, 0.25, 0.25, kmax=K)
+    print("travel spill grid 486 x 486, 6 jobs", flush=True)
+    assert_same(run_dev(d, G, pv), oracle(d, G, pv))
+
+
 def test_host_entry_point_matches_device_entry_point():
     nx, ny, K = 22, 26, 3
     G = configs.travel_geometry(nx, ny, K, 4, seed=2)
