@@ -237,8 +237,26 @@ __device__ __forceinline__ int find_pos_t(const Heap& h, int n) {
   return 0;
 }
 
+// LDS-only form: every lane reads its eight entries unconditionally (the index
+// clamped into the array; entries past the tree are masked by the position
+// test), so the search has no branches until the ballots.
+__device__ __forceinline__ int find_pos_lds(const Heap& h, int n) {
+  const int lane = threadIdx.x;
+  for (int base = 1; base <= h.ntr; base += 512) {
+    int nd[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) nd[u] = h.e[min(base + u * 64 + lane, HEAP_LDS)].y;
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const unsigned long long m = __ballot(nd[u] == n && base + u * 64 + lane <= h.ntr);
+      if (m) return base + u * 64 + __ffsll((long long)m) - 1;
+    }
+  }
+  return 0;
+}
+
 __device__ __forceinline__ int find_pos(const Heap& h, int n) {
-  return h.ntr <= HEAP_LDS ? find_pos_t<false>(h, n) : find_pos_t<true>(h, n);
+  return h.ntr <= HEAP_LDS ? find_pos_lds(h, n) : find_pos_t<true>(h, n);
 }
 
 __device__ __forceinline__ void set_status(const Grid& g, int packed, int s) {
