@@ -290,6 +290,32 @@ def bench_travel(dk, configs, np, torch, out, nx, ny, threads, nsta=100, reps=3,
                node_updates_per_s=njob * nnx * nnz / (ms * 1e-3),
                failed_jobs=int((status != 0).sum()))
     plan.close()
+    # the reference example's station layout (tests/golden/yunnan_travel.npz: its data
+    # file's 1,469 records and 20,877 receivers on the 38 x 42 Yunnan grid, with the
+    # example model's phase velocities)
+    yt = os.path.join(HERE, "tests", "golden", "yunnan_travel.npz")
+    if os.path.exists(yt):
+        with np.load(yt, allow_pickle=False) as f:
+            Y = {k: f[k] for k in f.files}
+        GY = configs.stations_radians(Y)
+        dy = travel.desc(38, 42, 29.0, 98.0, 0.25, 0.25, kmax=Y["pv"].shape[0])
+        py = travel.TravelPlan(dy, GY["job_k"], GY["scx"], GY["scz"], GY["rec_off"], GY["rcx"],
+                               GY["rcz"])
+        pvy = torch.as_tensor(Y["pv"].astype(np.float64), device="cuda")
+        ry = py.run(pvy)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(reps):
+            py.run(pvy, out=ry)
+        e1.record(st)
+        torch.cuda.synchronize()
+        msy = e0.elapsed_time(e1) / reps
+        same = bool(np.array_equal(ry["times"].cpu().numpy(), Y["times"]))
+        rec["example_layout"] = dict(
+            workload="reference example data file (Yunnan, 38 x 42 grid, 176 x 196 nodes)",
+            jobs=py.njob, receivers=py.nrec, ms=msy, jobs_per_s=py.njob / (msy * 1e-3),
+            receivers_per_s=py.nrec / (msy * 1e-3), identical_to_reference=same)
+        py.close()
     try:
         from oracle import port as P
         od = P.fmm_desc(d.nvx, d.nvz, 29.0, 98.0, 0.25, 0.25)

@@ -346,3 +346,18 @@ def smooth_pv(nx: int, ny: int, K: int, seed: int = 5, lo: float = 2.8, hi: floa
         m = (m - m.min()) / max(m.max() - m.min(), 1e-12)
         out[k] = (lo + (hi - lo) * m).reshape(-1) + 0.02 * k
     return out
+
+
+def stations_radians(S):
+    """Station records in degrees (src_lat, src_lon, period (1-based index),
+    rec_off, rec_lat, rec_lon: one source header per (period, source) record,
+    as a DAzi data file lists them) to traveltime job arrays: colatitude and
+    longitude in radians computed in float32 as the data reader does
+    ((90 - lat)*pi/180, lon*pi/180 with pi = 3.1415926535898 single precision,
+    Main_Jt.f90:410-431), job_k the 0-based period index."""
+    pi = np.float32(3.1415926535898)
+    c = lambda lat: ((np.float32(90.0) - np.asarray(lat, F)) * pi / np.float32(180.0)).astype(F)  # noqa: E731
+    z = lambda lon: (np.asarray(lon, F) * pi / np.float32(180.0)).astype(F)  # noqa: E731
+    return dict(job_k=(np.asarray(S["period"], np.int32) - 1), scx=c(S["src_lat"]),
+                scz=z(S["src_lon"]), rec_off=np.asarray(S["rec_off"], np.int64),
+                rcx=c(S["rec_lat"]), rcz=z(S["rec_lon"]))

@@ -155,3 +155,16 @@ def test_host_entry_point_matches_device_entry_point():
     g = run_dev(d, G, pv)
     assert np.array_equal(t, g[0]) and np.array_equal(st, g[1])
     assert np.array_equal(ttn.reshape(len(G["job_k"]), -1), g[2])
+
+
+def test_example_station_layout_matches_reference(golden):
+    """All 1,469 records of the reference example's data file (20,877
+    receivers, Yunnan grid 38 x 42) on the example model's phase velocities,
+    against the reference's travel/srtimes outputs."""
+    g = golden("yunnan_travel.npz")
+    G = configs.stations_radians(g)
+    d = T.desc(38, 42, 29.0, 98.0, 0.25, 0.25, kmax=g["pv"].shape[0])
+    print(f"travel example layout: {len(G['job_k'])} jobs", flush=True)
+    times, st, _ = run_dev(d, G, g["pv"].astype(np.float64), want_ttn=False)
+    assert np.array_equal(st, g["rc"])
+    assert np.array_equal(times, g["times"])

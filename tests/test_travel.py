@@ -198,3 +198,20 @@ def test_travel_descriptor_layout():
     d = T.desc(64, 64, 29.0, 98.0, 0.25, 0.25, kmax=30)
     assert (d.nvx, d.nvz, d.gdx, d.sgdl, d.sgs, d.asgr, d.fom) == (62, 62, 5, 8, 8, 1, 1)
     assert T.grid_shape(d) == (306, 306)
+
+
+def test_port_matches_reference_on_the_example_station_layout(golden):
+    """The reference example's own station layout (1,469 (period, source)
+    records, 20,877 receivers on the Yunnan grid) on the example model's phase
+    velocities: every 7th record against the reference's receiver times."""
+    g = golden("yunnan_travel.npz")
+    G = configs.stations_radians(g)
+    sel = np.arange(0, len(G["job_k"]), 7)
+    off = G["rec_off"]
+    rec = np.concatenate([np.arange(off[q], off[q + 1]) for q in sel])
+    sub_off = np.concatenate([[0], np.cumsum(off[sel + 1] - off[sel])])
+    d = P.fmm_desc(36, 40, 29.0, 98.0, 0.25, 0.25)
+    t, _, st = P.fmm_jobs(d, g["pv"].astype(np.float64), G["job_k"][sel], G["scx"][sel],
+                          G["scz"][sel], sub_off, G["rcx"][rec], G["rcz"][rec], nthreads=8)
+    assert np.array_equal(st, g["rc"][sel])
+    assert np.array_equal(t, g["times"][rec])
