@@ -444,6 +444,7 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
 // running MIN over its quadrants; the times are finite, so the order of the
 // minimum does not matter).  Returns 0, or 2 when the band outgrows the tree
 // capacity (the reference writes past it).
+template <bool REFINED, bool FOM>
 __device__ __forceinline__ int march(Heap& h, const Grid& g) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x;
@@ -458,7 +459,7 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
   while (h.ntr > 0) {
     const int n0 = h.get<false>(1).y;
     const int ix = n0 >> 16, iz = n0 & 0xffff;
-    if (g.urg == 1) {
+    if (REFINED) {
       const bool edge = (ix == 1 && g.vnl != 1) || (ix == g.nx && g.vnr != g.nx) ||
                         (iz == 1 && g.vnt != 1) || (iz == g.nz && g.vnb != g.nz);
       if (edge) {
@@ -481,7 +482,7 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
     const int2 rj = g.rec[r0 + oj];
     const int2 rk = g.rec[r0 + ok];
     int2 rj2 = make_int2(0, FAR), rk2 = make_int2(0, FAR);
-    if (g.fom) {
+    if (FOM) {
       rj2 = g.rec[r0 + oj2];
       rk2 = g.rec[r0 + ok2];
     }
@@ -492,8 +493,8 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
     float trav = NOSOL;
     if (self != 0 && vj && vk) {
       const float slown = fdiv(1.0f, vel);
-      trav = g.fom ? fouds2_quadrant(g, slown, risti, rj, rj2, true, rk, rk2, true)
-                   : fouds1_quadrant(g, slown, risti, rj, rk);
+      trav = FOM ? fouds2_quadrant(g, slown, risti, rj, rj2, true, rk, rk2, true)
+                 : fouds1_quadrant(g, slown, risti, rj, rk);
     }
     // minimum over each neighbour's quad of lanes (DPP quad permutations
     // [1,0,3,2] and [2,3,0,1])
@@ -574,6 +575,7 @@ __device__ __forceinline__ int receiver_time(const TravelDev& p, const Grid& c, 
   return 0;
 }
 
+template <bool FOM>
 __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
 #pragma clang fp contract(off)
   __shared__ int2 hent[HEAP_LDS + 1];
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
       __syncthreads();
       h.ntr = 0;
       rc = start_at_source(h, r, scx, scz);
-      if (rc == 0) rc = march(h, r);
+      if (rc == 0) rc = march<true, FOM>(h, r);
       if (lane == 0) s_rc = rc;
       for (long e = lane; e < ncp; e += 64) recc[e] = make_int2(0, FAR);
       __syncthreads();
@@ -708,7 +710,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
               sift_up(h, pack(iz, ix), tof(v), h.ntr);
             }
           }
-        if (rc == 0) rc = march(h, c);
+        if (rc == 0) rc = march<false, FOM>(h, c);
         if (lane == 0) s_rc = rc;
       }
     } else {
@@ -718,7 +720,7 @@ __global__ __launch_bounds__(64) void k_travel(TravelDev p) {
       c.urg = 0;
       c.maxbt = p.maxbt_c;
       rc = start_at_source(h, c, scx, scz);
-      if (rc == 0) rc = march(h, c);
+      if (rc == 0) rc = march<false, FOM>(h, c);
       if (lane == 0) s_rc = rc;
     }
     __syncthreads();
@@ -760,8 +762,12 @@ hipError_t launch_travel(const TravelDev& p, hipStream_t st) {
   const long n = (long)p.nnx * p.nnz * p.K;
   if (n > 0)
     hipLaunchKernelGGL(k_gridder, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p);
-  if (p.njob > 0 && p.slots > 0)
-    hipLaunchKernelGGL(k_travel, dim3((unsigned)p.slots), dim3(64), 0, st, p);
+  if (p.njob > 0 && p.slots > 0) {
+    if (p.fom)
+      hipLaunchKernelGGL(k_travel<true>, dim3((unsigned)p.slots), dim3(64), 0, st, p);
+    else
+      hipLaunchKernelGGL(k_travel<false>, dim3((unsigned)p.slots), dim3(64), 0, st, p);
+  }
   return hipGetLastError();
 }
 
