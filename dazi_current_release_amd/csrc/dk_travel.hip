@@ -444,8 +444,13 @@ __device__ __forceinline__ int start_at_source(Heap& h, const Grid& g, float scx
 // running MIN over its quadrants; the times are finite, so the order of the
 // minimum does not matter).  Returns 0, or 2 when the band outgrows the tree
 // capacity (the reference writes past it).
-template <bool REFINED, bool FOM>
-__device__ __forceinline__ int march(Heap& h, const Grid& g) {
+// FAST: the loop runs while the band has at most HEAP_LDS - 4 entries, so a
+// pop (one removal, at most four additions) keeps every tree position in LDS
+// and inside the tree capacity (the caller takes this form only when the
+// capacity is at least HEAP_LDS): no spill tests and no capacity test.  It
+// returns -1 when the band grows past that, and the general form continues.
+template <bool REFINED, bool FOM, bool FAST>
+__device__ __forceinline__ int march_t(Heap& h, const Grid& g) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x;
   // lane 4n+2q+r: quadrant (row side q, column side r) of neighbour n, the
@@ -457,6 +462,7 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
   const int on = dzn + dxn * g.ld;
   const int oj = on + dj * g.ld, oj2 = on + 2 * dj * g.ld, ok = on + dk, ok2 = on + 2 * dk;
   while (h.ntr > 0) {
+    if (FAST && h.ntr > HEAP_LDS - 4) return -1;
     const int n0 = h.get<false>(1).y;
     const int ix = n0 >> 16, iz = n0 & 0xffff;
     if (REFINED) {
@@ -488,7 +494,12 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
     }
     const float vel = g.vel[okn ? g.vlin(izn, ixn) : 0];
     const float risti = g.risti[okn ? ixn - 1 : 0];
-    downtree(h);
+    if (FAST) {
+      if (h.ntr == 1) h.ntr = 0;
+      else downtree_t<false>(h);
+    } else {
+      downtree(h);
+    }
     const int self = okn ? rs.y : 0;  // 0: nothing to update
     float trav = NOSOL;
     if (self != 0 && vj && vk) {
@@ -511,7 +522,14 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
       const int ixq = ix + (q == 0 ? -1 : q == 1 ? 1 : 0);
       const int nq = pack(izq, ixq);
       g.rec[g.lin(izq, ixq)] = make_int2(__float_as_int(t), 1);
-      if (s == FAR) {
+      if (FAST) {
+        if (s == FAR) {
+          h.ntr++;
+          sift_up_t<false>(h, nq, t, h.ntr);
+        } else {
+          sift_up_t<false>(h, nq, t, find_pos_lds(h, nq));
+        }
+      } else if (s == FAR) {
         if (h.ntr + 1 > g.maxbt) return 2;
         h.ntr++;
         sift_up(h, nq, t, h.ntr);
@@ -521,6 +539,15 @@ __device__ __forceinline__ int march(Heap& h, const Grid& g) {
     }
   }
   return 0;
+}
+
+template <bool REFINED, bool FOM>
+__device__ __forceinline__ int march(Heap& h, const Grid& g) {
+  if (g.maxbt >= HEAP_LDS) {
+    const int rc = march_t<REFINED, FOM, true>(h, g);
+    if (rc >= 0) return rc;
+  }
+  return march_t<REFINED, FOM, false>(h, g);
 }
 
 __device__ __forceinline__ int nint_f(float x) { return (int)roundf(x); }
