@@ -42,7 +42,8 @@ int set_err(int code, const char* fmt, ...) {
 
 enum Slot {
   S_LAYERS,
-  S_BTP,
+  S_BTP,  // btp, then btr = btp^-2.275, one double each per layer
+  S_GEO,
   S_CC,
   S_BETMX,
   S_BAD,
@@ -299,7 +300,10 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   int64_t nsolves = (int64_t)ncol * d->kmax;
   void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work;
   if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 32, &layers)) return rc;
-  if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 8, &btp)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 16, &btp)) return rc;
+  double* btr = (double*)btp + (size_t)ncol * L;
+  void* geo;
+  if (int rc = pool_get(P, S_GEO, (size_t)L * dk::PREP_GEOM_BYTES, &geo)) return rc;
   if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)ncol * 4, &bad)) return rc;
@@ -346,8 +350,8 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
                                d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                               (double4*)layers, (double*)btp, (double*)cc, (float*)betmx,
-                               (int*)bad, st));
+                               geo, (double4*)layers, (double*)btp, btr, (double*)cc,
+                               (float*)betmx, (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = ncol;
@@ -356,6 +360,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   B.Ku = d->kmax;
   B.layers = (const double4*)layers;
   B.btp = (const double*)btp;
+  B.btr = btr;
   B.periods = tRc;
   B.cc = (const double*)cc;
   B.betmx = (const float*)betmx;
@@ -1077,7 +1082,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   }
   void *layers, *btp, *cc, *betmx, *bad, *croot, *work;
   if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
-  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 16, &btp)) return rc;
+  double* btr = (double*)btp + nlay_total;
   if (int rc = pool_get(P, S_CC, (size_t)b->ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)b->ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)b->ncol * 4, &bad)) return rc;
@@ -1092,7 +1098,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
-                                (double*)btp, (double*)cc, (float*)betmx, (int*)bad, st));
+                                (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;
@@ -1103,6 +1109,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   B.toff = b->toff;
   B.layers = (const double4*)layers;
   B.btp = (const double*)btp;
+  B.btr = btr;
   B.periods = b->periods;
   B.cc = (const double*)cc;
   B.betmx = (const float*)betmx;
@@ -1159,7 +1166,8 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   }
   void *layers, *btp, *cc, *betmx, *bad, *tape;
   if (int rc = pool_get(P, S_LAYERS, (size_t)nlay_total * 32, &layers)) return rc;
-  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 8, &btp)) return rc;
+  if (int rc = pool_get(P, S_BTP, (size_t)nlay_total * 16, &btp)) return rc;
+  double* btr = (double*)btp + nlay_total;
   if (int rc = pool_get(P, S_CC, (size_t)b->ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)b->ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)b->ncol * 4, &bad)) return rc;
@@ -1172,7 +1180,7 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
-                                (double*)btp, (double*)cc, (float*)betmx, (int*)bad, st));
+                                (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;
@@ -1183,6 +1191,7 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   B.toff = b->toff;
   B.layers = (const double4*)layers;
   B.btp = (const double*)btp;
+  B.btr = btr;
   B.periods = b->periods;
   B.croot = b->c_unrounded;
   HIPCHK(dk::launch_vjp(B, grad_c, grad_vs, grad_vp, grad_rho, grad_thk, m64 ? nullptr : b->thk,

@@ -23,6 +23,7 @@ struct BatchDev {
   const int64_t* toff;
   const double4* layers;  // (d, a, b, rho) flattened
   const double* btp;      // flattening velocity factor per layer
+  const double* btr;      // btp^-2.275 per layer (rho's flattening factor)
   const double* periods;
   const double* cc;      // start velocity (= cm) per column
   const float* betmx;    // max flattened shear velocity per column
@@ -72,14 +73,18 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
                      double* scratch, int nthr, int* queue, hipStream_t st);
 
+// launch_prep_nodes first writes the L shared flattening records of the node
+// geometry to geo (PREP_GEOM_BYTES each), then the columns' layers.
+constexpr size_t PREP_GEOM_BYTES = 32;
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, double4* layers, double* btp,
-                             double* cc, float* betmx, int* bad, hipStream_t st);
+                             const float* depz, float minthk0, int L, void* geo, double4* layers,
+                             double* btp, double* btr, double* cc, float* betmx, int* bad,
+                             hipStream_t st);
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
                               const float* vp, const float* vs, const float* rho,
                               const double* thk64, const double* vp64, const double* vs64,
-                              const double* rho64, double4* layers, double* btp, double* cc,
-                              float* betmx, int* bad, hipStream_t st);
+                              const double* rho64, double4* layers, double* btp, double* btr,
+                              double* cc, float* betmx, int* bad, hipStream_t st);
 hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, int cols_per_wg,
                           int persistent, int grid, int lds_stride, hipStream_t st);
 int forward_max_blocks(int S, int cols_per_wg, int lds_stride);

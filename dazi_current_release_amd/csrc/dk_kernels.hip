@@ -123,22 +123,41 @@ struct FlatState {
   int bad = 0;
 };
 
-__device__ double4 flat_push(FlatState& s, double4 l, int m, int L, double* btp_m) {
+// The part of one layer's flattening that depends only on the thicknesses:
+// radii r0 (top) and r1 = 6370 - dr (bottom, dr the thickness sum down to the
+// layer's bottom), the flattened thickness, the velocity factor tmp and its
+// float32 value bt, rho's factor bt^-2.275 in float32 (the flattening) and in
+// float64 (the chain rule of the kernel passes).
+struct FlatGeom {
+  double tmp, btr;
+  float d, bt, rpow;
+};
+
+__device__ __forceinline__ FlatGeom flat_geom(double r0, double dr, bool half) {
 #pragma clang fp contract(off)
   const double ar = 6370.0;
-  float thk = (m == L - 1) ? 1.0f : (float)l.x;
-  s.dr = s.dr + (double)thk;
-  double r1 = ar - s.dr;
-  double z0 = ar * log(ar / s.r0);
-  double z1 = ar * log(ar / r1);
-  float d = (m == L - 1) ? 0.0f : (float)(z1 - z0);
-  double tmp = (ar + ar) / (s.r0 + r1);
-  float a = (float)((double)(float)l.y * tmp);
-  float b = (float)((double)(float)l.z * tmp);
-  float bt = (float)tmp;
-  float rho = (float)l.w * powf(bt, -2.275f);
-  s.r0 = r1;
-  if (btp_m) *btp_m = bt;
+  const double r1 = ar - dr;
+  const double z0 = ar * log(ar / r0);
+  const double z1 = ar * log(ar / r1);
+  FlatGeom g;
+  g.d = half ? 0.0f : (float)(z1 - z0);
+  g.tmp = (ar + ar) / (r0 + r1);
+  g.bt = (float)g.tmp;
+  g.rpow = powf(g.bt, -2.275f);
+  g.btr = pow((double)g.bt, -2.275);
+  return g;
+}
+
+// flat_push with the geometry of the layer given.
+__device__ double4 flat_push_g(FlatState& s, double4 l, int m, bool half, const FlatGeom& g,
+                               double* btp_m, double* btr_m) {
+#pragma clang fp contract(off)
+  const float d = g.d;
+  float a = (float)((double)(float)l.y * g.tmp);
+  float b = (float)((double)(float)l.z * g.tmp);
+  float rho = (float)l.w * g.rpow;
+  if (btp_m) *btp_m = g.bt;
+  if (btr_m) *btr_m = g.btr;
   const double4 o = make_double4(d, a, b, rho);
   if (m == 0) {
     s.amn = a;
@@ -158,9 +177,19 @@ __device__ double4 flat_push(FlatState& s, double4 l, int m, int L, double* btp_
   }
   if (b > s.betmx) s.betmx = b;
   bool ok = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(o.w) && o.y > 0.0 &&
-            o.z > 0.0 && o.w > 0.0 && (o.x > 0.0 || m == L - 1);
+            o.z > 0.0 && o.w > 0.0 && (o.x > 0.0 || half);
   if (!ok) s.bad = 1;
   return o;
+}
+
+__device__ double4 flat_push(FlatState& s, double4 l, int m, int L, double* btp_m,
+                             double* btr_m) {
+#pragma clang fp contract(off)
+  const float thk = (m == L - 1) ? 1.0f : (float)l.x;
+  s.dr = s.dr + (double)thk;
+  const FlatGeom g = flat_geom(s.r0, s.dr, m == L - 1);
+  s.r0 = 6370.0 - s.dr;
+  return flat_push_g(s, l, m, m == L - 1, g, btp_m, btr_m);
 }
 
 __device__ void flat_finish(const FlatState& s, double* cc_out, float* betmx_out, int* bad) {
@@ -176,8 +205,8 @@ __device__ void flat_finish(const FlatState& s, double* cc_out, float* betmx_out
 
 // The same flattening carried out in float64 for float64 models (the
 // differentiable path); the start velocity follows the float32 rule.
-__device__ void flatten_column_f64(double4* lay, double* btp_out, int L, double* cc_out,
-                                   float* betmx_out, int* bad) {
+__device__ void flatten_column_f64(double4* lay, double* btp_out, double* btr_out, int L,
+                                   double* cc_out, float* betmx_out, int* bad) {
   const double ar = 6370.0;
   double dr = 0.0, r0 = ar;
   for (int m = 0; m < L; m++) {
@@ -188,18 +217,53 @@ __device__ void flatten_column_f64(double4* lay, double* btp_out, int L, double*
     double d = (m == L - 1) ? 0.0 : ar * log(ar / r1) - ar * log(ar / r0);
     double tmp = (ar + ar) / (r0 + r1);
     r0 = r1;
-    lay[m] = make_double4(d, l.y * tmp, l.z * tmp, l.w * pow(tmp, -2.275));
+    const double tr = pow(tmp, -2.275);
+    lay[m] = make_double4(d, l.y * tmp, l.z * tmp, l.w * tr);
     if (btp_out) btp_out[m] = tmp;
+    if (btr_out) btr_out[m] = tr;
   }
   start_velocity(lay, L, cc_out, betmx_out, bad);
 }
 
+static_assert(sizeof(FlatGeom) == PREP_GEOM_BYTES, "prep geometry record");
+
+// Flattening geometry of the refined layers of a node model, which every
+// column shares: thread m sums the thicknesses down to layer m in the order
+// of k_prep_nodes' layer loop (same sublayer rule, same float64 sums), so the
+// values are those flat_push computes per column.
+__global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk0, int L,
+                            FlatGeom* __restrict__ geo) {
+#pragma clang fp contract(off)
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= L) return;
+  double dr = 0.0, dr0 = 0.0;
+  int k = 0;
+  for (int i = 0; i < nz - 1 && k <= m; i++) {
+    float thk = depz[i + 1] - depz[i];
+    float minthk = thk / minthk0;
+    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
+    float newthk = thk / (float)ns;
+    for (int j = 1; j <= ns && k < L - 1 && k <= m; j++, k++) {
+      dr0 = dr;
+      dr = dr + (double)newthk;
+    }
+  }
+  const bool half = (m == L - 1);
+  if (half) {  // the half-space is flattened with a unit thickness
+    dr0 = dr;
+    dr = dr + 1.0;
+  }
+  geo[m] = flat_geom(6370.0 - dr0, dr, half);
+}
+
 // Node models: vel is (ncol, nz) column-fastest (Fortran vel(nx,ny,nz)).
-// vp/rho optional (both or neither); otherwise Brocher from vs.
+// vp/rho optional (both or neither); otherwise Brocher from vs.  geo holds
+// k_prep_geom's records of the same depz, minthk0 and L.
 __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
                              const float* __restrict__ vpin, const float* __restrict__ rhoin,
                              const float* __restrict__ depz, float minthk0, int L,
-                             double4* __restrict__ layers, double* __restrict__ btp,
+                             const FlatGeom* __restrict__ geo, double4* __restrict__ layers,
+                             double* __restrict__ btp, double* __restrict__ btr,
                              double* __restrict__ cc, float* __restrict__ betmx,
                              int* __restrict__ bad_out) {
 #pragma clang fp contract(off)
@@ -207,6 +271,7 @@ __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
   if (col >= ncol) return;
   double4* lay = layers + (size_t)col * L;
   double* bt = btp + (size_t)col * L;
+  double* br = btr + (size_t)col * L;
   FlatState fs;
   int k = 0;
   float vs_i = vel[col], vp_i, rho_i;
@@ -237,14 +302,16 @@ __global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
       float rvp = vp_i + w * (vp_n - vp_i) / den;
       float rvs = vs_i + w * (vs_n - vs_i) / den;
       float rrho = rho_i + w * (rho_n - rho_i) / den;
-      lay[k] = flat_push(fs, make_double4(newthk, rvp, rvs, rrho), k, L, bt + k);
+      lay[k] = flat_push_g(fs, make_double4(newthk, rvp, rvs, rrho), k, false, geo[k], bt + k,
+                           br + k);
     }
     vs_i = vs_n;
     vp_i = vp_n;
     rho_i = rho_n;
   }
   int bad = (k != L - 1);
-  lay[L - 1] = flat_push(fs, make_double4(0.0, vp_i, vs_i, rho_i), L - 1, L, bt + L - 1);
+  lay[L - 1] = flat_push_g(fs, make_double4(0.0, vp_i, vs_i, rho_i), L - 1, true, geo[L - 1],
+                           bt + L - 1, br + L - 1);
   flat_finish(fs, cc + col, betmx + col, &bad);
   bad_out[col] = bad;
 }
@@ -257,8 +324,8 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
                               const double* __restrict__ thk64, const double* __restrict__ vp64,
                               const double* __restrict__ vs64, const double* __restrict__ rho64,
                               double4* __restrict__ layers, double* __restrict__ btp,
-                              double* __restrict__ cc, float* __restrict__ betmx,
-                              int* __restrict__ bad_out) {
+                              double* __restrict__ btr, double* __restrict__ cc,
+                              float* __restrict__ betmx, int* __restrict__ bad_out) {
   int col = blockIdx.x * blockDim.x + threadIdx.x;
   if (col >= ncol) return;
   int L = Lc[col];
@@ -268,12 +335,12 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
   if (thk64) {
     for (int m = 0; m < L; m++)
       lay[m] = make_double4(thk64[o + m], vp64[o + m], vs64[o + m], rho64[o + m]);
-    if (L >= 1) flatten_column_f64(lay, btp + o, L, cc + col, betmx + col, &bad);
+    if (L >= 1) flatten_column_f64(lay, btp + o, btr + o, L, cc + col, betmx + col, &bad);
   } else if (L >= 1) {
     FlatState fs;
     for (int m = 0; m < L; m++)
       lay[m] = flat_push(fs, make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]), m, L,
-                         btp + o + m);
+                         btp + o + m, btr + o + m);
     flat_finish(fs, cc + col, betmx + col, &bad);
   }
   bad_out[col] = bad;
@@ -901,6 +968,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   }
   const double4* lay = B.layers + loff;
   const double* btp = B.btp + loff;
+  const double* btr = B.btr + loff;
   double omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
   if (omega < 1.0e-4) omega = 1.0e-4;
   const double wv = omega / c;
@@ -1009,7 +1077,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
     // rho_flat = rho*btp^-2.275
     double bt = (double)btp[m];
-    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * pow(bt, -2.275);
+    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * btr[m];
     if (!O.node_mode) {
       int64_t o = out_index(m);
       if (O.dvp) O.dvp[o] = pa;
@@ -1097,6 +1165,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
     col_geom(B, col, &L, &loff, &K, &toff);
     const double4* lay = B.layers + loff;
     const double* btp = B.btp + loff;
+    const double* btr = B.btr + loff;
     // a column without periods still writes its (zero) gradients
     for (int k0 = 0; k0 == 0 || k0 < K; k0 += 64) {
       const int k = k0 + lane;
@@ -1237,7 +1306,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
           R1 = ttm - ((m < L - 1) ? part[3] * (ar / r1) : 0.0);
           part[0] *= bt;
           part[1] *= bt;
-          part[2] *= pow(bt, -2.275);
+          part[2] *= btr[m];
         }
         const double sa = wave_sum(part[0]), sb = wave_sum(part[1]), sr = wave_sum(part[2]);
         double s0 = 0.0, s1 = 0.0;
@@ -1383,24 +1452,28 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
 }
 
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, double4* layers, double* btp,
-                             double* cc, float* betmx, int* bad, hipStream_t st) {
+                             const float* depz, float minthk0, int L, void* geo, double4* layers,
+                             double* btp, double* btr, double* cc, float* betmx, int* bad,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_prep_geom, dim3((L + 63) / 64), dim3(64), 0, st, nz, depz, minthk0, L,
+                     (FlatGeom*)geo);
+  if (hipError_t e = hipGetLastError()) return e;
   int block = 128;
   int grid = (ncol + block - 1) / block;
   hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(block), 0, st, ncol, nz, vel, vp, rho, depz,
-                     minthk0, L, layers, btp, cc, betmx, bad);
+                     minthk0, L, (const FlatGeom*)geo, layers, btp, btr, cc, betmx, bad);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
                               const float* vp, const float* vs, const float* rho,
                               const double* thk64, const double* vp64, const double* vs64,
-                              const double* rho64, double4* layers, double* btp, double* cc,
-                              float* betmx, int* bad, hipStream_t st) {
+                              const double* rho64, double4* layers, double* btp, double* btr,
+                              double* cc, float* betmx, int* bad, hipStream_t st) {
   int block = 128;
   int grid = (ncol + block - 1) / block;
   hipLaunchKernelGGL(k_prep_layers, dim3(grid), dim3(block), 0, st, ncol, Lc, loff, thk, vp, vs,
-                     rho, thk64, vp64, vs64, rho64, layers, btp, cc, betmx, bad);
+                     rho, thk64, vp64, vs64, rho64, layers, btp, btr, cc, betmx, bad);
   return hipGetLastError();
 }
 
