@@ -105,6 +105,16 @@ struct Pool {
   float geo_minthk = 0.0f;
   int geo_L = 0;
   bool geo_valid = false;
+  // what the S_NSUB and S_GEO slots hold: the sublayer counts, and the shared
+  // layer records (k_prep_geom) of the node geometry (depz values, minthk, L)
+  // last computed; geo_table is set once those records are launched.  A new
+  // geometry gets new buffers (the old ones are retired, not overwritten), so a
+  // graph captured with the old geometry keeps reading its own.
+  std::vector<int> nsub_host;
+  std::vector<float> tab_depz;
+  float tab_minthk = 0.0f;
+  int tab_L = 0;
+  bool geo_table = false;
   std::vector<void*> retired;  // buffers replaced by growth (see pool_get)
 };
 
@@ -133,6 +143,18 @@ int get_pool(Pool** out) {
 // still be read by work in flight or by a captured HIP graph, so it is retired
 // rather than freed: retired buffers live until the process ends, at most about
 // twice the final size of each slot.
+// A new buffer for slot s (the previous one is retired).
+int pool_new(Pool* P, Slot s, size_t bytes, void** out) {
+  if (bytes == 0) bytes = 8;
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, bytes));
+  if (P->p[s]) P->retired.push_back(P->p[s]);
+  P->p[s] = q;
+  P->n[s] = bytes;
+  *out = q;
+  return 0;
+}
+
 int pool_get(Pool* P, Slot s, size_t bytes, void** out) {
   if (bytes == 0) bytes = 8;
   if (P->n[s] < bytes) {
@@ -258,9 +280,22 @@ int geometry(Pool* P, const dazi_dk_desc* d, const float* depz_dev, const float*
   if (Lc < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
   if (Lc > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", Lc);
   P->geo_valid = false;
-  if (int rc = pool_get(P, S_NSUB, nsub.size() * 4, nsubd)) return rc;
-  HIPCHK(hipMemcpyAsync(*nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
-  HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
+  if (nsub == P->nsub_host && P->p[S_NSUB]) {
+    *nsubd = P->p[S_NSUB];
+  } else {
+    P->nsub_host.clear();
+    if (int rc = pool_new(P, S_NSUB, nsub.size() * 4, nsubd)) return rc;
+    HIPCHK(hipMemcpyAsync(*nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
+    P->nsub_host = nsub;
+  }
+  std::vector<float> dzv(depz_host, depz_host + d->nz);
+  if (!(dzv == P->tab_depz && d->minthk0 == P->tab_minthk && Lc == P->tab_L)) {
+    P->tab_depz = dzv;
+    P->tab_minthk = d->minthk0;
+    P->tab_L = Lc;
+    P->geo_table = false;
+  }
   if (from_dev) {
     P->geo_depz = depz_dev;
     P->geo_nz = d->nz;
@@ -302,8 +337,13 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 32, &layers)) return rc;
   if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 16, &btp)) return rc;
   double* btr = (double*)btp + (size_t)ncol * L;
-  void* geo;
-  if (int rc = pool_get(P, S_GEO, (size_t)L * dk::PREP_GEOM_BYTES, &geo)) return rc;
+  // the shared layer records of the node geometry, built in a buffer of its
+  // own when the geometry changes
+  void* geo = P->p[S_GEO];
+  if (!P->geo_table || !geo) {
+    P->geo_table = false;
+    if (int rc = pool_new(P, S_GEO, (size_t)L * dk::PREP_GEOM_BYTES, &geo)) return rc;
+  }
   if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)ncol * 4, &bad)) return rc;
@@ -348,10 +388,13 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
                               (dk::TiLayerDev*)til, (int*)tiq, P->side));
     HIPCHK(hipEventRecord(P->fk[2], P->side));
   }
+  if (!P->geo_table) {
+    HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, st));
+    P->geo_table = true;
+  }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                               d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                               geo, (double4*)layers, (double*)btp, btr, (double*)cc,
-                               (float*)betmx, (int*)bad, st));
+                               d->vp_rho_mode == 1 ? rho : nullptr, L, geo, (double4*)layers,
+                               (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = ncol;
@@ -933,6 +976,7 @@ int32_t dazi_dk_reset_geometry(void) {
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
   P->geo_valid = false;
+  P->geo_table = false;
   return 0;
 }
 

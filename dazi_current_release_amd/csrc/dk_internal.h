@@ -73,13 +73,15 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
                      double* scratch, int nthr, int* queue, hipStream_t st);
 
-// launch_prep_nodes first writes the L shared flattening records of the node
-// geometry to geo (PREP_GEOM_BYTES each), then the columns' layers.
-constexpr size_t PREP_GEOM_BYTES = 32;
+// launch_prep_geom writes the L flattening and node-mapping records that
+// every column of a node geometry (depz, minthk0) shares to geo
+// (PREP_GEOM_BYTES each); launch_prep_nodes reads them for the columns' layers.
+constexpr size_t PREP_GEOM_BYTES = 48;
+hipError_t launch_prep_geom(int nz, const float* depz, float minthk0, int L, void* geo,
+                            hipStream_t st);
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, void* geo, double4* layers,
-                             double* btp, double* btr, double* cc, float* betmx, int* bad,
-                             hipStream_t st);
+                             int L, const void* geo, double4* layers, double* btp, double* btr,
+                             double* cc, float* betmx, int* bad, hipStream_t st);
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
                               const float* vp, const float* vs, const float* rho,
                               const double* thk64, const double* vp64, const double* vs64,

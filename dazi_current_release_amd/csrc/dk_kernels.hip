@@ -131,6 +131,11 @@ struct FlatState {
 struct FlatGeom {
   double tmp, btr;
   float d, bt, rpow;
+  // node models (k_prep_geom): the layer lies in node interval `node` as
+  // sublayer `sub` (1-based) of `ns`; the half-space takes node `node` itself
+  // (sub = ns = 0); node = -1 marks a layer the sublayer rule never forms.
+  // bad (half-space record only): the rule formed other than L - 1 layers.
+  int node, sub, ns, bad, pad;
 };
 
 __device__ __forceinline__ FlatGeom flat_geom(double r0, double dr, bool half) {
@@ -148,17 +153,30 @@ __device__ __forceinline__ FlatGeom flat_geom(double r0, double dr, bool half) {
   return g;
 }
 
-// flat_push with the geometry of the layer given.
-__device__ double4 flat_push_g(FlatState& s, double4 l, int m, bool half, const FlatGeom& g,
-                               double* btp_m, double* btr_m) {
+// One layer (thk, vp, vs, rho as float32 values) flattened with its geometry
+// into (d, a, b, rho) rounded to float32 as the reference stores them; *ok = 0
+// for a non-finite or non-positive value (the half-space may have d = 0).
+__device__ __forceinline__ double4 flat_layer(double4 l, bool half, const FlatGeom& g, bool* ok) {
 #pragma clang fp contract(off)
   const float d = g.d;
   float a = (float)((double)(float)l.y * g.tmp);
   float b = (float)((double)(float)l.z * g.tmp);
   float rho = (float)l.w * g.rpow;
+  const double4 o = make_double4(d, a, b, rho);
+  *ok = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(o.w) && o.y > 0.0 &&
+        o.z > 0.0 && o.w > 0.0 && (o.x > 0.0 || half);
+  return o;
+}
+
+// flat_push with the geometry of the layer given.
+__device__ double4 flat_push_g(FlatState& s, double4 l, int m, bool half, const FlatGeom& g,
+                               double* btp_m, double* btr_m) {
+#pragma clang fp contract(off)
+  bool ok;
+  const double4 o = flat_layer(l, half, g, &ok);
+  const float a = (float)o.y, b = (float)o.z;
   if (btp_m) *btp_m = g.bt;
   if (btr_m) *btr_m = g.btr;
-  const double4 o = make_double4(d, a, b, rho);
   if (m == 0) {
     s.amn = a;
     s.bmn = b;
@@ -176,8 +194,6 @@ __device__ double4 flat_push_g(FlatState& s, double4 l, int m, bool half, const 
     s.jsol = 0;
   }
   if (b > s.betmx) s.betmx = b;
-  bool ok = isfinite(o.x) && isfinite(o.y) && isfinite(o.z) && isfinite(o.w) && o.y > 0.0 &&
-            o.z > 0.0 && o.w > 0.0 && (o.x > 0.0 || half);
   if (!ok) s.bad = 1;
   return o;
 }
@@ -227,17 +243,18 @@ __device__ void flatten_column_f64(double4* lay, double* btp_out, double* btr_ou
 
 static_assert(sizeof(FlatGeom) == PREP_GEOM_BYTES, "prep geometry record");
 
-// Flattening geometry of the refined layers of a node model, which every
-// column shares: thread m sums the thicknesses down to layer m in the order
-// of k_prep_nodes' layer loop (same sublayer rule, same float64 sums), so the
-// values are those flat_push computes per column.
+// Flattening geometry and node mapping of the refined layers of a node model,
+// which every column shares (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196,
+// then sphere): thread m runs the sublayer rule down to layer m and sums the
+// thicknesses in the order a per-column pass would, so every value is the one
+// flat_push computes.
 __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk0, int L,
                             FlatGeom* __restrict__ geo) {
 #pragma clang fp contract(off)
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= L) return;
   double dr = 0.0, dr0 = 0.0;
-  int k = 0;
+  int k = 0, node = -1, sub = 0, nsm = 0;
   for (int i = 0; i < nz - 1 && k <= m; i++) {
     float thk = depz[i + 1] - depz[i];
     float minthk = thk / minthk0;
@@ -246,6 +263,11 @@ __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk
     for (int j = 1; j <= ns && k < L - 1 && k <= m; j++, k++) {
       dr0 = dr;
       dr = dr + (double)newthk;
+      if (k == m) {
+        node = i;
+        sub = j;
+        nsm = ns;
+      }
     }
   }
   const bool half = (m == L - 1);
@@ -253,67 +275,116 @@ __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk
     dr0 = dr;
     dr = dr + 1.0;
   }
-  geo[m] = flat_geom(6370.0 - dr0, dr, half);
+  FlatGeom g = flat_geom(6370.0 - dr0, dr, half);
+  g.node = half ? nz - 1 : node;
+  g.sub = sub;
+  g.ns = nsm;
+  g.bad = half ? (k != L - 1) : 0;
+  g.pad = 0;
+  geo[m] = g;
 }
 
 // Node models: vel is (ncol, nz) column-fastest (Fortran vel(nx,ny,nz)).
 // vp/rho optional (both or neither); otherwise Brocher from vs.  geo holds
-// k_prep_geom's records of the same depz, minthk0 and L.
-__global__ void k_prep_nodes(int ncol, int nz, const float* __restrict__ vel,
-                             const float* __restrict__ vpin, const float* __restrict__ rhoin,
-                             const float* __restrict__ depz, float minthk0, int L,
-                             const FlatGeom* __restrict__ geo, double4* __restrict__ layers,
-                             double* __restrict__ btp, double* __restrict__ btr,
-                             double* __restrict__ cc, float* __restrict__ betmx,
-                             int* __restrict__ bad_out) {
+// k_prep_geom's records of the same depz, minthk0 and L.  One wavefront per
+// column, lane = layer: each lane interpolates its layer between its two
+// nodes and flattens it; the start-velocity layer (the first layer of least
+// b, or a where b <= 0.01: the layer the sequential rule of flat_push keeps),
+// betmx and the bad flags are wave reductions.
+__global__ void __launch_bounds__(256) k_prep_nodes(
+    int ncol, int nz, const float* __restrict__ vel, const float* __restrict__ vpin,
+    const float* __restrict__ rhoin, int L, const FlatGeom* __restrict__ geo,
+    double4* __restrict__ layers, double* __restrict__ btp, double* __restrict__ btr,
+    double* __restrict__ cc, float* __restrict__ betmx, int* __restrict__ bad_out) {
 #pragma clang fp contract(off)
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  double4* lay = layers + (size_t)col * L;
-  double* bt = btp + (size_t)col * L;
-  double* br = btr + (size_t)col * L;
-  FlatState fs;
-  int k = 0;
-  float vs_i = vel[col], vp_i, rho_i;
-  if (vpin) {
-    vp_i = vpin[col];
-    rho_i = rhoin[col];
-  } else {
-    vp_i = brocher_vp(vs_i);
-    rho_i = brocher_rho(vp_i);
-  }
-  for (int i = 0; i < nz - 1; i++) {
-    size_t o = (size_t)(i + 1) * ncol + col;
-    float vs_n = vel[o], vp_n, rho_n;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  auto node_vals = [&](int col, int i, float* vs, float* vp, float* rho) {
+    const size_t o = (size_t)i * ncol + col;
+    *vs = vel[o];
     if (vpin) {
-      vp_n = vpin[o];
-      rho_n = rhoin[o];
+      *vp = vpin[o];
+      *rho = rhoin[o];
     } else {
-      vp_n = brocher_vp(vs_n);
-      rho_n = brocher_rho(vp_n);
+      *vp = brocher_vp(*vs);
+      *rho = brocher_rho(*vp);
     }
-    float thk = depz[i + 1] - depz[i];
-    float minthk = thk / minthk0;
-    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
-    float newthk = thk / (float)ns;
-    float den = (float)(2 * ns);
-    for (int j = 1; j <= ns && k < L - 1; j++, k++) {
-      float w = (float)(2 * j - 1);
-      float rvp = vp_i + w * (vp_n - vp_i) / den;
-      float rvs = vs_i + w * (vs_n - vs_i) / den;
-      float rrho = rho_i + w * (rho_n - rho_i) / den;
-      lay[k] = flat_push_g(fs, make_double4(newthk, rvp, rvs, rrho), k, false, geo[k], bt + k,
-                           br + k);
+  };
+  for (int col = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < ncol; col += nw) {
+    double4* lay = layers + (size_t)col * L;
+    float kmin = __builtin_inff(), amn = 0.0f, bmn = 0.0f, bmx = -1.e20f;
+    bool have = false, bad = false;
+    for (int m0 = 0; m0 < L; m0 += 64) {
+      const int m = m0 + lane;
+      bool cand = false, isbad = false;
+      float key = __builtin_inff(), a = 0.0f, b = -1.e20f;
+      if (m < L) {
+        const FlatGeom g = geo[m];
+        if (g.node >= 0) {
+          const bool half = (m == L - 1);
+          float vs, vp, rho;
+          node_vals(col, g.node, &vs, &vp, &rho);
+          if (!half) {
+            float vs_n, vp_n, rho_n;
+            node_vals(col, g.node + 1, &vs_n, &vp_n, &rho_n);
+            const float den = (float)(2 * g.ns);
+            const float w = (float)(2 * g.sub - 1);
+            vp = vp + w * (vp_n - vp) / den;
+            vs = vs + w * (vs_n - vs) / den;
+            rho = rho + w * (rho_n - rho) / den;
+          }
+          bool ok;
+          const double4 o = flat_layer(make_double4(0.0, vp, vs, rho), half, g, &ok);
+          lay[m] = o;
+          btp[(size_t)col * L + m] = g.bt;
+          btr[(size_t)col * L + m] = g.btr;
+          a = (float)o.y;
+          b = (float)o.z;
+          isbad = !ok || (m == 0 && o.z <= 0.0) || g.bad;
+          if (b > 0.01f) {
+            key = b;
+            cand = true;
+          } else if (b <= 0.01f) {
+            key = a;
+            cand = true;
+          }
+          cand = cand && (key < 1.e20f);
+          if (!cand) key = __builtin_inff();
+        }
+      }
+      if (m0 == 0) {  // the rule starts from layer 0's values
+        amn = __shfl(a, 0, 64);
+        bmn = __shfl(b, 0, 64);
+      }
+      float kw = key, bw = b;
+#pragma unroll
+      for (int sh = 1; sh < 64; sh <<= 1) {
+        kw = fminf(kw, __shfl_xor(kw, sh, 64));
+        bw = fmaxf(bw, __shfl_xor(bw, sh, 64));
+      }
+      bmx = fmaxf(bmx, bw);
+      bad = bad || (__ballot(isbad) != 0ull);
+      const uint64_t at = __ballot(cand && key == kw);
+      if (at != 0ull && (!have || kw < kmin)) {
+        const int j = __ffsll((unsigned long long)at) - 1;
+        kmin = kw;
+        amn = __shfl(a, j, 64);
+        bmn = __shfl(b, j, 64);
+        have = true;
+      }
     }
-    vs_i = vs_n;
-    vp_i = vp_n;
-    rho_i = rho_n;
+    // flat_finish: jsol = 0 (a fluid start layer) is outside this stage's model class
+    const bool jsol0 = have && !(bmn > 0.01f);
+    float cc1 = gtsolh_f32(amn, bmn);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    if (!(cc1 > 0.0f) || !isfinite(cc1)) bad = true;
+    if (lane == 0) {
+      cc[col] = (double)cc1;
+      betmx[col] = bmx;
+      bad_out[col] = (bad || jsol0) ? 1 : 0;
+    }
   }
-  int bad = (k != L - 1);
-  lay[L - 1] = flat_push_g(fs, make_double4(0.0, vp_i, vs_i, rho_i), L - 1, true, geo[L - 1],
-                           bt + L - 1, br + L - 1);
-  flat_finish(fs, cc + col, betmx + col, &bad);
-  bad_out[col] = bad;
 }
 
 // Layered models (ragged): thk/vp/vs/rho concatenated at loff[col], float32
@@ -1451,17 +1522,20 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
   return nb;
 }
 
-hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             const float* depz, float minthk0, int L, void* geo, double4* layers,
-                             double* btp, double* btr, double* cc, float* betmx, int* bad,
-                             hipStream_t st) {
+hipError_t launch_prep_geom(int nz, const float* depz, float minthk0, int L, void* geo,
+                            hipStream_t st) {
   hipLaunchKernelGGL(k_prep_geom, dim3((L + 63) / 64), dim3(64), 0, st, nz, depz, minthk0, L,
                      (FlatGeom*)geo);
-  if (hipError_t e = hipGetLastError()) return e;
-  int block = 128;
-  int grid = (ncol + block - 1) / block;
-  hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(block), 0, st, ncol, nz, vel, vp, rho, depz,
-                     minthk0, L, (const FlatGeom*)geo, layers, btp, btr, cc, betmx, bad);
+  return hipGetLastError();
+}
+
+hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
+                             int L, const void* geo, double4* layers, double* btp, double* btr,
+                             double* cc, float* betmx, int* bad, hipStream_t st) {
+  // four columns per 256-thread workgroup (one per wavefront)
+  const int grid = (ncol + 3) / 4;
+  hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(256), 0, st, ncol, nz, vel, vp, rho, L,
+                     (const FlatGeom*)geo, layers, btp, btr, cc, betmx, bad);
   return hipGetLastError();
 }
 

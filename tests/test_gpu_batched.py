@@ -84,7 +84,14 @@ def test_graph_replay_stage_iso_and_ti(dk):
         torch.cuda.synchronize()
         for k in eager:
             assert torch.equal(out[k], eager[k]), (rep, k)
-        dk.stage_dev(vel[:, :100].contiguous(), depz, per, W["minthk"], iso=True, ti=False)
+        # eager work between replays; from the second replay on with another node
+        # geometry (other depths and sublayer count), which must not touch the
+        # geometry the graph was captured with
+        if rep == 0:
+            dk.stage_dev(vel[:, :100].contiguous(), depz, per, W["minthk"], iso=True, ti=False)
+        else:
+            dk.stage_dev(vel[:, :100].contiguous(), depz * (1.0 + 0.25 * rep), per,
+                         W["minthk"] + rep, iso=True, ti=True)
         torch.cuda.synchronize()
 
 
@@ -104,6 +111,11 @@ def test_geometry_cache_follows_depz(dk):
     d2 = d1 * 1.5
     b = dk.stage_dev(vel, d2, per, 2.0, iso=False, ti=False)["pv"].clone()
     assert not torch.equal(a, b)
+    # back to the first geometry, then another sublayer count on it
+    assert torch.equal(a, dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"])
+    e = dk.stage_dev(vel, d1, per, 3.0, iso=False, ti=False)["pv"].clone()
+    assert not torch.equal(a, e)
+    assert torch.equal(a, dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"])
     d1.mul_(1.5)
     dk.lib().dazi_dk_reset_geometry()
     c = dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"].clone()
