@@ -364,8 +364,8 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * kern_nthr * 8, &tape))
       return rc;
   }
-  // the TI model prep needs only the inputs: it runs on the side stream from
-  // the start of the call
+  // the TI model prep needs only the inputs and the shared layer records: it
+  // runs on the side stream from the start of the call
   void *til = nullptr, *tiscr = nullptr, *tiq = nullptr;
   int ti_nthr = 0;
   if (lsen) {
@@ -380,17 +380,17 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   P->have_events = g_timing;
   rec(P, 0, st);
-  if (lsen) {
-    HIPCHK(hipEventRecord(P->fk[0], st));
-    HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
-    HIPCHK(dk::launch_prep_ti(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                              d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, d->minthk0, L,
-                              (dk::TiLayerDev*)til, (int*)tiq, P->side));
-    HIPCHK(hipEventRecord(P->fk[2], P->side));
-  }
   if (!P->geo_table) {
     HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, st));
     P->geo_table = true;
+  }
+  if (lsen) {
+    HIPCHK(hipEventRecord(P->fk[0], st));
+    HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
+    HIPCHK(dk::launch_prep_ti(ncol, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                              d->vp_rho_mode == 1 ? rho : nullptr, L, geo,
+                              (dk::TiLayerDev*)til, (int*)tiq, P->side));
+    HIPCHK(hipEventRecord(P->fk[2], P->side));
   }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
                                d->vp_rho_mode == 1 ? rho : nullptr, L, geo, (double4*)layers,

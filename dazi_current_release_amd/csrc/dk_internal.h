@@ -63,12 +63,12 @@ struct TiLayerDev {
 constexpr int TI_SCRATCH = 9;
 constexpr int TI_SCRATCH_CX = 14;
 
-// k_prep_ti also clears queue[0]; launch_ti runs k_ti, which appends the solves
+// k_prep_ti (node model and the shared records of launch_prep_geom) also
+// clears queue[0]; launch_ti runs k_ti, which appends the solves
 // with a complex-conjugate eigenvalue pair to queue[1..], then k_ti_cx over
 // them (queue holds 1 + ncol * K ints).
-hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                          const float* depz, float minthk0, int L, TiLayerDev* out, int* queue,
-                          hipStream_t st);
+hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
+                          const void* geo, TiLayerDev* out, int* queue, hipStream_t st);
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
                      double* scratch, int nthr, int* queue, hipStream_t st);
@@ -76,7 +76,7 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
 // launch_prep_geom writes the L flattening and node-mapping records that
 // every column of a node geometry (depz, minthk0) shares to geo
 // (PREP_GEOM_BYTES each); launch_prep_nodes reads them for the columns' layers.
-constexpr size_t PREP_GEOM_BYTES = 48;
+constexpr size_t PREP_GEOM_BYTES = 80;
 hipError_t launch_prep_geom(int nz, const float* depz, float minthk0, int L, void* geo,
                             hipStream_t st);
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,

@@ -32,6 +32,7 @@
 
 #include "dk_brocher.h"
 #include "dk_physics.h"
+#include "dk_geom.h"
 #include "dk_internal.h"
 
 namespace dk {
@@ -123,36 +124,6 @@ struct FlatState {
   int bad = 0;
 };
 
-// The part of one layer's flattening that depends only on the thicknesses:
-// radii r0 (top) and r1 = 6370 - dr (bottom, dr the thickness sum down to the
-// layer's bottom), the flattened thickness, the velocity factor tmp and its
-// float32 value bt, rho's factor bt^-2.275 in float32 (the flattening) and in
-// float64 (the chain rule of the kernel passes).
-struct FlatGeom {
-  double tmp, btr;
-  float d, bt, rpow;
-  // node models (k_prep_geom): the layer lies in node interval `node` as
-  // sublayer `sub` (1-based) of `ns`; the half-space takes node `node` itself
-  // (sub = ns = 0); node = -1 marks a layer the sublayer rule never forms.
-  // bad (half-space record only): the rule formed other than L - 1 layers.
-  int node, sub, ns, bad, pad;
-};
-
-__device__ __forceinline__ FlatGeom flat_geom(double r0, double dr, bool half) {
-#pragma clang fp contract(off)
-  const double ar = 6370.0;
-  const double r1 = ar - dr;
-  const double z0 = ar * log(ar / r0);
-  const double z1 = ar * log(ar / r1);
-  FlatGeom g;
-  g.d = half ? 0.0f : (float)(z1 - z0);
-  g.tmp = (ar + ar) / (r0 + r1);
-  g.bt = (float)g.tmp;
-  g.rpow = powf(g.bt, -2.275f);
-  g.btr = pow((double)g.bt, -2.275);
-  return g;
-}
-
 // One layer (thk, vp, vs, rho as float32 values) flattened with its geometry
 // into (d, a, b, rho) rounded to float32 as the reference stores them; *ok = 0
 // for a non-finite or non-positive value (the half-space may have d = 0).
@@ -241,13 +212,11 @@ __device__ void flatten_column_f64(double4* lay, double* btp_out, double* btr_ou
   start_velocity(lay, L, cc_out, betmx_out, bad);
 }
 
-static_assert(sizeof(FlatGeom) == PREP_GEOM_BYTES, "prep geometry record");
-
 // Flattening geometry and node mapping of the refined layers of a node model,
 // which every column shares (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196,
-// then sphere): thread m runs the sublayer rule down to layer m and sums the
-// thicknesses in the order a per-column pass would, so every value is the one
-// flat_push computes.
+// then sphere and the TI factors): thread m runs the sublayer rule down to
+// layer m and sums the thicknesses in the order a per-column pass would, so
+// every value is the one flat_push (and ti_geom's recurrence) computes.
 __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk0, int L,
                             FlatGeom* __restrict__ geo) {
 #pragma clang fp contract(off)
@@ -280,7 +249,22 @@ __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk
   g.sub = sub;
   g.ns = nsm;
   g.bad = half ? (k != L - 1) : 0;
-  g.pad = 0;
+  // the TI recurrences run through every layer above m (the sphericity chain
+  // needs each one's flattened thickness): the same sequence a per-column pass
+  // of the TI prep makes, once per geometry
+  TiFlat ts;
+  k = 0;
+  for (int i = 0; i < nz - 1 && k <= m; i++) {
+    float thk = depz[i + 1] - depz[i];
+    float minthk = thk / minthk0;
+    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
+    float newthk = thk / (float)ns;
+    for (int j = 1; j <= ns && k < L - 1 && k <= m; j++, k++) {
+      const TiGeom t = ti_geom(ts, newthk, false);
+      if (k == m) g.ti = t;
+    }
+  }
+  if (half) g.ti = ti_geom(ts, 0.0f, true);
   geo[m] = g;
 }
 

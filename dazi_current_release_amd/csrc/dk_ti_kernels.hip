@@ -15,6 +15,7 @@
 //               register budget.
 #include <hip/hip_runtime.h>
 
+#include "dk_geom.h"
 #include "dk_internal.h"
 #include "dk_ti.h"
 
@@ -36,17 +37,14 @@ __device__ float brocher_rho_ti(float p) {
 
 // One refined layer of the TI model: moduli (depthkernelTI.f90:124-138), TI
 // flattening with radius 6371 (sphere_tdisp96, tregn96.f:774-841) and the
-// sphericity factor with radius 6370 (bldsph, tregn96.f:1301-1383), pushed top
-// to bottom so that each layer is finished as soon as it is formed.
-struct TiFlat {
-  double r0 = 6371.0;   // flattening recurrence
-  double rv = 6370.0;   // sphericity recurrence
-};
-
-__device__ TiLayerDev ti_push(TiFlat& s, float d, float rvp, float rvs, float rrho, bool half) {
+// sphericity factor with radius 6370 (bldsph, tregn96.f:1301-1383).  The
+// flattening and sphericity factors depend on the layer thicknesses only, so
+// they are computed once per node geometry (ti_geom in dk_geom.h, run by
+// k_prep_geom) and every column applies them (ti_layer).
+__device__ __forceinline__ TiLayerDev ti_layer(const TiGeom& g, float rvp, float rvs, float rrho) {
 #pragma clang fp contract(off)
   TiLayerDev t;
-  t.d = d;
+  t.d = g.d;
   t.rvp = rvp;
   t.rvs = rvs;
   t.rrho = rrho;
@@ -55,75 +53,59 @@ __device__ TiLayerDev ti_push(TiFlat& s, float d, float rvp, float rvs, float rr
   t.TL = t.rrho * s2;
   float tl2 = 2.0f * t.TL;
   t.TF = 1.0f * (t.TA - tl2);
-  const double ar = 6371.0;
-  const double e_r = (double)(-2.275f), e_m = (double)(-0.2750f);
-  float td = half ? 1.0f : t.d;
-  double r1 = s.r0 - (double)td;
-  double z0 = ar * log(ar / s.r0);
-  double z1 = ar * log(ar / r1);
-  float tdf = (float)(z1 - z0);
-  double tmp = (ar + ar) / (s.r0 + r1);
-  t.rho = (float)((double)t.rrho * pow(tmp, e_r));
-  double fm = pow(tmp, e_m);
-  t.A = (float)((double)t.TA * fm);
-  t.L = (float)((double)t.TL * fm);
+  t.rho = (float)((double)t.rrho * g.pr);
+  t.A = (float)((double)t.TA * g.fm);
+  t.L = (float)((double)t.TL * g.fm);
   t.F = t.TF;
-  t.df = half ? 0.0f : tdf;
-  s.r0 = r1;
-  const double ar2 = 6370.0;
-  double zd = half ? 1.0 : (double)t.df;
-  double rv1 = s.rv * exp(-zd / ar2);
-  double tv = (ar2 + ar2) / (s.rv + rv1);
-  t.vtp = (float)tv;
-  s.rv = rv1;
+  t.df = g.df;
+  t.vtp = g.vtp;
   return t;
 }
 
-__global__ void k_prep_ti(int ncol, int nz, const float* __restrict__ vel,
-                          const float* __restrict__ vpin, const float* __restrict__ rhoin,
-                          const float* __restrict__ depz, float minthk0, int L,
-                          TiLayerDev* __restrict__ out, int* __restrict__ queue) {
+// Node models, one wavefront per column and lane = layer (k_prep_nodes'
+// layout): each lane interpolates its layer between its nodes (the sublayer
+// mapping and TI factors of geo) and writes its TiLayerDev.
+__global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restrict__ vel,
+                                                 const float* __restrict__ vpin,
+                                                 const float* __restrict__ rhoin, int L,
+                                                 const FlatGeom* __restrict__ geo,
+                                                 TiLayerDev* __restrict__ out,
+                                                 int* __restrict__ queue) {
 #pragma clang fp contract(off)
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col == 0) queue[0] = 0;  // the complex-pair queue of the k_ti launch that follows
-  if (col >= ncol) return;
-  TiLayerDev* lay = out + (size_t)col * L;
-  TiFlat fs;
-  // refined, unflattened layers (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196)
-  int k = 0;
-  float vs_i = vel[col], vp_i, rho_i;
-  if (vpin) {
-    vp_i = vpin[col];
-    rho_i = rhoin[col];
-  } else {
-    vp_i = brocher_vp_ti(vs_i);
-    rho_i = brocher_rho_ti(vp_i);
-  }
-  for (int i = 0; i < nz - 1; i++) {
-    size_t o = (size_t)(i + 1) * ncol + col;
-    float vs_n = vel[o], vp_n, rho_n;
+  const int lane = threadIdx.x & 63;
+  const int w0 = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (w0 == 0 && lane == 0) queue[0] = 0;  // the complex-pair queue of the k_ti launch that follows
+  auto node_vals = [&](int col, int i, float* vs, float* vp, float* rho) {
+    const size_t o = (size_t)i * ncol + col;
+    *vs = vel[o];
     if (vpin) {
-      vp_n = vpin[o];
-      rho_n = rhoin[o];
+      *vp = vpin[o];
+      *rho = rhoin[o];
     } else {
-      vp_n = brocher_vp_ti(vs_n);
-      rho_n = brocher_rho_ti(vp_n);
+      *vp = brocher_vp_ti(*vs);
+      *rho = brocher_rho_ti(*vp);
     }
-    float thk = depz[i + 1] - depz[i];
-    float minthk = thk / minthk0;
-    int ns = (int)((thk + 1.0e-4f) / minthk) + 1;
-    float newthk = thk / (float)ns;
-    float den = (float)(2 * ns);
-    for (int j = 1; j <= ns && k < L - 1; j++, k++) {
-      float w = (float)(2 * j - 1);
-      lay[k] = ti_push(fs, newthk, vp_i + w * (vp_n - vp_i) / den, vs_i + w * (vs_n - vs_i) / den,
-                       rho_i + w * (rho_n - rho_i) / den, false);
+  };
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  for (int col = w0; col < ncol; col += nw) {
+    TiLayerDev* lay = out + (size_t)col * L;
+    for (int m = lane; m < L; m += 64) {
+      const FlatGeom& g = geo[m];
+      if (g.node < 0) continue;
+      float vs, vp, rho;
+      node_vals(col, g.node, &vs, &vp, &rho);
+      if (m < L - 1) {
+        float vs_n, vp_n, rho_n;
+        node_vals(col, g.node + 1, &vs_n, &vp_n, &rho_n);
+        const float den = (float)(2 * g.ns);
+        const float w = (float)(2 * g.sub - 1);
+        vp = vp + w * (vp_n - vp) / den;
+        vs = vs + w * (vs_n - vs) / den;
+        rho = rho + w * (rho_n - rho) / den;
+      }
+      lay[m] = ti_layer(g.ti, vp, vs, rho);
     }
-    vs_i = vs_n;
-    vp_i = vp_n;
-    rho_i = rho_n;
   }
-  lay[L - 1] = ti_push(fs, 0.0f, vp_i, vs_i, rho_i, true);
 }
 
 // One (column, period) solve s of the TI splice: writes Lsen_Gsc of every
@@ -209,12 +191,11 @@ __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
                      scratch + tid, nthr);
 }
 
-hipError_t launch_prep_ti(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                          const float* depz, float minthk0, int L, TiLayerDev* out, int* queue,
-                          hipStream_t st) {
-  int block = 128;
-  hipLaunchKernelGGL(k_prep_ti, dim3((ncol + block - 1) / block), dim3(block), 0, st, ncol, nz,
-                     vel, vp, rho, depz, minthk0, L, out, queue);
+hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
+                          const void* geo, TiLayerDev* out, int* queue, hipStream_t st) {
+  // four columns per 256-thread workgroup (one per wavefront)
+  hipLaunchKernelGGL(k_prep_ti, dim3((ncol + 3) / 4), dim3(256), 0, st, ncol, vel, vp, rho, L,
+                     (const FlatGeom*)geo, out, queue);
   return hipGetLastError();
 }
 
