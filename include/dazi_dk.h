@@ -19,7 +19,9 @@
  *
  * Device work buffers belong to a per-device pool that only grows; buffers a
  * growth replaces are kept until the process ends, so a HIP graph captured
- * around a device entry point stays valid after later, larger calls.
+ * around a device entry point stays valid after later, larger calls, and
+ * after calls with another node geometry (each geometry's sublayer counts and
+ * layer records live in buffers of their own).
  *
  * Return codes: 0 ok; otherwise one of DAZI_DK_E* (the Fortran shim turns a
  * non-zero code into `stop <code>`), and dazi_dk_last_error() describes it.
