@@ -1041,13 +1041,19 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
                          dconst<1>((double)h.w), omega, e);
   }
   int Ef = 0;
+  // each layer's record is loaded one iteration ahead, before that iteration's
+  // tape stores (loads and stores share one counter: a load issued after the
+  // stores would wait for them too); the loads are unconditional, at clamped
+  // indices, so that they add no branch to the iteration
+  double4 lnext = lay[L >= 2 ? L - 2 : 0];
   for (int m = L - 1; m >= 0; m--) {
     if (m < L - 1) {
+      const double4 l = lnext;
+      lnext = lay[m >= 1 ? m - 1 : 0];
       double* tp = tape + (int64_t)(m * 6) * nthr + tid;
 #pragma unroll
       for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i].v;
       tp[(int64_t)5 * nthr] = (double)Ef;
-      double4 l = lay[m];
       double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
       layer_ca_visit<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
                               dconst<1>((double)l.w), (double)l.x, omega,
@@ -1087,15 +1093,36 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   // node accumulation state
   int node = 0, sub = 0;
   double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
+  // the next layer's record, tape entry and flattening factors are loaded one
+  // iteration ahead, so their latency overlaps this layer's matrix
+  auto load_tape = [&](int m, double* t) {
+    const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+#pragma unroll
+    for (int i = 0; i < 6; i++) t[i] = tp[(int64_t)i * nthr];
+  };
+  // (unconditional loads at clamped indices: no branch added to the iteration;
+  // the tape holds at least one entry per thread)
+  const int mt = L >= 2 ? L - 2 : 0;  // last tape entry
+  double4 ln = lay[0];
+  double tn[6];
+  load_tape(0, tn);
+  double btn = btp[0], brn = btr[0];
+  int ns_cur = (O.node_mode && O.nz > 1) ? O.nsub[0] : 1;
   for (int m = 0; m <= L - 1; m++) {
-    double4 l = lay[m];
+    const double4 l = ln;
+    double tc[6];  // this layer's tape entry (read only below the half-space)
+#pragma unroll
+    for (int i = 0; i < 6; i++) tc[i] = tn[i];
+    const double bt = btn, btr_m = brn;
+    const int mn = min(m + 1, L - 1);
+    ln = lay[mn];
+    load_tape(min(m + 1, mt), tn);
+    btn = btp[mn];
+    brn = btr[mn];
     double part[3];
     if (m < L - 1) {
-      const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
-      double et[5];
-#pragma unroll
-      for (int i = 0; i < 5; i++) et[i] = tp[(int64_t)i * nthr];
-      int Ee = (int)tp[(int64_t)5 * nthr];
+      const double* et = tc;
+      const int Ee = (int)tc[5];
       // contribution sum_ji e_j dCA_ji g_i and the propagation g <- CA g,
       // reduced entry by entry as the layer matrix is formed
       double acc[3] = {0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1131,8 +1158,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     }
     // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
     // rho_flat = rho*btp^-2.275
-    double bt = (double)btp[m];
-    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * btr[m];
+    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * btr_m;
     if (!O.node_mode) {
       int64_t o = out_index(m);
       if (O.dvp) O.dvp[o] = pa;
@@ -1142,7 +1168,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     }
     // node mode: sublayer j (1-based) of interval `node` with ns sublayers
     if (m < L - 1) {
-      int ns = (node < O.nz - 1) ? O.nsub[node] : 1;
+      const int ns = ns_cur;
       int j = sub + 1;
       double whi = (double)(2 * j - 1) / (double)(2 * ns);
       double wlo = 1.0 - whi;
@@ -1165,6 +1191,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
         }
         node++;
         sub = 0;
+        ns_cur = (node < O.nz - 1) ? O.nsub[node] : 1;
       }
     } else {
       int64_t o = out_index(node);
@@ -1250,9 +1277,13 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
         halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
                              dconst<1>((double)h.w), omega, e);
         int Ef = 0;
+        // each layer's record is loaded one iteration ahead, before the tape
+        // stores of the iteration (a load issued after them waits for them too)
+        double4 lnext = lay[L >= 2 ? L - 2 : 0];
         for (int m = L - 1; m >= 0; m--) {
           if (m < L - 1) {
-            double4 l = lay[m];
+            const double4 l = lnext;
+            lnext = lay[m >= 1 ? m - 1 : 0];
             Dual<1> ca[5][5];
             layer_ca<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
                               dconst<1>((double)l.w), (double)l.x, omega, ca);
@@ -1304,20 +1335,48 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
       int Eg = 0;
       double Z = 0.0, prevR1 = 0.0;
       const double ar = 6370.0;
+      // the next layer's thickness, record, tape entry and flattening factors
+      // are loaded one iteration ahead, so their latency overlaps this layer
+      auto thk_at = [&](int m) -> double {
+        return (m == L - 1) ? 1.0 : (thk64 ? thk64[loff + m] : (double)thk32[loff + m]);
+      };
+      // lanes without a solve read lane 0's slot (the same cache lines)
+      const long tslot = act ? slot : slot - lane;
+      auto load_tape = [&](int m, double* t) {
+        const double* tp = tape + (long)(m * 6) * nslot + tslot;
+#pragma unroll
+        for (int i = 0; i < 6; i++) t[i] = tp[(long)i * nslot];
+      };
+      // (unconditional loads at clamped indices, also on lanes without a
+      // solve: no branch added to the iteration; the tape holds at least one
+      // entry per slot)
+      const int mt = L >= 2 ? L - 2 : 0;  // last tape entry
+      double thn = thk_at(0);
+      double4 ln = lay[0];
+      double tn[6];
+      load_tape(0, tn);
+      double btn = btp[0], brn = btr[0];
       for (int m = 0; m < L; m++) {
-        const double thk_m = (m == L - 1) ? 1.0 : (thk64 ? thk64[loff + m] : (double)thk32[loff + m]);
+        const double thk_m = thn;
+        const double4 l = ln;
+        double tc[6];  // this layer's tape entry (read only below the half-space)
+#pragma unroll
+        for (int i = 0; i < 6; i++) tc[i] = tn[i];
+        const double bt = btn, btr_m = brn;
+        const int mn = min(m + 1, L - 1);
+        thn = thk_at(mn);
+        ln = lay[mn];
+        load_tape(min(m + 1, mt), tn);
+        btn = btp[mn];
+        brn = btr[mn];
         const double r0 = ar - Z, r1 = r0 - thk_m;
         Z += thk_m;
         double part[4] = {0.0, 0.0, 0.0, 0.0};
         double R0 = 0.0, R1 = 0.0;
         if (act) {
-          double4 l = lay[m];
           if (m < L - 1) {
-            const double* tp = tape + (long)(m * 6) * nslot + slot;
-            double et[5];
-#pragma unroll
-            for (int i = 0; i < 5; i++) et[i] = tp[(long)i * nslot];
-            const int Ee = (int)tp[(long)5 * nslot];
+            const double* et = tc;
+            const int Ee = (int)tc[5];
             // Dual<3> in (a, b, d); the rho partial from the entries' rho powers
             double acc[4] = {0.0, 0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             layer_ca_visit<Dual<3>, Dual<3>>(
@@ -1353,7 +1412,6 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
             for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Eg - EF) * lane_scale;
           }
           // part = grad_c * dc/d(flattened a, b, rho, d)
-          const double bt = (double)btp[m];
           const double dtmp = part[0] * ((double)l.y / bt) + part[1] * ((double)l.z / bt) -
                               2.275 * part[2] * ((double)l.w / bt);
           const double ttm = -dtmp * bt / (r0 + r1);
@@ -1361,7 +1419,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
           R1 = ttm - ((m < L - 1) ? part[3] * (ar / r1) : 0.0);
           part[0] *= bt;
           part[1] *= bt;
-          part[2] *= btr[m];
+          part[2] *= btr_m;
         }
         const double sa = wave_sum(part[0]), sb = wave_sum(part[1]), sr = wave_sum(part[2]);
         double s0 = 0.0, s1 = 0.0;
