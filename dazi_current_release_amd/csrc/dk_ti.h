@@ -721,14 +721,22 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   double exsum = 0.0;
   for (int m = L - 2; m >= 0; m--) {
     const Layer ly = lay(m);
+    // the previous layer's vector is stored after this layer's reads are
+    // issued (a read issued after stores waits for them too); the vector and
+    // its scale are this iteration's inputs, so they are in registers anyway.
+    // At m = L - 2 this rewrites the half-space entries with their own values.
+    put_cd(m + 1, cd);
+    S(m + 1, Q::EX) = exsum;
     eigen(ly, omega, wvno, &g);
     ok &= g.ok;
     hyper(g, ly.d, &h);
     compound_step(g, h, cd);
     const double exn = norm5(cd);
     exsum = exsum + h.pex + h.svex + exn;
-    put_cd(m, cd);
-    S(m, Q::EX) = exsum;
+  }
+  if (L >= 2) {
+    put_cd(0, cd);
+    S(0, Q::EX) = exsum;
   }
   if (!ok) return -2;
   // ---- surface eigenfunction: ur = Re(cd(1,3)/cd(1,2)), f1213 = -Re cd(1,2)
