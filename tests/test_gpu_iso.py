@@ -76,6 +76,31 @@ def test_plain_1024_kernels_vs_tight_fd(dk):
             assert rel.max() <= KERN_REL, rel.max()
 
 
+def test_deep_refinement_kernels_vs_tight_fd(dk):
+    """157 refined layers (12 node intervals of 13 sublayers): the model prep
+    takes them in three 64-layer chunks per column, with the start-velocity
+    layer chosen across chunks; phase velocities against the oracle, kernels
+    against the tight finite differences."""
+    W = dict(node_cfg("plain_1024", ncol=64))
+    W["minthk"] = 12.0
+    pv, svs, svp, srho = dk.run_depthkernel(W["vel"], W["periods"], W["depz"], W["minthk"],
+                                            vp=W["vp"], rho=W["rho"])
+    check_pv(pv, oracle_pv(W))
+    nx = W["vel"].shape[0]
+    for j in (0, 21, 42, 63):
+        ii, jj = j % nx, j // nx
+        kw = dict(vp=W["vp"][ii, jj], rho=W["rho"][ii, jj], tol_rel=1e-13, round_f32=False,
+                  exact_step=True)
+        t1 = P.column_fd(W["vel"][ii, jj], W["periods"], W["depz"], W["minthk"], dln=0.01, **kw)
+        t2 = P.column_fd(W["vel"][ii, jj], W["periods"], W["depz"], W["minthk"], dln=0.005,
+                         **kw)
+        tight = [(4 * b - a) / 3 for a, b in zip(t1[1:], t2[1:])]
+        for g, ref in zip((svs[j], svp[j], srho[j]), tight):
+            scale = np.abs(ref).max(axis=1, keepdims=True)
+            rel = np.abs(g - ref) / scale
+            assert rel.max() <= KERN_REL, rel.max()
+
+
 @pytest.mark.parametrize("fixture", ["ref_plain_1024.npz", "ref_plain_4096.npz"])
 def test_vs_reference_fd_fixture(dk, fixture):
     """Brocher vp/rho (the reference's depthkernel semantics) against the reference's

@@ -54,6 +54,19 @@ def test_ti_vs_live_reference_64_columns(dk):
     check_ti(pv, ls, np.array([r[0] for r in ref]), np.array([r[1] for r in ref]))
 
 
+@pytest.mark.skipif(not P.ref_available(), reason="reference library not built")
+def test_ti_deep_refinement_vs_live_reference(dk):
+    """157 refined layers (three 64-layer chunks in the TI model prep)."""
+    from dazi_current_release_amd import configs
+    W = configs.node_workload("plain_1024", ncol=16)
+    minthk = 12.0
+    pv, ls = dk.run_depthkernelTI(W["vel"], W["periods"], W["depz"], minthk)
+    nx = W["vel"].shape[0]
+    ref = [P.ti_column(W["vel"][j % nx, j // nx], W["periods"], W["depz"], minthk)
+           for j in range(16)]
+    check_ti(pv, ls, np.array([r[0] for r in ref]), np.array([r[1] for r in ref]))
+
+
 def test_fused_stage_equals_separate_calls(dk):
     import torch
     from dazi_current_release_amd import configs
