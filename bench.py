@@ -1,6 +1,6 @@
 """Benchmark of the depth-kernel stage on one MI355X.
 
-Workload (BASELINE metric's production configuration): plain_4096 — 64x64 = 4,096
+Workload (the production configuration of the solves/s metric): plain_4096 — 64x64 = 4,096
 columns, 25 depth nodes 0-120 km refined to 49 layers, 30 log-spaced periods
 1-60 s; one step = model prep + Rayleigh fundamental-mode root search (batched
 persistent launch) + isotropic node kernels dc/dVs, dc/dVp, dc/drho + the TI
