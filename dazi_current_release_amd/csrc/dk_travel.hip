@@ -237,15 +237,19 @@ __device__ __forceinline__ int find_pos_t(const Heap& h, int n) {
   return 0;
 }
 
-// LDS-only form: every lane reads its eight entries unconditionally (the index
-// clamped into the array; entries past the tree are masked by the position
-// test), so the search has no branches until the ballots.
+// LDS-only form (ntr <= HEAP_LDS): every lane reads its eight entries
+// unconditionally (entries past the tree are masked by the position test), so
+// the search has no branches until the ballots.  Rounds start at 1, 513, ...,
+// so the highest position read is HEAP_LDS, the array's last entry: the eight
+// reads share one address with constant offsets.
+static_assert(HEAP_LDS % 512 == 0, "find_pos_lds reads positions 1 .. HEAP_LDS");
 __device__ __forceinline__ int find_pos_lds(const Heap& h, int n) {
   const int lane = threadIdx.x;
   for (int base = 1; base <= h.ntr; base += 512) {
     int nd[8];
+    const int2* row = h.e + base + lane;
 #pragma unroll
-    for (int u = 0; u < 8; u++) nd[u] = h.e[min(base + u * 64 + lane, HEAP_LDS)].y;
+    for (int u = 0; u < 8; u++) nd[u] = row[u * 64].y;
 #pragma unroll
     for (int u = 0; u < 8; u++) {
       const unsigned long long m = __ballot(nd[u] == n && base + u * 64 + lane <= h.ntr);
