@@ -16,6 +16,7 @@ DAZI_DK_EINVAL = 9301
 DAZI_DK_EHIP = 9302
 DAZI_DK_ELIMIT = 9303
 DAZI_DK_ENODEV = 9304
+NOALLOC_SCRATCH_BYTES = 32 << 20
 
 EXPORTS = [
     "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
@@ -24,7 +25,7 @@ EXPORTS = [
     "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
     "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows", "dazi_dk_azimuth_map_dev",
     "dazi_dk_azimuth_map", "dazi_dk_travel_plan_create", "dazi_dk_travel_run_dev",
-    "dazi_dk_travel_plan_destroy", "dazi_dk_travel",
+    "dazi_dk_travel_plan_destroy", "dazi_dk_travel", "dazi_dk_layered_noalloc_dev",
 ]
 
 
@@ -98,6 +99,7 @@ def lib():
         L.dazi_dk_iso.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 9
         L.dazi_dk_iso_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 11
         L.dazi_dk_layered_dev.argtypes = [C.POINTER(Layered), C.c_void_p, C.c_void_p]
+        L.dazi_dk_layered_noalloc_dev.argtypes = [C.POINTER(Layered), C.c_void_p, C.c_void_p]
         L.dazi_dk_ti.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 5
         L.dazi_dk_ti_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 9
         L.dazi_dk_stage_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 12

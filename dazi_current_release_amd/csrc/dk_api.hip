@@ -209,34 +209,47 @@ void rec(Pool* P, int i, hipStream_t st) {
   if (g_timing) (void)hipEventRecord(P->ev[i], st);
 }
 
-// Root search over columns [0, ncol) with the configured mode.
-int run_forward(Pool* P, const dk::BatchDev& B, const dazi_dk_tune& T, int lmax,
-                hipStream_t st) {
-  int S = T.lanes_per_scan, cpw = T.cols_per_wg;
-  // two double4 records per layer and column in LDS
-  int lds_stride = 2 * lmax;
-  size_t lds = (size_t)cpw * lds_stride * 32;
+// The root search's LDS staging holds two double4 records per layer and column.
+int check_lds(const dazi_dk_tune& T, int lmax) {
+  size_t lds = (size_t)T.cols_per_wg * 2 * lmax * 32;
   if (lds > 64 * 1024)
     return set_err(DAZI_DK_ELIMIT, "model too deep for LDS staging: %d layers x %d columns", lmax,
-                   cpw);
-  P->nforward = 0;
+                   T.cols_per_wg);
+  return 0;
+}
+
+// Root search over columns [0, ncol) with the configured mode on a device of
+// ncu compute units; *nlaunch receives the number of launches.  The persistent
+// work counter B.work is cleared by a kernel (no memset node in a graph).
+int forward_launches(const dk::BatchDev& B, const dazi_dk_tune& T, int lmax, int ncu,
+                     hipStream_t st, int* nlaunch) {
+  int S = T.lanes_per_scan, cpw = T.cols_per_wg;
+  int lds_stride = 2 * lmax;
+  if (int rc = check_lds(T, lmax)) return rc;
+  *nlaunch = 0;
   if (T.mode == 0) {
     HIPCHK(dk::launch_zero_int(B.work, 1, st));
     int nb = dk::forward_max_blocks(S, cpw, lds_stride);
-    int grid = P->ncu * nb;
+    int grid = ncu * nb;
     int need = (B.ncol + cpw - 1) / cpw;
     if (grid > need) grid = need;
     HIPCHK(dk::launch_forward(B, S, 0, B.ncol, cpw, 1, grid, lds_stride, st));
-    P->nforward = 1;
+    *nlaunch = 1;
   } else {
     for (int c0 = 0; c0 < B.ncol; c0 += T.tile_columns) {
       int c1 = c0 + T.tile_columns;
       if (c1 > B.ncol) c1 = B.ncol;
       HIPCHK(dk::launch_forward(B, S, c0, c1, cpw, 0, 0, lds_stride, st));
-      P->nforward++;
+      ++*nlaunch;
     }
   }
   return 0;
+}
+
+int run_forward(Pool* P, const dk::BatchDev& B, const dazi_dk_tune& T, int lmax,
+                hipStream_t st) {
+  P->nforward = 0;
+  return forward_launches(B, T, lmax, P->ncu, st, &P->nforward);
 }
 
 int kernel_threads(Pool* P, int64_t nsolves) {
@@ -1183,6 +1196,130 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   rec(P, 3, st);
   rec(P, 4, st);  // no TI phase
   return 0;
+}
+
+}  // extern "C"
+
+// ---- the allocation-free root search
+//
+// dazi_dk_layered_noalloc_dev keeps its work memory (flattened layers, their
+// flattening factors, the per-column start velocity, betmx and prep flags, the
+// unrounded roots when the caller passes none, and the persistent work
+// counter) in this file-scope device array instead of the pool.  Every device
+// the library runs on holds its own copy until the process ends.  On the host
+// the array's name is only a stand-in: host code takes the device address that
+// hipGetSymbolAddress returns for the current device (NoAllocDev::ptr) and
+// passes that to the kernels, never the name itself.  The array keeps the last
+// call's contents; the work counter, the one value that must start at zero, is
+// cleared by a kernel at every call.  Calls that overlap on one device race on
+// the array.
+static __device__ __align__(256) unsigned char scratch[DAZI_DK_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct NoAllocDev {
+  void* ptr = nullptr;  // device address of `scratch` on this device
+  int ncu = 0;
+};
+NoAllocDev g_noalloc[16];
+
+// Address of the array and CU count of the current device, looked up on the
+// first call on that device (no allocation: the array is part of the loaded
+// code object).
+int noalloc_device(NoAllocDev** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_err(DAZI_DK_ENODEV, "hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= 16) return set_err(DAZI_DK_ENODEV, "device index %d unsupported", dev);
+  NoAllocDev* N = &g_noalloc[dev];
+  if (!N->ptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return set_err(DAZI_DK_EHIP, "hipGetSymbolAddress(scratch) failed: %s",
+                     hipGetErrorString(e));
+    int ncu = 0;
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    N->ncu = ncu > 0 ? ncu : 1;
+    N->ptr = p;
+  }
+  *out = N;
+  return 0;
+}
+
+inline size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t dazi_dk_layered_noalloc_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune,
+                                    void* stream) {
+  bool m32 = b && b->thk && b->vp && b->vs && b->rho;
+  bool m64 = b && b->thk64 && b->vp64 && b->vs64 && b->rho64;
+  if (!b || !b->nl || !b->loff || !b->nk || !b->toff || !(m32 || m64) || !b->periods || !b->c ||
+      !b->status)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument in dazi_dk_layered");
+  if (b->dc_dvs || b->dc_dvp || b->dc_drho)
+    return set_err(DAZI_DK_EINVAL,
+                   "dazi_dk_layered_noalloc_dev computes phase velocities only (kernel outputs "
+                   "must be NULL)");
+  if (b->ncol < 0 || b->max_layers < 1 || b->nsolves < 0)
+    return set_err(DAZI_DK_EINVAL, "bad sizes ncol=%d max_layers=%d", b->ncol, b->max_layers);
+  if (b->nlayers_total <= 0)
+    return set_err(DAZI_DK_EINVAL, "dazi_dk_layered_noalloc_dev needs nlayers_total");
+  if (b->ncol == 0) return 0;
+  dazi_dk_tune T;
+  if (int rc = check_tune(tune, &T)) return rc;
+  if (int rc = check_lds(T, b->max_layers)) return rc;
+  // the array's layout: work counter, per-column values, per-layer records,
+  // then the unrounded roots unless the caller gives c_unrounded
+  const size_t ncol = (size_t)b->ncol, nlay = (size_t)b->nlayers_total;
+  const size_t o_cc = 256, o_betmx = o_cc + align256(ncol * 8),
+               o_bad = o_betmx + align256(ncol * 4), o_lay = o_bad + align256(ncol * 4),
+               o_btp = o_lay + align256(nlay * 32), o_croot = o_btp + align256(nlay * 16);
+  const size_t need =
+      o_croot + (b->c_unrounded ? 0 : align256((size_t)(b->nsolves > 0 ? b->nsolves : 1) * 8));
+  if (need > (size_t)DAZI_DK_NOALLOC_SCRATCH_BYTES)
+    return set_err(DAZI_DK_ELIMIT,
+                   "batch needs %zu bytes of work memory, the allocation-free entry point has "
+                   "%zu (%zu columns, %zu layers, %lld solves)",
+                   need, (size_t)DAZI_DK_NOALLOC_SCRATCH_BYTES, ncol, nlay,
+                   (long long)b->nsolves);
+  NoAllocDev* N;
+  if (int rc = noalloc_device(&N)) return rc;
+  unsigned char* base = (unsigned char*)N->ptr;
+  hipStream_t st = (hipStream_t)stream;
+  double4* layers = (double4*)(base + o_lay);
+  double* btp = (double*)(base + o_btp);
+  double* btr = btp + nlay;
+  double* cc = (double*)(base + o_cc);
+  float* betmx = (float*)(base + o_betmx);
+  int* bad = (int*)(base + o_bad);
+  double* croot = b->c_unrounded ? b->c_unrounded : (double*)(base + o_croot);
+  HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho, b->thk64,
+                                b->vp64, b->vs64, b->rho64, layers, btp, btr, cc, betmx, bad, st));
+  dk::BatchDev B = {};
+  B.ncol = b->ncol;
+  B.uniform = 0;
+  B.L = b->nl;
+  B.loff = b->loff;
+  B.K = b->nk;
+  B.toff = b->toff;
+  B.layers = layers;
+  B.btp = btp;
+  B.btr = btr;
+  B.periods = b->periods;
+  B.cc = cc;
+  B.betmx = betmx;
+  B.bad = bad;
+  B.croot = croot;
+  B.pv = b->c;
+  B.status = b->status;
+  B.work = (int*)base;
+  B.tol = T.refine_tol;
+  int nlaunch = 0;
+  return forward_launches(B, T, b->max_layers, N->ncu, st, &nlaunch);
 }
 
 int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, double* grad_vs,

@@ -167,6 +167,26 @@ int32_t dazi_dk_stage_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, co
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
                             void* stream);
 
+/* Bytes of the device work memory of dazi_dk_layered_noalloc_dev (a static
+ * array that every device the library runs on holds until the process ends). */
+#define DAZI_DK_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* The ragged batch's root search without any device allocation: the same
+ * phase velocities c, c_unrounded (optional) and status as
+ * dazi_dk_layered_dev, bit for bit, with its work memory in the library's
+ * static array of DAZI_DK_NOALLOC_SCRATCH_BYTES instead of the growing pool.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The kernel outputs dc_dvs/dc_dvp/dc_drho must be
+ * NULL and nlayers_total must be set.  Limit: 256 + 48 B per layer + 16 B per
+ * column (+ 8 B per solve when c_unrounded is NULL), each part rounded up to
+ * 256 B, within DAZI_DK_NOALLOC_SCRATCH_BYTES; a larger batch returns
+ * DAZI_DK_ELIMIT before any device work.  All calls on one device share the
+ * array, so calls that overlap on a device (other streams, other host
+ * threads) race. */
+int32_t dazi_dk_layered_noalloc_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
+                                    void* stream);
+
 /* Vector-Jacobian product of the layered batch's phase velocities: for every
  * layer m of every column, grad_x[loff+m] = sum over the column's periods of
  * grad_c[s] * dc_s/dx_m, x in {vs, vp, rho, thk} of the unflattened model, by

@@ -65,3 +65,22 @@ def test_tune_validation():
     L.dazi_dk_get_tune(C.byref(got))
     assert (got.lanes_per_scan, got.cols_per_wg, got.mode) == (32, 8, 1)
     assert L.dazi_dk_set_tune(C.byref(saved)) == 0
+
+
+def test_noalloc_size_limit_checked_before_any_gpu_call():
+    """The allocation-free entry point rejects a batch beyond its static work
+    memory with DAZI_DK_ELIMIT before touching the device."""
+    L = _lib.lib()
+    buf = np.zeros(64, np.float64)
+    p = buf.ctypes.data_as(C.c_void_p).value
+    b = _lib.Layered()
+    b.ncol, b.max_layers, b.nsolves = 4, 8, 8
+    for f in ("nl", "loff", "nk", "toff", "thk", "vp", "vs", "rho", "periods", "c", "status"):
+        setattr(b, f, p)
+    b.nlayers_total = _lib.NOALLOC_SCRATCH_BYTES // 48 + 1
+    assert L.dazi_dk_layered_noalloc_dev(C.byref(b), None, None) == _lib.DAZI_DK_ELIMIT
+    assert b"allocation-free" in L.dazi_dk_last_error()
+    b.nlayers_total = 0
+    assert L.dazi_dk_layered_noalloc_dev(C.byref(b), None, None) == _lib.DAZI_DK_EINVAL
+    b.nlayers_total, b.dc_dvs = 32, p
+    assert L.dazi_dk_layered_noalloc_dev(C.byref(b), None, None) == _lib.DAZI_DK_EINVAL
