@@ -32,6 +32,46 @@ FLOP_PER_LAYER_EVAL = 172    # FP64 ops per layer per secular-function evaluatio
 N_EV = 12                    # evaluations per root of the reference's bracket + refinement
 N_K_ISO = 3                  # evaluation-equivalents of the isotropic kernel pass
 N_K_TI = 2                   # evaluation-equivalents of the TI kernel pass
+N_VJP = 5                    # evaluation-equivalents of the VJP pass
+PIPE = "FP64 VALU (vector ALU; no MFMA: nothing here is a dense contraction)"
+
+
+def roofline_rec(flop, seconds, scope, **extra):
+    """Algorithmic FP64 rate of `flop` operations in `seconds` against the
+    MI355X FP64 vector peak."""
+    a = flop / seconds
+    return dict(bound="valu", achieved=a / 1e12, peak=PEAK_FP64 / 1e12, unit="TFLOP/s",
+                frac=a / PEAK_FP64, pipe=PIPE, scope=scope, flop=flop, seconds=seconds, **extra)
+
+
+def host_cpus():
+    """(threads, description) of the CPU share this process may use: the CPUs of
+    its affinity mask, limited by OMP_NUM_THREADS where the job sets it (the
+    share a one-GPU job gets on a node), with nproc and the CPU model."""
+    aff = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "").strip()
+    n = aff
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:  # noqa: BLE001 - lscpu missing: fall back to /proc/cpuinfo
+        pass
+    if not model:
+        try:
+            with open("/proc/cpuinfo") as fh:
+                for line in fh:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return n, dict(nproc=os.cpu_count(), affinity_cpus=aff, omp_num_threads=omp or None,
+                   model=model)
 
 
 def _ti_chunk(args):
@@ -42,16 +82,23 @@ def _ti_chunk(args):
     return len(vel)
 
 
-def cpu_baseline_child(ncols: int, threads: int) -> None:
-    """Child process: the reference stage on the first ncols columns of the
-    workload — depthkernel (FD kernels, OpenMP over columns) and the TI column
-    routine in a pool of `threads` processes (tregn96 keeps global state)."""
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    import multiprocessing as mp
-    import numpy as np
-    from oracle import port as P
+def _node_sample(name, ncols):
     from dazi_current_release_amd import configs
-    W = configs.node_workload("plain_4096", ncol=ncols)
+    if name == "voronoi_100":
+        W = configs.voronoi_workload(1)
+        return dict(vel=W["vel"][:ncols], depz=W["depz"], periods=W["periods"],
+                    minthk=W["minthk"])
+    return configs.node_workload(name, ncol=ncols)
+
+
+def cpu_child_node(name, ncols, threads, ti):
+    """The reference stage on the first ncols columns of a node workload:
+    depthkernel (FD kernels, OpenMP over columns, CalSurfG.f90:86-107) and, with
+    ti, the TI column routine in a pool of `threads` processes (tregn96 keeps
+    global state, so it cannot run in threads)."""
+    import multiprocessing as mp
+    from oracle import port as P
+    W = _node_sample(name, ncols)
     kind = "reference" if P.ref_available() else "port"
     lib = "ref" if kind == "reference" else "port"
     # the reference stage derives vp and rho from vs (Brocher); same layer and
@@ -60,7 +107,7 @@ def cpu_baseline_child(ncols: int, threads: int) -> None:
     P.depthkernel(W["vel"], W["periods"], W["depz"], W["minthk"], lib=lib, nthreads=threads)
     t_iso = time.perf_counter() - t0
     t_ti = 0.0
-    if kind == "reference":
+    if ti and kind == "reference":
         vel = W["vel"][:, 0, :]
         chunks = [(vel[i::threads], W["periods"], W["depz"], W["minthk"]) for i in range(threads)]
         ctx = mp.get_context("fork")
@@ -70,81 +117,188 @@ def cpu_baseline_child(ncols: int, threads: int) -> None:
             t_ti = time.perf_counter() - t1
     dt = t_iso + t_ti
     nsol = ncols * len(W["periods"])
+    nz = W["vel"].shape[-1]
+    what = (f"reference depthkernel, FD node kernels = {1 + 6 * nz} surfdisp96 curves per "
+            f"column, OpenMP {t_iso:.1f} s")
+    if ti and kind == "reference":
+        what += (f" + reference depthkernel_ti_column (surfdisp96 + tregn96) in a process "
+                 f"pool {t_ti:.1f} s")
+    return dict(value=nsol / dt, seconds=dt, kind=kind, cols=ncols, threads=threads, solves=nsol,
+                sample=f"first {ncols} columns of {name} ({nsol} column-period solves): {what}")
+
+
+def cpu_child_layered(name, ncols, threads):
+    """The reference's own layered forward driver
+    (depthkernel_lab/surfdisp96_forward_batch.f90, built from source by
+    oracle/Makefile) on the finite-difference model set of each of the first
+    ncols columns (the model and 6 perturbed models per layer, as
+    DepthKernel1D.f90:146-183 perturbs node models), one driver process per
+    column, `threads` at a time: c and every layer's dc/dVs, dc/dVp, dc/drho the
+    reference's way."""
+    import tempfile
+    import numpy as np
+    from oracle import port as P
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload(name, ncol=ncols)
+    nl, nk = W["nl"], W["nk"]
+    loff = np.concatenate([[0], np.cumsum(nl)[:-1]])
+    toff = np.concatenate([[0], np.cumsum(nk)[:-1]])
+    with tempfile.TemporaryDirectory() as d:
+        files = []
+        for j in range(ncols):
+            sl = slice(loff[j], loff[j] + nl[j])
+            f = os.path.join(d, f"col{j}.in")
+            P.write_layered_fd_input(f, *(W[k][sl] for k in ("thk", "vp", "vs", "rho")),
+                                     W["periods"][toff[j]:toff[j] + nk[j]])
+            files.append(f)
+        dt, _ = P.run_forward_batch(files, threads)
+    nsol = int(nk.sum())
+    curves = int(((1 + 6 * nl) * nk).sum())
+    return dict(value=nsol / dt, seconds=dt, kind="reference", cols=ncols, threads=threads,
+                solves=nsol,
+                sample=(f"first {ncols} columns of {name} ({nsol} column-period solves): the "
+                        f"reference's surfdisp96_forward_batch on each column's model and its "
+                        f"6 L FD perturbations ({curves} period curves), one process per "
+                        f"column, {threads} at a time"))
+
+
+def cpu_baseline_child(spec: str, threads: int) -> None:
+    """Child process of one CPU baseline: spec = node:<workload>:<ncols>:<ti> or
+    layered:<workload>:<ncols>."""
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    parts = spec.split(":")
+    if parts[0] == "node":
+        rec = cpu_child_node(parts[1], int(parts[2]), threads, parts[3] == "1")
+    else:
+        rec = cpu_child_layered(parts[1], int(parts[2]), threads)
     sys.stdout.flush()
-    print("CPUBASE " + json.dumps(dict(value=nsol / dt, seconds=dt, iso_seconds=t_iso,
-                                       ti_seconds=t_ti, kind=kind, cols=ncols,
-                                       threads=threads, solves=nsol)), flush=True)
+    print("CPUBASE " + json.dumps(rec), flush=True)
 
 
-def run_cpu_baseline(ncols: int, threads: int):
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", str(ncols),
+def run_cpu_baseline(spec: str, threads: int, cpuinfo: dict):
+    """Runs one CPU baseline in a child process; returns the cpu_baseline record
+    (or a record saying why it failed: the GPU figures stand on their own)."""
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", spec,
            "--threads", str(threads)]
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
-    for line in r.stdout.splitlines():
-        if line.startswith("CPUBASE "):
-            return json.loads(line[len("CPUBASE "):])
-    raise RuntimeError(f"cpu baseline failed: rc={r.returncode} {r.stderr[-2000:]}")
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+        for line in r.stdout.splitlines():
+            if line.startswith("CPUBASE "):
+                cb = json.loads(line[len("CPUBASE "):])
+                return dict(value=cb["value"], unit="solves/s", cores=cb["threads"],
+                            kind=cb["kind"], sample=cb["sample"], seconds=cb["seconds"],
+                            host=cpuinfo)
+        raise RuntimeError(f"rc={r.returncode} {r.stderr[-1500:]}")
+    except Exception as e:  # noqa: BLE001
+        return dict(value=None, unit="solves/s", cores=threads, kind="reference",
+                    sample=f"{spec} failed: {e}", host=cpuinfo)
 
 
-def bench_vjp(dk, configs, np, torch, reps=5):
-    """Differentiable path on diff_4096 (4,096 columns x 40 layers x 30 periods,
-    float64 models): forward root search vs the backward VJP kernel."""
+def _timed(torch, fn, n):
+    fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / n
+
+
+def bench_c1(dk, configs, np, torch, cpu, reps=10):
+    """plain_1024 (32 x 32 columns, 13 depth nodes 0-60 km, 18 periods 2-40 s, Vp =
+    1.73 Vs): c and the isotropic node kernels of every column x period on the
+    device, HIP-event-free wall clock per call with inputs resident."""
+    W = configs.node_workload("plain_1024")
+    nz = W["vel"].shape[-1]
+    T = lambda a: torch.as_tensor(np.ascontiguousarray(a.reshape(-1, nz, order="F").T),  # noqa
+                                  device="cuda")
+    vel, vp, rho = T(W["vel"]), T(W["vp"]), T(W["rho"])
+    depz = torch.as_tensor(W["depz"], device="cuda")
+    per = torch.as_tensor(W["periods"], device="cuda")
+    out = dk.stage_dev(vel, depz, per, W["minthk"], vp=vp, rho=rho, iso=True, ti=False)
+    t = _timed(torch, lambda: dk.stage_dev(vel, depz, per, W["minthk"], vp=vp, rho=rho, iso=True,
+                                           ti=False, out=out), reps)
+    ncol, K, L = vel.shape[1], per.numel(), 13
+    ns = ncol * K
+    rec = dict(workload="plain_1024", columns=ncol, nodes=nz, layers=L, periods=K,
+               outputs="c + dc/dVs, dc/dVp, dc/drho node kernels", ms_per_step=1e3 * t,
+               solves=ns, solves_per_s=ns / t,
+               roofline=roofline_rec(ns * (N_EV + N_K_ISO) * FLOP_PER_LAYER_EVAL * L, t,
+                                     "whole step (prep, root search, kernel pass)",
+                                     flop_per_solve=(N_EV + N_K_ISO) * FLOP_PER_LAYER_EVAL * L))
+    if cpu:
+        rec["cpu_baseline"] = cpu("node:plain_1024:1024:0")
+        rec["cpu_baseline"]["note"] = ("the reference's depthkernel derives Vp and rho from Vs "
+                                       "(Brocher); same columns, layers and periods")
+    return rec
+
+
+def bench_vjp(dk, configs, np, torch, cpu, reps=5):
+    """diff_4096 (4,096 columns x 40 layers x 30 periods, float64 models): the
+    forward root search and the backward VJP kernel."""
     W = configs.layered_workload("diff_4096")
     b = dk.LayeredBatch(W["nl"], W["nk"], *(W[k].astype(np.float64)
                                             for k in ("thk", "vp", "vs", "rho")), W["periods"])
     out = dk.layered_dev(b)
     gc = torch.ones(b.nsolves, dtype=torch.float64, device="cuda")
     dk.layered_vjp(b, out["c_unrounded"], gc)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        dk.layered_dev(b, out=out)
-    torch.cuda.synchronize()
-    t_fwd = (time.perf_counter() - t0) / reps
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        dk.layered_vjp(b, out["c_unrounded"], gc)
-    torch.cuda.synchronize()
-    t_vjp = (time.perf_counter() - t0) / reps
-    return dict(workload="diff_4096", columns=b.ncol, layers=40, periods=30,
-                forward_ms=1e3 * t_fwd, vjp_ms=1e3 * t_vjp, vjp_over_forward=t_vjp / t_fwd)
+    t_fwd = _timed(torch, lambda: dk.layered_dev(b, out=out), reps)
+    t_vjp = _timed(torch, lambda: dk.layered_vjp(b, out["c_unrounded"], gc), reps)
+    ns, L = b.nsolves, 40
+    step = t_fwd + t_vjp
+    rec = dict(workload="diff_4096", columns=b.ncol, layers=L, periods=30,
+               forward_ms=1e3 * t_fwd, vjp_ms=1e3 * t_vjp, vjp_over_forward=t_vjp / t_fwd,
+               solves=ns, solves_per_s=ns / step,
+               roofline=roofline_rec(ns * (N_EV + N_VJP) * FLOP_PER_LAYER_EVAL * L, step,
+                                     "forward + VJP",
+                                     forward_frac=ns * N_EV * FLOP_PER_LAYER_EVAL * L / t_fwd /
+                                     PEAK_FP64,
+                                     vjp_frac=ns * N_VJP * FLOP_PER_LAYER_EVAL * L / t_vjp /
+                                     PEAK_FP64))
+    if cpu:
+        rec["cpu_baseline"] = cpu("layered:diff_4096:48")
+        rec["cpu_baseline"]["note"] = ("the reference has no VJP: its gradient is the "
+                                       "contraction of FD kernels, whose curves are timed")
+    return rec
 
 
-def bench_c3(dk, configs, np, torch, reps=5):
+def bench_c3(dk, configs, np, torch, cpu, reps=5):
     """ragged_16384 (16,384 layered columns, 8-64 layers, 10-40 periods, c and
-    refined-layer kernels): plain one launch per 64-column tile vs the batched
-    persistent launch with the autotuned shape, eager and replayed as a HIP
-    graph.  Times are wall-clock per call with inputs resident on the device."""
+    refined-layer kernels): plain one launch per 64-column tile, one
+    non-persistent launch over the whole batch, and the batched persistent
+    launch with the autotuned shape, eager and replayed as a HIP graph.  Times
+    are wall-clock per call with inputs resident on the device."""
     from dazi_current_release_amd import batched
     W = configs.layered_workload("ragged_16384")
     b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"], W["periods"])
     out = b.alloc_outputs(True)
-
-    def timed(fn, n):
-        fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(n):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) / n
-
-    plain_tune = dk.make_tune(16, 4, 1, 64, 2e-6)
-    t_plain = timed(lambda: dk.layered_dev(b, kernels=True, tune=plain_tune, out=out), 2)
-    tune, table = batched.autotune(lambda t: dk.layered_dev(b, kernels=True, tune=t, out=out))
-    t_batch = timed(lambda: dk.layered_dev(b, kernels=True, tune=tune, out=out), reps)
-    g = batched.GraphRunner(lambda: dk.layered_dev(b, kernels=True, tune=tune, out=out))
-    t_graph = timed(g.replay, reps)
+    run = lambda t: dk.layered_dev(b, kernels=True, tune=t, out=out)  # noqa: E731
+    t_plain = _timed(torch, lambda: run(dk.make_tune(16, 4, 1, 64, 2e-6)), 2)
+    t_one = _timed(torch, lambda: run(dk.make_tune(16, 4, 1, b.ncol, 2e-6)), reps)
+    tune, table = batched.autotune(run)
+    t_batch = _timed(torch, lambda: run(tune), reps)
+    g = batched.GraphRunner(lambda: run(tune))
+    t_graph = _timed(torch, g.replay, reps)
     ns = b.nsolves
-    return dict(workload="ragged_16384", columns=b.ncol, solves=ns,
-                plain_solves_per_s=ns / t_plain, plain_launches=int(np.ceil(b.ncol / 64)),
-                batched_solves_per_s=ns / t_batch, graph_solves_per_s=ns / t_graph,
-                batched_over_plain=t_plain / t_batch, graph_over_plain=t_plain / t_graph,
-                tuned=dict(lanes_per_scan=tune.lanes_per_scan, cols_per_wg=tune.cols_per_wg),
-                autotune_ms={f"{l}x{c}": round(ms, 4) for l, c, ms in table})
+    flop = (N_EV + N_K_ISO) * FLOP_PER_LAYER_EVAL * int((W["nl"].astype(np.int64) *
+                                                         W["nk"]).sum())
+    rec = dict(workload="ragged_16384", columns=b.ncol, solves=ns,
+               solves_per_s=ns / t_batch,
+               plain_solves_per_s=ns / t_plain, plain_launches=int(np.ceil(b.ncol / 64)),
+               one_launch_solves_per_s=ns / t_one,
+               batched_solves_per_s=ns / t_batch, graph_solves_per_s=ns / t_graph,
+               batched_over_plain=t_plain / t_batch, graph_over_plain=t_plain / t_graph,
+               batched_over_one_launch=t_one / t_batch,
+               tuned=dict(lanes_per_scan=tune.lanes_per_scan, cols_per_wg=tune.cols_per_wg),
+               autotune_ms={f"{l}x{c}": round(ms, 4) for l, c, ms in table},
+               roofline=roofline_rec(flop, t_batch, "whole step, batched persistent",
+                                     flop_per_solve_mean=flop / ns))
+    if cpu:
+        rec["cpu_baseline"] = cpu("layered:ragged_16384:64")
+    return rec
 
 
-def bench_c5(dk, configs, np, torch, chunk=40960):
+def bench_c5(dk, configs, np, torch, cpu, chunk=40960):
     """voronoi_100: 100 Voronoi realizations (9 layers x 100 Vs cells + 50 ANI cells
     per layer, projected onto 64x64 columns x 25 nodes) = 409,600 node columns x 30
     periods.  Per chunk of `chunk` columns one graph replay computes c, the
@@ -187,12 +341,22 @@ def bench_c5(dk, configs, np, torch, chunk=40960):
     nb = ctypes.c_int64(0)
     pool.dazi_dk_pool_bytes(ctypes.byref(nb))
     solves = N * per.numel()
-    return dict(workload="voronoi_100", realizations=100, columns=N, solves=solves,
-                seconds=dt, solves_per_s=solves / dt, chunk_columns=chunk,
-                chunks=int(np.ceil(N / chunk)), outputs="c, node kernels, Lsen_Gsc, VJP",
-                torch_peak_bytes=int(torch.cuda.max_memory_allocated()),
-                library_pool_bytes=int(nb.value), misfit=float(misfit),
-                cell_grad_norm=float(acc.norm()))
+    L = 49
+    rec = dict(workload="voronoi_100", realizations=100, columns=N, solves=solves,
+               seconds=dt, solves_per_s=solves / dt, chunk_columns=chunk,
+               chunks=int(np.ceil(N / chunk)), outputs="c, node kernels, Lsen_Gsc, VJP",
+               torch_peak_bytes=int(torch.cuda.max_memory_allocated()),
+               library_pool_bytes=int(nb.value), misfit=float(misfit),
+               cell_grad_norm=float(acc.norm()),
+               roofline=roofline_rec(solves * (N_EV + N_K_ISO + N_K_TI) * FLOP_PER_LAYER_EVAL * L,
+                                     dt, "whole population: stage + VJP + cell gradients",
+                                     flop_per_solve=(N_EV + N_K_ISO + N_K_TI) *
+                                     FLOP_PER_LAYER_EVAL * L))
+    if cpu:
+        rec["cpu_baseline"] = cpu("node:voronoi_100:192:1")
+        rec["cpu_baseline"]["note"] = ("the reference has no VJP: the sample times its "
+                                       "depthkernel and depthkernelTI on Voronoi columns")
+    return rec
 
 
 def bench_grows(dk, configs, np, torch, vel, out, nvx, nvz, reps=5):
@@ -369,6 +533,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-cols", type=int, default=640,
                     help="columns of the CPU baseline sample (about 10-20 s at 16 threads)")
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -382,14 +547,13 @@ def main():
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device-to-device copies of MIB MiB (calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters against known bytes)")
-    ap.add_argument("--cpu-baseline-child", type=int, default=0)
+    ap.add_argument("--cpu-baseline-child", default="")
     ap.add_argument("--lanes", type=int, default=16)
     ap.add_argument("--cpw", type=int, default=16)
     args = ap.parse_args()
 
-    threads = args.threads or min(16, os.cpu_count() or 1)
     if args.cpu_baseline_child:
-        cpu_baseline_child(args.cpu_baseline_child, threads)
+        cpu_baseline_child(args.cpu_baseline_child, args.threads or host_cpus()[0])
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this stage shards columns across GPUs with no "
@@ -412,6 +576,11 @@ def main():
 
     import dazi_current_release_amd as dk
     from dazi_current_release_amd import configs, _lib
+
+    threads, cpuinfo = host_cpus()
+    if args.threads:
+        threads = args.threads
+    cpu = None if args.no_cpu else (lambda spec: run_cpu_baseline(spec, threads, cpuinfo))
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -481,12 +650,13 @@ def main():
             t = json.load(fh)
         if t.get("workload") == "plain_4096":
             traffic, traffic_src = t["bytes_per_launch"], t["source"]
-    roofline = dict(bound="mfma", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12,
+    roofline = dict(bound="valu", achieved=achieved / 1e12, peak=PEAK_FP64 / 1e12,
                     unit="TFLOP/s", frac=achieved / PEAK_FP64, traffic=traffic,
-                    traffic_source=traffic_src,
-                    kernel=dom, pipe="FP64 VALU (no MFMA: not a contraction)",
+                    traffic_source=traffic_src, kernel=dom, pipe=PIPE,
                     flop_per_launch=flop_dom, avg_launch_ms=dom_ms,
-                    step_frac=flop_step / (ms_per_step * 1e-3) / PEAK_FP64)
+                    step_frac=flop_step / (ms_per_step * 1e-3) / PEAK_FP64,
+                    phase_frac={k: nsolves * n * FLOP_PER_LAYER_EVAL * L / (ms * 1e-3) / PEAK_FP64
+                                for k, (ms, n) in phases.items()})
 
     plain = None
     if not args.no_plain:
@@ -502,15 +672,18 @@ def main():
         plain = dict(solves_per_s=nsolves / plain_s, ms_per_step=1e3 * plain_s,
                      launches_per_step=int(np.ceil(ncol / 64)))
 
+    c1 = None
+    if not args.no_c1:
+        c1 = bench_c1(dk, configs, np, torch, cpu)
     vjp = None
     if not args.no_vjp:
-        vjp = bench_vjp(dk, configs, np, torch)
+        vjp = bench_vjp(dk, configs, np, torch, cpu)
     c3 = None
     if not args.no_c3:
-        c3 = bench_c3(dk, configs, np, torch)
+        c3 = bench_c3(dk, configs, np, torch, cpu)
     c5 = None
     if not args.no_c5:
-        c5 = bench_c5(dk, configs, np, torch)
+        c5 = bench_c5(dk, configs, np, torch, cpu)
     host = None
     if not args.no_host:
         host = bench_host_path(dk, configs, np)
@@ -523,20 +696,9 @@ def main():
         trav = bench_travel(dk, configs, np, torch, out, W["nx"], W["ny"], threads,
                             nsta=args.stations)
 
-    cpu = None
-    if not args.no_cpu:
-        try:
-            cb = run_cpu_baseline(args.cpu_cols, threads)
-            cpu = dict(value=cb["value"], unit="solves/s", cores=cb["threads"], kind=cb["kind"],
-                       sample=(f"first {cb['cols']} columns of plain_4096 ({cb['solves']} "
-                               f"column-period solves): reference depthkernel, FD node "
-                               f"kernels = 151 surfdisp96 curves per column, OpenMP "
-                               f"{cb['iso_seconds']:.1f} s + reference depthkernel_ti_column "
-                               f"(surfdisp96 + tregn96) in a process pool "
-                               f"{cb['ti_seconds']:.1f} s"))
-        except Exception as e:  # the GPU figure stands on its own
-            cpu = dict(value=None, unit="solves/s", cores=threads, kind="reference",
-                       sample=f"failed: {e}")
+    cpu_main = None
+    if cpu:
+        cpu_main = cpu(f"node:plain_4096:{args.cpu_cols}:1")
 
     rec = dict(
         metric=METRIC, value=value, unit="solves/s", n_gpus=1, steps=args.steps,
@@ -552,9 +714,12 @@ def main():
                     zero_periods=int((pv == 0).sum()),
                     plain=plain,
                     batched_over_plain=(value / plain["solves_per_s"]) if plain else None,
-                    vjp=vjp, ragged=c3, realizations=c5, host_path=host, g_rows=grow,
-                    travel=trav),
-        roofline=roofline, cpu_baseline=cpu)
+                    step_roofline=roofline_rec(flop_step, ms_per_step * 1e-3,
+                                               "whole step (prep, root search, kernels, TI)",
+                                               flop_per_solve=flop_step // nsolves),
+                    plain_1024=c1, vjp=vjp, ragged=c3, realizations=c5, host_path=host,
+                    g_rows=grow, travel=trav),
+        roofline=roofline, cpu_baseline=cpu_main)
     print(json.dumps(rec), flush=True)
 
 

@@ -17,6 +17,8 @@ DAZI_DK_EHIP = 9302
 DAZI_DK_ELIMIT = 9303
 DAZI_DK_ENODEV = 9304
 NOALLOC_SCRATCH_BYTES = 32 << 20
+DEPZ_UNCHANGED = 1
+TI_FAILED_NAN = 2
 
 EXPORTS = [
     "dazi_dk_abi_version", "dazi_dk_last_error", "dazi_dk_set_tune", "dazi_dk_get_tune",
@@ -38,7 +40,7 @@ class DaziDkError(RuntimeError):
 class Desc(C.Structure):
     _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("kmax", C.c_int32),
                 ("iwave", C.c_int32), ("igr", C.c_int32), ("tile_columns", C.c_int32),
-                ("vp_rho_mode", C.c_int32), ("minthk0", C.c_float), ("reserved", C.c_int32)]
+                ("vp_rho_mode", C.c_int32), ("minthk0", C.c_float), ("flags", C.c_int32)]
 
 
 class Tune(C.Structure):
@@ -139,7 +141,8 @@ def make_tune(lanes_per_scan=0, cols_per_wg=0, mode=0, tile_columns=0, refine_to
 
 
 def last_timings():
-    """[prep, root search, iso kernels, TI kernels] in ms and the launch count."""
-    buf = (C.c_double * 5)()
-    n = lib().dazi_dk_last_timings(buf, 5)
+    """[prep, root search, iso kernels, TI kernels] in ms, the root-search launch
+    count, the refined layers per column and the TI launch's device threads."""
+    buf = (C.c_double * 7)()
+    n = lib().dazi_dk_last_timings(buf, 7)
     return list(buf)[:n]

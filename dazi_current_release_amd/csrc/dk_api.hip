@@ -43,7 +43,6 @@ int set_err(int code, const char* fmt, ...) {
 enum Slot {
   S_LAYERS,
   S_BTP,  // btp, then btr = btp^-2.275, one double each per layer
-  S_GEO,
   S_CC,
   S_BETMX,
   S_BAD,
@@ -51,7 +50,6 @@ enum Slot {
   S_STATUS,
   S_WORK,
   S_TAPE,
-  S_NSUB,
   S_H_VEL,
   S_H_VP,
   S_H_RHO,
@@ -85,6 +83,19 @@ enum Slot {
   S_COUNT
 };
 
+// One node geometry: the depth values and minthk it was derived from, its
+// refined layer count L, the sublayer count of every node interval (device,
+// nz - 1 ints) and the layer records that every column shares (device: L
+// FlatGeom records, then the nz depths they were built from).
+struct Geometry {
+  std::vector<float> depz;
+  float minthk = 0.0f;
+  int L = 0;
+  void* nsub = nullptr;
+  void* recs = nullptr;
+  bool built = false;  // k_prep_geom has been enqueued for recs
+};
+
 struct Pool {
   int dev = -1;
   void* p[S_COUNT] = {};
@@ -97,25 +108,19 @@ struct Pool {
   hipStream_t side = nullptr;
   hipEvent_t fk[4] = {};
   int nforward = 0;
+  int last_L = 0, last_ti_threads = 0;  // refined layers and TI threads of the last call
   bool have_events = false;
-  // node geometry of the last call that took depz as a device pointer: the
-  // sublayer counts per node interval, resident in the S_NSUB slot
-  const float* geo_depz = nullptr;
-  int geo_nz = 0;
-  float geo_minthk = 0.0f;
-  int geo_L = 0;
-  bool geo_valid = false;
-  // what the S_NSUB and S_GEO slots hold: the sublayer counts, and the shared
-  // layer records (k_prep_geom) of the node geometry (depz values, minthk, L)
-  // last computed; geo_table is set once those records are launched.  A new
-  // geometry gets new buffers (the old ones are retired, not overwritten), so a
-  // graph captured with the old geometry keeps reading its own.
-  std::vector<int> nsub_host;
-  std::vector<float> tab_depz;
-  float tab_minthk = 0.0f;
-  int tab_L = 0;
-  bool geo_table = false;
+  // every node geometry met on this device (never freed: a captured graph
+  // may read any of them), and the one of the last call that took depz as a
+  // device pointer, keyed by that pointer, nz and minthk
+  std::vector<std::unique_ptr<Geometry>> geos;
+  Geometry* cur = nullptr;
+  const float* cur_depz = nullptr;
+  int cur_nz = 0;
+  float cur_minthk = 0.0f;
+  size_t geo_bytes = 0;
   std::vector<void*> retired;  // buffers replaced by growth (see pool_get)
+  size_t retired_bytes = 0;
 };
 
 Pool g_pool[16];
@@ -141,27 +146,19 @@ int get_pool(Pool** out) {
 
 // A slot only grows (by at least half its size).  The buffer it replaces may
 // still be read by work in flight or by a captured HIP graph, so it is retired
-// rather than freed: retired buffers live until the process ends, at most about
-// twice the final size of each slot.
-// A new buffer for slot s (the previous one is retired).
-int pool_new(Pool* P, Slot s, size_t bytes, void** out) {
-  if (bytes == 0) bytes = 8;
-  void* q = nullptr;
-  HIPCHK(hipMalloc(&q, bytes));
-  if (P->p[s]) P->retired.push_back(P->p[s]);
-  P->p[s] = q;
-  P->n[s] = bytes;
-  *out = q;
-  return 0;
-}
-
+// rather than freed: retired buffers live until the process ends; since each
+// growth is at least 1.5x, a slot's retired buffers add up to at most twice
+// its final size.
 int pool_get(Pool* P, Slot s, size_t bytes, void** out) {
   if (bytes == 0) bytes = 8;
   if (P->n[s] < bytes) {
     size_t nb = std::max(bytes, P->n[s] + P->n[s] / 2);
     void* q = nullptr;
     HIPCHK(hipMalloc(&q, nb));
-    if (P->p[s]) P->retired.push_back(P->p[s]);
+    if (P->p[s]) {
+      P->retired.push_back(P->p[s]);
+      P->retired_bytes += P->n[s];
+    }
     P->p[s] = q;
     P->n[s] = nb;
   }
@@ -273,50 +270,83 @@ int ti_threads(Pool* P, int64_t nsolves) {
 // depz_host NULL, depz_dev is read back once per (pointer, nz, minthk) and the
 // result is reused by later calls, which then enqueue kernels only (no host
 // synchronisation: such calls can be captured in a HIP graph).
+// The node geometry of a call (L, sublayer counts, shared layer records),
+// keyed on the values of depz and minthk, not on the address of depz.  A call
+// on device pointers whose (depz pointer, nz, minthk) matches the previous
+// such call reads depz back and compares its values, unless the caller
+// declares them unchanged (DAZI_DK_DEPZ_UNCHANGED) or the stream is being
+// captured (no host synchronisation is possible there); the prep kernels then
+// compare depz with the geometry's own copy on the device and mark every
+// column of a changed depz with status DK_STATUS_GEOMETRY.  A geometry met
+// before is reused with its buffers; a new one gets buffers of its own.
 int geometry(Pool* P, const dazi_dk_desc* d, const float* depz_dev, const float* depz_host,
-             hipStream_t st, int* L, void** nsubd) {
-  if (!depz_host && P->geo_valid && P->geo_depz == depz_dev && P->geo_nz == d->nz &&
-      P->geo_minthk == d->minthk0) {
-    *L = P->geo_L;
-    return pool_get(P, S_NSUB, (size_t)(d->nz - 1) * 4, nsubd);
-  }
+             hipStream_t st, Geometry** out) {
   std::vector<float> dz;
   const bool from_dev = depz_host == nullptr;
+  if (from_dev && P->cur && P->cur_depz == depz_dev && P->cur_nz == d->nz &&
+      P->cur_minthk == d->minthk0) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(st, &cs));
+    bool same = (d->flags & DAZI_DK_DEPZ_UNCHANGED) || cs != hipStreamCaptureStatusNone;
+    if (!same) {
+      dz.resize(d->nz);
+      HIPCHK(hipMemcpyAsync(dz.data(), depz_dev, sizeof(float) * d->nz, hipMemcpyDeviceToHost,
+                            st));
+      HIPCHK(hipStreamSynchronize(st));
+      same = std::memcmp(dz.data(), P->cur->depz.data(), sizeof(float) * d->nz) == 0;
+    }
+    if (same) {
+      *out = P->cur;
+      return 0;
+    }
+  }
   if (from_dev) {
-    dz.resize(d->nz);
-    HIPCHK(hipMemcpyAsync(dz.data(), depz_dev, sizeof(float) * d->nz, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    if (dz.empty()) {
+      dz.resize(d->nz);
+      HIPCHK(hipMemcpyAsync(dz.data(), depz_dev, sizeof(float) * d->nz, hipMemcpyDeviceToHost,
+                            st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
     depz_host = dz.data();
   }
-  std::vector<int> nsub;
-  int Lc = node_geometry(d->nz, depz_host, d->minthk0, &nsub);
-  if (Lc < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
-  if (Lc > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", Lc);
-  P->geo_valid = false;
-  if (nsub == P->nsub_host && P->p[S_NSUB]) {
-    *nsubd = P->p[S_NSUB];
-  } else {
-    P->nsub_host.clear();
-    if (int rc = pool_new(P, S_NSUB, nsub.size() * 4, nsubd)) return rc;
-    HIPCHK(hipMemcpyAsync(*nsubd, nsub.data(), nsub.size() * 4, hipMemcpyHostToDevice, st));
+  Geometry* G = nullptr;
+  for (auto& g : P->geos)
+    if (g->minthk == d->minthk0 && g->depz.size() == (size_t)d->nz &&
+        std::memcmp(g->depz.data(), depz_host, sizeof(float) * d->nz) == 0) {
+      G = g.get();
+      break;
+    }
+  if (!G) {
+    std::vector<int> nsub;
+    int Lc = node_geometry(d->nz, depz_host, d->minthk0, &nsub);
+    if (Lc < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
+    if (Lc > 4096) return set_err(DAZI_DK_ELIMIT, "refined model has %d layers (limit 4096)", Lc);
+    auto g = std::make_unique<Geometry>();
+    g->depz.assign(depz_host, depz_host + d->nz);
+    g->minthk = d->minthk0;
+    g->L = Lc;
+    const size_t nb = nsub.size() * 4, rb = (size_t)Lc * dk::PREP_GEOM_BYTES + (size_t)d->nz * 4;
+    HIPCHK(hipMalloc(&g->nsub, nb));
+    if (hipMalloc(&g->recs, rb) != hipSuccess) {
+      (void)hipFree(g->nsub);
+      return set_err(DAZI_DK_EHIP, "hipMalloc of %zu bytes of layer records failed", rb);
+    }
+    P->geo_bytes += nb + rb;
+    Geometry* gp = g.get();
+    P->geos.push_back(std::move(g));
+    HIPCHK(hipMemcpyAsync(gp->nsub, nsub.data(), nb, hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));  // nsub is a host temporary
-    P->nsub_host = nsub;
-  }
-  std::vector<float> dzv(depz_host, depz_host + d->nz);
-  if (!(dzv == P->tab_depz && d->minthk0 == P->tab_minthk && Lc == P->tab_L)) {
-    P->tab_depz = dzv;
-    P->tab_minthk = d->minthk0;
-    P->tab_L = Lc;
-    P->geo_table = false;
+    G = gp;
   }
   if (from_dev) {
-    P->geo_depz = depz_dev;
-    P->geo_nz = d->nz;
-    P->geo_minthk = d->minthk0;
-    P->geo_L = Lc;
-    P->geo_valid = true;
+    P->cur = G;
+    P->cur_depz = depz_dev;
+    P->cur_nz = d->nz;
+    P->cur_minthk = d->minthk0;
+  } else {
+    P->cur = nullptr;
   }
-  *L = Lc;
+  *out = G;
   return 0;
 }
 
@@ -328,6 +358,10 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
                double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st) {
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
+  // plain mode's tile: the para.in control (pdsurf_tile_columns /
+  // ti_tile_columns, CalSurfG.f90:173, depthkernelTI.f90:250) unless the
+  // caller's own tune sets one
+  if (d->tile_columns > 0 && !(tune && tune->tile_columns > 0)) T.tile_columns = d->tile_columns;
   if (d->iwave != 2) return set_err(DAZI_DK_EINVAL, "iwave=%d: only Rayleigh (2) is supported", d->iwave);
   if (d->igr != 0) return set_err(DAZI_DK_EINVAL, "igr=%d: only phase velocity (0) is supported", d->igr);
   if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
@@ -341,22 +375,18 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     return set_err(DAZI_DK_EINVAL, "sen_vs, sen_vp, sen_rho must be all set or all NULL");
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  int L;
-  void* nsubd;
-  if (int rc = geometry(P, d, depz_dev, depz_host, st, &L, &nsubd)) return rc;
+  Geometry* G;
+  if (int rc = geometry(P, d, depz_dev, depz_host, st, &G)) return rc;
+  const int L = G->L;
+  void* nsubd = G->nsub;
+  void* geo = G->recs;
+  const float* geo_depz = (const float*)((const char*)geo + (size_t)L * dk::PREP_GEOM_BYTES);
   int ncol = d->nx * d->ny;
   int64_t nsolves = (int64_t)ncol * d->kmax;
   void *layers, *btp, *cc, *betmx, *bad, *croot, *stat, *work;
   if (int rc = pool_get(P, S_LAYERS, (size_t)ncol * L * 32, &layers)) return rc;
   if (int rc = pool_get(P, S_BTP, (size_t)ncol * L * 16, &btp)) return rc;
   double* btr = (double*)btp + (size_t)ncol * L;
-  // the shared layer records of the node geometry, built in a buffer of its
-  // own when the geometry changes
-  void* geo = P->p[S_GEO];
-  if (!P->geo_table || !geo) {
-    P->geo_table = false;
-    if (int rc = pool_new(P, S_GEO, (size_t)L * dk::PREP_GEOM_BYTES, &geo)) return rc;
-  }
   if (int rc = pool_get(P, S_CC, (size_t)ncol * 8, &cc)) return rc;
   if (int rc = pool_get(P, S_BETMX, (size_t)ncol * 4, &betmx)) return rc;
   if (int rc = pool_get(P, S_BAD, (size_t)ncol * 4, &bad)) return rc;
@@ -392,10 +422,12 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     if (int rc = pool_get(P, S_TI_QUEUE, (size_t)(1 + nsolves) * 4, &tiq)) return rc;
   }
   P->have_events = g_timing;
+  P->last_L = L;
+  P->last_ti_threads = lsen ? ti_nthr : 0;
   rec(P, 0, st);
-  if (!P->geo_table) {
-    HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, st));
-    P->geo_table = true;
+  if (!G->built) {
+    HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, (float*)geo_depz, st));
+    G->built = true;
   }
   if (lsen) {
     HIPCHK(hipEventRecord(P->fk[0], st));
@@ -406,8 +438,9 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     HIPCHK(hipEventRecord(P->fk[2], P->side));
   }
   HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                               d->vp_rho_mode == 1 ? rho : nullptr, L, geo, (double4*)layers,
-                               (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
+                               d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, geo_depz, L, geo,
+                               (double4*)layers, (double*)btp, btr, (double*)cc, (float*)betmx,
+                               (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = ncol;
@@ -444,7 +477,8 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     // join the TI model prep, then the TI kernels (the roots are ready)
     HIPCHK(hipStreamWaitEvent(st, P->fk[2], 0));
     HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
-                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, (int*)tiq, st));
+                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, (int*)tiq,
+                         (d->flags & DAZI_DK_TI_FAILED_NAN) != 0, st));
   }
   rec(P, 4, st);
   return 0;
@@ -550,14 +584,15 @@ int32_t dazi_dk_last_timings(double* out, int32_t n) {
   Pool* P;
   if (int rc = get_pool(&P)) return -rc;
   if (!P->have_events || n < 1) return 0;
-  double v[5] = {0, 0, 0, 0, (double)P->nforward};
+  double v[7] = {0, 0, 0, 0, (double)P->nforward, (double)P->last_L,
+                 (double)P->last_ti_threads};
   for (int i = 0; i < 4; i++) {
     float ms = 0.f;
     if (hipEventSynchronize(P->ev[i + 1]) != hipSuccess) return 0;
     if (hipEventElapsedTime(&ms, P->ev[i], P->ev[i + 1]) != hipSuccess) return 0;
     v[i] = ms;
   }
-  int m = n < 5 ? n : 5;
+  int m = n < 7 ? n : 7;
   for (int i = 0; i < m; i++) out[i] = v[i];
   return m;
 }
@@ -979,7 +1014,7 @@ int32_t dazi_dk_travel(const dazi_dk_travel_desc* d, int64_t njob, const int32_t
 int32_t dazi_dk_pool_bytes(int64_t* out) {
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  int64_t n = 0;
+  int64_t n = (int64_t)(P->retired_bytes + P->geo_bytes);
   for (int s = 0; s < S_COUNT; s++) n += (int64_t)P->n[s];
   *out = n;
   return 0;
@@ -988,8 +1023,7 @@ int32_t dazi_dk_pool_bytes(int64_t* out) {
 int32_t dazi_dk_reset_geometry(void) {
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
-  P->geo_valid = false;
-  P->geo_table = false;
+  P->cur = nullptr;
   return 0;
 }
 
@@ -1152,6 +1186,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   }
   if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
   P->have_events = g_timing;
+  P->last_L = 0;
+  P->last_ti_threads = 0;
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,

@@ -5,7 +5,12 @@
 
 namespace dk {
 
-enum { DK_STATUS_BADMODEL = -1 };
+// Column status codes besides 0 and the 1-based first failed period: a model
+// outside the stage's class, and (graph replays only) a depz whose values
+// differ from those the captured node geometry was built from.
+enum { DK_STATUS_BADMODEL = -1, DK_STATUS_GEOMETRY = -2 };
+// Prep flag per column (BatchDev::bad): 0 ok, 1 bad model, 2 changed geometry.
+enum { DK_PREP_BADMODEL = 1, DK_PREP_GEOMETRY = 2 };
 
 // A batch of columns with refined, flattened layered models.
 //   uniform = 1: every column has Lu layers (at col*Lu) and the Ku shared
@@ -69,19 +74,24 @@ constexpr int TI_SCRATCH_CX = 14;
 // them (queue holds 1 + ncol * K ints).
 hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
                           const void* geo, TiLayerDev* out, int* queue, hipStream_t st);
+// nan_failed: periods without a phase velocity get NaN in every interval
+// (the reference's tregn96 at c = 0) instead of 0.
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* queue, hipStream_t st);
+                     double* scratch, int nthr, int* queue, int nan_failed, hipStream_t st);
 
 // launch_prep_geom writes the L flattening and node-mapping records that
 // every column of a node geometry (depz, minthk0) shares to geo
 // (PREP_GEOM_BYTES each); launch_prep_nodes reads them for the columns' layers.
 constexpr size_t PREP_GEOM_BYTES = 80;
+// launch_prep_geom also copies depz to geo_depz; launch_prep_nodes compares
+// depz with that copy and flags every column DK_PREP_GEOMETRY on a mismatch.
 hipError_t launch_prep_geom(int nz, const float* depz, float minthk0, int L, void* geo,
-                            hipStream_t st);
+                            float* geo_depz, hipStream_t st);
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             int L, const void* geo, double4* layers, double* btp, double* btr,
-                             double* cc, float* betmx, int* bad, hipStream_t st);
+                             const float* depz, const float* geo_depz, int L, const void* geo,
+                             double4* layers, double* btp, double* btr, double* cc, float* betmx,
+                             int* bad, hipStream_t st);
 hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, const float* thk,
                               const float* vp, const float* vs, const float* rho,
                               const double* thk64, const double* vp64, const double* vs64,

@@ -218,10 +218,11 @@ __device__ void flatten_column_f64(double4* lay, double* btp_out, double* btr_ou
 // layer m and sums the thicknesses in the order a per-column pass would, so
 // every value is the one flat_push (and ti_geom's recurrence) computes.
 __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk0, int L,
-                            FlatGeom* __restrict__ geo) {
+                            FlatGeom* __restrict__ geo, float* __restrict__ geo_depz) {
 #pragma clang fp contract(off)
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= L) return;
+  if (m < nz) geo_depz[m] = depz[m];  // L >= nz: every depth is copied
   double dr = 0.0, dr0 = 0.0;
   int k = 0, node = -1, sub = 0, nsm = 0;
   for (int i = 0; i < nz - 1 && k <= m; i++) {
@@ -277,12 +278,19 @@ __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk
 // betmx and the bad flags are wave reductions.
 __global__ void __launch_bounds__(256) k_prep_nodes(
     int ncol, int nz, const float* __restrict__ vel, const float* __restrict__ vpin,
-    const float* __restrict__ rhoin, int L, const FlatGeom* __restrict__ geo,
+    const float* __restrict__ rhoin, const float* __restrict__ depz,
+    const float* __restrict__ geo_depz, int L, const FlatGeom* __restrict__ geo,
     double4* __restrict__ layers, double* __restrict__ btp, double* __restrict__ btr,
     double* __restrict__ cc, float* __restrict__ betmx, int* __restrict__ bad_out) {
 #pragma clang fp contract(off)
   const int lane = threadIdx.x & 63;
   const int nw = gridDim.x * (blockDim.x >> 6);
+  // the records were built from geo_depz: a depz with other values (a graph
+  // replayed after depz changed) must not use them
+  bool moved = false;
+  for (int i = lane; i < nz; i += 64)
+    moved = moved || (__float_as_uint(depz[i]) != __float_as_uint(geo_depz[i]));
+  moved = __ballot(moved) != 0ull;
   auto node_vals = [&](int col, int i, float* vs, float* vp, float* rho) {
     const size_t o = (size_t)i * ncol + col;
     *vs = vel[o];
@@ -366,7 +374,7 @@ __global__ void __launch_bounds__(256) k_prep_nodes(
     if (lane == 0) {
       cc[col] = (double)cc1;
       betmx[col] = bmx;
-      bad_out[col] = (bad || jsol0) ? 1 : 0;
+      bad_out[col] = moved ? DK_PREP_GEOMETRY : (bad || jsol0) ? DK_PREP_BADMODEL : 0;
     }
   }
 }
@@ -658,7 +666,10 @@ __global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int 
         B.croot[s] = 0.0;
         B.pv[s] = 0.0;
       }
-      if (r == 0) B.status[col] = B.bad[col] ? DK_STATUS_BADMODEL : 0;
+      if (r == 0)
+        B.status[col] = B.bad[col] == DK_PREP_GEOMETRY ? DK_STATUS_GEOMETRY
+                        : B.bad[col]                    ? DK_STATUS_BADMODEL
+                                                        : 0;
       st = ST_DONE;
       return;
     }
@@ -1565,19 +1576,20 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride) {
 }
 
 hipError_t launch_prep_geom(int nz, const float* depz, float minthk0, int L, void* geo,
-                            hipStream_t st) {
+                            float* geo_depz, hipStream_t st) {
   hipLaunchKernelGGL(k_prep_geom, dim3((L + 63) / 64), dim3(64), 0, st, nz, depz, minthk0, L,
-                     (FlatGeom*)geo);
+                     (FlatGeom*)geo, geo_depz);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_nodes(int ncol, int nz, const float* vel, const float* vp, const float* rho,
-                             int L, const void* geo, double4* layers, double* btp, double* btr,
-                             double* cc, float* betmx, int* bad, hipStream_t st) {
+                             const float* depz, const float* geo_depz, int L, const void* geo,
+                             double4* layers, double* btp, double* btr, double* cc, float* betmx,
+                             int* bad, hipStream_t st) {
   // four columns per 256-thread workgroup (one per wavefront)
   const int grid = (ncol + 3) / 4;
-  hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(256), 0, st, ncol, nz, vel, vp, rho, L,
-                     (const FlatGeom*)geo, layers, btp, btr, cc, betmx, bad);
+  hipLaunchKernelGGL(k_prep_nodes, dim3(grid), dim3(256), 0, st, ncol, nz, vel, vp, rho, depz,
+                     geo_depz, L, (const FlatGeom*)geo, layers, btp, btr, cc, betmx, bad);
   return hipGetLastError();
 }
 

@@ -115,7 +115,7 @@ template <class G_>
 __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* __restrict__ layers,
                       const double* __restrict__ periods, const double* __restrict__ pv,
                       const int* __restrict__ nsub, float* __restrict__ lsen, double* scratch,
-                      long stride) {
+                      long stride, int nan_failed) {
   const float twopi = 2.0f * 3.141592654f;
   const int col = (int)(s % ncol), k = (int)(s / ncol);
   const TiLayerDev* lay = layers + (size_t)col * L;
@@ -123,7 +123,8 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   const double c = (double)(float)pv[s];
   if (!(c > 0.0)) {
     // no phase velocity for this period (root search failed): no kernel
-    for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+    const float fill = nan_failed ? __builtin_nanf("") : 0.0f;
+    for (int j = 0; j < nz - 1; j++) out(j) = fill;
     return 0;
   }
   const double omega = (double)twopi / (double)(float)periods[k];
@@ -164,13 +165,14 @@ __global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
                                             const double* __restrict__ periods,
                                             const double* __restrict__ pv,
                                             const int* __restrict__ nsub, float* __restrict__ lsen,
-                                            double* __restrict__ scratch, int* __restrict__ queue) {
+                                            double* __restrict__ scratch, int* __restrict__ queue,
+                                            int nan_failed) {
   const long nthr = (long)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const long nsolves = (long)ncol * K;
   for (long s = tid; s < nsolves; s += nthr) {
     const int rc = ti_one<ti::Eig>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen,
-                                   scratch + tid, nthr);
+                                   scratch + tid, nthr, nan_failed);
     if (rc == -2) queue[1 + atomicAdd(queue, 1)] = (int)s;
   }
 }
@@ -188,7 +190,7 @@ __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
   const int n = queue[0];
   for (long i = tid; i < n; i += nthr)
     ti_one<ti::EigC>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen,
-                     scratch + tid, nthr);
+                     scratch + tid, nthr, 0);
 }
 
 hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
@@ -207,12 +209,12 @@ int ti_cx_threads(int nthr) {
 
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* queue, hipStream_t st) {
+                     double* scratch, int nthr, int* queue, int nan_failed, hipStream_t st) {
   int block = 128;
   int grid = nthr / block;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_ti, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods, pv,
-                     nsub, lsen, scratch, queue);
+                     nsub, lsen, scratch, queue, nan_failed);
   if (hipError_t e = hipGetLastError()) return e;
   // the queued complex-pair solves; the grid is sized for the worst case and
   // exits at once when the queue is empty

@@ -42,7 +42,7 @@ module dazi_dk
   type, bind(c) :: dazi_dk_desc
     integer(c_int32_t) :: nx, ny, nz, kmax, iwave, igr, tile_columns, vp_rho_mode
     real(c_float) :: minthk0
-    integer(c_int32_t) :: reserved
+    integer(c_int32_t) :: flags
   end type dazi_dk_desc
 
   type, bind(c) :: dazi_dk_tune
@@ -144,6 +144,19 @@ module dazi_dk
       integer(c_int32_t) :: rc
     end function c_dazi_dk_travel
 
+    function c_dazi_dk_enable_timing(on) bind(c, name='dazi_dk_enable_timing') result(rc)
+      import :: c_int32_t
+      integer(c_int32_t), value :: on
+      integer(c_int32_t) :: rc
+    end function c_dazi_dk_enable_timing
+
+    function c_dazi_dk_last_timings(out, n) bind(c, name='dazi_dk_last_timings') result(m)
+      import :: c_int32_t, c_double
+      real(c_double), intent(out) :: out(*)
+      integer(c_int32_t), value :: n
+      integer(c_int32_t) :: m
+    end function c_dazi_dk_last_timings
+
     function c_dazi_dk_last_error() bind(c, name='dazi_dk_last_error') result(p)
       import :: c_ptr
       type(c_ptr) :: p
@@ -190,6 +203,15 @@ contains
     end if
     if (env_int < 1) env_int = 1
   end function env_int
+
+  ! True when environment variable `name` is set to a non-empty value other than 0.
+  logical function env_on(name)
+    character(len=*), intent(in) :: name
+    character(len=64) :: v
+    integer :: st
+    call get_environment_variable(name, v, status=st)
+    env_on = st == 0 .and. len_trim(v) > 0 .and. trim(adjustl(v)) /= '0'
+  end function env_on
 
   subroutine log_line(text)
     character(len=*), intent(in) :: text
@@ -309,16 +331,47 @@ contains
     type(dazi_dk_desc) :: d
     character(len=160) :: line
     character(len=:), allocatable :: dir
+    real(c_double) :: tm(7)
+    integer(c_int32_t) :: rc, nt, lref
+    integer(c_int32_t), parameter :: TI_FAILED_NAN = 2   ! DAZI_DK_TI_FAILED_NAN
 
     d = dazi_dk_desc(nx, ny, nz, kmaxRc, iwave, igr, &
                      env_int('DAZI_DEPTHKERNEL_TI_TILE_COLUMNS', 64), 0, minthk, 0)
+    ! the reference leaves NaN in Lsen_Gsc at periods without a phase velocity
+    ! (tregn96 at c = 0); on request this backend does the same, else 0
+    if (env_on('DAZI_DEPTHKERNEL_TI_FAILED_NAN')) d%flags = ior(d%flags, TI_FAILED_NAN)
     call log_line('Depth kernel TI backend: hip')
     write(line, '(A,I0,A,I0,A,I0,A,I0)') 'Depth kernel TI grid/periods: nx=', nx, ' ny=', ny, &
         ' nz=', nz, ' kmax=', kmaxRc
     call log_line(trim(line))
     write(line, '(A,I0)') 'Depth kernel TI batch tile columns: ', d%tile_columns
     call log_line(trim(line))
-    call check(c_dazi_dk_ti(d, vel, depz, tRc, pvRc, Lsen_Gsc), 'depthkernelTI')
+    rc = c_dazi_dk_enable_timing(1_c_int32_t)
+    rc = c_dazi_dk_ti(d, vel, depz, tRc, pvRc, Lsen_Gsc)
+    nt = 0
+    if (rc == DAZI_DK_OK) nt = c_dazi_dk_last_timings(tm, 7_c_int32_t)
+    lref = c_dazi_dk_enable_timing(0_c_int32_t)
+    call check(rc, 'depthkernelTI')
+    ! the reference's internal timing lines (depthkernelTI.f90:170-186), from
+    ! the library's event timings: model prep and root search, the TI kernels
+    ! (tregn96 and the splice are one kernel here), the refined layer count
+    ! (every column of a node model has the same) and the TI launch's threads
+    if (nt >= 7) then
+      lref = nint(tm(6))
+      call log_line('  depthkernelTI internal timing:')
+      write(line, '(a,f12.3,a)') '    refine+surfdisp96 phase velocity = ', &
+          (tm(1) + tm(2)) * 1.0d-3, ' s'
+      call log_line(trim(line))
+      write(line, '(a,f12.3,a)') '    tregn96 sensitivity kernel = ', tm(4) * 1.0d-3, ' s'
+      call log_line(trim(line))
+      write(line, '(a,f12.3,a)') '    splice refined kernel = ', 0.0d0, ' s'
+      call log_line(trim(line))
+      write(line, '(a,i0,a,i0,a,f10.2)') '    refined sublayers rmax min/max/avg = ', lref, '/', &
+          lref, '/', dble(lref)
+      call log_line(trim(line))
+      write(line, '(a,i0)') '    observed worker threads = ', nint(tm(7))
+      call log_line(trim(line))
+    end if
     if (dump_dir_of('DAZI_DEPTHKERNEL_TI_DUMP_DIR', 'Depth kernel TI dump dir: ', dir)) then
       call dump_f64(dir // '/pv_ti.bin', pvRc, nx*ny*kmaxRc)
       call dump_f32(dir // '/lsen_gsc_ti.bin', Lsen_Gsc, nx*ny*kmaxRc*(nz-1))

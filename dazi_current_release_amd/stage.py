@@ -42,6 +42,12 @@ def _env_int(name, default):
     return max(v, 1)
 
 
+def _env_on(name):
+    import os
+    v = os.environ.get(name, "").strip()
+    return bool(v) and v != "0"
+
+
 def _dump(var, label, log, arrays):
     """Writes arrays {file name: Fortran-ordered array} as raw stream files into
     the directory named by environment variable `var`, as the reference's
@@ -119,6 +125,76 @@ def _tptr(t):
     return None if t is None else C.c_void_p(t.data_ptr())
 
 
+def _check_dev(t, dtype, shape, name, optional=False):
+    """Raises ValueError unless t is a contiguous GPU tensor of the given dtype
+    and shape (the C ABI reads raw pointers: a wrong dtype, a strided view or a
+    short buffer would read or write out of bounds)."""
+    import torch
+    if t is None:
+        if optional:
+            return
+        raise ValueError(f"{name} is required")
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name} must be a torch tensor on the GPU")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if t.device.type != "cuda":
+        raise ValueError(f"{name} must be on the GPU, got {t.device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name} has shape {tuple(t.shape)}, expected {tuple(shape)}")
+
+
+def _check_node_inputs(vel, depz, periods, vp, rho):
+    """Node-model inputs of the device entry points: vel (nz, ncol) and
+    optional vp, rho (both or neither) float32, depz (nz,) float32, periods
+    (K,) float64; all contiguous on one GPU."""
+    import torch
+    if not isinstance(vel, torch.Tensor) or vel.dim() != 2:
+        raise ValueError("vel must be a (nz, ncol) float32 GPU tensor")
+    nz, ncol = vel.shape
+    _check_dev(vel, torch.float32, (nz, ncol), "vel")
+    _check_dev(depz, torch.float32, (nz,), "depz")
+    if not isinstance(periods, torch.Tensor) or periods.dim() != 1:
+        raise ValueError("periods must be a (K,) float64 GPU tensor")
+    _check_dev(periods, torch.float64, (periods.numel(),), "periods")
+    if (vp is None) != (rho is None):
+        raise ValueError("vp and rho must be given together")
+    _check_dev(vp, torch.float32, (nz, ncol), "vp", optional=True)
+    _check_dev(rho, torch.float32, (nz, ncol), "rho", optional=True)
+    for t in (depz, periods, vp, rho):
+        if t is not None and t.device != vel.device:
+            raise ValueError("all inputs must be on the same GPU")
+    return nz, ncol, periods.numel()
+
+
+# The depz tensor of the last device call on each GPU, with its address and
+# torch version counter at that call.
+_DEPZ_SEEN = {}
+
+
+def _depz_flags(depz):
+    """DAZI_DK_DEPZ_UNCHANGED when depz is the very tensor of the previous call
+    on its GPU and torch has recorded no write to it since; the library then
+    skips reading depz back to check its cached node geometry.  Holding the
+    tensor keeps its memory from being handed to another tensor meanwhile."""
+    key = (depz.data_ptr(), depz._version)
+    dev = depz.device.index
+    prev = _DEPZ_SEEN.get(dev)
+    _DEPZ_SEEN[dev] = (depz, key)
+    same = prev is not None and prev[0] is depz and prev[1] == key
+    return _lib.DEPZ_UNCHANGED if same else 0
+
+
+def _check_out(out, spec):
+    """Caller-given outputs: spec {key: (dtype, shape)}; every key must be present."""
+    for k, (dt, shape) in spec.items():
+        if k not in out:
+            raise ValueError(f"out is missing {k!r}")
+        _check_dev(out[k], dt, shape, f"out[{k!r}]")
+
+
 def _stream_ptr(stream):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -134,17 +210,16 @@ def iso_dev(vel, depz, periods, minthk, vp=None, rho=None, kernels=True, tune=No
     Returns dict(pv (K, ncol), sen_vs/sen_vp/sen_rho (nz, K, ncol) or None, status (ncol,)).
     """
     import torch
-    nz, ncol = vel.shape
-    K = periods.numel()
+    nz, ncol, K = _check_node_inputs(vel, depz, periods, vp, rho)
     dev = vel.device
+    spec = dict(pv=(torch.float64, (K, ncol)), status=(torch.int32, (ncol,)))
+    if kernels:
+        spec.update({k: (torch.float64, (nz, K, ncol)) for k in ("sen_vs", "sen_vp", "sen_rho")})
     if out is None:
-        out = dict(pv=torch.empty((K, ncol), dtype=torch.float64, device=dev),
-                   status=torch.empty(ncol, dtype=torch.int32, device=dev))
-        if kernels:
-            for k in ("sen_vs", "sen_vp", "sen_rho"):
-                out[k] = torch.empty((nz, K, ncol), dtype=torch.float64, device=dev)
+        out = {k: torch.empty(shape, dtype=dt, device=dev) for k, (dt, shape) in spec.items()}
+    _check_out(out, spec)
     mode = 1 if vp is not None else 0
-    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, mode, float(minthk), 0)
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, mode, float(minthk), _depz_flags(depz))
     t = tune if tune is not None else None
     rc = _lib.lib().dazi_dk_iso_dev(
         C.byref(d), None if t is None else C.byref(t), _tptr(vel), _tptr(vp), _tptr(rho),
@@ -289,11 +364,32 @@ def depthkernelTI(nx, ny, nz, vel, pvRc, iwave, igr, kmaxRc, tRc, depz, minthk, 
     if log is not None:
         print(f"Depth kernel TI backend: {_BACKEND_NAME}", file=log)
         print(f"Depth kernel TI grid/periods: nx={nx} ny={ny} nz={nz} kmax={kmaxRc}", file=log)
+    # NaN at periods without a phase velocity, as the reference leaves there
+    # (tregn96 at c = 0), on request; 0 otherwise
+    flags = _lib.TI_FAILED_NAN if _env_on("DAZI_DEPTHKERNEL_TI_FAILED_NAN") else 0
     d = _lib.Desc(nx, ny, nz, kmaxRc, iwave, igr,
-                  _env_int("DAZI_DEPTHKERNEL_TI_TILE_COLUMNS", 64), 0, float(minthk), 0)
-    rc = _lib.lib().dazi_dk_ti(C.byref(d), _ptr(vel), _ptr(depz), _ptr(tRc), _ptr(pvRc),
-                               _ptr(Lsen_Gsc))
+                  _env_int("DAZI_DEPTHKERNEL_TI_TILE_COLUMNS", 64), 0, float(minthk), flags)
+    L = _lib.lib()
+    if log is not None:
+        L.dazi_dk_enable_timing(1)
+    try:
+        rc = L.dazi_dk_ti(C.byref(d), _ptr(vel), _ptr(depz), _ptr(tRc), _ptr(pvRc),
+                          _ptr(Lsen_Gsc))
+        tm = _lib.last_timings() if (log is not None and rc == 0) else []
+    finally:
+        if log is not None:
+            L.dazi_dk_enable_timing(0)
     _lib.check(rc)
+    if len(tm) >= 7:
+        # the reference's internal timing lines (depthkernelTI.f90:170-186)
+        lref = int(round(tm[5]))
+        print("  depthkernelTI internal timing:", file=log)
+        print(f"    refine+surfdisp96 phase velocity = {(tm[0] + tm[1]) * 1e-3:12.3f} s", file=log)
+        print(f"    tregn96 sensitivity kernel = {tm[3] * 1e-3:12.3f} s", file=log)
+        print(f"    splice refined kernel = {0.0:12.3f} s", file=log)
+        print(f"    refined sublayers rmax min/max/avg = {lref}/{lref}/{float(lref):10.2f}",
+              file=log)
+        print(f"    observed worker threads = {int(round(tm[6]))}", file=log)
     _dump("DAZI_DEPTHKERNEL_TI_DUMP_DIR", "Depth kernel TI dump dir: ", log,
           {"pv_ti.bin": pvRc, "lsen_gsc_ti.bin": Lsen_Gsc})
 
@@ -317,18 +413,18 @@ def stage_dev(vel, depz, periods, minthk, vp=None, rho=None, iso=True, ti=True, 
     (nz,K,ncol) when iso, lsen (nz-1,K,ncol) float32 when ti, status (ncol,)).
     """
     import torch
-    nz, ncol = vel.shape
-    K = periods.numel()
+    nz, ncol, K = _check_node_inputs(vel, depz, periods, vp, rho)
     dev = vel.device
+    spec = dict(pv=(torch.float64, (K, ncol)), status=(torch.int32, (ncol,)))
+    if iso:
+        spec.update({k: (torch.float64, (nz, K, ncol)) for k in ("sen_vs", "sen_vp", "sen_rho")})
+    if ti:
+        spec["lsen"] = (torch.float32, (nz - 1, K, ncol))
     if out is None:
-        out = dict(pv=torch.empty((K, ncol), dtype=torch.float64, device=dev),
-                   status=torch.empty(ncol, dtype=torch.int32, device=dev))
-        if iso:
-            for k in ("sen_vs", "sen_vp", "sen_rho"):
-                out[k] = torch.empty((nz, K, ncol), dtype=torch.float64, device=dev)
-        if ti:
-            out["lsen"] = torch.empty((nz - 1, K, ncol), dtype=torch.float32, device=dev)
-    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 1 if vp is not None else 0, float(minthk), 0)
+        out = {k: torch.empty(shape, dtype=dt, device=dev) for k, (dt, shape) in spec.items()}
+    _check_out(out, spec)
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 1 if vp is not None else 0, float(minthk),
+                  _depz_flags(depz))
     rc = _lib.lib().dazi_dk_stage_dev(
         C.byref(d), None if tune is None else C.byref(tune), _tptr(vel), _tptr(vp), _tptr(rho),
         _tptr(depz), _tptr(periods), _tptr(out["pv"]),
@@ -348,14 +444,25 @@ def effective_dev(vel, sen_vs, sen_vp, sen_rho, grad_c=None, eff=True, grad=True
     vel (nz, ncol) f32, sen_* (nz, K, ncol) f64, grad_c (K, ncol) f64; returns
     dict(eff (nz-1, K, ncol), grad_vs (nz, ncol)) as requested."""
     import torch
+    if not isinstance(sen_vs, torch.Tensor) or sen_vs.dim() != 3:
+        raise ValueError("sen_vs must be a (nz, K, ncol) float64 GPU tensor")
     nz, K, ncol = sen_vs.shape
     dev = sen_vs.device
+    _check_dev(vel, torch.float32, (nz, ncol), "vel")
+    for t, n in ((sen_vs, "sen_vs"), (sen_vp, "sen_vp"), (sen_rho, "sen_rho")):
+        _check_dev(t, torch.float64, (nz, K, ncol), n)
+    _check_dev(grad_c, torch.float64, (K, ncol), "grad_c", optional=True)
+    shapes = dict(eff=(torch.float64, (nz - 1, K, ncol)), grad_vs=(torch.float64, (nz, ncol)))
     if out is None:
-        out = {}
-        if eff:
-            out["eff"] = torch.empty((nz - 1, K, ncol), dtype=torch.float64, device=dev)
-        if grad and grad_c is not None:
-            out["grad_vs"] = torch.empty((nz, ncol), dtype=torch.float64, device=dev)
+        keys = [k for k, want in (("eff", eff), ("grad_vs", grad and grad_c is not None)) if want]
+        out = {k: torch.empty(shapes[k][1], dtype=torch.float64, device=dev) for k in keys}
+    # with caller-given outputs, the keys present name the outputs computed
+    if "grad_vs" in out and grad_c is None:
+        raise ValueError("out['grad_vs'] needs grad_c")
+    _check_out(out, {k: v for k, v in shapes.items() if k in out})
+    for t in (vel, sen_vp, sen_rho, grad_c, *out.values()):
+        if t is not None and t.device != dev:
+            raise ValueError("all tensors must be on the same GPU")
     d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 0, 1.0, 0)
     rc = _lib.lib().dazi_dk_effective_dev(
         C.byref(d), _tptr(vel), _tptr(sen_vs), _tptr(sen_vp), _tptr(sen_rho), _tptr(grad_c),

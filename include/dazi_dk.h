@@ -52,12 +52,27 @@ typedef struct dazi_dk_desc {
   int32_t kmax;            /* number of periods (kmaxRc) */
   int32_t iwave;           /* 2 = Rayleigh: the only value the reference stage uses */
   int32_t igr;             /* 0 = phase velocity: the only value the reference stage uses */
-  int32_t tile_columns;    /* columns per launch in plain mode (default 64) */
+  int32_t tile_columns;    /* columns per launch in plain mode (tune.mode = 1), unless an
+                              explicit tune argument sets its own; 0 = 64.  The default
+                              persistent mode runs the whole batch in one launch */
   int32_t vp_rho_mode;     /* 0: vp, rho from Brocher(vs) as depthkernel does;
                               1: explicit vp and rho node arrays */
   float minthk0;           /* para.in "number of sublayers" (minthk) */
-  int32_t reserved;
+  int32_t flags;           /* 0, or DAZI_DK_DEPZ_UNCHANGED */
 } dazi_dk_desc;
+
+/* desc->flags bit for the device entry points: the caller guarantees that the
+ * values behind depz are those of the previous call with the same depz
+ * pointer, so the library may skip reading them back (a host synchronisation)
+ * to check its cached node geometry. */
+#define DAZI_DK_DEPZ_UNCHANGED 1
+
+/* desc->flags bit for the TI splice: periods without a phase velocity get NaN
+ * in every interval of Lsen_Gsc, as the reference's depthkernelTI produces
+ * there (it calls tregn96 with c = 0, depthkernelTI.f90:139-146); without it
+ * they get 0.  Either way the G builder drops them (abs(NaN) > ftol is false,
+ * CalSurfGAniso_Joint.f90:782-789). */
+#define DAZI_DK_TI_FAILED_NAN 2
 
 /* Launch configuration of the root search. Zero fields take defaults. */
 typedef struct dazi_dk_tune {
@@ -127,18 +142,26 @@ int32_t dazi_dk_iso(const dazi_dk_desc* desc, const float* vel, const float* vp,
 
 /* Same stage on device pointers, enqueued on `stream` (a hipStream_t, NULL =
  * default stream).  status(nx*ny) i32 may be NULL.  tune may be NULL.
- * The refined node geometry is derived from depz on the first call with a
- * given (depz pointer, nz, minthk) and kept on the device; later calls with
- * the same triple only enqueue kernels (no host synchronisation), so after one
- * such call an identical call can be captured in a HIP graph.  After
- * overwriting depz in place call dazi_dk_reset_geometry(). */
+ * The refined node geometry (sublayer counts, shared layer records) is derived
+ * from the values of depz and kept on the device.  A later call with the same
+ * (depz pointer, nz, minthk) reads the nz depths back (a host
+ * synchronisation) and reuses the geometry when they are unchanged; with
+ * desc->flags = DAZI_DK_DEPZ_UNCHANGED, or on a stream being captured, it only
+ * enqueues kernels, so after one such call an identical call can be captured
+ * in a HIP graph.  A captured graph keeps the geometry it was captured with:
+ * when a replay finds depz holding other values, every column gets status -2
+ * and zero outputs.  Status per column: 0, the 1-based first period without
+ * a root, -1 for a model outside the stage's class (fluid or non-positive
+ * layer), -2 as just described. */
 int32_t dazi_dk_iso_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, const float* vel,
                         const float* vp, const float* rho, const float* depz, const double* tRc,
                         double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
                         int32_t* status, void* stream);
 
-/* Forgets the cached node geometry of the current device (no reference
- * counterpart: the reference recomputes it on every call). */
+/* Makes the next device call on the current device read depz back and look
+ * its values up among the node geometries met (no reference counterpart: the
+ * reference recomputes the geometry on every call).  Not needed for
+ * correctness: without DAZI_DK_DEPZ_UNCHANGED every eager call compares. */
 int32_t dazi_dk_reset_geometry(void);
 
 /* Replaces depthkernelTI (depthkernelTI.f90:2-34); host pointers, synchronous.
@@ -362,15 +385,18 @@ int32_t dazi_dk_travel(const dazi_dk_travel_desc* desc, int64_t njob, const int3
                        const float* rcx, const float* rcz, const double* pvRc, float* times,
                        float* ttn, int32_t* status);
 
-/* Bytes of device memory the library's pool holds on the current device
- * (current slots; buffers retired by growth are not counted). */
+/* Bytes of device memory the library's pool holds on the current device:
+ * current slots, buffers retired by growth, and the buffers of every node
+ * geometry met (one set per distinct depz values and minthk). */
 int32_t dazi_dk_pool_bytes(int64_t* out);
 
 /* Event timings (ms) of the last call's phases on its stream:
  * [0] model prep, [1] root search, [2] isotropic kernel pass, [3] TI kernels
  * (their model prep runs on a second stream beside the root search), [4]
- * number of root-search launches.  Waits for those events.  Returns the
- * number of values written. */
+ * number of root-search launches, [5] refined layers per column of the last
+ * node-model call (0 for a ragged batch), [6] device threads of its TI
+ * launch (0 without TI).  Waits for those events.  Returns the number of
+ * values written (at most n, at most 7). */
 int32_t dazi_dk_last_timings(double* out, int32_t n);
 
 /* Enables (1) or disables (0) recording of the timings above. */

@@ -279,6 +279,77 @@ def forward_layered(nl, kk, thk, vp, vs, rho, periods, nthreads=0):
     return cg, fail
 
 
+FORWARD_BATCH = os.path.join(_HERE, "_ref", "surfdisp96_forward_batch")
+
+
+def forward_batch_available() -> bool:
+    return os.path.exists(FORWARD_BATCH)
+
+
+def write_layered_fd_input(path, thk, vp, vs, rho, periods, dln=0.01):
+    """Input file of the reference's layered forward driver
+    (depthkernel_lab/surfdisp96_forward_batch.f90:26-54) holding the model and
+    the 6 L perturbed models of its finite-difference kernels, ordered as
+    depthkernel_column_fd perturbs (DepthKernel1D.f90:146-183): for each layer i,
+    vs_i -/+ 0.5 dln vs_i, then vp, then rho, in float32.  Values are written
+    with 9 significant digits, so the driver reads back the same float32 values.
+    Returns the number of models (1 + 6 L)."""
+    f = np.float32
+    base = [np.asarray(a, np.float32).copy() for a in (thk, vp, vs, rho)]
+    L = base[0].size
+    periods = np.asarray(periods, np.float64)
+    models = [base]
+    half = f(0.5) * f(dln)
+    for i in range(L):
+        for q in (2, 1, 3):  # vs, vp, rho
+            for sgn in (-1, 1):
+                m = [a.copy() for a in base]
+                v = base[q][i]
+                m[q][i] = (v - half * v) if sgn < 0 else (v + half * v)
+                models.append(m)
+    with open(path, "w") as fh:
+        fh.write(f"{len(models)} {L} {periods.size} 1 2 0\n")
+        fh.write(" ".join(f"{p:.17g}" for p in periods) + "\n")
+        for m in models:
+            for a in m:
+                fh.write(" ".join(f"{x:.9g}" for x in a) + "\n")
+    return len(models)
+
+
+def run_forward_batch(inputs, nproc):
+    """Runs the reference's layered forward driver on every input file, nproc
+    processes at a time.  Returns (wall seconds, output paths)."""
+    import subprocess
+    import time
+    if not forward_batch_available():
+        raise RuntimeError(f"reference forward driver missing: {FORWARD_BATCH} (make -C oracle ref)")
+    outs = [p + ".out" for p in inputs]
+    pending = list(zip(inputs, outs))
+    running = []
+    t0 = time.perf_counter()
+    while pending or running:
+        while pending and len(running) < nproc:
+            fi, fo = pending.pop(0)
+            running.append(subprocess.Popen([FORWARD_BATCH, fi, fo], stdout=subprocess.DEVNULL,
+                                            stderr=subprocess.DEVNULL))
+        p = running.pop(0)
+        if p.wait() != 0:
+            raise RuntimeError(f"forward driver failed with {p.returncode}")
+    return time.perf_counter() - t0, outs
+
+
+def read_forward_batch(path, nmodels, kmax):
+    """Curves (nmodels, kmax) of the driver's output file."""
+    c = np.zeros((nmodels, kmax))
+    with open(path) as fh:
+        for line in fh:
+            if line.startswith("#"):
+                continue
+            m, k, _, v = line.split()
+            c[int(m) - 1, int(k) - 1] = float(v)
+    return c
+
+
 def ti_column(vs, periods, depz, minthk):
     """Reference TI column (depthkernel_ti_column): pv (kmax,), lsen (kmax, nz-1) float32."""
     if not ref_available():

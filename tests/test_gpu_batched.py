@@ -96,8 +96,8 @@ def test_graph_replay_stage_iso_and_ti(dk):
 
 
 def test_geometry_cache_follows_depz(dk):
-    """A second depz tensor gives its own geometry; an in-place change needs a
-    reset of the cached geometry."""
+    """A second depz tensor gives its own geometry; an in-place change of depz
+    is seen without any reset."""
     import torch
     from dazi_current_release_amd import configs
     W = configs.node_workload("plain_1024", ncol=64)
@@ -117,6 +117,91 @@ def test_geometry_cache_follows_depz(dk):
     assert not torch.equal(a, e)
     assert torch.equal(a, dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"])
     d1.mul_(1.5)
-    dk.lib().dazi_dk_reset_geometry()
     c = dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"].clone()
     assert torch.equal(b, c)
+
+
+def _geometry_case(dk):
+    import torch
+    from dazi_current_release_amd import configs
+    W = configs.node_workload("plain_1024", ncol=64)
+    nz = W["vel"].shape[-1]
+    vel = torch.as_tensor(np.ascontiguousarray(W["vel"].reshape(-1, nz, order="F").T),
+                          device="cuda")
+    per = torch.as_tensor(W["periods"], device="cuda")
+    return W, vel, per
+
+
+def test_geometry_cache_keyed_on_depz_values_not_address(dk):
+    """depz freed and another depz with other depths allocated at the same
+    address: the cached geometry must not be reused (no reset call), through
+    the Python layer and through the raw C ABI without the unchanged flag."""
+    import ctypes as C
+    import torch
+    from dazi_current_release_amd import _lib
+    W, vel, per = _geometry_case(dk)
+    want = {}
+    for scale in (1.0, 1.5):
+        d = torch.as_tensor(W["depz"] * np.float32(scale), device="cuda")
+        want[scale] = dk.stage_dev(vel, d, per, 2.0, iso=False, ti=False)["pv"].clone()
+        del d
+    assert not torch.equal(want[1.0], want[1.5])
+    # a block of memory that holds one depz, then (after that tensor is gone)
+    # another depz with other depths at the same address: what the caching
+    # allocator does when a freed block is handed out again
+    nz = len(W["depz"])
+    block = torch.empty(1024, dtype=torch.float32, device="cuda")
+    d1 = block[:nz]
+    d1.copy_(torch.as_tensor(W["depz"]))
+    addr = d1.data_ptr()
+    a = dk.stage_dev(vel, d1, per, 2.0, iso=False, ti=False)["pv"].clone()
+    assert torch.equal(a, want[1.0])
+    del d1
+    block[:nz].copy_(torch.as_tensor(W["depz"] * np.float32(1.5)))
+    d2 = block[:nz]
+    assert d2.data_ptr() == addr
+    b = dk.stage_dev(vel, d2, per, 2.0, iso=False, ti=False)["pv"].clone()
+    assert torch.equal(b, want[1.5])
+    # the raw ABI with flags = 0 (no promise that depz is unchanged): new
+    # values at the same address are seen, since the library compares them
+    pv = torch.empty_like(b)
+    st = torch.empty(vel.shape[1], dtype=torch.int32, device="cuda")
+    desc = _lib.Desc(vel.shape[1], 1, nz, per.numel(), 2, 0, 64, 0, 2.0, 0)
+    d2.copy_(torch.as_tensor(W["depz"], device="cuda"))
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    rc = _lib.lib().dazi_dk_iso_dev(C.byref(desc), None, C.c_void_p(vel.data_ptr()), None, None,
+                                    C.c_void_p(d2.data_ptr()), C.c_void_p(per.data_ptr()),
+                                    C.c_void_p(pv.data_ptr()), None, None, None,
+                                    C.c_void_p(st.data_ptr()), s)
+    _lib.check(rc)
+    torch.cuda.synchronize()
+    assert torch.equal(pv, want[1.0])
+
+
+def test_graph_replay_with_changed_depz_reports_geometry_status(dk):
+    """A captured call keeps its geometry; a replay after depz changed gives
+    status -2 and zero phase velocities instead of stale results."""
+    import torch
+    from dazi_current_release_amd import batched
+    W, vel, per = _geometry_case(dk)
+    depz = torch.as_tensor(W["depz"], device="cuda")
+    out = dk.stage_dev(vel, depz, per, 2.0, iso=True, ti=True)
+    want = {k: v.clone() for k, v in out.items()}
+    g = batched.GraphRunner(lambda: dk.stage_dev(vel, depz, per, 2.0, iso=True, ti=True,
+                                                 out=out))
+    g.replay()
+    torch.cuda.synchronize()
+    for k in want:
+        assert torch.equal(out[k], want[k]), k
+    orig = depz.clone()
+    depz.mul_(1.25)
+    g.replay()
+    torch.cuda.synchronize()
+    assert torch.all(out["status"] == -2)
+    for k in ("pv", "sen_vs", "sen_vp", "sen_rho", "lsen"):
+        assert torch.count_nonzero(out[k]) == 0, k
+    depz.copy_(orig)
+    g.replay()
+    torch.cuda.synchronize()
+    for k in want:
+        assert torch.equal(out[k], want[k]), k

@@ -133,3 +133,40 @@ def test_fortran_surface_traveltimes_match_the_oracle(tmp_path):
                              G["rec_off"], G["rcx"], G["rcz"])
     assert np.all(st == 0)
     assert dsurf.size == want.size and np.array_equal(dsurf, want)
+
+
+def test_fortran_ti_timing_lines_and_failed_period_nan(tmp_path):
+    """depthkernelTI_hip prints the reference's internal timing lines
+    (depthkernelTI.f90:170-186) and, with DAZI_DEPTHKERNEL_TI_FAILED_NAN=1,
+    leaves NaN in Lsen_Gsc exactly where the reference does: every interval of
+    the periods without a phase velocity (its tregn96 at c = 0); without the
+    flag those entries are 0."""
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    f = load_golden("ref_ti_failures.npz")
+    ncol, nz = f["vel"].shape
+    K = len(f["periods"])
+    vel = np.asfortranarray(f["vel"].reshape(ncol, 1, nz, order="F"))
+    outs = {}
+    for nan in (False, True):
+        env = dict(DAZI_DEPTHKERNEL_TI_FAILED_NAN="1") if nan else {}
+        d = tmp_path / ("nan" if nan else "zero")
+        d.mkdir()
+        r, fout = FIO.run(str(d), vel, f["depz"], f["periods"], float(f["minthk"]), env=env)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[nan] = FIO.read_output(fout, ncol, K, nz)
+        lines = r.stdout.splitlines()
+        assert "  depthkernelTI internal timing:" in lines
+        for key in ("refine+surfdisp96 phase velocity = ", "tregn96 sensitivity kernel = ",
+                    "splice refined kernel = ", "refined sublayers rmax min/max/avg = ",
+                    "observed worker threads = "):
+            assert any(line.startswith("    " + key) for line in lines), key
+        rmax = [line for line in lines if "rmax min/max/avg" in line][0].split("=")[1].split("/")
+        assert int(rmax[0]) == int(rmax[1]) > nz and float(rmax[2]) == int(rmax[0])
+    failed = outs[True]["ti_pv"] == 0
+    assert np.array_equal(failed, f["ti_pv"] == 0) and failed.any()
+    got = outs[True]["ti_lsen"]
+    assert np.array_equal(np.isnan(got), np.isnan(f["ti_lsen"]))
+    assert np.all(np.isnan(got[failed]))
+    assert np.all(outs[False]["ti_lsen"][failed] == 0)
+    assert np.array_equal(got[~failed], outs[False]["ti_lsen"][~failed])
