@@ -62,6 +62,9 @@ def port():
                                       f32p, C.c_float, C.c_int]
         L.orc_forward_layered.argtypes = [C.c_int, i32p, i64p, i32p, i64p, f32p, f32p, f32p,
                                           f32p, f64p, f64p, C.c_void_p, C.c_int]
+        L.orc_forward_layered_x.argtypes = [C.c_int, i32p, i64p, i32p, i64p, f32p, f32p, f32p,
+                                            f32p, f64p, f64p, C.c_void_p, C.c_int, C.c_double,
+                                            C.c_int]
         L.orc_max_threads.restype = C.c_int
         _port = L
     return _port
@@ -266,17 +269,64 @@ def depthkernel(vel, tRc, depz, minthk, lib="port", nthreads=0, kernels=True, vp
     return (pv,) + tuple(s.reshape(ncol, kmax, nz, order="F") for s in sens)
 
 
-def forward_layered(nl, kk, thk, vp, vs, rho, periods, nthreads=0):
-    """Ragged batch forward: per column surfdisp96 over flat concatenated arrays."""
+def forward_layered(nl, kk, thk, vp, vs, rho, periods, nthreads=0, tol_rel=1e-6,
+                    round_f32=True):
+    """Ragged batch forward: per column surfdisp96 over flat concatenated arrays.
+    tol_rel=1e-13, round_f32=False give the tight roots of a finite-difference
+    oracle."""
     nl = np.ascontiguousarray(nl, np.int32)
     kk = np.ascontiguousarray(kk, np.int32)
     loff = np.concatenate([[0], np.cumsum(nl)[:-1]]).astype(np.int64)
     toff = np.concatenate([[0], np.cumsum(kk)[:-1]]).astype(np.int64)
     cg = np.zeros(int(kk.sum()), np.float64)
     fail = np.zeros(nl.size, np.int32)
-    port().orc_forward_layered(nl.size, nl, loff, kk, toff, _f32(thk), _f32(vp), _f32(vs),
-                               _f32(rho), _f64(periods), cg, _ptr(fail), nthreads)
+    port().orc_forward_layered_x(nl.size, nl, loff, kk, toff, _f32(thk), _f32(vp), _f32(vs),
+                                 _f32(rho), _f64(periods), cg, _ptr(fail), nthreads, tol_rel,
+                                 1 if round_f32 else 0)
     return cg, fail
+
+
+def layer_kernels_fd(thk, vp, vs, rho, periods, rel=(0.005, 0.0025), nthreads=0):
+    """Tight finite-difference layer kernels of ONE layered model: dc/dVs,
+    dc/dVp, dc/drho (each (kmax, L)) by central differences at float32 model
+    steps of rel[0] and rel[1] of each value, combined by Richardson
+    extrapolation (O(h^4) truncation), with roots to 1e-13 relative and no
+    float32 rounding of c.  All 12 L perturbed models run as one OpenMP batch.
+    Returns (c (kmax,), [dvs, dvp, drho])."""
+    base = [np.asarray(a, np.float32).copy() for a in (thk, vp, vs, rho)]
+    L = base[0].size
+    periods = np.asarray(periods, np.float64)
+    K = periods.size
+    models, steps = [base], []
+    for q in (2, 1, 3):                      # vs, vp, rho
+        for m in range(L):
+            for r in rel:
+                v0 = base[q][m]
+                h = np.float32(r) * v0
+                lo, hi = np.float32(v0 - h), np.float32(v0 + h)
+                for v in (lo, hi):
+                    mm = [a.copy() for a in base]
+                    mm[q][m] = v
+                    models.append(mm)
+                steps.append(float(hi) - float(lo))
+    n = len(models)
+    cat = [np.concatenate([mm[i] for mm in models]) for i in range(4)]
+    c, _ = forward_layered(np.full(n, L, np.int32), np.full(n, K, np.int32), *cat,
+                           np.tile(periods, n), nthreads=nthreads, tol_rel=1e-13,
+                           round_f32=False)
+    c = c.reshape(n, K)
+    out, i, s = [], 1, 0
+    for q in range(3):
+        g = np.zeros((K, L))
+        for m in range(L):
+            d = []
+            for _ in rel:
+                d.append((c[i + 1] - c[i]) / steps[s])
+                i += 2
+                s += 1
+            g[:, m] = (4 * d[1] - d[0]) / 3
+        out.append(g)
+    return c[0], out
 
 
 FORWARD_BATCH = os.path.join(_HERE, "_ref", "surfdisp96_forward_batch")

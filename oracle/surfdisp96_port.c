@@ -684,20 +684,30 @@ void orc_depthkernel(int nx, int ny, int nz, const float* vel, const float* vp_o
 }
 
 /* Layered-model forward only, for ragged batches: column j has nl[j] layers at
- * offset loff[j] in the flat arrays and kk[j] periods at offset toff[j]. */
-void orc_forward_layered(int ncol, const int* nl, const long* loff, const int* kk,
-                         const long* toff, const float* thk, const float* vp, const float* vs,
-                         const float* rho, const double* periods, double* cg, int* fail,
-                         int nthreads) {
+ * offset loff[j] in the flat arrays and kk[j] periods at offset toff[j].
+ * tol_rel and round_f32 as orc_surfdisp96_x (1e-6 and 1: the reference's). */
+void orc_forward_layered_x(int ncol, const int* nl, const long* loff, const int* kk,
+                           const long* toff, const float* thk, const float* vp, const float* vs,
+                           const float* rho, const double* periods, double* cg, int* fail,
+                           int nthreads, double tol_rel, int round_f32) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #pragma omp parallel for schedule(dynamic)
 #endif
   for (int j = 0; j < ncol; j++) {
     int f = orc_surfdisp96_x(thk + loff[j], vp + loff[j], vs + loff[j], rho + loff[j], nl[j], 1,
-                             kk[j], periods + toff[j], cg + toff[j], 1.0e-6, 1, NULL, NULL);
+                             kk[j], periods + toff[j], cg + toff[j], tol_rel, round_f32, NULL,
+                             NULL);
     if (fail) fail[j] = f;
   }
+}
+
+void orc_forward_layered(int ncol, const int* nl, const long* loff, const int* kk,
+                         const long* toff, const float* thk, const float* vp, const float* vs,
+                         const float* rho, const double* periods, double* cg, int* fail,
+                         int nthreads) {
+  orc_forward_layered_x(ncol, nl, loff, kk, toff, thk, vp, vs, rho, periods, cg, fail, nthreads,
+                        1.0e-6, 1);
 }
 
 int orc_max_threads(void) {

@@ -147,9 +147,46 @@ def ref_ti_failures(n=4, seed=0):
                         ti_lsen=np.array(tls))
 
 
+VORONOI_NOTE = ("reference column FD (Brocher, depthkernel_column_fd) on 24 columns of the "
+                "first Voronoi realization of voronoi_100 (seed 5; columns "
+                "linspace(0, 4095, 24)); depthkernel_ti_column on the first 12")
+
+
+def ref_voronoi(ncol=24):
+    """Reference depthkernel_column_fd and depthkernel_ti_column on Voronoi
+    columns: piecewise-constant node models with sharp steps at the cell and
+    layer boundaries, projected as the realizations of voronoi_100."""
+    W = configs.voronoi_workload(1)
+    cols = np.linspace(0, 4095, ncol).astype(np.int64)
+    vel = W["vel"][cols, 0, :]
+    pv, svs, svp, srho = [], [], [], []
+    for j in range(ncol):
+        out = P.column_fd(vel[j], W["periods"], W["depz"], W["minthk"], lib="ref")
+        for lst, a in zip((pv, svs, svp, srho), out):
+            lst.append(a)
+    ti_pv, ti_l = [], []
+    for j in range(ncol // 2):
+        a, b = P.ti_column(vel[j], W["periods"], W["depz"], W["minthk"])
+        ti_pv.append(a)
+        ti_l.append(b)
+    np.savez_compressed(os.path.join(HERE, "ref_voronoi.npz"), cols=cols, vel=vel,
+                        depz=W["depz"], periods=W["periods"], minthk=np.float32(W["minthk"]),
+                        pv=np.array(pv), sen_vs=np.array(svs), sen_vp=np.array(svp),
+                        sen_rho=np.array(srho), ti_pv=np.array(ti_pv), ti_lsen=np.array(ti_l))
+
+
 def main():
     if not P.ref_available():
         raise SystemExit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["voronoi"]:
+        ref_voronoi()
+        with open(os.path.join(HERE, "MANIFEST.json")) as f:
+            man = json.load(f)
+        man["fixtures"]["ref_voronoi.npz"] = VORONOI_NOTE
+        with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
+            json.dump(man, f, indent=1)
+        print("ref_voronoi.npz written")
+        return
     if sys.argv[1:] == ["ti_failures"]:
         ref_ti_failures()
         with open(os.path.join(HERE, "MANIFEST.json")) as f:
@@ -176,6 +213,8 @@ def main():
         "ref_ti_failures.npz": FAILURES_NOTE,
     }
     ref_ti_failures()
+    ref_voronoi()
+    man["fixtures"]["ref_voronoi.npz"] = VORONOI_NOTE
     with open(os.path.join(HERE, "MANIFEST.json"), "w") as f:
         json.dump(man, f, indent=1)
     print("fixtures written")

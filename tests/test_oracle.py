@@ -27,16 +27,38 @@ def test_dltar4_matches_reference_fixture():
         assert got == pytest.approx(v, rel=1e-12, abs=1e-300)
 
 
-@pytest.mark.parametrize("name", ["ref_plain_1024.npz", "ref_plain_4096.npz"])
+@pytest.mark.parametrize("name", ["ref_plain_1024.npz", "ref_plain_4096.npz",
+                                  "ref_voronoi.npz"])
 def test_column_fd_bit_exact_vs_reference_fixture(name):
     f = load_golden(name)
-    n = 4 if "4096" in name else 8
+    n = 8 if "1024" in name else 4
     for j in range(n):
         pv, s1, s2, s3 = P.column_fd(f["vel"][j], f["periods"], f["depz"], float(f["minthk"]))
         assert np.array_equal(pv, f["pv"][j])
         assert np.array_equal(s1, f["sen_vs"][j])
         assert np.array_equal(s2, f["sen_vp"][j])
         assert np.array_equal(s3, f["sen_rho"][j])
+
+
+def test_tight_layer_fd_density_invariant_and_bracket():
+    """The layered FD oracle (tight roots, Richardson) against the exact
+    identities of dispersion: sum rho dc/drho = 0 (c does not change when every
+    density scales), and sum (alpha dc/dalpha + beta dc/dbeta) = c + T dc/dT
+    (scaling every velocity by s scales c by s at period T/s), with dc/dT from
+    tight roots at T(1 -/+ eps).  A dropped layer or a wrong step size fails."""
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload("diff_4096", ncol=1)
+    L, K = int(W["nl"][0]), int(W["nk"][0])
+    mdl = [W[k][:L] for k in ("thk", "vp", "vs", "rho")]
+    per = W["periods"][:K]
+    c, (dvs, dvp, drho) = P.layer_kernels_fd(*mdl, per, nthreads=4)
+    assert np.abs((mdl[3][None, :] * drho).sum(1) / c).max() < 1e-5
+    eps = 1e-5
+    cp, _ = P.forward_layered([L], [K], *mdl, per * (1 + eps), tol_rel=1e-13, round_f32=False)
+    cm, _ = P.forward_layered([L], [K], *mdl, per * (1 - eps), tol_rel=1e-13, round_f32=False)
+    rhs = c + per * (cp - cm) / (2 * eps * per)
+    lhs = (mdl[2][None, :] * dvs + mdl[1][None, :] * dvp).sum(1)
+    assert np.abs(lhs - rhs).max() / c.max() < 1e-5
 
 
 def test_ragged_surfdisp96_bit_exact_vs_reference_fixture():
