@@ -7,8 +7,10 @@ behind a C ABI (include/dazi_dk.h).  The Python layer mirrors the reference's
 """
 from ._lib import DaziDkError, lib, make_tune, last_timings  # noqa: F401
 from .stage import (depthkernel, run_depthkernel, iso_dev, layered_dev, LayeredBatch,  # noqa: F401
-                    depthkernelTI, run_depthkernelTI, stage_dev, layered_vjp, effective_dev)
+                    depthkernelTI, run_depthkernelTI, stage_dev, layered_vjp, effective_dev,
+                    node_layers, ti_kernels_dev, ti_vjp_dev)
 
 __all__ = ["DaziDkError", "lib", "make_tune", "last_timings", "depthkernel", "run_depthkernel",
            "iso_dev", "layered_dev", "LayeredBatch", "depthkernelTI", "run_depthkernelTI",
-           "stage_dev", "layered_vjp", "effective_dev"]
+           "stage_dev", "layered_vjp", "effective_dev", "node_layers", "ti_kernels_dev",
+           "ti_vjp_dev"]

@@ -28,6 +28,7 @@ EXPORTS = [
     "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows", "dazi_dk_azimuth_map_dev",
     "dazi_dk_azimuth_map", "dazi_dk_travel_plan_create", "dazi_dk_travel_run_dev",
     "dazi_dk_travel_plan_destroy", "dazi_dk_travel", "dazi_dk_layered_noalloc_dev",
+    "dazi_dk_node_layers", "dazi_dk_ti_kernels_dev", "dazi_dk_ti_vjp_dev",
 ]
 
 
@@ -106,6 +107,10 @@ def lib():
         L.dazi_dk_ti_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 9
         L.dazi_dk_stage_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 12
         L.dazi_dk_layered_vjp_dev.argtypes = [C.POINTER(Layered)] + [C.c_void_p] * 6
+        L.dazi_dk_node_layers.argtypes = [C.c_int32, C.c_void_p, C.c_float,
+                                          C.POINTER(C.c_int32)]
+        L.dazi_dk_ti_kernels_dev.argtypes = [C.POINTER(Desc), C.c_void_p] + [C.c_void_p] * 11
+        L.dazi_dk_ti_vjp_dev.argtypes = [C.c_int32] * 3 + [C.c_void_p] * 6
         L.dazi_dk_last_timings.argtypes = [C.POINTER(C.c_double), C.c_int32]
         L.dazi_dk_last_timings.restype = C.c_int32
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]

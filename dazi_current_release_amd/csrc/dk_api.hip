@@ -355,7 +355,8 @@ int geometry(Pool* P, const dazi_dk_desc* d, const float* depz_dev, const float*
 int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
                const float* vp, const float* rho, const float* depz_dev,
                const float* depz_host, const double* tRc, double* pv, double* svs,
-               double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st) {
+               double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st,
+               float* dcdA = nullptr, float* dcdL = nullptr) {
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
   // plain mode's tile: the para.in control (pdsurf_tile_columns /
@@ -373,6 +374,9 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   bool kern = svs || svp || srho;
   if (kern && !(svs && svp && srho))
     return set_err(DAZI_DK_EINVAL, "sen_vs, sen_vp, sen_rho must be all set or all NULL");
+  if ((dcdA == nullptr) != (dcdL == nullptr))
+    return set_err(DAZI_DK_EINVAL, "dc_dA and dc_dL must be both set or both NULL");
+  const bool ti = lsen || dcdA;
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
   Geometry* G;
@@ -411,7 +415,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   // runs on the side stream from the start of the call
   void *til = nullptr, *tiscr = nullptr, *tiq = nullptr;
   int ti_nthr = 0;
-  if (lsen) {
+  if (ti) {
     if (nsolves >= INT32_MAX)
       return set_err(DAZI_DK_ELIMIT, "TI splice: %lld solves (limit 2^31 - 1)", (long long)nsolves);
     if (int rc = pool_get(P, S_TI_LAYERS, (size_t)ncol * L * sizeof(dk::TiLayerDev), &til))
@@ -423,13 +427,13 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   P->have_events = g_timing;
   P->last_L = L;
-  P->last_ti_threads = lsen ? ti_nthr : 0;
+  P->last_ti_threads = ti ? ti_nthr : 0;
   rec(P, 0, st);
   if (!G->built) {
     HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, (float*)geo_depz, st));
     G->built = true;
   }
-  if (lsen) {
+  if (ti) {
     HIPCHK(hipEventRecord(P->fk[0], st));
     HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
     HIPCHK(dk::launch_prep_ti(ncol, vel, d->vp_rho_mode == 1 ? vp : nullptr,
@@ -473,11 +477,11 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     HIPCHK(dk::launch_kernels(B, O, (double*)tape, kern_nthr, st));
   }
   rec(P, 3, st);
-  if (lsen) {
+  if (ti) {
     // join the TI model prep, then the TI kernels (the roots are ready)
     HIPCHK(hipStreamWaitEvent(st, P->fk[2], 0));
     HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
-                         (const int*)nsubd, lsen, (double*)tiscr, ti_nthr, (int*)tiq,
+                         (const int*)nsubd, lsen, dcdA, dcdL, (double*)tiscr, ti_nthr, (int*)tiq,
                          (d->flags & DAZI_DK_TI_FAILED_NAN) != 0, st));
   }
   rec(P, 4, st);
@@ -1017,6 +1021,40 @@ int32_t dazi_dk_pool_bytes(int64_t* out) {
   int64_t n = (int64_t)(P->retired_bytes + P->geo_bytes);
   for (int s = 0; s < S_COUNT; s++) n += (int64_t)P->n[s];
   *out = n;
+  return 0;
+}
+
+int32_t dazi_dk_node_layers(int32_t nz, const float* depz, float minthk, int32_t* L) {
+  if (!depz || !L || nz < 2 || !(minthk > 0.0f))
+    return set_err(DAZI_DK_EINVAL, "bad arguments (nz=%d, minthk=%g)", nz, minthk);
+  std::vector<int> nsub;
+  const int Lc = node_geometry(nz, depz, minthk, &nsub);
+  if (Lc < 2) return set_err(DAZI_DK_EINVAL, "depz must be strictly increasing");
+  *L = Lc;
+  return 0;
+}
+
+int32_t dazi_dk_ti_kernels_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune,
+                               const float* vel, const float* vp, const float* rho,
+                               const float* depz, const double* tRc, double* pvRc,
+                               float* Lsen_Gsc, float* dc_dA, float* dc_dL, int32_t* status,
+                               void* stream) {
+  if (!desc || !vel || !depz || !tRc || !pvRc || !dc_dA || !dc_dL)
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  return stage_impl(desc, tune, vel, vp, rho, depz, nullptr, tRc, pvRc, nullptr, nullptr, nullptr,
+                    Lsen_Gsc, status, (hipStream_t)stream, dc_dA, dc_dL);
+}
+
+int32_t dazi_dk_ti_vjp_dev(int32_t ncol, int32_t L, int32_t kmax, const float* dc_dA,
+                           const float* dc_dL, const double* grad_c, double* grad_A,
+                           double* grad_L, void* stream) {
+  if (ncol < 0 || L < 1 || kmax < 1)
+    return set_err(DAZI_DK_EINVAL, "bad sizes ncol=%d L=%d kmax=%d", ncol, L, kmax);
+  if (!dc_dA || !dc_dL || !grad_c || (!grad_A && !grad_L))
+    return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if (ncol == 0) return 0;
+  HIPCHK(dk::launch_ti_vjp(ncol, L, kmax, dc_dA, dc_dL, grad_c, grad_A, grad_L,
+                           (hipStream_t)stream));
   return 0;
 }
 

@@ -76,9 +76,14 @@ hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const flo
                           const void* geo, TiLayerDev* out, int* queue, hipStream_t st);
 // nan_failed: periods without a phase velocity get NaN in every interval
 // (the reference's tregn96 at c = 0) instead of 0.
+// lsen may be NULL; dcdA/dcdL (both or neither) receive the refined-layer
+// kernels dc/dA, dc/dL at ((m*K + k)*ncol + col).
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* queue, int nan_failed, hipStream_t st);
+                     float* dcdA, float* dcdL, double* scratch, int nthr, int* queue,
+                     int nan_failed, hipStream_t st);
+hipError_t launch_ti_vjp(int ncol, int L, int K, const float* dcdA, const float* dcdL,
+                         const double* gc, double* gA, double* gL, hipStream_t st);
 
 // launch_prep_geom writes the L flattening and node-mapping records that
 // every column of a node geometry (depz, minthk0) shares to geo

@@ -850,16 +850,24 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   return 0;
 }
 
-// depthkernelTI splice in float32 (depthkernelTI.f90:152-162): interval j sums
-// its nsub[j] sublayers, each as acc = (acc + dc/dA*TA) + dc/dL*TL (the
-// reference's left-to-right order).
-DK_HD float splice_add(float acc, float rrho, float rvp, float rvs, float TA, float TL, float TF,
-                       float dah, float dbv, float dn) {
+// depthkernelTI's chain to the moduli in float32 (depthkernelTI.f90:152-159):
+// dc/dA = dc/dalpha_h / (2 rho vp) - F/(A-2L)^2 dc/deta and
+// dc/dL = dc/dbeta_v / (2 rho vs) + 2F/(A-2L)^2 dc/deta of one refined layer,
+// from tregn96's float32 kernels dah, dbv, dn.
+DK_HD void chain_AL(float rrho, float rvp, float rvs, float TA, float TL, float TF, float dah,
+                    float dbv, float dn, float* dA, float* dL) {
 #pragma clang fp contract(off)
   float den = TA - 2.0f * TL;
   float den2 = den * den;
-  float dA = 0.5f / (rrho * rvp) * dah - TF / den2 * dn;
-  float dL = 0.5f / (rrho * rvs) * dbv + 2.0f * TF / den2 * dn;
+  *dA = 0.5f / (rrho * rvp) * dah - TF / den2 * dn;
+  *dL = 0.5f / (rrho * rvs) * dbv + 2.0f * TF / den2 * dn;
+}
+
+// depthkernelTI splice in float32 (depthkernelTI.f90:152-162): interval j sums
+// its nsub[j] sublayers, each as acc = (acc + dc/dA*TA) + dc/dL*TL (the
+// reference's left-to-right order).
+DK_HD float splice_add(float acc, float TA, float TL, float dA, float dL) {
+#pragma clang fp contract(off)
   return (acc + dA * TA) + dL * TL;
 }
 

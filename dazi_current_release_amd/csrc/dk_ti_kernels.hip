@@ -109,22 +109,36 @@ __global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restri
 }
 
 // One (column, period) solve s of the TI splice: writes Lsen_Gsc of every
-// interval, or zeros where no kernel exists (no phase velocity, vanishing
-// surface minor).  Returns solve<G_>'s code; on -2 nothing is written.
+// interval (when lsen is set) and, when dcdA/dcdL are set, the chained
+// kernels dc/dA, dc/dL of every refined layer m at ((m*K + k)*ncol + col);
+// zeros where no kernel exists (no phase velocity, vanishing surface minor),
+// or NaN for a missing phase velocity with nan_failed.  Returns solve<G_>'s
+// code; on -2 nothing is written.
 template <class G_>
 __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* __restrict__ layers,
                       const double* __restrict__ periods, const double* __restrict__ pv,
-                      const int* __restrict__ nsub, float* __restrict__ lsen, double* scratch,
-                      long stride, int nan_failed) {
+                      const int* __restrict__ nsub, float* __restrict__ lsen, float* __restrict__ dcdA,
+                      float* __restrict__ dcdL, double* scratch, long stride, int nan_failed) {
   const float twopi = 2.0f * 3.141592654f;
   const int col = (int)(s % ncol), k = (int)(s / ncol);
   const TiLayerDev* lay = layers + (size_t)col * L;
   auto out = [&](int j) -> float& { return lsen[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * j)]; };
+  auto lay_out = [&](float* a, int m) -> float& {
+    return a[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * m)];
+  };
+  auto fill_all = [&](float v) {
+    if (lsen)
+      for (int j = 0; j < nz - 1; j++) out(j) = v;
+    if (dcdA)
+      for (int m = 0; m < L; m++) {
+        lay_out(dcdA, m) = v;
+        lay_out(dcdL, m) = v;
+      }
+  };
   const double c = (double)(float)pv[s];
   if (!(c > 0.0)) {
     // no phase velocity for this period (root search failed): no kernel
-    const float fill = nan_failed ? __builtin_nanf("") : 0.0f;
-    for (int j = 0; j < nz - 1; j++) out(j) = fill;
+    fill_all(nan_failed ? __builtin_nanf("") : 0.0f);
     return 0;
   }
   const double omega = (double)twopi / (double)(float)periods[k];
@@ -144,9 +158,15 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   };
   auto vtpf = [&](int m) { return lay[m].vtp; };
   auto emit = [&](int m, float dah, float dbv, float dn) {
-    if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
     const TiLayerDev& t = lay[m];
-    acc = ti::splice_add(acc, t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn);
+    float dA, dL;
+    ti::chain_AL(t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn, &dA, &dL);
+    if (dcdA) {
+      lay_out(dcdA, m) = dA;
+      lay_out(dcdL, m) = dL;
+    }
+    if (!lsen || m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
+    acc = ti::splice_add(acc, t.TA, t.TL, dA, dL);
     if (++sub == nsub[interval]) {
       out(interval) = acc;
       acc = 0.0f;
@@ -155,8 +175,7 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
     }
   };
   const int rc = ti::solve<G_>(L, layf, vtpf, omega, c, scratch, stride, emit);
-  if (rc == -1)
-    for (int j = 0; j < nz - 1; j++) out(j) = 0.0f;
+  if (rc == -1) fill_all(0.0f);
   return rc;
 }
 
@@ -165,14 +184,15 @@ __global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
                                             const double* __restrict__ periods,
                                             const double* __restrict__ pv,
                                             const int* __restrict__ nsub, float* __restrict__ lsen,
+                                            float* __restrict__ dcdA, float* __restrict__ dcdL,
                                             double* __restrict__ scratch, int* __restrict__ queue,
                                             int nan_failed) {
   const long nthr = (long)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const long nsolves = (long)ncol * K;
   for (long s = tid; s < nsolves; s += nthr) {
-    const int rc = ti_one<ti::Eig>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen,
-                                   scratch + tid, nthr, nan_failed);
+    const int rc = ti_one<ti::Eig>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA,
+                                   dcdL, scratch + tid, nthr, nan_failed);
     if (rc == -2) queue[1 + atomicAdd(queue, 1)] = (int)s;
   }
 }
@@ -183,13 +203,15 @@ __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
                                               const double* __restrict__ pv,
                                               const int* __restrict__ nsub,
                                               float* __restrict__ lsen,
+                                              float* __restrict__ dcdA,
+                                              float* __restrict__ dcdL,
                                               double* __restrict__ scratch,
                                               const int* __restrict__ queue) {
   const long nthr = (long)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = queue[0];
   for (long i = tid; i < n; i += nthr)
-    ti_one<ti::EigC>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen,
+    ti_one<ti::EigC>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA, dcdL,
                      scratch + tid, nthr, 0);
 }
 
@@ -209,12 +231,13 @@ int ti_cx_threads(int nthr) {
 
 hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
                      const double* periods, const double* pv, const int* nsub, float* lsen,
-                     double* scratch, int nthr, int* queue, int nan_failed, hipStream_t st) {
+                     float* dcdA, float* dcdL, double* scratch, int nthr, int* queue,
+                     int nan_failed, hipStream_t st) {
   int block = 128;
   int grid = nthr / block;
   if (grid < 1) grid = 1;
   hipLaunchKernelGGL(k_ti, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods, pv,
-                     nsub, lsen, scratch, queue, nan_failed);
+                     nsub, lsen, dcdA, dcdL, scratch, queue, nan_failed);
   if (hipError_t e = hipGetLastError()) return e;
   // the queued complex-pair solves; the grid is sized for the worst case and
   // exits at once when the queue is empty
@@ -222,7 +245,40 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
   int cap = ncol * K;
   if (ncx > (cap + 63) / 64 * 64) ncx = (cap + 63) / 64 * 64;
   hipLaunchKernelGGL(k_ti_cx, dim3(ncx / 64), dim3(64), 0, st, ncol, K, L, nz, layers, periods,
-                     pv, nsub, lsen, scratch, (const int*)queue);
+                     pv, nsub, lsen, dcdA, dcdL, scratch, (const int*)queue);
+  return hipGetLastError();
+}
+
+// grad_A(m, col) = sum_k grad_c(k, col) dc_k/dA_m, and the same for L, over the
+// refined layers of node models: one thread per (layer, column), periods in
+// order (deterministic).  dcdA, dcdL (L, K, ncol) f32; grad_c (K, ncol) f64;
+// grad_A, grad_L (L, ncol) f64.  Periods with a non-finite kernel (NaN at a
+// missing phase velocity) or a zero grad_c contribute nothing.
+__global__ void k_ti_vjp(int ncol, int L, int K, const float* __restrict__ dcdA,
+                         const float* __restrict__ dcdL, const double* __restrict__ gc,
+                         double* __restrict__ gA, double* __restrict__ gL) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)ncol * L) return;
+  const int m = (int)(t / ncol), col = (int)(t % ncol);
+  double a = 0.0, l = 0.0;
+  for (int k = 0; k < K; k++) {
+    const double g = gc[(long)k * ncol + col];
+    const long o = ((long)m * K + k) * ncol + col;
+    const double da = (double)dcdA[o], dl = (double)dcdL[o];
+    if (g != 0.0 && isfinite(da) && isfinite(dl)) {
+      a = fma(g, da, a);
+      l = fma(g, dl, l);
+    }
+  }
+  if (gA) gA[t] = a;
+  if (gL) gL[t] = l;
+}
+
+hipError_t launch_ti_vjp(int ncol, int L, int K, const float* dcdA, const float* dcdL,
+                         const double* gc, double* gA, double* gL, hipStream_t st) {
+  const long n = (long)ncol * L;
+  hipLaunchKernelGGL(k_ti_vjp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ncol, L, K,
+                     dcdA, dcdL, gc, gA, gL);
   return hipGetLastError();
 }
 

@@ -469,3 +469,60 @@ def effective_dev(vel, sen_vs, sen_vp, sen_rho, grad_c=None, eff=True, grad=True
         _tptr(out.get("eff")), _tptr(out.get("grad_vs")), _stream_ptr(stream))
     _lib.check(rc)
     return out
+
+
+def node_layers(depz, minthk):
+    """Refined layers per column (half-space included) of the node geometry."""
+    d = np.ascontiguousarray(np.asarray(depz.cpu() if hasattr(depz, "cpu") else depz,
+                                        np.float32))
+    n = C.c_int32(0)
+    _lib.check(_lib.lib().dazi_dk_node_layers(d.size, _ptr(d), float(minthk), C.byref(n)))
+    return n.value
+
+
+def ti_kernels_dev(vel, depz, periods, minthk, vp=None, rho=None, lsen=True, tune=None,
+                   out=None, stream=None):
+    """The TI stage with the per-refined-layer kernels depthkernelTI splices
+    (depthkernelTI.f90:152-162) on device tensors.
+
+    vel/vp/rho (nz, ncol) float32, depz (nz,) float32, periods (K,) float64.
+    Returns dict(pv (K, ncol) f64, dc_dA, dc_dL (L, K, ncol) f32, lsen (nz-1, K,
+    ncol) f32 when lsen, status (ncol,) i32), L = node_layers(depz, minthk)."""
+    import torch
+    nz, ncol, K = _check_node_inputs(vel, depz, periods, vp, rho)
+    L = node_layers(depz, minthk)
+    dev = vel.device
+    spec = dict(pv=(torch.float64, (K, ncol)), status=(torch.int32, (ncol,)),
+                dc_dA=(torch.float32, (L, K, ncol)), dc_dL=(torch.float32, (L, K, ncol)))
+    if lsen:
+        spec["lsen"] = (torch.float32, (nz - 1, K, ncol))
+    if out is None:
+        out = {k: torch.empty(shape, dtype=dt, device=dev) for k, (dt, shape) in spec.items()}
+    _check_out(out, spec)
+    d = _lib.Desc(ncol, 1, nz, K, 2, 0, 64, 1 if vp is not None else 0, float(minthk),
+                  _depz_flags(depz))
+    rc = _lib.lib().dazi_dk_ti_kernels_dev(
+        C.byref(d), None if tune is None else C.byref(tune), _tptr(vel), _tptr(vp), _tptr(rho),
+        _tptr(depz), _tptr(periods), _tptr(out["pv"]), _tptr(out.get("lsen")) if lsen else None,
+        _tptr(out["dc_dA"]), _tptr(out["dc_dL"]), _tptr(out["status"]), _stream_ptr(stream))
+    _lib.check(rc)
+    return out
+
+
+def ti_vjp_dev(dc_dA, dc_dL, grad_c, stream=None):
+    """grad_A, grad_L (L, ncol) float64: sum over periods of grad_c * dc/dA, dc/dL
+    per refined layer (dc_dA, dc_dL (L, K, ncol) float32; grad_c (K, ncol))."""
+    import torch
+    if not isinstance(dc_dA, torch.Tensor) or dc_dA.dim() != 3:
+        raise ValueError("dc_dA must be an (L, K, ncol) float32 GPU tensor")
+    L, K, ncol = dc_dA.shape
+    _check_dev(dc_dA, torch.float32, (L, K, ncol), "dc_dA")
+    _check_dev(dc_dL, torch.float32, (L, K, ncol), "dc_dL")
+    gc = grad_c.to(torch.float64).contiguous()
+    _check_dev(gc, torch.float64, (K, ncol), "grad_c")
+    gA = torch.empty((L, ncol), dtype=torch.float64, device=dc_dA.device)
+    gL = torch.empty_like(gA)
+    rc = _lib.lib().dazi_dk_ti_vjp_dev(ncol, L, K, _tptr(dc_dA), _tptr(dc_dL), _tptr(gc),
+                                       _tptr(gA), _tptr(gL), _stream_ptr(stream))
+    _lib.check(rc)
+    return gA, gL

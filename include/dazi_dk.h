@@ -186,6 +186,35 @@ int32_t dazi_dk_stage_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune, co
                           double* pvRc, double* sen_vsRc, double* sen_vpRc, double* sen_rhoRc,
                           float* Lsen_Gsc, int32_t* status, void* stream);
 
+/* Refined layers per column (half-space included) of the node geometry
+ * (depz(nz), minthk), computed on the host as CalSurfG.f90:2506-2520 does: the
+ * first dimension of dazi_dk_ti_kernels_dev's per-layer outputs. */
+int32_t dazi_dk_node_layers(int32_t nz, const float* depz, float minthk, int32_t* L);
+
+/* The TI stage of dazi_dk_ti_dev with, in addition, the per-refined-layer
+ * kernels that depthkernelTI splices into Lsen_Gsc (depthkernelTI.f90:152-162):
+ *   dc_dA(m, k, col) = dc/dalpha_h / (2 rho vp) - F/(A - 2L)^2 dc/deta,
+ *   dc_dL(m, k, col) = dc/dbeta_v / (2 rho vs) + 2F/(A - 2L)^2 dc/deta,
+ * f32, (L, kmax, nx*ny) with the column fastest, for all L refined layers
+ * (dazi_dk_node_layers; the half-space last, which the splice leaves out),
+ * from tregn96's kernels at the float32 phase velocity.  Lsen_Gsc may be
+ * NULL.  No reference counterpart as an interface: the reference forms these
+ * values inside depthkernelTI and keeps only their splice. */
+int32_t dazi_dk_ti_kernels_dev(const dazi_dk_desc* desc, const dazi_dk_tune* tune,
+                               const float* vel, const float* vp, const float* rho,
+                               const float* depz, const double* tRc, double* pvRc,
+                               float* Lsen_Gsc, float* dc_dA, float* dc_dL, int32_t* status,
+                               void* stream);
+
+/* Vector-Jacobian product over the TI kernels: for every refined layer m and
+ * column, grad_A(m, col) = sum_k grad_c(k, col) dc_dA(m, k, col) and grad_L
+ * likewise (f64, (L, ncol)); periods whose kernels are not finite contribute
+ * nothing.  Deterministic.  grad_A or grad_L may be NULL.  No reference
+ * counterpart (the reference has no gradient code). */
+int32_t dazi_dk_ti_vjp_dev(int32_t ncol, int32_t L, int32_t kmax, const float* dc_dA,
+                           const float* dc_dL, const double* grad_c, double* grad_A,
+                           double* grad_L, void* stream);
+
 /* Ragged layered batch (the batched persistent solver's native input). */
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* batch, const dazi_dk_tune* tune,
                             void* stream);

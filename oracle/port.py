@@ -417,14 +417,17 @@ def ti_column(vs, periods, depz, minthk):
     return pv, lsen.reshape(nz - 1, kmax).T.copy()
 
 
-def ti_at(vs, periods, cp, depz, minthk):
+def ti_at(vs, periods, cp, depz, minthk, layers=False):
     """Reference TI kernels at GIVEN float32 phase velocities cp (kmax,): the model
     of depthkernel_ti_column restated (Brocher and refineLayerMdl through this
     port, moduli TA = rho vp^2, TL = rho vs^2, TF = TA - 2 TL in float32,
     depthkernel_ti_worker.f90:118-142), the reference's own tregn96
     (tregn96.f:52) and the float32 splice restated
     (depthkernel_ti_worker.f90:155-164).  lsen (kmax, nz-1) float32; NaN rows
-    where cp = 0 (tregn96 at c = 0), as the reference."""
+    where cp = 0 (tregn96 at c = 0), as the reference.  With layers=True also
+    returns the chained refined-layer kernels dcR_dA, dcR_dL (kmax, L) float32
+    of depthkernelTI.f90:155-156 that the splice sums, and the moduli TA, TL
+    (L,) float32 it weights them with."""
     if not ref_available():
         raise RuntimeError("reference library not built")
     vs = _f32(vs)
@@ -468,6 +471,8 @@ def ti_at(vs, periods, cp, depz, minthk):
                 # left to right, as the reference: (lsen + dA*TA) + dL*TL
                 lsen[:, j] = (lsen[:, j] + tA[:, k]) + tL[:, k]
                 k += 1
+    if layers:
+        return lsen, dA, dL, TA[:L].copy(), TL[:L].copy()
     return lsen
 
 
