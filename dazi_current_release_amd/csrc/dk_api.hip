@@ -6,7 +6,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <condition_variable>
 #include <cstdarg>
+#include <mutex>
+#include <thread>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -121,6 +124,26 @@ struct Pool {
   size_t geo_bytes = 0;
   std::vector<void*> retired;  // buffers replaced by growth (see pool_get)
   size_t retired_bytes = 0;
+  // bumped by every call that may overwrite the stage's work slots; a host
+  // call may reuse the root search of the previous host call only while it is
+  // unchanged
+  uint64_t gen = 0;
+  // the previous host entry point call: its inputs, launch configuration and
+  // the pool generation it left (HostKey)
+  struct HostKey {
+    bool valid = false;
+    uint64_t gen = 0;
+    int nx = 0, ny = 0, nz = 0, kmax = 0, mode = 0;
+    float minthk = 0.0f;
+    dazi_dk_tune tune = {};
+    std::vector<float> vel, vp, rho, depz;
+    std::vector<double> t;
+  } hkey;
+  // pinned staging of the host entry points' outputs (two buffers: the DMA
+  // of one chunk overlaps the host copy of the previous)
+  void* stage_buf[2] = {nullptr, nullptr};
+  hipEvent_t stage_ev[2] = {};
+  bool stage_init = false;
 };
 
 Pool g_pool[16];
@@ -350,13 +373,171 @@ int geometry(Pool* P, const dazi_dk_desc* d, const float* depz_dev, const float*
   return 0;
 }
 
+// A small pool of host threads that splits large host memcpys (the staged
+// copy-out of the host entry points), created on first use.
+class CopyThreads {
+ public:
+  static constexpr int kWorkers = 3;
+  ~CopyThreads() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  void copy(void* dst, const void* src, size_t n) {
+    if (n < ((size_t)1 << 20)) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    start();
+    const size_t part = (n / (kWorkers + 1) + 63) / 64 * 64;
+    {
+      std::lock_guard<std::mutex> g(m_);
+      dst_ = (char*)dst;
+      src_ = (const char*)src;
+      n_ = n;
+      part_ = part;
+      pending_ = kWorkers;
+      epoch_++;
+    }
+    cv_.notify_all();
+    std::memcpy(dst, src, std::min(part, n));
+    std::unique_lock<std::mutex> lk(m_);
+    done_cv_.wait(lk, [&] { return pending_ == 0; });
+  }
+
+ private:
+  void start() {
+    if (!th_.empty()) return;
+    for (int w = 0; w < kWorkers; w++) th_.emplace_back([this, w] { run(w + 1); });
+  }
+  void run(int slice) {
+    uint64_t seen = 0;
+    while (true) {
+      char* d;
+      const char* s;
+      size_t n, part;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || epoch_ != seen; });
+        if (stop_) return;
+        seen = epoch_;
+        d = dst_;
+        s = src_;
+        n = n_;
+        part = part_;
+      }
+      const size_t off = (size_t)slice * part;
+      if (off < n) std::memcpy(d + off, s + off, std::min(part, n - off));
+      {
+        std::lock_guard<std::mutex> g(m_);
+        if (--pending_ == 0) done_cv_.notify_all();
+      }
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  char* dst_ = nullptr;
+  const char* src_ = nullptr;
+  size_t n_ = 0, part_ = 0;
+  int pending_ = 0;
+  uint64_t epoch_ = 0;
+  bool stop_ = false;
+};
+CopyThreads g_copy;
+
+constexpr size_t kStageChunk = (size_t)8 << 20;
+
+// Copies `bytes` of device memory at src to the caller's host memory at dst
+// after everything enqueued on `st`: chunks go by DMA (the side stream) into
+// the two pinned staging buffers and from there by host threads to dst, the
+// DMA of chunk c+1 overlapping the host copy of chunk c.  Synchronous.
+int copy_out(Pool* P, void* dst, const void* src, size_t bytes, hipStream_t st) {
+  if (bytes == 0) return 0;
+  if (!P->stage_init) {
+    for (int i = 0; i < 2; i++) {
+      HIPCHK(hipHostMalloc(&P->stage_buf[i], kStageChunk, hipHostMallocDefault));
+      HIPCHK(hipEventCreateWithFlags(&P->stage_ev[i], hipEventDisableTiming));
+    }
+    P->stage_init = true;
+  }
+  HIPCHK(hipEventRecord(P->fk[3], st));
+  HIPCHK(hipStreamWaitEvent(P->side, P->fk[3], 0));
+  const size_t nchunk = (bytes + kStageChunk - 1) / kStageChunk;
+  auto issue = [&](size_t c) -> int {
+    const size_t off = c * kStageChunk, n = std::min(kStageChunk, bytes - off);
+    HIPCHK(hipMemcpyAsync(P->stage_buf[c & 1], (const char*)src + off, n, hipMemcpyDeviceToHost,
+                          P->side));
+    HIPCHK(hipEventRecord(P->stage_ev[c & 1], P->side));
+    return 0;
+  };
+  if (int rc = issue(0)) return rc;
+  for (size_t c = 0; c < nchunk; c++) {
+    // buffer (c+1)&1 held chunk c-1, whose host copy ended last iteration
+    if (c + 1 < nchunk)
+      if (int rc = issue(c + 1)) return rc;
+    HIPCHK(hipEventSynchronize(P->stage_ev[c & 1]));
+    const size_t off = c * kStageChunk, n = std::min(kStageChunk, bytes - off);
+    g_copy.copy((char*)dst + off, P->stage_buf[c & 1], n);
+  }
+  return 0;
+}
+
+// True when a host call with these inputs may reuse the root search of the
+// previous host call (same inputs, same launch configuration, and no other
+// call has touched the pool since).
+bool host_key_match(const Pool* P, const dazi_dk_desc* d, const float* vel, const float* vp,
+                    const float* rho, const float* depz, const double* t) {
+  const auto& k = P->hkey;
+  const size_t nv = (size_t)d->nx * d->ny * d->nz;
+  const int mode = d->vp_rho_mode == 1 ? 1 : 0;
+  if (!k.valid || k.gen != P->gen || k.nx != d->nx || k.ny != d->ny || k.nz != d->nz ||
+      k.kmax != d->kmax || k.mode != mode || k.minthk != d->minthk0 ||
+      std::memcmp(&k.tune, &g_tune, sizeof g_tune) != 0)
+    return false;
+  if (std::memcmp(k.vel.data(), vel, nv * 4) || std::memcmp(k.depz.data(), depz, d->nz * 4) ||
+      std::memcmp(k.t.data(), t, (size_t)d->kmax * 8))
+    return false;
+  if (mode && (std::memcmp(k.vp.data(), vp, nv * 4) || std::memcmp(k.rho.data(), rho, nv * 4)))
+    return false;
+  return true;
+}
+
+void host_key_store(Pool* P, const dazi_dk_desc* d, const float* vel, const float* vp,
+                    const float* rho, const float* depz, const double* t) {
+  auto& k = P->hkey;
+  const size_t nv = (size_t)d->nx * d->ny * d->nz;
+  k.valid = true;
+  k.gen = P->gen;
+  k.nx = d->nx;
+  k.ny = d->ny;
+  k.nz = d->nz;
+  k.kmax = d->kmax;
+  k.mode = d->vp_rho_mode == 1 ? 1 : 0;
+  k.minthk = d->minthk0;
+  k.tune = g_tune;
+  k.vel.assign(vel, vel + nv);
+  k.depz.assign(depz, depz + d->nz);
+  k.t.assign(t, t + d->kmax);
+  if (k.mode) {
+    k.vp.assign(vp, vp + nv);
+    k.rho.assign(rho, rho + nv);
+  } else {
+    k.vp.clear();
+    k.rho.clear();
+  }
+}
+
 // The whole stage on device pointers: root search, optional isotropic node
 // kernels (svs/svp/srho) and optional TI splice (lsen).
 int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel,
                const float* vp, const float* rho, const float* depz_dev,
                const float* depz_host, const double* tRc, double* pv, double* svs,
                double* svp, double* srho, float* lsen, int32_t* status, hipStream_t st,
-               float* dcdA = nullptr, float* dcdL = nullptr) {
+               float* dcdA = nullptr, float* dcdL = nullptr, bool reuse = false) {
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
   // plain mode's tile: the para.in control (pdsurf_tile_columns /
@@ -428,6 +609,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   P->have_events = g_timing;
   P->last_L = L;
   P->last_ti_threads = ti ? ti_nthr : 0;
+  if (!reuse) P->gen++;
   rec(P, 0, st);
   if (!G->built) {
     HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, (float*)geo_depz, st));
@@ -441,10 +623,13 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
                               (dk::TiLayerDev*)til, (int*)tiq, P->side));
     HIPCHK(hipEventRecord(P->fk[2], P->side));
   }
-  HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                               d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, geo_depz, L, geo,
-                               (double4*)layers, (double*)btp, btr, (double*)cc, (float*)betmx,
-                               (int*)bad, st));
+  // a reused call keeps the flattened layers, start velocities and roots that
+  // the previous host call left in these slots
+  if (!reuse)
+    HIPCHK(dk::launch_prep_nodes(ncol, d->nz, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                                 d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, geo_depz, L, geo,
+                                 (double4*)layers, (double*)btp, btr, (double*)cc, (float*)betmx,
+                                 (int*)bad, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = ncol;
@@ -463,7 +648,11 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   B.status = (int*)stat;
   B.work = (int*)work;
   B.tol = T.refine_tol;
-  if (int rc = run_forward(P, B, T, L, st)) return rc;
+  if (!reuse) {
+    if (int rc = run_forward(P, B, T, L, st)) return rc;
+  } else {
+    P->nforward = 0;
+  }
   rec(P, 2, st);
   if (kern) {
     dk::KernDev O = {};
@@ -550,6 +739,70 @@ int grows_impl(const dazi_dk_grow_desc* d, const int32_t* ray_k, const int64_t* 
   g.row_off = (int64_t*)off;
   g.nar = nar;
   HIPCHK(dk::launch_grows(g, st));
+  return 0;
+}
+
+// The host entry points (the Fortran drop-in): inputs copied to the pool's
+// host-path slots, the stage, outputs copied back through pinned staging.  A
+// call whose inputs are byte-identical to the previous host call's (the G
+// builder calls depthkernelTI and then depthkernel on the same model,
+// CalSurfGAniso_Joint.f90:478,486) reuses that call's root search and runs
+// only its own kernel pass.
+int host_stage(const dazi_dk_desc* d, const float* vel, const float* vp, const float* rho,
+               const float* depz, const double* tRc, double* pvRc, double* svs, double* svp,
+               double* srho, float* lsen) {
+  Pool* P;
+  if (int rc = get_pool(&P)) return rc;
+  const size_t ncol = (size_t)d->nx * d->ny, nz = d->nz, K = d->kmax;
+  const bool mode1 = d->vp_rho_mode == 1;
+  const bool kern = svs != nullptr;
+  void *dvel, *dvp = nullptr, *drho = nullptr, *ddepz, *dt, *dpv, *ds1 = nullptr, *ds2 = nullptr,
+              *ds3 = nullptr, *dls = nullptr;
+  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
+  if (mode1) {
+    if (int rc = pool_get(P, S_H_VP, ncol * nz * 4, &dvp)) return rc;
+    if (int rc = pool_get(P, S_H_RHO, ncol * nz * 4, &drho)) return rc;
+  }
+  if (int rc = pool_get(P, S_H_DEPZ, nz * 4, &ddepz)) return rc;
+  if (int rc = pool_get(P, S_H_T, K * 8, &dt)) return rc;
+  if (int rc = pool_get(P, S_H_PV, ncol * K * 8, &dpv)) return rc;
+  if (kern) {
+    if (int rc = pool_get(P, S_H_SVS, ncol * K * nz * 8, &ds1)) return rc;
+    if (int rc = pool_get(P, S_H_SVP, ncol * K * nz * 8, &ds2)) return rc;
+    if (int rc = pool_get(P, S_H_SRHO, ncol * K * nz * 8, &ds3)) return rc;
+  }
+  if (lsen)
+    if (int rc = pool_get(P, S_H_LSEN, ncol * K * (nz - 1) * 4, &dls)) return rc;
+  // (sizing can grow a slot and so retire the previous call's buffers: the
+  // generation check below runs after it)
+  const bool reuse = host_key_match(P, d, vel, vp, rho, depz, tRc);
+  hipStream_t st = nullptr;
+  if (!reuse) {
+    HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
+    if (mode1) {
+      HIPCHK(hipMemcpy(dvp, vp, ncol * nz * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(drho, rho, ncol * nz * 4, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
+  }
+  int rc = stage_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
+                      (const float*)ddepz, depz, (const double*)dt, (double*)dpv, (double*)ds1,
+                      (double*)ds2, (double*)ds3, (float*)dls, nullptr, st, nullptr, nullptr,
+                      reuse);
+  if (rc) {
+    P->hkey.valid = false;
+    return rc;
+  }
+  host_key_store(P, d, vel, vp, rho, depz, tRc);
+  if (int r = copy_out(P, pvRc, dpv, ncol * K * 8, st)) return r;
+  if (kern) {
+    if (int r = copy_out(P, svs, ds1, ncol * K * nz * 8, st)) return r;
+    if (int r = copy_out(P, svp, ds2, ncol * K * nz * 8, st)) return r;
+    if (int r = copy_out(P, srho, ds3, ncol * K * nz * 8, st)) return r;
+  }
+  if (lsen)
+    if (int r = copy_out(P, lsen, dls, ncol * K * (nz - 1) * 4, st)) return r;
   return 0;
 }
 
@@ -664,6 +917,7 @@ int32_t dazi_dk_grows(const dazi_dk_grow_desc* d, const int32_t* ray_k, const in
     return set_err(DAZI_DK_EINVAL, "cap > 0 needs rw, iw and col");
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
+  P->gen++;  // the host-path input slots are reused below
   const size_t ncol = (size_t)(d->nvx + 2) * (d->nvz + 2), nz = d->nz, K = d->kmax;
   const size_t nc = (size_t)cell_off[nray], nr = (size_t)(nray > 0 ? nray : 1);
   void *dk_, *doff, *dcell, *dw, *dvel, *d1, *d2, *d3, *dls, *dnar, *drw = nullptr,
@@ -745,6 +999,7 @@ int32_t dazi_dk_azimuth_map(const dazi_dk_azim_desc* d, const float* lsen, const
     return set_err(DAZI_DK_EINVAL, "NULL required argument");
   Pool* P;
   if (int rc = get_pool(&P)) return rc;
+  P->gen++;  // the host-path output slot S_H_LSEN is reused below
   const size_t ncol = (size_t)d->nx * d->ny, nd = d->nz - 1, K = d->kmax;
   const size_t cells = (size_t)(d->nx - 2) * (d->ny - 2);
   void *dl, *dg, *dt, *dout;
@@ -1112,26 +1367,8 @@ int32_t dazi_dk_ti(const dazi_dk_desc* d, const float* vel, const float* depz, c
                    d->kmax);
   if (d->vp_rho_mode != 0)
     return set_err(DAZI_DK_EINVAL, "dazi_dk_ti derives vp and rho from vs (vp_rho_mode 0)");
-  Pool* P;
-  if (int rc = get_pool(&P)) return rc;
-  size_t ncol = (size_t)d->nx * d->ny, nz = d->nz, K = d->kmax;
-  void *dvel, *ddepz, *dt, *dpv, *dls;
-  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
-  if (int rc = pool_get(P, S_H_DEPZ, nz * 4, &ddepz)) return rc;
-  if (int rc = pool_get(P, S_H_T, K * 8, &dt)) return rc;
-  if (int rc = pool_get(P, S_H_PV, ncol * K * 8, &dpv)) return rc;
-  if (int rc = pool_get(P, S_H_LSEN, ncol * K * (nz - 1) * 4, &dls)) return rc;
-  HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
-  int rc = stage_impl(d, nullptr, (const float*)dvel, nullptr, nullptr, (const float*)ddepz, depz,
-                      (const double*)dt, (double*)dpv, nullptr, nullptr, nullptr, (float*)dls,
-                      nullptr, nullptr);
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(nullptr));
-  HIPCHK(hipMemcpy(pvRc, dpv, ncol * K * 8, hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(Lsen_Gsc, dls, ncol * K * (nz - 1) * 4, hipMemcpyDeviceToHost));
-  return 0;
+  return host_stage(d, vel, nullptr, nullptr, depz, tRc, pvRc, nullptr, nullptr, nullptr,
+                    Lsen_Gsc);
 }
 
 int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, const float* rho,
@@ -1141,46 +1378,12 @@ int32_t dazi_dk_iso(const dazi_dk_desc* d, const float* vel, const float* vp, co
   if (d->nx < 1 || d->ny < 1 || d->nz < 2 || d->kmax < 1)
     return set_err(DAZI_DK_EINVAL, "bad sizes nx=%d ny=%d nz=%d kmax=%d", d->nx, d->ny, d->nz,
                    d->kmax);
-  Pool* P;
-  if (int rc = get_pool(&P)) return rc;
-  size_t ncol = (size_t)d->nx * d->ny, nz = d->nz, K = d->kmax;
-  bool kern = sen_vsRc != nullptr;
-  void *dvel, *dvp = nullptr, *drho = nullptr, *ddepz, *dt, *dpv, *ds1 = nullptr, *ds2 = nullptr,
-              *ds3 = nullptr;
-  if (int rc = pool_get(P, S_H_VEL, ncol * nz * 4, &dvel)) return rc;
-  if (d->vp_rho_mode == 1) {
-    if (!vp || !rho) return set_err(DAZI_DK_EINVAL, "vp_rho_mode=1 needs vp and rho");
-    if (int rc = pool_get(P, S_H_VP, ncol * nz * 4, &dvp)) return rc;
-    if (int rc = pool_get(P, S_H_RHO, ncol * nz * 4, &drho)) return rc;
-  }
-  if (int rc = pool_get(P, S_H_DEPZ, nz * 4, &ddepz)) return rc;
-  if (int rc = pool_get(P, S_H_T, K * 8, &dt)) return rc;
-  if (int rc = pool_get(P, S_H_PV, ncol * K * 8, &dpv)) return rc;
-  if (kern) {
-    if (int rc = pool_get(P, S_H_SVS, ncol * K * nz * 8, &ds1)) return rc;
-    if (int rc = pool_get(P, S_H_SVP, ncol * K * nz * 8, &ds2)) return rc;
-    if (int rc = pool_get(P, S_H_SRHO, ncol * K * nz * 8, &ds3)) return rc;
-  }
-  hipStream_t st = nullptr;
-  HIPCHK(hipMemcpy(dvel, vel, ncol * nz * 4, hipMemcpyHostToDevice));
-  if (dvp) {
-    HIPCHK(hipMemcpy(dvp, vp, ncol * nz * 4, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(drho, rho, ncol * nz * 4, hipMemcpyHostToDevice));
-  }
-  HIPCHK(hipMemcpy(ddepz, depz, nz * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dt, tRc, K * 8, hipMemcpyHostToDevice));
-  int rc = stage_impl(d, nullptr, (const float*)dvel, (const float*)dvp, (const float*)drho,
-                      (const float*)ddepz, depz, (const double*)dt, (double*)dpv, (double*)ds1,
-                      (double*)ds2, (double*)ds3, nullptr, nullptr, st);
-  if (rc) return rc;
-  HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipMemcpy(pvRc, dpv, ncol * K * 8, hipMemcpyDeviceToHost));
-  if (kern) {
-    HIPCHK(hipMemcpy(sen_vsRc, ds1, ncol * K * nz * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(sen_vpRc, ds2, ncol * K * nz * 8, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(sen_rhoRc, ds3, ncol * K * nz * 8, hipMemcpyDeviceToHost));
-  }
-  return 0;
+  if (d->vp_rho_mode == 1 && (!vp || !rho))
+    return set_err(DAZI_DK_EINVAL, "vp_rho_mode=1 needs vp and rho");
+  const bool kern = sen_vsRc || sen_vpRc || sen_rhoRc;
+  if (kern && !(sen_vsRc && sen_vpRc && sen_rhoRc))
+    return set_err(DAZI_DK_EINVAL, "sen_vs, sen_vp, sen_rho must be all set or all NULL");
+  return host_stage(d, vel, vp, rho, depz, tRc, pvRc, sen_vsRc, sen_vpRc, sen_rhoRc, nullptr);
 }
 
 int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, void* stream) {
@@ -1223,6 +1426,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     return rc;
   }
   if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
+  P->gen++;
   P->have_events = g_timing;
   P->last_L = 0;
   P->last_ti_threads = 0;
@@ -1431,6 +1635,7 @@ int32_t dazi_dk_layered_vjp_dev(const dazi_dk_layered* b, const double* grad_c, 
   int L = b->max_layers;
   if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L > 1 ? L - 1 : 1) * nblocks * 64 * 8, &tape))
     return rc;
+  P->gen++;
   P->have_events = g_timing;
   rec(P, 0, st);
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,

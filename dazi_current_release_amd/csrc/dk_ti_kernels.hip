@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restri
 // zeros where no kernel exists (no phase velocity, vanishing surface minor),
 // or NaN for a missing phase velocity with nan_failed.  Returns solve<G_>'s
 // code; on -2 nothing is written.
-template <class G_>
+template <class G_, bool KL>
 __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* __restrict__ layers,
                       const double* __restrict__ periods, const double* __restrict__ pv,
                       const int* __restrict__ nsub, float* __restrict__ lsen, float* __restrict__ dcdA,
@@ -129,7 +129,7 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   auto fill_all = [&](float v) {
     if (lsen)
       for (int j = 0; j < nz - 1; j++) out(j) = v;
-    if (dcdA)
+    if constexpr (KL)
       for (int m = 0; m < L; m++) {
         lay_out(dcdA, m) = v;
         lay_out(dcdL, m) = v;
@@ -159,13 +159,16 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   auto vtpf = [&](int m) { return lay[m].vtp; };
   auto emit = [&](int m, float dah, float dbv, float dn) {
     const TiLayerDev& t = lay[m];
+    if constexpr (!KL) {
+      if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
+    }
     float dA, dL;
     ti::chain_AL(t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn, &dA, &dL);
-    if (dcdA) {
+    if constexpr (KL) {
       lay_out(dcdA, m) = dA;
       lay_out(dcdL, m) = dL;
+      if (!lsen || m >= L - 1 || interval >= nz - 1) return;
     }
-    if (!lsen || m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
     acc = ti::splice_add(acc, t.TA, t.TL, dA, dL);
     if (++sub == nsub[interval]) {
       out(interval) = acc;
@@ -179,7 +182,11 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   return rc;
 }
 
-__global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
+// KL: also write the per-layer kernels dcdA, dcdL (a variant of its own, so
+// that the splice-only kernel keeps its register budget of two waves per SIMD)
+template <bool KL>
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(2)))
+k_ti(int ncol, int K, int L, int nz,
                                             const TiLayerDev* __restrict__ layers,
                                             const double* __restrict__ periods,
                                             const double* __restrict__ pv,
@@ -191,12 +198,13 @@ __global__ void __launch_bounds__(128) k_ti(int ncol, int K, int L, int nz,
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const long nsolves = (long)ncol * K;
   for (long s = tid; s < nsolves; s += nthr) {
-    const int rc = ti_one<ti::Eig>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA,
-                                   dcdL, scratch + tid, nthr, nan_failed);
+    const int rc = ti_one<ti::Eig, KL>(s, ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA,
+                                       dcdL, scratch + tid, nthr, nan_failed);
     if (rc == -2) queue[1 + atomicAdd(queue, 1)] = (int)s;
   }
 }
 
+template <bool KL>
 __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
                                               const TiLayerDev* __restrict__ layers,
                                               const double* __restrict__ periods,
@@ -211,8 +219,8 @@ __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = queue[0];
   for (long i = tid; i < n; i += nthr)
-    ti_one<ti::EigC>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA, dcdL,
-                     scratch + tid, nthr, 0);
+    ti_one<ti::EigC, KL>(queue[1 + i], ncol, K, L, nz, layers, periods, pv, nsub, lsen, dcdA,
+                         dcdL, scratch + tid, nthr, 0);
 }
 
 hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
@@ -236,16 +244,24 @@ hipError_t launch_ti(int ncol, int K, int L, int nz, const TiLayerDev* layers,
   int block = 128;
   int grid = nthr / block;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_ti, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods, pv,
-                     nsub, lsen, dcdA, dcdL, scratch, queue, nan_failed);
+  if (dcdA)
+    hipLaunchKernelGGL(k_ti<true>, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers, periods,
+                       pv, nsub, lsen, dcdA, dcdL, scratch, queue, nan_failed);
+  else
+    hipLaunchKernelGGL(k_ti<false>, dim3(grid), dim3(block), 0, st, ncol, K, L, nz, layers,
+                       periods, pv, nsub, lsen, dcdA, dcdL, scratch, queue, nan_failed);
   if (hipError_t e = hipGetLastError()) return e;
   // the queued complex-pair solves; the grid is sized for the worst case and
   // exits at once when the queue is empty
   int ncx = ti_cx_threads(nthr);
   int cap = ncol * K;
   if (ncx > (cap + 63) / 64 * 64) ncx = (cap + 63) / 64 * 64;
-  hipLaunchKernelGGL(k_ti_cx, dim3(ncx / 64), dim3(64), 0, st, ncol, K, L, nz, layers, periods,
-                     pv, nsub, lsen, dcdA, dcdL, scratch, (const int*)queue);
+  if (dcdA)
+    hipLaunchKernelGGL(k_ti_cx<true>, dim3(ncx / 64), dim3(64), 0, st, ncol, K, L, nz, layers,
+                       periods, pv, nsub, lsen, dcdA, dcdL, scratch, (const int*)queue);
+  else
+    hipLaunchKernelGGL(k_ti_cx<false>, dim3(ncx / 64), dim3(64), 0, st, ncol, K, L, nz, layers,
+                       periods, pv, nsub, lsen, dcdA, dcdL, scratch, (const int*)queue);
   return hipGetLastError();
 }
 

@@ -210,3 +210,36 @@ def test_bad_model_zero_filled(dk):
     assert (pv[3] == 0).all() and (svs[3] == 0).all()
     ok = [j for j in range(8) if j != 3]
     check_pv(pv[ok], oracle_pv(W)[ok])
+
+
+def test_host_path_reuses_root_search_bit_identically(dk):
+    """The G builder's order (depthkernelTI, then depthkernel on the same model,
+    CalSurfGAniso_Joint.f90:478,486): the second host call reuses the first
+    call's root search (no root-search launch) and its outputs equal those of
+    a call that runs its own, bit for bit; any other call in between, or other
+    inputs, cancel the reuse."""
+    from dazi_current_release_amd import _lib
+    W = node_cfg("plain_4096", ncol=256)
+    other = node_cfg("plain_1024", ncol=64)
+    args = (W["vel"], W["periods"], W["depz"], W["minthk"])
+    dk.run_depthkernel(other["vel"], other["periods"], other["depz"], other["minthk"])
+    _lib.lib().dazi_dk_enable_timing(1)
+    try:
+        tpv1, tls1 = dk.run_depthkernelTI(*args)
+        assert dk.last_timings()[4] >= 1          # its own root search
+        pv2, s2, p2, r2 = dk.run_depthkernel(*args)
+        assert dk.last_timings()[4] == 0          # reused
+        dk.run_depthkernel(other["vel"], other["periods"], other["depz"], other["minthk"])
+        pv3, s3, p3, r3 = dk.run_depthkernel(*args)
+        assert dk.last_timings()[4] >= 1
+        tpv4, tls4 = dk.run_depthkernelTI(*args)
+        assert dk.last_timings()[4] == 0          # reused after the depthkernel call
+        vel2 = W["vel"].copy()
+        vel2[0, 0, 3] *= np.float32(1.01)
+        dk.run_depthkernel(vel2, *args[1:])
+        assert dk.last_timings()[4] >= 1          # one value differs: no reuse
+    finally:
+        _lib.lib().dazi_dk_enable_timing(0)
+    for a, b in ((pv2, pv3), (s2, s3), (p2, p3), (r2, r3), (tpv1, tpv4), (tls1, tls4),
+                 (tpv1, pv3)):
+        assert np.array_equal(a, b)
