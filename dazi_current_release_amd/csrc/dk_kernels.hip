@@ -495,14 +495,11 @@ __device__ __forceinline__ void sincos_phase(double p, double* sp, double* cp) {
 // of F below 1e-13 relative.)
 __device__ __forceinline__ double hyp_fwd(double X, double d, double* C, double* S, double* R,
                                           double* ex) {
+  // X = 0 (the trial velocity equal to the layer's) is the limit of the
+  // oscillatory form: taken at X = -1e-300 it gives C = 1, S = d, R = 0 to
+  // rounding, without a branch of its own
+  if (X == 0.0) X = -1.0e-300;
   const double ax = fabs(X);
-  if (!(ax > 0.0)) {
-    *C = 1.0;
-    *S = d;
-    *R = 0.0;
-    *ex = 0.0;
-    return 1.0;
-  }
   double y = __builtin_amdgcn_rsq(ax);
   y = y * fma(-0.5 * ax * y, y, 1.5);  // one Newton step: 1/sqrt|X|
   const double ra = ax * y;            // sqrt|X|
