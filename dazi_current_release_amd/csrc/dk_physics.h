@@ -18,7 +18,7 @@
 #include <math.h>
 #include <stdint.h>
 
-#define DK_HD __host__ __device__ inline
+#include "dk_fastmath.h"
 
 namespace dk {
 
@@ -183,89 +183,6 @@ struct Num<Dual<N>> {
 template <class T>
 DK_HD T tconst(double v) {
   return Num<T>::make(v);
-}
-
-// ----------------------------------------------------------------- elementary functions
-// exp(-p) for p >= 0 (0 once it underflows), for the secular function and
-// its derivatives: Cody-Waite reduction p = n ln2 - r with the two-part ln2 of
-// fdlibm, r in [-ln2/2, ln2/2], and the degree-11 Taylor polynomial of e^r
-// (remainder below 7e-15 relative), scaled by 2^-n.
-// a*b + c as one three-operand v_fma_f64 (the backend's two-operand
-// v_fmac_f64 form would first copy a loop-invariant addend into the result
-// register, one extra move per Horner step)
-DK_HD double fma3(double a, double b, double c) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  double d;
-  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
-  return d;
-#else
-  return fma(a, b, c);
-#endif
-}
-
-DK_HD double exp_neg(double p) {
-  p = fmin(p, 745.0);  // beyond, e^-p is below the smallest denormal
-  const double n = rint(p * 1.4426950408889634074);
-  double r = fma(n, 6.93147180369123816490e-01, -p);
-  r = fma(n, 1.90821492927058770002e-10, r);
-  double q = 2.5052108385441718775e-08;              // 1/11!
-  q = fma3(q, r, 2.7557319223985890653e-07);          // 1/10!
-  q = fma3(q, r, 2.7557319223985892511e-06);          // 1/9!
-  q = fma3(q, r, 2.4801587301587301566e-05);          // 1/8!
-  q = fma3(q, r, 1.9841269841269841253e-04);          // 1/7!
-  q = fma3(q, r, 1.3888888888888889419e-03);          // 1/6!
-  q = fma3(q, r, 8.3333333333333332177e-03);          // 1/5!
-  q = fma3(q, r, 4.1666666666666664354e-02);          // 1/4!
-  q = fma3(q, r, 1.6666666666666665741e-01);          // 1/3!
-  q = fma(q, r, 0.5);
-  q = fma(q, r, 1.0);
-  q = fma(q, r, 1.0);
-  return ldexp(q, -(int)n);
-}
-
-// sin and cos of p, 0 <= p below about 1e5 (a layer's phase p = sqrt(-X) d is
-// at most omega d / beta): p = n pi/2 + r with the
-// three-part pi/2 of fdlibm (Cody-Waite), r in [-pi/4, pi/4], Taylor
-// polynomials to degree 15 and 16 (remainders below 3e-14 and 1e-15), and the
-// quadrant n mod 4.
-DK_HD void sincos_phase(double p, double* sp, double* cp) {
-  const double n = rint(p * 6.36619772367581382433e-01);
-  double r = fma(n, -1.57079632673412561417e+00, p);
-  r = fma(n, -6.07710050650619224932e-11, r);
-  r = fma(n, -2.02226624879595063154e-21, r);
-  const double r2 = r * r;
-  double ps = 7.6471637318198164759e-13;               // 1/15!
-  ps = fma3(ps, r2, -1.6059043836821614599e-10);        // -1/13!
-  ps = fma3(ps, r2, 2.5052108385441718775e-08);         // 1/11!
-  ps = fma3(ps, r2, -2.7557319223985892511e-06);        // -1/9!
-  ps = fma3(ps, r2, 1.9841269841269841253e-04);         // 1/7!
-  ps = fma3(ps, r2, -8.3333333333333332177e-03);        // -1/5!
-  ps = fma3(ps, r2, 1.6666666666666665741e-01);         // 1/3!, subtracted below
-  const double sn = fma(-r * r2, ps, r);
-  double pc = 4.7794773323873852974e-14;               // 1/16!
-  pc = fma3(pc, r2, -1.1470745597729724714e-11);        // -1/14!
-  pc = fma3(pc, r2, 2.0876756987868098979e-09);         // 1/12!
-  pc = fma3(pc, r2, -2.7557319223985890653e-07);        // -1/10!
-  pc = fma3(pc, r2, 2.4801587301587301566e-05);         // 1/8!
-  pc = fma3(pc, r2, -1.3888888888888889419e-03);        // -1/6!
-  pc = fma3(pc, r2, 4.1666666666666664354e-02);         // 1/4!
-  pc = fma(pc, r2, -0.5);
-  const double cs = fma(pc, r2, 1.0);
-  const int q = (int)n & 3;
-  const double s0 = (q & 1) ? cs : sn, c0 = (q & 1) ? sn : cs;
-  *sp = (q & 2) ? -s0 : s0;
-  *cp = ((q + 1) & 2) ? -c0 : c0;
-}
-
-
-// 1/sqrt(x) for x > 0: the hardware estimate and one Newton step.
-DK_HD double rsqrt_nr(double x) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  double y = __builtin_amdgcn_rsq(x);
-  return y * fma(-0.5 * x * y, y, 1.5);
-#else
-  return 1.0 / sqrt(x);
-#endif
 }
 
 // ----------------------------------------------------------------- layer matrix

@@ -27,9 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
-#ifndef DK_HD
-#define DK_HD __host__ __device__ inline
-#endif
+#include "dk_fastmath.h"
 
 namespace dk {
 namespace ti {
@@ -57,6 +55,12 @@ DK_HD cx operator/(cx a, cx b) {
 DK_HD cx operator/(cx a, double s) { return cx{a.r / s, a.i / s}; }
 DK_HD cx operator/(double s, cx b) { return cx{s, 0.0} / b; }
 DK_HD double cabs_(cx a) { return hypot(a.r, a.i); }
+// a / b through 1/|b|^2: one division, for divisors of physical size (vertical
+// wavenumbers and their sums), where |b|^2 neither overflows nor underflows
+DK_HD cx cdiv_fast(cx a, cx b) {
+  const double inv = 1.0 / (b.r * b.r + b.i * b.i);
+  return cx{(a.r * b.r + a.i * b.i) * inv, (a.i * b.r - a.r * b.i) * inv};
+}
 // Scalar-generic helpers: the layer algebra below is written once for real
 // (double) and complex (cx) eigenvector components.
 DK_HD cx mk(cx a) { return a; }
@@ -77,11 +81,6 @@ DK_HD cx csqrt_(cx z) {
   double t = sqrt(0.5 * (m + fabs(z.r)));
   if (z.r >= 0.0) return cx{t, z.i / (2.0 * t)};
   return cx{fabs(z.i) / (2.0 * t), copysign(t, z.i)};
-}
-DK_HD cx cexp_(cx z) {
-  double e = exp(z.r), s, c;
-  sincos(z.i, &s, &c);
-  return cx{e * c, e * s};
 }
 
 // ----------------------------------------------------------------- layer model
@@ -162,6 +161,7 @@ struct Eig {
   using H = Hyp;      // its layer phase functions
   double L1, L2;      // nu_P^2, nu_SV^2
   double nP, nS;      // |nu_P|, |nu_SV|
+  double rnP, rnS;    // 1/|nu_P|, 1/|nu_SV|
   int iP, iS;         // nu imaginary
   double x11, x21, x31, x41, x12, x22, x32, x42;
   double NP, NSV;
@@ -186,8 +186,11 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
   o->L2 = L2;
   o->iP = L1 < 0.0;
   o->iS = L2 < 0.0;
-  o->nP = sqrt(fabs(L1));
-  o->nS = sqrt(fabs(L2));
+  const double aP = fabs(L1), aS = fabs(L2);
+  o->rnP = rsqrt_nr(aP);
+  o->rnS = rsqrt_nr(aS);
+  o->nP = aP > 0.0 ? aP * o->rnP : 0.0;
+  o->nS = aS > 0.0 ? aS * o->rnS : 0.0;
   double x[8];
   eigvec(q, wvno, L1, L2, x);
   o->x11 = x[0];
@@ -207,10 +210,12 @@ DK_HD cx nu_P(const Eig& g) { return g.iP ? cx{0.0, g.nP} : cx{g.nP, 0.0}; }
 DK_HD cx nu_S(const Eig& g) { return g.iS ? cx{0.0, g.nS} : cx{g.nS, 0.0}; }
 // 1/(2 nu N): nu real -> real, nu = i|nu| -> -i/(2|nu|N)
 DK_HD cx inv2nuN_P(const Eig& g) {
-  return g.iP ? cx{0.0, -0.5 / (g.nP * g.NP)} : cx{0.5 / (g.nP * g.NP), 0.0};
+  const double v = (0.5 / g.NP) * g.rnP;
+  return g.iP ? cx{0.0, -v} : cx{v, 0.0};
 }
 DK_HD cx inv2nuN_S(const Eig& g) {
-  return g.iS ? cx{0.0, -0.5 / (g.nS * g.NSV)} : cx{0.5 / (g.nS * g.NSV), 0.0};
+  const double v = (0.5 / g.NSV) * g.rnS;
+  return g.iS ? cx{0.0, -v} : cx{v, 0.0};
 }
 DK_HD double half_inv(double N) { return 0.5 / N; }
 
@@ -277,8 +282,8 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, EigC* o) {
 
 DK_HD cx nu_P(const EigC& g) { return g.rp; }
 DK_HD cx nu_S(const EigC& g) { return g.rs; }
-DK_HD cx inv2nuN_P(const EigC& g) { return 0.5 / (g.rp * g.NP); }
-DK_HD cx inv2nuN_S(const EigC& g) { return 0.5 / (g.rs * g.NSV); }
+DK_HD cx inv2nuN_P(const EigC& g) { return cdiv_fast(mk(0.5), g.rp * g.NP); }
+DK_HD cx inv2nuN_S(const EigC& g) { return cdiv_fast(mk(0.5), g.rs * g.NSV); }
 DK_HD cx half_inv(cx N) { return 0.5 / N; }
 
 // Eigenvector matrix E (columns P+, S+, P-, S-) and its inverse G (evalg).
@@ -323,35 +328,44 @@ DK_HD void eig_matrices(const G_& g, cx E[4][4], cx G[4][4]) {
 
 // Exponentially scaled cosh/sinh of the P and SV phases (varsv), real:
 // nu real: cosh, nu sinh, sinh/nu of p = nu d scaled by e^{-p}; nu = i|nu|:
-// cos, -|nu| sin, sin/|nu| of |nu| d.
+// cos, -|nu| sin, sin/|nu| of |nu| d.  Per wave also its scale e^{-Re p}
+// (scl) and the exponential e^{-p} itself (e), which the layer integrals use.
 struct Hyp {
   double cosp, rsinp, sinpr, cossv, rsinsv, sinsvr;
   double pex, svex;
+  double sclp, sclsv;
+  cx eP, eS;
 };
 
-DK_HD void hyp_wave(int imag, double n, double dm, double* co, double* rsi, double* sir,
-                    double* ex) {
+DK_HD void hyp_wave(int imag, double n, double rn, double dm, double* co, double* rsi,
+                    double* sir, double* ex, double* scl, cx* e) {
   const double p = n * dm;
   if (!imag) {
-    const double fac = (p < 15.0) ? exp(-2.0 * p) : 0.0;
+    const double s = exp_neg(p);
+    const double fac = (p < 15.0) ? s * s : 0.0;  // var drops e^{-2p} from p = 15
     const double sh = 0.5 - 0.5 * fac;
     *co = 0.5 + 0.5 * fac;
     *rsi = n * sh;
-    *sir = (p < 1.0e-5 && n < 1.0e-5) ? dm : sh / n;
+    *sir = (p < 1.0e-5 && n < 1.0e-5) ? dm : sh * rn;
     *ex = p;
+    *scl = s;
+    *e = cx{s, 0.0};
   } else {
     double s, c;
-    sincos(p, &s, &c);
+    sincos_phase(p, &s, &c);
     *co = c;
     *rsi = -n * s;
-    *sir = (n < 1.0e-5) ? dm : s / n;
+    *sir = (n < 1.0e-5) ? dm : s * rn;
     *ex = 0.0;
+    *scl = 1.0;
+    *e = cx{c, -s};
   }
 }
 
 DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
-  hyp_wave(g.iP, g.nP, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex);
-  hyp_wave(g.iS, g.nS, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex);
+  hyp_wave(g.iP, g.nP, g.rnP, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex, &h->sclp, &h->eP);
+  hyp_wave(g.iS, g.nS, g.rnS, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex, &h->sclsv,
+           &h->eS);
 }
 
 // The same for complex nu (varsv's general form, tregn96.f:3363-3440): with
@@ -359,24 +373,31 @@ DK_HD void hyper(const Eig& g, double dm, Hyp* h) {
 struct HypC {
   cx cosp, rsinp, sinpr, cossv, rsinsv, sinsvr;
   double pex, svex;
+  double sclp, sclsv;
+  cx eP, eS;
 };
 
-DK_HD void hyp_wave(cx n, double dm, cx* co, cx* rsi, cx* sir, double* ex) {
+// (Re nu >= 0, so Re p >= 0)
+DK_HD void hyp_wave(cx n, double dm, cx* co, cx* rsi, cx* sir, double* ex, double* scl, cx* e) {
   const cx p = n * dm;
   double s, c;
-  sincos(p.i, &s, &c);
+  sincos_phase(fabs(p.i), &s, &c);
+  if (p.i < 0.0) s = -s;
   const cx ep = cx{0.5 * c, 0.5 * s}, em = cx{0.5 * c, -0.5 * s};
-  const double fac = (p.r < 15.0) ? exp(-2.0 * p.r) : 0.0;
+  const double sc = exp_neg(p.r);
+  const double fac = (p.r < 15.0) ? sc * sc : 0.0;
   const cx sh = ep - fac * em;
   *co = ep + fac * em;
   *rsi = n * sh;
-  *sir = (fabs(p.r) < 1.0e-5 && cabs_(n) < 1.0e-5) ? mk(dm) : sh / n;
+  *sir = (fabs(p.r) < 1.0e-5 && cabs_(n) < 1.0e-5) ? mk(dm) : cdiv_fast(sh, n);
   *ex = p.r;
+  *scl = sc;
+  *e = cx{sc * c, -(sc * s)};
 }
 
 DK_HD void hyper(const EigC& g, double dm, HypC* h) {
-  hyp_wave(g.rp, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex);
-  hyp_wave(g.rs, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex);
+  hyp_wave(g.rp, dm, &h->cosp, &h->rsinp, &h->sinpr, &h->pex, &h->sclp, &h->eP);
+  hyp_wave(g.rs, dm, &h->cossv, &h->rsinsv, &h->sinsvr, &h->svex, &h->sclsv, &h->eS);
 }
 
 // Per wave type, H = [u v~] M [a~ b~]^T / N with
@@ -453,7 +474,7 @@ __host__ __device__ constexpr int RED(int r) { return r < 3 ? r : r + 1; }
 template <class G_, class H_, class T = typename G_::T>
 DK_HD void compound_step(const G_& g, const H_& h, T e[5]) {
   const double ex = h.pex + h.svex;
-  const double dfac = (ex > 35.0) ? 0.0 : exp(-ex);
+  const double dfac = (ex > 35.0) ? 0.0 : h.sclp * h.sclsv;  // e^{-ex}
   Wave<T> P, S;
   waves(g, h, 1.0, 1.0, &P, &S);
   T HP[4][4], HS[4][4];
@@ -520,35 +541,36 @@ DK_HD void halfspace_constraint(const G_& g, T cd[5]) {
   cd[4] = lift<T>(CG[5].r);
 }
 
-// ffunc, gfunc, h1func, h2func (tregn96.f:1385-1473)
+// ffunc, gfunc, h1func, h2func (tregn96.f:1385-1473), with ea = e^{-nua dm}
+// and eb = e^{-nub dm} given: every exponential they take is a product of
+// these (e^{-2 nub dm} = eb^2, e^{-(nua+nub) dm} = ea eb), with the
+// reference's cut-offs on Re of the exponent.
 DK_HD double cabs2_(cx a) { return a.r * a.r + a.i * a.i; }
-DK_HD cx ffunc(cx nub, double dm) {
+DK_HD cx ffunc(cx nub, double dm, cx eb) {
   if (cabs2_(nub) < 1.0e-16) return mk(dm);
-  cx arg = nub * dm;
-  cx ex = (arg.r < 40.0) ? cexp_(-2.0 * arg) : mk(0.0);
-  return (mk(1.0) - ex) / (2.0 * nub);
+  cx ex = (nub.r * dm < 40.0) ? eb * eb : mk(0.0);
+  return cdiv_fast(mk(1.0) - ex, 2.0 * nub);
 }
-DK_HD cx gfunc(cx nub, double dm) {
-  cx arg = nub * dm;
-  return (arg.r < 75.0) ? cexp_(-arg) * dm : mk(0.0);
+DK_HD cx gfunc(cx nub, double dm, cx eb) {
+  return (nub.r * dm < 75.0) ? eb * dm : mk(0.0);
 }
-DK_HD cx h1func(cx nua, cx nub, double dm) {
-  if (cabs2_(nub + nua) < 1.0e-16) return mk(dm);
-  cx arg = (nua + nub) * dm;
-  cx ex = (arg.r < 40.0) ? cexp_(-arg) : mk(0.0);
-  return (mk(1.0) - ex) / (nub + nua);
+DK_HD cx h1func(cx nua, cx nub, double dm, cx ea, cx eb) {
+  const cx s = nub + nua;
+  if (cabs2_(s) < 1.0e-16) return mk(dm);
+  cx ex = (s.r * dm < 40.0) ? ea * eb : mk(0.0);
+  return cdiv_fast(mk(1.0) - ex, s);
 }
-DK_HD cx h2func(cx nua, cx nub, double dm) {
+DK_HD cx h2func(cx nua, cx nub, double dm, cx ea, cx eb) {
   if (cabs2_(nub - nua) < 1.0e-16) return mk(dm);
-  cx ap = nua * dm, aq = nub * dm;
-  cx exp_ = (ap.r < 40.0) ? cexp_(-ap) : mk(0.0);
-  cx exq = (aq.r < 40.0) ? cexp_(-aq) : mk(0.0);
-  return (exq - exp_) / (nua - nub);
+  cx exp_ = (nua.r * dm < 40.0) ? ea : mk(0.0);
+  cx exq = (nub.r * dm < 40.0) ? eb : mk(0.0);
+  return cdiv_fast(exq - exp_, nua - nub);
 }
 
 // Layer integrals INT11, INT13, INT22, INT24, INT33, INT44 of the field whose
 // up-going amplitudes come from the bottom interface value yb and down-going
-// ones from the top value yt (intijr); half = 1 for the half-space.
+// ones from the top value yt (intijr); half = 1 for the half-space, else eP,
+// eS are e^{-nu dm} of the P and SV waves (unused for the half-space).
 // With the eigenvector matrix E written as E[i] = (a_i, b_i, s_i a_i, -s_i b_i),
 // a = (x11, x21 nuP, x31, x41 nuP), b = (x12 nuS, x22, x32 nuS, x42), s_i = +1
 // for even rows and -1 for odd rows, and every integral pairing rows of equal
@@ -556,7 +578,7 @@ DK_HD cx h2func(cx nua, cx nub, double dm) {
 //   Re(a_i a_j A_s + b_i b_j B_s + (a_i b_j + b_i a_j) C_s).
 template <class G_>
 DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const double yb[4],
-                           int half, double out[6]) {
+                           int half, cx eP, cx eS, double out[6]) {
   const cx ra = nu_P(g), rb = nu_S(g);
   // rows of G = E^-1: 1/(2 nu N) and nu/(2 nu N) = 1/(2N)
   const cx iP = inv2nuN_P(g), iS = inv2nuN_S(g);
@@ -571,12 +593,13 @@ DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const dou
   const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
   cx Ae, Be, Ce, Ao, Bo, Co;
   if (half) {
-    Ae = Ao = k2 * k2 / (2.0 * ra);
-    Be = Bo = k3 * k3 / (2.0 * rb);
-    Ce = Co = -(k2 * k3 / (ra + rb));
+    Ae = Ao = cdiv_fast(k2 * k2, 2.0 * ra);
+    Be = Bo = cdiv_fast(k3 * k3, 2.0 * rb);
+    Ce = Co = -cdiv_fast(k2 * k3, ra + rb);
   } else {
-    const cx FA = ffunc(ra, dm), GA = gfunc(ra, dm), FB = ffunc(rb, dm), GB = gfunc(rb, dm);
-    const cx H1 = h1func(ra, rb, dm), H2 = h2func(ra, rb, dm);
+    const cx FA = ffunc(ra, dm, eP), GA = gfunc(ra, dm, eP), FB = ffunc(rb, dm, eS),
+             GB = gfunc(rb, dm, eS);
+    const cx H1 = h1func(ra, rb, dm, eP, eS), H2 = h2func(ra, rb, dm, eP, eS);
     const cx p00 = (k0 * k0 + k2 * k2) * FA, p02 = 2.0 * (GA * (k0 * k2));
     const cx p11 = (k1 * k1 + k3 * k3) * FB, p13 = 2.0 * (GB * (k1 * k3));
     const cx p01 = H1 * (k0 * k1 - k2 * k3), p21 = H2 * (k2 * k1 - k0 * k3);
@@ -743,6 +766,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   const double exe1 = S(0, Q::EX);
   const double f1213 = -cd_re(0, 1);
   if (!(f1213 != 0.0) || !isfinite(f1213)) return -1;
+  const double if1213 = 1.0 / f1213, iwv = 1.0 / wvno, iwv2 = iwv * iwv;
   double ur0;
   if constexpr (Q::W == 1) {
     ur0 = S(0, Q::CD + 2) / S(0, Q::CD + 1);
@@ -774,10 +798,10 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       hyper(g, ly.d, &h);
       double sP = 1.0, sS = 1.0, cpex;
       if (h.pex > h.svex) {
-        sS = (h.pex - h.svex > 40.0) ? 0.0 : exp(-(h.pex - h.svex));
+        sS = (h.pex - h.svex > 40.0) ? 0.0 : exp_neg(h.pex - h.svex);
         cpex = h.pex;
       } else {
-        sP = (h.svex - h.pex > 40.0) ? 0.0 : exp(-(h.svex - h.pex));
+        sP = (h.svex - h.pex > 40.0) ? 0.0 : exp_neg(h.svex - h.pex);
         cpex = h.svex;
       }
       double AA[4][4];
@@ -797,20 +821,22 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
       double uu4 = -tz1 * cd4 + tz2 * cd2 - tz3 * cd1;
       double ext = exd + exn_up - exe1;
       if (ext > -80.0 && ext < 80.0) {
-        double fact = exp(ext);
-        yb[0] = uu1 * fact / f1213;
-        yb[1] = uu2 * fact / f1213;
-        yb[2] = uu3 * fact / f1213;
-        yb[3] = uu4 * fact / f1213;
+        const double fact = exp_neg(-ext) * if1213;
+        yb[0] = uu1 * fact;
+        yb[1] = uu2 * fact;
+        yb[2] = uu3 * fact;
+        yb[3] = uu4 * fact;
       }
     }
     double I[6];
-    layer_integrals(g, ly.d, yt, yb, m == L - 1, I);
+    layer_integrals(g, ly.d, yt, yb, m == L - 1, h.eP, h.eS, I);
     const double INT11 = I[0], INT13 = I[1], INT22 = I[2], INT24 = I[3], INT33 = I[4],
                  INT44 = I[5];
     // energy (getmat coefficients)
     const double rho = ly.rho;
-    const double ah = sqrt(ly.A / rho), av = sqrt(ly.C / rho), bv = sqrt(ly.L / rho);
+    // sqrt(M / rho) = M / sqrt(M rho)
+    const double ah = ly.A * rsqrt_nr(ly.A * rho), av = ly.C * rsqrt_nr(ly.C * rho),
+                 bv = ly.L * rsqrt_nr(ly.L * rho);
     const double eta = ly.F / (ly.A - 2.0 * ly.L);
     const double a12 = -wvno, a14 = 1.0 / ly.L, a21 = wvno * ly.F / ly.C, a23 = 1.0 / ly.C;
     const double URUR = INT11, UZUZ = INT22;
@@ -822,10 +848,10 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     sumi1 = sumi1 + ly.L * UZUZ + ly.A * URUR;
     sumi2 = sumi2 + ly.L * UZDUR - ly.F * URDUZ;
     sumi3 = sumi3 + ly.L * DURDUR + ly.C * DUZDUZ;
-    const double facah = rho * ah * (URUR - 2.0 * eta * URDUZ / wvno);
-    const double facav = rho * av * DUZDUZ / wvno2;
-    const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR / wvno + DURDUR / wvno2 + 4.0 * eta * URDUZ / wvno);
-    const double facn = -ly.F * URDUZ / (wvno * eta);
+    const double facah = rho * ah * (URUR - 2.0 * eta * URDUZ * iwv);
+    const double facav = rho * av * DUZDUZ * iwv2;
+    const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR * iwv + DURDUR * iwv2 + 4.0 * eta * URDUZ * iwv);
+    const double facn = -ly.F * URDUZ * iwv / eta;
     S(m, Q::F + 0) = facah;
     S(m, Q::F + 1) = facbv;
     S(m, Q::F + 2) = facn;
@@ -842,10 +868,11 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   const double tm3 = tm * tm * tm;
   (void)om2;
   auto f32 = [](double x) -> float { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; };
+  const double inrm = 1.0 / nrm, isc = inrm / tm3;
   for (int m = 0; m < L; m++) {
     const double vt = (double)vtpf(m);
-    emit(m, f32(S(m, Q::F + 0) / nrm * vt / tm3), f32(S(m, Q::F + 1) / nrm * vt / tm3),
-         f32(S(m, Q::F + 2) / nrm));
+    emit(m, f32(S(m, Q::F + 0) * vt * isc), f32(S(m, Q::F + 1) * vt * isc),
+         f32(S(m, Q::F + 2) * inrm));
   }
   return 0;
 }
