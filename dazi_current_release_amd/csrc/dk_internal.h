@@ -58,10 +58,13 @@ struct KernDev {
 // One refined layer of the TI model: unflattened values used by the splice
 // (d, rvp, rvs, rrho, TA, TL, TF) and flattened ones used by the eigen-solution
 // (df, A = C, L = N, F, rho) plus the sphericity factor vtp; float32 as in the
-// reference.
+// reference.  k holds the layer's period-independent FP64 constants of the
+// eigen-solution and energy integrals, in ti::Layer's order: 1/C, 1/L,
+// eta = F/(A-2L), F/C, sqrt(A/rho), sqrt(C/rho), sqrt(L/rho), F/eta.
 struct TiLayerDev {
   float d, rvp, rvs, rrho, TA, TL, TF;
-  float df, A, L, F, rho, vtp;
+  float df, A, L, F, rho, vtp, pad;
+  double k[8];
 };
 
 // Doubles of TI scratch per layer and thread: real solves, complex-pair solves.

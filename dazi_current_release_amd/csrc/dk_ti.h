@@ -86,7 +86,22 @@ DK_HD cx csqrt_(cx z) {
 // ----------------------------------------------------------------- layer model
 struct Layer {  // flattened TI layer (float32 values of the reference, as double)
   double d, A, C, F, L, N, rho;
+  // period-independent constants: 1/C, 1/L, eta = F/(A-2L), F/C,
+  // sqrt(A/rho), sqrt(C/rho), sqrt(L/rho) and F/eta
+  double iC, iL, eta, FiC, ah, av, bv, Feta;
 };
+
+// The constants of a layer whose d, A, C, F, L, N, rho are set.
+DK_HD void layer_constants(Layer* m) {
+  m->iC = 1.0 / m->C;
+  m->iL = 1.0 / m->L;
+  m->eta = m->F / (m->A - 2.0 * m->L);
+  m->FiC = m->F / m->C;
+  m->ah = sqrt(m->A / m->rho);
+  m->av = sqrt(m->C / m->rho);
+  m->bv = sqrt(m->L / m->rho);
+  m->Feta = m->F / m->eta;
+}
 
 // Coefficients of gettiegn's characteristic equation L^2 - bb L + cc = 0 in
 // L = nu^2 (tregn96.f:3212-3244), with the eigenvector algebra's a..f.
@@ -96,7 +111,7 @@ struct Quad {
 
 DK_HD Quad quad(const Layer& m, double omega, double wvno) {
   const double om2 = omega * omega, wvno2 = wvno * wvno;
-  const double iC = 1.0 / m.C, iL = 1.0 / m.L;
+  const double iC = m.iC, iL = m.iL;
   Quad q;
   q.a = wvno * m.F * iC;
   q.b = iC;
@@ -165,10 +180,12 @@ struct Eig {
   int iP, iS;         // nu imaginary
   double x11, x21, x31, x41, x12, x22, x32, x42;
   double NP, NSV;
+  double rNP, rNSV;   // 1/NP, 1/NSV
   int ok;
 };
 
-DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
+// (wvno > 0, iwv2 = 1/wvno^2)
+DK_HD void eigen(const Layer& m, double omega, double wvno, double iwv2, Eig* o) {
   const double wvno2 = wvno * wvno;
   const Quad q = quad(m, omega, wvno);
   const double bb = q.bb, cc = q.cc;
@@ -191,18 +208,33 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, Eig* o) {
   o->rnS = rsqrt_nr(aS);
   o->nP = aP > 0.0 ? aP * o->rnP : 0.0;
   o->nS = aS > 0.0 ? aS * o->rnS : 0.0;
-  double x[8];
-  eigvec(q, wvno, L1, L2, x);
-  o->x11 = x[0];
-  o->x21 = x[1];
-  o->x31 = x[2];
-  o->x41 = x[3];
-  o->x12 = x[4];
-  o->x22 = x[5];
-  o->x32 = x[6];
-  o->x42 = x[7];
-  o->NP = x[0] * x[3] - x[1] * x[2];
-  o->NSV = x[4] * x[7] - x[5] * x[6];
+  // eigvec's columns before their normalisation z = k/x11 (k/x22), and
+  // N = z^2 Nr of the normalised ones: one division per wave gives both z
+  // and 1/N (R = 1/(x11 Nr): z = k Nr R, 1/N = x11^3 R / k^2)
+  const double bcaa = q.b * q.c + q.a * q.a, ddef2 = q.d * q.d + q.e * q.f;
+  const double x12 = q.b * q.d - q.a * q.e;
+  const double x22 = q.b * L2 - q.e * bcaa;
+  const double x32 = L2 - (q.a * q.d + q.c * q.e);
+  const double x42 = -(q.a * L2) + q.d * bcaa;
+  const double x11 = -(q.e * L1) + q.b * ddef2;
+  const double x21 = q.b * q.d - q.a * q.e;
+  const double x31 = q.d * L1 - q.a * ddef2;
+  const double x41 = -(L1 - q.a * q.d - q.b * q.f);
+  const double NPr = x11 * x41 - x21 * x31, NSr = x12 * x42 - x22 * x32;
+  const double R1 = 1.0 / (x11 * NPr), R2 = 1.0 / (x22 * NSr);
+  const double z1 = wvno * NPr * R1, z2 = wvno * NSr * R2;
+  o->x11 = x11 * z1;
+  o->x21 = x21 * z1;
+  o->x31 = x31 * z1;
+  o->x41 = x41 * z1;
+  o->x12 = x12 * z2;
+  o->x22 = x22 * z2;
+  o->x32 = x32 * z2;
+  o->x42 = x42 * z2;
+  o->NP = o->x11 * o->x41 - o->x21 * o->x31;
+  o->NSV = o->x12 * o->x42 - o->x22 * o->x32;
+  o->rNP = x11 * x11 * (x11 * R1) * iwv2;
+  o->rNSV = x22 * x22 * (x22 * R2) * iwv2;
 }
 
 // Complex vertical wavenumbers of the eigen-solution.
@@ -210,14 +242,16 @@ DK_HD cx nu_P(const Eig& g) { return g.iP ? cx{0.0, g.nP} : cx{g.nP, 0.0}; }
 DK_HD cx nu_S(const Eig& g) { return g.iS ? cx{0.0, g.nS} : cx{g.nS, 0.0}; }
 // 1/(2 nu N): nu real -> real, nu = i|nu| -> -i/(2|nu|N)
 DK_HD cx inv2nuN_P(const Eig& g) {
-  const double v = (0.5 / g.NP) * g.rnP;
+  const double v = (0.5 * g.rNP) * g.rnP;
   return g.iP ? cx{0.0, -v} : cx{v, 0.0};
 }
 DK_HD cx inv2nuN_S(const Eig& g) {
-  const double v = (0.5 / g.NSV) * g.rnS;
+  const double v = (0.5 * g.rNSV) * g.rnS;
   return g.iS ? cx{0.0, -v} : cx{v, 0.0};
 }
-DK_HD double half_inv(double N) { return 0.5 / N; }
+// 1/(2 nu)
+DK_HD cx inv2nu_P(const Eig& g) { return g.iP ? cx{0.0, -0.5 * g.rnP} : cx{0.5 * g.rnP, 0.0}; }
+DK_HD cx inv2nu_S(const Eig& g) { return g.iS ? cx{0.0, -0.5 * g.rnS} : cx{0.5 * g.rnS, 0.0}; }
 
 // General complex eigen-solution: the form tregn96 evaluates in complex*16
 // throughout.  Layers with real roots take exactly Eig's values (same
@@ -230,12 +264,13 @@ struct EigC {
   cx rp, rs;  // nu_P, nu_SV
   cx x11, x21, x31, x41, x12, x22, x32, x42;
   cx NP, NSV;
+  cx rNP, rNSV;  // 1/NP, 1/NSV
   int ok;     // always 1
 };
 
-DK_HD void eigen(const Layer& m, double omega, double wvno, EigC* o) {
+DK_HD void eigen(const Layer& m, double omega, double wvno, double iwv2, EigC* o) {
   Eig g;
-  eigen(m, omega, wvno, &g);
+  eigen(m, omega, wvno, iwv2, &g);
   o->ok = 1;
   if (g.ok) {
     o->rp = nu_P(g);
@@ -250,6 +285,8 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, EigC* o) {
     o->x42 = mk(g.x42);
     o->NP = mk(g.NP);
     o->NSV = mk(g.NSV);
+    o->rNP = mk(g.rNP);
+    o->rNSV = mk(g.rNSV);
     return;
   }
   const double wvno2 = wvno * wvno;
@@ -278,20 +315,23 @@ DK_HD void eigen(const Layer& m, double omega, double wvno, EigC* o) {
   o->x42 = x[7];
   o->NP = x[0] * x[3] - x[1] * x[2];
   o->NSV = x[4] * x[7] - x[5] * x[6];
+  o->rNP = cdiv_fast(mk(1.0), o->NP);
+  o->rNSV = cdiv_fast(mk(1.0), o->NSV);
 }
 
 DK_HD cx nu_P(const EigC& g) { return g.rp; }
 DK_HD cx nu_S(const EigC& g) { return g.rs; }
 DK_HD cx inv2nuN_P(const EigC& g) { return cdiv_fast(mk(0.5), g.rp * g.NP); }
 DK_HD cx inv2nuN_S(const EigC& g) { return cdiv_fast(mk(0.5), g.rs * g.NSV); }
-DK_HD cx half_inv(cx N) { return 0.5 / N; }
+DK_HD cx inv2nu_P(const EigC& g) { return cdiv_fast(mk(0.5), g.rp); }
+DK_HD cx inv2nu_S(const EigC& g) { return cdiv_fast(mk(0.5), g.rs); }
 
 // Eigenvector matrix E (columns P+, S+, P-, S-) and its inverse G (evalg).
 template <class G_>
 DK_HD void eig_matrices(const G_& g, cx E[4][4], cx G[4][4]) {
   const cx rp = nu_P(g), rs = nu_S(g);
   const cx iP = inv2nuN_P(g), iS = inv2nuN_S(g);
-  const auto hP = half_inv(g.NP), hS = half_inv(g.NSV);  // nu/(2 nu N)
+  const auto hP = 0.5 * g.rNP, hS = 0.5 * g.rNSV;  // nu/(2 nu N)
   G[0][0] = mk(g.x41 * hP);
   G[1][0] = g.x42 * iS;
   G[2][0] = mk(g.x41 * hP);
@@ -432,7 +472,7 @@ DK_HD void waves(const G_& g, const H_& h, double sP, double sS, Wave<T>* P, Wav
   S->z[1] = -g.x32;
   S->z[2] = -g.x22;
   S->z[3] = g.x12;
-  const T iNP = sP / g.NP, iNS = sS / g.NSV;
+  const T iNP = sP * g.rNP, iNS = sS * g.rNSV;
   P->m00 = h.cosp * iNP;
   P->m01 = h.sinpr * iNP;
   P->m10 = h.rsinp * iNP;
@@ -487,7 +527,7 @@ DK_HD void compound_step(const G_& g, const H_& h, T e[5]) {
     }
   // (u^v~) and (a~^b~) of each wave: u, v~ = w on even / odd rows, a~, b~ = z
   // on even / odd columns
-  const T iNP2 = dfac / (g.NP * g.NP), iNS2 = dfac / (g.NSV * g.NSV);
+  const T iNP2 = dfac * (g.rNP * g.rNP), iNS2 = dfac * (g.rNSV * g.rNSV);
   T uvP[6], uvS[6], abP[6], abS[6];
 #pragma unroll
   for (int r = 0; r < 6; r++) {
@@ -546,10 +586,11 @@ DK_HD void halfspace_constraint(const G_& g, T cd[5]) {
 // these (e^{-2 nub dm} = eb^2, e^{-(nua+nub) dm} = ea eb), with the
 // reference's cut-offs on Re of the exponent.
 DK_HD double cabs2_(cx a) { return a.r * a.r + a.i * a.i; }
-DK_HD cx ffunc(cx nub, double dm, cx eb) {
+// (i2nub = 1/(2 nub))
+DK_HD cx ffunc(cx nub, double dm, cx eb, cx i2nub) {
   if (cabs2_(nub) < 1.0e-16) return mk(dm);
   cx ex = (nub.r * dm < 40.0) ? eb * eb : mk(0.0);
-  return cdiv_fast(mk(1.0) - ex, 2.0 * nub);
+  return (mk(1.0) - ex) * i2nub;
 }
 DK_HD cx gfunc(cx nub, double dm, cx eb) {
   return (nub.r * dm < 75.0) ? eb * dm : mk(0.0);
@@ -582,7 +623,8 @@ DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const dou
   const cx ra = nu_P(g), rb = nu_S(g);
   // rows of G = E^-1: 1/(2 nu N) and nu/(2 nu N) = 1/(2N)
   const cx iP = inv2nuN_P(g), iS = inv2nuN_S(g);
-  const auto hP = half_inv(g.NP), hS = half_inv(g.NSV);
+  const auto hP = 0.5 * g.rNP, hS = 0.5 * g.rNSV;
+  const cx i2a = inv2nu_P(g), i2b = inv2nu_S(g);
   // amplitudes: up-going P, SV from yb; down-going P, SV from yt
   const cx k0 = mk(hP * (g.x41 * yb[0] - g.x21 * yb[2])) + iP * (g.x11 * yb[3] - g.x31 * yb[1]);
   const cx k1 = iS * (g.x42 * yb[0] - g.x22 * yb[2]) + mk(hS * (g.x12 * yb[3] - g.x32 * yb[1]));
@@ -593,11 +635,11 @@ DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const dou
   const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
   cx Ae, Be, Ce, Ao, Bo, Co;
   if (half) {
-    Ae = Ao = cdiv_fast(k2 * k2, 2.0 * ra);
-    Be = Bo = cdiv_fast(k3 * k3, 2.0 * rb);
+    Ae = Ao = (k2 * k2) * i2a;
+    Be = Bo = (k3 * k3) * i2b;
     Ce = Co = -cdiv_fast(k2 * k3, ra + rb);
   } else {
-    const cx FA = ffunc(ra, dm, eP), GA = gfunc(ra, dm, eP), FB = ffunc(rb, dm, eS),
+    const cx FA = ffunc(ra, dm, eP, i2a), GA = gfunc(ra, dm, eP), FB = ffunc(rb, dm, eS, i2b),
              GB = gfunc(rb, dm, eS);
     const cx H1 = h1func(ra, rb, dm, eP, eS), H2 = h2func(ra, rb, dm, eP, eS);
     const cx p00 = (k0 * k0 + k2 * k2) * FA, p02 = 2.0 * (GA * (k0 * k2));
@@ -680,6 +722,7 @@ DK_HD void flatten(int L, const float* thk, const float* TA, const float* TL, co
     out[i].L = lf;
     out[i].N = lf;
     out[i].rho = rf;
+    layer_constants(&out[i]);
     r0 = r1;
   }
   const double ar2 = 6370.0;
@@ -731,12 +774,13 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   // real part of compound entry q at interface m (svfunc reads real parts)
   auto cd_re = [&](int m, int q) -> double { return S(m, Q::CD + Q::W * q); };
   const double wvno = omega / c, wvno2 = wvno * wvno, om2 = omega * omega;
+  const double iwv = 1.0 / wvno, iwv2 = iwv * iwv;
   G_ g;
   typename G_::H h;
   int ok = 1;
   // ---- up: compound constraint vectors
   T cd[5];
-  eigen(lay(L - 1), omega, wvno, &g);
+  eigen(lay(L - 1), omega, wvno, iwv2, &g);
   ok &= g.ok;
   halfspace_constraint(g, cd);
   put_cd(L - 1, cd);
@@ -750,7 +794,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     // At m = L - 2 this rewrites the half-space entries with their own values.
     put_cd(m + 1, cd);
     S(m + 1, Q::EX) = exsum;
-    eigen(ly, omega, wvno, &g);
+    eigen(ly, omega, wvno, iwv2, &g);
     ok &= g.ok;
     hyper(g, ly.d, &h);
     compound_step(g, h, cd);
@@ -766,7 +810,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   const double exe1 = S(0, Q::EX);
   const double f1213 = -cd_re(0, 1);
   if (!(f1213 != 0.0) || !isfinite(f1213)) return -1;
-  const double if1213 = 1.0 / f1213, iwv = 1.0 / wvno, iwv2 = iwv * iwv;
+  const double if1213 = 1.0 / f1213;
   double ur0;
   if constexpr (Q::W == 1) {
     ur0 = S(0, Q::CD + 2) / S(0, Q::CD + 1);
@@ -792,7 +836,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     const int n = m + 1 < L ? m + 1 : L - 1;
     const double cd1 = cd_re(n, 0), cd2 = cd_re(n, 1), cd3 = cd_re(n, 2), cd5 = cd_re(n, 3),
                  cd6 = cd_re(n, 4), exn_up = S(n, Q::EX);
-    eigen(ly, omega, wvno, &g);
+    eigen(ly, omega, wvno, iwv2, &g);
     double yb[4] = {0.0, 0.0, 0.0, 0.0};  // interface m+1 (zero below the half-space top)
     if (m < L - 1) {
       hyper(g, ly.d, &h);
@@ -834,11 +878,9 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
                  INT44 = I[5];
     // energy (getmat coefficients)
     const double rho = ly.rho;
-    // sqrt(M / rho) = M / sqrt(M rho)
-    const double ah = ly.A * rsqrt_nr(ly.A * rho), av = ly.C * rsqrt_nr(ly.C * rho),
-                 bv = ly.L * rsqrt_nr(ly.L * rho);
-    const double eta = ly.F / (ly.A - 2.0 * ly.L);
-    const double a12 = -wvno, a14 = 1.0 / ly.L, a21 = wvno * ly.F / ly.C, a23 = 1.0 / ly.C;
+    const double ah = ly.ah, av = ly.av, bv = ly.bv;
+    const double eta = ly.eta;
+    const double a12 = -wvno, a14 = ly.iL, a21 = wvno * ly.FiC, a23 = ly.iC;
     const double URUR = INT11, UZUZ = INT22;
     const double DURDUR = a12 * a12 * INT22 + 2.0 * a12 * a14 * INT24 + a14 * a14 * INT44;
     const double DUZDUZ = a21 * a21 * INT11 + 2.0 * a21 * a23 * INT13 + a23 * a23 * INT33;
@@ -851,7 +893,7 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
     const double facah = rho * ah * (URUR - 2.0 * eta * URDUZ * iwv);
     const double facav = rho * av * DUZDUZ * iwv2;
     const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR * iwv + DURDUR * iwv2 + 4.0 * eta * URDUZ * iwv);
-    const double facn = -ly.F * URDUZ * iwv / eta;
+    const double facn = -ly.Feta * URDUZ * iwv;
     S(m, Q::F + 0) = facah;
     S(m, Q::F + 1) = facbv;
     S(m, Q::F + 2) = facn;

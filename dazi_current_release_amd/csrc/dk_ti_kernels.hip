@@ -35,6 +35,20 @@ __device__ float brocher_rho_ti(float p) {
   return 1.6612f * p - 0.4721f * p2 + 0.0671f * p3 - 0.0043f * p4 + 0.000106f * p5;
 }
 
+// The flattened layer of a record: TC = TA, TN = TL (depthkernelTI.f90:124-138),
+// without its constants.
+__device__ __forceinline__ ti::Layer layer_of(const TiLayerDev& t) {
+  ti::Layer r;
+  r.d = (double)t.df;
+  r.A = (double)t.A;
+  r.C = (double)t.A;
+  r.F = (double)t.F;
+  r.L = (double)t.L;
+  r.N = (double)t.L;
+  r.rho = (double)t.rho;
+  return r;
+}
+
 // One refined layer of the TI model: moduli (depthkernelTI.f90:124-138), TI
 // flattening with radius 6371 (sphere_tdisp96, tregn96.f:774-841) and the
 // sphericity factor with radius 6370 (bldsph, tregn96.f:1301-1383).  The
@@ -59,6 +73,17 @@ __device__ __forceinline__ TiLayerDev ti_layer(const TiGeom& g, float rvp, float
   t.F = t.TF;
   t.df = g.df;
   t.vtp = g.vtp;
+  t.pad = 0.0f;
+  ti::Layer r = layer_of(t);
+  ti::layer_constants(&r);
+  t.k[0] = r.iC;
+  t.k[1] = r.iL;
+  t.k[2] = r.eta;
+  t.k[3] = r.FiC;
+  t.k[4] = r.ah;
+  t.k[5] = r.av;
+  t.k[6] = r.bv;
+  t.k[7] = r.Feta;
   return t;
 }
 
@@ -146,14 +171,15 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   float acc = 0.0f;
   auto layf = [&](int m) {
     const TiLayerDev& t = lay[m];
-    ti::Layer r;
-    r.d = (double)t.df;
-    r.A = (double)t.A;
-    r.C = (double)t.A;
-    r.F = (double)t.F;
-    r.L = (double)t.L;
-    r.N = (double)t.L;
-    r.rho = (double)t.rho;
+    ti::Layer r = layer_of(t);
+    r.iC = t.k[0];
+    r.iL = t.k[1];
+    r.eta = t.k[2];
+    r.FiC = t.k[3];
+    r.ah = t.k[4];
+    r.av = t.k[5];
+    r.bv = t.k[6];
+    r.Feta = t.k[7];
     return r;
   };
   auto vtpf = [&](int m) { return lay[m].vtp; };
