@@ -488,16 +488,28 @@ DK_HD T wave_entry(const Wave<T>& w, int i, int j) {
   return w.w[i] * m * w.z[j];
 }
 
-// Real layer propagator (hska: the real part of the complex one), P part
-// scaled by sP and S part by sS.
+// out = H v for the real layer propagator H (hska: the real part of the
+// complex one), P part scaled by sP and S part by sS.  Per wave,
+// (H v)_i = w_i (M (z . v))_i%2 with the even and odd halves of z . v.
 template <class G_, class H_>
-DK_HD void propagator(const G_& g, const H_& h, double sP, double sS, double H[4][4]) {
-  Wave<typename G_::T> P, S;
+DK_HD void propagate(const G_& g, const H_& h, double sP, double sS, const double v[4],
+                     double out[4]) {
+  using T = typename G_::T;
+  Wave<T> P, S;
   waves(g, h, sP, sS, &P, &S);
+  auto wave = [&](const Wave<T>& w, T o[4]) {
+    const T te = w.z[0] * v[0] + w.z[2] * v[2], to = w.z[1] * v[1] + w.z[3] * v[3];
+    const T ev = w.m00 * te + w.m01 * to, od = w.m10 * te + w.m00 * to;
+    o[0] = w.w[0] * ev;
+    o[1] = w.w[1] * od;
+    o[2] = w.w[2] * ev;
+    o[3] = w.w[3] * od;
+  };
+  T oP[4], oS[4];
+  wave(P, oP);
+  wave(S, oS);
 #pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) H[i][j] = re(wave_entry(P, i, j) + wave_entry(S, i, j));
+  for (int i = 0; i < 4; i++) out[i] = re(oP[i] + oS[i]);
 }
 
 // Minor index pairs in the order (12),(13),(14),(23),(24),(34); the reduced
@@ -608,6 +620,47 @@ DK_HD cx h2func(cx nua, cx nub, double dm, cx ea, cx eb) {
   return cdiv_fast(exq - exp_, nua - nub);
 }
 
+// The six integrals out[t] = Re(a_i a_j A + b_i b_j B + (a_i b_j + b_i a_j) C)
+// of intijr's (i, j) = (0,0) (0,2) (1,1) (1,3) (2,2) (3,3), with (A, B, C) =
+// (Ae, Be, Ce) for even rows and (Ao, Bo, Co) for odd rows.
+DK_HD void integrals_out(const EigC& g, cx Ae, cx Be, cx Ce, cx Ao, cx Bo, cx Co,
+                         double out[6]) {
+  const cx ra = nu_P(g), rb = nu_S(g);
+  const cx a[4] = {mk(g.x11), g.x21 * ra, mk(g.x31), g.x41 * ra};
+  const cx b[4] = {g.x12 * rb, mk(g.x22), g.x32 * rb, mk(g.x42)};
+  const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
+#pragma unroll
+  for (int t = 0; t < 6; t++) {
+    const int i = I[t], j = J[t];
+    const bool ev = (i & 1) == 0;
+    const cx v = a[i] * a[j] * (ev ? Ae : Ao) + b[i] * b[j] * (ev ? Be : Bo) +
+                 (a[i] * b[j] + b[i] * a[j]) * (ev ? Ce : Co);
+    out[t] = v.r;
+  }
+}
+// Real eigenvector components: a = (x11, x21 nuP, x31, x41 nuP) and
+// b = (x12 nuS, x22, x32 nuS, x42), so each product above is real up to one
+// factor: b_i b_j carries nuS^2 and the cross term nuS on even rows, a_i a_j
+// nuP^2 and the cross term nuP on odd rows (nu^2 real).
+DK_HD void integrals_out(const Eig& g, cx Ae, cx Be, cx Ce, cx Ao, cx Bo, cx Co, double out[6]) {
+  const cx ra = nu_P(g), rb = nu_S(g);
+  const double nuP2 = (g.iP ? -g.nP : g.nP) * g.nP, nuS2 = (g.iS ? -g.nS : g.nS) * g.nS;
+  const double Ae_ = Ae.r, Be_ = nuS2 * Be.r, Ce_ = (rb * Ce).r;
+  const double Ao_ = nuP2 * Ao.r, Bo_ = Bo.r, Co_ = (ra * Co).r;
+  auto ev = [&](double ai, double aj, double bi, double bj) {
+    return ai * aj * Ae_ + bi * bj * Be_ + (ai * bj + bi * aj) * Ce_;
+  };
+  auto od = [&](double ai, double aj, double bi, double bj) {
+    return ai * aj * Ao_ + bi * bj * Bo_ + (ai * bj + bi * aj) * Co_;
+  };
+  out[0] = ev(g.x11, g.x11, g.x12, g.x12);
+  out[1] = ev(g.x11, g.x31, g.x12, g.x32);
+  out[2] = od(g.x21, g.x21, g.x22, g.x22);
+  out[3] = od(g.x21, g.x41, g.x22, g.x42);
+  out[4] = ev(g.x31, g.x31, g.x32, g.x32);
+  out[5] = od(g.x41, g.x41, g.x42, g.x42);
+}
+
 // Layer integrals INT11, INT13, INT22, INT24, INT33, INT44 of the field whose
 // up-going amplitudes come from the bottom interface value yb and down-going
 // ones from the top value yt (intijr); half = 1 for the half-space, else eP,
@@ -630,9 +683,6 @@ DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const dou
   const cx k1 = iS * (g.x42 * yb[0] - g.x22 * yb[2]) + mk(hS * (g.x12 * yb[3] - g.x32 * yb[1]));
   const cx k2 = mk(hP * (g.x41 * yt[0] - g.x21 * yt[2])) + iP * (g.x31 * yt[1] - g.x11 * yt[3]);
   const cx k3 = iS * (g.x22 * yt[2] - g.x42 * yt[0]) + mk(hS * (g.x12 * yt[3] - g.x32 * yt[1]));
-  const cx a[4] = {mk(g.x11), g.x21 * ra, mk(g.x31), g.x41 * ra};
-  const cx b[4] = {g.x12 * rb, mk(g.x22), g.x32 * rb, mk(g.x42)};
-  const int I[6] = {0, 0, 1, 1, 2, 3}, J[6] = {0, 2, 1, 3, 2, 3};
   cx Ae, Be, Ce, Ao, Bo, Co;
   if (half) {
     Ae = Ao = (k2 * k2) * i2a;
@@ -652,14 +702,7 @@ DK_HD void layer_integrals(const G_& g, double dm, const double yt[4], const dou
     Ce = p01 + p21;
     Co = p01 - p21;
   }
-#pragma unroll
-  for (int t = 0; t < 6; t++) {
-    const int i = I[t], j = J[t];
-    const bool ev = (i & 1) == 0;
-    const cx v = a[i] * a[j] * (ev ? Ae : Ao) + b[i] * b[j] * (ev ? Be : Bo) +
-                 (a[i] * b[j] + b[i] * a[j]) * (ev ? Ce : Co);
-    out[t] = v.r;
-  }
+  integrals_out(g, Ae, Be, Ce, Ao, Bo, Co, out);
 }
 
 // Exact power-of-two normalisations; they return the natural log of the
@@ -848,11 +891,8 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
         sP = (h.svex - h.pex > 40.0) ? 0.0 : exp_neg(h.svex - h.pex);
         cpex = h.svex;
       }
-      double AA[4][4];
-      propagator(g, h, sP, sS, AA);
       double a0[4];
-      for (int i = 0; i < 4; i++)
-        a0[i] = AA[i][0] * vv[0] + AA[i][1] * vv[1] + AA[i][2] * vv[2] + AA[i][3] * vv[3];
+      propagate(g, h, sP, sS, vv, a0);
       double ex2 = norm4(a0);
       exd = exd + cpex + ex2;
       for (int i = 0; i < 4; i++) vv[i] = a0[i];
