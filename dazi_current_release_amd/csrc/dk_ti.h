@@ -791,16 +791,28 @@ struct Slots {
   static constexpr int CD = 0, EX = 5 * W, F = 5 * W + 1, N = 5 * W + 4;
 };
 enum { Q_N = Slots<double>::N, Q_NC = Slots<cx>::N };
-// solve<Eig> runs in real arithmetic and returns -2, before any emit, when a
-// layer's eigenvalues form a complex-conjugate pair; solve<EigC> takes every
-// layer in complex arithmetic (the same values where the roots are real).
-// emit(m, dcdah, dcdbv, dcdn) receives, top layer first, tregn96's kernels after
-// the sphericity back-mapping, rounded to float as chksiz does.  lay(m) returns
-// the flattened Layer, vtpf(m) the float sphericity factor.  Returns 0, -1 when
-// the surface minor vanishes, or -2 as above.
-template <class G_, class LayerFn, class VtpFn, class EmitFn>
-DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* scratch,
-                long stride, EmitFn emit) {
+// Kernel factor q (facah, facbv, facn) of layer m in the scratch of a solve
+// in T arithmetic.
+template <class T>
+DK_HD double factor(const double* scratch, long stride, int m, int q) {
+  return scratch[(long)(m * Slots<T>::N + Slots<T>::F + q) * stride];
+}
+// chksiz's float rounding of a kernel value
+DK_HD float kernel_f32(double x) { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; }
+
+// solve<Eig> runs in real arithmetic and returns -2 when a layer's eigenvalues
+// form a complex-conjugate pair; solve<EigC> takes every layer in complex
+// arithmetic (the same values where the roots are real).  On return 0 the
+// scratch holds each layer's kernel factors (factor<T>), and tregn96's kernels
+// after the sphericity back-mapping are, top layer first,
+//   dcdah = f32(facah vtp isc), dcdbv = f32(facbv vtp isc), dcdn = f32(facn inrm)
+// with f32 = kernel_f32 and vtp the layer's float sphericity factor.
+// head(m) returns the flattened Layer with d, A, C, F, L, N, rho, iC and iL
+// set, tail(m, &layer) sets the rest of its constants.  Returns 0, -1 when the
+// surface minor vanishes, or -2 as above.
+template <class G_, class HeadFn, class TailFn>
+DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double* scratch,
+                long stride, double* isc_out, double* inrm_out) {
   using T = typename G_::T;
   using Q = Slots<T>;
   auto S = [&](int m, int q) -> double& { return scratch[(long)(m * Q::N + q) * stride]; };
@@ -823,15 +835,19 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   int ok = 1;
   // ---- up: compound constraint vectors
   T cd[5];
-  eigen(lay(L - 1), omega, wvno, iwv2, &g);
+  eigen(head(L - 1), omega, wvno, iwv2, &g);
   ok &= g.ok;
   halfspace_constraint(g, cd);
   put_cd(L - 1, cd);
   S(L - 1, Q::EX) = 0.0;
   double exsum = 0.0;
+  // each layer is loaded one iteration ahead, so that its latency overlaps the
+  // layer before it
+  Layer ahead = head(L >= 2 ? L - 2 : 0);
   for (int m = L - 2; m >= 0; m--) {
-    const Layer ly = lay(m);
-    // the previous layer's vector is stored after this layer's reads are
+    const Layer ly = ahead;
+    ahead = head(m >= 1 ? m - 1 : 0);
+    // the previous layer's vector is stored after the next layer's reads are
     // issued (a read issued after stores waits for them too); the vector and
     // its scale are this iteration's inputs, so they are in registers anyway.
     // At m = L - 2 this rewrites the half-space entries with their own values.
@@ -872,11 +888,15 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   double vv[4] = {1.0, 0.0, 0.0, 0.0};
   double exd = 0.0;
   double sumi0 = 0, sumi1 = 0, sumi2 = 0, sumi3 = 0;
+  ahead = head(0);
   for (int m = 0; m < L; m++) {
-    const Layer ly = lay(m);
-    // issue this iteration's reads of the up sweep (interface m+1) first, so
-    // that their latency overlaps the propagator work
+    Layer ly = ahead;
     const int n = m + 1 < L ? m + 1 : L - 1;
+    ahead = head(n);
+    // this layer's remaining constants (used last) and the up sweep's values at
+    // interface m+1 are read before the propagator work, which hides their
+    // latency
+    tail(m, &ly);
     const double cd1 = cd_re(n, 0), cd2 = cd_re(n, 1), cd3 = cd_re(n, 2), cd5 = cd_re(n, 3),
                  cd6 = cd_re(n, 4), exn_up = S(n, Q::EX);
     eigen(ly, omega, wvno, iwv2, &g);
@@ -949,13 +969,8 @@ DK_HD int solve(int L, LayerFn lay, VtpFn vtpf, double omega, double c, double* 
   const double tm = sqrt(1.0 + (cq / (2.0 * ar * omega)) * (cq / (2.0 * ar * omega)));
   const double tm3 = tm * tm * tm;
   (void)om2;
-  auto f32 = [](double x) -> float { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; };
-  const double inrm = 1.0 / nrm, isc = inrm / tm3;
-  for (int m = 0; m < L; m++) {
-    const double vt = (double)vtpf(m);
-    emit(m, f32(S(m, Q::F + 0) * vt * isc), f32(S(m, Q::F + 1) * vt * isc),
-         f32(S(m, Q::F + 2) * inrm));
-  }
+  *inrm_out = 1.0 / nrm;
+  *isc_out = *inrm_out / tm3;
   return 0;
 }
 

@@ -167,45 +167,80 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
     return 0;
   }
   const double omega = (double)twopi / (double)(float)periods[k];
-  int interval = 0, sub = 0;
-  float acc = 0.0f;
-  auto layf = [&](int m) {
+  auto head = [&](int m) {
     const TiLayerDev& t = lay[m];
     ti::Layer r = layer_of(t);
     r.iC = t.k[0];
     r.iL = t.k[1];
-    r.eta = t.k[2];
-    r.FiC = t.k[3];
-    r.ah = t.k[4];
-    r.av = t.k[5];
-    r.bv = t.k[6];
-    r.Feta = t.k[7];
     return r;
   };
-  auto vtpf = [&](int m) { return lay[m].vtp; };
-  auto emit = [&](int m, float dah, float dbv, float dn) {
+  auto tail = [&](int m, ti::Layer* r) {
     const TiLayerDev& t = lay[m];
-    if constexpr (!KL) {
-      if (m >= L - 1 || interval >= nz - 1) return;  // the half-space feeds no interval
-    }
+    r->eta = t.k[2];
+    r->FiC = t.k[3];
+    r->ah = t.k[4];
+    r->av = t.k[5];
+    r->bv = t.k[6];
+    r->Feta = t.k[7];
+  };
+  double isc, inrm;
+  const int rc = ti::solve<G_>(L, head, tail, omega, c, scratch, stride, &isc, &inrm);
+  if (rc == -1) fill_all(0.0f);
+  if (rc != 0) return rc;
+  // tregn96's kernels top layer first, chained to dc/dA, dc/dL and spliced into
+  // the intervals; the factors and record of layer m+1 are loaded while layer
+  // m is chained (one memory latency per layer instead of one per load)
+  using T = typename G_::T;
+  struct Rec {
+    double f0, f1, f2;
+    float vtp, rrho, rvp, rvs, TA, TL, TF;
+  };
+  auto load = [&](int m) {
+    Rec r;
+    r.f0 = ti::factor<T>(scratch, stride, m, 0);
+    r.f1 = ti::factor<T>(scratch, stride, m, 1);
+    r.f2 = ti::factor<T>(scratch, stride, m, 2);
+    const TiLayerDev& t = lay[m];
+    r.vtp = t.vtp;
+    r.rrho = t.rrho;
+    r.rvp = t.rvp;
+    r.rvs = t.rvs;
+    r.TA = t.TA;
+    r.TL = t.TL;
+    r.TF = t.TF;
+    return r;
+  };
+  // the half-space feeds no interval; without KL the loop stops at the last
+  // interval
+  const int mend = KL ? L : (nz > 1 ? L - 1 : 0);
+  int interval = 0, sub = 0;
+  int ns = nz > 1 ? nsub[0] : 0;
+  float acc = 0.0f;
+  Rec nx = load(0);
+  for (int m = 0; m < mend; m++) {
+    const Rec r = nx;
+    nx = load(m + 1 < L ? m + 1 : L - 1);
+    const double vt = (double)r.vtp;
+    const float dah = ti::kernel_f32(r.f0 * vt * isc), dbv = ti::kernel_f32(r.f1 * vt * isc),
+                dn = ti::kernel_f32(r.f2 * inrm);
     float dA, dL;
-    ti::chain_AL(t.rrho, t.rvp, t.rvs, t.TA, t.TL, t.TF, dah, dbv, dn, &dA, &dL);
+    ti::chain_AL(r.rrho, r.rvp, r.rvs, r.TA, r.TL, r.TF, dah, dbv, dn, &dA, &dL);
     if constexpr (KL) {
       lay_out(dcdA, m) = dA;
       lay_out(dcdL, m) = dL;
-      if (!lsen || m >= L - 1 || interval >= nz - 1) return;
+      if (!lsen) continue;
     }
-    acc = ti::splice_add(acc, t.TA, t.TL, dA, dL);
-    if (++sub == nsub[interval]) {
+    if (m >= L - 1 || interval >= nz - 1) continue;
+    acc = ti::splice_add(acc, r.TA, r.TL, dA, dL);
+    if (++sub == ns) {
       out(interval) = acc;
       acc = 0.0f;
       sub = 0;
       interval++;
+      ns = nsub[interval < nz - 1 ? interval : nz - 2];
     }
-  };
-  const int rc = ti::solve<G_>(L, layf, vtpf, omega, c, scratch, stride, emit);
-  if (rc == -1) fill_all(0.0f);
-  return rc;
+  }
+  return 0;
 }
 
 // KL: also write the per-layer kernels dcdA, dcdL (a variant of its own, so
