@@ -1136,9 +1136,9 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       // contribution sum_ji e_j dCA_ji g_i and the propagation g <- CA g,
       // reduced entry by entry as the layer matrix is formed
       double acc[3] = {0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      layer_ca_visit<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
-                              dvar<2>((double)l.z, 1), dconst<2>((double)l.w), (double)l.x,
-                              omega, [&](int j, int i, const Dual<2>& v) {
+      // partials in (a, b) only where an entry depends on them (SD masks)
+      layer_ca_visit<double>(wv, wv2, sd_var<1>((double)l.y), sd_var<2>((double)l.z),
+                             (double)l.w, (double)l.x, omega, [&](int j, int i, const auto& v) {
                                 const double w = et[j] * g[i];
                                 acc[0] = fma(w, v.g[0], acc[0]);
                                 acc[1] = fma(w, v.g[1], acc[1]);
@@ -1387,12 +1387,12 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
           if (m < L - 1) {
             const double* et = tc;
             const int Ee = (int)tc[5];
-            // Dual<3> in (a, b, d); the rho partial from the entries' rho powers
+            // the rho partial from the entries' rho powers
             double acc[4] = {0.0, 0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            layer_ca_visit<Dual<3>, Dual<3>>(
-                dconst<3>(wv), dconst<3>(wv2), dvar<3>((double)l.y, 0), dvar<3>((double)l.z, 1),
-                dconst<3>((double)l.w), dvar<3>((double)l.x, 2), omega,
-                [&](int j, int i, const Dual<3>& v) {
+            // partials in (a, b, d) only where an entry depends on them (SD masks)
+            layer_ca_visit<double>(
+                wv, wv2, sd_var<1>((double)l.y), sd_var<2>((double)l.z), (double)l.w,
+                sd_var<4>((double)l.x), omega, [&](int j, int i, const auto& v) {
                   const double w = et[j] * g[i];
                   acc[0] = fma(w, v.g[0], acc[0]);
                   acc[1] = fma(w, v.g[1], acc[1]);

@@ -185,6 +185,125 @@ DK_HD T tconst(double v) {
   return Num<T>::make(v);
 }
 
+// Dual number in up to three directions (bit 0: alpha, bit 1: beta, bit 2:
+// thickness) whose partials outside the mask M are zero by construction, so
+// that operations skip them: a layer matrix mixes P-wave quantities (alpha
+// only) with S-wave ones (beta only), and the compiler cannot drop a product
+// with a zero partial on its own (0 * x is not 0 for every x).  Only the
+// partials in M are stored values; the others are never read.
+template <int M>
+struct SD {
+  double v;
+  double g[3];
+};
+template <int M>
+DK_HD constexpr bool sd_has(int k) {
+  return ((M >> k) & 1) != 0;
+}
+template <int M>
+DK_HD SD<M> sd_var(double v) {  // v as the variable of direction M (one bit)
+  SD<M> r;
+  r.v = v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.g[k] = sd_has<M>(k) ? 1.0 : 0.0;
+  return r;
+}
+template <int A, int B>
+DK_HD SD<A | B> operator+(const SD<A>& a, const SD<B>& b) {
+  SD<A | B> r;
+  r.v = a.v + b.v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (sd_has<A>(k) && sd_has<B>(k)) r.g[k] = a.g[k] + b.g[k];
+    else if (sd_has<A>(k)) r.g[k] = a.g[k];
+    else if (sd_has<B>(k)) r.g[k] = b.g[k];
+    else r.g[k] = 0.0;
+  }
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator-(const SD<A>& a) {
+  SD<A> r;
+  r.v = -a.v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.g[k] = sd_has<A>(k) ? -a.g[k] : 0.0;
+  return r;
+}
+template <int A, int B>
+DK_HD SD<A | B> operator-(const SD<A>& a, const SD<B>& b) {
+  SD<A | B> r;
+  r.v = a.v - b.v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (sd_has<A>(k) && sd_has<B>(k)) r.g[k] = a.g[k] - b.g[k];
+    else if (sd_has<A>(k)) r.g[k] = a.g[k];
+    else if (sd_has<B>(k)) r.g[k] = -b.g[k];
+    else r.g[k] = 0.0;
+  }
+  return r;
+}
+template <int A, int B>
+DK_HD SD<A | B> operator*(const SD<A>& a, const SD<B>& b) {
+  SD<A | B> r;
+  r.v = a.v * b.v;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (sd_has<A>(k) && sd_has<B>(k)) r.g[k] = a.g[k] * b.v + a.v * b.g[k];
+    else if (sd_has<A>(k)) r.g[k] = a.g[k] * b.v;
+    else if (sd_has<B>(k)) r.g[k] = a.v * b.g[k];
+    else r.g[k] = 0.0;
+  }
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator+(const SD<A>& a, double b) {
+  SD<A> r = a;
+  r.v += b;
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator+(double b, const SD<A>& a) {
+  return a + b;
+}
+template <int A>
+DK_HD SD<A> operator-(const SD<A>& a, double b) {
+  SD<A> r = a;
+  r.v -= b;
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator-(double b, const SD<A>& a) {
+  SD<A> r = -a;
+  r.v += b;
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator*(const SD<A>& a, double b) {
+  SD<A> r;
+  r.v = a.v * b;
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.g[k] = sd_has<A>(k) ? a.g[k] * b : 0.0;
+  return r;
+}
+template <int A>
+DK_HD SD<A> operator*(double b, const SD<A>& a) {
+  return a * b;
+}
+template <int A>
+DK_HD SD<A> operator/(double a, const SD<A>& b) {
+  SD<A> r;
+  double ib = 1.0 / b.v;
+  r.v = a * ib;
+  double f = -r.v * ib;
+#pragma unroll
+  for (int k = 0; k < 3; k++) r.g[k] = sd_has<A>(k) ? b.g[k] * f : 0.0;
+  return r;
+}
+template <int A>
+DK_HD double val(const SD<A>& a) {
+  return a.v;
+}
+
 // ----------------------------------------------------------------- layer matrix
 // Vertical wavenumber sqrt((k+kx)|k-kx|) written as sqrt(|k^2-kx^2|): the two
 // forms agree in exact arithmetic and the latter keeps the derivative simple.
@@ -251,6 +370,68 @@ DK_HD double hyp3_t(const double& X, double d, double* S, double* R, double* ex,
   *ex = hyp3(X, d, &C, S, R, &dC, &dS, &dR, scale);
   return C;
 }
+// C, S, R of hyp3 as one value, for the generic layer matrix.
+template <class T>
+struct Hyp3 {
+  T C, S, R;
+};
+DK_HD Hyp3<double> hyp3_v(double X, double d, double* ex, double* scale) {
+  Hyp3<double> h;
+  h.C = hyp3_t(X, d, &h.S, &h.R, ex, scale);
+  return h;
+}
+template <int N, class D>
+DK_HD Hyp3<Dual<N>> hyp3_v(const Dual<N>& X, const D& d, double* ex, double* scale) {
+  Hyp3<Dual<N>> h;
+  h.C = hyp3_t(X, d, &h.S, &h.R, ex, scale);
+  return h;
+}
+template <int M>
+DK_HD Hyp3<SD<M>> hyp3_v(const SD<M>& X, double d, double* ex, double* scale) {
+  double C, Sv, Rv, dC, dS, dR;
+  *ex = hyp3(X.v, d, &C, &Sv, &Rv, &dC, &dS, &dR, scale);
+  Hyp3<SD<M>> h;
+  h.C.v = C;
+  h.S.v = Sv;
+  h.R.v = Rv;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    h.C.g[k] = sd_has<M>(k) ? X.g[k] * dC : 0.0;
+    h.S.g[k] = sd_has<M>(k) ? X.g[k] * dS : 0.0;
+    h.R.g[k] = sd_has<M>(k) ? X.g[k] * dR : 0.0;
+  }
+  return h;
+}
+template <int M, int MD>
+DK_HD Hyp3<SD<M | MD>> hyp3_v(const SD<M>& X, const SD<MD>& d, double* ex, double* scale) {
+  double C, Sv, Rv, dC, dS, dR;
+  *ex = hyp3(X.v, d.v, &C, &Sv, &Rv, &dC, &dS, &dR, scale);
+  Hyp3<SD<M | MD>> h;
+  h.C.v = C;
+  h.S.v = Sv;
+  h.R.v = Rv;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double xg = sd_has<M>(k) ? X.g[k] : 0.0, dg = sd_has<MD>(k) ? d.g[k] : 0.0;
+    if (sd_has<M>(k) && sd_has<MD>(k)) {
+      h.C.g[k] = xg * dC + dg * Rv;
+      h.S.g[k] = xg * dS + dg * C;
+      h.R.g[k] = xg * dR + dg * X.v * C;
+    } else if (sd_has<M>(k)) {
+      h.C.g[k] = xg * dC;
+      h.S.g[k] = xg * dS;
+      h.R.g[k] = xg * dR;
+    } else if (sd_has<MD>(k)) {
+      h.C.g[k] = dg * Rv;
+      h.S.g[k] = dg * C;
+      h.R.g[k] = dg * X.v * C;
+    } else {
+      h.C.g[k] = h.S.g[k] = h.R.g[k] = 0.0;
+    }
+  }
+  return h;
+}
+
 // X and d both carry derivatives: with the exponent scaling held constant,
 // dC/dd = R, dS/dd = C, dR/dd = X C.
 template <int N>
@@ -293,90 +474,91 @@ DK_HD Dual<N> hyp3_t(const Dual<N>& X, double d, Dual<N>* S, Dual<N>* R, double*
 // d its flattened thickness (a double, or a Dual<N> to differentiate in d).
 // visit(j, i, value) receives every entry ca[j][i] once as it is formed (so a
 // consumer can reduce the matrix without storing it); layer_ca stores them.
-template <class T, class D = double, class V>
-DK_HD void layer_ca_visit(const T& wvno, const T& wvno2, const T& a, const T& b, const T& rho,
-                          const D& d, double omega, V&& visit) {
-  T xka = omega / a;
-  T xkb = omega / b;
-  T t = b * (1.0 / omega);
-  T gammk = 2.0 * t * t;
-  T gam = gammk * wvno2;
+template <class T, class D = double, class V, class TA = T, class TB = T, class TR = T>
+DK_HD void layer_ca_visit(const T& wvno, const T& wvno2, const TA& a, const TB& b,
+                          const TR& rho, const D& d, double omega, V&& visit) {
+  const auto xka = omega / a;
+  const auto xkb = omega / b;
+  const auto t = b * (1.0 / omega);
+  const auto gammk = 2.0 * t * t;
+  const auto gam = gammk * wvno2;
   // ---- var (surfdisp96.f:868-985): cosp=C_P, w=S_P, x=R_P; cosq=C_S, y=S_S, z=R_S
   double pex, sex, sP, sS;
-  T w, x, y, z;
-  T cosp = hyp3_t((wvno + xka) * (wvno - xka), d, &w, &x, &pex, &sP);
-  T cosq = hyp3_t((wvno + xkb) * (wvno - xkb), d, &y, &z, &sex, &sS);
+  const auto hP = hyp3_v((wvno + xka) * (wvno - xka), d, &pex, &sP);
+  const auto hS = hyp3_v((wvno + xkb) * (wvno - xkb), d, &sex, &sS);
+  const auto &cosp = hP.C, &w = hP.S, &x = hP.R;
+  const auto &cosq = hS.C, &y = hS.S, &z = hS.R;
   // exp(-(pex + sex)); the reference's var drops it for pex + sex >= 60 (< 1e-26)
-  T a0 = tconst<T>(sP * sS);
-  T cpcq = cosp * cosq;
-  T cpy = cosp * y;
-  T cpz = cosp * z;
-  T cqw = cosq * w;
-  T cqx = cosq * x;
-  T xy = x * y;
-  T xz = x * z;
-  T wy = w * y;
-  T wz = w * z;
+  const double a0 = sP * sS;
+  const auto cpcq = cosp * cosq;
+  const auto cpy = cosp * y;
+  const auto cpz = cosp * z;
+  const auto cqw = cosq * w;
+  const auto cqx = cosq * x;
+  const auto xy = x * y;
+  const auto xz = x * z;
+  const auto wy = w * y;
+  const auto wz = w * z;
   // ---- dnka (surfdisp96.f:1018-1062)
-  T gamm1 = gam - 1.0;
-  T twgm1 = gam + gamm1;
-  T gmgmk = gam * gammk;
-  T gmgm1 = gam * gamm1;
-  T gm1sq = gamm1 * gamm1;
-  T irho = 1.0 / rho;
-  T rho2 = rho * rho;
-  T irho2 = irho * irho;
-  T a0pq = a0 - cpcq;
-  T tt = -2.0 * wvno2;
+  const auto gamm1 = gam - 1.0;
+  const auto twgm1 = gam + gamm1;
+  const auto gmgmk = gam * gammk;
+  const auto gmgm1 = gam * gamm1;
+  const auto gm1sq = gamm1 * gamm1;
+  const auto irho = 1.0 / rho;
+  const auto rho2 = rho * rho;
+  const auto irho2 = irho * irho;
+  const auto a0pq = a0 - cpcq;
+  const auto tt = -2.0 * wvno2;
   {
-    const T c00 = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
+    const auto c00 = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
     visit(0, 0, c00);
     visit(4, 4, c00);
     visit(2, 2, a0 + 2.0 * (cpcq - c00));
   }
   {
-    const T c01 = (wvno2 * cpy - cqx) * irho;
+    const auto c01 = (wvno2 * cpy - cqx) * irho;
     visit(0, 1, c01);
     visit(3, 4, c01);
   }
   {
-    const T c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
+    const auto c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
     visit(0, 2, c02);
     visit(2, 4, tt * c02);
   }
   {
-    const T c03 = (cpz - wvno2 * cqw) * irho;
+    const auto c03 = (cpz - wvno2 * cqw) * irho;
     visit(0, 3, c03);
     visit(1, 4, c03);
   }
   visit(0, 4, -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2);
   {
-    const T c10 = (gmgmk * cpz - gm1sq * cqw) * rho;
+    const auto c10 = (gmgmk * cpz - gm1sq * cqw) * rho;
     visit(1, 0, c10);
     visit(4, 3, c10);
   }
   visit(1, 1, cpcq);
   visit(3, 3, cpcq);
   {
-    const T c12 = gammk * cpz - gamm1 * cqw;
+    const auto c12 = gammk * cpz - gamm1 * cqw;
     visit(1, 2, c12);
     visit(2, 3, tt * c12);
   }
   visit(1, 3, -wz);
   {
-    const T c30 = (gm1sq * cpy - gmgmk * cqx) * rho;
+    const auto c30 = (gm1sq * cpy - gmgmk * cqx) * rho;
     visit(3, 0, c30);
     visit(4, 1, c30);
   }
   visit(3, 1, -xy);
   {
-    const T c32 = gamm1 * cpy - gammk * cqx;
+    const auto c32 = gamm1 * cpy - gammk * cqx;
     visit(3, 2, c32);
     visit(2, 1, tt * c32);
   }
   visit(4, 0, -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2);
   {
-    const T c42 =
+    const auto c42 =
         -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
     visit(4, 2, c42);
     visit(2, 0, tt * c42);
