@@ -1044,6 +1044,13 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   Dual<1> wvd = dconst<1>(wv);
   wvd.g[0] = -wv / c;
   const Dual<1> wv2d = wvd * wvd;
+  // the same for the layers, where only k carries a partial (a, b, rho are
+  // constants in c)
+  SD<1> wvs;
+  wvs.v = wv;
+  wvs.g[0] = -wv / c;
+  wvs.g[1] = wvs.g[2] = 0.0;
+  const SD<1> wv2s = wvs * wvs;
   Dual<1> e[5];
   {
     double4 h = lay[L - 1];
@@ -1065,12 +1072,11 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i].v;
       tp[(int64_t)5 * nthr] = (double)Ef;
       double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      layer_ca_visit<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
-                              dconst<1>((double)l.w), (double)l.x, omega,
-                              [&](int j, int i, const Dual<1>& v) {
-                                ev[i] = fma(e[j].v, v.v, ev[i]);
-                                eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
-                              });
+      layer_ca_visit<SD<1>>(wvs, wv2s, (double)l.y, (double)l.z, (double)l.w, (double)l.x,
+                            omega, [&](int j, int i, const auto& v) {
+                              ev[i] = fma(e[j].v, v.v, ev[i]);
+                              eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
+                            });
 #pragma unroll
       for (int i = 0; i < 5; i++) {
         e[i].v = ev[i];
@@ -1282,6 +1288,11 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
         Dual<1> wvd = dconst<1>(wv);
         wvd.g[0] = -wv / c;
         Dual<1> wv2d = wvd * wvd;
+        SD<1> wvs;  // the layers' k, the only one of their inputs with a partial in c
+        wvs.v = wv;
+        wvs.g[0] = -wv / c;
+        wvs.g[1] = wvs.g[2] = 0.0;
+        const SD<1> wv2s = wvs * wvs;
         Dual<1> e[5];
         double4 h = lay[L - 1];
         halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
@@ -1294,23 +1305,17 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
           if (m < L - 1) {
             const double4 l = lnext;
             lnext = lay[m >= 1 ? m - 1 : 0];
-            Dual<1> ca[5][5];
-            layer_ca<Dual<1>>(wvd, wv2d, dconst<1>((double)l.y), dconst<1>((double)l.z),
-                              dconst<1>((double)l.w), (double)l.x, omega, ca);
-            Dual<1> ee[5];
+            double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            layer_ca_visit<SD<1>>(wvs, wv2s, (double)l.y, (double)l.z, (double)l.w,
+                                  (double)l.x, omega, [&](int j, int i, const auto& v) {
+                                    ev[i] = fma(e[j].v, v.v, ev[i]);
+                                    eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
+                                  });
 #pragma unroll
             for (int i = 0; i < 5; i++) {
-              double v = 0.0, g = 0.0;
-#pragma unroll
-              for (int j = 0; j < 5; j++) {
-                v = fma(e[j].v, ca[j][i].v, v);
-                g = fma(e[j].g[0], ca[j][i].v, fma(e[j].v, ca[j][i].g[0], g));
-              }
-              ee[i].v = v;
-              ee[i].g[0] = g;
+              e[i].v = ev[i];
+              e[i].g[0] = eg[i];
             }
-#pragma unroll
-            for (int i = 0; i < 5; i++) e[i] = ee[i];
           }
           // normalise values and tangent together (exact power of two)
           double mx = 0.0;

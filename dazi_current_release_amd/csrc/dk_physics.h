@@ -185,12 +185,13 @@ DK_HD T tconst(double v) {
   return Num<T>::make(v);
 }
 
-// Dual number in up to three directions (bit 0: alpha, bit 1: beta, bit 2:
-// thickness) whose partials outside the mask M are zero by construction, so
-// that operations skip them: a layer matrix mixes P-wave quantities (alpha
-// only) with S-wave ones (beta only), and the compiler cannot drop a product
-// with a zero partial on its own (0 * x is not 0 for every x).  Only the
-// partials in M are stored values; the others are never read.
+// Dual number in up to three directions whose partials outside the mask M are
+// zero by construction, so that operations skip them: a layer matrix mixes
+// P-wave quantities (alpha only) with S-wave ones (beta only), and in the
+// tangent in c only k carries a partial; the compiler cannot drop a product
+// with a zero partial on its own (0 * x is not 0 for every x).  The callers
+// choose the directions: (alpha, beta) in the kernel pass, (alpha, beta,
+// thickness) in the VJP, c alone in the tangent sweeps.
 template <int M>
 struct SD {
   double v;
