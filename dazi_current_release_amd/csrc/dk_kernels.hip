@@ -525,11 +525,21 @@ __device__ __forceinline__ double secular_fwd(double c, double omega, int L, con
   int E = pow2_normalize(e);
   // two layers per step: the second layer's matrix is independent of the
   // first layer's update, and one normalisation per pair keeps the vector far
-  // inside the double range (a layer scales it by at most ~1e4)
+  // inside the double range (a layer scales it by at most ~1e4).  The next
+  // pair's records are read from LDS one step ahead (clamped indices), so
+  // that the read latency overlaps this pair's work.
   int m = L - 2;
+  const int m0 = m >= 1 ? m : 1;
+  double4 ga = mdl[2 * m0], ra = mdl[2 * m0 + 1], gb = mdl[2 * m0 - 2], rb = mdl[2 * m0 - 1];
   for (; m >= 1; m -= 2) {
-    layer_step_fwd(wvno2, om2, iom2, mdl[2 * m], mdl[2 * m + 1], e);
-    layer_step_fwd(wvno2, om2, iom2, mdl[2 * m - 2], mdl[2 * m - 1], e);
+    const double4 g0 = ga, r0 = ra, g1 = gb, r1 = rb;
+    const int mn = m >= 3 ? m - 2 : 1;
+    ga = mdl[2 * mn];
+    ra = mdl[2 * mn + 1];
+    gb = mdl[2 * mn - 2];
+    rb = mdl[2 * mn - 1];
+    layer_step_fwd(wvno2, om2, iom2, g0, r0, e);
+    layer_step_fwd(wvno2, om2, iom2, g1, r1, e);
     E += pow2_normalize(e);
   }
   if (m == 0) {
