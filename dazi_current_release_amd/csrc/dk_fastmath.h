@@ -16,11 +16,23 @@ namespace dk {
 
 // a*b + c as one three-operand v_fma_f64 (the backend's two-operand
 // v_fmac_f64 form would first copy a loop-invariant addend into the result
-// register, one extra move per Horner step)
+// register, one extra move per Horner step).  fma3 takes the constant addend
+// c from a VGPR pair, fma3s from an SGPR pair (the one scalar operand a VALU
+// instruction may read): SGPR constants free vector registers, at the cost of
+// scalar moves where the scalar registers run out.
 DK_HD double fma3(double a, double b, double c) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double d;
   asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+  return d;
+#else
+  return fma(a, b, c);
+#endif
+}
+DK_HD double fma3s(double a, double b, double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  double d;
+  asm("v_fma_f64 %0, %1, %2, %3" : "=v"(d) : "v"(a), "v"(b), "s"(c));
   return d;
 #else
   return fma(a, b, c);
@@ -37,14 +49,14 @@ DK_HD double exp_neg(double p) {
   double r = fma(n, 6.93147180369123816490e-01, -p);
   r = fma(n, 1.90821492927058770002e-10, r);
   double q = 2.5052108385441718775e-08;              // 1/11!
-  q = fma3(q, r, 2.7557319223985890653e-07);          // 1/10!
-  q = fma3(q, r, 2.7557319223985892511e-06);          // 1/9!
-  q = fma3(q, r, 2.4801587301587301566e-05);          // 1/8!
-  q = fma3(q, r, 1.9841269841269841253e-04);          // 1/7!
-  q = fma3(q, r, 1.3888888888888889419e-03);          // 1/6!
-  q = fma3(q, r, 8.3333333333333332177e-03);          // 1/5!
-  q = fma3(q, r, 4.1666666666666664354e-02);          // 1/4!
-  q = fma3(q, r, 1.6666666666666665741e-01);          // 1/3!
+  q = fma3s(q, r, 2.7557319223985890653e-07);          // 1/10!
+  q = fma3s(q, r, 2.7557319223985892511e-06);          // 1/9!
+  q = fma3s(q, r, 2.4801587301587301566e-05);          // 1/8!
+  q = fma3s(q, r, 1.9841269841269841253e-04);          // 1/7!
+  q = fma3s(q, r, 1.3888888888888889419e-03);          // 1/6!
+  q = fma3s(q, r, 8.3333333333333332177e-03);          // 1/5!
+  q = fma3s(q, r, 4.1666666666666664354e-02);          // 1/4!
+  q = fma3s(q, r, 1.6666666666666665741e-01);          // 1/3!
   q = fma(q, r, 0.5);
   q = fma(q, r, 1.0);
   q = fma(q, r, 1.0);
