@@ -524,24 +524,29 @@ __device__ __forceinline__ double secular_fwd(double c, double omega, int L, con
   halfspace_fwd(wvno2, om2, iom2, mdl[2 * (L - 1)], mdl[2 * (L - 1) + 1], e);
   int E = pow2_normalize(e);
   // two layers per step: the second layer's matrix is independent of the
-  // first layer's update, and one normalisation per pair keeps the vector far
-  // inside the double range (a layer scales it by at most ~1e4).  The next
-  // pair's records are read from LDS one step ahead (clamped indices), so
-  // that the read latency overlaps this pair's work.
+  // first layer's update.  One normalisation per two steps keeps the vector
+  // far inside the double range (a layer scales it by at most ~1e4).  The
+  // next pair's records are read from LDS one step ahead (clamped indices),
+  // so that the read latency overlaps this pair's work.
   int m = L - 2;
-  const int m0 = m >= 1 ? m : 1;
-  double4 ga = mdl[2 * m0], ra = mdl[2 * m0 + 1], gb = mdl[2 * m0 - 2], rb = mdl[2 * m0 - 1];
+  auto cl = [&](int i) { return min(max(i, 0), L - 1); };  // reads past the end stay in range
+  int ia = cl(m), ib = cl(m - 1);
+  double4 ga = mdl[2 * ia], ra = mdl[2 * ia + 1], gb = mdl[2 * ib], rb = mdl[2 * ib + 1];
+  bool norm = false;
   for (; m >= 1; m -= 2) {
     const double4 g0 = ga, r0 = ra, g1 = gb, r1 = rb;
-    const int mn = m >= 3 ? m - 2 : 1;
-    ga = mdl[2 * mn];
-    ra = mdl[2 * mn + 1];
-    gb = mdl[2 * mn - 2];
-    rb = mdl[2 * mn - 1];
+    ia = cl(m - 2);
+    ib = cl(m - 3);
+    ga = mdl[2 * ia];
+    ra = mdl[2 * ia + 1];
+    gb = mdl[2 * ib];
+    rb = mdl[2 * ib + 1];
     layer_step_fwd(wvno2, om2, iom2, g0, r0, e);
     layer_step_fwd(wvno2, om2, iom2, g1, r1, e);
-    E += pow2_normalize(e);
+    if (norm) E += pow2_normalize(e);
+    norm = !norm;
   }
+  if (norm) E += pow2_normalize(e);
   if (m == 0) {
     layer_step_fwd(wvno2, om2, iom2, mdl[0], mdl[1], e);
     E += pow2_normalize(e);
@@ -619,9 +624,10 @@ __device__ __forceinline__ double inv_interp4(const double* X, const double* Y, 
 // verification round; the other lanes scan the next period.
 constexpr int DK_NW = 4;
 
+// (at most 256 VGPRs: two waves per SIMD on batches with enough columns)
 template <int S>
-__global__ void __launch_bounds__(256) k_forward(BatchDev B, int col_begin, int col_end,
-                                                 int persistent, int lds_stride) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride) {
   extern __shared__ double4 lds[];
   const int lane = threadIdx.x & 63;
   const int r = lane & (S - 1);
