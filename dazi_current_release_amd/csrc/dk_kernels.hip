@@ -662,6 +662,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
   int nref = 0;     // multisection rounds of the current bracket
   int nround = 0;   // evaluation rounds of the current column (hang guard)
   double cstar = 0, c1n = 0, omega_n = 1;  // verification round: estimate, next scan start
+  double pcur = 1, pnext = 1;              // periods k and k+1
   int nver = 0;                            // verification rounds of the current period
 
   auto begin_column = [&]() {
@@ -694,7 +695,9 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
     k = 0;
     c1 = cm;
     clow = cm;
-    omega = twopi / period_of(B, toff, 0);
+    pcur = period_of(B, toff, 0);
+    omega = twopi / pcur;
+    pnext = K > 1 ? period_of(B, toff, 1) : 1.0;
     st = ST_START;
   };
   begin_column();
@@ -727,6 +730,10 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
       st = ST_DONE;
       return false;
     }
+    // period k is in pnext; period k+1 is read now, a round or more before
+    // its first use
+    pcur = pnext;
+    pnext = k + 1 < K ? period_of(B, toff, k + 1) : 1.0;
     return true;
   };
 
@@ -741,33 +748,57 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
     }
     if (!__any(st != ST_DONE)) break;
 
-    // ---- trial point of this lane
-    double x, om = omega;
-    bool elig = true;
-    if (st == ST_START)
-      x = c1 + (double)r * dc;
-    else if (st == ST_UP)
-      x = c1 + (double)(r + 1) * dc;
-    else if (st == ST_DN) {
-      x = c1 - (double)(r + 1) * dc;
-      elig = x > clow;
-    } else if (st == ST_REF)
-      x = lo + (double)(r + 1) * ((hi - lo) / (double)(S + 1));
-    else if (st == ST_VER) {
-      if (r < DK_NW) {
+    // ---- trial points: lane r evaluates trial(r); the search logic below
+    // recomputes any other lane's point from the same expression and the
+    // state at the round's start (the same bits, and no cross-lane read).
+    // Every state's grid is xb + (j + jo) * xs (a decreasing scan as a
+    // negative step: c1 - a*dc and c1 + a*(-dc) round alike); the
+    // verification window's four points are cstar + off * tol.
+    double xb = 1.0, xs = 0.0;
+    int jo = 0;
+    if (st == ST_START) {
+      xb = c1;
+      xs = dc;
+    } else if (st == ST_UP) {
+      xb = c1;
+      xs = dc;
+      jo = 1;
+    } else if (st == ST_DN) {
+      xb = c1;
+      xs = -dc;
+      jo = 1;
+    } else if (st == ST_REF) {
+      xb = lo;
+      xs = (hi - lo) / (double)(S + 1);
+      jo = 1;
+    } else if (st == ST_VER && k + 1 < K) {
+      xb = c1n;
+      xs = dc;
+      jo = -DK_NW;
+    }
+    const bool ver_r = st == ST_VER;
+    const double cstar_r = cstar;
+    auto trial = [&](int j) -> double {
+      if (ver_r && j < DK_NW) {
         // window cstar -/+ {0.95, 0.05} tol: every sub-interval is below tol
-        const double off = (r == 0) ? -0.95 : (r == 1) ? -0.05 : (r == 2) ? 0.05 : 0.95;
-        x = cstar + off * tol;
-        elig = (x > lo) && (x < hi);
-      } else if (k + 1 < K) {
-        x = c1n + (double)(r - DK_NW) * dc;
-        om = omega_n;
-      } else {
-        x = 1.0;
-        elig = false;
+        const double off = (j == 0) ? -0.95 : (j == 1) ? -0.05 : (j == 2) ? 0.05 : 0.95;
+        return cstar_r + off * tol;
       }
-    } else {
-      x = 1.0;
+      return xb + (double)(j + jo) * xs;
+    };
+    const double x = trial(r);
+    double om = omega;
+    bool elig = true;
+    if (st == ST_DN) {
+      elig = x > clow;
+    } else if (st == ST_VER) {
+      if (r < DK_NW)
+        elig = (x > lo) && (x < hi);
+      else if (k + 1 < K)
+        om = omega_n;
+      else
+        elig = false;
+    } else if (st != ST_START && st != ST_UP && st != ST_REF) {
       elig = false;
     }
     FV f = {1.0, 0};
@@ -779,7 +810,14 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
     const uint64_t flall = group_bits<S>(__ballot(fl), gbase);
     auto sign_at = [&](int j) { return (int)((neg >> j) & 1ull); };
     auto fv_at = [&](int j) { return FV{__shfl(f.m, gbase + j, 64), __shfl(f.e, gbase + j, 64)}; };
-    auto x_at = [&](int j) { return __shfl(x, gbase + j, 64); };
+    auto x_at = [&](int j) { return trial(j); };
+    // sign changes between consecutive trial points j-1, j of the first n
+    // (p0: the sign before point 0)
+    auto changes = [&](uint64_t s, int p0, int n) {
+      const uint64_t m = n >= 64 ? ~0ull : ((1ull << n) - 1ull);
+      s &= m;
+      return (s ^ ((s << 1) | (uint64_t)p0)) & m;
+    };
 
     if (st == ST_DONE) continue;
     // every state is bounded by the betmx test, the clow floor or the bracket
@@ -810,11 +848,9 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
           return;
         }
       }
-      uint64_t chg = 0;
-      for (int j = first; j < n; j++) {
-        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(base + j - 1);
-        if (sign_at(base + j) != prev) chg |= 1ull << j;
-      }
+      // changes over j = first .. n-1 (j = 0 against the stored del1 sign)
+      uint64_t chg = changes(neg >> base, signbit(d1.m) ? 1 : 0, n);
+      if (first) chg &= ~1ull;
       uint64_t flm = (flall >> base) & ((n >= 64) ? ~0ull : ((1ull << n) - 1ull));
       if (first) flm &= ~1ull;
       int jc = ffs64(chg), jf = ffs64(flm);
@@ -859,7 +895,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
         if (!(cs > lo && cs < hi)) cs = fv_ratio_root(lo, hi, flo, fhi);
         cstar = cs;
         c1n = cs - onea * dc;
-        omega_n = (k + 1 < K) ? twopi / period_of(B, toff, k + 1) : omega;
+        omega_n = (k + 1 < K) ? twopi / pnext : omega;
         nver = 0;
         st = ST_VER;
       } else if (jf >= 0) {
@@ -945,11 +981,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
     if (st0 == ST_VER || st == ST_DONE) continue;
     if (st == ST_DN) {
       int nv = __popcll(okm);  // eligible lanes form a prefix
-      uint64_t chg = 0;
-      for (int j = 0; j < nv; j++) {
-        int prev = (j == 0) ? (signbit(d1.m) ? 1 : 0) : sign_at(j - 1);
-        if (sign_at(j) != prev) chg |= 1ull << j;
-      }
+      const uint64_t chg = changes(neg, signbit(d1.m) ? 1 : 0, nv);
       int jc = ffs64(chg);
       if (jc >= 0) {
         // bracket (x_jc, x_jc-1): lower end is the new point
@@ -982,11 +1014,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
         continue;
       }
     } else if (st == ST_REF) {
-      uint64_t chg = 0;
-      for (int j = 0; j < S; j++) {
-        int prev = (j == 0) ? slo : sign_at(j - 1);
-        if (sign_at(j) != prev) chg |= 1ull << j;
-      }
+      const uint64_t chg = changes(neg, slo, S);
       int jc = ffs64(chg);
       if (jc >= 0) {
         if (jc > 0) {
@@ -1015,7 +1043,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
       else
         root = 0.5 * (lo + hi);
       if (!emit(root)) continue;
-      omega = twopi / period_of(B, toff, k);
+      omega = twopi / pcur;
       c1 = root - onea * dc;
       clow = cm;
       st = ST_START;
