@@ -23,7 +23,8 @@
 namespace {
 
 thread_local std::string g_err;
-dazi_dk_tune g_tune = {16, 4, 0, 64, 2.0e-6};
+// cols_per_wg 0: chosen per launch (resolve_shape)
+dazi_dk_tune g_tune = {16, 0, 0, 64, 2.0e-6};
 int g_timing = 0;
 
 int set_err(int code, const char* fmt, ...) {
@@ -216,7 +217,7 @@ int check_tune(const dazi_dk_tune* t, dazi_dk_tune* out) {
   if (S != 8 && S != 16 && S != 32 && S != 64)
     return set_err(DAZI_DK_EINVAL, "lanes_per_scan must be 8, 16, 32 or 64 (got %d)", S);
   int block = S * out->cols_per_wg;
-  if (out->cols_per_wg < 1 || block > 256 || (block % 64) != 0)
+  if (out->cols_per_wg != 0 && (out->cols_per_wg < 1 || block > 256 || (block % 64) != 0))
     return set_err(DAZI_DK_EINVAL,
                    "lanes_per_scan*cols_per_wg must be a multiple of 64 and <= 256 (got %d*%d)",
                    S, out->cols_per_wg);
@@ -229,23 +230,51 @@ void rec(Pool* P, int i, hipStream_t st) {
   if (g_timing) (void)hipEventRecord(P->ev[i], st);
 }
 
-// The root search's LDS staging holds two double4 records per layer and column.
-int check_lds(const dazi_dk_tune& T, int lmax) {
-  size_t lds = (size_t)T.cols_per_wg * 2 * lmax * 32;
-  if (lds > 64 * 1024)
-    return set_err(DAZI_DK_ELIMIT, "model too deep for LDS staging: %d layers x %d columns", lmax,
-                   T.cols_per_wg);
+// LDS per workgroup of the current device (160 KiB on gfx950), looked up once
+// per device.
+size_t lds_per_block() {
+  static int lim[16] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 16) return 64 * 1024;
+  if (lim[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+      v = 64 * 1024;
+    lim[dev] = v;
+  }
+  return (size_t)lim[dev];
+}
+
+// The root search's workgroup shape for models of up to lmax layers: the LDS
+// staging holds two double4 records per layer and column.  cols_per_wg = 0
+// picks 256-thread workgroups (one wave on each SIMD of a CU: smaller groups
+// can leave the waves of a one-wave-per-SIMD launch crowded on part of the
+// device), halved while the layer records do not fit.
+int resolve_shape(dazi_dk_tune* T, int lmax) {
+  const size_t per_col = (size_t)2 * lmax * 32, lim = lds_per_block();
+  if (T->cols_per_wg == 0) {
+    int cpw = 256 / T->lanes_per_scan;
+    while (T->lanes_per_scan * cpw > 64 && (size_t)cpw * per_col > lim) cpw /= 2;
+    T->cols_per_wg = cpw;
+  }
+  if ((size_t)T->cols_per_wg * per_col > lim)
+    return set_err(DAZI_DK_ELIMIT,
+                   "model too deep for LDS staging: %d layers x %d columns need %zu bytes of LDS "
+                   "(the device has %zu per workgroup)",
+                   lmax, T->cols_per_wg, (size_t)T->cols_per_wg * per_col, lim);
   return 0;
 }
 
 // Root search over columns [0, ncol) with the configured mode on a device of
 // ncu compute units; *nlaunch receives the number of launches.  The persistent
 // work counter B.work is cleared by a kernel (no memset node in a graph).
-int forward_launches(const dk::BatchDev& B, const dazi_dk_tune& T, int lmax, int ncu,
+int forward_launches(const dk::BatchDev& B, const dazi_dk_tune& T0, int lmax, int ncu,
                      hipStream_t st, int* nlaunch) {
+  dazi_dk_tune T = T0;
+  if (int rc = resolve_shape(&T, lmax)) return rc;
   int S = T.lanes_per_scan, cpw = T.cols_per_wg;
   int lds_stride = 2 * lmax;
-  if (int rc = check_lds(T, lmax)) return rc;
   *nlaunch = 0;
   if (T.mode == 0) {
     HIPCHK(dk::launch_zero_int(B.work, 1, st));
@@ -1549,7 +1578,6 @@ int32_t dazi_dk_layered_noalloc_dev(const dazi_dk_layered* b, const dazi_dk_tune
   if (b->ncol == 0) return 0;
   dazi_dk_tune T;
   if (int rc = check_tune(tune, &T)) return rc;
-  if (int rc = check_lds(T, b->max_layers)) return rc;
   // the array's layout: work counter, per-column values, per-layer records,
   // then the unrounded roots unless the caller gives c_unrounded
   const size_t ncol = (size_t)b->ncol, nlay = (size_t)b->nlayers_total;
@@ -1566,6 +1594,10 @@ int32_t dazi_dk_layered_noalloc_dev(const dazi_dk_layered* b, const dazi_dk_tune
                    (long long)b->nsolves);
   NoAllocDev* N;
   if (int rc = noalloc_device(&N)) return rc;
+  {
+    dazi_dk_tune R = T;  // the LDS staging of the deepest model fits (device limit)
+    if (int rc = resolve_shape(&R, b->max_layers)) return rc;
+  }
   unsigned char* base = (unsigned char*)N->ptr;
   hipStream_t st = (hipStream_t)stream;
   double4* layers = (double4*)(base + o_lay);

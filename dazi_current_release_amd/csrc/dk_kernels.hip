@@ -1587,6 +1587,12 @@ static hipError_t launch_forward_S(const BatchDev& B, int col_begin, int col_end
   size_t lds = (size_t)cols_per_wg * lds_stride * sizeof(double4);
   if (!persistent) grid = (col_end - col_begin + cols_per_wg - 1) / cols_per_wg;
   if (grid <= 0) return hipSuccess;
+  if (lds > 64 * 1024) {
+    // dynamic LDS beyond 64 KiB (up to the device's per-workgroup limit)
+    hipError_t e = hipFuncSetAttribute((const void*)k_forward<S>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(k_forward<S>, dim3(grid), dim3(block), lds, st, B, col_begin, col_end,
                      persistent, lds_stride);
   return hipGetLastError();

@@ -77,7 +77,10 @@ typedef struct dazi_dk_desc {
 /* Launch configuration of the root search. Zero fields take defaults. */
 typedef struct dazi_dk_tune {
   int32_t lanes_per_scan;  /* trial velocities per scan step: 8, 16, 32 or 64 */
-  int32_t cols_per_wg;     /* columns per workgroup (lanes_per_scan*cols_per_wg <= 256) */
+  int32_t cols_per_wg;     /* columns per workgroup (lanes_per_scan*cols_per_wg <= 256);
+                              0 (default): 256-thread workgroups, fewer columns where the
+                              deepest model's LDS staging (64 B per layer and column) would
+                              exceed the device's LDS per workgroup */
   int32_t mode;            /* 0: batched persistent launch (default);
                               1: plain, one launch per tile of tile_columns columns */
   int32_t tile_columns;    /* plain-mode tile (default 64 = pdsurf_tile_columns) */
