@@ -522,57 +522,55 @@ __host__ __device__ constexpr int RED(int r) { return r < 3 ? r : r + 1; }
 // e <- e * CA with CA the reduced 5x5 compound of the layer propagator scaled
 // by e^{-(pex+svex)} (dnka_tregn): pure-wave compounds
 // dfac (u^v~)(a~^b~)^T / N^2 plus the mixed terms of H_P and H_S; row (14)
-// absorbs (23) = -(14).  CA is formed column by column and never stored.
+// absorbs (23) = -(14).  CA is never formed.  With the vector's coefficients
+// as the antisymmetric 4x4 C (C01 = e0, C02 = e1, C03 = e2, C12 = -e2,
+// C13 = e3, C23 = e4), the mixed part of column pair (k,l) is
+// sum_{i<j} C_ij (HP_ik HS_jl - HP_il HS_jk + HS_ik HP_jl - HS_il HP_jk)
+// = A_kl - A_lk with A = HP^T C HS, and H_ik = w_i M(i%2, k%2) z_k per wave
+// gives A_kl = zP_k zS_l G[k%2][l%2], G from the four parity sums of
+// wP_i C_ij wS_j.  The pure-wave part is (sum_r c_r (u^v~)_r) (a~^b~)_s per
+// wave, where (u^v~) is (w0 w1, 0, w0 w3, -w1 w2, 0, w2 w3) over the pairs
+// (01, 02, 03, 12, 13, 23), and the same for (a~^b~) with z.
 template <class G_, class H_, class T = typename G_::T>
 DK_HD void compound_step(const G_& g, const H_& h, T e[5]) {
   const double ex = h.pex + h.svex;
   const double dfac = (ex > 35.0) ? 0.0 : h.sclp * h.sclsv;  // e^{-ex}
   Wave<T> P, S;
   waves(g, h, 1.0, 1.0, &P, &S);
-  T HP[4][4], HS[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-      HP[i][j] = wave_entry(P, i, j);
-      HS[i][j] = wave_entry(S, i, j);
-    }
-  // (u^v~) and (a~^b~) of each wave: u, v~ = w on even / odd rows, a~, b~ = z
-  // on even / odd columns
+  const T e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
+  const T* wP = P.w;
+  const T* wS = S.w;
+  // pure-wave part: c . (u^v~) per wave, times dfac / N^2
   const T iNP2 = dfac * (g.rNP * g.rNP), iNS2 = dfac * (g.rNSV * g.rNSV);
-  T uvP[6], uvS[6], abP[6], abS[6];
+  auto cw = [&](const T* w) { return e0 * (w[0] * w[1]) + e2 * (w[0] * w[3] + w[1] * w[2]) +
+                                     e4 * (w[2] * w[3]); };
+  const T CP = cw(wP) * iNP2, CS = cw(wS) * iNS2;
+  // mixed part: parity sums E[a][b] of wP_i C_ij wS_j (i%2 = a, j%2 = b)
+  const T E00 = e1 * (wP[0] * wS[2] - wP[2] * wS[0]);
+  const T E11 = e3 * (wP[1] * wS[3] - wP[3] * wS[1]);
+  const T E01 = wP[0] * (e0 * wS[1] + e2 * wS[3]) + wP[2] * (e2 * wS[1] + e4 * wS[3]);
+  const T E10 = -(wP[1] * (e0 * wS[0] + e2 * wS[2]) + wP[3] * (e2 * wS[0] + e4 * wS[2]));
+  // M(a, b) of each wave: (0,0) = (1,1) = m00, (0,1) = m01, (1,0) = m10
+  auto mP = [&](int a, int b) { return a == b ? P.m00 : (a == 0 ? P.m01 : P.m10); };
+  auto mS = [&](int a, int b) { return a == b ? S.m00 : (a == 0 ? S.m01 : S.m10); };
+  T F[2][2], Gp[2][2];
 #pragma unroll
-  for (int r = 0; r < 6; r++) {
-    const int i = PI0(r), j = PI1(r);
-    auto wedge = [&](const T* w, int i, int j) {
-      // u = w on even indices, v~ = w on odd indices
-      const T ui = (i & 1) ? T{} : w[i], uj = (j & 1) ? T{} : w[j];
-      const T vi = (i & 1) ? w[i] : T{}, vj = (j & 1) ? w[j] : T{};
-      return ui * vj - uj * vi;
-    };
-    uvP[r] = wedge(P.w, i, j) * iNP2;
-    uvS[r] = wedge(S.w, i, j) * iNS2;
-    abP[r] = wedge(P.z, i, j);
-    abS[r] = wedge(S.z, i, j);
-  }
-  T out[5];
+  for (int a = 0; a < 2; a++)
 #pragma unroll
-  for (int sc = 0; sc < 5; sc++) {
-    const int s = RED(sc), k = PI0(s), l = PI1(s);
-    T acc{};
+    for (int be = 0; be < 2; be++)
+      F[a][be] = (a == 0 ? E00 : E10) * mS(0, be) + (a == 0 ? E01 : E11) * mS(1, be);
 #pragma unroll
-    for (int rr = 0; rr < 6; rr++) {
-      const int i = PI0(rr), j = PI1(rr);
-      const T v = uvP[rr] * abP[s] + uvS[rr] * abS[s] + HP[i][k] * HS[j][l] -
-                  HP[i][l] * HS[j][k] + HS[i][k] * HP[j][l] - HS[i][l] * HP[j][k];
-      // reduced rows: 0,1,2 -> pairs 0,1,2; 3,4 -> pairs 4,5; row 2 minus pair 3
-      const T coef = (rr == 3) ? -e[2] : (rr < 3 ? e[rr] : e[rr - 1]);
-      acc = mad(coef, v, acc);
-    }
-    out[sc] = acc;
-  }
+  for (int al = 0; al < 2; al++)
 #pragma unroll
-  for (int q = 0; q < 5; q++) e[q] = out[q];
+    for (int be = 0; be < 2; be++) Gp[al][be] = mP(0, al) * F[0][be] + mP(1, al) * F[1][be];
+  auto A = [&](int k, int l) { return (P.z[k] * S.z[l]) * Gp[k & 1][l & 1]; };
+  const T* zP = P.z;
+  const T* zS = S.z;
+  e[0] = CP * (zP[0] * zP[1]) + CS * (zS[0] * zS[1]) + (A(0, 1) - A(1, 0));
+  e[1] = A(0, 2) - A(2, 0);
+  e[2] = CP * (zP[0] * zP[3]) + CS * (zS[0] * zS[3]) + (A(0, 3) - A(3, 0));
+  e[3] = A(1, 3) - A(3, 1);
+  e[4] = CP * (zP[2] * zP[3]) + CS * (zS[2] * zS[3]) + (A(2, 3) - A(3, 2));
 }
 
 // Half-space radiation constraint: minors of rows P+ and S+ of G (evalg gbr),
