@@ -410,15 +410,43 @@ __global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_
 }
 
 // ------------------------------------------------------------------ value-only secular function
-// The root search only needs values of F.  Each layer is staged in LDS as two
-// double4, (d, 1/a^2, 1/b^2, b^2) and (rho, 1/rho, rho^2, 1/rho^2) of the
-// flattened layer, so an evaluation divides by no layer constant; per wave type
-// one exp gives both exp(-p) and exp(-2p), and exp(-(p+q)) is their product.
-// Same function as dk::secular (surfdisp96.f:767-865 dltar4 with var/dnka),
-// rounded differently.
-__device__ __forceinline__ void layer_rec(const double4& l, double4* g, double4* r) {
-  *g = make_double4(l.x, 1.0 / (l.y * l.y), 1.0 / (l.z * l.z), l.z * l.z);
-  *r = make_double4(l.w, 1.0 / l.w, l.w * l.w, 1.0 / (l.w * l.w));
+// The root search only needs the sign and value of F(c, omega), the Rayleigh
+// determinant that dltar4 forms from the Dunkin compound matrices of the layers
+// (surfdisp96.f:767-865 with var and dnka).  Here the same determinant is
+// propagated in each layer's P-SV potential basis (phi, phi', psi, psi'), in
+// which the compound matrices factor:
+//  * inside a layer the potentials propagate block-diagonally, P_a (+) P_b with
+//    P = [[C, S/r], [r S, C]] of unit determinant, so the compound keeps its
+//    (12) and (34) entries and maps the 2x2 array Y of the mixed entries
+//    (13) (14) / (23) (24) to P_a^T Y P_b (16 multiply-adds);
+//  * at an interface the change of basis M_lo^-1 M_up is two 2x2 blocks with
+//    the same four entries; its compound maps the symmetric array
+//    S = [[(13), (12)], [(12), (24)]] to N S N^T with
+//    N = [[a, a - 1], [eta, 1 + eta]], a = eps - eta, eps = rho_up / rho_lo,
+//    eta = 2 (mu_up - mu_lo) / (rho_lo c^2), and scales (14), (23) by eps
+//    (18 operations);
+//  * reciprocity (the propagators are symplectic) keeps (34) = -(12), so five
+//    entries carry the state.
+// The entries are scaled by powers of k so that a layer needs only
+// X_n = 1 - c^2 / v^2 and k d, and an evanescent layer's functions by
+// e^-(p+q) as var does.  The value is dltar4's times a positive factor (the
+// Jacobians of the basis changes), so roots, signs and the reference's scan
+// decisions are the same; a layer costs about 35 operations besides its two
+// wave functions, against about 95 for building and applying the Dunkin
+// matrix.  (This is the factorisation of the fast delta matrix, Buchen and
+// Ben-Hador, Geophys. J. Int. 124 (1996) 869-887.)
+//
+// LDS record of layer m (two double4): (d, 1/a^2, 1/b^2, eps_m) and
+// (dmu_m, -, -, -), with eps_m = rho_{m-1}/rho_m and
+// dmu_m = 2 (mu_{m-1} - mu_m) / rho_m for the interface above layer m (the
+// identity, eps = 1 and dmu = 0, above the top layer).
+__device__ __forceinline__ void layer_rec(const double4& l, const double4& up, bool top,
+                                          double4* g, double4* r) {
+  const double irho = 1.0 / l.w;
+  const double eps = top ? 1.0 : up.w * irho;
+  const double dmu = top ? 0.0 : 2.0 * (up.w * up.z * up.z - l.w * l.z * l.z) * irho;
+  *g = make_double4(l.x, 1.0 / (l.y * l.y), 1.0 / (l.z * l.z), eps);
+  *r = make_double4(dmu, 0.0, 0.0, 0.0);
 }
 
 // One wave type of a layer for X = k^2 - kx^2: C = cosh(p), S = sinh(p)/sqrt(X),
@@ -456,62 +484,38 @@ __device__ __forceinline__ double hyp_fwd(double X, double d, double* C, double*
   return 1.0;
 }
 
-__device__ __forceinline__ void halfspace_fwd(double wvno2, double om2, double iom2,
-                                              const double4& g, const double4& r, double e[5]) {
-  const double ra = sqrt(fabs(fma(-om2, g.y, wvno2)));
-  const double rb = sqrt(fabs(fma(-om2, g.z, wvno2)));
-  const double gammk = 2.0 * g.w * iom2;
-  const double gam = gammk * wvno2;
-  const double gamm1 = gam - 1.0;
-  const double rarb = ra * rb;
-  const double rho = r.x;
-  e[0] = r.z * (gamm1 * gamm1 - gam * gammk * rarb);
-  e[1] = -(rho * ra);
-  e[2] = rho * (gamm1 - gammk * rarb);
-  e[3] = rho * rb;
-  e[4] = wvno2 - rarb;
+// State e = (S11, S12, S22, Y12, Y21): S11 = (13), S12 = k (12), S22 = -k^2 (24),
+// Y12 = -k (14), Y21 = k (23) of the compound row vector, up to a common factor.
+// Propagation through a layer with the record g = (d, 1/a^2, 1/b^2, -).
+__device__ __forceinline__ void fdm_layer(double c2, double k, const double4& g, double e[5]) {
+  const double kd = k * g.x;
+  double Ca, Sa, Ra, Cb, Sb, Rb, pa, pb;
+  const double sa = hyp_fwd(fma(-c2, g.y, 1.0), kd, &Ca, &Sa, &Ra, &pa);
+  const double sb = hyp_fwd(fma(-c2, g.z, 1.0), kd, &Cb, &Sb, &Rb, &pb);
+  const double s11 = e[0], s22 = e[2], y12 = e[3], y21 = e[4];
+  // P_a^T Y, P_a^T = [[Ca, Ra], [Sa, Ca]] (k-scaled)
+  const double t11 = fma(Ca, s11, Ra * y21), t12 = fma(Ca, y12, Ra * s22);
+  const double t21 = fma(Sa, s11, Ca * y21), t22 = fma(Sa, y12, Ca * s22);
+  // ... P_b, P_b = [[Cb, -Sb], [-Rb, Cb]] (k-scaled)
+  e[0] = fma(t11, Cb, -t12 * Rb);
+  e[3] = fma(t12, Cb, -t11 * Sb);
+  e[4] = fma(t21, Cb, -t22 * Rb);
+  e[2] = fma(t22, Cb, -t21 * Sb);
+  e[1] *= sa * sb;  // (12) passes the layer unchanged: only the layer's scale
 }
 
-// e <- e * CA(layer) with the Dunkin matrix of dnka, fused (no 5x5 array).
-__device__ __forceinline__ void layer_step_fwd(double wvno2, double om2, double iom2,
-                                               const double4& g, const double4& r, double e[5]) {
-  const double d = g.x;
-  const double gammk = 2.0 * g.w * iom2;
-  const double gam = gammk * wvno2;
-  double cosp, w, x, cosq, y, z, pex, sex;
-  const double sP = hyp_fwd(fma(-om2, g.y, wvno2), d, &cosp, &w, &x, &pex);
-  const double sS = hyp_fwd(fma(-om2, g.z, wvno2), d, &cosq, &y, &z, &sex);
-  const double a0 = sP * sS;  // the reference's var drops it for p + q >= 60 (< 1e-26)
-  const double cpcq = cosp * cosq, cpy = cosp * y, cpz = cosp * z, cqw = cosq * w,
-               cqx = cosq * x, xy = x * y, xz = x * z, wy = w * y, wz = w * z;
-  const double gamm1 = gam - 1.0, twgm1 = gam + gamm1, gmgmk = gam * gammk, gmgm1 = gam * gamm1,
-               gm1sq = gamm1 * gamm1;
-  const double rho = r.x, irho = r.y, rho2 = r.z, irho2 = r.w;
-  const double a0pq = a0 - cpcq;
-  const double c00 = cpcq - 2.0 * gmgm1 * a0pq - gmgmk * xz - wvno2 * gm1sq * wy;
-  const double c01 = (wvno2 * cpy - cqx) * irho;
-  const double c02 = -(twgm1 * a0pq + gammk * xz + wvno2 * gamm1 * wy) * irho;
-  const double c03 = (cpz - wvno2 * cqw) * irho;
-  const double c04 = -(2.0 * wvno2 * a0pq + xz + wvno2 * wvno2 * wy) * irho2;
-  const double c10 = (gmgmk * cpz - gm1sq * cqw) * rho;
-  const double c12 = gammk * cpz - gamm1 * cqw;
-  const double c13 = -wz;
-  const double c30 = (gm1sq * cpy - gmgmk * cqx) * rho;
-  const double c31 = -xy;
-  const double c32 = gamm1 * cpy - gammk * cqx;
-  const double c40 = -(2.0 * gmgmk * gm1sq * a0pq + gmgmk * gmgmk * xz + gm1sq * gm1sq * wy) * rho2;
-  const double c42 = -(gammk * gamm1 * twgm1 * a0pq + gam * gammk * gammk * xz + gamm1 * gm1sq * wy) * rho;
-  const double tt = -2.0 * wvno2;
-  const double c20 = tt * c42, c21 = tt * c32, c22 = a0 + 2.0 * (cpcq - c00), c23 = tt * c12,
-               c24 = tt * c02;
-  // rows of CA: 1: (c10, cpcq, c12, c13, c03); 3: (c30, c31, c32, cpcq, c01);
-  //             4: (c40, c30, c42, c10, c00)
-  const double e0 = e[0], e1 = e[1], e2 = e[2], e3 = e[3], e4 = e[4];
-  e[0] = fma(e0, c00, fma(e1, c10, fma(e2, c20, fma(e3, c30, e4 * c40))));
-  e[1] = fma(e0, c01, fma(e1, cpcq, fma(e2, c21, fma(e3, c31, e4 * c30))));
-  e[2] = fma(e0, c02, fma(e1, c12, fma(e2, c22, fma(e3, c32, e4 * c42))));
-  e[3] = fma(e0, c03, fma(e1, c13, fma(e2, c23, fma(e3, cpcq, e4 * c10))));
-  e[4] = fma(e0, c04, fma(e1, c03, fma(e2, c24, fma(e3, c01, e4 * c00))));
+// The interface above a layer: eps, dmu of its record, ic2 = 1/c^2.
+__device__ __forceinline__ void fdm_iface(double ic2, double eps, double dmu, double e[5]) {
+  const double eta = dmu * ic2;
+  const double a = eps - eta, am1 = a - 1.0, b = 1.0 + eta;
+  const double s11 = e[0], s12 = e[1], s22 = e[2];
+  const double y1 = fma(a, s11, am1 * s12), y2 = fma(a, s12, am1 * s22);
+  const double z1 = fma(eta, s11, b * s12), z2 = fma(eta, s12, b * s22);
+  e[0] = fma(a, y1, am1 * y2);
+  e[1] = fma(eta, y1, b * y2);
+  e[2] = fma(eta, z1, b * z2);
+  e[3] *= eps;
+  e[4] *= eps;
 }
 
 // F(c, omega) as mantissa and exponent over the LDS-staged records of L layers
@@ -519,40 +523,47 @@ __device__ __forceinline__ void layer_step_fwd(double wvno2, double om2, double 
 __device__ __forceinline__ double secular_fwd(double c, double omega, int L, const double4* mdl,
                                               int* ex) {
   if (omega < 1.0e-4) omega = 1.0e-4;
-  const double wvno = omega / c, wvno2 = wvno * wvno, om2 = omega * omega, iom2 = 1.0 / om2;
-  double e[5];
-  halfspace_fwd(wvno2, om2, iom2, mdl[2 * (L - 1)], mdl[2 * (L - 1) + 1], e);
-  int E = pow2_normalize(e);
-  // two layers per step: the second layer's matrix is independent of the
-  // first layer's update.  One normalisation per two steps keeps the vector
-  // far inside the double range (a layer scales it by at most ~1e4).  The
-  // next pair's records are read from LDS one step ahead (clamped indices),
-  // so that the read latency overlaps this pair's work.
+  const double k = omega / c, c2 = c * c, ic2 = 1.0 / c2;
+  // the half-space's decaying P and S potentials, then the interface above it
+  const double4 gH = mdl[2 * (L - 1)];
+  const double ran = sqrt(fabs(fma(-c2, gH.y, 1.0)));
+  const double rbn = sqrt(fabs(fma(-c2, gH.z, 1.0)));
+  double e[5] = {ran * rbn, 0.0, -1.0, -ran, rbn};
+  fdm_iface(ic2, gH.w, mdl[2 * (L - 1) + 1].x, e);
+  int E = 0;
+  // two layers per step; one normalisation per two steps keeps the state far
+  // inside the double range.  The next pair's records are read from LDS one
+  // step ahead (clamped indices), so that the read latency overlaps this
+  // pair's work.
   int m = L - 2;
   auto cl = [&](int i) { return min(max(i, 0), L - 1); };  // reads past the end stay in range
   int ia = cl(m), ib = cl(m - 1);
-  double4 ga = mdl[2 * ia], ra = mdl[2 * ia + 1], gb = mdl[2 * ib], rb = mdl[2 * ib + 1];
+  double4 ga = mdl[2 * ia], gb = mdl[2 * ib];
+  double da = mdl[2 * ia + 1].x, db = mdl[2 * ib + 1].x;
   bool norm = false;
   for (; m >= 1; m -= 2) {
-    const double4 g0 = ga, r0 = ra, g1 = gb, r1 = rb;
+    const double4 g0 = ga, g1 = gb;
+    const double d0 = da, d1 = db;
     ia = cl(m - 2);
     ib = cl(m - 3);
     ga = mdl[2 * ia];
-    ra = mdl[2 * ia + 1];
     gb = mdl[2 * ib];
-    rb = mdl[2 * ib + 1];
-    layer_step_fwd(wvno2, om2, iom2, g0, r0, e);
-    layer_step_fwd(wvno2, om2, iom2, g1, r1, e);
+    da = mdl[2 * ia + 1].x;
+    db = mdl[2 * ib + 1].x;
+    fdm_layer(c2, k, g0, e);
+    fdm_iface(ic2, g0.w, d0, e);
+    fdm_layer(c2, k, g1, e);
+    fdm_iface(ic2, g1.w, d1, e);  // the identity above the top layer
     if (norm) E += pow2_normalize(e);
     norm = !norm;
   }
-  if (norm) E += pow2_normalize(e);
-  if (m == 0) {
-    layer_step_fwd(wvno2, om2, iom2, mdl[0], mdl[1], e);
-    E += pow2_normalize(e);
-  }
+  if (m == 0) fdm_layer(c2, k, mdl[0], e);
+  E += pow2_normalize(e);
+  // free surface of the top layer: F = -(4 S11 + 4 t S12 + t^2 S22),
+  // t = 2 - c^2 / b_0^2
+  const double t = fma(-c2, mdl[0].z, 2.0);
   *ex = E;
-  return e[0];
+  return -fma(t, fma(t, e[2], 4.0 * e[1]), 4.0 * e[0]);
 }
 
 // ------------------------------------------------------------------ forward root search
@@ -671,7 +682,9 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
       return;
     }
     col_geom(B, col, &L, &loff, &K, &toff);
-    for (int m = r; m < L; m += S) layer_rec(B.layers[loff + m], &mdl[2 * m], &mdl[2 * m + 1]);
+    for (int m = r; m < L; m += S)
+      layer_rec(B.layers[loff + m], B.layers[loff + max(m - 1, 0)], m == 0, &mdl[2 * m],
+                &mdl[2 * m + 1]);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
