@@ -41,7 +41,25 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     for o in objs:
         os.remove(o)
+    build_hostcheck(verbose)
     return LIB
+
+
+HOSTCHECK = os.path.join(HERE, "libdazi_dk_hostcheck.so")
+
+
+def build_hostcheck(verbose: bool = False) -> str:
+    """Host-only build of the root search's secular function (csrc/dk_fdm.h)
+    for the CPU tests that pin it against the oracle; not loaded by the
+    library or its bindings."""
+    tmp = HOSTCHECK + ".tmp"
+    cmd = [HIPCC, "-O2", "-std=c++17", "-fPIC", "-shared", os.path.join(CSRC, "dk_hostcheck.cpp"),
+           "-o", tmp]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, HOSTCHECK)
+    return HOSTCHECK
 
 
 FORTRAN = os.path.join(HERE, "fortran")

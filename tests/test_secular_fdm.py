@@ -1,0 +1,158 @@
+"""The root search's secular function (csrc/dk_fdm.h, compiled for the host)
+against the oracle's dltar4 (surfdisp96.f:767-865, Dunkin compound matrices).
+
+The root search evaluates the Rayleigh determinant in each layer's potential
+basis, which makes it dltar4's value times a positive factor.  These checks
+pin that on random flattened models (gradients, low-velocity layers, slow
+surface layers, thin and thick layers, periods 0.5-100 s, trial velocities
+from 0.3 b_min to 1.1 b_max):
+  * the sign agrees with dltar4's at every trial velocity (so every scan step
+    of the reference's getsol takes the same decision);
+  * the ratio to dltar4 is positive and finite, and smooth: the two
+    functions' roots, bisected to 1e-12 km/s, agree to 1e-9 km/s;
+  * a planted mistake (the interface's eta with the wrong sign) fails the
+    sign check, so the comparison can fail.
+Closed form: one half-space alone gives the Rayleigh function
+(2 - c^2/b^2)^2 - 4 sqrt(1 - c^2/a^2) sqrt(1 - c^2/b^2) exactly."""
+import ctypes as C
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from oracle import port as P
+
+LIB = os.path.join(ROOT, "dazi_current_release_amd", "libdazi_dk_hostcheck.so")
+
+
+@pytest.fixture(scope="module")
+def hc():
+    if not os.path.exists(LIB):
+        from dazi_current_release_amd import build as B
+        B.build_hostcheck()
+    lib = C.CDLL(LIB)
+    f = lib.dk_host_secular
+    f.restype = C.c_double
+    fp = C.POINTER(C.c_float)
+    f.argtypes = [C.c_double, C.c_double, C.c_int, fp, fp, fp, fp, C.POINTER(C.c_int)]
+
+    def F(c, T, d, a, b, rho):
+        ex = C.c_int(0)
+        arr = [np.ascontiguousarray(x, np.float32) for x in (d, a, b, rho)]
+        m = f(c, T, len(d), *[x.ctypes.data_as(fp) for x in arr], C.byref(ex))
+        return np.ldexp(m, ex.value)
+    return F
+
+
+def random_model(rng):
+    n = int(rng.integers(2, 14))
+    b = np.sort(rng.uniform(0.8, 4.7, n)).astype(np.float32)
+    if rng.random() < 0.4:
+        i = rng.integers(0, n - 1)
+        b[i] *= rng.uniform(0.6, 0.95)
+    if rng.random() < 0.2:
+        b[0] = rng.uniform(0.3, 1.0)
+    a = (b * rng.uniform(1.6, 2.2, n)).astype(np.float32)
+    rho = (1.7 + 0.2 * a + rng.uniform(-0.3, 0.3, n)).astype(np.float32)
+    d = rng.uniform(0.05, 20, n).astype(np.float32)
+    d[-1] = 0
+    T = float(np.exp(rng.uniform(np.log(0.5), np.log(100))))
+    return d, a, b, rho, T
+
+
+def bisect(f, lo, hi, flo):
+    for _ in range(200):
+        if hi - lo <= 1e-12:
+            break
+        mid = 0.5 * (lo + hi)
+        fm = f(mid)
+        if (fm < 0) == (flo < 0):
+            lo, flo = mid, fm
+        else:
+            hi = mid
+    return 0.5 * (lo + hi)
+
+
+def test_halfspace_is_the_rayleigh_function(hc):
+    a, b, rho = 6.0, 3.5, 2.8
+    for c in (1.0, 2.5, 3.1, 3.2, 3.4):
+        want = (2 - c * c / b ** 2) ** 2 - 4 * np.sqrt(1 - c * c / a ** 2) * np.sqrt(1 - c * c / b ** 2)
+        got = hc(c, 10.0, [0.0], [a], [b], [rho])
+        assert abs(got - want) <= 1e-13 * max(1.0, abs(want)), (c, got, want)
+
+
+def test_signs_and_roots_match_dltar4(hc):
+    rng = np.random.default_rng(20261204)
+    nroots = 0
+    for _ in range(120):
+        d, a, b, rho, T = random_model(rng)
+        cs = np.linspace(0.3 * float(b.min()), 1.1 * float(b.max()), 160)
+        f1 = np.array([P.dltar4(c, T, d, a, b, rho) for c in cs])
+        f2 = np.array([hc(c, T, d, a, b, rho) for c in cs])
+        assert np.all(np.isfinite(f2))
+        nz = f1 != 0
+        assert np.array_equal(np.signbit(f1[nz]), np.signbit(f2[nz]))
+        r = f2[nz] / f1[nz]
+        assert np.all(r > 0)
+        # roots of both in the first three sign changes
+        chg = np.nonzero(np.signbit(f1[:-1]) != np.signbit(f1[1:]))[0][:3]
+        for j in chg:
+            r1 = bisect(lambda c: P.dltar4(c, T, d, a, b, rho), cs[j], cs[j + 1], f1[j])
+            r2 = bisect(lambda c: hc(c, T, d, a, b, rho), cs[j], cs[j + 1], f2[j])
+            assert abs(r1 - r2) <= 1e-9, (r1, r2)
+            nroots += 1
+    assert nroots > 100
+
+
+def test_planted_sign_mistake_is_caught(hc):
+    """The same comparison on a restatement with eta's sign flipped (numpy,
+    the algebra of dk_fdm.h): its signs disagree with dltar4's."""
+    def fdm(c, T, d, a, b, rho, s_eta):
+        d, a, b, rho = (np.asarray(x, np.float64) for x in (d, a, b, rho))
+        om = 2 * np.pi / T
+        k, c2 = om / c, c * c
+        mu = rho * b * b
+        H = len(d) - 1
+        ran, rbn = np.sqrt(abs(1 - c2 / a[H] ** 2)), np.sqrt(abs(1 - c2 / b[H] ** 2))
+        e = [ran * rbn, 0.0, -1.0, -ran, rbn]
+
+        def iface(m, e):
+            eps, eta = rho[m - 1] / rho[m], s_eta * 2 * (mu[m - 1] - mu[m]) / (rho[m] * c2)
+            A, B = eps - eta, 1 + eta
+            y1, y2 = A * e[0] + (A - 1) * e[1], A * e[1] + (A - 1) * e[2]
+            z1, z2 = eta * e[0] + B * e[1], eta * e[1] + B * e[2]
+            return [A * y1 + (A - 1) * y2, eta * y1 + B * y2, eta * z1 + B * z2, eps * e[3], eps * e[4]]
+
+        def hyp(X, kd):
+            r = np.sqrt(abs(X))
+            if X > 0:
+                return np.cosh(r * kd), np.sinh(r * kd) / r, r * np.sinh(r * kd)
+            return np.cos(r * kd), np.sin(r * kd) / r, -r * np.sin(r * kd)
+        e = iface(H, e)
+        for m in range(H - 1, -1, -1):
+            Ca, Sa, Ra = hyp(1 - c2 / a[m] ** 2, k * d[m])
+            Cb, Sb, Rb = hyp(1 - c2 / b[m] ** 2, k * d[m])
+            t11, t12 = Ca * e[0] + Ra * e[4], Ca * e[3] + Ra * e[2]
+            t21, t22 = Sa * e[0] + Ca * e[4], Sa * e[3] + Ca * e[2]
+            e = [t11 * Cb - t12 * Rb, e[1], -t21 * Sb + t22 * Cb, -t11 * Sb + t12 * Cb,
+                 t21 * Cb - t22 * Rb]
+            if m > 0:
+                e = iface(m, e)
+        t = 2 - c2 / b[0] ** 2
+        return -(4 * e[0] + 4 * t * e[1] + t * t * e[2])
+
+    rng = np.random.default_rng(5)
+    caught = 0
+    for _ in range(20):
+        d, a, b, rho, T = random_model(rng)
+        # moderate phases: the unscaled restatement stays in range
+        d = np.minimum(d, 3.0)
+        T = max(T, 5.0)
+        cs = np.linspace(0.5 * float(b.min()), 0.98 * float(b.max()), 60)
+        f1 = np.array([P.dltar4(c, T, d, a, b, rho) for c in cs])
+        good = np.array([fdm(c, T, d, a, b, rho, 1.0) for c in cs])
+        bad = np.array([fdm(c, T, d, a, b, rho, -1.0) for c in cs])
+        assert np.array_equal(np.signbit(f1), np.signbit(good))
+        caught += not np.array_equal(np.signbit(f1), np.signbit(bad))
+    assert caught >= 10
