@@ -21,3 +21,14 @@ extern "C" double dk_host_secular(double c, double period, int L, const float* d
   const double omega = 2.0 * 3.141592653589793 / period;
   return dk::secular_fwd(c, omega, L, mdl.data(), ex);
 }
+
+// Kernels -(dF/dm)/(dF/dc) at (c, period) in the flattened (a, b, rho, d) of
+// each layer (out: 4 L), from the building blocks the kernel pass and the VJP
+// use; returns dF/dc (mantissa, exponent in *ex).
+extern "C" double dk_host_kernels(double c, double period, int L, const float* d, const float* a,
+                                  const float* b, const float* rho, double* out, int* ex) {
+  if (L < 1 || L > 1024) return 0.0;
+  std::vector<double4> lay((size_t)L);
+  for (int m = 0; m < L; m++) lay[m] = make_double4(d[m], a[m], b[m], rho[m]);
+  return dk::fdm_kernels_ref(c, 2.0 * 3.141592653589793 / period, L, lay.data(), out, ex);
+}

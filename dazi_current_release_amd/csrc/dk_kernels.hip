@@ -909,7 +909,19 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
 }
 
 // ------------------------------------------------------------------ kernel pass
-// Tape layout: tape[(m*6 + i) * nthr + tid], i<5 vector entries, i==5 exponent.
+// Implicit differentiation at the root, dc/dm = -(dF/dm)/(dF/dc), on the
+// potential-basis recursion of dk_fdm.h (the value is dltar4's times a
+// positive factor, which drops out at F = 0).
+//   sweep 1, bottom-up: the state and its tangent in c; the state entering
+//            each layer is taped with its exponent;
+//   sweep 2, top-down: the adjoint dF/d(state); at layer m the taped state is
+//            propagated through the layer again (its wave functions are needed
+//            for the adjoint anyway), the interface above it and the layer are
+//            differentiated, and the partials in the flattened (a, b, rho) are
+//            chained to the unflattened layer.  The interface above layer m
+//            also depends on layer m-1's (rho, b), so each layer's partials are
+//            complete, and emitted, one step later.
+// Tape layout: tape[(m*6 + i) * nthr + tid], i<5 state entries, i==5 exponent.
 __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, int col, int k,
                                            int64_t s, double* __restrict__ tape, int64_t nthr,
                                            int tid) {
@@ -937,164 +949,119 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   const double* btr = B.btr + loff;
   double omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
   if (omega < 1.0e-4) omega = 1.0e-4;
-  const double wv = omega / c;
-  const double wv2 = wv * wv;
+  const double wk = omega / c, c2 = c * c, ic2 = 1.0 / c2, dwk = -wk / c;
+  const int H = L - 1;
+  auto cl = [&](int i) { return min(max(i, 0), H); };  // clamped record index
 
-  // ---- sweep 1: bottom-up values and their tangent in c; the tape holds the
-  // (normalised) vector entering layer m and its exponent
-  Dual<1> wvd = dconst<1>(wv);
-  wvd.g[0] = -wv / c;
-  const Dual<1> wv2d = wvd * wvd;
-  // the same for the layers, where only k carries a partial (a, b, rho are
-  // constants in c)
-  SD<1> wvs;
-  wvs.v = wv;
-  wvs.g[0] = -wv / c;
-  wvs.g[1] = wvs.g[2] = 0.0;
-  const SD<1> wv2s = wvs * wvs;
-  Dual<1> e[5];
+  // ---- sweep 1
+  double v[5], dv[5], ran, rbn, sgA, sgB;
+  const double4 lh = lay[H];
+  fdm_start(c2, lh, v, &ran, &rbn, &sgA, &sgB);
   {
-    double4 h = lay[L - 1];
-    halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
-                         dconst<1>((double)h.w), omega, e);
+    const double dra = start_dr(ran, sgA, 2.0 * c / (lh.y * lh.y));
+    const double drb = start_dr(rbn, sgB, 2.0 * c / (lh.z * lh.z));
+    dv[0] = fma(dra, rbn, ran * drb);
+    dv[1] = 0.0;
+    dv[2] = 0.0;
+    dv[3] = -dra;
+    dv[4] = drb;
   }
-  int Ef = 0;
-  // each layer's record is loaded one iteration ahead, before that iteration's
-  // tape stores (loads and stores share one counter: a load issued after the
-  // stores would wait for them too); the loads are unconditional, at clamped
-  // indices, so that they add no branch to the iteration
-  double4 lnext = lay[L >= 2 ? L - 2 : 0];
-  for (int m = L - 1; m >= 0; m--) {
-    if (m < L - 1) {
-      const double4 l = lnext;
-      lnext = lay[m >= 1 ? m - 1 : 0];
-      double* tp = tape + (int64_t)(m * 6) * nthr + tid;
-#pragma unroll
-      for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = e[i].v;
-      tp[(int64_t)5 * nthr] = (double)Ef;
-      double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      layer_ca_visit<SD<1>>(wvs, wv2s, (double)l.y, (double)l.z, (double)l.w, (double)l.x,
-                            omega, [&](int j, int i, const auto& v) {
-                              ev[i] = fma(e[j].v, v.v, ev[i]);
-                              eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
-                            });
-#pragma unroll
-      for (int i = 0; i < 5; i++) {
-        e[i].v = ev[i];
-        e[i].g[0] = eg[i];
-      }
-    }
-    // normalise values and tangent together (exact power of two)
-    double mx = 0.0;
-#pragma unroll
-    for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(e[i].v));
-    int kk = 0;
-    if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
+  auto iface_t = [&](const double4& up, const double4& lo) {
+    double eps, dmu, yz[4], w[5], dw[5];
+    iface_consts(up, lo, &eps, &dmu);
+    const double eta = dmu * ic2;
+    fdm_iface_fn(eps, eta, v, yz, w);
+    fdm_iface_tan(eps, eta, -2.0 * eta / c, v, dv, yz, dw);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      e[i].v = ldexp(e[i].v, -kk);
-      e[i].g[0] = ldexp(e[i].g[0], -kk);
+      v[i] = w[i];
+      dv[i] = dw[i];
     }
-    Ef += kk;
-  }
-  const int EF = Ef;  // exponent of the top vector: reference scale
-  // dc/dm = -(dF/dm)/(dF/dc), both at the reference scale 2^EF
-  const double dFdc = e[0].g[0];
-  const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
-
-  // ---- sweep 2: adjoint top-down with Dual<2> layer matrices in (a, b); the
-  // rho partial comes from the entries' rho powers
-  // g = dF/de (true scale g * 2^Eg); indices 0:a 1:b 2:rho
-  double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
-  int Eg = 0;
-  // node accumulation state
-  int node = 0, sub = 0;
-  double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
-  // the next layer's record, tape entry and flattening factors are loaded one
-  // iteration ahead, so their latency overlaps this layer's matrix
-  auto load_tape = [&](int m, double* t) {
-    const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
-#pragma unroll
-    for (int i = 0; i < 6; i++) t[i] = tp[(int64_t)i * nthr];
   };
-  // (unconditional loads at clamped indices: no branch added to the iteration;
-  // the tape holds at least one entry per thread)
-  const int mt = L >= 2 ? L - 2 : 0;  // last tape entry
-  double4 ln = lay[0];
-  double tn[6];
-  load_tape(0, tn);
-  double btn = btp[0], brn = btr[0];
-  int ns_cur = (O.node_mode && O.nz > 1) ? O.nsub[0] : 1;
-  for (int m = 0; m <= L - 1; m++) {
-    const double4 l = ln;
-    double tc[6];  // this layer's tape entry (read only below the half-space)
+  // layer records one iteration ahead (loads before the iteration's tape
+  // stores: a load issued after them would wait for them too)
+  double4 lcur = lay[cl(H - 1)];
+  if (H >= 1) iface_t(lcur, lh);
+  double4 lup = lay[cl(H - 2)];
+  int Ef = 0;
+  for (int m = H - 1; m >= 0; m--) {
+    const double4 l = lcur, lu = lup;
+    lcur = lup;
+    lup = lay[cl(m - 2)];
+    double* tp = tape + (int64_t)(m * 6) * nthr + tid;
 #pragma unroll
-    for (int i = 0; i < 6; i++) tc[i] = tn[i];
-    const double bt = btn, btr_m = brn;
-    const int mn = min(m + 1, L - 1);
-    ln = lay[mn];
-    load_tape(min(m + 1, mt), tn);
-    btn = btp[mn];
-    brn = btr[mn];
-    double part[3];
-    if (m < L - 1) {
-      const double* et = tc;
-      const int Ee = (int)tc[5];
-      // contribution sum_ji e_j dCA_ji g_i and the propagation g <- CA g,
-      // reduced entry by entry as the layer matrix is formed
-      double acc[3] = {0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-      // partials in (a, b) only where an entry depends on them (SD masks)
-      layer_ca_visit<double>(wv, wv2, sd_var<1>((double)l.y), sd_var<2>((double)l.z),
-                             (double)l.w, (double)l.x, omega, [&](int j, int i, const auto& v) {
-                                const double w = et[j] * g[i];
-                                acc[0] = fma(w, v.g[0], acc[0]);
-                                acc[1] = fma(w, v.g[1], acc[1]);
-                                acc[2] = fma(w * (double)rho_pow(j, i), v.v, acc[2]);
-                                gn[j] = fma(v.v, g[i], gn[j]);
-                              });
-      acc[2] /= (double)l.w;
+    for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = v[i];
+    tp[(int64_t)5 * nthr] = (double)Ef;
+    const double kd = wk * l.x, dkd = dwk * l.x;
+    WaveFn A, Bw;
+    wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
+    wave_fn(fma(-c2, 1.0 / (l.z * l.z), 1.0), kd, Bw);
+    double dA[4], dB[4], t[4], u[5], du[5];
+    wave_tan(A, kd, -2.0 * c / (l.y * l.y), dkd, dA);
+    wave_tan(Bw, kd, -2.0 * c / (l.z * l.z), dkd, dB);
+    fdm_layer_fn(A, Bw, v, t, u);
+    fdm_layer_tan(A, Bw, dA, dB, v, dv, t, du);
 #pragma unroll
-      for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * sc;
+    for (int i = 0; i < 5; i++) {
+      v[i] = u[i];
+      dv[i] = du[i];
+    }
+    if (m >= 1) iface_t(lu, l);
+    if ((m & 1) == 0 || m == H - 1) {
+      // values and tangent by the same exact power of two
+      double mx = 0.0;
 #pragma unroll
-      for (int j = 0; j < 5; j++) g[j] = gn[j];
-      Eg += pow2_normalize(g);
-    } else {
-      Dual<2> eh[5];
-      halfspace_e<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
-                           dvar<2>((double)l.z, 1), dconst<2>((double)l.w), omega, eh);
-      double acc[3] = {0.0, 0.0, 0.0};
+      for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(v[i]));
+      int kk = 0;
+      if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
 #pragma unroll
       for (int i = 0; i < 5; i++) {
-        acc[0] = fma(eh[i].g[0], g[i], acc[0]);
-        acc[1] = fma(eh[i].g[1], g[i], acc[1]);
-        acc[2] = fma((double)half_rho_pow(i) * eh[i].v, g[i], acc[2]);
+        v[i] = ldexp(v[i], -kk);
+        dv[i] = ldexp(dv[i], -kk);
       }
-      acc[2] /= (double)l.w;
-#pragma unroll
-      for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Eg - EF) * sc;
+      Ef += kk;
     }
-    // chain to the unflattened layer: a_flat = a*btp, b_flat = b*btp,
-    // rho_flat = rho*btp^-2.275
-    double pa = part[0] * bt, pb = part[1] * bt, pr = part[2] * btr_m;
+  }
+  const int EF = Ef;  // exponent of the surface state: reference scale
+  const double4 l0 = lay[0];
+  const double ib0 = 1.0 / (l0.z * l0.z);
+  const double tt = fma(-c2, ib0, 2.0);
+  const double tb = -fma(2.0 * tt, v[2], 4.0 * v[1]);  // dF/dt
+  const double dFdc = -fma(tt, fma(tt, dv[2], 4.0 * dv[1]), 4.0 * dv[0]) + tb * (-2.0 * c * ib0);
+  const double sc = (dFdc != 0.0) ? -1.0 / dFdc : 0.0;
+
+  // ---- sweep 2
+  double g[5] = {-4.0, -4.0 * tt, -tt * tt, 0.0, 0.0};
+  int Eg = 0;
+  // pending partials (flattened a, b, rho) of the previous layer, in kernel
+  // units; the surface's t depends on b_0
+  double pa = 0.0, pb = tb * 2.0 * c2 * ib0 / l0.z * sc, pr = 0.0;
+  int node = 0, sub = 0;
+  double acc_lo[3] = {0, 0, 0}, acc_hi[3] = {0, 0, 0};
+  int ns_cur = (O.node_mode && O.nz > 1) ? O.nsub[0] : 1;
+  // emission of layer m's partials: the flattening chain (a_flat = a btp,
+  // b_flat = b btp, rho_flat = rho btp^-2.275), then per layer or per node
+  auto emit = [&](int m, double qa, double qb, double qr, double bt, double brt) {
+    const double ea = qa * bt, eb = qb * bt, er = qr * brt;
     if (!O.node_mode) {
       int64_t o = out_index(m);
-      if (O.dvp) O.dvp[o] = pa;
-      if (O.dvs) O.dvs[o] = pb;
-      if (O.drho) O.drho[o] = pr;
-      continue;
+      if (O.dvp) O.dvp[o] = ea;
+      if (O.dvs) O.dvs[o] = eb;
+      if (O.drho) O.drho[o] = er;
+      return;
     }
     // node mode: sublayer j (1-based) of interval `node` with ns sublayers
-    if (m < L - 1) {
+    if (m < H) {
       const int ns = ns_cur;
       int j = sub + 1;
       double whi = (double)(2 * j - 1) / (double)(2 * ns);
       double wlo = 1.0 - whi;
-      acc_lo[0] += wlo * pb;
-      acc_lo[1] += wlo * pa;
-      acc_lo[2] += wlo * pr;
-      acc_hi[0] += whi * pb;
-      acc_hi[1] += whi * pa;
-      acc_hi[2] += whi * pr;
+      acc_lo[0] += wlo * eb;
+      acc_lo[1] += wlo * ea;
+      acc_lo[2] += wlo * er;
+      acc_hi[0] += whi * eb;
+      acc_hi[1] += whi * ea;
+      acc_hi[2] += whi * er;
       sub++;
       if (sub == ns && node < O.nz - 1) {
         int64_t o = out_index(node);
@@ -1112,11 +1079,100 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       }
     } else {
       int64_t o = out_index(node);
-      if (O.dvs) O.dvs[o] = acc_lo[0] + pb;
-      if (O.dvp) O.dvp[o] = acc_lo[1] + pa;
-      if (O.drho) O.drho[o] = acc_lo[2] + pr;
+      if (O.dvs) O.dvs[o] = acc_lo[0] + eb;
+      if (O.dvp) O.dvp[o] = acc_lo[1] + ea;
+      if (O.drho) O.drho[o] = acc_lo[2] + er;
     }
+  };
+  // records, tape entries and flattening factors one iteration ahead
+  // (unconditional loads at clamped indices; the tape holds at least one
+  // entry per thread)
+  const int mt = H >= 1 ? H - 1 : 0;  // last tape entry
+  auto load_tape = [&](int m, double* t6) {
+    const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+#pragma unroll
+    for (int i = 0; i < 6; i++) t6[i] = tp[(int64_t)i * nthr];
+  };
+  double4 ln = l0, lprev = l0;
+  double tn[6];
+  load_tape(0, tn);
+  double btn = btp[0], brn = btr[0], bt_prev = 0.0, br_prev = 0.0;
+  for (int m = 0; m < H; m++) {
+    const double4 l = ln;
+    double tc[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) tc[i] = tn[i];
+    const double bt = btn, brt = brn;
+    ln = lay[m + 1];
+    load_tape(min(m + 1, mt), tn);
+    btn = btp[m + 1];
+    brn = btr[m + 1];
+    const double kd = wk * l.x;
+    WaveFn A, Bw;
+    wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
+    wave_fn(fma(-c2, 1.0 / (l.z * l.z), 1.0), kd, Bw);
+    double t[4], u[5], ub[5], vb[5], ab[4], bb[4];
+    fdm_layer_fn(A, Bw, tc, t, u);
+    const double scl = ldexp(sc, (int)tc[5] + Eg - EF);
+    double ca = 0.0, cb = 0.0, cr = 0.0;  // this layer's partials
+    if (m >= 1) {
+      double eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
+      iface_consts(lprev, l, &eps, &dmu);
+      fdm_iface_fn(eps, dmu * ic2, u, yz, w);
+      fdm_iface_adj(eps, dmu * ic2, u, yz, g, ub, &epsb, &etab);
+      iface_chain(lprev, l, eps, dmu, ic2, epsb, etab, &ru, &bu, &rl, &bl);
+      emit(m - 1, pa, fma(bu, scl, pb), fma(ru, scl, pr), bt_prev, br_prev);
+      cb = bl;
+      cr = rl;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 5; i++) ub[i] = g[i];
+    }
+    fdm_layer_adj(A, Bw, tc, t, ub, vb, ab, bb);
+    double Xa, kda, Xb, kdb;
+    wave_adj(A, kd, ab, &Xa, &kda);
+    wave_adj(Bw, kd, bb, &Xb, &kdb);
+    ca = Xa * 2.0 * c2 / (l.y * l.y * l.y);
+    cb += Xb * 2.0 * c2 / (l.z * l.z * l.z);
+    if (m == 0) {
+      pa = ca * scl;
+      pb = fma(cb, scl, pb);
+      pr = cr * scl;
+    } else {
+      pa = ca * scl;
+      pb = cb * scl;
+      pr = cr * scl;
+    }
+    bt_prev = bt;
+    br_prev = brt;
+    lprev = l;
+#pragma unroll
+    for (int i = 0; i < 5; i++) g[i] = vb[i];
+    if (m & 1) Eg += pow2_normalize(g);
   }
+  // the half-space: the interface above it, then its start state
+  double ub[5];
+  double ha = 0.0, hb = 0.0, hr = 0.0;
+  const double sch = ldexp(sc, Eg - EF);
+  if (H >= 1) {
+    double v0[5], r1, r2, s1, s2, eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
+    fdm_start(c2, lh, v0, &r1, &r2, &s1, &s2);
+    iface_consts(lprev, lh, &eps, &dmu);
+    fdm_iface_fn(eps, dmu * ic2, v0, yz, w);
+    fdm_iface_adj(eps, dmu * ic2, v0, yz, g, ub, &epsb, &etab);
+    iface_chain(lprev, lh, eps, dmu, ic2, epsb, etab, &ru, &bu, &rl, &bl);
+    emit(H - 1, pa, fma(bu, sch, pb), fma(ru, sch, pr), bt_prev, br_prev);
+    hb = bl * sch;
+    hr = rl * sch;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 5; i++) ub[i] = g[i];
+    hb = pb;  // the surface term of layer 0, the half-space itself
+  }
+  const double rab = fma(ub[0], rbn, -ub[3]), rbb = fma(ub[0], ran, ub[4]);
+  ha = rab * start_dr(ran, sgA, -2.0 * c2 / (lh.y * lh.y * lh.y)) * sch;
+  hb = fma(rbb * start_dr(rbn, sgB, -2.0 * c2 / (lh.z * lh.z * lh.z)), sch, hb);
+  emit(H, ha, hb, hr, btn, brn);
 }
 
 __global__ void __launch_bounds__(256) k_kernels(BatchDev B, KernDev O, double* tape) {

@@ -13,7 +13,14 @@ from 0.3 b_min to 1.1 b_max):
   * a planted mistake (the interface's eta with the wrong sign) fails the
     sign check, so the comparison can fail.
 Closed form: one half-space alone gives the Rayleigh function
-(2 - c^2/b^2)^2 - 4 sqrt(1 - c^2/a^2) sqrt(1 - c^2/b^2) exactly."""
+(2 - c^2/b^2)^2 - 4 sqrt(1 - c^2/a^2) sqrt(1 - c^2/b^2) exactly.
+
+The kernel pass and the VJP differentiate the same function through its
+building blocks (forward tangent in c, adjoint sweep): at dltar4's own roots
+their kernels -(dF/dm)/(dF/dc) in each layer's flattened a, b, rho and d
+match central differences of dltar4 (relative step 1e-4, float32 model
+values as the oracle takes them) to 2e-5 of the largest kernel, including
+the half-space alone against the closed form's derivatives."""
 import ctypes as C
 import os
 
@@ -43,6 +50,25 @@ def hc():
         m = f(c, T, len(d), *[x.ctypes.data_as(fp) for x in arr], C.byref(ex))
         return np.ldexp(m, ex.value)
     return F
+
+
+@pytest.fixture(scope="module")
+def hk():
+    lib = C.CDLL(LIB)
+    f = lib.dk_host_kernels
+    f.restype = C.c_double
+    fp = C.POINTER(C.c_float)
+    f.argtypes = [C.c_double, C.c_double, C.c_int, fp, fp, fp, fp, C.POINTER(C.c_double),
+                  C.POINTER(C.c_int)]
+
+    def K(c, T, d, a, b, rho):
+        out = np.zeros(4 * len(d))
+        ex = C.c_int(0)
+        arr = [np.ascontiguousarray(x, np.float32) for x in (d, a, b, rho)]
+        f(c, T, len(d), *[x.ctypes.data_as(fp) for x in arr],
+          out.ctypes.data_as(C.POINTER(C.c_double)), C.byref(ex))
+        return out.reshape(len(d), 4)  # columns a, b, rho, d
+    return K
 
 
 def random_model(rng):
@@ -156,3 +182,46 @@ def test_planted_sign_mistake_is_caught(hc):
         assert np.array_equal(np.signbit(f1), np.signbit(good))
         caught += not np.array_equal(np.signbit(f1), np.signbit(bad))
     assert caught >= 10
+
+
+def test_kernels_match_dltar4_differences_at_its_roots(hk):
+    rng = np.random.default_rng(3)
+    F = P.dltar4
+    checked = 0
+    for _ in range(30):
+        n = int(rng.integers(1, 9))
+        b = np.sort(rng.uniform(1.0, 4.7, n)).astype(np.float32)
+        if n > 2 and rng.random() < 0.4:
+            b[1] *= 0.85
+        a = (b * rng.uniform(1.65, 1.9, n)).astype(np.float32)
+        rho = (1.7 + 0.2 * a).astype(np.float32)
+        d = rng.uniform(0.5, 8, n).astype(np.float32)
+        d[-1] = 0
+        T = float(np.exp(rng.uniform(np.log(2), np.log(60))))
+        cs = np.linspace(0.5 * float(b.min()), 0.999 * float(b.max()), 400)
+        fv = np.array([F(c, T, d, a, b, rho) for c in cs])
+        j = np.nonzero(np.signbit(fv[:-1]) != np.signbit(fv[1:]))[0]
+        if len(j) == 0:
+            continue
+        c = bisect(lambda x: F(x, T, d, a, b, rho), cs[j[0]], cs[j[0] + 1], fv[j[0]])
+        got = hk(c, T, d, a, b, rho)
+        hc = 1e-6 * c
+        dfc = (F(c + hc, T, d, a, b, rho) - F(c - hc, T, d, a, b, rho)) / (2 * hc)
+        want = np.zeros((n, 4))
+        model = [a, b, rho, d]
+        for m in range(n):
+            for p in range(4):
+                if p == 3 and m == n - 1:
+                    continue  # the half-space has no thickness
+                x = float(model[p][m])
+                up, dn = model[p].copy(), model[p].copy()
+                up[m], dn[m] = np.float32(x * (1 + 1e-4)), np.float32(x * (1 - 1e-4))
+                mp, mm = list(model), list(model)
+                mp[p], mm[p] = up, dn
+                dfm = (F(c, T, mp[3], mp[0], mp[1], mp[2]) - F(c, T, mm[3], mm[0], mm[1], mm[2])) \
+                    / (float(up[m]) - float(dn[m]))
+                want[m, p] = -dfm / dfc
+        err = np.abs(got - want).max() / np.abs(want).max()
+        assert err <= 2e-5, (n, T, c, err)
+        checked += 1
+    assert checked >= 25
