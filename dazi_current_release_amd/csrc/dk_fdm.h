@@ -177,11 +177,12 @@ DK_HDF double secular_fwd(double c, double omega, int L, const double4* mdl, int
 // a wave type's functions follow from d/dp and d/dr (p = r kd, r = sqrt|X|).
 
 // One wave type of a layer: the values of hyp_fwd with their p-derivatives
-// (dS/dr = -S y and dR/dr = R y in both branches), r, y = 1/r, rX = dr/dX.
+// dC/dp = Cp, dS/dp = Q y, dR/dp = 2 h Q r, ds/dp = sp (dS/dr = -S y and
+// dR/dr = R y in both branches), r, y = 1/r and h = +-1/2 (dr/dX = h y).
 struct WaveFn {
   double C, S, R, s;
-  double Cp, Sp, Rp, sp;
-  double r, y, rX;
+  double Cp, Q, sp;
+  double r, y, h;
 };
 
 DK_HDF void wave_fn(double X, double kd, WaveFn& w) {
@@ -199,10 +200,9 @@ DK_HDF void wave_fn(double X, double kd, WaveFn& w) {
     w.R = r * sh;         // r sinh(p) e^-p
     w.s = s;
     w.Cp = -f;
-    w.Sp = f * y;
-    w.Rp = r * f;
+    w.Q = f;
     w.sp = -s;
-    w.rX = 0.5 * y;
+    w.h = 0.5;
   } else {
     double sp, cp;
     sincos_phase(p, &sp, &cp);
@@ -211,29 +211,29 @@ DK_HDF void wave_fn(double X, double kd, WaveFn& w) {
     w.R = -r * sp;
     w.s = 1.0;
     w.Cp = -sp;
-    w.Sp = cp * y;
-    w.Rp = -r * cp;
+    w.Q = cp;  // dS/dp = cos(p)/r, dR/dp = -r cos(p)
     w.sp = 0.0;
-    w.rX = -0.5 * y;
+    w.h = -0.5;
   }
 }
 
 // Tangent of a wave type's functions for dX, dkd: (dC, dS, dR, ds).
 DK_HDF void wave_tan(const WaveFn& w, double kd, double dX, double dkd, double d[4]) {
-  const double dr = w.rX * dX;
+  const double dr = w.h * w.y * dX;
   const double dp = fma(kd, dr, w.r * dkd);
   d[0] = w.Cp * dp;
-  d[1] = fma(w.Sp, dp, -w.S * w.y * dr);
-  d[2] = fma(w.Rp, dp, w.R * w.y * dr);
+  d[1] = w.y * fma(w.Q, dp, -w.S * dr);
+  d[2] = fma(2.0 * w.h * w.Q * w.r, dp, w.R * w.y * dr);
   d[3] = w.sp * dp;
 }
 
 // Adjoint of a wave type's functions: from (Cb, Sb, Rb, sb) = dF/d(C, S, R, s)
 // to dF/dX (*Xb) and dF/dkd (*kdb).
 DK_HDF void wave_adj(const WaveFn& w, double kd, const double b[4], double* Xb, double* kdb) {
-  const double pb = fma(b[0], w.Cp, fma(b[1], w.Sp, fma(b[2], w.Rp, b[3] * w.sp)));
+  const double qb = w.Q * fma(b[1], w.y, 2.0 * w.h * w.r * b[2]);
+  const double pb = fma(b[0], w.Cp, fma(b[3], w.sp, qb));
   const double rb = fma(pb, kd, w.y * fma(b[2], w.R, -b[1] * w.S));
-  *Xb = rb * w.rX;
+  *Xb = rb * w.h * w.y;
   *kdb = pb * w.r;
 }
 

@@ -954,10 +954,11 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   auto cl = [&](int i) { return min(max(i, 0), H); };  // clamped record index
 
   // ---- sweep 1
-  double v[5], dv[5], ran, rbn, sgA, sgB;
-  const double4 lh = lay[H];
-  fdm_start(c2, lh, v, &ran, &rbn, &sgA, &sgB);
+  double v[5], dv[5];
   {
+    const double4 lh = lay[H];
+    double ran, rbn, sgA, sgB;
+    fdm_start(c2, lh, v, &ran, &rbn, &sgA, &sgB);
     const double dra = start_dr(ran, sgA, 2.0 * c / (lh.y * lh.y));
     const double drb = start_dr(rbn, sgB, 2.0 * c / (lh.z * lh.z));
     dv[0] = fma(dra, rbn, ran * drb);
@@ -981,7 +982,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   // layer records one iteration ahead (loads before the iteration's tape
   // stores: a load issued after them would wait for them too)
   double4 lcur = lay[cl(H - 1)];
-  if (H >= 1) iface_t(lcur, lh);
+  if (H >= 1) iface_t(lcur, lay[H]);
   double4 lup = lay[cl(H - 2)];
   int Ef = 0;
   for (int m = H - 1; m >= 0; m--) {
@@ -1093,7 +1094,8 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
 #pragma unroll
     for (int i = 0; i < 6; i++) t6[i] = tp[(int64_t)i * nthr];
   };
-  double4 ln = l0, lprev = l0;
+  double4 ln = l0;
+  double4 lprev = make_double4(0.0, 0.0, l0.z, l0.w);  // (rho, b) of the layer above
   double tn[6];
   load_tape(0, tn);
   double btn = btp[0], brn = btr[0], bt_prev = 0.0, br_prev = 0.0;
@@ -1145,7 +1147,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     }
     bt_prev = bt;
     br_prev = brt;
-    lprev = l;
+    lprev = make_double4(0.0, 0.0, l.z, l.w);
 #pragma unroll
     for (int i = 0; i < 5; i++) g[i] = vb[i];
     if (m & 1) Eg += pow2_normalize(g);
@@ -1154,9 +1156,11 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   double ub[5];
   double ha = 0.0, hb = 0.0, hr = 0.0;
   const double sch = ldexp(sc, Eg - EF);
+  const double4 lh = lay[H];
+  double v0[5], ran, rbn, sgA, sgB;
+  fdm_start(c2, lh, v0, &ran, &rbn, &sgA, &sgB);
   if (H >= 1) {
-    double v0[5], r1, r2, s1, s2, eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
-    fdm_start(c2, lh, v0, &r1, &r2, &s1, &s2);
+    double eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
     iface_consts(lprev, lh, &eps, &dmu);
     fdm_iface_fn(eps, dmu * ic2, v0, yz, w);
     fdm_iface_adj(eps, dmu * ic2, v0, yz, g, ub, &epsb, &etab);
@@ -1175,7 +1179,8 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   emit(H, ha, hb, hr, btn, brn);
 }
 
-__global__ void __launch_bounds__(256) k_kernels(BatchDev B, KernDev O, double* tape) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_kernels(BatchDev B, KernDev O, double* tape) {
   const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   for (int64_t s = tid; s < O.nsolves; s += nthr) {
@@ -1231,164 +1236,120 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
         gc = grad_c[s];
       }
       const bool act = (c > 0.0) && (gc != 0.0) && isfinite(gc);
-      double omega = 1.0, wv = 1.0, wv2 = 1.0;
+      double omega = 1.0;
       if (act) {
         omega = 2.0 * 3.141592653589793 / period_of(B, toff, k);
         if (omega < 1.0e-4) omega = 1.0e-4;
-        wv = omega / c;
-        wv2 = wv * wv;
       }
-      // ---- sweep 1: values (taped) and the tangent in c
-      double lane_scale = 0.0;
+      // ---- sweep 1: the state (taped) and its tangent in c, as in the
+      // kernel pass (dk_fdm.h)
+      const int H = L - 1;
+      auto cl = [&](int i) { return min(max(i, 0), H); };
+      // (lanes without a solve compute nothing below: c = 0 there)
+      const double wk = act ? omega / c : 0.0, c2 = c * c, ic2 = act ? 1.0 / c2 : 1.0,
+                   dwk = act ? -wk / c : 0.0;
+      double lane_scale = 0.0, tb = 0.0, ib0 = 0.0;
       int EF = 0;
       if (act) {
-        Dual<1> wvd = dconst<1>(wv);
-        wvd.g[0] = -wv / c;
-        Dual<1> wv2d = wvd * wvd;
-        SD<1> wvs;  // the layers' k, the only one of their inputs with a partial in c
-        wvs.v = wv;
-        wvs.g[0] = -wv / c;
-        wvs.g[1] = wvs.g[2] = 0.0;
-        const SD<1> wv2s = wvs * wvs;
-        Dual<1> e[5];
-        double4 h = lay[L - 1];
-        halfspace_e<Dual<1>>(wvd, wv2d, dconst<1>((double)h.y), dconst<1>((double)h.z),
-                             dconst<1>((double)h.w), omega, e);
-        int Ef = 0;
-        // each layer's record is loaded one iteration ahead, before the tape
-        // stores of the iteration (a load issued after them waits for them too)
-        double4 lnext = lay[L >= 2 ? L - 2 : 0];
-        for (int m = L - 1; m >= 0; m--) {
-          if (m < L - 1) {
-            const double4 l = lnext;
-            lnext = lay[m >= 1 ? m - 1 : 0];
-            double ev[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, eg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            layer_ca_visit<SD<1>>(wvs, wv2s, (double)l.y, (double)l.z, (double)l.w,
-                                  (double)l.x, omega, [&](int j, int i, const auto& v) {
-                                    ev[i] = fma(e[j].v, v.v, ev[i]);
-                                    eg[i] = fma(e[j].g[0], v.v, fma(e[j].v, v.g[0], eg[i]));
-                                  });
-#pragma unroll
-            for (int i = 0; i < 5; i++) {
-              e[i].v = ev[i];
-              e[i].g[0] = eg[i];
-            }
-          }
-          // normalise values and tangent together (exact power of two)
-          double mx = 0.0;
-#pragma unroll
-          for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(e[i].v));
-          int kk = 0;
-          if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
+        double v[5], dv[5];
+        {
+          const double4 lh = lay[H];
+          double ran, rbn, sgA, sgB;
+          fdm_start(c2, lh, v, &ran, &rbn, &sgA, &sgB);
+          const double dra = start_dr(ran, sgA, 2.0 * c / (lh.y * lh.y));
+          const double drb = start_dr(rbn, sgB, 2.0 * c / (lh.z * lh.z));
+          dv[0] = fma(dra, rbn, ran * drb);
+          dv[1] = 0.0;
+          dv[2] = 0.0;
+          dv[3] = -dra;
+          dv[4] = drb;
+        }
+        auto iface_t = [&](const double4& up, const double4& lo) {
+          double eps, dmu, yz[4], w[5], dw[5];
+          iface_consts(up, lo, &eps, &dmu);
+          const double eta = dmu * ic2;
+          fdm_iface_fn(eps, eta, v, yz, w);
+          fdm_iface_tan(eps, eta, -2.0 * eta / c, v, dv, yz, dw);
 #pragma unroll
           for (int i = 0; i < 5; i++) {
-            e[i].v = ldexp(e[i].v, -kk);
-            e[i].g[0] = ldexp(e[i].g[0], -kk);
+            v[i] = w[i];
+            dv[i] = dw[i];
           }
-          Ef += kk;
-          if (m > 0) {
-            double* tp = tape + (long)((m - 1) * 6) * nslot + slot;
+        };
+        // records one iteration ahead, before the iteration's tape stores
+        double4 lcur = lay[cl(H - 1)];
+        if (H >= 1) iface_t(lcur, lay[H]);
+        double4 lup = lay[cl(H - 2)];
+        int Ef = 0;
+        for (int m = H - 1; m >= 0; m--) {
+          const double4 l = lcur, lu = lup;
+          lcur = lup;
+          lup = lay[cl(m - 2)];
+          double* tp = tape + (long)(m * 6) * nslot + slot;
 #pragma unroll
-            for (int i = 0; i < 5; i++) tp[(long)i * nslot] = e[i].v;
-            tp[(long)5 * nslot] = (double)Ef;
+          for (int i = 0; i < 5; i++) tp[(long)i * nslot] = v[i];
+          tp[(long)5 * nslot] = (double)Ef;
+          const double kd = wk * l.x, dkd = dwk * l.x;
+          WaveFn A, Bw;
+          wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
+          wave_fn(fma(-c2, 1.0 / (l.z * l.z), 1.0), kd, Bw);
+          double dA[4], dB[4], t[4], u[5], du[5];
+          wave_tan(A, kd, -2.0 * c / (l.y * l.y), dkd, dA);
+          wave_tan(Bw, kd, -2.0 * c / (l.z * l.z), dkd, dB);
+          fdm_layer_fn(A, Bw, v, t, u);
+          fdm_layer_tan(A, Bw, dA, dB, v, dv, t, du);
+#pragma unroll
+          for (int i = 0; i < 5; i++) {
+            v[i] = u[i];
+            dv[i] = du[i];
+          }
+          if (m >= 1) iface_t(lu, l);
+          if ((m & 1) == 0 || m == H - 1) {
+            double mx = 0.0;
+#pragma unroll
+            for (int i = 0; i < 5; i++) mx = fmax(mx, fabs(v[i]));
+            int kk = 0;
+            if (mx > 1e-300 && isfinite(mx)) frexp(mx, &kk);
+#pragma unroll
+            for (int i = 0; i < 5; i++) {
+              v[i] = ldexp(v[i], -kk);
+              dv[i] = ldexp(dv[i], -kk);
+            }
+            Ef += kk;
           }
         }
         EF = Ef;
-        const double dFdc = e[0].g[0];
+        const double4 l0 = lay[0];
+        ib0 = 1.0 / (l0.z * l0.z);
+        const double tt = fma(-c2, ib0, 2.0);
+        tb = -fma(2.0 * tt, v[2], 4.0 * v[1]);
+        const double dFdc =
+            -fma(tt, fma(tt, dv[2], 4.0 * dv[1]), 4.0 * dv[0]) + tb * (-2.0 * c * ib0);
         lane_scale = (dFdc != 0.0) ? -gc / dFdc : 0.0;
       }
-      // ---- sweep 2: adjoint, per-layer contributions reduced over the lanes.
-      // Partials in the flattened (a, b, rho, d) of each layer; the thickness
-      // chain goes through the layer radii r0 (top) and r1 (bottom) of the
-      // flattening (d = ar ln(r0/r1), tmp = 2 ar/(r0 + r1) scales a, b and
-      // rho^-2.275): R0, R1 = dF/dr0, dF/dr1, and a thickness change moves r1
-      // of its layer and both radii of every deeper layer.
-      double g[5] = {1.0, 0.0, 0.0, 0.0, 0.0};
-      int Eg = 0;
-      double Z = 0.0, prevR1 = 0.0;
+      // ---- sweep 2: the adjoint; per-layer partials in the flattened
+      // (a, b, rho, d), scaled by -grad_c/(dF/dc), are reduced over the lanes.
+      // The thickness chain goes through the layer radii r0 (top) and r1
+      // (bottom) of the flattening (d = ar ln(r0/r1), tmp = 2 ar/(r0 + r1)
+      // scales a, b and rho^-2.275): R0, R1 = dF/dr0, dF/dr1, and a thickness
+      // change moves r1 of its layer and both radii of every deeper layer.
+      // Layer m's partials are complete one step later (the interface above
+      // layer m+1 depends on them), and are emitted then.
       const double ar = 6370.0;
-      // the next layer's thickness, record, tape entry and flattening factors
-      // are loaded one iteration ahead, so their latency overlaps this layer
       auto thk_at = [&](int m) -> double {
-        return (m == L - 1) ? 1.0 : (thk64 ? thk64[loff + m] : (double)thk32[loff + m]);
+        return (m == H) ? 1.0 : (thk64 ? thk64[loff + m] : (double)thk32[loff + m]);
       };
-      // lanes without a solve read lane 0's slot (the same cache lines)
-      const long tslot = act ? slot : slot - lane;
-      auto load_tape = [&](int m, double* t) {
-        const double* tp = tape + (long)(m * 6) * nslot + tslot;
-#pragma unroll
-        for (int i = 0; i < 6; i++) t[i] = tp[(long)i * nslot];
-      };
-      // (unconditional loads at clamped indices, also on lanes without a
-      // solve: no branch added to the iteration; the tape holds at least one
-      // entry per slot)
-      const int mt = L >= 2 ? L - 2 : 0;  // last tape entry
-      double thn = thk_at(0);
-      double4 ln = lay[0];
-      double tn[6];
-      load_tape(0, tn);
-      double btn = btp[0], brn = btr[0];
-      for (int m = 0; m < L; m++) {
-        const double thk_m = thn;
-        const double4 l = ln;
-        double tc[6];  // this layer's tape entry (read only below the half-space)
-#pragma unroll
-        for (int i = 0; i < 6; i++) tc[i] = tn[i];
-        const double bt = btn, btr_m = brn;
-        const int mn = min(m + 1, L - 1);
-        thn = thk_at(mn);
-        ln = lay[mn];
-        load_tape(min(m + 1, mt), tn);
-        btn = btp[mn];
-        brn = btr[mn];
-        const double r0 = ar - Z, r1 = r0 - thk_m;
-        Z += thk_m;
-        double part[4] = {0.0, 0.0, 0.0, 0.0};
+      double prevR1 = 0.0;
+      auto emit = [&](int m, const double* part_in, const double4& l, double bt, double btr_m,
+                      double r0, double r1) {
+        double part[4] = {part_in[0], part_in[1], part_in[2], part_in[3]};
         double R0 = 0.0, R1 = 0.0;
         if (act) {
-          if (m < L - 1) {
-            const double* et = tc;
-            const int Ee = (int)tc[5];
-            // the rho partial from the entries' rho powers
-            double acc[4] = {0.0, 0.0, 0.0, 0.0}, gn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-            // partials in (a, b, d) only where an entry depends on them (SD masks)
-            layer_ca_visit<double>(
-                wv, wv2, sd_var<1>((double)l.y), sd_var<2>((double)l.z), (double)l.w,
-                sd_var<4>((double)l.x), omega, [&](int j, int i, const auto& v) {
-                  const double w = et[j] * g[i];
-                  acc[0] = fma(w, v.g[0], acc[0]);
-                  acc[1] = fma(w, v.g[1], acc[1]);
-                  acc[2] = fma(w * (double)rho_pow(j, i), v.v, acc[2]);
-                  acc[3] = fma(w, v.g[2], acc[3]);
-                  gn[j] = fma(v.v, g[i], gn[j]);
-                });
-            acc[2] /= (double)l.w;
-#pragma unroll
-            for (int p = 0; p < 4; p++) part[p] = ldexp(acc[p], Ee + Eg - EF) * lane_scale;
-#pragma unroll
-            for (int j = 0; j < 5; j++) g[j] = gn[j];
-            Eg += pow2_normalize(g);
-          } else {
-            Dual<2> eh[5];
-            halfspace_e<Dual<2>>(dconst<2>(wv), dconst<2>(wv2), dvar<2>((double)l.y, 0),
-                                 dvar<2>((double)l.z, 1), dconst<2>((double)l.w), omega, eh);
-            double acc[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-            for (int i = 0; i < 5; i++) {
-              acc[0] = fma(eh[i].g[0], g[i], acc[0]);
-              acc[1] = fma(eh[i].g[1], g[i], acc[1]);
-              acc[2] = fma((double)half_rho_pow(i) * eh[i].v, g[i], acc[2]);
-            }
-            acc[2] /= (double)l.w;
-#pragma unroll
-            for (int p = 0; p < 3; p++) part[p] = ldexp(acc[p], Eg - EF) * lane_scale;
-          }
-          // part = grad_c * dc/d(flattened a, b, rho, d)
-          const double dtmp = part[0] * ((double)l.y / bt) + part[1] * ((double)l.z / bt) -
-                              2.275 * part[2] * ((double)l.w / bt);
+          const double dtmp = part[0] * (l.y / bt) + part[1] * (l.z / bt) -
+                              2.275 * part[2] * (l.w / bt);
           const double ttm = -dtmp * bt / (r0 + r1);
-          R0 = ttm + ((m < L - 1) ? part[3] * (ar / r0) : 0.0);
-          R1 = ttm - ((m < L - 1) ? part[3] * (ar / r1) : 0.0);
+          R0 = ttm + ((m < H) ? part[3] * (ar / r0) : 0.0);
+          R1 = ttm - ((m < H) ? part[3] * (ar / r1) : 0.0);
           part[0] *= bt;
           part[1] *= bt;
           part[2] *= btr_m;
@@ -1419,7 +1380,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
               else
                 gthk[o - 1] += w;
             }
-            if (m == L - 1) {
+            if (m == H) {
               if (k0 == 0)
                 gthk[o] = s1;
               else
@@ -1428,6 +1389,121 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
             prevR1 = s1;
           }
         }
+      };
+      // lanes without a solve read lane 0's slot (the same cache lines)
+      const long tslot = act ? slot : slot - lane;
+      auto load_tape = [&](int m, double* t6) {
+        const double* tp = tape + (long)(m * 6) * nslot + tslot;
+#pragma unroll
+        for (int i = 0; i < 6; i++) t6[i] = tp[(long)i * nslot];
+      };
+      const int mt = H >= 1 ? H - 1 : 0;  // last tape entry
+      double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      int Eg = 0;
+      double pend[4] = {0.0, 0.0, 0.0, 0.0};  // the previous layer's partials
+      if (act) {
+        const double tt = fma(-c2, ib0, 2.0);
+        g[0] = -4.0;
+        g[1] = -4.0 * tt;
+        g[2] = -tt * tt;
+        pend[1] = tb * 2.0 * c2 * ib0 / lay[0].z * lane_scale;
+      }
+      double4 ln = lay[0];
+      double4 lprev = ln;
+      double tn[6];
+      load_tape(0, tn);
+      double thn = thk_at(0), btn = btp[0], brn = btr[0];
+      double bt_prev = 1.0, br_prev = 1.0, r0_prev = ar, r1_prev = ar, Z = 0.0;
+      for (int m = 0; m < H; m++) {
+        const double4 l = ln;
+        double tc[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) tc[i] = tn[i];
+        const double bt = btn, brt = brn, thk_m = thn;
+        ln = lay[m + 1];
+        load_tape(min(m + 1, mt), tn);
+        btn = btp[m + 1];
+        brn = btr[m + 1];
+        thn = thk_at(m + 1);
+        const double r0 = ar - Z, r1 = r0 - thk_m;
+        Z += thk_m;
+        double cur[4] = {0.0, 0.0, 0.0, 0.0};
+        if (act) {
+          const double kd = wk * l.x;
+          WaveFn A, Bw;
+          wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
+          wave_fn(fma(-c2, 1.0 / (l.z * l.z), 1.0), kd, Bw);
+          double t[4], u[5], ub[5], vb[5], ab[4], bb[4];
+          fdm_layer_fn(A, Bw, tc, t, u);
+          const double scl = ldexp(lane_scale, (int)tc[5] + Eg - EF);
+          if (m >= 1) {
+            double eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
+            iface_consts(lprev, l, &eps, &dmu);
+            fdm_iface_fn(eps, dmu * ic2, u, yz, w);
+            fdm_iface_adj(eps, dmu * ic2, u, yz, g, ub, &epsb, &etab);
+            iface_chain(lprev, l, eps, dmu, ic2, epsb, etab, &ru, &bu, &rl, &bl);
+            pend[1] = fma(bu, scl, pend[1]);
+            pend[2] = fma(ru, scl, pend[2]);
+            cur[1] = bl;
+            cur[2] = rl;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 5; i++) ub[i] = g[i];
+          }
+          fdm_layer_adj(A, Bw, tc, t, ub, vb, ab, bb);
+          double Xa, kda, Xb, kdb;
+          wave_adj(A, kd, ab, &Xa, &kda);
+          wave_adj(Bw, kd, bb, &Xb, &kdb);
+          cur[0] = Xa * 2.0 * c2 / (l.y * l.y * l.y) * scl;
+          cur[1] = (cur[1] + Xb * 2.0 * c2 / (l.z * l.z * l.z)) * scl;
+          cur[2] *= scl;
+          cur[3] = (kda + kdb) * wk * scl;
+#pragma unroll
+          for (int i = 0; i < 5; i++) g[i] = vb[i];
+          if (m & 1) Eg += pow2_normalize(g);
+        }
+        if (m >= 1) emit(m - 1, pend, lprev, bt_prev, br_prev, r0_prev, r1_prev);
+        if (m == 0) {
+          cur[1] += pend[1];  // the surface term of layer 0
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) pend[i] = cur[i];
+        lprev = l;
+        bt_prev = bt;
+        br_prev = brt;
+        r0_prev = r0;
+        r1_prev = r1;
+      }
+      // the half-space: the interface above it, then its start state
+      {
+        const double4 lh = lay[H];
+        const double r0 = ar - Z, r1 = r0 - thn;
+        double hp[4] = {0.0, 0.0, 0.0, 0.0};
+        if (act) {
+          double ub[5], v0[5], ran, rbn, sgA, sgB;
+          fdm_start(c2, lh, v0, &ran, &rbn, &sgA, &sgB);
+          const double sch = ldexp(lane_scale, Eg - EF);
+          if (H >= 1) {
+            double eps, dmu, yz[4], w[5], epsb, etab, ru, bu, rl, bl;
+            iface_consts(lprev, lh, &eps, &dmu);
+            fdm_iface_fn(eps, dmu * ic2, v0, yz, w);
+            fdm_iface_adj(eps, dmu * ic2, v0, yz, g, ub, &epsb, &etab);
+            iface_chain(lprev, lh, eps, dmu, ic2, epsb, etab, &ru, &bu, &rl, &bl);
+            pend[1] = fma(bu, sch, pend[1]);
+            pend[2] = fma(ru, sch, pend[2]);
+            hp[1] = bl * sch;
+            hp[2] = rl * sch;
+          } else {
+#pragma unroll
+            for (int i = 0; i < 5; i++) ub[i] = g[i];
+            hp[1] = pend[1];  // the surface term: layer 0 is the half-space
+          }
+          const double rab = fma(ub[0], rbn, -ub[3]), rbb = fma(ub[0], ran, ub[4]);
+          hp[0] = rab * start_dr(ran, sgA, -2.0 * c2 / (lh.y * lh.y * lh.y)) * sch;
+          hp[1] = fma(rbb * start_dr(rbn, sgB, -2.0 * c2 / (lh.z * lh.z * lh.z)), sch, hp[1]);
+        }
+        if (H >= 1) emit(H - 1, pend, lprev, bt_prev, br_prev, r0_prev, r1_prev);
+        emit(H, hp, lh, btn, brn, r0, r1);
       }
     }
     // dc/dthk[i] = -sum_{j >= i} W[j]; the half-space thickness does not enter
