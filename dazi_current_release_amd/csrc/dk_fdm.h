@@ -38,11 +38,24 @@
 #include <math.h>
 
 #include "dk_fastmath.h"
-#include "dk_physics.h"  // pow2_normalize
 
 #define DK_HDF __host__ __device__ __forceinline__
 
 namespace dk {
+
+// Exact power-of-two normalisation of a 5-entry state: scales e by 2^-k with
+// k the exponent of max|e| and returns k (0 for a zero or non-finite state).
+DK_HD int pow2_normalize(double e[5]) {
+  double m = fabs(e[0]);
+#pragma unroll
+  for (int i = 1; i < 5; i++) m = fmax(m, fabs(e[i]));
+  if (!(m > 1e-300) || !isfinite(m)) return 0;
+  int k;
+  frexp(m, &k);
+#pragma unroll
+  for (int i = 0; i < 5; i++) e[i] = ldexp(e[i], -k);
+  return k;
+}
 
 DK_HDF void layer_rec(const double4& l, const double4& up, bool top, double4* g,
                       double4* r) {

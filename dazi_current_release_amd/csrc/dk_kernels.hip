@@ -9,13 +9,19 @@
 //                  one column and walk its periods in order, reproducing the
 //                  reference's scan grid, direction, floor and failure rules
 //                  (surfdisp96.f:223-348, getsol :384-476); each scan step
-//                  evaluates S trial velocities at once and a __ballot finds
-//                  the first sign change; the bracket is refined by
-//                  S-point multisection and a final secant step.
+//                  evaluates S trial velocities at once (the secular
+//                  function of dk_fdm.h) and a __ballot finds the first sign
+//                  change; the root is verified in a window around an
+//                  inverse-cubic estimate, else refined by S-point
+//                  multisection, and placed by a final secant step.
 //   k_kernels      per (column, period) implicit differentiation of the
-//                  secular function at the root, dc/dm = -(dF/dm)/(dF/dc),
-//                  via a forward tape and an adjoint sweep with dual numbers;
-//                  maps refined-layer partials to the node or layer outputs.
+//                  secular function at the root, dc/dm = -(dF/dm)/(dF/dc):
+//                  a bottom-up sweep with the tangent in c and a tape, and a
+//                  hand-written adjoint sweep of the potential-basis
+//                  recursion (dk_fdm.h); maps refined-layer partials to the
+//                  node or layer outputs.
+//   k_vjp          the same sweeps per (column, period), contracted with
+//                  grad_c over a column's periods into layer gradients.
 //
 // All secular-function arithmetic is FP64 on the vector ALUs; nothing here is
 // a dense contraction, so MFMA is not used.
@@ -32,7 +38,6 @@
 
 #include "dk_brocher.h"
 #include "dk_fdm.h"
-#include "dk_physics.h"
 #include "dk_geom.h"
 #include "dk_internal.h"
 
