@@ -16,14 +16,14 @@
 //    S = [[(13), (12)], [(12), (24)]] to N S N^T with
 //    N = [[a, a - 1], [eta, 1 + eta]], a = eps - eta, eps = rho_up / rho_lo,
 //    eta = 2 (mu_up - mu_lo) / (rho_lo c^2), and scales (14), (23) by eps
-//    (18 operations);
+//    (13 operations in the rank-one form of fdm_iface);
 //  * reciprocity (the propagators are symplectic) keeps (34) = -(12), so five
 //    entries carry the state.
 // The entries are scaled by powers of k so that a layer needs only
 // X_n = 1 - c^2 / v^2 and k d, and an evanescent layer's functions by
 // e^-(p+q) as var does.  The value is dltar4's times a positive factor (the
 // Jacobians of the basis changes), so roots, signs and the reference's scan
-// decisions are the same; a layer costs about 35 operations besides its two
+// decisions are the same; a layer costs about 30 operations besides its two
 // wave functions, against about 95 for building and applying the Dunkin
 // matrix.  (This is the factorisation of the fast delta matrix, Buchen and
 // Ben-Hador, Geophys. J. Int. 124 (1996) 869-887.)
@@ -63,10 +63,11 @@ DK_HDF void layer_rec(const double4& l, const double4& up, bool top, double4* g,
   const double eps = top ? 1.0 : up.w * irho;
   const double dmu = top ? 0.0 : 2.0 * (up.w * up.z * up.z - l.w * l.z * l.z) * irho;
   *g = make_double4(l.x, 1.0 / (l.y * l.y), 1.0 / (l.z * l.z), eps);
-  *r = make_double4(dmu, 0.0, 0.0, 0.0);
+  *r = make_double4(dmu, top ? 0.0 : up.w * irho - 1.0, 0.0, 0.0);
 }
 
-// One wave type of a layer for X = k^2 - kx^2: C = cosh(p), S = sinh(p)/sqrt(X),
+// One wave type of a layer for X = k^2 - kx^2 (the root search's evaluation,
+// with the shorter exp and sincos): C = cosh(p), S = sinh(p)/sqrt(X),
 // R = sqrt(X) sinh(p), p = sqrt(X) d, all scaled by exp(-p) (X > 0), or
 // cos, sin/sqrt(-X), -sqrt(-X) sin (X < 0).  Returns the scale, *ex = p or 0.
 // (exp(-2p) is kept where the reference's var drops it for p >= 16: a change
@@ -81,7 +82,7 @@ DK_HDF double hyp_fwd(double X, double d, double* C, double* S, double* R, doubl
   const double ra = ax * y;            // sqrt|X|
   const double p = ra * d;
   if (X > 0.0) {
-    const double s = exp_neg(p);
+    const double s = exp_neg_rs(p);
     const double fac = s * s;
     const double sh = 0.5 - 0.5 * fac;
     *C = 0.5 + 0.5 * fac;
@@ -91,7 +92,7 @@ DK_HDF double hyp_fwd(double X, double d, double* C, double* S, double* R, doubl
     return s;
   }
   double sp, cp;
-  sincos_phase(p, &sp, &cp);
+  sincos_phase_rs(p, &sp, &cp);
   *C = cp;
   *S = sp * y;
   *R = -ra * sp;
@@ -119,16 +120,18 @@ DK_HDF void fdm_layer(double c2, double k, const double4& g, double e[5]) {
   e[1] *= sa * sb;  // (12) passes the layer unchanged: only the layer's scale
 }
 
-// The interface above a layer: eps, dmu of its record, ic2 = 1/c^2.
-DK_HDF void fdm_iface(double ic2, double eps, double dmu, double e[5]) {
-  const double eta = dmu * ic2;
-  const double a = eps - eta, am1 = a - 1.0, b = 1.0 + eta;
-  const double s11 = e[0], s12 = e[1], s22 = e[2];
-  const double y1 = fma(a, s11, am1 * s12), y2 = fma(a, s12, am1 * s22);
-  const double z1 = fma(eta, s11, b * s12), z2 = fma(eta, s12, b * s22);
-  e[0] = fma(a, y1, am1 * y2);
-  e[1] = fma(eta, y1, b * y2);
-  e[2] = fma(eta, z1, b * z2);
+// The interface above a layer: eps, eps - 1 and dmu of its record, ic2 = 1/c^2.
+// N = [[a, a - 1], [eta, 1 + eta]] is I + n (1, 1) with n = (a - 1, eta), so
+// N S N^T = S + n q^T + q n^T + (q1 + q2) n n^T with q = S (1, 1)^T.
+DK_HDF void fdm_iface(double ic2, double eps, double epsm1, double dmu, double e[5]) {
+  const double n2 = dmu * ic2;  // eta
+  const double n1 = epsm1 - n2;  // a - 1
+  const double q1 = e[0] + e[1], q2 = e[1] + e[2], sg = q1 + q2;
+  const double t1 = fma(sg, n1, q1), t2 = fma(sg, n2, q2);
+  const double s12 = fma(n1, t2, fma(n2, q1, e[1]));
+  e[0] = fma(n1, q1 + t1, e[0]);
+  e[2] = fma(n2, q2 + t2, e[2]);
+  e[1] = s12;
   e[3] *= eps;
   e[4] *= eps;
 }
@@ -143,7 +146,10 @@ DK_HDF double secular_fwd(double c, double omega, int L, const double4* mdl, int
   const double ran = sqrt(fabs(fma(-c2, gH.y, 1.0)));
   const double rbn = sqrt(fabs(fma(-c2, gH.z, 1.0)));
   double e[5] = {ran * rbn, 0.0, -1.0, -ran, rbn};
-  fdm_iface(ic2, gH.w, mdl[2 * (L - 1) + 1].x, e);
+  {
+    const double4 rH = mdl[2 * (L - 1) + 1];
+    fdm_iface(ic2, gH.w, rH.y, rH.x, e);
+  }
   int E = 0;
   // two layers per step; one normalisation per two steps keeps the state far
   // inside the double range.  The next pair's records are read from LDS one
@@ -153,21 +159,20 @@ DK_HDF double secular_fwd(double c, double omega, int L, const double4* mdl, int
   auto cl = [&](int i) { return i < 0 ? 0 : (i > L - 1 ? L - 1 : i); };  // reads stay in range
   int ia = cl(m), ib = cl(m - 1);
   double4 ga = mdl[2 * ia], gb = mdl[2 * ib];
-  double da = mdl[2 * ia + 1].x, db = mdl[2 * ib + 1].x;
+  double4 ra = mdl[2 * ia + 1], rb = mdl[2 * ib + 1];
   bool norm = false;
   for (; m >= 1; m -= 2) {
-    const double4 g0 = ga, g1 = gb;
-    const double d0 = da, d1 = db;
+    const double4 g0 = ga, g1 = gb, r0 = ra, r1 = rb;
     ia = cl(m - 2);
     ib = cl(m - 3);
     ga = mdl[2 * ia];
     gb = mdl[2 * ib];
-    da = mdl[2 * ia + 1].x;
-    db = mdl[2 * ib + 1].x;
+    ra = mdl[2 * ia + 1];
+    rb = mdl[2 * ib + 1];
     fdm_layer(c2, k, g0, e);
-    fdm_iface(ic2, g0.w, d0, e);
+    fdm_iface(ic2, g0.w, r0.y, r0.x, e);
     fdm_layer(c2, k, g1, e);
-    fdm_iface(ic2, g1.w, d1, e);  // the identity above the top layer
+    fdm_iface(ic2, g1.w, r1.y, r1.x, e);  // the identity above the top layer
     if (norm) E += pow2_normalize(e);
     norm = !norm;
   }
