@@ -481,8 +481,12 @@ __device__ __forceinline__ double inv_interp4(const double* X, const double* Y, 
 }
 
 // Lanes of a group that check the window around the interpolated root in a
-// verification round; the other lanes scan the next period.
-constexpr int DK_NW = 4;
+// verification round (cstar -/+ 0.49 tol: the two points bracket the root in
+// less than tol when the estimate is within 0.49 tol of it, which the inverse
+// cubic through grid points 0.005 km/s apart reaches with orders of magnitude
+// to spare; a miss is retried once around a Newton step); the other lanes
+// scan the next period.
+constexpr int DK_NW = 2;
 
 // (at most 256 VGPRs: two waves per SIMD on batches with enough columns)
 template <int S>
@@ -642,9 +646,7 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
     const double cstar_r = cstar;
     auto trial = [&](int j) -> double {
       if (ver_r && j < DK_NW) {
-        // window cstar -/+ {0.95, 0.05} tol: every sub-interval is below tol
-        const double off = (j == 0) ? -0.95 : (j == 1) ? -0.05 : (j == 2) ? 0.05 : 0.95;
-        return cstar_r + off * tol;
+        return cstar_r + (j == 0 ? -0.49 : 0.49) * tol;
       }
       return xb + (double)(j + jo) * xs;
     };
