@@ -98,14 +98,18 @@ DK_HD void sincos_phase(double p, double* sp, double* cp) {
 }
 
 // Shorter forms for the root search, which needs F's sign and its root to far
-// better than the 2e-6 km/s refinement bracket but not to full precision: the
-// degree-9 Taylor polynomial of e^r (remainder below 7e-12 relative on
-// |r| <= ln2/2) and degrees 11 and 12 for sin and cos (below 7e-12 and 4e-13
-// on |r| <= pi/4), with the same reductions as exp_neg and sincos_phase.  A
-// relative error e in every layer's wave functions moves a root by about
-// e times the c-scale of F's variation (below 1e-11 km/s).
+// better than the 2e-6 km/s refinement bracket but not to full precision.
+//  * exp_neg_rs: the degree-9 Taylor polynomial of e^r (remainder below
+//    7e-12 relative on |r| <= ln2/2), same reduction as exp_neg; no clamp of
+//    p: a layer's phase is bounded by omega d / c (a few thousand at most),
+//    where ldexp already gives 0.
+//  * sincos_phase_rs: p = n pi + r, |r| <= pi/2 (the three-part pi of fdlibm,
+//    doubled), Taylor polynomials to degrees 15 and 16 (remainders below 7e-12
+//    and 6e-13), and (-1)^n as a flip of both sign bits: fewer instructions
+//    than the quadrant selects of a reduction by pi/2.
+// A relative error e in every layer's wave functions moves a root by about e
+// times the c-scale of F's variation (below 1e-11 km/s).
 DK_HD double exp_neg_rs(double p) {
-  p = fmin(p, 745.0);
   const double n = rint(p * 1.4426950408889634074);
   double r = fma(n, 6.93147180369123816490e-01, -p);
   r = fma(n, 1.90821492927058770002e-10, r);
@@ -123,28 +127,32 @@ DK_HD double exp_neg_rs(double p) {
 }
 
 DK_HD void sincos_phase_rs(double p, double* sp, double* cp) {
-  const double n = rint(p * 6.36619772367581382433e-01);
-  double r = fma(n, -1.57079632673412561417e+00, p);
-  r = fma(n, -6.07710050650619224932e-11, r);
-  r = fma(n, -2.02226624879595063154e-21, r);
+  const double n = rint(p * 3.18309886183790671538e-01);  // 1/pi
+  double r = fma(n, -3.14159265346825122834e+00, p);
+  r = fma(n, -1.21542010130123844986e-10, r);
+  r = fma(n, -4.04453249759190126308e-21, r);
   const double r2 = r * r;
-  double ps = 2.5052108385441718775e-08;               // 1/11!
+  double ps = 7.6471637318198164759e-13;               // 1/15!
+  ps = fma3(ps, r2, -1.6059043836821614599e-10);        // -1/13!
+  ps = fma3(ps, r2, 2.5052108385441718775e-08);         // 1/11!
   ps = fma3(ps, r2, -2.7557319223985892511e-06);        // -1/9!
   ps = fma3(ps, r2, 1.9841269841269841253e-04);         // 1/7!
   ps = fma3(ps, r2, -8.3333333333333332177e-03);        // -1/5!
   ps = fma3(ps, r2, 1.6666666666666665741e-01);         // 1/3!, subtracted below
   const double sn = fma(-r * r2, ps, r);
-  double pc = 2.0876756987868098979e-09;               // 1/12!
+  double pc = 4.7794773323873852974e-14;               // 1/16!
+  pc = fma3(pc, r2, -1.1470745597729724714e-11);        // -1/14!
+  pc = fma3(pc, r2, 2.0876756987868098979e-09);         // 1/12!
   pc = fma3(pc, r2, -2.7557319223985890653e-07);        // -1/10!
   pc = fma3(pc, r2, 2.4801587301587301566e-05);         // 1/8!
   pc = fma3(pc, r2, -1.3888888888888889419e-03);        // -1/6!
   pc = fma3(pc, r2, 4.1666666666666664354e-02);         // 1/4!
   pc = fma(pc, r2, -0.5);
   const double cs = fma(pc, r2, 1.0);
-  const int q = (int)n & 3;
-  const double s0 = (q & 1) ? cs : sn, c0 = (q & 1) ? sn : cs;
-  *sp = (q & 2) ? -s0 : s0;
-  *cp = ((q + 1) & 2) ? -c0 : c0;
+  // (-1)^n: n's parity into both sign bits
+  const unsigned long long flip = (unsigned long long)(((unsigned)(int)n & 1u) << 31) << 32;
+  *sp = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, sn) ^ flip);
+  *cp = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, cs) ^ flip);
 }
 
 // 1/sqrt(x) for x > 0: the hardware estimate and one Newton step.

@@ -74,10 +74,10 @@ DK_HDF void layer_rec(const double4& l, const double4& up, bool top, double4* g,
 // of F below 1e-13 relative.)
 DK_HDF double hyp_fwd(double X, double d, double* C, double* S, double* R, double* ex) {
   // X = 0 (the trial velocity equal to the layer's) is the limit of the
-  // oscillatory form: taken at X = -1e-300 it gives C = 1, S = d, R = 0 to
-  // rounding, without a branch of its own
-  if (X == 0.0) X = -1.0e-300;
-  const double ax = fabs(X);
+  // oscillatory form: with |X| + 1e-300 (which changes no X != 0: |X| is at
+  // least the rounding granularity of 1 - c^2/v^2) it gives C = 1, S = d,
+  // R = 0 to rounding, without a select of its own
+  const double ax = fabs(X) + 1.0e-300;
   const double y = rsqrt_nr(ax);       // 1/sqrt|X|
   const double ra = ax * y;            // sqrt|X|
   const double p = ra * d;
