@@ -618,7 +618,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   int kern_nthr = 0;
   if (kern) {
     kern_nthr = kernel_threads(P, nsolves);
-    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * kern_nthr * 8, &tape))
+    if (int rc = pool_get(P, S_TAPE, dk::kern_tape_bytes(L, kern_nthr), &tape))
       return rc;
   }
   // the TI model prep needs only the inputs and the shared layer records: it
@@ -692,7 +692,7 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     O.dvs = svs;
     O.dvp = svp;
     O.drho = srho;
-    HIPCHK(dk::launch_kernels(B, O, (double*)tape, kern_nthr, st));
+    HIPCHK(dk::launch_kernels(B, O, (double*)tape, dk::kern_checkpoints(L), kern_nthr, st));
   }
   rec(P, 3, st);
   if (ti) {
@@ -1496,9 +1496,9 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     int nthr = kernel_threads(P, b->nsolves);
     int L = b->max_layers;
     void* tape;
-    if (int rc = pool_get(P, S_TAPE, (size_t)6 * (L - 1 > 0 ? L - 1 : 1) * nthr * 8, &tape))
+    if (int rc = pool_get(P, S_TAPE, dk::kern_tape_bytes(L, nthr), &tape))
       return rc;
-    HIPCHK(dk::launch_kernels(B, O, (double*)tape, nthr, st));
+    HIPCHK(dk::launch_kernels(B, O, (double*)tape, dk::kern_checkpoints(L), nthr, st));
   }
   rec(P, 3, st);
   rec(P, 4, st);  // no TI phase

@@ -71,6 +71,21 @@ struct TiLayerDev {
 constexpr int TI_SCRATCH = 9;
 constexpr int TI_SCRATCH_CX = 14;
 
+// Kernel pass: layers per checkpoint segment.  The tangent sweep keeps the
+// state entering every KERN_SEG-th layer (5 doubles and an int exponent per
+// checkpoint and thread); the adjoint sweep recomputes a segment's states into LDS
+// (KERN_SEG x (5 doubles + 1 int) per thread: 78,848 B for 256 threads, so
+// two 256-thread blocks share a CU's 160 KiB).
+constexpr int KERN_SEG = 7;
+constexpr int KERN_BLOCK = 256;
+constexpr int KERN_LDS = KERN_SEG * KERN_BLOCK * (5 * 8 + 4);
+// checkpoints per thread for models of at most L layers (half-space included),
+// and the checkpoint buffer's bytes for nthr threads
+inline int kern_checkpoints(int L) { return L > 1 ? (L - 1 + KERN_SEG - 1) / KERN_SEG : 1; }
+inline size_t kern_tape_bytes(int L, int64_t nthr) {
+  return (size_t)kern_checkpoints(L) * (size_t)nthr * (5 * 8 + 4);
+}
+
 // k_prep_ti (node model and the shared records of launch_prep_geom) also
 // clears queue[0]; launch_ti runs k_ti, which appends the solves
 // with a complex-conjugate eigenvalue pair to queue[1..], then k_ti_cx over
@@ -108,8 +123,8 @@ hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, cons
 hipError_t launch_forward(const BatchDev& B, int S, int col_begin, int col_end, int cols_per_wg,
                           int persistent, int grid, int lds_stride, hipStream_t st);
 int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
-hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
-                          hipStream_t st);
+hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int nq,
+                          int grid_threads, hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 // G-row assembly (CalSurfGAniso_Joint.f90:750-797), device pointers; see
 // dazi_dk_grows_dev.  row_nnz, row_off: per ray; nar: one int64.

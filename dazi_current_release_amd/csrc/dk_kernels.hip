@@ -920,18 +920,29 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
 // potential-basis recursion of dk_fdm.h (the value is dltar4's times a
 // positive factor, which drops out at F = 0).
 //   sweep 1, bottom-up: the state and its tangent in c; the state entering
-//            each layer is taped with its exponent;
-//   sweep 2, top-down: the adjoint dF/d(state); at layer m the taped state is
-//            propagated through the layer again (its wave functions are needed
-//            for the adjoint anyway), the interface above it and the layer are
-//            differentiated, and the partials in the flattened (a, b, rho) are
-//            chained to the unflattened layer.  The interface above layer m
-//            also depends on layer m-1's (rho, b), so each layer's partials are
-//            complete, and emitted, one step later.
-// Tape layout: tape[(m*6 + i) * nthr + tid], i<5 state entries, i==5 exponent.
+//            the bottom layer of every segment of KERN_SEG layers (counted
+//            from the top) is kept as a checkpoint with its exponent;
+//   sweep 2, top-down, a segment at a time: the segment's states are
+//            recomputed from its checkpoint (value steps only) into LDS, then
+//            the adjoint dF/d(state) runs through it.  At layer m the state
+//            is propagated through the layer again (its wave functions are
+//            needed for the adjoint anyway), the interface above it and the
+//            layer are differentiated, and the partials in the flattened
+//            (a, b, rho) are chained to the unflattened layer.  The interface
+//            above layer m also depends on layer m-1's (rho, b), so each
+//            layer's partials are complete, and emitted, one step later.
+// Checkpoint layout for segment q: state entry i at tape[(q*5 + i) * nthr + tid],
+// its exponent at ((int*)(tape + nq*5*nthr))[q * nthr + tid] (nq checkpoints
+// per thread).  LDS segment buffer of the thread (blockDim.x = bd
+// threads): state entry (j, i) at seg[(j*5 + i) * bd], exponent j at
+// sege[j * bd].  States are stored with an exponent of their own, so the
+// recomputation may normalise them on another schedule than sweep 1: the
+// represented values are the same (exact powers of two).
 __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, int col, int k,
                                            int64_t s, double* __restrict__ tape, int64_t nthr,
-                                           int tid) {
+                                           int tid, double* __restrict__ seg,
+                                           int* __restrict__ sege, int bd, int nq) {
+  int* __restrict__ tape_e = (int*)(tape + (int64_t)nq * 5 * nthr);
   int L;
   int64_t loff, toff;
   int K;
@@ -996,10 +1007,13 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     const double4 l = lcur, lu = lup;
     lcur = lup;
     lup = lay[cl(m - 2)];
-    double* tp = tape + (int64_t)(m * 6) * nthr + tid;
+    if (m == H - 1 || m % KERN_SEG == KERN_SEG - 1) {  // the bottom of segment m / KERN_SEG
+      const int q = m / KERN_SEG;
+      double* tp = tape + (int64_t)(q * 5) * nthr + tid;
 #pragma unroll
-    for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = v[i];
-    tp[(int64_t)5 * nthr] = (double)Ef;
+      for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = v[i];
+      tape_e[(int64_t)q * nthr + tid] = Ef;
+    }
     const double kd = wk * l.x, dkd = dwk * l.x;
     WaveFn A, Bw;
     wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
@@ -1092,28 +1106,50 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       if (O.drho) O.drho[o] = acc_lo[2] + er;
     }
   };
-  // records, tape entries and flattening factors one iteration ahead
-  // (unconditional loads at clamped indices; the tape holds at least one
-  // entry per thread)
-  const int mt = H >= 1 ? H - 1 : 0;  // last tape entry
-  auto load_tape = [&](int m, double* t6) {
-    const double* tp = tape + (int64_t)(m * 6) * nthr + tid;
-#pragma unroll
-    for (int i = 0; i < 6; i++) t6[i] = tp[(int64_t)i * nthr];
-  };
+  // records and flattening factors one iteration ahead (unconditional loads)
   double4 ln = l0;
   double4 lprev = make_double4(0.0, 0.0, l0.z, l0.w);  // (rho, b) of the layer above
-  double tn[6];
-  load_tape(0, tn);
   double btn = btp[0], brn = btr[0], bt_prev = 0.0, br_prev = 0.0;
-  for (int m = 0; m < H; m++) {
+  for (int m0 = 0; m0 < H; m0 += KERN_SEG) {
+    const int m1 = min(m0 + KERN_SEG - 1, H - 1);
+    {
+      // the segment's states: its checkpoint (entering layer m1), then up
+      const int q = m0 / KERN_SEG;
+      const double* tp = tape + (int64_t)(q * 5) * nthr + tid;
+      double v[5];
+#pragma unroll
+      for (int i = 0; i < 5; i++) v[i] = tp[(int64_t)i * nthr];
+      int Ev = tape_e[(int64_t)q * nthr + tid];
+      auto put = [&](int j) {
+#pragma unroll
+        for (int i = 0; i < 5; i++) seg[(j * 5 + i) * bd] = v[i];
+        sege[j * bd] = Ev;
+      };
+      put(m1 - m0);
+      double4 l = lay[m1];
+      for (int m = m1; m > m0; m--) {
+        const double4 lu = lay[m - 1];
+        const double kd = wk * l.x;
+        WaveFn A, Bw;
+        wave_fn(fma(-c2, 1.0 / (l.y * l.y), 1.0), kd, A);
+        wave_fn(fma(-c2, 1.0 / (l.z * l.z), 1.0), kd, Bw);
+        double t[4], u[5], eps, dmu, yz[4];
+        fdm_layer_fn(A, Bw, v, t, u);
+        iface_consts(lu, l, &eps, &dmu);
+        fdm_iface_fn(eps, dmu * ic2, u, yz, v);
+        Ev += pow2_normalize(v);
+        put(m - 1 - m0);
+        l = lu;
+      }
+    }
+    for (int m = m0; m <= m1; m++) {
     const double4 l = ln;
     double tc[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) tc[i] = tn[i];
+    for (int i = 0; i < 5; i++) tc[i] = seg[((m - m0) * 5 + i) * bd];
+    tc[5] = (double)sege[(m - m0) * bd];
     const double bt = btn, brt = brn;
     ln = lay[m + 1];
-    load_tape(min(m + 1, mt), tn);
     btn = btp[m + 1];
     brn = btr[m + 1];
     const double kd = wk * l.x;
@@ -1158,6 +1194,7 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
 #pragma unroll
     for (int i = 0; i < 5; i++) g[i] = vb[i];
     if (m & 1) Eg += pow2_normalize(g);
+    }
   }
   // the half-space: the interface above it, then its start state
   double ub[5];
@@ -1186,10 +1223,14 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   emit(H, ha, hb, hr, btn, brn);
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
-k_kernels(BatchDev B, KernDev O, double* tape) {
+__global__ void __launch_bounds__(KERN_BLOCK) __attribute__((amdgpu_waves_per_eu(2)))
+k_kernels(BatchDev B, KernDev O, double* tape, int nq) {
+  extern __shared__ double kseg_lds[];
   const int64_t nthr = (int64_t)gridDim.x * blockDim.x;
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int bd = blockDim.x;
+  double* seg = kseg_lds + threadIdx.x;
+  int* sege = (int*)(kseg_lds + KERN_SEG * 5 * bd) + threadIdx.x;
   for (int64_t s = tid; s < O.nsolves; s += nthr) {
     int col, k;
     if (B.uniform) {
@@ -1199,7 +1240,7 @@ k_kernels(BatchDev B, KernDev O, double* tape) {
       col = O.solve_col[s];
       k = (int)(s - B.toff[col]);
     }
-    kern_solve(B, O, col, k, s, tape, nthr, tid);
+    kern_solve(B, O, col, k, s, tape, nthr, tid, seg, sege, bd, nq);
   }
 }
 
@@ -1655,11 +1696,16 @@ hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, cons
   return hipGetLastError();
 }
 
-hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int grid_threads,
-                          hipStream_t st) {
-  int block = 256;
+hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int nq,
+                          int grid_threads, hipStream_t st) {
+  int block = KERN_BLOCK;
   int grid = std::max(1, grid_threads / block);
-  hipLaunchKernelGGL(k_kernels, dim3(grid), dim3(block), 0, st, B, O, tape);
+  // the checkpoint segments' LDS buffer (beyond the default 64 KiB of
+  // dynamic LDS; gfx950 has 160 KiB per workgroup)
+  const hipError_t attr = hipFuncSetAttribute(
+      (const void*)k_kernels, hipFuncAttributeMaxDynamicSharedMemorySize, KERN_LDS);
+  if (attr != hipSuccess) return attr;
+  hipLaunchKernelGGL(k_kernels, dim3(grid), dim3(block), KERN_LDS, st, B, O, tape, nq);
   return hipGetLastError();
 }
 
