@@ -103,11 +103,40 @@ DK_HDF double hyp_fwd(double X, double d, double* C, double* S, double* R, doubl
 // State e = (S11, S12, S22, Y12, Y21): S11 = (13), S12 = k (12), S22 = -k^2 (24),
 // Y12 = -k (14), Y21 = k (23) of the compound row vector, up to a common factor.
 // Propagation through a layer with the record g = (d, 1/a^2, 1/b^2, -).
+// The evanescent form of hyp_fwd (X > 0), without its branch.
+DK_HDF double hyp_ev(double X, double d, double* C, double* S, double* R) {
+  const double y = rsqrt_nr(X);
+  const double ra = X * y;
+  const double s = exp_neg_rs(ra * d);
+  const double fac = s * s;
+  const double sh = 0.5 - 0.5 * fac;
+  *C = 0.5 + 0.5 * fac;
+  *S = sh * y;
+  *R = ra * sh;
+  return s;
+}
+
 DK_HDF void fdm_layer(double c2, double k, const double4& g, double e[5]) {
   const double kd = k * g.x;
-  double Ca, Sa, Ra, Cb, Sb, Rb, pa, pb;
-  const double sa = hyp_fwd(fma(-c2, g.y, 1.0), kd, &Ca, &Sa, &Ra, &pa);
-  const double sb = hyp_fwd(fma(-c2, g.z, 1.0), kd, &Cb, &Sb, &Rb, &pb);
+  const double XP = fma(-c2, g.y, 1.0), XS = fma(-c2, g.z, 1.0);
+  double Ca, Sa, Ra, Cb, Sb, Rb, sa, sb;
+  // Both wave types evanescent on every lane of the wave (the common case
+  // below the top layers): one wave-uniform branch instead of two divergent
+  // ones, whose exec-mask bookkeeping is scalar work on the wave's own
+  // timeline.
+#if defined(__HIP_DEVICE_COMPILE__)
+  const bool all_ev = __all(XP > 0.0 && XS > 0.0);
+#else
+  const bool all_ev = XP > 0.0 && XS > 0.0;
+#endif
+  if (all_ev) {
+    sa = hyp_ev(XP, kd, &Ca, &Sa, &Ra);
+    sb = hyp_ev(XS, kd, &Cb, &Sb, &Rb);
+  } else {
+    double pa, pb;
+    sa = hyp_fwd(XP, kd, &Ca, &Sa, &Ra, &pa);
+    sb = hyp_fwd(XS, kd, &Cb, &Sb, &Rb, &pb);
+  }
   const double s11 = e[0], s22 = e[2], y12 = e[3], y21 = e[4];
   // P_a^T Y, P_a^T = [[Ca, Ra], [Sa, Ca]] (k-scaled)
   const double t11 = fma(Ca, s11, Ra * y21), t12 = fma(Ca, y12, Ra * s22);
