@@ -272,9 +272,10 @@ def bench_vjp(dk, configs, np, torch, cpu, reps=5):
 def bench_c3(dk, configs, np, torch, cpu, reps=5):
     """ragged_16384 (16,384 layered columns, 8-64 layers, 10-40 periods, c and
     refined-layer kernels): plain one launch per 64-column tile, one
-    non-persistent launch over the whole batch, and the batched persistent
-    launch with the autotuned shape, eager and replayed as a HIP graph.  Times
-    are wall-clock per call with inputs resident on the device."""
+    non-persistent launch over the whole batch, and the batched launch with
+    the autotuned shape and mode (persistent or one launch), eager and
+    replayed as a HIP graph.  Times are wall-clock per call with inputs
+    resident on the device."""
     from dazi_current_release_amd import batched
     W = configs.layered_workload("ragged_16384")
     b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"], W["periods"])
@@ -296,9 +297,11 @@ def bench_c3(dk, configs, np, torch, cpu, reps=5):
                batched_solves_per_s=ns / t_batch, graph_solves_per_s=ns / t_graph,
                batched_over_plain=t_plain / t_batch, graph_over_plain=t_plain / t_graph,
                batched_over_one_launch=t_one / t_batch,
-               tuned=dict(lanes_per_scan=tune.lanes_per_scan, cols_per_wg=tune.cols_per_wg),
-               autotune_ms={f"{l}x{c}": round(ms, 4) for l, c, ms in table},
-               roofline=roofline_rec(flop, t_batch, "whole step, batched persistent",
+               tuned=dict(lanes_per_scan=tune.lanes_per_scan, cols_per_wg=tune.cols_per_wg,
+                          mode="persistent" if tune.mode == 0 else "one launch"),
+               autotune_ms={f"{l}x{c}{'p' if m == 0 else 'o'}": round(ms, 4)
+                            for l, c, m, ms in table},
+               roofline=roofline_rec(flop, t_batch, "whole step, autotuned launch",
                                      flop_per_solve_mean=flop / ns))
     if cpu:
         rec["cpu_baseline"] = cpu("layered:ragged_16384:64")

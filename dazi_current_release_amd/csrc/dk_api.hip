@@ -54,6 +54,8 @@ enum Slot {
   S_STATUS,
   S_WORK,
   S_TAPE,
+  S_ORDER,   // ragged batches: columns in scheduling order, then the sort's counters
+  S_SORDER,  // ragged batches: solves in scheduling order
   S_H_VEL,
   S_H_VP,
   S_H_RHO,
@@ -1455,6 +1457,14 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     return rc;
   }
   if (int rc = pool_get(P, S_WORK, 64, &work)) return rc;
+  // scheduling order: columns (and, for the kernel pass, solves) by layer
+  // count, largest first
+  void *order, *sorder = nullptr;
+  if (int rc = pool_get(P, S_ORDER, ((size_t)b->ncol + dk::order_work_ints()) * 4, &order))
+    return rc;
+  if (kern && b->nsolves > 0) {
+    if (int rc = pool_get(P, S_SORDER, (size_t)b->nsolves * 4, &sorder)) return rc;
+  }
   P->gen++;
   P->have_events = g_timing;
   P->last_L = 0;
@@ -1463,6 +1473,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
                                 (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
+  HIPCHK(dk::launch_order(b->ncol, b->nl, b->nk, b->toff, b->nsolves, (int*)order, (int*)sorder,
+                          (int*)order + b->ncol, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;
@@ -1482,6 +1494,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   B.pv = b->c;
   B.status = b->status;
   B.work = (int*)work;
+  B.order = (const int*)order;
   B.tol = T.refine_tol;
   if (int rc = run_forward(P, B, T, b->max_layers, st)) return rc;
   rec(P, 2, st);
@@ -1490,6 +1503,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
     O.node_mode = 0;
     O.nsolves = b->nsolves;
     O.solve_col = b->solve_col;
+    O.solve_order = (const int*)sorder;
     O.dvs = b->dc_dvs;
     O.dvp = b->dc_dvp;
     O.drho = b->dc_drho;

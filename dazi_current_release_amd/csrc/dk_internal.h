@@ -37,6 +37,9 @@ struct BatchDev {
   double* pv;            // float32-rounded phase velocity per solve
   int* status;           // 0, or 1-based first failed period, or -1 bad model
   int* work;             // persistent work counter
+  const int* order;      // ragged batches: the columns in scheduling order
+                         // (position w of a launch's range solves column
+                         // order[w]); NULL: index order
   double tol;            // refinement bracket width (km/s)
 };
 
@@ -50,6 +53,8 @@ struct KernDev {
   const int* nsub;
   int64_t nsolves;
   const int* solve_col;  // ragged batches: column of each solve
+  const int* solve_order;  // ragged batches: solves in scheduling order (NULL:
+                           // index order)
   double* dvs;
   double* dvp;
   double* drho;
@@ -126,6 +131,18 @@ int forward_max_blocks(int S, int cols_per_wg, int lds_stride);
 hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int nq,
                           int grid_threads, hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
+// Scheduling order of a ragged batch: its columns, and (solve_order non-NULL)
+// its solves, sorted by the column's layer count, largest first, so that the
+// groups of a wavefront and the threads of the kernel pass run columns of
+// nearly equal depth, and a persistent launch ends on its shortest columns.
+// A counting sort over ORDER_BINS keys (layer counts from ORDER_BINS - 1 up
+// share the last key); work holds order_work_ints() ints.  The order within
+// a key is unspecified (no result depends on it).
+constexpr int ORDER_BINS = 1024;
+inline int order_work_ints() { return 4 * ORDER_BINS; }
+hipError_t launch_order(int ncol, const int* Lc, const int* Kc, const int64_t* toff,
+                        int64_t nsolves, int* order, int* solve_order, int* work,
+                        hipStream_t st);
 // G-row assembly (CalSurfGAniso_Joint.f90:750-797), device pointers; see
 // dazi_dk_grows_dev.  row_nnz, row_off: per ray; nar: one int64.
 struct GrowDev {

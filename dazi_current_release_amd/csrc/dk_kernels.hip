@@ -504,14 +504,12 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
   const double twopi = 2.0 * 3.141592653589793;
   const double tol = B.tol;
 
-  int col;
-  if (persistent) {
-    int c0 = 0;
-    if (r == 0) c0 = atomicAdd(B.work, 1);
-    col = __shfl(c0, gbase, 64) + col_begin;
-  } else {
-    col = col_begin + blockIdx.x * gpb + gid_blk;
-  }
+  // w: the group's position in the scheduling order, col: its column.  The
+  // first column of every group is its group index; a persistent group then
+  // takes positions ngroups, ngroups + 1, ... from the work counter (no burst
+  // of same-address atomics when the launch starts).
+  const int ngroups = gridDim.x * gpb;
+  int w = col_begin + blockIdx.x * gpb + gid_blk, col = 0;
 
   // group state (identical in all S lanes of the group)
   int L = 0, K = 0, k = 0, st = ST_DONE;
@@ -530,10 +528,11 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
   int nver = 0;                            // verification rounds of the current period
 
   auto begin_column = [&]() {
-    if (col >= col_end) {
+    if (w >= col_end) {
       st = ST_DONE;
       return;
     }
+    col = B.order ? B.order[w] : w;
     col_geom(B, col, &L, &loff, &K, &toff);
     for (int m = r; m < L; m += S)
       layer_rec(B.layers[loff + m], B.layers[loff + max(m - 1, 0)], m == 0, &mdl[2 * m],
@@ -606,10 +605,10 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
   while (true) {
     // a wave runs while any of its groups has work; finished groups in a
     // persistent launch fetch the next column
-    if (persistent && st == ST_DONE && col < col_end) {
+    if (persistent && st == ST_DONE && w < col_end) {
       int c0 = 0;
       if (r == 0) c0 = atomicAdd(B.work, 1);
-      col = __shfl(c0, gbase, 64) + col_begin;
+      w = __shfl(c0, gbase, 64) + col_begin + ngroups;
       begin_column();
     }
     if (!__any(st != ST_DONE)) break;
@@ -1231,7 +1230,9 @@ k_kernels(BatchDev B, KernDev O, double* tape, int nq) {
   const int bd = blockDim.x;
   double* seg = kseg_lds + threadIdx.x;
   int* sege = (int*)(kseg_lds + KERN_SEG * 5 * bd) + threadIdx.x;
-  for (int64_t s = tid; s < O.nsolves; s += nthr) {
+  for (int64_t t = tid; t < O.nsolves; t += nthr) {
+    const int64_t s = O.solve_order ? (int64_t)O.solve_order[t] : t;
+    if (s < 0 || s >= O.nsolves) continue;  // (an order of inconsistent nk, toff)
     int col, k;
     if (B.uniform) {
       col = (int)(s % B.ncol);
@@ -1616,6 +1617,98 @@ __global__ void k_zero_int(int* p, int n) {
   if (i < n) p[i] = 0;
 }
 
+// ------------------------------------------------------------------ scheduling order
+// Counting sort of a ragged batch's columns (and solves) by layer count,
+// largest first: k_order_hist counts columns and solves per key, k_order_scan
+// turns the counts into each key's first position, k_order_scatter places the
+// items.  work = [column counts | solve counts | column bases | solve bases],
+// ORDER_BINS ints each.
+__device__ __forceinline__ int order_key(int L) { return min(max(L, 0), ORDER_BINS - 1); }
+
+__global__ void __launch_bounds__(256) k_order_hist(int ncol, const int* Lc, const int* Kc,
+                                                    int* work) {
+  __shared__ int hc[ORDER_BINS], hs[ORDER_BINS];
+  for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) hc[b] = hs[b] = 0;
+  __syncthreads();
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
+    const int key = order_key(Lc[c]);
+    atomicAdd(&hc[key], 1);
+    atomicAdd(&hs[key], max(Kc[c], 0));
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) {
+    if (hc[b]) atomicAdd(&work[b], hc[b]);
+    if (hs[b]) atomicAdd(&work[ORDER_BINS + b], hs[b]);
+  }
+}
+
+// one workgroup of ORDER_BINS threads; thread t owns key ORDER_BINS-1-t (the
+// keys in descending order), so an exclusive scan over t gives each key's
+// first position
+__global__ void __launch_bounds__(ORDER_BINS) k_order_scan(int* work) {
+  __shared__ int2 buf[ORDER_BINS];
+  const int t = threadIdx.x;
+  const int key = ORDER_BINS - 1 - t;
+  const int2 own = make_int2(work[key], work[ORDER_BINS + key]);
+  buf[t] = own;
+  __syncthreads();
+  for (int d = 1; d < ORDER_BINS; d <<= 1) {
+    int2 v = buf[t];
+    if (t >= d) {
+      const int2 u = buf[t - d];
+      v.x += u.x;
+      v.y += u.y;
+    }
+    __syncthreads();
+    buf[t] = v;
+    __syncthreads();
+  }
+  work[2 * ORDER_BINS + key] = buf[t].x - own.x;
+  work[3 * ORDER_BINS + key] = buf[t].y - own.y;
+}
+
+// ORDER_ITEMS columns per thread, one chunk of 256 * ORDER_ITEMS columns per
+// workgroup: positions within the chunk from LDS atomics, one global
+// reservation per (workgroup, key)
+constexpr int ORDER_ITEMS = 4;
+__global__ void __launch_bounds__(256) k_order_scatter(int ncol, const int* Lc, const int* Kc,
+                                                       const int64_t* toff, int64_t nsolves,
+                                                       int* order, int* solve_order, int* work) {
+  __shared__ int pc[ORDER_BINS], ps[ORDER_BINS];
+  for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) pc[b] = ps[b] = 0;
+  __syncthreads();
+  int key[ORDER_ITEMS], oc[ORDER_ITEMS], os[ORDER_ITEMS], kk[ORDER_ITEMS];
+  const int c0 = blockIdx.x * blockDim.x * ORDER_ITEMS + threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < ORDER_ITEMS; i++) {
+    const int c = c0 + i * blockDim.x;
+    key[i] = -1;
+    if (c < ncol) {
+      key[i] = order_key(Lc[c]);
+      kk[i] = max(Kc[c], 0);
+      oc[i] = atomicAdd(&pc[key[i]], 1);
+      os[i] = atomicAdd(&ps[key[i]], kk[i]);
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) {
+    if (pc[b]) pc[b] = atomicAdd(&work[2 * ORDER_BINS + b], pc[b]);
+    if (ps[b]) ps[b] = atomicAdd(&work[3 * ORDER_BINS + b], ps[b]);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < ORDER_ITEMS; i++) {
+    if (key[i] < 0) continue;
+    const int c = c0 + i * blockDim.x;
+    order[pc[key[i]] + oc[i]] = c;
+    if (solve_order) {
+      const int64_t p0 = (int64_t)ps[key[i]] + os[i];
+      const int64_t t0 = toff[c];
+      for (int k = 0; k < kk[i] && p0 + k < nsolves; k++) solve_order[p0 + k] = (int)(t0 + k);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int S>
 static hipError_t launch_forward_S(const BatchDev& B, int col_begin, int col_end, int cols_per_wg,
@@ -1711,6 +1804,21 @@ hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int
 
 hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_order(int ncol, const int* Lc, const int* Kc, const int64_t* toff,
+                        int64_t nsolves, int* order, int* solve_order, int* work,
+                        hipStream_t st) {
+  if (ncol <= 0) return hipSuccess;
+  hipError_t e = launch_zero_int(work, 2 * ORDER_BINS, st);
+  if (e != hipSuccess) return e;
+  const int hist_grid = std::min((ncol + 255) / 256, 256);
+  hipLaunchKernelGGL(k_order_hist, dim3(hist_grid), dim3(256), 0, st, ncol, Lc, Kc, work);
+  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_BINS), 0, st, work);
+  const int chunk = 256 * ORDER_ITEMS;
+  hipLaunchKernelGGL(k_order_scatter, dim3((ncol + chunk - 1) / chunk), dim3(256), 0, st, ncol,
+                     Lc, Kc, toff, nsolves, order, solve_order, work);
   return hipGetLastError();
 }
 

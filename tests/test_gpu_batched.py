@@ -27,8 +27,10 @@ def test_autotune_ragged_picks_a_valid_shape(dk):
     b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"], W["periods"])
     out = b.alloc_outputs(False)
     tune, table = batched.autotune(lambda t: dk.layered_dev(b, tune=t, out=out))
-    assert len(table) >= 8 and all(ms > 0 for _, _, ms in table)
-    assert (tune.lanes_per_scan, tune.cols_per_wg) == min(table, key=lambda r: r[2])[:2]
+    assert len(table) >= 16 and all(ms > 0 for *_, ms in table)
+    assert {m for _, _, m, _ in table} == {0, 1}
+    assert (tune.lanes_per_scan, tune.cols_per_wg, tune.mode) == min(table,
+                                                                     key=lambda r: r[3])[:3]
     o = {k: v.cpu().numpy() for k, v in dk.layered_dev(b, tune=tune).items()}
     ref, fail = P.forward_layered(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"],
                                   W["periods"], nthreads=16)

@@ -161,3 +161,69 @@ def test_invariants_full_ragged(dk):
     assert (np.abs(lhs - rhs) / c)[ok].max() < 1e-5
     assert ok.mean() > 0.99
     del torch
+
+
+def test_scheduling_order_every_solve_once(dk):
+    """The batch runs in layer-count order (a device counting sort with the top
+    key shared by every column of 1023 layers or more): every column and
+    solve is written, with the oracle's roots and status, and the outputs do
+    not depend on the order of the columns in the batch."""
+    import torch
+    from dazi_current_release_amd import configs
+    W = configs.layered_workload("ragged_16384", ncol=1500)
+    rng = np.random.default_rng(11)
+    nl, nk = list(W["nl"]), list(W["nk"])
+    thk, vs, per = [W["thk"]], [W["vs"]], [W["periods"]]
+    for L in (1023, 1100, 1030):  # deep columns: at and beyond the top key
+        t, v = configs._layered_profile(rng, L, 40.0, 1.0, 1.5, None)
+        nl.append(L)
+        nk.append(12)
+        thk.append(t.astype(np.float32))
+        vs.append(v.astype(np.float32))
+        per.append(np.sort(rng.uniform(2.0, 60.0, 12)))
+    vs = np.concatenate(vs)
+    vp = configs.brocher_vp(vs)
+    D = dict(nl=np.array(nl, np.int32), nk=np.array(nk, np.int32), thk=np.concatenate(thk),
+             vs=vs, vp=vp, rho=configs.brocher_rho(vp), periods=np.concatenate(per))
+    tune = dk.make_tune(64, 1, 0, 64, 2e-6)  # a 1100-layer column fits one group per workgroup
+
+    def solve(D):
+        b = dk.LayeredBatch(D["nl"], D["nk"], D["thk"], D["vp"], D["vs"], D["rho"], D["periods"])
+        out = b.alloc_outputs(True)
+        for v in out.values():
+            v.fill_(-7) if v.dtype == torch.int32 else v.fill_(float("nan"))
+        dk.layered_dev(b, kernels=True, tune=tune, out=out)
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
+    o = solve(D)
+    # the oracle, like the reference (NL = 200), takes at most 200 layers: the
+    # deep columns are checked for a root in every period and for order
+    # independence below
+    n0, s0 = W["nl"].size, int(W["nk"].sum())
+    ref, fail = P.forward_layered(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"],
+                                  W["periods"], nthreads=16)
+    assert np.array_equal(o["status"][:n0], fail)
+    assert np.array_equal(o["c"][:s0] == 0, ref == 0)
+    assert np.abs(o["c"][:s0] - ref).max() <= 1e-5
+    assert (o["status"][n0:] == 0).all() and (o["c"][s0:] > 0).all()
+    col = np.repeat(np.arange(len(nl)), D["nk"])
+    found = o["c"] > 0
+    for key in ("dc_dvs", "dc_dvp", "dc_drho"):
+        rows = np.arange(o[key].shape[0])[:, None] < D["nl"][col][None, :]
+        assert np.isfinite(o[key][rows & found[None, :]]).all(), key
+    # the same columns in reverse order: bit-identical outputs per column
+    n = len(nl)
+    loff, toff = offsets(D["nl"]), offsets(D["nk"])
+    rev = np.arange(n)[::-1]
+    R = dict(nl=D["nl"][rev], nk=D["nk"][rev])
+    for k in ("thk", "vp", "vs", "rho"):
+        R[k] = np.concatenate([D[k][loff[j]:loff[j] + D["nl"][j]] for j in rev])
+    R["periods"] = np.concatenate([D["periods"][toff[j]:toff[j] + D["nk"][j]] for j in rev])
+    r = solve(R)
+    sidx = np.concatenate([np.arange(toff[j], toff[j] + D["nk"][j]) for j in rev])
+    assert np.array_equal(r["status"], o["status"][rev])
+    assert np.array_equal(r["c_unrounded"], o["c_unrounded"][sidx])
+    for key in ("dc_dvs", "dc_dvp", "dc_drho"):
+        a, bb = r[key], o[key][:, sidx]
+        same = (a == bb) | (np.isnan(a) & np.isnan(bb))
+        assert same.all(), key
