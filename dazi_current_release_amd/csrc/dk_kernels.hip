@@ -920,7 +920,9 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
 // positive factor, which drops out at F = 0).
 //   sweep 1, bottom-up: the state and its tangent in c; the state entering
 //            the bottom layer of every segment of KERN_SEG layers (counted
-//            from the top) is kept as a checkpoint with its exponent;
+//            from the top) but the lowest is kept as a checkpoint with its
+//            exponent (the lowest segment starts from the half-space, which
+//            sweep 2 evaluates again);
 //   sweep 2, top-down, a segment at a time: the segment's states are
 //            recomputed from its checkpoint (value steps only) into LDS, then
 //            the adjoint dF/d(state) runs through it.  At layer m the state
@@ -930,6 +932,8 @@ k_forward(BatchDev B, int col_begin, int col_end, int persistent, int lds_stride
 //            (a, b, rho) are chained to the unflattened layer.  The interface
 //            above layer m also depends on layer m-1's (rho, b), so each
 //            layer's partials are complete, and emitted, one step later.
+// Checkpoints and outputs are written and read with non-temporal accesses
+// (streamed past the caches, which keep the layer records instead).
 // Checkpoint layout for segment q: state entry i at tape[(q*5 + i) * nthr + tid],
 // its exponent at ((int*)(tape + nq*5*nthr))[q * nthr + tid] (nq checkpoints
 // per thread).  LDS segment buffer of the thread (blockDim.x = bd
@@ -955,9 +959,9 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   if (!(c > 0.0)) {
     for (int m = 0; m < nout; m++) {
       int64_t o = out_index(m);
-      if (O.dvs) O.dvs[o] = 0.0;
-      if (O.dvp) O.dvp[o] = 0.0;
-      if (O.drho) O.drho[o] = 0.0;
+      if (O.dvs) __builtin_nontemporal_store(0.0, O.dvs + o);
+      if (O.dvp) __builtin_nontemporal_store(0.0, O.dvp + o);
+      if (O.drho) __builtin_nontemporal_store(0.0, O.drho + o);
     }
     return;
   }
@@ -1006,12 +1010,12 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     const double4 l = lcur, lu = lup;
     lcur = lup;
     lup = lay[cl(m - 2)];
-    if (m == H - 1 || m % KERN_SEG == KERN_SEG - 1) {  // the bottom of segment m / KERN_SEG
+    if (m % KERN_SEG == KERN_SEG - 1 && m != H - 1) {  // the bottom of segment m / KERN_SEG
       const int q = m / KERN_SEG;
       double* tp = tape + (int64_t)(q * 5) * nthr + tid;
 #pragma unroll
-      for (int i = 0; i < 5; i++) tp[(int64_t)i * nthr] = v[i];
-      tape_e[(int64_t)q * nthr + tid] = Ef;
+      for (int i = 0; i < 5; i++) __builtin_nontemporal_store(v[i], tp + (int64_t)i * nthr);
+      __builtin_nontemporal_store(Ef, tape_e + (int64_t)q * nthr + tid);
     }
     const double kd = wk * l.x, dkd = dwk * l.x;
     WaveFn A, Bw;
@@ -1066,9 +1070,9 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
     const double ea = qa * bt, eb = qb * bt, er = qr * brt;
     if (!O.node_mode) {
       int64_t o = out_index(m);
-      if (O.dvp) O.dvp[o] = ea;
-      if (O.dvs) O.dvs[o] = eb;
-      if (O.drho) O.drho[o] = er;
+      if (O.dvp) __builtin_nontemporal_store(ea, O.dvp + o);
+      if (O.dvs) __builtin_nontemporal_store(eb, O.dvs + o);
+      if (O.drho) __builtin_nontemporal_store(er, O.drho + o);
       return;
     }
     // node mode: sublayer j (1-based) of interval `node` with ns sublayers
@@ -1086,9 +1090,9 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       sub++;
       if (sub == ns && node < O.nz - 1) {
         int64_t o = out_index(node);
-        if (O.dvs) O.dvs[o] = acc_lo[0];
-        if (O.dvp) O.dvp[o] = acc_lo[1];
-        if (O.drho) O.drho[o] = acc_lo[2];
+        if (O.dvs) __builtin_nontemporal_store(acc_lo[0], O.dvs + o);
+        if (O.dvp) __builtin_nontemporal_store(acc_lo[1], O.dvp + o);
+        if (O.drho) __builtin_nontemporal_store(acc_lo[2], O.drho + o);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
           acc_lo[q] = acc_hi[q];
@@ -1100,9 +1104,9 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
       }
     } else {
       int64_t o = out_index(node);
-      if (O.dvs) O.dvs[o] = acc_lo[0] + eb;
-      if (O.dvp) O.dvp[o] = acc_lo[1] + ea;
-      if (O.drho) O.drho[o] = acc_lo[2] + er;
+      if (O.dvs) __builtin_nontemporal_store(acc_lo[0] + eb, O.dvs + o);
+      if (O.dvp) __builtin_nontemporal_store(acc_lo[1] + ea, O.dvp + o);
+      if (O.drho) __builtin_nontemporal_store(acc_lo[2] + er, O.drho + o);
     }
   };
   // records and flattening factors one iteration ahead (unconditional loads)
@@ -1112,13 +1116,25 @@ __device__ __forceinline__ void kern_solve(const BatchDev& B, const KernDev& O, 
   for (int m0 = 0; m0 < H; m0 += KERN_SEG) {
     const int m1 = min(m0 + KERN_SEG - 1, H - 1);
     {
-      // the segment's states: its checkpoint (entering layer m1), then up
-      const int q = m0 / KERN_SEG;
-      const double* tp = tape + (int64_t)(q * 5) * nthr + tid;
+      // the segment's states: its checkpoint (entering layer m1), then up;
+      // the bottom segment's starts from the half-space as sweep 1 does
       double v[5];
+      int Ev = 0;
+      if (m1 == H - 1) {
+        const double4 lh = lay[H];
+        double ran, rbn, sgA, sgB, eps, dmu, yz[4], w[5];
+        fdm_start(c2, lh, v, &ran, &rbn, &sgA, &sgB);
+        iface_consts(lay[H - 1], lh, &eps, &dmu);
+        fdm_iface_fn(eps, dmu * ic2, v, yz, w);
 #pragma unroll
-      for (int i = 0; i < 5; i++) v[i] = tp[(int64_t)i * nthr];
-      int Ev = tape_e[(int64_t)q * nthr + tid];
+        for (int i = 0; i < 5; i++) v[i] = w[i];
+      } else {
+        const int q = m0 / KERN_SEG;
+        const double* tp = tape + (int64_t)(q * 5) * nthr + tid;
+#pragma unroll
+        for (int i = 0; i < 5; i++) v[i] = __builtin_nontemporal_load(tp + (int64_t)i * nthr);
+        Ev = __builtin_nontemporal_load(tape_e + (int64_t)q * nthr + tid);
+      }
       auto put = [&](int j) {
 #pragma unroll
         for (int i = 0; i < 5; i++) seg[(j * 5 + i) * bd] = v[i];
