@@ -789,11 +789,27 @@ struct Slots {
   static constexpr int CD = 0, EX = 5 * W, F = 5 * W + 1, N = 5 * W + 4;
 };
 enum { Q_N = Slots<double>::N, Q_NC = Slots<cx>::N };
+// Scratch accesses: on the device they are non-temporal (streamed past the
+// caches, which keep the layer records instead).
+DK_HD void scr_st(double* p, double v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+DK_HD double scr_ld(const double* p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
 // Kernel factor q (facah, facbv, facn) of layer m in the scratch of a solve
 // in T arithmetic.
 template <class T>
 DK_HD double factor(const double* scratch, long stride, int m, int q) {
-  return scratch[(long)(m * Slots<T>::N + Slots<T>::F + q) * stride];
+  return scr_ld(scratch + (long)(m * Slots<T>::N + Slots<T>::F + q) * stride);
 }
 // chksiz's float rounding of a kernel value
 DK_HD float kernel_f32(double x) { return fabs(x) < 1.0e-36 ? 0.0f : (float)x; }
@@ -813,14 +829,15 @@ DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double*
                 long stride, double* isc_out, double* inrm_out) {
   using T = typename G_::T;
   using Q = Slots<T>;
-  auto S = [&](int m, int q) -> double& { return scratch[(long)(m * Q::N + q) * stride]; };
+  auto P = [&](int m, int q) -> double* { return scratch + (long)(m * Q::N + q) * stride; };
+  auto S = [&](int m, int q) -> double { return scr_ld(P(m, q)); };
   auto put_cd = [&](int m, const T* e) {
     for (int q = 0; q < 5; q++) {
       if constexpr (Q::W == 1) {
-        S(m, Q::CD + q) = re(e[q]);
+        scr_st(P(m, Q::CD + q), re(e[q]));
       } else {
-        S(m, Q::CD + 2 * q) = re(e[q]);
-        S(m, Q::CD + 2 * q + 1) = e[q].i;
+        scr_st(P(m, Q::CD + 2 * q), re(e[q]));
+        scr_st(P(m, Q::CD + 2 * q + 1), e[q].i);
       }
     }
   };
@@ -837,7 +854,7 @@ DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double*
   ok &= g.ok;
   halfspace_constraint(g, cd);
   put_cd(L - 1, cd);
-  S(L - 1, Q::EX) = 0.0;
+  scr_st(P(L - 1, Q::EX), 0.0);
   double exsum = 0.0;
   // each layer is loaded one iteration ahead, so that its latency overlaps the
   // layer before it
@@ -850,7 +867,7 @@ DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double*
     // its scale are this iteration's inputs, so they are in registers anyway.
     // At m = L - 2 this rewrites the half-space entries with their own values.
     put_cd(m + 1, cd);
-    S(m + 1, Q::EX) = exsum;
+    scr_st(P(m + 1, Q::EX), exsum);
     eigen(ly, omega, wvno, iwv2, &g);
     ok &= g.ok;
     hyper(g, ly.d, &h);
@@ -860,7 +877,7 @@ DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double*
   }
   if (L >= 2) {
     put_cd(0, cd);
-    S(0, Q::EX) = exsum;
+    scr_st(P(0, Q::EX), exsum);
   }
   if (!ok) return -2;
   // ---- surface eigenfunction: ur = Re(cd(1,3)/cd(1,2)), f1213 = -Re cd(1,2)
@@ -952,9 +969,9 @@ DK_HD int solve(int L, HeadFn head, TailFn tail, double omega, double c, double*
     const double facav = rho * av * DUZDUZ * iwv2;
     const double facbv = rho * bv * (UZUZ + 2.0 * UZDUR * iwv + DURDUR * iwv2 + 4.0 * eta * URDUZ * iwv);
     const double facn = -ly.Feta * URDUZ * iwv;
-    S(m, Q::F + 0) = facah;
-    S(m, Q::F + 1) = facbv;
-    S(m, Q::F + 2) = facn;
+    scr_st(P(m, Q::F + 0), facah);
+    scr_st(P(m, Q::F + 1), facbv);
+    scr_st(P(m, Q::F + 2), facn);
     dcq = dcq + lw * (facbv * bv * qbi) + lw * (facav * av * qai + facah * ah * qai);
 #pragma unroll
     for (int i = 0; i < 4; i++) yt[i] = yb[i];
