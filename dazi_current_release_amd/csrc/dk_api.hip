@@ -24,7 +24,7 @@ namespace {
 
 thread_local std::string g_err;
 // cols_per_wg 0: chosen per launch (resolve_shape)
-dazi_dk_tune g_tune = {16, 0, 0, 64, 2.0e-6};
+dazi_dk_tune g_tune = {0, 0, 0, 64, 2.0e-6};
 int g_timing = 0;
 
 int set_err(int code, const char* fmt, ...) {
@@ -208,6 +208,16 @@ int node_geometry(int nz, const float* depz, float minthk0, std::vector<int>* ns
   return L + 1;
 }
 
+// lanes_per_scan x cols_per_wg, once both are known: whole waves, at most 256
+int check_block(int S, int cpw) {
+  const int block = S * cpw;
+  if (S != 0 && cpw != 0 && (block > 256 || (block % 64) != 0))
+    return set_err(DAZI_DK_EINVAL,
+                   "lanes_per_scan*cols_per_wg must be a multiple of 64 and <= 256 (got %d*%d)",
+                   S, cpw);
+  return 0;
+}
+
 int check_tune(const dazi_dk_tune* t, dazi_dk_tune* out) {
   *out = t ? *t : g_tune;
   if (out->lanes_per_scan == 0) out->lanes_per_scan = g_tune.lanes_per_scan;
@@ -216,13 +226,11 @@ int check_tune(const dazi_dk_tune* t, dazi_dk_tune* out) {
   if (!(out->refine_tol > 0.0)) out->refine_tol = 2.0e-6;
   if (out->refine_tol < 1.0e-12) out->refine_tol = 1.0e-12;
   int S = out->lanes_per_scan;
-  if (S != 8 && S != 16 && S != 32 && S != 64)
-    return set_err(DAZI_DK_EINVAL, "lanes_per_scan must be 8, 16, 32 or 64 (got %d)", S);
-  int block = S * out->cols_per_wg;
-  if (out->cols_per_wg != 0 && (out->cols_per_wg < 1 || block > 256 || (block % 64) != 0))
-    return set_err(DAZI_DK_EINVAL,
-                   "lanes_per_scan*cols_per_wg must be a multiple of 64 and <= 256 (got %d*%d)",
-                   S, out->cols_per_wg);
+  if (S != 0 && S != 8 && S != 16 && S != 32 && S != 64)
+    return set_err(DAZI_DK_EINVAL, "lanes_per_scan must be 0, 8, 16, 32 or 64 (got %d)", S);
+  if (out->cols_per_wg < 0)
+    return set_err(DAZI_DK_EINVAL, "cols_per_wg must be >= 0 (got %d)", out->cols_per_wg);
+  if (int rc = check_block(S, out->cols_per_wg)) return rc;
   if (out->mode != 0 && out->mode != 1) return set_err(DAZI_DK_EINVAL, "mode must be 0 or 1");
   if (out->tile_columns < 1) return set_err(DAZI_DK_EINVAL, "tile_columns must be >= 1");
   return 0;
@@ -248,12 +256,24 @@ size_t lds_per_block() {
   return (size_t)lim[dev];
 }
 
-// The root search's workgroup shape for models of up to lmax layers: the LDS
-// staging holds two double4 records per layer and column.  cols_per_wg = 0
-// picks 256-thread workgroups (one wave on each SIMD of a CU: smaller groups
-// can leave the waves of a one-wave-per-SIMD launch crowded on part of the
-// device), halved while the layer records do not fit.
-int resolve_shape(dazi_dk_tune* T, int lmax) {
+// The root search's workgroup shape for ncol columns of up to lmax layers on
+// ncu compute units.  lanes_per_scan = 0 takes the most lanes per column, of
+// 16, 32 and 64, that keep the launch within one wave per SIMD (16 when even
+// those exceed it): a small batch then scans more trial velocities per round
+// and needs fewer rounds, a large one keeps the wave count at the device's
+// SIMD count or above.  The LDS staging holds two double4 records per layer
+// and column.  cols_per_wg = 0 picks 256-thread workgroups (one wave on each
+// SIMD of a CU: smaller groups can leave the waves of a one-wave-per-SIMD
+// launch crowded on part of the device), halved while the layer records do
+// not fit.
+int resolve_shape(dazi_dk_tune* T, int lmax, int64_t ncol, int ncu) {
+  if (T->lanes_per_scan == 0) {
+    const int64_t simd = 4 * (int64_t)(ncu > 0 ? ncu : 1);
+    int S = 64;
+    while (S > 16 && ncol * S / 64 > simd) S /= 2;
+    T->lanes_per_scan = S;
+    if (int rc = check_block(S, T->cols_per_wg)) return rc;
+  }
   const size_t per_col = (size_t)2 * lmax * 32, lim = lds_per_block();
   if (T->cols_per_wg == 0) {
     int cpw = 256 / T->lanes_per_scan;
@@ -274,7 +294,7 @@ int resolve_shape(dazi_dk_tune* T, int lmax) {
 int forward_launches(const dk::BatchDev& B, const dazi_dk_tune& T0, int lmax, int ncu,
                      hipStream_t st, int* nlaunch) {
   dazi_dk_tune T = T0;
-  if (int rc = resolve_shape(&T, lmax)) return rc;
+  if (int rc = resolve_shape(&T, lmax, B.ncol, ncu)) return rc;
   int S = T.lanes_per_scan, cpw = T.cols_per_wg;
   int lds_stride = 2 * lmax;
   *nlaunch = 0;
@@ -1610,7 +1630,7 @@ int32_t dazi_dk_layered_noalloc_dev(const dazi_dk_layered* b, const dazi_dk_tune
   if (int rc = noalloc_device(&N)) return rc;
   {
     dazi_dk_tune R = T;  // the LDS staging of the deepest model fits (device limit)
-    if (int rc = resolve_shape(&R, b->max_layers)) return rc;
+    if (int rc = resolve_shape(&R, b->max_layers, b->ncol, N->ncu)) return rc;
   }
   unsigned char* base = (unsigned char*)N->ptr;
   hipStream_t st = (hipStream_t)stream;

@@ -76,7 +76,10 @@ typedef struct dazi_dk_desc {
 
 /* Launch configuration of the root search. Zero fields take defaults. */
 typedef struct dazi_dk_tune {
-  int32_t lanes_per_scan;  /* trial velocities per scan step: 8, 16, 32 or 64 */
+  int32_t lanes_per_scan;  /* trial velocities per scan step: 8, 16, 32 or 64; 0 (default):
+                              the most of 16, 32 and 64 that keep the launch within one
+                              wave per SIMD (64 up to 1,024 columns on 256 CUs, 16 from
+                              2,049 on) */
   int32_t cols_per_wg;     /* columns per workgroup (lanes_per_scan*cols_per_wg <= 256);
                               0 (default): 256-thread workgroups, fewer columns where the
                               deepest model's LDS staging (64 B per layer and column) would

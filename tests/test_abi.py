@@ -59,6 +59,8 @@ def test_tune_validation():
     assert L.dazi_dk_set_tune(C.byref(bad)) == _lib.DAZI_DK_EINVAL
     bad = _lib.make_tune(16, 3, 0, 64, 2e-6)      # 48 threads: not whole waves
     assert L.dazi_dk_set_tune(C.byref(bad)) == _lib.DAZI_DK_EINVAL
+    auto = _lib.make_tune(0, 0, 0, 64, 2e-6)      # lanes by batch size
+    assert L.dazi_dk_set_tune(C.byref(auto)) == 0
     good = _lib.make_tune(32, 8, 1, 64, 1e-6)
     assert L.dazi_dk_set_tune(C.byref(good)) == 0
     got = _lib.Tune()
