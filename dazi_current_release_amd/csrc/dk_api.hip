@@ -1493,8 +1493,8 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
                                 (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
-  HIPCHK(dk::launch_order(b->ncol, b->nl, b->nk, b->toff, b->nsolves, (int*)order, (int*)sorder,
-                          (int*)order + b->ncol, st));
+  HIPCHK(dk::launch_order(b->ncol, b->max_layers, b->nl, b->nk, b->toff, b->nsolves, (int*)order,
+                          (int*)sorder, (int*)order + b->ncol, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;

@@ -132,17 +132,19 @@ hipError_t launch_kernels(const BatchDev& B, const KernDev& O, double* tape, int
                           int grid_threads, hipStream_t st);
 hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 // Scheduling order of a ragged batch: its columns, and (solve_order non-NULL)
-// its solves, sorted by the column's layer count, largest first, so that the
-// groups of a wavefront and the threads of the kernel pass run columns of
-// nearly equal depth, and a persistent launch ends on its shortest columns.
-// A counting sort over ORDER_BINS keys (layer counts from ORDER_BINS - 1 up
-// share the last key); work holds order_work_ints() ints.  The order within
-// a key is unspecified (no result depends on it).
-constexpr int ORDER_BINS = 1024;
+// its solves, sorted by the column's layer count and then its period count,
+// largest first, so that the groups of a wavefront and the threads of the
+// kernel pass run columns of nearly equal depth and length, and a launch ends
+// on its shortest columns.  A counting sort over ORDER_BINS keys: the layer
+// count (those from ORDER_BINS - 1 up share the last value) and, where the
+// keys leave room for it below max_layers, up to 4 bits of the period count;
+// work holds order_work_ints() ints.  The order within a key is unspecified
+// (no result depends on it).
+constexpr int ORDER_BINS = 4096;
 inline int order_work_ints() { return 4 * ORDER_BINS; }
-hipError_t launch_order(int ncol, const int* Lc, const int* Kc, const int64_t* toff,
-                        int64_t nsolves, int* order, int* solve_order, int* work,
-                        hipStream_t st);
+hipError_t launch_order(int ncol, int max_layers, const int* Lc, const int* Kc,
+                        const int64_t* toff, int64_t nsolves, int* order, int* solve_order,
+                        int* work, hipStream_t st);
 // G-row assembly (CalSurfGAniso_Joint.f90:750-797), device pointers; see
 // dazi_dk_grows_dev.  row_nnz, row_off: per ray; nar: one int64.
 struct GrowDev {

@@ -1634,22 +1634,28 @@ __global__ void k_zero_int(int* p, int n) {
 }
 
 // ------------------------------------------------------------------ scheduling order
-// Counting sort of a ragged batch's columns (and solves) by layer count,
-// largest first: k_order_hist counts columns and solves per key, k_order_scan
-// turns the counts into each key's first position, k_order_scatter places the
-// items.  work = [column counts | solve counts | column bases | solve bases],
-// ORDER_BINS ints each.
-__device__ __forceinline__ int order_key(int L) { return min(max(L, 0), ORDER_BINS - 1); }
+// Counting sort of a ragged batch's columns (and solves) by layer count, then
+// period count, largest first: k_order_hist counts columns and solves per
+// key, k_order_scan turns the counts into each key's first position,
+// k_order_scatter places the items.  work = [column counts | solve counts |
+// column bases | solve bases], ORDER_BINS ints each.  The key is
+// min(L, ORDER_BINS / 2^kb - 1) * 2^kb + the top kb bits of min(K, 63).
+__device__ __forceinline__ int order_key(int L, int K, int kb) {
+  const int lk = min(max(L, 0), (ORDER_BINS >> kb) - 1);
+  const int kq = kb > 0 ? min(max(K, 0), 63) >> (6 - kb) : 0;
+  return (lk << kb) | kq;
+}
 
-__global__ void __launch_bounds__(256) k_order_hist(int ncol, const int* Lc, const int* Kc,
+__global__ void __launch_bounds__(256) k_order_hist(int ncol, const int* Lc, const int* Kc, int kb,
                                                     int* work) {
   __shared__ int hc[ORDER_BINS], hs[ORDER_BINS];
   for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) hc[b] = hs[b] = 0;
   __syncthreads();
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < ncol; c += gridDim.x * blockDim.x) {
-    const int key = order_key(Lc[c]);
+    const int K = Kc[c];
+    const int key = order_key(Lc[c], K, kb);
     atomicAdd(&hc[key], 1);
-    atomicAdd(&hs[key], max(Kc[c], 0));
+    atomicAdd(&hs[key], max(K, 0));
   }
   __syncthreads();
   for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) {
@@ -1658,17 +1664,27 @@ __global__ void __launch_bounds__(256) k_order_hist(int ncol, const int* Lc, con
   }
 }
 
-// one workgroup of ORDER_BINS threads; thread t owns key ORDER_BINS-1-t (the
-// keys in descending order), so an exclusive scan over t gives each key's
-// first position
-__global__ void __launch_bounds__(ORDER_BINS) k_order_scan(int* work) {
-  __shared__ int2 buf[ORDER_BINS];
+// one workgroup of ORDER_SCAN threads; thread t owns the ORDER_BINS /
+// ORDER_SCAN consecutive positions of the keys in descending order from
+// t * ORDER_BINS / ORDER_SCAN, so an exclusive scan over the positions gives
+// each key's first position
+constexpr int ORDER_SCAN = 1024;
+constexpr int ORDER_PER = ORDER_BINS / ORDER_SCAN;
+__global__ void __launch_bounds__(ORDER_SCAN) k_order_scan(int* work) {
+  __shared__ int2 buf[ORDER_SCAN];
   const int t = threadIdx.x;
-  const int key = ORDER_BINS - 1 - t;
-  const int2 own = make_int2(work[key], work[ORDER_BINS + key]);
-  buf[t] = own;
+  int2 own[ORDER_PER];
+  int2 sum = make_int2(0, 0);
+#pragma unroll
+  for (int i = 0; i < ORDER_PER; i++) {
+    const int key = ORDER_BINS - 1 - (t * ORDER_PER + i);
+    own[i] = make_int2(work[key], work[ORDER_BINS + key]);
+    sum.x += own[i].x;
+    sum.y += own[i].y;
+  }
+  buf[t] = sum;
   __syncthreads();
-  for (int d = 1; d < ORDER_BINS; d <<= 1) {
+  for (int d = 1; d < ORDER_SCAN; d <<= 1) {
     int2 v = buf[t];
     if (t >= d) {
       const int2 u = buf[t - d];
@@ -1679,8 +1695,15 @@ __global__ void __launch_bounds__(ORDER_BINS) k_order_scan(int* work) {
     buf[t] = v;
     __syncthreads();
   }
-  work[2 * ORDER_BINS + key] = buf[t].x - own.x;
-  work[3 * ORDER_BINS + key] = buf[t].y - own.y;
+  int2 run = make_int2(buf[t].x - sum.x, buf[t].y - sum.y);
+#pragma unroll
+  for (int i = 0; i < ORDER_PER; i++) {
+    const int key = ORDER_BINS - 1 - (t * ORDER_PER + i);
+    work[2 * ORDER_BINS + key] = run.x;
+    work[3 * ORDER_BINS + key] = run.y;
+    run.x += own[i].x;
+    run.y += own[i].y;
+  }
 }
 
 // ORDER_ITEMS columns per thread, one chunk of 256 * ORDER_ITEMS columns per
@@ -1688,8 +1711,9 @@ __global__ void __launch_bounds__(ORDER_BINS) k_order_scan(int* work) {
 // reservation per (workgroup, key)
 constexpr int ORDER_ITEMS = 4;
 __global__ void __launch_bounds__(256) k_order_scatter(int ncol, const int* Lc, const int* Kc,
-                                                       const int64_t* toff, int64_t nsolves,
-                                                       int* order, int* solve_order, int* work) {
+                                                       int kb, const int64_t* toff,
+                                                       int64_t nsolves, int* order,
+                                                       int* solve_order, int* work) {
   __shared__ int pc[ORDER_BINS], ps[ORDER_BINS];
   for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) pc[b] = ps[b] = 0;
   __syncthreads();
@@ -1700,8 +1724,9 @@ __global__ void __launch_bounds__(256) k_order_scatter(int ncol, const int* Lc, 
     const int c = c0 + i * blockDim.x;
     key[i] = -1;
     if (c < ncol) {
-      key[i] = order_key(Lc[c]);
-      kk[i] = max(Kc[c], 0);
+      const int K = Kc[c];
+      key[i] = order_key(Lc[c], K, kb);
+      kk[i] = max(K, 0);
       oc[i] = atomicAdd(&pc[key[i]], 1);
       os[i] = atomicAdd(&ps[key[i]], kk[i]);
     }
@@ -1823,18 +1848,22 @@ hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_order(int ncol, const int* Lc, const int* Kc, const int64_t* toff,
-                        int64_t nsolves, int* order, int* solve_order, int* work,
-                        hipStream_t st) {
+hipError_t launch_order(int ncol, int max_layers, const int* Lc, const int* Kc,
+                        const int64_t* toff, int64_t nsolves, int* order, int* solve_order,
+                        int* work, hipStream_t st) {
   if (ncol <= 0) return hipSuccess;
+  // period-count bits of the key: as many (up to 4) as leave a key for every
+  // layer count up to max_layers
+  int kb = 0;
+  while (kb < 4 && ((int64_t)(max_layers + 1) << (kb + 1)) <= ORDER_BINS) kb++;
   hipError_t e = launch_zero_int(work, 2 * ORDER_BINS, st);
   if (e != hipSuccess) return e;
   const int hist_grid = std::min((ncol + 255) / 256, 256);
-  hipLaunchKernelGGL(k_order_hist, dim3(hist_grid), dim3(256), 0, st, ncol, Lc, Kc, work);
-  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_BINS), 0, st, work);
+  hipLaunchKernelGGL(k_order_hist, dim3(hist_grid), dim3(256), 0, st, ncol, Lc, Kc, kb, work);
+  hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_SCAN), 0, st, work);
   const int chunk = 256 * ORDER_ITEMS;
   hipLaunchKernelGGL(k_order_scatter, dim3((ncol + chunk - 1) / chunk), dim3(256), 0, st, ncol,
-                     Lc, Kc, toff, nsolves, order, solve_order, work);
+                     Lc, Kc, kb, toff, nsolves, order, solve_order, work);
   return hipGetLastError();
 }
 

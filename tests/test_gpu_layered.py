@@ -164,17 +164,18 @@ def test_invariants_full_ragged(dk):
 
 
 def test_scheduling_order_every_solve_once(dk):
-    """The batch runs in layer-count order (a device counting sort with the top
-    key shared by every column of 1023 layers or more): every column and
-    solve is written, with the oracle's roots and status, and the outputs do
-    not depend on the order of the columns in the batch."""
+    """The batch runs in (layer count, period count) order, a device counting
+    sort whose key keeps fewer period-count bits for deep batches (here one,
+    with columns of 1,023-1,100 layers): every column and solve is written,
+    with the oracle's roots and status, and the outputs do not depend on the
+    order of the columns in the batch."""
     import torch
     from dazi_current_release_amd import configs
     W = configs.layered_workload("ragged_16384", ncol=1500)
     rng = np.random.default_rng(11)
     nl, nk = list(W["nl"]), list(W["nk"])
     thk, vs, per = [W["thk"]], [W["vs"]], [W["periods"]]
-    for L in (1023, 1100, 1030):  # deep columns: at and beyond the top key
+    for L in (1023, 1100, 1030):  # deep columns
         t, v = configs._layered_profile(rng, L, 40.0, 1.0, 1.5, None)
         nl.append(L)
         nk.append(12)
