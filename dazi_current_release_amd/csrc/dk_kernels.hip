@@ -77,59 +77,6 @@ __device__ float gtsolh_f32(float a, float b) {
   return c;
 }
 
-// Earth flattening of one column in place (sphere(0,0) then sphere(2,1) for
-// Rayleigh waves), followed by the start-velocity rule of surfdisp96.
-// lay[m] = (thk, vp, vs, rho) on input (float32 values), flattened (d, a, b, rho)
-// on output, rounded to float32 as the reference stores them.
-__device__ void start_velocity(const double4* lay, int L, double* cc_out, float* betmx_out,
-                               int* bad) {
-#pragma clang fp contract(off)
-  float betmx = -1.e20f, betmn = 1.e20f;
-  int jmn = 0, jsol = 1;
-  for (int m = 0; m < L; m++) {
-    float a = (float)lay[m].y, b = (float)lay[m].z;
-    if (b > 0.01f && b < betmn) {
-      betmn = b;
-      jmn = m;
-      jsol = 1;
-    } else if (b <= 0.01f && a < betmn) {
-      betmn = a;
-      jmn = m;
-      jsol = 0;
-    }
-    if (b > betmx) betmx = b;
-  }
-  // a fluid layer (b <= 0.01) is outside this stage's model class, and so is
-  // any non-finite or non-positive layer value
-  if (jsol == 0 || lay[0].z <= 0.0) *bad = 1;
-  for (int m = 0; m < L; m++) {
-    double4 l = lay[m];
-    bool ok = isfinite(l.x) && isfinite(l.y) && isfinite(l.z) && isfinite(l.w) && l.y > 0.0 &&
-              l.z > 0.0 && l.w > 0.0 && (l.x > 0.0 || m == L - 1);
-    if (!ok) *bad = 1;
-  }
-  float cc1 = gtsolh_f32((float)lay[jmn].y, (float)lay[jmn].z);
-  cc1 = 0.95f * cc1;
-  cc1 = 0.90f * cc1;
-  if (!(cc1 > 0.0f) || !isfinite(cc1)) *bad = 1;
-  *cc_out = (double)cc1;
-  *betmx_out = betmx;
-}
-
-// Earth flattening of a column (sphere(0,0) then sphere(2,1) for Rayleigh
-// waves, surfdisp96.f:480-547) followed by the start-velocity rule of
-// surfdisp96 (:139-207), in streaming form: layers (thk, vp, vs, rho as float32
-// values) are pushed top to bottom and flattened as soon as they are formed (no
-// read-back of the column from memory) into (d, a, b, rho) rounded to float32
-// as the reference stores them, while the start-velocity state is tracked.
-struct FlatState {
-  double dr = 0.0, r0 = 6370.0;
-  float betmx = -1.e20f, betmn = 1.e20f;
-  float amn = 0.0f, bmn = 0.0f;  // flattened a, b of the start-velocity layer
-  int jsol = 1;
-  int bad = 0;
-};
-
 // One layer (thk, vp, vs, rho as float32 values) flattened with its geometry
 // into (d, a, b, rho) rounded to float32 as the reference stores them; *ok = 0
 // for a non-finite or non-positive value (the half-space may have d = 0).
@@ -145,84 +92,11 @@ __device__ __forceinline__ double4 flat_layer(double4 l, bool half, const FlatGe
   return o;
 }
 
-// flat_push with the geometry of the layer given.
-__device__ double4 flat_push_g(FlatState& s, double4 l, int m, bool half, const FlatGeom& g,
-                               double* btp_m, double* btr_m) {
-#pragma clang fp contract(off)
-  bool ok;
-  const double4 o = flat_layer(l, half, g, &ok);
-  const float a = (float)o.y, b = (float)o.z;
-  if (btp_m) *btp_m = g.bt;
-  if (btr_m) *btr_m = g.btr;
-  if (m == 0) {
-    s.amn = a;
-    s.bmn = b;
-    if (o.z <= 0.0) s.bad = 1;
-  }
-  if (b > 0.01f && b < s.betmn) {
-    s.betmn = b;
-    s.amn = a;
-    s.bmn = b;
-    s.jsol = 1;
-  } else if (b <= 0.01f && a < s.betmn) {
-    s.betmn = a;
-    s.amn = a;
-    s.bmn = b;
-    s.jsol = 0;
-  }
-  if (b > s.betmx) s.betmx = b;
-  if (!ok) s.bad = 1;
-  return o;
-}
-
-__device__ double4 flat_push(FlatState& s, double4 l, int m, int L, double* btp_m,
-                             double* btr_m) {
-#pragma clang fp contract(off)
-  const float thk = (m == L - 1) ? 1.0f : (float)l.x;
-  s.dr = s.dr + (double)thk;
-  const FlatGeom g = flat_geom(s.r0, s.dr, m == L - 1);
-  s.r0 = 6370.0 - s.dr;
-  return flat_push_g(s, l, m, m == L - 1, g, btp_m, btr_m);
-}
-
-__device__ void flat_finish(const FlatState& s, double* cc_out, float* betmx_out, int* bad) {
-#pragma clang fp contract(off)
-  if (s.jsol == 0 || s.bad) *bad = 1;
-  float cc1 = gtsolh_f32(s.amn, s.bmn);
-  cc1 = 0.95f * cc1;
-  cc1 = 0.90f * cc1;
-  if (!(cc1 > 0.0f) || !isfinite(cc1)) *bad = 1;
-  *cc_out = (double)cc1;
-  *betmx_out = s.betmx;
-}
-
-// The same flattening carried out in float64 for float64 models (the
-// differentiable path); the start velocity follows the float32 rule.
-__device__ void flatten_column_f64(double4* lay, double* btp_out, double* btr_out, int L,
-                                   double* cc_out, float* betmx_out, int* bad) {
-  const double ar = 6370.0;
-  double dr = 0.0, r0 = ar;
-  for (int m = 0; m < L; m++) {
-    double4 l = lay[m];
-    double thk = (m == L - 1) ? 1.0 : l.x;
-    dr = dr + thk;
-    double r1 = ar - dr;
-    double d = (m == L - 1) ? 0.0 : ar * log(ar / r1) - ar * log(ar / r0);
-    double tmp = (ar + ar) / (r0 + r1);
-    r0 = r1;
-    const double tr = pow(tmp, -2.275);
-    lay[m] = make_double4(d, l.y * tmp, l.z * tmp, l.w * tr);
-    if (btp_out) btp_out[m] = tmp;
-    if (btr_out) btr_out[m] = tr;
-  }
-  start_velocity(lay, L, cc_out, betmx_out, bad);
-}
-
 // Flattening geometry and node mapping of the refined layers of a node model,
 // which every column shares (refineLayerMdl, CalSurfGAniso_Joint.f90:146-196,
 // then sphere and the TI factors): thread m runs the sublayer rule down to
 // layer m and sums the thicknesses in the order a per-column pass would, so
-// every value is the one flat_push (and ti_geom's recurrence) computes.
+// every value is the one a per-column pass (and ti_geom's recurrence) computes.
 __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk0, int L,
                             FlatGeom* __restrict__ geo, float* __restrict__ geo_depz) {
 #pragma clang fp contract(off)
@@ -280,7 +154,7 @@ __global__ void k_prep_geom(int nz, const float* __restrict__ depz, float minthk
 // k_prep_geom's records of the same depz, minthk0 and L.  One wavefront per
 // column, lane = layer: each lane interpolates its layer between its two
 // nodes and flattens it; the start-velocity layer (the first layer of least
-// b, or a where b <= 0.01: the layer the sequential rule of flat_push keeps),
+// b, or a where b <= 0.01: the layer surfdisp96's sequential rule keeps),
 // betmx and the bad flags are wave reductions.
 __global__ void __launch_bounds__(256) k_prep_nodes(
     int ncol, int nz, const float* __restrict__ vel, const float* __restrict__ vpin,
@@ -385,34 +259,138 @@ __global__ void __launch_bounds__(256) k_prep_nodes(
   }
 }
 
+// Float64 flattening of one layer (the differentiable path): radii r0 = 6370 -
+// (thickness sum above the layer) and r1 = 6370 - (the sum down to its bottom).
+__device__ __forceinline__ double4 flat_layer_f64(double4 l, double r0, double r1, bool half,
+                                                  double* tmp_out, double* tr_out) {
+  const double ar = 6370.0;
+  const double d = half ? 0.0 : ar * log(ar / r1) - ar * log(ar / r0);
+  const double tmp = (ar + ar) / (r0 + r1);
+  const double tr = pow(tmp, -2.275);
+  *tmp_out = tmp;
+  *tr_out = tr;
+  return make_double4(d, l.y * tmp, l.z * tmp, l.w * tr);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int j) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, j);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 // Layered models (ragged): thk/vp/vs/rho concatenated at loff[col], float32
-// (reference semantics) or, when m64 is set, float64 arrays flattened in float64.
-__global__ void k_prep_layers(int ncol, const int* __restrict__ Lc, const int64_t* __restrict__ loff,
-                              const float* __restrict__ thk, const float* __restrict__ vp,
-                              const float* __restrict__ vs, const float* __restrict__ rho,
-                              const double* __restrict__ thk64, const double* __restrict__ vp64,
-                              const double* __restrict__ vs64, const double* __restrict__ rho64,
-                              double4* __restrict__ layers, double* __restrict__ btp,
-                              double* __restrict__ btr, double* __restrict__ cc,
-                              float* __restrict__ betmx, int* __restrict__ bad_out) {
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncol) return;
-  int L = Lc[col];
-  int64_t o = loff[col];
-  double4* lay = layers + o;
-  int bad = (L < 1);
-  if (thk64) {
-    for (int m = 0; m < L; m++)
-      lay[m] = make_double4(thk64[o + m], vp64[o + m], vs64[o + m], rho64[o + m]);
-    if (L >= 1) flatten_column_f64(lay, btp + o, btr + o, L, cc + col, betmx + col, &bad);
-  } else if (L >= 1) {
-    FlatState fs;
-    for (int m = 0; m < L; m++)
-      lay[m] = flat_push(fs, make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]), m, L,
-                         btp + o + m, btr + o + m);
-    flat_finish(fs, cc + col, betmx + col, &bad);
+// (reference semantics) or, when thk64 is set, float64 arrays flattened in
+// float64 (sphere, surfdisp96.f:480-547, then the start-velocity rule,
+// :139-207).  One wavefront per column, lane = layer, 64 layers at a time:
+// lane m adds the thicknesses of layers 0..m one by one in layer order (the
+// sequential recurrence's sums, bit for bit), flattens its layer, and the
+// start-velocity layer (the first layer of least b, or a where b <= 0.01),
+// betmx and the bad flags are wave reductions, as in k_prep_nodes.
+__global__ void __launch_bounds__(256) k_prep_layers(
+    int ncol, const int* __restrict__ Lc, const int64_t* __restrict__ loff,
+    const float* __restrict__ thk, const float* __restrict__ vp, const float* __restrict__ vs,
+    const float* __restrict__ rho, const double* __restrict__ thk64,
+    const double* __restrict__ vp64, const double* __restrict__ vs64,
+    const double* __restrict__ rho64, double4* __restrict__ layers, double* __restrict__ btp,
+    double* __restrict__ btr, double* __restrict__ cc, float* __restrict__ betmx,
+    int* __restrict__ bad_out) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (blockDim.x >> 6);
+  const bool f64 = thk64 != nullptr;
+  for (int col = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); col < ncol; col += nw) {
+    const int L = Lc[col];
+    const int64_t o = loff[col];
+    if (L < 1) {
+      if (lane == 0) bad_out[col] = 1;
+      continue;
+    }
+    double carry = 0.0;  // thickness sum of the layers of the previous chunks
+    float kmin = __builtin_inff(), amn = 0.0f, bmn = 0.0f, bmx = -1.e20f;
+    bool have = false, bad = false;
+    for (int m0 = 0; m0 < L; m0 += 64) {
+      const int m = m0 + lane;
+      const bool in = m < L, half = (m == L - 1);
+      double4 l = make_double4(0.0, 0.0, 0.0, 0.0);
+      if (in)
+        l = f64 ? make_double4(thk64[o + m], vp64[o + m], vs64[o + m], rho64[o + m])
+                : make_double4(thk[o + m], vp[o + m], vs[o + m], rho[o + m]);
+      // the half-space enters the sum with a unit thickness
+      const double t = !in ? 0.0 : f64 ? (half ? 1.0 : l.x) : (double)(half ? 1.0f : (float)l.x);
+      const int n = min(64, L - m0);
+      double acc = carry, accp = carry;
+      for (int jj = 0; jj < n; jj++) {
+        const double tj = readlane_f64(t, jj);
+        if (jj == lane) accp = acc;
+        if (jj <= lane) acc = acc + tj;
+      }
+      carry = readlane_f64(acc, n - 1);
+      bool cand = false, isbad = false;
+      float key = __builtin_inff(), a = 0.0f, b = -1.e20f;
+      if (in) {
+        double4 f;
+        double bt, br;
+        bool ok;
+        if (f64) {
+          f = flat_layer_f64(l, 6370.0 - accp, 6370.0 - acc, half, &bt, &br);
+          ok = isfinite(f.x) && isfinite(f.y) && isfinite(f.z) && isfinite(f.w) && f.y > 0.0 &&
+               f.z > 0.0 && f.w > 0.0 && (f.x > 0.0 || half);
+        } else {
+          const FlatGeom g = flat_geom(6370.0 - accp, acc, half);
+          f = flat_layer(l, half, g, &ok);
+          bt = g.bt;
+          br = g.btr;
+        }
+        layers[o + m] = f;
+        btp[o + m] = bt;
+        btr[o + m] = br;
+        a = (float)f.y;
+        b = (float)f.z;
+        isbad = !ok || (m == 0 && f.z <= 0.0);
+        if (b > 0.01f) {
+          key = b;
+          cand = true;
+        } else if (b <= 0.01f) {
+          key = a;
+          cand = true;
+        }
+        cand = cand && (key < 1.e20f);
+        if (!cand) key = __builtin_inff();
+      }
+      if (m0 == 0) {  // the rule starts from layer 0's values
+        amn = __shfl(a, 0, 64);
+        bmn = __shfl(b, 0, 64);
+      }
+      float kw = key, bw = b;
+#pragma unroll
+      for (int sh = 1; sh < 64; sh <<= 1) {
+        kw = fminf(kw, __shfl_xor(kw, sh, 64));
+        bw = fmaxf(bw, __shfl_xor(bw, sh, 64));
+      }
+      bmx = fmaxf(bmx, bw);
+      bad = bad || (__ballot(isbad) != 0ull);
+      const uint64_t at = __ballot(cand && key == kw);
+      if (at != 0ull && (!have || kw < kmin)) {
+        const int jm = __ffsll((unsigned long long)at) - 1;
+        kmin = kw;
+        amn = __shfl(a, jm, 64);
+        bmn = __shfl(b, jm, 64);
+        have = true;
+      }
+    }
+    // a fluid start layer (jsol = 0) is outside this stage's model class
+    const bool jsol0 = have && !(bmn > 0.01f);
+    float cc1 = gtsolh_f32(amn, bmn);
+    cc1 = 0.95f * cc1;
+    cc1 = 0.90f * cc1;
+    if (!(cc1 > 0.0f) || !isfinite(cc1)) bad = true;
+    if (lane == 0) {
+      cc[col] = (double)cc1;
+      betmx[col] = bmx;
+      bad_out[col] = (bad || jsol0) ? DK_PREP_BADMODEL : 0;
+    }
   }
-  bad_out[col] = bad;
 }
 
 // ------------------------------------------------------------------ forward root search
@@ -1823,9 +1801,9 @@ hipError_t launch_prep_layers(int ncol, const int* Lc, const int64_t* loff, cons
                               const double* thk64, const double* vp64, const double* vs64,
                               const double* rho64, double4* layers, double* btp, double* btr,
                               double* cc, float* betmx, int* bad, hipStream_t st) {
-  int block = 128;
-  int grid = (ncol + block - 1) / block;
-  hipLaunchKernelGGL(k_prep_layers, dim3(grid), dim3(block), 0, st, ncol, Lc, loff, thk, vp, vs,
+  // four columns per 256-thread workgroup (one per wavefront)
+  const int grid = (ncol + 3) / 4;
+  hipLaunchKernelGGL(k_prep_layers, dim3(grid), dim3(256), 0, st, ncol, Lc, loff, thk, vp, vs,
                      rho, thk64, vp64, vs64, rho64, layers, btp, btr, cc, betmx, bad);
   return hipGetLastError();
 }
