@@ -1242,14 +1242,16 @@ k_kernels(BatchDev B, KernDev O, double* tape, int nq) {
 // ------------------------------------------------------------------ VJP kernel
 // grad_m = sum_k grad_c[s(col,k)] dc_k/dm for the unflattened layer vs, vp, rho
 // of a ragged batch (outputs laid out like the model, at loff[col] + m).
-// One wavefront per column, lane k owning period k (columns with more than 64
-// periods take several chunks).  Sweep 1 carries the tangent dF/dc with the
-// tape; sweep 2 is the adjoint with (alpha, beta, rho) duals, already scaled by
-// -grad_c/(dF/dc); per-layer sums reduce across lanes with shuffles, so the
-// result does not depend on scheduling.
-__device__ __forceinline__ double wave_sum(double v) {
+// Two columns per wavefront, one per half (VJP_LANES lanes), lane k of a half
+// owning period k (columns with more than VJP_LANES periods take several
+// chunks).  Sweep 1 carries the tangent dF/dc with the tape; sweep 2 is the
+// adjoint with (alpha, beta, rho) duals, already scaled by -grad_c/(dF/dc);
+// per-layer sums reduce across the half's lanes with shuffles, so the result
+// does not depend on scheduling or on the other columns of the batch.
+constexpr int VJP_LANES = 32;
+__device__ __forceinline__ double half_sum(double v) {
 #pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  for (int o = VJP_LANES / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
   return v;
 }
 
@@ -1259,10 +1261,12 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
                                             const float* __restrict__ thk32,
                                             const double* __restrict__ thk64,
                                             double* __restrict__ tape) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & (VJP_LANES - 1);
+  const int half = threadIdx.x / VJP_LANES;
+  constexpr int CPW = 64 / VJP_LANES;  // columns per wavefront
   const long nslot = (long)gridDim.x * 64;
-  const long slot = (long)blockIdx.x * 64 + lane;
-  for (int col = blockIdx.x; col < B.ncol; col += gridDim.x) {
+  const long slot = (long)blockIdx.x * 64 + threadIdx.x;
+  for (int col = blockIdx.x * CPW + half; col < B.ncol; col += gridDim.x * CPW) {
     int L, K;
     int64_t loff, toff;
     col_geom(B, col, &L, &loff, &K, &toff);
@@ -1270,7 +1274,7 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
     const double* btp = B.btp + loff;
     const double* btr = B.btr + loff;
     // a column without periods still writes its (zero) gradients
-    for (int k0 = 0; k0 == 0 || k0 < K; k0 += 64) {
+    for (int k0 = 0; k0 == 0 || k0 < K; k0 += VJP_LANES) {
       const int k = k0 + lane;
       double c = 0.0, gc = 0.0;
       if (k < K) {
@@ -1397,11 +1401,11 @@ __global__ void __launch_bounds__(64) k_vjp(BatchDev B, const double* __restrict
           part[1] *= bt;
           part[2] *= btr_m;
         }
-        const double sa = wave_sum(part[0]), sb = wave_sum(part[1]), sr = wave_sum(part[2]);
+        const double sa = half_sum(part[0]), sb = half_sum(part[1]), sr = half_sum(part[2]);
         double s0 = 0.0, s1 = 0.0;
         if (gthk) {
-          s0 = wave_sum(R0);
-          s1 = wave_sum(R1);
+          s0 = half_sum(R0);
+          s1 = half_sum(R1);
         }
         if (lane == 0) {
           const int64_t o = loff + m;
