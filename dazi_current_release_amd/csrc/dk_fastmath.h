@@ -19,7 +19,10 @@ namespace dk {
 // register, one extra move per Horner step).  fma3 takes the constant addend
 // c from a VGPR pair, fma3s from an SGPR pair (the one scalar operand a VALU
 // instruction may read): SGPR constants free vector registers, at the cost of
-// scalar moves where the scalar registers run out.
+// scalar moves where the scalar registers run out.  The full-precision exp and
+// sin/cos (kernel, VJP and TI passes, which sit at their two-wave register
+// limit) take every coefficient from SGPRs; the root search's shorter sin/cos
+// keeps its coefficients in VGPRs.
 DK_HD double fma3(double a, double b, double c) {
 #if defined(__HIP_DEVICE_COMPILE__)
   double d;
@@ -75,20 +78,20 @@ DK_HD void sincos_phase(double p, double* sp, double* cp) {
   r = fma(n, -2.02226624879595063154e-21, r);
   const double r2 = r * r;
   double ps = 7.6471637318198164759e-13;               // 1/15!
-  ps = fma3(ps, r2, -1.6059043836821614599e-10);        // -1/13!
-  ps = fma3(ps, r2, 2.5052108385441718775e-08);         // 1/11!
-  ps = fma3(ps, r2, -2.7557319223985892511e-06);        // -1/9!
-  ps = fma3(ps, r2, 1.9841269841269841253e-04);         // 1/7!
-  ps = fma3(ps, r2, -8.3333333333333332177e-03);        // -1/5!
-  ps = fma3(ps, r2, 1.6666666666666665741e-01);         // 1/3!, subtracted below
+  ps = fma3s(ps, r2, -1.6059043836821614599e-10);        // -1/13!
+  ps = fma3s(ps, r2, 2.5052108385441718775e-08);         // 1/11!
+  ps = fma3s(ps, r2, -2.7557319223985892511e-06);        // -1/9!
+  ps = fma3s(ps, r2, 1.9841269841269841253e-04);         // 1/7!
+  ps = fma3s(ps, r2, -8.3333333333333332177e-03);        // -1/5!
+  ps = fma3s(ps, r2, 1.6666666666666665741e-01);         // 1/3!, subtracted below
   const double sn = fma(-r * r2, ps, r);
   double pc = 4.7794773323873852974e-14;               // 1/16!
-  pc = fma3(pc, r2, -1.1470745597729724714e-11);        // -1/14!
-  pc = fma3(pc, r2, 2.0876756987868098979e-09);         // 1/12!
-  pc = fma3(pc, r2, -2.7557319223985890653e-07);        // -1/10!
-  pc = fma3(pc, r2, 2.4801587301587301566e-05);         // 1/8!
-  pc = fma3(pc, r2, -1.3888888888888889419e-03);        // -1/6!
-  pc = fma3(pc, r2, 4.1666666666666664354e-02);         // 1/4!
+  pc = fma3s(pc, r2, -1.1470745597729724714e-11);        // -1/14!
+  pc = fma3s(pc, r2, 2.0876756987868098979e-09);         // 1/12!
+  pc = fma3s(pc, r2, -2.7557319223985890653e-07);        // -1/10!
+  pc = fma3s(pc, r2, 2.4801587301587301566e-05);         // 1/8!
+  pc = fma3s(pc, r2, -1.3888888888888889419e-03);        // -1/6!
+  pc = fma3s(pc, r2, 4.1666666666666664354e-02);         // 1/4!
   pc = fma(pc, r2, -0.5);
   const double cs = fma(pc, r2, 1.0);
   const int q = (int)n & 3;
