@@ -32,3 +32,10 @@ extern "C" double dk_host_kernels(double c, double period, int L, const float* d
   for (int m = 0; m < L; m++) lay[m] = make_double4(d[m], a[m], b[m], rho[m]);
   return dk::fdm_kernels_ref(c, 2.0 * 3.141592653589793 / period, L, lay.data(), out, ex);
 }
+
+// The root search's shorter elementary functions (dk_fastmath.h), for the CPU
+// test of their error bounds.
+extern "C" double dk_host_exp_neg_rs(double p) { return dk::exp_neg_rs(p); }
+extern "C" void dk_host_sincos_phase_rs(double p, double* s, double* c) {
+  dk::sincos_phase_rs(p, s, c);
+}

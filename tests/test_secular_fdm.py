@@ -225,3 +225,41 @@ def test_kernels_match_dltar4_differences_at_its_roots(hk):
         assert err <= 2e-5, (n, T, c, err)
         checked += 1
     assert checked >= 25
+
+
+def test_root_search_elementary_functions_error_bounds():
+    """The root search's exp(-p) and sin/cos of the layer phase (near-minimax
+    polynomials on the reduced intervals) against long-double references:
+    exp within 2e-12 relative on p in [0, 700], sin and cos within 2e-11
+    absolute on p in [0, 2e4] (the phases a layer takes), and both exactly
+    their Taylor limits at 0."""
+    if not os.path.exists(LIB):
+        from dazi_current_release_amd import build as B
+        B.build_hostcheck()
+    lib = C.CDLL(LIB)
+    ex = lib.dk_host_exp_neg_rs
+    ex.restype = C.c_double
+    ex.argtypes = [C.c_double]
+    sc = lib.dk_host_sincos_phase_rs
+    sc.argtypes = [C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    rng = np.random.default_rng(7)
+    L = np.longdouble
+    p = np.concatenate([np.linspace(0.0, 3.0, 3001), rng.uniform(0.0, 700.0, 20000)])
+    got = np.array([ex(float(v)) for v in p])
+    want = np.exp(-p.astype(L))
+    assert float(np.max(np.abs((got - want) / want))) < 2e-12
+    assert ex(0.0) == 1.0
+    q = np.concatenate([np.linspace(0.0, 7.0, 7001), rng.uniform(0.0, 2e4, 20000)])
+    s_, c_ = C.c_double(), C.c_double()
+    gs, gc = [], []
+    for v in q:
+        sc(float(v), C.byref(s_), C.byref(c_))
+        gs.append(s_.value)
+        gc.append(c_.value)
+    # long double sin/cos of a double argument: the reduction's own error is
+    # part of what is bounded
+    ql = q.astype(L)
+    assert float(np.max(np.abs(np.array(gs) - np.sin(ql)))) < 2e-11
+    assert float(np.max(np.abs(np.array(gc) - np.cos(ql)))) < 2e-11
+    sc(0.0, C.byref(s_), C.byref(c_))
+    assert s_.value == 0.0 and c_.value == 1.0
