@@ -54,7 +54,8 @@ enum Slot {
   S_STATUS,
   S_WORK,
   S_TAPE,
-  S_ORDER,   // ragged batches: columns in scheduling order, then the sort's counters
+  S_ORDER,   // ragged batches: columns in scheduling order, their first solve positions,
+             // then the sort's counters
   S_SORDER,  // ragged batches: solves in scheduling order
   S_H_VEL,
   S_H_VP,
@@ -1480,7 +1481,7 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   // scheduling order: columns (and, for the kernel pass, solves) by layer
   // count, largest first
   void *order, *sorder = nullptr;
-  if (int rc = pool_get(P, S_ORDER, ((size_t)b->ncol + dk::order_work_ints()) * 4, &order))
+  if (int rc = pool_get(P, S_ORDER, (2 * (size_t)b->ncol + dk::order_work_ints()) * 4, &order))
     return rc;
   if (kern && b->nsolves > 0) {
     if (int rc = pool_get(P, S_SORDER, (size_t)b->nsolves * 4, &sorder)) return rc;
@@ -1493,8 +1494,10 @@ int32_t dazi_dk_layered_dev(const dazi_dk_layered* b, const dazi_dk_tune* tune, 
   HIPCHK(dk::launch_prep_layers(b->ncol, b->nl, b->loff, b->thk, b->vp, b->vs, b->rho,
                                 b->thk64, b->vp64, b->vs64, b->rho64, (double4*)layers,
                                 (double*)btp, btr, (double*)cc, (float*)betmx, (int*)bad, st));
-  HIPCHK(dk::launch_order(b->ncol, b->max_layers, b->nl, b->nk, b->toff, b->nsolves, (int*)order,
-                          (int*)sorder, (int*)order + b->ncol, st));
+  // S_ORDER: the column order, each column's first solve position, the counters
+  HIPCHK(dk::launch_order(b->ncol, b->max_layers, b->nl, b->nk, b->toff, b->solve_col, b->nsolves,
+                          (int*)order, (int*)sorder, (int*)order + b->ncol,
+                          (int*)order + 2 * (size_t)b->ncol, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = b->ncol;

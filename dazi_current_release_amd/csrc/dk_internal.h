@@ -139,12 +139,13 @@ hipError_t launch_zero_int(int* p, int n, hipStream_t st);
 // count (those from ORDER_BINS - 1 up share the last value) and, where the
 // keys leave room for it below max_layers, up to 4 bits of the period count;
 // work holds order_work_ints() ints.  The order within a key is unspecified
-// (no result depends on it).
+// (no result depends on it).  The solve order needs solve_col and spos, ncol
+// ints of work (each column's first position in it).
 constexpr int ORDER_BINS = 4096;
 inline int order_work_ints() { return 4 * ORDER_BINS; }
 hipError_t launch_order(int ncol, int max_layers, const int* Lc, const int* Kc,
-                        const int64_t* toff, int64_t nsolves, int* order, int* solve_order,
-                        int* work, hipStream_t st);
+                        const int64_t* toff, const int* solve_col, int64_t nsolves, int* order,
+                        int* solve_order, int* spos, int* work, hipStream_t st);
 // G-row assembly (CalSurfGAniso_Joint.f90:750-797), device pointers; see
 // dazi_dk_grows_dev.  row_nnz, row_off: per ray; nar: one int64.
 struct GrowDev {

@@ -1693,9 +1693,8 @@ __global__ void __launch_bounds__(ORDER_SCAN) k_order_scan(int* work) {
 // reservation per (workgroup, key)
 constexpr int ORDER_ITEMS = 4;
 __global__ void __launch_bounds__(256) k_order_scatter(int ncol, const int* Lc, const int* Kc,
-                                                       int kb, const int64_t* toff,
-                                                       int64_t nsolves, int* order,
-                                                       int* solve_order, int* work) {
+                                                       int kb, int* order, int* spos,
+                                                       int* work) {
   __shared__ int pc[ORDER_BINS], ps[ORDER_BINS];
   for (int b = threadIdx.x; b < ORDER_BINS; b += blockDim.x) pc[b] = ps[b] = 0;
   __syncthreads();
@@ -1724,12 +1723,20 @@ __global__ void __launch_bounds__(256) k_order_scatter(int ncol, const int* Lc, 
     if (key[i] < 0) continue;
     const int c = c0 + i * blockDim.x;
     order[pc[key[i]] + oc[i]] = c;
-    if (solve_order) {
-      const int64_t p0 = (int64_t)ps[key[i]] + os[i];
-      const int64_t t0 = toff[c];
-      for (int k = 0; k < kk[i] && p0 + k < nsolves; k++) solve_order[p0 + k] = (int)(t0 + k);
-    }
+    if (spos) spos[c] = ps[key[i]] + os[i];  // the column's first solve in the order
   }
+}
+
+// solve_order[spos[col] + k] = s for every solve s = toff[col] + k (one thread
+// per solve; positions of an inconsistent nk, toff beyond nsolves are skipped)
+__global__ void __launch_bounds__(256) k_order_solves(int64_t nsolves, const int* solve_col,
+                                                      const int64_t* toff, const int* spos,
+                                                      int* solve_order) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsolves) return;
+  const int col = solve_col[s];
+  const int64_t p = (int64_t)spos[col] + (s - toff[col]);
+  if (p >= 0 && p < nsolves) solve_order[p] = (int)s;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -1831,8 +1838,8 @@ hipError_t launch_zero_int(int* p, int n, hipStream_t st) {
 }
 
 hipError_t launch_order(int ncol, int max_layers, const int* Lc, const int* Kc,
-                        const int64_t* toff, int64_t nsolves, int* order, int* solve_order,
-                        int* work, hipStream_t st) {
+                        const int64_t* toff, const int* solve_col, int64_t nsolves, int* order,
+                        int* solve_order, int* spos, int* work, hipStream_t st) {
   if (ncol <= 0) return hipSuccess;
   // period-count bits of the key: as many (up to 4) as leave a key for every
   // layer count up to max_layers
@@ -1844,8 +1851,12 @@ hipError_t launch_order(int ncol, int max_layers, const int* Lc, const int* Kc,
   hipLaunchKernelGGL(k_order_hist, dim3(hist_grid), dim3(256), 0, st, ncol, Lc, Kc, kb, work);
   hipLaunchKernelGGL(k_order_scan, dim3(1), dim3(ORDER_SCAN), 0, st, work);
   const int chunk = 256 * ORDER_ITEMS;
+  const bool solves = solve_order && solve_col && nsolves > 0;
   hipLaunchKernelGGL(k_order_scatter, dim3((ncol + chunk - 1) / chunk), dim3(256), 0, st, ncol,
-                     Lc, Kc, kb, toff, nsolves, order, solve_order, work);
+                     Lc, Kc, kb, order, solves ? spos : nullptr, work);
+  if (solves)
+    hipLaunchKernelGGL(k_order_solves, dim3((unsigned)((nsolves + 255) / 256)), dim3(256), 0, st,
+                       nsolves, solve_col, toff, spos, solve_order);
   return hipGetLastError();
 }
 
