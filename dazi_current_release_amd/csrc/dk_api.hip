@@ -109,9 +109,8 @@ struct Pool {
   size_t n[S_COUNT] = {};
   int ncu = 0;
   hipEvent_t ev[8] = {};
-  // side stream for work that runs beside the caller's stream (the TI model
-  // prep of a stage call, beside the root search), joined back with events;
-  // default priority
+  // side stream for work that runs beside the caller's stream (the output DMA
+  // of the host entry points), joined back with events; default priority
   hipStream_t side = nullptr;
   hipEvent_t fk[4] = {};
   int nforward = 0;
@@ -308,10 +307,10 @@ int forward_launches(const dk::BatchDev& B, const dazi_dk_tune& T0, int lmax, in
     HIPCHK(dk::launch_forward(B, S, 0, B.ncol, cpw, 1, grid, lds_stride, st));
     *nlaunch = 1;
   } else {
-    for (int c0 = 0; c0 < B.ncol; c0 += T.tile_columns) {
-      int c1 = c0 + T.tile_columns;
-      if (c1 > B.ncol) c1 = B.ncol;
-      HIPCHK(dk::launch_forward(B, S, c0, c1, cpw, 0, 0, lds_stride, st));
+    // (64-bit bounds: a tile of the whole batch may be given as a huge count)
+    for (int64_t c0 = 0; c0 < B.ncol; c0 += T.tile_columns) {
+      const int64_t c1 = std::min<int64_t>(c0 + T.tile_columns, B.ncol);
+      HIPCHK(dk::launch_forward(B, S, (int)c0, (int)c1, cpw, 0, 0, lds_stride, st));
       ++*nlaunch;
     }
   }
@@ -667,14 +666,6 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
     HIPCHK(dk::launch_prep_geom(d->nz, depz_dev, d->minthk0, L, geo, (float*)geo_depz, st));
     G->built = true;
   }
-  if (ti) {
-    HIPCHK(hipEventRecord(P->fk[0], st));
-    HIPCHK(hipStreamWaitEvent(P->side, P->fk[0], 0));
-    HIPCHK(dk::launch_prep_ti(ncol, vel, d->vp_rho_mode == 1 ? vp : nullptr,
-                              d->vp_rho_mode == 1 ? rho : nullptr, L, geo,
-                              (dk::TiLayerDev*)til, (int*)tiq, P->side));
-    HIPCHK(hipEventRecord(P->fk[2], P->side));
-  }
   // a reused call keeps the flattened layers, start velocities and roots that
   // the previous host call left in these slots
   if (!reuse)
@@ -682,6 +673,12 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
                                  d->vp_rho_mode == 1 ? rho : nullptr, depz_dev, geo_depz, L, geo,
                                  (double4*)layers, (double*)btp, btr, (double*)cc, (float*)betmx,
                                  (int*)bad, st));
+  // the TI model prep on the same stream: a side stream joined by events
+  // left the device idle for about 90 us between calls
+  if (ti)
+    HIPCHK(dk::launch_prep_ti(ncol, vel, d->vp_rho_mode == 1 ? vp : nullptr,
+                              d->vp_rho_mode == 1 ? rho : nullptr, L, geo,
+                              (dk::TiLayerDev*)til, (int*)tiq, st));
   rec(P, 1, st);
   dk::BatchDev B = {};
   B.ncol = ncol;
@@ -719,8 +716,6 @@ int stage_impl(const dazi_dk_desc* d, const dazi_dk_tune* tune, const float* vel
   }
   rec(P, 3, st);
   if (ti) {
-    // join the TI model prep, then the TI kernels (the roots are ready)
-    HIPCHK(hipStreamWaitEvent(st, P->fk[2], 0));
     HIPCHK(dk::launch_ti(ncol, d->kmax, L, d->nz, (const dk::TiLayerDev*)til, tRc, pv,
                          (const int*)nsubd, lsen, dcdA, dcdL, (double*)tiscr, ti_nthr, (int*)tiq,
                          (d->flags & DAZI_DK_TI_FAILED_NAN) != 0, st));
