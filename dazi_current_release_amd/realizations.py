@@ -65,8 +65,11 @@ def cell_gradient(grad_vs, layer_of_node, cell, ncells, c0, n, ncol_real, acc):
     import torch
     nl = acc.shape[1]
     g = grad_vs[:, :n]
-    gl = torch.zeros((nl, n), dtype=torch.float64, device=g.device)
-    gl.index_add_(0, layer_of_node, g)
+    # node -> layer sums as a product with the (nl, nz) 0/1 membership matrix
+    # (a GEMM in a fixed order, where an index_add over the columns was an
+    # atomic scatter of nz x n values)
+    member = torch.nn.functional.one_hot(layer_of_node, nl).to(torch.float64).t().contiguous()
+    gl = member @ g
     cols = torch.arange(c0, c0 + n, device=g.device)
     r, j = cols // ncol_real, cols % ncol_real
     idx = (r[None, :] * nl + torch.arange(nl, device=g.device)[:, None]) * ncells + \
