@@ -1,10 +1,12 @@
 // HIP kernels of the depth-kernel stage for gfx950 (CDNA4).
 //
 //   k_prep_nodes   node model (vel, depz, minthk) -> refined, earth-flattened
-//                  layered model + start velocity; one thread per column.
+//                  layered model + start velocity; one wavefront per column.
 //                  (CalSurfG.f90:2490-2549, DepthKernel1D.f90:187-200,
 //                   surfdisp96.f:99-207 with sphere :480-547, gtsolh :361-382)
 //   k_prep_layers  the same for layered (ragged) models.
+//   k_order_*      the scheduling order of a ragged batch (a counting sort of
+//                  its columns and solves by layer count, then period count).
 //   k_forward<S>   Rayleigh fundamental-mode root search: S lanes of a wave own
 //                  one column and walk its periods in order, reproducing the
 //                  reference's scan grid, direction, floor and failure rules
@@ -16,12 +18,13 @@
 //                  multisection, and placed by a final secant step.
 //   k_kernels      per (column, period) implicit differentiation of the
 //                  secular function at the root, dc/dm = -(dF/dm)/(dF/dc):
-//                  a bottom-up sweep with the tangent in c and a tape, and a
-//                  hand-written adjoint sweep of the potential-basis
-//                  recursion (dk_fdm.h); maps refined-layer partials to the
-//                  node or layer outputs.
-//   k_vjp          the same sweeps per (column, period), contracted with
-//                  grad_c over a column's periods into layer gradients.
+//                  a bottom-up sweep with the tangent in c and checkpoints,
+//                  and a hand-written adjoint sweep of the potential-basis
+//                  recursion (dk_fdm.h) over segments recomputed into LDS;
+//                  maps refined-layer partials to the node or layer outputs.
+//   k_vjp          the same sweeps per (column, period), with a per-layer
+//                  tape, contracted with grad_c over a column's periods into
+//                  layer gradients (two columns per wavefront).
 //
 // All secular-function arithmetic is FP64 on the vector ALUs; nothing here is
 // a dense contraction, so MFMA is not used.

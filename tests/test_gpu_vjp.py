@@ -63,7 +63,7 @@ def test_vjp_equals_kernel_contraction(dk):
 
 
 def test_vjp_ragged_long_and_empty_columns(dk):
-    """Columns with more than 64 periods (several lane chunks), columns with no
+    """Columns with more than 64 periods (several 32-lane chunks), columns with no
     periods (zero gradients) and float32 models."""
     import torch
     from dazi_current_release_amd import configs
