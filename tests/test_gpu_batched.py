@@ -207,3 +207,22 @@ def test_graph_replay_with_changed_depz_reports_geometry_status(dk):
     torch.cuda.synchronize()
     for k in want:
         assert torch.equal(out[k], want[k]), k
+
+
+def test_launch_mode_and_shape_do_not_change_results(dk):
+    """At one lanes-per-scan value every column runs the same sequence of
+    operations whatever the launch: persistent or one launch over the batch,
+    any columns per workgroup, tiles of any size.  Roots, status and
+    refined-layer kernels are bit-identical."""
+    from dazi_current_release_amd import batched, configs
+    W = configs.layered_workload("ragged_16384", ncol=3000, seed=12)
+    b = dk.LayeredBatch(W["nl"], W["nk"], W["thk"], W["vp"], W["vs"], W["rho"], W["periods"])
+    runs = {}
+    for lanes, cpw, mode, tile in ((16, 4, 0, 64), (16, 16, 0, 64), (16, 4, 1, batched.ONE_LAUNCH),
+                                   (16, 8, 1, 257)):
+        o = dk.layered_dev(b, kernels=True, tune=dk.make_tune(lanes, cpw, mode, tile, 2e-6))
+        runs[(lanes, cpw, mode, tile)] = {k: v.cpu().numpy() for k, v in o.items()}
+    ref = runs[(16, 4, 0, 64)]
+    for key, o in runs.items():
+        for k in ("c", "c_unrounded", "status", "dc_dvs", "dc_dvp", "dc_drho"):
+            assert np.array_equal(o[k], ref[k]), (key, k)
