@@ -25,6 +25,7 @@ EXPORTS = [
     "dazi_dk_iso", "dazi_dk_iso_dev", "dazi_dk_layered_dev", "dazi_dk_last_timings",
     "dazi_dk_enable_timing", "dazi_dk_ti", "dazi_dk_ti_dev", "dazi_dk_stage_dev",
     "dazi_dk_layered_vjp_dev", "dazi_dk_reset_geometry", "dazi_dk_effective_dev",
+    "dazi_dk_cell_sum_dev",
     "dazi_dk_pool_bytes", "dazi_dk_grows_dev", "dazi_dk_grows", "dazi_dk_azimuth_map_dev",
     "dazi_dk_azimuth_map", "dazi_dk_travel_plan_create", "dazi_dk_travel_run_dev",
     "dazi_dk_travel_plan_destroy", "dazi_dk_travel", "dazi_dk_layered_noalloc_dev",
@@ -116,6 +117,9 @@ def lib():
         L.dazi_dk_enable_timing.argtypes = [C.c_int32]
         L.dazi_dk_reset_geometry.argtypes = []
         L.dazi_dk_effective_dev.argtypes = [C.POINTER(Desc)] + [C.c_void_p] * 8
+        L.dazi_dk_cell_sum_dev.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_int64,
+                                           C.c_int64, C.c_int32, C.c_int32, C.c_void_p,
+                                           C.c_int32, C.c_void_p, C.c_void_p]
         L.dazi_dk_pool_bytes.argtypes = [C.POINTER(C.c_int64)]
         L.dazi_dk_grows_dev.argtypes = ([C.POINTER(GrowDesc)] + [C.c_void_p] * 11 +
                                         [C.c_int64] + [C.c_void_p] * 6)

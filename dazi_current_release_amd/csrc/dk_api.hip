@@ -916,6 +916,25 @@ int32_t dazi_dk_effective_dev(const dazi_dk_desc* d, const float* vel, const dou
   return 0;
 }
 
+int32_t dazi_dk_cell_sum_dev(const double* gl, int64_t ld, int32_t nl, int64_t c0, int64_t n,
+                             int32_t ncol_real, int32_t nreal, const int64_t* cell,
+                             int32_t ncells, double* acc, void* stream) {
+  if (nl < 1 || ncol_real < 1 || nreal < 1 || ncells < 1 || c0 < 0 || n < 0 || ld < n)
+    return set_err(DAZI_DK_EINVAL, "bad sizes nl=%d ncol_real=%d nreal=%d ncells=%d c0=%lld n=%lld "
+                   "ld=%lld", nl, ncol_real, nreal, ncells, (long long)c0, (long long)n,
+                   (long long)ld);
+  if (n == 0) return 0;
+  if (!gl || !cell || !acc) return set_err(DAZI_DK_EINVAL, "NULL required argument");
+  if ((c0 + n - 1) / ncol_real >= nreal)
+    return set_err(DAZI_DK_EINVAL, "columns %lld..%lld reach beyond %d realizations of %d columns",
+                   (long long)c0, (long long)(c0 + n - 1), nreal, ncol_real);
+  const int64_t nr = (c0 + n - 1) / ncol_real - c0 / ncol_real + 1;
+  if (nr * nl > INT32_MAX) return set_err(DAZI_DK_ELIMIT, "too many (realization, layer) pairs");
+  HIPCHK(dk::launch_cell_sum(gl, ld, nl, c0, n, ncol_real, cell, ncells, acc,
+                             (hipStream_t)stream));
+  return 0;
+}
+
 int32_t dazi_dk_grows_dev(const dazi_dk_grow_desc* d, const int32_t* ray_k,
                           const int64_t* cell_off, const int32_t* cell, const float* fdm,
                           const float* fdmc, const float* fdms, const float* vel,

@@ -226,6 +226,9 @@ hipError_t launch_travel(const TravelDev& p, hipStream_t st);
 hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const double* svs,
                             const double* svp, const double* srho, const double* gc, double* eff,
                             double* grad, hipStream_t st);
+// Per-cell sums of a population gradient; see dazi_dk_cell_sum_dev.
+hipError_t launch_cell_sum(const double* gl, int64_t ld, int nl, int64_t c0, int64_t n, int ncr,
+                           const int64_t* cell, int ncells, double* acc, hipStream_t st);
 hipError_t launch_vjp(const BatchDev& B, const double* grad_c, double* gvs, double* gvp,
                       double* grho, double* gthk, const float* thk32, const double* thk64,
                       double* tape, int nblocks, hipStream_t st);

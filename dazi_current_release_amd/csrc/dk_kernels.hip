@@ -1613,6 +1613,55 @@ hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const dou
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ cell sums
+// Per-cell sums of a population gradient: acc[(r*nl + l)*ncells + cell[(r*nl +
+// l)*ncr + j]] += gl[l*ld + (c - c0)] over the chunk's columns c = r*ncr + j in
+// [c0, c0 + n).  One workgroup per (realization, layer) that the chunk touches;
+// thread t owns cells t, t + 256, ...  and scans the realization's (cell,
+// value) pairs from LDS in column order, so every sum runs in a fixed order
+// (no atomics, the same bits on every run).  Cell ids outside [0, ncells) add
+// nowhere.
+constexpr int CELL_TILE = 1024;
+__global__ void __launch_bounds__(256) k_cell_sum(const double* __restrict__ gl, int64_t ld,
+                                                  int nl, int64_t c0, int64_t n, int ncr,
+                                                  const int64_t* __restrict__ cell, int ncells,
+                                                  double* __restrict__ acc) {
+  __shared__ int cid[CELL_TILE];
+  __shared__ double val[CELL_TILE];
+  const int l = blockIdx.x % nl;
+  const int64_t r = c0 / ncr + blockIdx.x / nl;
+  const int64_t base = r * ncr;
+  const int64_t jlo = max(c0, base) - base, jhi = min(c0 + n, base + ncr) - base;
+  const int64_t* crow = cell + (r * nl + l) * (int64_t)ncr;
+  const double* grow = gl + (int64_t)l * ld + (base - c0);  // + j: column base + j of the chunk
+  for (int cb = 0; cb < ncells; cb += blockDim.x) {
+    const int t = cb + threadIdx.x;
+    double sum = 0.0;
+    for (int64_t j0 = jlo; j0 < jhi; j0 += CELL_TILE) {
+      const int m = (int)min((int64_t)CELL_TILE, jhi - j0);
+      __syncthreads();
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        const int64_t c = crow[j0 + i];
+        cid[i] = (c >= 0 && c < ncells) ? (int)c : -1;
+        val[i] = grow[j0 + i];
+      }
+      __syncthreads();
+      for (int i = 0; i < m; i++)
+        if (cid[i] == t) sum += val[i];
+    }
+    if (t < ncells) acc[(r * nl + l) * (int64_t)ncells + t] += sum;
+  }
+}
+
+hipError_t launch_cell_sum(const double* gl, int64_t ld, int nl, int64_t c0, int64_t n, int ncr,
+                           const int64_t* cell, int ncells, double* acc, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int64_t nr = (c0 + n - 1) / ncr - c0 / ncr + 1;
+  hipLaunchKernelGGL(k_cell_sum, dim3((unsigned)(nr * nl)), dim3(256), 0, st, gl, ld, nl, c0, n,
+                     ncr, cell, ncells, acc);
+  return hipGetLastError();
+}
+
 __global__ void k_zero_int(int* p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;

@@ -61,7 +61,10 @@ class RealizationStep:
 
 def cell_gradient(grad_vs, layer_of_node, cell, ncells, c0, n, ncol_real, acc):
     """Adds the node gradient of chunk columns c0..c0+n-1 (grad_vs (nz, >= n)) to
-    acc (nreal, nlayers, ncells) f64: the sum over the nodes of every cell."""
+    acc (nreal, nlayers, ncells) f64: the sum over the nodes of every cell.
+    cell (nreal, nlayers, ncol_real) int64.  On the GPU the per-cell sums are
+    the library's dazi_dk_cell_sum_dev (fixed summation order); CPU tensors
+    take torch's index_add."""
     import torch
     nl = acc.shape[1]
     g = grad_vs[:, :n]
@@ -69,7 +72,16 @@ def cell_gradient(grad_vs, layer_of_node, cell, ncells, c0, n, ncol_real, acc):
     # (a GEMM in a fixed order, where an index_add over the columns was an
     # atomic scatter of nz x n values)
     member = torch.nn.functional.one_hot(layer_of_node, nl).to(torch.float64).t().contiguous()
-    gl = member @ g
+    gl = (member @ g).contiguous()
+    if gl.is_cuda:
+        from . import _lib
+        if cell.dtype != torch.int64 or not cell.is_contiguous() or not acc.is_contiguous():
+            raise ValueError("cell must be contiguous int64 and acc contiguous")
+        rc = _lib.lib().dazi_dk_cell_sum_dev(
+            gl.data_ptr(), n, nl, c0, n, ncol_real, acc.shape[0], cell.data_ptr(), ncells,
+            acc.data_ptr(), stage._stream_ptr(None))
+        _lib.check(rc)
+        return
     cols = torch.arange(c0, c0 + n, device=g.device)
     r, j = cols // ncol_real, cols % ncol_real
     idx = (r[None, :] * nl + torch.arange(nl, device=g.device)[:, None]) * ncells + \

@@ -273,6 +273,19 @@ int32_t dazi_dk_effective_dev(const dazi_dk_desc* desc, const float* vel, const 
                               const double* sen_vp, const double* sen_rho, const double* grad_c,
                               double* eff, double* grad_vs, void* stream);
 
+/* Per-cell sums of a population gradient (the Voronoi realizations' cell
+ * gradient; no counterpart in the reference, whose Voronoi projection is
+ * outside this library): for the chunk of columns c in [c0, c0+n) of nreal
+ * realizations of ncol_real columns each (c = r*ncol_real + j),
+ *   acc[(r*nl + l)*ncells + cell[(r*nl + l)*ncol_real + j]] += gl[l*ld + c - c0]
+ * for every layer l < nl.  Device pointers: gl (nl, ld) f64, cell (nreal, nl,
+ * ncol_real) i64, acc (nreal, nl, ncells) f64.  Each sum runs in column order
+ * (deterministic, no atomics); cell ids outside [0, ncells) are skipped.
+ * Calls on one stream accumulate in order. */
+int32_t dazi_dk_cell_sum_dev(const double* gl, int64_t ld, int32_t nl, int64_t c0, int64_t n,
+                             int32_t ncol_real, int32_t nreal, const int64_t* cell,
+                             int32_t ncells, double* acc, void* stream);
+
 /* G-row assembly: the sparse rows (rw, iw, col) of the joint inversion's G
  * matrix that the reference builds ray by ray from the path weights and the
  * stage outputs (CalSurfGAniso_Joint.f90:750-797, inside the source/period/

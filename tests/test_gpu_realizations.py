@@ -110,3 +110,37 @@ def test_chunked_graph_equals_eager_and_cell_gradient(dk):
                                       minlength=40)
     a = acc.cpu().numpy()
     assert np.abs(a - want).max() <= 1e-12 * np.abs(want).max()
+
+
+def test_cell_sum_chunks_across_realizations_deterministic(dk):
+    """dazi_dk_cell_sum_dev through cell_gradient: chunks that start and end
+    inside realizations, cell ids outside [0, ncells) skipped, sums equal to a
+    float64 bincount and bit-identical between runs."""
+    import torch
+    from dazi_current_release_amd.realizations import cell_gradient
+    rng = np.random.default_rng(5)
+    nreal, nl, ncr, ncells, nz = 3, 4, 1000, 37, 9
+    lon = torch.as_tensor(np.sort(rng.integers(0, nl, nz)), device="cuda")
+    cell_np = rng.integers(0, ncells, (nreal, nl, ncr))
+    cell_np[0, 1, :5] = ncells + 3          # outside: skipped
+    cell_np[2, 3, 7] = -1
+    cell = torch.as_tensor(cell_np, dtype=torch.int64, device="cuda")
+    g = rng.standard_normal((nz, nreal * ncr))
+    want = np.zeros((nreal, nl, ncells))
+    lon_np = lon.cpu().numpy()
+    for r in range(nreal):
+        for z in range(nz):
+            l = lon_np[z]
+            ok = (cell_np[r, l] >= 0) & (cell_np[r, l] < ncells)
+            want[r, l] += np.bincount(cell_np[r, l][ok], weights=g[z, r * ncr:(r + 1) * ncr][ok],
+                                      minlength=ncells)
+    runs = []
+    for _ in range(2):
+        acc = torch.zeros((nreal, nl, ncells), dtype=torch.float64, device="cuda")
+        for c0 in range(0, nreal * ncr, 700):
+            n = min(700, nreal * ncr - c0)
+            gv = torch.as_tensor(np.ascontiguousarray(g[:, c0:c0 + n]), device="cuda")
+            cell_gradient(gv, lon, cell, ncells, c0, n, ncr, acc)
+        runs.append(acc.cpu().numpy())
+    assert np.allclose(runs[0], want, rtol=0, atol=1e-10)
+    assert np.array_equal(runs[0], runs[1])
