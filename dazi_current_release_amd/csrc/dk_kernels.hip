@@ -1618,16 +1618,16 @@ hipError_t launch_effective(int ncol, int nz, int K, const float* vel, const dou
 // l)*ncr + j]] += gl[l*ld + (c - c0)] over the chunk's columns c = r*ncr + j in
 // [c0, c0 + n).  One workgroup per (realization, layer) that the chunk touches;
 // thread t owns cells t, t + 256, ...  and scans the realization's (cell,
-// value) pairs from LDS in column order, so every sum runs in a fixed order
-// (no atomics, the same bits on every run).  Cell ids outside [0, ncells) add
-// nowhere.
-constexpr int CELL_TILE = 1024;
+// value) pairs from LDS in column order into eight partial sums, so every sum
+// runs in a fixed order (no atomics, the same bits on every run).  Cell ids
+// outside [0, ncells) add nowhere.
+constexpr int CELL_TILE = 1024;  // a multiple of 8
 __global__ void __launch_bounds__(256) k_cell_sum(const double* __restrict__ gl, int64_t ld,
                                                   int nl, int64_t c0, int64_t n, int ncr,
                                                   const int64_t* __restrict__ cell, int ncells,
                                                   double* __restrict__ acc) {
-  __shared__ int cid[CELL_TILE];
-  __shared__ double val[CELL_TILE];
+  __shared__ __align__(16) int cid[CELL_TILE];
+  __shared__ __align__(16) double val[CELL_TILE];
   const int l = blockIdx.x % nl;
   const int64_t r = c0 / ncr + blockIdx.x / nl;
   const int64_t base = r * ncr;
@@ -1636,19 +1636,43 @@ __global__ void __launch_bounds__(256) k_cell_sum(const double* __restrict__ gl,
   const double* grow = gl + (int64_t)l * ld + (base - c0);  // + j: column base + j of the chunk
   for (int cb = 0; cb < ncells; cb += blockDim.x) {
     const int t = cb + threadIdx.x;
-    double sum = 0.0;
+    // eight partial sums, column j into sum (j - jlo) mod 8, added in a fixed
+    // order at the end (independent chains for the LDS reads to overlap)
+    double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t j0 = jlo; j0 < jhi; j0 += CELL_TILE) {
       const int m = (int)min((int64_t)CELL_TILE, jhi - j0);
+      const int mp = (m + 7) & ~7;
       __syncthreads();
-      for (int i = threadIdx.x; i < m; i += blockDim.x) {
-        const int64_t c = crow[j0 + i];
-        cid[i] = (c >= 0 && c < ncells) ? (int)c : -1;
-        val[i] = grow[j0 + i];
+      for (int i = threadIdx.x; i < mp; i += blockDim.x) {
+        int id = -1;
+        double v = 0.0;
+        if (i < m) {
+          const int64_t c = crow[j0 + i];
+          id = (c >= 0 && c < ncells) ? (int)c : -1;
+          v = grow[j0 + i];
+        }
+        cid[i] = id;
+        val[i] = v;
       }
       __syncthreads();
-      for (int i = 0; i < m; i++)
-        if (cid[i] == t) sum += val[i];
+      for (int i = 0; i < mp; i += 8) {
+        const int4 ca = *reinterpret_cast<const int4*>(&cid[i]);
+        const int4 cc = *reinterpret_cast<const int4*>(&cid[i + 4]);
+        const double2 v0 = *reinterpret_cast<const double2*>(&val[i]);
+        const double2 v1 = *reinterpret_cast<const double2*>(&val[i + 2]);
+        const double2 v2 = *reinterpret_cast<const double2*>(&val[i + 4]);
+        const double2 v3 = *reinterpret_cast<const double2*>(&val[i + 6]);
+        if (ca.x == t) a[0] += v0.x;
+        if (ca.y == t) a[1] += v0.y;
+        if (ca.z == t) a[2] += v1.x;
+        if (ca.w == t) a[3] += v1.y;
+        if (cc.x == t) a[4] += v2.x;
+        if (cc.y == t) a[5] += v2.y;
+        if (cc.z == t) a[6] += v3.x;
+        if (cc.w == t) a[7] += v3.y;
+      }
     }
+    const double sum = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     if (t < ncells) acc[(r * nl + l) * (int64_t)ncells + t] += sum;
   }
 }
