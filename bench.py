@@ -540,8 +540,8 @@ def bench_host_path(dk, configs, np, reps=10):
         dk.depthkernel(nx, ny, nz, vel, pv, *sens, 2, 0, K, W["periods"], W["depz"],
                        W["minthk"], log=None)
 
-    # two untimed calls: the first allocates the library's pinned staging and
-    # device buffers, the second the first writes into the output arrays' pages
+    # two untimed calls (the first allocates the library's pinned staging and
+    # device buffers; a single one left the next calls about 15 % slower)
     step()
     step()
     t0 = time.perf_counter()
