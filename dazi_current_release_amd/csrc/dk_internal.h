@@ -60,7 +60,9 @@ struct KernDev {
   double* drho;
 };
 
-// One refined layer of the TI model: unflattened values used by the splice
+// One refined layer of the TI model (stored by k_prep_ti in word planes: word
+// w of (layer m, column col) at [(m * 15 + w) * ncol + col]): unflattened
+// values used by the splice
 // (d, rvp, rvs, rrho, TA, TL, TF) and flattened ones used by the eigen-solution
 // (df, A = C, L = N, F, rho) plus the sphericity factor vtp; float32 as in the
 // reference.  k holds the layer's period-independent FP64 constants of the

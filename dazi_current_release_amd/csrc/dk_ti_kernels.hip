@@ -15,6 +15,8 @@
 //               register budget.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "dk_geom.h"
 #include "dk_internal.h"
 #include "dk_ti.h"
@@ -87,9 +89,25 @@ __device__ __forceinline__ TiLayerDev ti_layer(const TiGeom& g, float rvp, float
   return t;
 }
 
-// Node models, one wavefront per column and lane = layer (k_prep_nodes'
-// layout): each lane interpolates its layer between its nodes (the sublayer
-// mapping and TI factors of geo) and writes its TiLayerDev.
+// The records are stored in word planes: 8-byte word w of the record of
+// (layer m, column col) at [(m * TI_WORDS + w) * ncol + col], so that the
+// lanes of a wavefront (consecutive columns) read a word of their records in
+// one contiguous access.  Words: 0 (d, rvp), 1 (rvs, rrho), 2 (TA, TL),
+// 3 (TF, df), 4 (A, L), 5 (F, rho), 6 (vtp, pad), 7..14 k[0..7].
+constexpr int TI_WORDS = (int)(sizeof(TiLayerDev) / 8);
+static_assert(sizeof(TiLayerDev) == 8 * TI_WORDS, "TI record words");
+union TiWords {
+  TiLayerDev t;
+  uint64_t u[TI_WORDS];
+};
+__device__ __forceinline__ uint64_t ti_word(const TiLayerDev* recs, int ncol, int col, int m,
+                                            int w) {
+  return reinterpret_cast<const uint64_t*>(recs)[((size_t)m * TI_WORDS + w) * ncol + col];
+}
+
+// Node models, one thread per (layer, column), columns fastest: each thread
+// interpolates its layer between its nodes (the sublayer mapping and TI
+// factors of geo) and writes its record's words.
 __global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restrict__ vel,
                                                  const float* __restrict__ vpin,
                                                  const float* __restrict__ rhoin, int L,
@@ -97,9 +115,6 @@ __global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restri
                                                  TiLayerDev* __restrict__ out,
                                                  int* __restrict__ queue) {
 #pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63;
-  const int w0 = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  if (w0 == 0 && lane == 0) queue[0] = 0;  // the complex-pair queue of the k_ti launch that follows
   auto node_vals = [&](int col, int i, float* vs, float* vp, float* rho) {
     const size_t o = (size_t)i * ncol + col;
     *vs = vel[o];
@@ -111,25 +126,29 @@ __global__ void __launch_bounds__(256) k_prep_ti(int ncol, const float* __restri
       *rho = brocher_rho_ti(*vp);
     }
   };
-  const int nw = gridDim.x * (blockDim.x >> 6);
-  for (int col = w0; col < ncol; col += nw) {
-    TiLayerDev* lay = out + (size_t)col * L;
-    for (int m = lane; m < L; m += 64) {
-      const FlatGeom& g = geo[m];
-      if (g.node < 0) continue;
-      float vs, vp, rho;
-      node_vals(col, g.node, &vs, &vp, &rho);
-      if (m < L - 1) {
-        float vs_n, vp_n, rho_n;
-        node_vals(col, g.node + 1, &vs_n, &vp_n, &rho_n);
-        const float den = (float)(2 * g.ns);
-        const float w = (float)(2 * g.sub - 1);
-        vp = vp + w * (vp_n - vp) / den;
-        vs = vs + w * (vs_n - vs) / den;
-        rho = rho + w * (rho_n - rho) / den;
-      }
-      lay[m] = ti_layer(g.ti, vp, vs, rho);
+  uint64_t* words = reinterpret_cast<uint64_t*>(out);
+  const size_t n = (size_t)L * ncol;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (size_t)gridDim.x * blockDim.x) {
+    if (t == 0) queue[0] = 0;  // the complex-pair queue of the k_ti launch that follows
+    const int m = (int)(t / ncol), col = (int)(t % ncol);
+    const FlatGeom& g = geo[m];
+    if (g.node < 0) continue;
+    float vs, vp, rho;
+    node_vals(col, g.node, &vs, &vp, &rho);
+    if (m < L - 1) {
+      float vs_n, vp_n, rho_n;
+      node_vals(col, g.node + 1, &vs_n, &vp_n, &rho_n);
+      const float den = (float)(2 * g.ns);
+      const float w = (float)(2 * g.sub - 1);
+      vp = vp + w * (vp_n - vp) / den;
+      vs = vs + w * (vs_n - vs) / den;
+      rho = rho + w * (rho_n - rho) / den;
     }
+    TiWords r;
+    r.t = ti_layer(g.ti, vp, vs, rho);
+#pragma unroll
+    for (int wd = 0; wd < TI_WORDS; wd++) words[((size_t)m * TI_WORDS + wd) * ncol + col] = r.u[wd];
   }
 }
 
@@ -146,7 +165,7 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
                       float* __restrict__ dcdL, double* scratch, long stride, int nan_failed) {
   const float twopi = 2.0f * 3.141592654f;
   const int col = (int)(s % ncol), k = (int)(s / ncol);
-  const TiLayerDev* lay = layers + (size_t)col * L;
+  auto W = [&](int m, int w) { return ti_word(layers, ncol, col, m, w); };
   auto out = [&](int j) -> float& { return lsen[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * j)]; };
   auto lay_out = [&](float* a, int m) -> float& {
     return a[(size_t)col + (size_t)ncol * ((size_t)k + (size_t)K * m)];
@@ -168,20 +187,27 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
   }
   const double omega = (double)twopi / (double)(float)periods[k];
   auto head = [&](int m) {
-    const TiLayerDev& t = lay[m];
-    ti::Layer r = layer_of(t);
-    r.iC = t.k[0];
-    r.iL = t.k[1];
+    TiWords q;
+    q.u[3] = W(m, 3);
+    q.u[4] = W(m, 4);
+    q.u[5] = W(m, 5);
+    q.u[7] = W(m, 7);
+    q.u[8] = W(m, 8);
+    ti::Layer r = layer_of(q.t);
+    r.iC = q.t.k[0];
+    r.iL = q.t.k[1];
     return r;
   };
   auto tail = [&](int m, ti::Layer* r) {
-    const TiLayerDev& t = lay[m];
-    r->eta = t.k[2];
-    r->FiC = t.k[3];
-    r->ah = t.k[4];
-    r->av = t.k[5];
-    r->bv = t.k[6];
-    r->Feta = t.k[7];
+    TiWords q;
+#pragma unroll
+    for (int w = 9; w < TI_WORDS; w++) q.u[w] = W(m, w);
+    r->eta = q.t.k[2];
+    r->FiC = q.t.k[3];
+    r->ah = q.t.k[4];
+    r->av = q.t.k[5];
+    r->bv = q.t.k[6];
+    r->Feta = q.t.k[7];
   };
   double isc, inrm;
   const int rc = ti::solve<G_>(L, head, tail, omega, c, scratch, stride, &isc, &inrm);
@@ -200,14 +226,19 @@ __device__ int ti_one(long s, int ncol, int K, int L, int nz, const TiLayerDev* 
     r.f0 = ti::factor<T>(scratch, stride, m, 0);
     r.f1 = ti::factor<T>(scratch, stride, m, 1);
     r.f2 = ti::factor<T>(scratch, stride, m, 2);
-    const TiLayerDev& t = lay[m];
-    r.vtp = t.vtp;
-    r.rrho = t.rrho;
-    r.rvp = t.rvp;
-    r.rvs = t.rvs;
-    r.TA = t.TA;
-    r.TL = t.TL;
-    r.TF = t.TF;
+    TiWords q;
+    q.u[0] = W(m, 0);
+    q.u[1] = W(m, 1);
+    q.u[2] = W(m, 2);
+    q.u[3] = W(m, 3);
+    q.u[6] = W(m, 6);
+    r.vtp = q.t.vtp;
+    r.rrho = q.t.rrho;
+    r.rvp = q.t.rvp;
+    r.rvs = q.t.rvs;
+    r.TA = q.t.TA;
+    r.TL = q.t.TL;
+    r.TF = q.t.TF;
     return r;
   };
   // the half-space feeds no interval; without KL the loop stops at the last
@@ -286,9 +317,11 @@ __global__ void __launch_bounds__(64) k_ti_cx(int ncol, int K, int L, int nz,
 
 hipError_t launch_prep_ti(int ncol, const float* vel, const float* vp, const float* rho, int L,
                           const void* geo, TiLayerDev* out, int* queue, hipStream_t st) {
-  // four columns per 256-thread workgroup (one per wavefront)
-  hipLaunchKernelGGL(k_prep_ti, dim3((ncol + 3) / 4), dim3(256), 0, st, ncol, vel, vp, rho, L,
-                     (const FlatGeom*)geo, out, queue);
+  // one thread per (layer, column)
+  const size_t n = (size_t)L * ncol;
+  const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_prep_ti, dim3(grid > 0 ? grid : 1), dim3(256), 0, st, ncol, vel, vp, rho,
+                     L, (const FlatGeom*)geo, out, queue);
   return hipGetLastError();
 }
 
